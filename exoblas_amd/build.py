@@ -1,0 +1,91 @@
+"""Builds libexoblas_amd.so in-tree for gfx950 (and the CPU oracle).
+
+    python -m exoblas_amd.build [--jobs N] [--no-oracle]
+
+Steps: regenerate the exported C-ABI glue (exoblas_amd.generator), compile
+every source under exoblas_amd/csrc with hipcc (objects in _build/), link the
+shared library next to this file.  Compilation always runs (the build check
+must exercise the compiler); unchanged objects are still rebuilt.
+"""
+from __future__ import annotations
+
+import argparse
+import concurrent.futures as cf
+import os
+import shutil
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CSRC = os.path.join(HERE, "csrc")
+BUILD = os.path.join(ROOT, "_build")
+LIB = os.path.join(HERE, "libexoblas_amd.so")
+ARCH = "gfx950"
+
+HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
+
+COMMON = ["-std=c++17", "-O3", "-fPIC", f"--offload-arch={ARCH}",
+          # packed FP32 instructions stay off (every kernel here is written
+          # for scalar / 16-byte vector arithmetic)
+          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+          "-I", os.path.join(ROOT, "include"), "-I", CSRC,
+          "-Wno-unused-command-line-argument"]
+
+
+def sources():
+    out = []
+    for f in sorted(os.listdir(CSRC)):
+        if f.endswith(".hip") or f.endswith(".cpp"):
+            out.append(os.path.join(CSRC, f))
+    return out
+
+
+def _compile(src: str) -> str:
+    obj = os.path.join(BUILD, os.path.basename(src) + ".o")
+    cmd = [HIPCC] + COMMON + ["-c", src, "-o", obj]
+    if src.endswith(".hip"):
+        cmd[1:1] = ["-x", "hip"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"compile failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    return obj
+
+
+def build_oracle() -> None:
+    odir = os.path.join(ROOT, "oracle")
+    r = subprocess.run(["make", "-s", "-C", odir, "-j3", "-B"], capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"oracle build failed\n{r.stdout}\n{r.stderr}")
+
+
+def build(jobs: int = 8, oracle: bool = True) -> str:
+    from exoblas_amd import generator
+    generator.write_all()
+    os.makedirs(BUILD, exist_ok=True)
+    srcs = sources()
+    with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
+        objs = list(ex.map(_compile, srcs))
+    tmp = LIB + ".tmp"
+    cmd = [HIPCC, "-shared", f"--offload-arch={ARCH}", "-o", tmp] + objs + ["-lpthread"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"link failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    os.replace(tmp, LIB)
+    if oracle:
+        build_oracle()
+    return LIB
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--jobs", type=int, default=min(8, os.cpu_count() or 8))
+    ap.add_argument("--no-oracle", action="store_true")
+    a = ap.parse_args(argv)
+    lib = build(a.jobs, oracle=not a.no_oracle)
+    print(lib)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

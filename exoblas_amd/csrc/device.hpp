@@ -1,0 +1,61 @@
+// Device-side helpers shared by the gfx950 kernels: 16-byte vector types,
+// wave64 and workgroup reductions, launch helpers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "runtime.hpp"
+
+namespace exoblas {
+
+constexpr int kWave = 64;
+
+// 16-byte vector of T: the unit of a global_load_dwordx4 / global_store_dwordx4.
+template <class T> struct Vec16;
+template <> struct Vec16<float> {
+  using type = float4;
+  static constexpr int n = 4;
+};
+template <> struct Vec16<double> {
+  using type = double2;
+  static constexpr int n = 2;
+};
+
+template <class T>
+__device__ __forceinline__ T& vget(typename Vec16<T>::type& v, int i) {
+  return reinterpret_cast<T*>(&v)[i];
+}
+template <class T>
+__device__ __forceinline__ const T& vget(const typename Vec16<T>::type& v, int i) {
+  return reinterpret_cast<const T*>(&v)[i];
+}
+
+template <class T>
+__device__ __forceinline__ T wave_sum(T v) {
+#pragma unroll
+  for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
+  return v;
+}
+
+// Sum over a workgroup of NT threads; the result is valid in every thread.
+// `red` must hold NT / 64 elements of LDS.
+template <int NT, class T>
+__device__ __forceinline__ T block_sum(T v, T* red) {
+  v = wave_sum(v);
+  const int w = threadIdx.x / kWave;
+  if ((threadIdx.x & (kWave - 1)) == 0) red[w] = v;
+  __syncthreads();
+  T s = 0;
+#pragma unroll
+  for (int i = 0; i < NT / kWave; ++i) s += red[i];
+  __syncthreads();
+  return s;
+}
+
+inline bool aligned16(const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15) == 0; }
+
+__host__ __device__ inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+}  // namespace exoblas

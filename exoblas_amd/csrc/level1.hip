@@ -1,0 +1,393 @@
+// Level-1 kernels for gfx950: streaming elementwise updates and reductions.
+//
+// All of these are HBM-bound.  The unit-stride path moves 16 bytes per lane
+// per access (global_load_dwordx4), unrolled so each lane keeps several loads
+// in flight; a wave64 touches 1 KiB contiguous per access.  Strided and
+// misaligned windows use a grid-stride scalar kernel.  Reductions are one
+// launch: per-workgroup partial sums (wave shuffle + LDS), written to the
+// stream's scratch, and the last workgroup to finish (atomic ticket) sums the
+// partials in a fixed order, so results are deterministic.
+#include "device.hpp"
+#include "ops.hpp"
+
+namespace exoblas {
+namespace {
+
+constexpr int kNT = 256;
+
+// ------------------------------------------------------------ elementwise
+
+template <class T, class Op, int U>
+__global__ __launch_bounds__(kNT) void ew_vec_kernel(int64_t nvec, int64_t n, T* __restrict__ x,
+                                                     T* __restrict__ y, Op op) {
+  using V = typename Vec16<T>::type;
+  constexpr int VN = Vec16<T>::n;
+  const int64_t i0 = (int64_t)blockIdx.x * (kNT * U) + threadIdx.x;
+  V xv[U], yv[U];
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    const int64_t i = i0 + (int64_t)u * kNT;
+    if (i < nvec) {
+      if (Op::rx) xv[u] = reinterpret_cast<const V*>(x)[i];
+      if (Op::ry) yv[u] = reinterpret_cast<const V*>(y)[i];
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    const int64_t i = i0 + (int64_t)u * kNT;
+    if (i < nvec) {
+#pragma unroll
+      for (int k = 0; k < VN; ++k) op(vget<T>(xv[u], k), vget<T>(yv[u], k));
+      if (Op::wx) reinterpret_cast<V*>(x)[i] = xv[u];
+      if (Op::wy) reinterpret_cast<V*>(y)[i] = yv[u];
+    }
+  }
+  if (blockIdx.x == gridDim.x - 1) {  // the < VN trailing elements
+    const int64_t t = nvec * VN + threadIdx.x;
+    if (t < n) {
+      T xs = Op::rx ? x[t] : T(0);
+      T ys = Op::ry ? y[t] : T(0);
+      op(xs, ys);
+      if (Op::wx) x[t] = xs;
+      if (Op::wy) y[t] = ys;
+    }
+  }
+}
+
+template <class T, class Op>
+__global__ __launch_bounds__(kNT) void ew_strided_kernel(int64_t n, T* x, int64_t incx, T* y, int64_t incy,
+                                                         Op op) {
+  const int64_t step = (int64_t)gridDim.x * kNT;
+  for (int64_t i = (int64_t)blockIdx.x * kNT + threadIdx.x; i < n; i += step) {
+    T xs = Op::rx ? x[i * incx] : T(0);
+    T ys = Op::ry ? y[i * incy] : T(0);
+    op(xs, ys);
+    if (Op::wx) x[i * incx] = xs;
+    if (Op::wy) y[i * incy] = ys;
+  }
+}
+
+template <class T, class Op>
+void launch_ew(Ctx& c, int64_t n, T* x, int64_t incx, T* y, int64_t incy, Op op) {
+  if (n <= 0) return;
+  constexpr int VN = Vec16<T>::n;
+  constexpr int U = 4;
+  const bool unit = (!(Op::rx || Op::wx) || (incx == 1 && aligned16(x))) &&
+                    (!(Op::ry || Op::wy) || (incy == 1 && aligned16(y)));
+  if (unit) {
+    const int64_t nvec = n / VN;
+    const int64_t blocks = std::max<int64_t>(1, ceil_div(nvec, kNT * U));
+    hipLaunchKernelGGL((ew_vec_kernel<T, Op, U>), dim3((unsigned)blocks), dim3(kNT), 0, c.stream, nvec, n, x, y,
+                       op);
+  } else {
+    const int64_t blocks = std::min<int64_t>(ceil_div(n, kNT), (int64_t)c.num_cus * 16);
+    hipLaunchKernelGGL((ew_strided_kernel<T, Op>), dim3((unsigned)blocks), dim3(kNT), 0, c.stream, n, x, incx,
+                       y, incy, op);
+  }
+  EXO_LAUNCH_CHECK();
+}
+
+// Elementwise functors: flags say which of (x, y) are read / written.
+template <class T> struct AxpyOp {
+  static constexpr bool rx = true, ry = true, wx = false, wy = true;
+  T alpha;
+  __device__ void operator()(T& x, T& y) const { y += alpha * x; }
+};
+template <class T> struct Axpy1Op {
+  static constexpr bool rx = true, ry = true, wx = false, wy = true;
+  __device__ void operator()(T& x, T& y) const { y += x; }
+};
+template <class T> struct CopyOp {
+  static constexpr bool rx = true, ry = false, wx = false, wy = true;
+  __device__ void operator()(T& x, T& y) const { y = x; }
+};
+template <class T> struct SwapOp {
+  static constexpr bool rx = true, ry = true, wx = true, wy = true;
+  __device__ void operator()(T& x, T& y) const {
+    T t = x;
+    x = y;
+    y = t;
+  }
+};
+template <class T> struct RotOp {
+  static constexpr bool rx = true, ry = true, wx = true, wy = true;
+  T c, s;
+  __device__ void operator()(T& x, T& y) const {
+    const T xr = x;
+    x = c * xr + s * y;
+    y = -s * xr + c * y;
+  }
+};
+// rotm, src/level1/rotm.py: H = {H00, H01, H10, H11}
+template <class T> struct RotmNegOneOp {
+  static constexpr bool rx = true, ry = true, wx = true, wy = true;
+  T h00, h01, h10, h11;
+  __device__ void operator()(T& x, T& y) const {
+    const T xr = x;
+    x = h00 * xr + h01 * y;
+    y = h10 * xr + h11 * y;
+  }
+};
+template <class T> struct RotmZeroOp {
+  static constexpr bool rx = true, ry = true, wx = true, wy = true;
+  T h01, h10;
+  __device__ void operator()(T& x, T& y) const {
+    const T xr = x;
+    x = xr + h01 * y;
+    y = h10 * xr + y;
+  }
+};
+template <class T> struct RotmOneOp {
+  static constexpr bool rx = true, ry = true, wx = true, wy = true;
+  T h00, h11;
+  __device__ void operator()(T& x, T& y) const {
+    const T xr = x;
+    x = h00 * xr + y;
+    y = -xr + h11 * y;
+  }
+};
+template <class T> struct ScalOp {
+  static constexpr bool rx = true, ry = false, wx = true, wy = false;
+  T alpha;
+  __device__ void operator()(T& x, T&) const { x = alpha * x; }
+};
+template <class T> struct ZeroOp {
+  static constexpr bool rx = false, ry = false, wx = true, wy = false;
+  __device__ void operator()(T& x, T&) const { x = T(0); }
+};
+
+// ------------------------------------------------------------ reductions
+
+template <class T> struct AbsTerm {
+  using Acc = T;
+  __device__ Acc operator()(T x, T) const { return x > T(0) ? x : -x; }
+  static constexpr bool ry = false;
+};
+template <class T> struct DotTerm {
+  using Acc = T;
+  __device__ Acc operator()(T x, T y) const { return x * y; }
+  static constexpr bool ry = true;
+};
+struct DsdotTerm {
+  using Acc = double;
+  __device__ Acc operator()(float x, float y) const { return (double)x * (double)y; }
+  static constexpr bool ry = true;
+};
+
+template <class Acc, class Out> struct Finish {
+  Acc add;  // sdsdot's sb
+  __device__ Out operator()(Acc s) const { return (Out)(s + add); }
+};
+
+template <class T, class Term, class Out, int U>
+__global__ __launch_bounds__(kNT) void reduce_kernel(int64_t n, const T* __restrict__ x, int64_t incx,
+                                                     const T* __restrict__ y, int64_t incy, int vec,
+                                                     typename Term::Acc* partials, unsigned* ticket,
+                                                     Out* result, Finish<typename Term::Acc, Out> fin,
+                                                     Term term) {
+  using Acc = typename Term::Acc;
+  using V = typename Vec16<T>::type;
+  constexpr int VN = Vec16<T>::n;
+  __shared__ Acc red[kNT / kWave];
+  __shared__ int is_last;
+  Acc acc[U];
+#pragma unroll
+  for (int u = 0; u < U; ++u) acc[u] = Acc(0);
+
+  if (vec) {
+    const int64_t nvec = n / VN;
+    const int64_t chunk = ceil_div(ceil_div(nvec, gridDim.x), kNT * U) * (kNT * U);
+    const int64_t beg = (int64_t)blockIdx.x * chunk;
+    const int64_t end = beg + chunk < nvec ? beg + chunk : nvec;
+    int64_t i = beg + threadIdx.x;
+    for (; i + (U - 1) * kNT < end; i += U * kNT) {
+      V xv[U], yv[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        xv[u] = reinterpret_cast<const V*>(x)[i + u * kNT];
+        if (Term::ry) yv[u] = reinterpret_cast<const V*>(y)[i + u * kNT];
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+#pragma unroll
+        for (int k = 0; k < VN; ++k) acc[u] += term(vget<T>(xv[u], k), Term::ry ? vget<T>(yv[u], k) : T(0));
+    }
+    for (; i < end; i += kNT) {
+      V xv = reinterpret_cast<const V*>(x)[i];
+      V yv;
+      if (Term::ry) yv = reinterpret_cast<const V*>(y)[i];
+#pragma unroll
+      for (int k = 0; k < VN; ++k) acc[0] += term(vget<T>(xv, k), Term::ry ? vget<T>(yv, k) : T(0));
+    }
+    if (blockIdx.x == 0) {
+      const int64_t t = nvec * VN + threadIdx.x;
+      if (t < n) acc[1 % U] += term(x[t], Term::ry ? y[t] : T(0));
+    }
+  } else {
+    const int64_t step = (int64_t)gridDim.x * kNT;
+    for (int64_t i = (int64_t)blockIdx.x * kNT + threadIdx.x; i < n; i += step)
+      acc[0] += term(x[i * incx], Term::ry ? y[i * incy] : T(0));
+  }
+  Acc s = acc[0];
+#pragma unroll
+  for (int u = 1; u < U; ++u) s += acc[u];
+  s = block_sum<kNT>(s, red);
+  if (threadIdx.x == 0) {
+    partials[blockIdx.x] = s;
+    __threadfence();
+    const unsigned prev = atomicAdd(ticket, 1u);
+    is_last = (prev == gridDim.x - 1);
+  }
+  __syncthreads();
+  if (is_last) {
+    __threadfence();
+    Acc t = Acc(0);
+    for (int i = threadIdx.x; i < (int)gridDim.x; i += kNT) t += partials[i];
+    t = block_sum<kNT>(t, red);
+    if (threadIdx.x == 0) {
+      *result = fin(t);
+      *ticket = 0u;
+    }
+  }
+}
+
+template <class T, class Term, class Out>
+void launch_reduce(Ctx& c, int64_t n, const T* x, int64_t incx, const T* y, int64_t incy, Out* result,
+                   typename Term::Acc add) {
+  using Acc = typename Term::Acc;
+  constexpr int VN = Vec16<T>::n;
+  constexpr int U = 4;
+  const bool vec = incx == 1 && aligned16(x) && (!Term::ry || (incy == 1 && aligned16(y)));
+  int64_t blocks = std::max<int64_t>(1, ceil_div(n, (int64_t)kNT * U * VN));
+  blocks = std::min<int64_t>(blocks, 2048);
+  Acc* partials = static_cast<Acc*>(c.scratch(sizeof(Acc) * blocks));
+  hipLaunchKernelGGL((reduce_kernel<T, Term, Out, U>), dim3((unsigned)blocks), dim3(kNT), 0, c.stream, n, x, incx,
+                     y, incy, vec ? 1 : 0, partials, c.counters(), result, Finish<Acc, Out>{add}, Term{});
+  EXO_LAUNCH_CHECK();
+}
+
+__global__ void store_zero_kernel(char* p, int bytes) {
+  if (threadIdx.x < bytes) p[threadIdx.x] = 0;
+}
+
+template <class Out>
+void write_zero(Ctx& c, Out* result) {
+  hipLaunchKernelGGL(store_zero_kernel, dim3(1), dim3(64), 0, c.stream, reinterpret_cast<char*>(result),
+                     (int)sizeof(Out));
+  EXO_LAUNCH_CHECK();
+}
+
+}  // namespace
+
+namespace ops {
+
+// src/level1/asum.py:11-15
+template <class T>
+void asum(Ctx& c, int64_t n, const T* x, int64_t incx, T* result) {
+  if (n <= 0) return write_zero(c, result);
+  launch_reduce<T, AbsTerm<T>, T>(c, n, x, incx, nullptr, 0, result, T(0));
+}
+
+// src/level1/axpy.py:9-12
+template <class T>
+void axpy(Ctx& c, int64_t n, T alpha, const T* x, int64_t incx, T* y, int64_t incy) {
+  launch_ew(c, n, const_cast<T*>(x), incx, y, incy, AxpyOp<T>{alpha});
+}
+
+// src/level1/axpy.py:15-18
+template <class T>
+void axpy_alpha_1(Ctx& c, int64_t n, const T* x, int64_t incx, T* y, int64_t incy) {
+  launch_ew(c, n, const_cast<T*>(x), incx, y, incy, Axpy1Op<T>{});
+}
+
+// src/level1/dot.py:9-13
+template <class T>
+void dot(Ctx& c, int64_t n, const T* x, int64_t incx, const T* y, int64_t incy, T* result) {
+  if (n <= 0) return write_zero(c, result);
+  launch_reduce<T, DotTerm<T>, T>(c, n, x, incx, y, incy, result, T(0));
+}
+
+// src/level1/dsdot.py:11-21
+void dsdot(Ctx& c, int64_t n, const float* x, int64_t incx, const float* y, int64_t incy, double* result) {
+  if (n <= 0) return write_zero(c, result);
+  launch_reduce<float, DsdotTerm, double>(c, n, x, incx, y, incy, result, 0.0);
+}
+
+// src/level1/dsdot.py:24-36: FP64 dot, + sb in FP64, one rounding to FP32
+void sdsdot(Ctx& c, int64_t n, float sb, const float* x, int64_t incx, const float* y, int64_t incy,
+            float* result) {
+  if (n <= 0) {
+    // d_dot = 0, d_result = sb
+    launch_reduce<float, DsdotTerm, float>(c, 0, x, 1, y, 1, result, (double)sb);
+    return;
+  }
+  launch_reduce<float, DsdotTerm, float>(c, n, x, incx, y, incy, result, (double)sb);
+}
+
+// src/level1/exo_copy.py:9-12 (bit-exact data movement)
+template <class T>
+void copy(Ctx& c, int64_t n, const T* x, int64_t incx, T* y, int64_t incy) {
+  launch_ew(c, n, const_cast<T*>(x), incx, y, incy, CopyOp<T>{});
+}
+
+// src/level1/swap.py:9-15 (bit-exact data movement)
+template <class T>
+void swap(Ctx& c, int64_t n, T* x, int64_t incx, T* y, int64_t incy) {
+  launch_ew(c, n, x, incx, y, incy, SwapOp<T>{});
+}
+
+// src/level1/rot.py:9-15
+template <class T>
+void rot(Ctx& c, int64_t n, T* x, int64_t incx, T* y, int64_t incy, T cc, T s) {
+  launch_ew(c, n, x, incx, y, incy, RotOp<T>{cc, s});
+}
+
+// src/level1/rotm.py:9-15
+template <class T>
+void rotm_flag_neg_one(Ctx& c, int64_t n, T* x, int64_t incx, T* y, int64_t incy, const T* H) {
+  launch_ew(c, n, x, incx, y, incy, RotmNegOneOp<T>{H[0], H[1], H[2], H[3]});
+}
+
+// src/level1/rotm.py:18-24
+template <class T>
+void rotm_flag_zero(Ctx& c, int64_t n, T* x, int64_t incx, T* y, int64_t incy, const T* H) {
+  launch_ew(c, n, x, incx, y, incy, RotmZeroOp<T>{H[1], H[2]});
+}
+
+// src/level1/rotm.py:27-33
+template <class T>
+void rotm_flag_one(Ctx& c, int64_t n, T* x, int64_t incx, T* y, int64_t incy, const T* H) {
+  launch_ew(c, n, x, incx, y, incy, RotmOneOp<T>{H[0], H[3]});
+}
+
+// src/level1/scal.py:9-12
+template <class T>
+void scal(Ctx& c, int64_t n, T alpha, T* x, int64_t incx) {
+  launch_ew(c, n, x, incx, static_cast<T*>(nullptr), 0, ScalOp<T>{alpha});
+}
+
+// src/level1/scal.py:15-19
+template <class T>
+void scal_alpha_0(Ctx& c, int64_t n, T* x, int64_t incx) {
+  launch_ew(c, n, x, incx, static_cast<T*>(nullptr), 0, ZeroOp<T>{});
+}
+
+#define EXO_L1_INST(T)                                                                           \
+  template void asum<T>(Ctx&, int64_t, const T*, int64_t, T*);                                  \
+  template void axpy<T>(Ctx&, int64_t, T, const T*, int64_t, T*, int64_t);                      \
+  template void axpy_alpha_1<T>(Ctx&, int64_t, const T*, int64_t, T*, int64_t);                 \
+  template void dot<T>(Ctx&, int64_t, const T*, int64_t, const T*, int64_t, T*);                \
+  template void copy<T>(Ctx&, int64_t, const T*, int64_t, T*, int64_t);                         \
+  template void swap<T>(Ctx&, int64_t, T*, int64_t, T*, int64_t);                               \
+  template void rot<T>(Ctx&, int64_t, T*, int64_t, T*, int64_t, T, T);                          \
+  template void rotm_flag_neg_one<T>(Ctx&, int64_t, T*, int64_t, T*, int64_t, const T*);        \
+  template void rotm_flag_zero<T>(Ctx&, int64_t, T*, int64_t, T*, int64_t, const T*);           \
+  template void rotm_flag_one<T>(Ctx&, int64_t, T*, int64_t, T*, int64_t, const T*);            \
+  template void scal<T>(Ctx&, int64_t, T, T*, int64_t);                                         \
+  template void scal_alpha_0<T>(Ctx&, int64_t, T*, int64_t);
+
+EXO_L1_INST(float)
+EXO_L1_INST(double)
+
+}  // namespace ops
+}  // namespace exoblas

@@ -1,0 +1,545 @@
+// Level-2 kernels for gfx950 (all HBM-bound on the matrix stream).
+//
+// gemv-N  one wave per R rows, x staged through LDS in panels, 16-byte loads
+//         of R rows in flight per lane, wave-shuffle row sums; wide matrices
+//         split their columns over workgroups (partials + a finishing pass).
+// gemv-T  a workgroup owns 64*VN consecutive columns and a chunk of rows;
+//         lanes hold column sums, waves split the rows and reduce through LDS.
+// ger, syr, syr2   row-wise rank-1/2 updates (read+write each element once),
+//         restricted to one triangle for syr/syr2.
+// symv    64x64 tiles of the stored triangle, each read once: a tile adds
+//         A_IJ x_J to y_I and A_IJ^T x_I to y_J; per-(tile-row, tile-column)
+//         partials are summed in a fixed order, so results are deterministic.
+// trmv    the triangular gemv-N / gemv-T kernels over a copy of x.
+#include "device.hpp"
+#include "ops.hpp"
+
+namespace exoblas {
+namespace {
+
+constexpr int kNT = 256;
+constexpr int kNoTrans = 111;
+constexpr int kUpper = 121;
+constexpr int kUnit = 132;
+
+enum Tri { kFull = 0, kLowerTri = 1, kUpperTri = 2 };
+
+// ------------------------------------------------------------ gemv-N
+
+// y[r] = beta*y[r] + alpha * sum_{c in row r's range} A[r,c] x[c]   (TRI == kFull)
+// For TRI != kFull the row range is the triangle; UNIT skips the diagonal and
+// adds x[r]; the result overwrites y (trmv).
+// With gridDim.y > 1 each workgroup row covers `cols_per` columns and writes
+// its raw sums to part[blockIdx.y * M + r].
+template <class T, int R, int P, bool VEC, int TRI, bool UNIT>
+__global__ __launch_bounds__(kNT) void gemv_n_kernel(int64_t M, int64_t N, T alpha, T beta,
+                                                     const T* __restrict__ A, int64_t lda,
+                                                     const T* __restrict__ x, int64_t incx, T* y,
+                                                     int64_t incy, int64_t cols_per, T* part) {
+  using V = typename Vec16<T>::type;
+  constexpr int VN = Vec16<T>::n;
+  __shared__ __align__(16) T xs[P];
+  const int wave = threadIdx.x / kWave;
+  const int lane = threadIdx.x % kWave;
+  const int64_t row0 = ((int64_t)blockIdx.x * (kNT / kWave) + wave) * R;
+  int64_t c_beg = (int64_t)blockIdx.y * cols_per;
+  int64_t c_end = c_beg + cols_per < N ? c_beg + cols_per : N;
+  if (TRI != kFull) {  // columns any of this workgroup's rows can touch
+    const int64_t rb0 = (int64_t)blockIdx.x * (kNT / kWave) * R;
+    const int64_t rb1 = rb0 + (kNT / kWave) * R;
+    if (TRI == kLowerTri && c_end > rb1) c_end = rb1;
+    if (TRI == kUpperTri && c_beg < rb0) c_beg = rb0 - rb0 % VN;
+  }
+  T acc[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) acc[r] = T(0);
+
+  for (int64_t p0 = c_beg; p0 < c_end; p0 += P) {
+    const int pn = (int)(c_end - p0 < P ? c_end - p0 : P);
+    __syncthreads();
+    for (int i = threadIdx.x; i < pn; i += kNT) xs[i] = x[(p0 + i) * incx];
+    __syncthreads();
+    if (VEC) {
+      const int nv = pn / VN;
+      for (int v = lane; v < nv; v += kWave) {
+        const V xv = reinterpret_cast<const V*>(xs)[v];
+        V av[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          const int64_t row = row0 + r;
+          if (row < M) av[r] = *reinterpret_cast<const V*>(A + row * lda + p0 + (int64_t)v * VN);
+        }
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          const int64_t row = row0 + r;
+          if (row < M) {
+#pragma unroll
+            for (int k = 0; k < VN; ++k) {
+              const int64_t col = p0 + (int64_t)v * VN + k;
+              bool in = true;
+              if (TRI == kLowerTri) in = UNIT ? col < row : col <= row;
+              if (TRI == kUpperTri) in = UNIT ? col > row : col >= row;
+              if (in) acc[r] += vget<T>(av[r], k) * vget<T>(xv, k);
+            }
+          }
+        }
+      }
+      for (int i = nv * VN + lane; i < pn; i += kWave) {
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          const int64_t row = row0 + r, col = p0 + i;
+          bool in = row < M;
+          if (TRI == kLowerTri) in = in && (UNIT ? col < row : col <= row);
+          if (TRI == kUpperTri) in = in && (UNIT ? col > row : col >= row);
+          if (in) acc[r] += A[row * lda + col] * xs[i];
+        }
+      }
+    } else {
+      for (int i = lane; i < pn; i += kWave) {
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          const int64_t row = row0 + r, col = p0 + i;
+          bool in = row < M;
+          if (TRI == kLowerTri) in = in && (UNIT ? col < row : col <= row);
+          if (TRI == kUpperTri) in = in && (UNIT ? col > row : col >= row);
+          if (in) acc[r] += A[row * lda + col] * xs[i];
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const T s = wave_sum(acc[r]);
+    const int64_t row = row0 + r;
+    if (lane == 0 && row < M) {
+      if (gridDim.y > 1) {
+        part[(int64_t)blockIdx.y * M + row] = s;
+      } else if (TRI != kFull) {
+        y[row * incy] = UNIT ? s + x[row * incx] : s;
+      } else {
+        y[row * incy] = beta * y[row * incy] + alpha * s;
+      }
+    }
+  }
+}
+
+// y[r] = beta*y[r] + alpha * sum_s part[s*M + r]  (or the triangular overwrite)
+template <class T, bool TRI, bool UNIT>
+__global__ __launch_bounds__(kNT) void gemv_finish_kernel(int64_t M, int splits, T alpha, T beta,
+                                                          const T* part, const T* x, int64_t incx, T* y,
+                                                          int64_t incy) {
+  const int64_t step = (int64_t)gridDim.x * kNT;
+  for (int64_t r = (int64_t)blockIdx.x * kNT + threadIdx.x; r < M; r += step) {
+    T s = T(0);
+    for (int k = 0; k < splits; ++k) s += part[(int64_t)k * M + r];
+    if (TRI)
+      y[r * incy] = UNIT ? s + x[r * incx] : s;
+    else
+      y[r * incy] = beta * y[r * incy] + alpha * s;
+  }
+}
+
+// ------------------------------------------------------------ gemv-T
+
+// y[c] = beta*y[c] + alpha * sum_r A[r,c] x[r], A is Nr x Mc (rows x cols).
+// Workgroup (bx, by): columns [bx*64*VN, +64*VN), rows [by*rows_per, +rows_per).
+template <class T, int RU, bool VEC, int TRI, bool UNIT>
+__global__ __launch_bounds__(kNT) void gemv_t_kernel(int64_t Mc, int64_t Nr, T alpha, T beta,
+                                                     const T* __restrict__ A, int64_t lda,
+                                                     const T* __restrict__ x, int64_t incx, T* y,
+                                                     int64_t incy, int64_t rows_per, T* part) {
+  using V = typename Vec16<T>::type;
+  constexpr int VN = Vec16<T>::n;
+  constexpr int NW = kNT / kWave;
+  __shared__ T red[NW][kWave * VN];
+  const int wave = threadIdx.x / kWave;
+  const int lane = threadIdx.x % kWave;
+  const int64_t c0 = (int64_t)blockIdx.x * kWave * VN + (int64_t)lane * VN;  // this lane's first column
+  int64_t r_beg = (int64_t)blockIdx.y * rows_per;
+  int64_t r_end = r_beg + rows_per < Nr ? r_beg + rows_per : Nr;
+  const int64_t cb0 = (int64_t)blockIdx.x * kWave * VN, cb1 = cb0 + kWave * VN;
+  if (TRI == kLowerTri && r_beg < cb0) r_beg = cb0;  // rows >= column
+  if (TRI == kUpperTri && r_end > cb1) r_end = cb1;  // rows <= column
+  T acc[VN];
+#pragma unroll
+  for (int k = 0; k < VN; ++k) acc[k] = T(0);
+
+  auto body = [&](int64_t r) {
+    const T xr = x[r * incx];
+    if (VEC && c0 + VN <= Mc) {
+      const V av = *reinterpret_cast<const V*>(A + r * lda + c0);
+#pragma unroll
+      for (int k = 0; k < VN; ++k) {
+        const int64_t c = c0 + k;
+        bool in = true;
+        if (TRI == kLowerTri) in = UNIT ? r > c : r >= c;
+        if (TRI == kUpperTri) in = UNIT ? r < c : r <= c;
+        if (in) acc[k] += vget<T>(av, k) * xr;
+      }
+    } else {
+#pragma unroll
+      for (int k = 0; k < VN; ++k) {
+        const int64_t c = c0 + k;
+        bool in = c < Mc;
+        if (TRI == kLowerTri) in = in && (UNIT ? r > c : r >= c);
+        if (TRI == kUpperTri) in = in && (UNIT ? r < c : r <= c);
+        if (in) acc[k] += A[r * lda + c] * xr;
+      }
+    }
+  };
+  int64_t r = r_beg + wave;
+  for (; r + (RU - 1) * NW < r_end; r += RU * NW) {
+#pragma unroll
+    for (int u = 0; u < RU; ++u) body(r + u * NW);
+  }
+  for (; r < r_end; r += NW) body(r);
+
+#pragma unroll
+  for (int k = 0; k < VN; ++k) red[wave][lane * VN + k] = acc[k];
+  __syncthreads();
+  for (int i = threadIdx.x; i < kWave * VN; i += kNT) {
+    T s = T(0);
+#pragma unroll
+    for (int w = 0; w < NW; ++w) s += red[w][i];
+    const int64_t c = cb0 + i;
+    if (c < Mc) {
+      if (gridDim.y > 1) {
+        part[(int64_t)blockIdx.y * Mc + c] = s;
+      } else if (TRI != kFull) {
+        y[c * incy] = UNIT ? s + x[c * incx] : s;
+      } else {
+        y[c * incy] = beta * y[c * incy] + alpha * s;
+      }
+    }
+  }
+}
+
+template <class T, int TRI, bool UNIT>
+void launch_gemv_n(Ctx& c, int64_t M, int64_t N, T alpha, T beta, const T* A, int64_t lda, const T* x,
+                   int64_t incx, T* y, int64_t incy) {
+  constexpr int VN = Vec16<T>::n;
+  constexpr int R = 4;
+  constexpr int P = 2048;
+  const bool vec = (lda % VN) == 0 && aligned16(A);
+  const int64_t row_blocks = ceil_div(M, (int64_t)(kNT / kWave) * R);
+  int64_t splits = 1;
+  const int64_t want = (int64_t)c.num_cus * 4;
+  if (row_blocks < want && N > 2 * P) splits = std::min<int64_t>(ceil_div(want, row_blocks), ceil_div(N, P));
+  int64_t cols_per = N > 0 ? ceil_div(ceil_div(N, splits), P) * P : P;
+  splits = N > 0 ? ceil_div(N, cols_per) : 1;
+  T* part = splits > 1 ? static_cast<T*>(c.scratch(sizeof(T) * splits * M)) : nullptr;
+  dim3 grid((unsigned)row_blocks, (unsigned)splits);
+  if (vec)
+    hipLaunchKernelGGL((gemv_n_kernel<T, R, P, true, TRI, UNIT>), grid, dim3(kNT), 0, c.stream, M, N, alpha,
+                       beta, A, lda, x, incx, y, incy, cols_per, part);
+  else
+    hipLaunchKernelGGL((gemv_n_kernel<T, R, P, false, TRI, UNIT>), grid, dim3(kNT), 0, c.stream, M, N, alpha,
+                       beta, A, lda, x, incx, y, incy, cols_per, part);
+  EXO_LAUNCH_CHECK();
+  if (splits > 1) {
+    const int64_t fb = std::min<int64_t>(ceil_div(M, kNT), 1024);
+    hipLaunchKernelGGL((gemv_finish_kernel<T, TRI != kFull, UNIT>), dim3((unsigned)fb), dim3(kNT), 0, c.stream, M,
+                       (int)splits, alpha, beta, part, x, incx, y, incy);
+    EXO_LAUNCH_CHECK();
+  }
+}
+
+template <class T, int TRI, bool UNIT>
+void launch_gemv_t(Ctx& c, int64_t Mc, int64_t Nr, T alpha, T beta, const T* A, int64_t lda, const T* x,
+                   int64_t incx, T* y, int64_t incy) {
+  constexpr int VN = Vec16<T>::n;
+  constexpr int RU = 8;
+  const bool vec = (lda % VN) == 0 && aligned16(A);
+  const int64_t col_blocks = ceil_div(Mc, (int64_t)kWave * VN);
+  const int64_t want = (int64_t)c.num_cus * 8;
+  int64_t splits = std::max<int64_t>(1, std::min<int64_t>(ceil_div(want, col_blocks), ceil_div(Nr, 64)));
+  const int64_t rows_per = Nr > 0 ? ceil_div(ceil_div(Nr, splits), 32) * 32 : 32;
+  splits = std::max<int64_t>(1, ceil_div(Nr, rows_per));
+  T* part = splits > 1 ? static_cast<T*>(c.scratch(sizeof(T) * splits * Mc)) : nullptr;
+  dim3 grid((unsigned)col_blocks, (unsigned)splits);
+  if (vec)
+    hipLaunchKernelGGL((gemv_t_kernel<T, RU, true, TRI, UNIT>), grid, dim3(kNT), 0, c.stream, Mc, Nr, alpha, beta,
+                       A, lda, x, incx, y, incy, rows_per, part);
+  else
+    hipLaunchKernelGGL((gemv_t_kernel<T, RU, false, TRI, UNIT>), grid, dim3(kNT), 0, c.stream, Mc, Nr, alpha,
+                       beta, A, lda, x, incx, y, incy, rows_per, part);
+  EXO_LAUNCH_CHECK();
+  if (splits > 1) {
+    const int64_t fb = std::min<int64_t>(ceil_div(Mc, kNT), 1024);
+    hipLaunchKernelGGL((gemv_finish_kernel<T, TRI != kFull, UNIT>), dim3((unsigned)fb), dim3(kNT), 0, c.stream,
+                       Mc, (int)splits, alpha, beta, part, x, incx, y, incy);
+    EXO_LAUNCH_CHECK();
+  }
+}
+
+// ------------------------------------------------------------ rank updates
+
+// A[i,j] += r1[i]*c1[j] + r2[i]*c2[j] over rows [0,m) and the column range
+// of TRI; r1[i] = alpha*u1[i], r2[i] = alpha*u2[i] (u2/c2 null for rank 1).
+// Workgroup (bx, by): rows by*RB .. +RB, column vectors bx*kNT .. +kNT.
+template <class T, bool VEC, int TRI, bool TWO>
+__global__ __launch_bounds__(kNT) void rank_update_kernel(int64_t m, int64_t n, T alpha, const T* u1,
+                                                          int64_t inc_u1, const T* c1, int64_t inc_c1,
+                                                          const T* u2, int64_t inc_u2, const T* c2,
+                                                          int64_t inc_c2, T* A, int64_t lda, int rows_per) {
+  using V = typename Vec16<T>::type;
+  constexpr int VN = Vec16<T>::n;
+  const int64_t j0 = ((int64_t)blockIdx.x * kNT + threadIdx.x) * VN;
+  if (j0 >= n) return;
+  const int64_t i_beg = (int64_t)blockIdx.y * rows_per;
+  const int64_t i_end = i_beg + rows_per < m ? i_beg + rows_per : m;
+  T cv1[VN], cv2[VN];
+#pragma unroll
+  for (int k = 0; k < VN; ++k) {
+    const int64_t j = j0 + k;
+    cv1[k] = j < n ? c1[j * inc_c1] : T(0);
+    cv2[k] = (TWO && j < n) ? c2[j * inc_c2] : T(0);
+  }
+  for (int64_t i = i_beg; i < i_end; ++i) {
+    int64_t jl = 0, jh = n;  // [jl, jh)
+    if (TRI == kLowerTri) jh = i + 1;
+    if (TRI == kUpperTri) jl = i;
+    if (j0 + VN <= jl || j0 >= jh) continue;
+    const T r1 = alpha * u1[i * inc_u1];
+    const T r2 = TWO ? alpha * u2[i * inc_u2] : T(0);
+    T* row = A + i * lda;
+    if (VEC && j0 >= jl && j0 + VN <= jh) {
+      V a = *reinterpret_cast<V*>(row + j0);
+#pragma unroll
+      for (int k = 0; k < VN; ++k) {
+        if (TWO)
+          vget<T>(a, k) += r1 * cv1[k] + r2 * cv2[k];
+        else
+          vget<T>(a, k) += r1 * cv1[k];
+      }
+      *reinterpret_cast<V*>(row + j0) = a;
+    } else {
+#pragma unroll
+      for (int k = 0; k < VN; ++k) {
+        const int64_t j = j0 + k;
+        if (j >= jl && j < jh) {
+          if (TWO)
+            row[j] += r1 * cv1[k] + r2 * cv2[k];
+          else
+            row[j] += r1 * cv1[k];
+        }
+      }
+    }
+  }
+}
+
+template <class T, int TRI, bool TWO>
+void launch_rank_update(Ctx& c, int64_t m, int64_t n, T alpha, const T* u1, int64_t iu1, const T* c1,
+                        int64_t ic1, const T* u2, int64_t iu2, const T* c2, int64_t ic2, T* A, int64_t lda) {
+  if (m <= 0 || n <= 0) return;
+  constexpr int VN = Vec16<T>::n;
+  const bool vec = (lda % VN) == 0 && aligned16(A);
+  const int64_t col_blocks = ceil_div(ceil_div(n, VN), kNT);
+  const int rows_per = 16;
+  dim3 grid((unsigned)col_blocks, (unsigned)ceil_div(m, rows_per));
+  if (vec)
+    hipLaunchKernelGGL((rank_update_kernel<T, true, TRI, TWO>), grid, dim3(kNT), 0, c.stream, m, n, alpha, u1, iu1,
+                       c1, ic1, u2, iu2, c2, ic2, A, lda, rows_per);
+  else
+    hipLaunchKernelGGL((rank_update_kernel<T, false, TRI, TWO>), grid, dim3(kNT), 0, c.stream, m, n, alpha, u1,
+                       iu1, c1, ic1, u2, iu2, c2, ic2, A, lda, rows_per);
+  EXO_LAUNCH_CHECK();
+}
+
+// ------------------------------------------------------------ symv
+
+constexpr int kSymTile = 64;
+
+// One workgroup per stored 64x64 tile (I, J).  UPPER selects the stored
+// triangle (I <= J for upper, I >= J for lower).  Writes
+//   part[J * n + I*64 + r] = (A_IJ x_J)[r]            (row sums -> y_I)
+//   part[I * n + J*64 + c] = (A_IJ^T x_I)[c]  I != J  (column sums -> y_J)
+template <class T, bool UPPER>
+__global__ __launch_bounds__(kNT) void symv_tile_kernel(int64_t n, const T* __restrict__ A, int64_t lda,
+                                                        const T* __restrict__ x, int64_t incx, T* part,
+                                                        int64_t nt) {
+  __shared__ T tile[kSymTile][kSymTile + 1];
+  __shared__ T xi[kSymTile], xj[kSymTile];
+  // tile index -> (I, J) with I >= J (lower enumeration), then mirrored for upper
+  const int64_t t = blockIdx.x;
+  int64_t I = (int64_t)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
+  while (I * (I + 1) / 2 > t) --I;
+  while ((I + 1) * (I + 2) / 2 <= t) ++I;
+  int64_t J = t - I * (I + 1) / 2;
+  if (UPPER) {  // stored tile (J, I) with J <= I: rename so (I, J) is the stored tile
+    const int64_t tmp = I;
+    I = J;
+    J = tmp;
+  }
+  const int64_t r0 = I * kSymTile, c0 = J * kSymTile;
+  const int tc = threadIdx.x % kSymTile;
+  const int tr = threadIdx.x / kSymTile;
+  for (int k = 0; k < kSymTile / (kNT / kSymTile); ++k) {
+    const int r = tr + k * (kNT / kSymTile);
+    const int64_t gr = r0 + r, gc = c0 + tc;
+    T v = T(0);
+    bool stored = gr < n && gc < n;
+    if (I == J) stored = stored && (UPPER ? tc >= r : tc <= r);
+    if (stored) v = A[gr * lda + gc];
+    tile[r][tc] = v;
+  }
+  if (threadIdx.x < kSymTile) {
+    const int64_t g = r0 + threadIdx.x;
+    xi[threadIdx.x] = g < n ? x[g * incx] : T(0);
+  } else if (threadIdx.x < 2 * kSymTile) {
+    const int64_t g = c0 + threadIdx.x - kSymTile;
+    xj[threadIdx.x - kSymTile] = g < n ? x[g * incx] : T(0);
+  }
+  __syncthreads();
+  if (I == J) {  // complete the symmetric diagonal tile
+    for (int k = 0; k < kSymTile / (kNT / kSymTile); ++k) {
+      const int r = tr + k * (kNT / kSymTile);
+      if (UPPER ? tc < r : tc > r) tile[r][tc] = tile[tc][r];
+    }
+    __syncthreads();
+  }
+  // 4 threads per row / column, 16 elements each
+  const int q = threadIdx.x % 4;
+  const int idx = threadIdx.x / 4;  // 0..63
+  T rs = T(0), cs = T(0);
+#pragma unroll
+  for (int k = 0; k < 16; ++k) {
+    const int cc = q * 16 + k;
+    rs += tile[idx][cc] * xj[cc];
+    cs += tile[cc][idx] * xi[cc];
+  }
+  rs += __shfl_xor(rs, 1, kWave);
+  rs += __shfl_xor(rs, 2, kWave);
+  cs += __shfl_xor(cs, 1, kWave);
+  cs += __shfl_xor(cs, 2, kWave);
+  if (q == 0) {
+    const int64_t npad = nt * kSymTile;
+    part[J * npad + r0 + idx] = rs;
+    if (I != J) part[I * npad + c0 + idx] = cs;
+  }
+}
+
+template <class T>
+__global__ __launch_bounds__(kNT) void symv_finish_kernel(int64_t n, int64_t nt, T alpha, T beta, const T* part,
+                                                          T* y, int64_t incy) {
+  const int64_t i = (int64_t)blockIdx.x * kNT + threadIdx.x;
+  if (i >= n) return;
+  const int64_t npad = nt * kSymTile;
+  T s = T(0);
+  for (int64_t o = 0; o < nt; ++o) s += part[o * npad + i];
+  y[i * incy] = beta * y[i * incy] + alpha * s;
+}
+
+}  // namespace
+
+namespace ops {
+
+// src/level2/gemv.py:10-22
+template <class T>
+void gemv_rm(Ctx& c, int64_t TransA, int64_t M, int64_t N, T alpha, T beta, const T* A, int64_t lda,
+             const T* AT, int64_t ldat, const T* x, int64_t incx, T* y, int64_t incy) {
+  if (M <= 0) return;
+  if (TransA == kNoTrans)
+    launch_gemv_n<T, kFull, false>(c, M, N, alpha, beta, A, lda, x, incx, y, incy);
+  else
+    launch_gemv_t<T, kFull, false>(c, M, N, alpha, beta, AT, ldat, x, incx, y, incy);
+}
+
+// src/level2/ger.py:9-15: A[i,j] += (alpha*x[i]) * y[j]
+template <class T>
+void ger_rm(Ctx& c, int64_t m, int64_t n, T alpha, const T* x, int64_t incx, const T* y, int64_t incy, T* A,
+            int64_t lda) {
+  launch_rank_update<T, kFull, false>(c, m, n, alpha, x, incx, y, incy, nullptr, 0, nullptr, 0, A, lda);
+}
+
+// src/level2/syr.py:10-19
+template <class T>
+void syr_rm(Ctx& c, int64_t Uplo, int64_t n, T alpha, const T* x, int64_t incx, const T* x_copy,
+            int64_t incxc, T* A, int64_t lda) {
+  if (Uplo == kUpper)
+    launch_rank_update<T, kUpperTri, false>(c, n, n, alpha, x, incx, x_copy, incxc, nullptr, 0, nullptr, 0, A,
+                                            lda);
+  else
+    launch_rank_update<T, kLowerTri, false>(c, n, n, alpha, x, incx, x_copy, incxc, nullptr, 0, nullptr, 0, A,
+                                            lda);
+}
+
+// src/level2/syr2.py:9-27: A[i,j] += (alpha*x[i])*y[j] + (alpha*y[i])*x[j]
+template <class T>
+void syr2_rm(Ctx& c, int64_t Uplo, int64_t n, T alpha, const T* x, int64_t incx, const T* x_copy,
+             int64_t incxc, const T* y, int64_t incy, const T* y_copy, int64_t incyc, T* A, int64_t lda) {
+  if (Uplo == kUpper)
+    launch_rank_update<T, kUpperTri, true>(c, n, n, alpha, x, incx, y, incy, y_copy, incyc, x_copy, incxc, A,
+                                           lda);
+  else
+    launch_rank_update<T, kLowerTri, true>(c, n, n, alpha, x, incx, y, incy, y_copy, incyc, x_copy, incxc, A,
+                                           lda);
+}
+
+// src/level2/symv.py:10-35
+template <class T>
+void symv_rm(Ctx& c, int64_t Uplo, int64_t n, T alpha, const T* A, int64_t lda, const T* x, int64_t incx,
+             T beta, T* y, int64_t incy) {
+  if (n <= 0) return;
+  const int64_t nt = ceil_div(n, kSymTile);
+  T* part = static_cast<T*>(c.scratch(sizeof(T) * nt * nt * kSymTile));
+  const int64_t tiles = nt * (nt + 1) / 2;
+  if (Uplo == kUpper)
+    hipLaunchKernelGGL((symv_tile_kernel<T, true>), dim3((unsigned)tiles), dim3(kNT), 0, c.stream, n, A, lda, x,
+                       incx, part, nt);
+  else
+    hipLaunchKernelGGL((symv_tile_kernel<T, false>), dim3((unsigned)tiles), dim3(kNT), 0, c.stream, n, A, lda, x,
+                       incx, part, nt);
+  EXO_LAUNCH_CHECK();
+  hipLaunchKernelGGL((symv_finish_kernel<T>), dim3((unsigned)ceil_div(n, kNT)), dim3(kNT), 0, c.stream, n, nt,
+                     alpha, beta, part, y, incy);
+  EXO_LAUNCH_CHECK();
+}
+
+// src/level2/trmv.py:9-43.  The product reads a copy of x (the spec's xCopy
+// buffer plays the same role) and overwrites x.  For a unit diagonal the
+// diagonal element is not read; x[i] is added instead.
+template <class T>
+void trmv_rm(Ctx& c, int64_t Uplo, int64_t TransA, int64_t Diag, int64_t n, T* x, int64_t incx, const T* A,
+             int64_t lda) {
+  if (n <= 0) return;
+  // Contiguous copy of x in scratch slot 1 (gemv partials use slot 0).
+  T* xs = static_cast<T*>(c.scratch(sizeof(T) * n, 1));
+  ops::copy<T>(c, n, x, incx, xs, 1);
+  const int64_t xinc = 1;
+  const bool up = Uplo == kUpper, tr = TransA != kNoTrans, unit = Diag == kUnit;
+  // Each variant: x = op(tri(A)) * xs.
+  auto run = [&](auto tri_tag, auto unit_tag) {
+    constexpr int TRI = decltype(tri_tag)::value;
+    constexpr bool UNIT = decltype(unit_tag)::value;
+    if (!tr)
+      launch_gemv_n<T, TRI, UNIT>(c, n, n, T(1), T(0), A, lda, xs, xinc, x, incx);
+    else
+      launch_gemv_t<T, TRI, UNIT>(c, n, n, T(1), T(0), A, lda, xs, xinc, x, incx);
+  };
+  using L = std::integral_constant<int, kLowerTri>;
+  using U = std::integral_constant<int, kUpperTri>;
+  using Y = std::integral_constant<bool, true>;
+  using N = std::integral_constant<bool, false>;
+  if (up) {
+    if (unit) run(U{}, Y{}); else run(U{}, N{});
+  } else {
+    if (unit) run(L{}, Y{}); else run(L{}, N{});
+  }
+}
+
+#define EXO_L2_INST(T)                                                                                       \
+  template void gemv_rm<T>(Ctx&, int64_t, int64_t, int64_t, T, T, const T*, int64_t, const T*, int64_t,     \
+                           const T*, int64_t, T*, int64_t);                                                 \
+  template void ger_rm<T>(Ctx&, int64_t, int64_t, T, const T*, int64_t, const T*, int64_t, T*, int64_t);     \
+  template void symv_rm<T>(Ctx&, int64_t, int64_t, T, const T*, int64_t, const T*, int64_t, T, T*, int64_t); \
+  template void syr_rm<T>(Ctx&, int64_t, int64_t, T, const T*, int64_t, const T*, int64_t, T*, int64_t);     \
+  template void syr2_rm<T>(Ctx&, int64_t, int64_t, T, const T*, int64_t, const T*, int64_t, const T*,        \
+                           int64_t, const T*, int64_t, T*, int64_t);                                        \
+  template void trmv_rm<T>(Ctx&, int64_t, int64_t, int64_t, int64_t, T*, int64_t, const T*, int64_t);
+
+EXO_L2_INST(float)
+EXO_L2_INST(double)
+
+}  // namespace ops
+}  // namespace exoblas

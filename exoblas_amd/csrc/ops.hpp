@@ -1,0 +1,70 @@
+// Kernel adapters behind the exo_* C-ABI.  One adapter per reference proc:
+// plain device pointers, element strides (negative allowed for vectors,
+// pointer at element 0), scalars by value.  Each validates nothing the
+// generated export already checked and picks a HIP kernel variant.
+#pragma once
+
+#include <cstdint>
+
+#include "runtime.hpp"
+
+namespace exoblas {
+namespace ops {
+
+// ---------------------------------------------------------------- level 1
+template <class T> void asum(Ctx&, int64_t n, const T* x, int64_t incx, T* result);
+template <class T> void axpy(Ctx&, int64_t n, T alpha, const T* x, int64_t incx, T* y, int64_t incy);
+template <class T> void axpy_alpha_1(Ctx&, int64_t n, const T* x, int64_t incx, T* y, int64_t incy);
+template <class T> void dot(Ctx&, int64_t n, const T* x, int64_t incx, const T* y, int64_t incy, T* result);
+void dsdot(Ctx&, int64_t n, const float* x, int64_t incx, const float* y, int64_t incy, double* result);
+void sdsdot(Ctx&, int64_t n, float sb, const float* x, int64_t incx, const float* y, int64_t incy,
+            float* result);
+template <class T> void copy(Ctx&, int64_t n, const T* x, int64_t incx, T* y, int64_t incy);
+template <class T> void swap(Ctx&, int64_t n, T* x, int64_t incx, T* y, int64_t incy);
+template <class T> void rot(Ctx&, int64_t n, T* x, int64_t incx, T* y, int64_t incy, T c, T s);
+template <class T> void rotm_flag_neg_one(Ctx&, int64_t n, T* x, int64_t incx, T* y, int64_t incy, const T* H);
+template <class T> void rotm_flag_zero(Ctx&, int64_t n, T* x, int64_t incx, T* y, int64_t incy, const T* H);
+template <class T> void rotm_flag_one(Ctx&, int64_t n, T* x, int64_t incx, T* y, int64_t incy, const T* H);
+template <class T> void scal(Ctx&, int64_t n, T alpha, T* x, int64_t incx);
+template <class T> void scal_alpha_0(Ctx&, int64_t n, T* x, int64_t incx);
+
+// ---------------------------------------------------------------- level 2
+template <class T>
+void gemv_rm(Ctx&, int64_t TransA, int64_t M, int64_t N, T alpha, T beta, const T* A, int64_t lda,
+             const T* AT, int64_t ldat, const T* x, int64_t incx, T* y, int64_t incy);
+template <class T>
+void ger_rm(Ctx&, int64_t m, int64_t n, T alpha, const T* x, int64_t incx, const T* y, int64_t incy, T* A,
+            int64_t lda);
+template <class T>
+void symv_rm(Ctx&, int64_t Uplo, int64_t n, T alpha, const T* A, int64_t lda, const T* x, int64_t incx,
+             T beta, T* y, int64_t incy);
+template <class T>
+void syr_rm(Ctx&, int64_t Uplo, int64_t n, T alpha, const T* x, int64_t incx, const T* x_copy,
+            int64_t incxc, T* A, int64_t lda);
+template <class T>
+void syr2_rm(Ctx&, int64_t Uplo, int64_t n, T alpha, const T* x, int64_t incx, const T* x_copy,
+             int64_t incxc, const T* y, int64_t incy, const T* y_copy, int64_t incyc, T* A, int64_t lda);
+template <class T>
+void trmv_rm(Ctx&, int64_t Uplo, int64_t TransA, int64_t Diag, int64_t n, T* x, int64_t incx, const T* A,
+             int64_t lda);
+template <class T>
+void trsv_rm(Ctx&, int64_t Uplo, int64_t TransA, int64_t Diag, int64_t n, T* x, int64_t incx, const T* A,
+             int64_t lda);
+
+// ---------------------------------------------------------------- level 3
+template <class T>
+void gemm(Ctx&, int64_t TransA, int64_t TransB, int64_t M, int64_t N, int64_t K, T alpha, const T* A,
+          int64_t lda, const T* B, int64_t ldb, const T* AT, int64_t ldat, const T* BT, int64_t ldbt, T* C,
+          int64_t ldc);
+template <class T> void mscal_rm(Ctx&, int64_t M, int64_t N, T alpha, T* A, int64_t lda);
+template <class T> void trmscal_rm(Ctx&, int64_t Uplo, int64_t N, T alpha, T* A, int64_t lda);
+template <class T>
+void symm_rm(Ctx&, int64_t Side, int64_t Uplo, int64_t M, int64_t N, T alpha, const T* Aleft,
+             int64_t ldal, const T* Aright, int64_t ldar, const T* B, int64_t ldb, T* C, int64_t ldc);
+template <class T>
+void syrk_rm(Ctx&, int64_t Uplo, int64_t Trans, int64_t N, int64_t K, T alpha, const T* A, int64_t lda,
+             const T* A_, int64_t lda_, const T* AT, int64_t ldat, const T* AT_, int64_t ldat_, T* C,
+             int64_t ldc);
+
+}  // namespace ops
+}  // namespace exoblas

@@ -1,0 +1,358 @@
+// Host runtime of the exo_* C-ABI (declarations and the two calling modes are
+// described in runtime.hpp).
+#include "runtime.hpp"
+
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+
+#include "exoblas_amd.h"
+
+namespace exoblas {
+
+struct StreamState {
+  hipStream_t stream = nullptr;
+  int device = 0;
+  void* scratch[Ctx::kSlots] = {};
+  size_t scratch_cap[Ctx::kSlots] = {};
+  unsigned* counters = nullptr;
+};
+
+namespace {
+
+// Grow-only pool of device buffers used to stage host windows.
+struct Pool {
+  struct Block {
+    void* p;
+    size_t cap;
+    bool busy;
+  };
+  std::vector<Block> blocks;
+  void* get(size_t bytes, size_t* cap) {
+    int best = -1;
+    for (size_t i = 0; i < blocks.size(); ++i)
+      if (!blocks[i].busy && blocks[i].cap >= bytes &&
+          (best < 0 || blocks[i].cap < blocks[best].cap))
+        best = (int)i;
+    if (best >= 0) {
+      blocks[best].busy = true;
+      *cap = blocks[best].cap;
+      return blocks[best].p;
+    }
+    size_t c = 1 << 16;
+    while (c < bytes) c <<= 1;
+    void* p = nullptr;
+    if (hipMalloc(&p, c) != hipSuccess) {
+      (void)hipGetLastError();
+      // drop idle blocks and retry once
+      for (auto it = blocks.begin(); it != blocks.end();) {
+        if (!it->busy) {
+          (void)hipFree(it->p);
+          it = blocks.erase(it);
+        } else {
+          ++it;
+        }
+      }
+      if (hipMalloc(&p, c) != hipSuccess) {
+        (void)hipGetLastError();
+        throw Error(kAllocError, "staging allocation of " + std::to_string(c) + " bytes failed");
+      }
+    }
+    blocks.push_back({p, c, true});
+    *cap = c;
+    return p;
+  }
+  void put(void* p) {
+    for (auto& b : blocks)
+      if (b.p == p) b.busy = false;
+  }
+};
+
+struct DeviceState {
+  std::mutex call_mutex;  // serialises kSync calls on this device
+  std::mutex map_mutex;   // guards `streams`
+  int device = 0;
+  int num_cus = 256;
+  StreamState* own = nullptr;  // the library's stream for kSync calls
+  std::map<hipStream_t, std::unique_ptr<StreamState>> streams;
+  Pool pool;
+};
+
+std::mutex g_devices_mutex;
+std::map<int, std::unique_ptr<DeviceState>> g_devices;
+
+thread_local int t_status = kOk;
+thread_local char t_message[512] = {0};
+
+StreamState* make_stream_state(int device, hipStream_t s) {
+  auto* ss = new StreamState();
+  ss->stream = s;
+  ss->device = device;
+  EXO_HIP_CHECK(hipMalloc(&ss->counters, sizeof(unsigned) * Ctx::kCounters));
+  EXO_HIP_CHECK(hipMemset(ss->counters, 0, sizeof(unsigned) * Ctx::kCounters));
+  EXO_HIP_CHECK(hipDeviceSynchronize());
+  return ss;
+}
+
+DeviceState& device_state(int device) {
+  std::lock_guard<std::mutex> g(g_devices_mutex);
+  auto it = g_devices.find(device);
+  if (it != g_devices.end()) return *it->second;
+  auto ds = std::make_unique<DeviceState>();
+  ds->device = device;
+  hipDeviceProp_t prop;
+  EXO_HIP_CHECK(hipGetDeviceProperties(&prop, device));
+  ds->num_cus = prop.multiProcessorCount;
+  hipStream_t s;
+  EXO_HIP_CHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  ds->own = make_stream_state(device, s);
+  DeviceState& ref = *ds;
+  g_devices.emplace(device, std::move(ds));
+  return ref;
+}
+
+StreamState* stream_state(DeviceState& ds, hipStream_t s) {
+  std::lock_guard<std::mutex> g(ds.map_mutex);
+  auto it = ds.streams.find(s);
+  if (it != ds.streams.end()) return it->second.get();
+  StreamState* ss = make_stream_state(ds.device, s);
+  ds.streams.emplace(s, std::unique_ptr<StreamState>(ss));
+  return ss;
+}
+
+bool is_device_pointer(const void* p) {
+  hipPointerAttribute_t a;
+  std::memset(&a, 0, sizeof(a));
+  hipError_t e = hipPointerGetAttributes(&a, p);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    return false;
+  }
+  return a.type == hipMemoryTypeDevice || a.type == hipMemoryTypeManaged;
+}
+
+}  // namespace
+
+void record_error(int code, const std::string& msg) {
+  t_status = code;
+  std::snprintf(t_message, sizeof(t_message), "%s", msg.c_str());
+  if (std::getenv("EXOBLAS_QUIET") == nullptr) std::fprintf(stderr, "exoblas_amd: %s\n", t_message);
+}
+
+void* Ctx::scratch(size_t bytes, int slot) {
+  if (bytes == 0) bytes = 256;
+  if (ss->scratch_cap[slot] < bytes) {
+    if (ss->scratch[slot]) {
+      EXO_HIP_CHECK(hipStreamSynchronize(stream));
+      EXO_HIP_CHECK(hipFree(ss->scratch[slot]));
+      ss->scratch[slot] = nullptr;
+      ss->scratch_cap[slot] = 0;
+    }
+    size_t c = 1 << 20;
+    while (c < bytes) c <<= 1;
+    EXO_HIP_CHECK(hipMalloc(&ss->scratch[slot], c));
+    ss->scratch_cap[slot] = c;
+  }
+  return ss->scratch[slot];
+}
+
+unsigned* Ctx::counters() { return ss->counters; }
+
+Call::Call(const char* name, Mode mode, void* src) : name_(name), mode_(mode) {
+  try {
+    int dev = 0;
+    EXO_HIP_CHECK(hipGetDevice(&dev));
+    DeviceState& ds = device_state(dev);
+    ctx_.device = dev;
+    ctx_.num_cus = ds.num_cus;
+    if (mode == kSync) {
+      lock_ = std::unique_lock<std::mutex>(ds.call_mutex);
+      ctx_.ss = ds.own;
+    } else {
+      ctx_.ss = stream_state(ds, static_cast<hipStream_t>(src));
+    }
+    ctx_.stream = ctx_.ss->stream;
+  } catch (const Error& e) {
+    fail(e.code, e.what());
+  }
+}
+
+Call::~Call() {
+  if (!ranges_.empty()) {
+    int dev = ctx_.device;
+    DeviceState& ds = device_state(dev);
+    for (auto& r : ranges_)
+      if (r.dev) ds.pool.put(r.dev);
+  }
+}
+
+int Call::fail(int code, const std::string& msg) {
+  if (status_ == kOk) {
+    status_ = code;
+    record_error(code, std::string(name_) + ": " + msg);
+  }
+  return status_;
+}
+
+int Call::fail_enum(const char* arg, int64_t v) {
+  return fail(kInvalidEnum, std::string(arg) + " = " + std::to_string(v) + " is not a valid CBLAS value");
+}
+int Call::fail_size(const char* arg, int64_t v) {
+  return fail(kInvalidSize, std::string(arg) + " = " + std::to_string(v) + " is negative");
+}
+int Call::fail_stride(const char* arg, int64_t v) {
+  return fail(kInvalidStride, std::string(arg) + " has column stride " + std::to_string(v) + " (must be 1)");
+}
+int Call::fail_ld(const char* arg, int64_t v) {
+  return fail(kInvalidLd, std::string(arg) + " has leading dimension " + std::to_string(v) +
+                              " smaller than its row length");
+}
+
+void Call::window(int idx, const void* p, int64_t lo, int64_t hi, size_t esize, bool write, bool used,
+                  bool result_only) {
+  Win& w = wins_[idx];
+  w.p = static_cast<const char*>(p);
+  w.lo = lo;
+  w.hi = hi;
+  w.esize = esize;
+  w.write = write;
+  w.used = used && hi > lo;
+  w.result_only = result_only;
+  w.dev = used ? const_cast<void*>(p) : nullptr;
+  nwin_ = std::max(nwin_, idx + 1);
+}
+
+bool Call::stage() {
+  if (status_ != kOk) return false;
+  if (mode_ == kAsync) {
+    for (int i = 0; i < nwin_; ++i)
+      if (wins_[i].used && wins_[i].p == nullptr) {
+        fail(kInvalidPointer, "null data pointer");
+        return false;
+      }
+    return true;
+  }
+  try {
+    // Host windows, merged where their byte ranges overlap.
+    struct Item {
+      const char* b;
+      const char* e;
+      int win;
+    };
+    std::vector<Item> host;
+    for (int i = 0; i < nwin_; ++i) {
+      Win& w = wins_[i];
+      if (!w.used) continue;
+      if (w.p == nullptr) {
+        fail(kInvalidPointer, "null data pointer");
+        return false;
+      }
+      const char* b = w.p + w.lo * (int64_t)w.esize;
+      const char* e = w.p + w.hi * (int64_t)w.esize;
+      if (is_device_pointer(b)) continue;
+      host.push_back({b, e, i});
+    }
+    std::sort(host.begin(), host.end(), [](const Item& a, const Item& b) { return a.b < b.b; });
+    for (auto& it : host) {
+      Win& w = wins_[it.win];
+      if (ranges_.empty() || it.b >= ranges_.back().end) {
+        ranges_.push_back({it.b, it.e, false, false, nullptr, 0});
+      } else {
+        ranges_.back().end = std::max(ranges_.back().end, it.e);
+      }
+      Range& r = ranges_.back();
+      r.read = r.read || !w.result_only;
+      r.write = r.write || w.write;
+      w.range = (int)ranges_.size() - 1;
+    }
+    DeviceState& ds = device_state(ctx_.device);
+    for (auto& r : ranges_) {
+      size_t bytes = (size_t)(r.end - r.begin);
+      r.dev = ds.pool.get(bytes, &r.cap);
+      if (r.read)
+        EXO_HIP_CHECK(hipMemcpyAsync(r.dev, r.begin, bytes, hipMemcpyHostToDevice, ctx_.stream));
+    }
+    for (int i = 0; i < nwin_; ++i) {
+      Win& w = wins_[i];
+      if (w.range >= 0) {
+        Range& r = ranges_[w.range];
+        w.dev = static_cast<char*>(r.dev) + (w.p - r.begin);
+      }
+    }
+  } catch (const Error& e) {
+    fail(e.code, e.what());
+    return false;
+  }
+  staged_ = true;
+  return true;
+}
+
+void Call::read_host(const void* p, void* out, size_t bytes) {
+  if (status_ != kOk) return;
+  if (p == nullptr) {
+    fail(kInvalidPointer, "null scalar pointer");
+    return;
+  }
+  if (mode_ == kSync && is_device_pointer(p)) {
+    hipError_t e = hipMemcpy(out, p, bytes, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) fail(kHipError, hipGetErrorString(e));
+    return;
+  }
+  std::memcpy(out, p, bytes);
+}
+
+int Call::finish() {
+  try {
+    if (status_ == kOk) EXO_LAUNCH_CHECK();
+    if (mode_ == kSync && ctx_.stream) {
+      if (status_ == kOk)
+        for (auto& r : ranges_)
+          if (r.write)
+            EXO_HIP_CHECK(hipMemcpyAsync(const_cast<char*>(r.begin), r.dev, (size_t)(r.end - r.begin),
+                                         hipMemcpyDeviceToHost, ctx_.stream));
+      EXO_HIP_CHECK(hipStreamSynchronize(ctx_.stream));
+    }
+  } catch (const Error& e) {
+    fail(e.code, e.what());
+  }
+  return status_;
+}
+
+}  // namespace exoblas
+
+// ------------------------------------------------------------ C utilities
+
+extern "C" int exoblas_last_status(void) { return exoblas::t_status; }
+extern "C" const char* exoblas_last_error(void) { return exoblas::t_message; }
+extern "C" void exoblas_clear_error(void) {
+  exoblas::t_status = exoblas::kOk;
+  exoblas::t_message[0] = 0;
+}
+
+extern "C" int exoblas_init(void) {
+  try {
+    int dev = 0;
+    EXO_HIP_CHECK(hipGetDevice(&dev));
+    exoblas::device_state(dev);
+  } catch (const exoblas::Error& e) {
+    exoblas::record_error(e.code, e.what());
+    return e.code;
+  }
+  return 0;
+}
+
+extern "C" int exoblas_synchronize(void) {
+  try {
+    int dev = 0;
+    EXO_HIP_CHECK(hipGetDevice(&dev));
+    auto& ds = exoblas::device_state(dev);
+    EXO_HIP_CHECK(hipStreamSynchronize(ds.own->stream));
+  } catch (const exoblas::Error& e) {
+    exoblas::record_error(e.code, e.what());
+    return e.code;
+  }
+  return 0;
+}

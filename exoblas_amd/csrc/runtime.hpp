@@ -1,0 +1,166 @@
+// Host runtime of the exo_* C-ABI: per-device state, stream-ordered scratch,
+// host-window staging and error reporting.
+//
+// Two calling modes share one set of kernels:
+//   kSync  -- the drop-in exo_* symbols.  Windows may point at host or device
+//             memory; host ranges are staged through pooled device buffers and
+//             the call returns after results are back in the caller's memory
+//             (the reference's functions are synchronous, test/level*/<k>/
+//             exo_<k>_wrapper.h).  The `void *ctxt` argument is ignored, as the
+//             reference always passes nullptr.
+//   kAsync -- the exoblas_*_async symbols.  Windows and results are device
+//             pointers; work is enqueued on the caller's stream.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace exoblas {
+
+enum Status : int {
+  kOk = 0,
+  kInvalidEnum = 1,
+  kInvalidSize = 2,
+  kInvalidStride = 3,
+  kInvalidLd = 4,
+  kHipError = 5,
+  kAllocError = 6,
+  kNotSupported = 7,
+  kInvalidPointer = 8,
+  kTimeout = 9,
+};
+
+// Element offset range [lo, hi) covered by a 1-D window of n elements with
+// stride s (element 0 at offset 0; negative strides extend downwards).
+inline int64_t span_lo(int64_t n, int64_t s) { return (n > 0 && s < 0) ? (n - 1) * s : 0; }
+inline int64_t span_hi(int64_t n, int64_t s) { return n > 0 ? (s > 0 ? (n - 1) * s : 0) + 1 : 0; }
+// Row-major rows x cols window with leading dimension ld and unit column stride.
+inline int64_t span2_hi(int64_t rows, int64_t cols, int64_t ld) {
+  return (rows > 0 && cols > 0) ? (rows - 1) * ld + cols : 0;
+}
+
+// Raised inside ops; converted to a status by Call::run.
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define EXO_HIP_CHECK(expr)                                                          \
+  do {                                                                               \
+    hipError_t _e = (expr);                                                          \
+    if (_e != hipSuccess)                                                            \
+      throw ::exoblas::Error(::exoblas::kHipError,                                   \
+                             std::string(#expr) + ": " + hipGetErrorString(_e));     \
+  } while (0)
+
+// Checks the launch that was just enqueued.
+#define EXO_LAUNCH_CHECK() EXO_HIP_CHECK(hipGetLastError())
+
+struct StreamState;
+
+// What an op launches with.
+struct Ctx {
+  hipStream_t stream = nullptr;
+  int device = 0;
+  int num_cus = 256;
+  StreamState* ss = nullptr;
+
+  // Stream-ordered device scratch of at least `bytes`, 256-byte aligned, in
+  // one of kSlots independent buffers.  A slot's memory is valid until the
+  // next scratch() call for that slot on this stream.  Growing a slot
+  // synchronises the stream first, so earlier kernels never lose theirs.
+  void* scratch(size_t bytes, int slot = 0);
+  static constexpr int kSlots = 4;
+  // kCounters zero-initialised 32-bit counters owned by this stream.  A kernel
+  // that uses them must leave them zero when it ends.
+  unsigned* counters();
+  static constexpr int kCounters = 4096;
+};
+
+void record_error(int code, const std::string& msg);
+
+class Call {
+ public:
+  enum Mode { kSync, kAsync };
+  Call(const char* name, Mode mode, void* ctxt_or_stream);
+  ~Call();
+
+  int fail_enum(const char* arg, int64_t v);
+  int fail_size(const char* arg, int64_t v);
+  int fail_stride(const char* arg, int64_t v);
+  int fail_ld(const char* arg, int64_t v);
+
+  // Registers window `idx`: element range [lo, hi) around p, of esize bytes each.
+  void window(int idx, const void* p, int64_t lo, int64_t hi, size_t esize, bool write, bool used,
+              bool result_only = false);
+  bool stage();
+
+  template <class T>
+  T scalar(const T* p) {
+    T v{};
+    read_host(p, &v, sizeof(T));
+    return v;
+  }
+  template <class T>
+  void array(const T* p, T* out, int n) {
+    read_host(p, out, sizeof(T) * n);
+  }
+  template <class T>
+  T* dev(int idx) {
+    return static_cast<T*>(wins_[idx].dev);
+  }
+
+  template <class F>
+  void run(F&& f) {
+    if (status_ != kOk) return;
+    try {
+      f();
+    } catch (const Error& e) {
+      fail(e.code, e.what());
+    } catch (const std::exception& e) {
+      fail(kHipError, e.what());
+    }
+  }
+
+  bool ok() const { return status_ == kOk; }
+  int status() const { return status_; }
+  int finish();
+  Ctx& ctx() { return ctx_; }
+
+ private:
+  struct Win {
+    const char* p = nullptr;
+    int64_t lo = 0, hi = 0;
+    size_t esize = 0;
+    bool write = false, used = false, result_only = false;
+    void* dev = nullptr;
+    int range = -1;
+  };
+  struct Range {
+    const char* begin;
+    const char* end;
+    bool read, write;
+    void* dev;
+    size_t cap;
+  };
+  int fail(int code, const std::string& msg);
+  void read_host(const void* p, void* out, size_t bytes);
+
+  const char* name_;
+  Mode mode_;
+  int status_ = kOk;
+  Ctx ctx_;
+  std::unique_lock<std::mutex> lock_;
+  Win wins_[16];
+  int nwin_ = 0;
+  std::vector<Range> ranges_;
+  bool staged_ = false;
+};
+
+}  // namespace exoblas

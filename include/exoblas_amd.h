@@ -1,0 +1,40 @@
+/* exoblas_amd: library-level utilities of the MI355X (gfx950) BLAS.
+ *
+ * The drop-in exo_* entry points are declared in exo_<kernel>.h (one header
+ * per reference library target) and their stream-ordered device-pointer forms
+ * in exoblas_amd_async.h.  None of the functions below has a counterpart in
+ * the reference, whose functions report no errors (every exo_* returns void).
+ */
+#ifndef EXOBLAS_AMD_H
+#define EXOBLAS_AMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes of the last call on this host thread. */
+#define EXOBLAS_STATUS_OK 0
+#define EXOBLAS_STATUS_INVALID_ENUM 1
+#define EXOBLAS_STATUS_INVALID_SIZE 2
+#define EXOBLAS_STATUS_INVALID_STRIDE 3
+#define EXOBLAS_STATUS_INVALID_LD 4
+#define EXOBLAS_STATUS_HIP_ERROR 5
+#define EXOBLAS_STATUS_ALLOC_ERROR 6
+#define EXOBLAS_STATUS_NOT_SUPPORTED 7
+#define EXOBLAS_STATUS_INVALID_POINTER 8
+#define EXOBLAS_STATUS_TIMEOUT 9
+
+/* Creates the per-device state (stream, counters) for the current HIP device. */
+int exoblas_init(void);
+/* Status and message of the most recent failing call on this host thread. */
+int exoblas_last_status(void);
+const char *exoblas_last_error(void);
+void exoblas_clear_error(void);
+/* Waits for the library's own stream on the current device. */
+int exoblas_synchronize(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* EXOBLAS_AMD_H */
