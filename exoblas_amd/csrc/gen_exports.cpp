@@ -23,6 +23,7 @@
 #include "exo_symm.h"
 #include "exo_syrk.h"
 #include "exoblas_amd_async.h"
+#include "exoblas_amd_ext.h"
 
 // src/level1/asum.py:11-15
 extern "C" void exo_sasum_stride_any(void *ctxt, int_fast32_t n, struct exo_win_1f32c x, float *result) {
@@ -3089,6 +3090,112 @@ extern "C" int exoblas_dsyrk_rm_stride_1_async(hipStream_t stream, int_fast32_t 
   const double alpha_v = call.scalar(alpha);
   if (!call.ok()) return call.status();
   call.run([&] { exoblas::ops::syrk_rm<double>(call.ctx(), (int64_t)Uplo, (int64_t)Trans, (int64_t)N, (int64_t)K, alpha_v, call.dev<const double>(0), (int64_t)A.strides[0], call.dev<const double>(1), (int64_t)A_.strides[0], call.dev<const double>(2), (int64_t)AT.strides[0], call.dev<const double>(3), (int64_t)AT_.strides[0], call.dev<double>(4), (int64_t)C.strides[0]); });
+  return call.finish();
+}
+
+// C += alpha op(A) op(B) with FP32 operands split into BF16 terms on the BF16 MFMA; arguments as exo_sgemm_stride_1 (src/level3/gemm.py:17-47) plus the number of BF16 products
+extern "C" int exoblas_sgemm_split(void *ctxt, int_fast32_t TransA, int_fast32_t TransB, int_fast32_t M, int_fast32_t N, int_fast32_t K, const float *alpha, struct exo_win_2f32c A, struct exo_win_2f32c B, struct exo_win_2f32c AT, struct exo_win_2f32c BT, struct exo_win_2f32 C, int_fast32_t terms) {
+  exoblas::Call call("exoblas_sgemm_split", exoblas::Call::kSync, ctxt);
+  if (!(TransA == 111 || TransA == 112 || TransA == 113)) return call.fail_enum("TransA", TransA);
+  if (!(TransB == 111 || TransB == 112 || TransB == 113)) return call.fail_enum("TransB", TransB);
+  if (M < 0) return call.fail_size("M", M);
+  if (N < 0) return call.fail_size("N", N);
+  if (K < 0) return call.fail_size("K", K);
+  if (!(terms == 3 || terms == 6)) return call.fail_enum("terms", terms);
+  if ((TransA == 111) && A.strides[1] != 1) return call.fail_stride("A", A.strides[1]);
+  if ((TransA == 111) && M > 1 && A.strides[0] < K) return call.fail_ld("A", A.strides[0]);
+  if ((TransB == 111) && B.strides[1] != 1) return call.fail_stride("B", B.strides[1]);
+  if ((TransB == 111) && K > 1 && B.strides[0] < N) return call.fail_ld("B", B.strides[0]);
+  if ((TransA != 111) && AT.strides[1] != 1) return call.fail_stride("AT", AT.strides[1]);
+  if ((TransA != 111) && K > 1 && AT.strides[0] < M) return call.fail_ld("AT", AT.strides[0]);
+  if ((TransB != 111) && BT.strides[1] != 1) return call.fail_stride("BT", BT.strides[1]);
+  if ((TransB != 111) && N > 1 && BT.strides[0] < K) return call.fail_ld("BT", BT.strides[0]);
+  if ((true) && C.strides[1] != 1) return call.fail_stride("C", C.strides[1]);
+  if ((true) && M > 1 && C.strides[0] < N) return call.fail_ld("C", C.strides[0]);
+  call.window(0, A.data, 0, exoblas::span2_hi(M, K, A.strides[0]), sizeof(float), false, TransA == 111);
+  call.window(1, B.data, 0, exoblas::span2_hi(K, N, B.strides[0]), sizeof(float), false, TransB == 111);
+  call.window(2, AT.data, 0, exoblas::span2_hi(K, M, AT.strides[0]), sizeof(float), false, TransA != 111);
+  call.window(3, BT.data, 0, exoblas::span2_hi(N, K, BT.strides[0]), sizeof(float), false, TransB != 111);
+  call.window(4, C.data, 0, exoblas::span2_hi(M, N, C.strides[0]), sizeof(float), true, true);
+  if (!call.stage()) return call.status();
+  const float alpha_v = call.scalar(alpha);
+  if (!call.ok()) return call.status();
+  call.run([&] { exoblas::ops::gemm_split(call.ctx(), (int64_t)TransA, (int64_t)TransB, (int64_t)M, (int64_t)N, (int64_t)K, alpha_v, call.dev<const float>(0), (int64_t)A.strides[0], call.dev<const float>(1), (int64_t)B.strides[0], call.dev<const float>(2), (int64_t)AT.strides[0], call.dev<const float>(3), (int64_t)BT.strides[0], call.dev<float>(4), (int64_t)C.strides[0], (int64_t)terms); });
+  return call.finish();
+}
+extern "C" int exoblas_sgemm_split_async(hipStream_t stream, int_fast32_t TransA, int_fast32_t TransB, int_fast32_t M, int_fast32_t N, int_fast32_t K, const float *alpha, struct exo_win_2f32c A, struct exo_win_2f32c B, struct exo_win_2f32c AT, struct exo_win_2f32c BT, struct exo_win_2f32 C, int_fast32_t terms) {
+  exoblas::Call call("exoblas_sgemm_split_async", exoblas::Call::kAsync, stream);
+  if (!(TransA == 111 || TransA == 112 || TransA == 113)) return call.fail_enum("TransA", TransA);
+  if (!(TransB == 111 || TransB == 112 || TransB == 113)) return call.fail_enum("TransB", TransB);
+  if (M < 0) return call.fail_size("M", M);
+  if (N < 0) return call.fail_size("N", N);
+  if (K < 0) return call.fail_size("K", K);
+  if (!(terms == 3 || terms == 6)) return call.fail_enum("terms", terms);
+  if ((TransA == 111) && A.strides[1] != 1) return call.fail_stride("A", A.strides[1]);
+  if ((TransA == 111) && M > 1 && A.strides[0] < K) return call.fail_ld("A", A.strides[0]);
+  if ((TransB == 111) && B.strides[1] != 1) return call.fail_stride("B", B.strides[1]);
+  if ((TransB == 111) && K > 1 && B.strides[0] < N) return call.fail_ld("B", B.strides[0]);
+  if ((TransA != 111) && AT.strides[1] != 1) return call.fail_stride("AT", AT.strides[1]);
+  if ((TransA != 111) && K > 1 && AT.strides[0] < M) return call.fail_ld("AT", AT.strides[0]);
+  if ((TransB != 111) && BT.strides[1] != 1) return call.fail_stride("BT", BT.strides[1]);
+  if ((TransB != 111) && N > 1 && BT.strides[0] < K) return call.fail_ld("BT", BT.strides[0]);
+  if ((true) && C.strides[1] != 1) return call.fail_stride("C", C.strides[1]);
+  if ((true) && M > 1 && C.strides[0] < N) return call.fail_ld("C", C.strides[0]);
+  call.window(0, A.data, 0, exoblas::span2_hi(M, K, A.strides[0]), sizeof(float), false, TransA == 111);
+  call.window(1, B.data, 0, exoblas::span2_hi(K, N, B.strides[0]), sizeof(float), false, TransB == 111);
+  call.window(2, AT.data, 0, exoblas::span2_hi(K, M, AT.strides[0]), sizeof(float), false, TransA != 111);
+  call.window(3, BT.data, 0, exoblas::span2_hi(N, K, BT.strides[0]), sizeof(float), false, TransB != 111);
+  call.window(4, C.data, 0, exoblas::span2_hi(M, N, C.strides[0]), sizeof(float), true, true);
+  if (!call.stage()) return call.status();
+  const float alpha_v = call.scalar(alpha);
+  if (!call.ok()) return call.status();
+  call.run([&] { exoblas::ops::gemm_split(call.ctx(), (int64_t)TransA, (int64_t)TransB, (int64_t)M, (int64_t)N, (int64_t)K, alpha_v, call.dev<const float>(0), (int64_t)A.strides[0], call.dev<const float>(1), (int64_t)B.strides[0], call.dev<const float>(2), (int64_t)AT.strides[0], call.dev<const float>(3), (int64_t)BT.strides[0], call.dev<float>(4), (int64_t)C.strides[0], (int64_t)terms); });
+  return call.finish();
+}
+
+// one triangle of C += alpha A A^T (or A^T A) in the split BF16 form; arguments as exo_ssyrk_rm_stride_1 (src/level3/syrk.py:17-32) without the duplicate windows
+extern "C" int exoblas_ssyrk_split(void *ctxt, int_fast32_t Uplo, int_fast32_t Trans, int_fast32_t N, int_fast32_t K, const float *alpha, struct exo_win_2f32c A, struct exo_win_2f32c AT, struct exo_win_2f32 C, int_fast32_t terms) {
+  exoblas::Call call("exoblas_ssyrk_split", exoblas::Call::kSync, ctxt);
+  if (!(Uplo == 121 || Uplo == 122)) return call.fail_enum("Uplo", Uplo);
+  if (!(Trans == 111 || Trans == 112 || Trans == 113)) return call.fail_enum("Trans", Trans);
+  if (N < 0) return call.fail_size("N", N);
+  if (K < 0) return call.fail_size("K", K);
+  if (!(terms == 3 || terms == 6)) return call.fail_enum("terms", terms);
+  if ((Trans == 111) && A.strides[1] != 1) return call.fail_stride("A", A.strides[1]);
+  if ((Trans == 111) && N > 1 && A.strides[0] < K) return call.fail_ld("A", A.strides[0]);
+  if ((Trans != 111) && AT.strides[1] != 1) return call.fail_stride("AT", AT.strides[1]);
+  if ((Trans != 111) && K > 1 && AT.strides[0] < N) return call.fail_ld("AT", AT.strides[0]);
+  if ((true) && C.strides[1] != 1) return call.fail_stride("C", C.strides[1]);
+  if ((true) && N > 1 && C.strides[0] < N) return call.fail_ld("C", C.strides[0]);
+  call.window(0, A.data, 0, exoblas::span2_hi(N, K, A.strides[0]), sizeof(float), false, Trans == 111);
+  call.window(1, AT.data, 0, exoblas::span2_hi(K, N, AT.strides[0]), sizeof(float), false, Trans != 111);
+  call.window(2, C.data, 0, exoblas::span2_hi(N, N, C.strides[0]), sizeof(float), true, true);
+  if (!call.stage()) return call.status();
+  const float alpha_v = call.scalar(alpha);
+  if (!call.ok()) return call.status();
+  call.run([&] { exoblas::ops::syrk_split(call.ctx(), (int64_t)Uplo, (int64_t)Trans, (int64_t)N, (int64_t)K, alpha_v, call.dev<const float>(0), (int64_t)A.strides[0], call.dev<const float>(1), (int64_t)AT.strides[0], call.dev<float>(2), (int64_t)C.strides[0], (int64_t)terms); });
+  return call.finish();
+}
+extern "C" int exoblas_ssyrk_split_async(hipStream_t stream, int_fast32_t Uplo, int_fast32_t Trans, int_fast32_t N, int_fast32_t K, const float *alpha, struct exo_win_2f32c A, struct exo_win_2f32c AT, struct exo_win_2f32 C, int_fast32_t terms) {
+  exoblas::Call call("exoblas_ssyrk_split_async", exoblas::Call::kAsync, stream);
+  if (!(Uplo == 121 || Uplo == 122)) return call.fail_enum("Uplo", Uplo);
+  if (!(Trans == 111 || Trans == 112 || Trans == 113)) return call.fail_enum("Trans", Trans);
+  if (N < 0) return call.fail_size("N", N);
+  if (K < 0) return call.fail_size("K", K);
+  if (!(terms == 3 || terms == 6)) return call.fail_enum("terms", terms);
+  if ((Trans == 111) && A.strides[1] != 1) return call.fail_stride("A", A.strides[1]);
+  if ((Trans == 111) && N > 1 && A.strides[0] < K) return call.fail_ld("A", A.strides[0]);
+  if ((Trans != 111) && AT.strides[1] != 1) return call.fail_stride("AT", AT.strides[1]);
+  if ((Trans != 111) && K > 1 && AT.strides[0] < N) return call.fail_ld("AT", AT.strides[0]);
+  if ((true) && C.strides[1] != 1) return call.fail_stride("C", C.strides[1]);
+  if ((true) && N > 1 && C.strides[0] < N) return call.fail_ld("C", C.strides[0]);
+  call.window(0, A.data, 0, exoblas::span2_hi(N, K, A.strides[0]), sizeof(float), false, Trans == 111);
+  call.window(1, AT.data, 0, exoblas::span2_hi(K, N, AT.strides[0]), sizeof(float), false, Trans != 111);
+  call.window(2, C.data, 0, exoblas::span2_hi(N, N, C.strides[0]), sizeof(float), true, true);
+  if (!call.stage()) return call.status();
+  const float alpha_v = call.scalar(alpha);
+  if (!call.ok()) return call.status();
+  call.run([&] { exoblas::ops::syrk_split(call.ctx(), (int64_t)Uplo, (int64_t)Trans, (int64_t)N, (int64_t)K, alpha_v, call.dev<const float>(0), (int64_t)A.strides[0], call.dev<const float>(1), (int64_t)AT.strides[0], call.dev<float>(2), (int64_t)C.strides[0], (int64_t)terms); });
   return call.finish();
 }
 

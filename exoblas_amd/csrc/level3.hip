@@ -1,0 +1,244 @@
+// Level-3 kernels for gfx950: gemm / symm / syrk on the MFMA tile engine
+// (gemm_core.hpp) and the mscal / trmscal beta passes.
+//
+// Every product is  C[i, j] += alpha * sum_k X_A(i, k) X_B(j, k)  with the
+// operand forms chosen on the host from TransA/TransB, Side/Uplo or
+// Uplo/Trans; syrk restricts the update to one triangle of C.  beta is not
+// applied here: the reference's wrappers scale C first with a separate
+// mscal / trmscal call (test/level3/gemm/exo_gemm_wrapper.h:18-20,
+// test/level3/syrk/exo_syrk_wrapper.h:18-20), and the same split is kept.
+#include "device.hpp"
+#include "gemm_core.hpp"
+#include "level3.hpp"
+#include "ops.hpp"
+
+namespace exoblas {
+namespace gemm {
+
+template <class C, int TRI>
+__global__ __launch_bounds__(C::NT) void gemm_kernel(int64_t M, int64_t N, int64_t K, typename C::type alpha,
+                                                     Operand<typename C::type> A, Operand<typename C::type> B,
+                                                     typename C::type* Cp, int64_t ldc, int cvec, int64_t tiles_m,
+                                                     int64_t tiles_n) {
+  using T = typename C::type;
+  extern __shared__ __align__(16) unsigned char smem[];
+  int64_t tm, tn;
+  tile_coords(blockIdx.x, gridDim.x, tiles_m, tiles_n, tm, tn);
+  const int64_t m0 = tm * C::BM, n0 = tn * C::BN;
+  if (TRI == kLower && n0 > m0 + C::BM - 1) return;
+  if (TRI == kUpper && n0 + C::BN - 1 < m0) return;
+  typename Mfma<T>::acc acc[C::SM][C::SN];
+#pragma unroll
+  for (int a = 0; a < C::SM; ++a)
+#pragma unroll
+    for (int b = 0; b < C::SN; ++b)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) acc[a][b][q] = T(0);
+  tile_mma<C>(A, B, M, N, K, m0, n0, 0, K, acc, reinterpret_cast<T*>(smem));
+  tile_store<C, TRI>(Cp, ldc, cvec, M, N, m0, n0, alpha, acc);
+}
+
+template <class C, int TRI>
+void launch_cfg(Ctx& c, int64_t M, int64_t N, int64_t K, typename C::type alpha,
+                const Operand<typename C::type>& A, const Operand<typename C::type>& B, typename C::type* Cp,
+                int64_t ldc) {
+  using T = typename C::type;
+  const int64_t tm = ceil_div(M, C::BM), tn = ceil_div(N, C::BN);
+  const int cvec = (ldc % C::EB == 0) && (reinterpret_cast<uintptr_t>(Cp) % (C::EB * sizeof(T)) == 0);
+  static bool attr_set = false;  // dynamic LDS above 64 KiB needs the attribute
+  if (!attr_set) {
+    EXO_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&gemm_kernel<C, TRI>),
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)C::LDS_BYTES));
+    attr_set = true;
+  }
+  hipLaunchKernelGGL((gemm_kernel<C, TRI>), dim3((unsigned)(tm * tn)), dim3(C::NT), C::LDS_BYTES, c.stream, M, N, K,
+                     alpha, A, B, Cp, ldc, cvec, tm, tn);
+  EXO_LAUNCH_CHECK();
+}
+
+template <class T> struct Tiles;
+template <> struct Tiles<float> {
+  using Big = Cfg<float, 128, 128, 16, 64, 64>;
+  using Small = Cfg<float, 64, 64, 16, 32, 32>;
+};
+template <> struct Tiles<double> {
+  using Big = Cfg<double, 128, 128, 16, 64, 64>;
+  using Small = Cfg<double, 64, 64, 16, 32, 32>;
+};
+
+template <class T, int TRI>
+void run(Ctx& c, int64_t M, int64_t N, int64_t K, T alpha, const Operand<T>& A, const Operand<T>& B, T* Cp,
+         int64_t ldc) {
+  if (M <= 0 || N <= 0 || K <= 0) return;
+  // Big tiles while they still give about one workgroup per CU.
+  const int64_t big = ceil_div(M, 128) * ceil_div(N, 128);
+  if (big >= c.num_cus)
+    launch_cfg<typename Tiles<T>::Big, TRI>(c, M, N, K, alpha, A, B, Cp, ldc);
+  else
+    launch_cfg<typename Tiles<T>::Small, TRI>(c, M, N, K, alpha, A, B, Cp, ldc);
+}
+
+template <class T>
+Operand<T> operand(const T* p, int64_t ld, int form) {
+  constexpr int VN = 16 / sizeof(T);
+  Operand<T> o;
+  o.p = p;
+  o.ld = ld;
+  o.form = form;
+  o.vec = (ld % VN == 0) && aligned16(p);
+  return o;
+}
+
+template <class T>
+void product(Ctx& c, int tri, int64_t M, int64_t N, int64_t K, T alpha, const Operand<T>& A, const Operand<T>& B,
+             T* Cp, int64_t ldc) {
+  if (tri == kLower)
+    run<T, kLower>(c, M, N, K, alpha, A, B, Cp, ldc);
+  else if (tri == kUpper)
+    run<T, kUpper>(c, M, N, K, alpha, A, B, Cp, ldc);
+  else
+    run<T, kFull>(c, M, N, K, alpha, A, B, Cp, ldc);
+}
+
+}  // namespace gemm
+
+namespace {
+
+constexpr int kNT = 256;
+constexpr int64_t kNoTrans = 111;
+constexpr int64_t kUpperV = 121;
+constexpr int64_t kLeftV = 141;
+
+// A[i, j] *= alpha over the full M x N matrix (TRI 0) or the lower (1) /
+// upper (2) triangle of an N x N matrix.  blockIdx.y picks the row (i0 +
+// blockIdx.y); VEC uses 16-byte accesses for the aligned body of the row's
+// column range and scalar accesses for its ragged ends.
+template <class T, int TRI, bool VEC>
+__global__ __launch_bounds__(kNT) void mscal_kernel(int64_t i0, int64_t N, T alpha, T* A, int64_t lda) {
+  constexpr int VN = Vec16<T>::n;
+  using V = typename Vec16<T>::type;
+  const int64_t i = i0 + blockIdx.y;
+  int64_t jl = 0, jh = N;
+  if (TRI == 1) jh = i + 1 < N ? i + 1 : N;
+  if (TRI == 2) jl = i;
+  T* row = A + i * lda;
+  const int64_t tid = (int64_t)blockIdx.x * kNT + threadIdx.x, step = (int64_t)gridDim.x * kNT;
+  if (VEC) {
+    const int64_t a = std::min<int64_t>(ceil_div(jl, VN) * VN, jh);
+    const int64_t b = std::max<int64_t>(jh / VN * VN, a);
+    for (int64_t v = a / VN + tid; v < b / VN; v += step) {
+      V x = reinterpret_cast<V*>(row)[v];
+#pragma unroll
+      for (int e = 0; e < VN; ++e) vget<T>(x, e) = vget<T>(x, e) * alpha;
+      reinterpret_cast<V*>(row)[v] = x;
+    }
+    if (tid < VN) {
+      const int64_t h = jl + tid;
+      if (h < a) row[h] = row[h] * alpha;
+      const int64_t t = b + tid;
+      if (t < jh) row[t] = row[t] * alpha;
+    }
+  } else {
+    for (int64_t j = jl + tid; j < jh; j += step) row[j] = row[j] * alpha;
+  }
+}
+
+template <class T, int TRI>
+void launch_mscal(Ctx& c, int64_t M, int64_t N, T alpha, T* A, int64_t lda) {
+  if (M <= 0 || N <= 0) return;
+  constexpr int VN = Vec16<T>::n;
+  const bool vec = lda % VN == 0 && aligned16(A);
+  const int64_t per_row = vec ? ceil_div(N, VN) : N;
+  const unsigned gx = (unsigned)std::max<int64_t>(1, std::min<int64_t>(ceil_div(per_row, kNT), 64));
+  for (int64_t r0 = 0; r0 < M; r0 += 65535) {
+    const int64_t rows = std::min<int64_t>(65535, M - r0);
+    dim3 grid(gx, (unsigned)rows);
+    if (vec)
+      hipLaunchKernelGGL((mscal_kernel<T, TRI, true>), grid, dim3(kNT), 0, c.stream, r0, N, alpha, A, lda);
+    else
+      hipLaunchKernelGGL((mscal_kernel<T, TRI, false>), grid, dim3(kNT), 0, c.stream, r0, N, alpha, A, lda);
+    EXO_LAUNCH_CHECK();
+  }
+}
+
+}  // namespace
+
+namespace ops {
+
+// src/level3/gemm.py:17-47.  X_A(i, k) = op(A)[i, k]; X_B(j, k) = op(B)[k, j].
+template <class T>
+void gemm(Ctx& c, int64_t TransA, int64_t TransB, int64_t M, int64_t N, int64_t K, T alpha, const T* A,
+          int64_t lda, const T* B, int64_t ldb, const T* AT, int64_t ldat, const T* BT, int64_t ldbt, T* Cp,
+          int64_t ldc) {
+  using namespace gemm;
+  const Operand<T> a = TransA == kNoTrans ? operand(A, lda, kRK) : operand(AT, ldat, kKR);
+  const Operand<T> b = TransB == kNoTrans ? operand(B, ldb, kKR) : operand(BT, ldbt, kRK);
+  product<T>(c, kFull, M, N, K, alpha, a, b, Cp, ldc);
+}
+
+// src/level3/symm.py:16-47.  Left: C += alpha S B (S = Aleft, M x M);
+// Right: C += alpha B S (S = Aright, N x N); S read from its Uplo triangle.
+template <class T>
+void symm_rm(Ctx& c, int64_t Side, int64_t Uplo, int64_t M, int64_t N, T alpha, const T* Aleft, int64_t ldal,
+             const T* Aright, int64_t ldar, const T* B, int64_t ldb, T* Cp, int64_t ldc) {
+  using namespace gemm;
+  const int sym = Uplo == kUpperV ? kSymUpper : kSymLower;
+  if (Side == kLeftV)
+    product<T>(c, kFull, M, N, M, alpha, operand(Aleft, ldal, sym), operand(B, ldb, kKR), Cp, ldc);
+  else
+    product<T>(c, kFull, M, N, N, alpha, operand(B, ldb, kRK), operand(Aright, ldar, sym), Cp, ldc);
+}
+
+// src/level3/syrk.py:17-32.  NoTrans: C += alpha A A^T; Trans: C += alpha A^T A,
+// on the Uplo triangle of C only.
+template <class T>
+void syrk_rm(Ctx& c, int64_t Uplo, int64_t Trans, int64_t N, int64_t K, T alpha, const T* A, int64_t lda,
+             const T* A_, int64_t lda_, const T* AT, int64_t ldat, const T* AT_, int64_t ldat_, T* Cp,
+             int64_t ldc) {
+  using namespace gemm;
+  const int tri = Uplo == kUpperV ? kUpper : kLower;
+  if (Trans == kNoTrans)
+    product<T>(c, tri, N, N, K, alpha, operand(A, lda, kRK), operand(A_, lda_, kRK), Cp, ldc);
+  else
+    product<T>(c, tri, N, N, K, alpha, operand(AT, ldat, kKR), operand(AT_, ldat_, kKR), Cp, ldc);
+}
+
+// src/level3/mscal.py:12-17
+template <class T>
+void mscal_rm(Ctx& c, int64_t M, int64_t N, T alpha, T* A, int64_t lda) {
+  launch_mscal<T, 0>(c, M, N, alpha, A, lda);
+}
+
+// src/level3/mscal.py:20-27
+template <class T>
+void trmscal_rm(Ctx& c, int64_t Uplo, int64_t N, T alpha, T* A, int64_t lda) {
+  if (Uplo == kUpperV)
+    launch_mscal<T, 2>(c, N, N, alpha, A, lda);
+  else
+    launch_mscal<T, 1>(c, N, N, alpha, A, lda);
+}
+
+#define EXO_L3_INST(T)                                                                                        \
+  template void gemm<T>(Ctx&, int64_t, int64_t, int64_t, int64_t, int64_t, T, const T*, int64_t, const T*,    \
+                        int64_t, const T*, int64_t, const T*, int64_t, T*, int64_t);                          \
+  template void mscal_rm<T>(Ctx&, int64_t, int64_t, T, T*, int64_t);                                          \
+  template void trmscal_rm<T>(Ctx&, int64_t, int64_t, T, T*, int64_t);                                        \
+  template void symm_rm<T>(Ctx&, int64_t, int64_t, int64_t, int64_t, T, const T*, int64_t, const T*, int64_t, \
+                           const T*, int64_t, T*, int64_t);                                                   \
+  template void syrk_rm<T>(Ctx&, int64_t, int64_t, int64_t, int64_t, T, const T*, int64_t, const T*, int64_t, \
+                           const T*, int64_t, const T*, int64_t, T*, int64_t);
+EXO_L3_INST(float)
+EXO_L3_INST(double)
+
+}  // namespace ops
+
+namespace gemm {
+// Exported to the other kernel families (batched, split precision).
+template void product<float>(Ctx&, int, int64_t, int64_t, int64_t, float, const Operand<float>&,
+                             const Operand<float>&, float*, int64_t);
+template void product<double>(Ctx&, int, int64_t, int64_t, int64_t, double, const Operand<double>&,
+                              const Operand<double>&, double*, int64_t);
+template Operand<float> operand<float>(const float*, int64_t, int);
+template Operand<double> operand<double>(const double*, int64_t, int);
+}  // namespace gemm
+}  // namespace exoblas

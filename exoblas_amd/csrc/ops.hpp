@@ -66,5 +66,14 @@ void syrk_rm(Ctx&, int64_t Uplo, int64_t Trans, int64_t N, int64_t K, T alpha, c
              const T* A_, int64_t lda_, const T* AT, int64_t ldat, const T* AT_, int64_t ldat_, T* C,
              int64_t ldc);
 
+// ---------------------------------------------------------------- split precision
+// C += alpha op(A) op(B) (or one triangle of alpha A A^T / A^T A) with each
+// FP32 operand split into `terms`-product BF16 pieces on the BF16 MFMA.
+void gemm_split(Ctx&, int64_t TransA, int64_t TransB, int64_t M, int64_t N, int64_t K, float alpha,
+                const float* A, int64_t lda, const float* B, int64_t ldb, const float* AT, int64_t ldat,
+                const float* BT, int64_t ldbt, float* C, int64_t ldc, int64_t terms);
+void syrk_split(Ctx&, int64_t Uplo, int64_t Trans, int64_t N, int64_t K, float alpha, const float* A, int64_t lda,
+                const float* AT, int64_t ldat, float* C, int64_t ldc, int64_t terms);
+
 }  // namespace ops
 }  // namespace exoblas

@@ -19,6 +19,9 @@ struct StreamState {
   void* scratch[Ctx::kSlots] = {};
   size_t scratch_cap[Ctx::kSlots] = {};
   unsigned* counters = nullptr;
+  unsigned* flags = nullptr;
+  size_t flags_cap = 0;
+  unsigned epoch = 0;
 };
 
 namespace {
@@ -160,6 +163,23 @@ void* Ctx::scratch(size_t bytes, int slot) {
 }
 
 unsigned* Ctx::counters() { return ss->counters; }
+
+unsigned* Ctx::epoch_flags(size_t n, unsigned* epoch) {
+  if (ss->flags_cap < n || ss->epoch == 0xffffffffu) {
+    EXO_HIP_CHECK(hipStreamSynchronize(stream));
+    if (ss->flags) EXO_HIP_CHECK(hipFree(ss->flags));
+    ss->flags = nullptr;
+    size_t c = 1024;
+    while (c < n) c <<= 1;
+    EXO_HIP_CHECK(hipMalloc(&ss->flags, c * sizeof(unsigned)));
+    EXO_HIP_CHECK(hipMemset(ss->flags, 0, c * sizeof(unsigned)));
+    EXO_HIP_CHECK(hipDeviceSynchronize());
+    ss->flags_cap = c;
+    ss->epoch = 0;
+  }
+  *epoch = ++ss->epoch;
+  return ss->flags;
+}
 
 Call::Call(const char* name, Mode mode, void* src) : name_(name), mode_(mode) {
   try {

@@ -81,6 +81,11 @@ struct Ctx {
   // that uses them must leave them zero when it ends.
   unsigned* counters();
   static constexpr int kCounters = 4096;
+  // At least n 32-bit completion flags owned by this stream, plus a fresh
+  // epoch value.  A kernel marks item i done by storing the epoch into
+  // flags[i]; since every call gets a new epoch the flags never need
+  // clearing.  Zeroed when (re)allocated; the epoch restarts then.
+  unsigned* epoch_flags(size_t n, unsigned* epoch);
 };
 
 void record_error(int code, const std::string& msg);

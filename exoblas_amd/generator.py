@@ -194,6 +194,32 @@ PROCS: List[Proc] = [
          "src/level3/syrk.py:17-32", "test/level3/syrk/exo_syrk_wrapper.h:21-27"),
 ]
 
+# Native entry points with no counterpart in the reference: the split-
+# precision and mixed-precision forms.  Exported as exoblas_<p><name> (host or
+# device pointers, synchronous) and exoblas_<p><name>_async (device pointers,
+# stream-ordered), declared in include/exoblas_amd_ext.h.
+EXT_PROCS: List[Proc] = [
+    Proc("gemm_split", "ext",
+         [enum("TransA", TRANS_VALUES), enum("TransB", TRANS_VALUES), size("M"), size("N"), size("K"),
+          scalar("alpha"),
+          win("A", "M", "K", used_when="TransA == 111"),
+          win("B", "K", "N", used_when="TransB == 111"),
+          win("AT", "K", "M", used_when="TransA != 111"),
+          win("BT", "N", "K", used_when="TransB != 111"),
+          win("C", "M", "N", const=False), enum("terms", (3, 6))],
+         "C += alpha op(A) op(B) with FP32 operands split into BF16 terms on the BF16 MFMA; "
+         "arguments as exo_sgemm_stride_1 (src/level3/gemm.py:17-47) plus the number of BF16 products",
+         "exoblas_amd.precision", precisions=("s",)),
+    Proc("syrk_split", "ext",
+         [enum("Uplo", UPLO_VALUES), enum("Trans", TRANS_VALUES), size("N"), size("K"), scalar("alpha"),
+          win("A", "N", "K", used_when="Trans == 111"),
+          win("AT", "K", "N", used_when="Trans != 111"),
+          win("C", "N", "N", const=False), enum("terms", (3, 6))],
+         "one triangle of C += alpha A A^T (or A^T A) in the split BF16 form; "
+         "arguments as exo_ssyrk_rm_stride_1 (src/level3/syrk.py:17-32) without the duplicate windows",
+         "exoblas_amd.precision", precisions=("s",)),
+]
+
 LIB_ORDER = ["asum", "axpy", "dot", "dsdot", "copy", "swap", "rot", "rotm", "scal",
              "gemv", "ger", "symv", "syr", "syr2", "trmv", "trsv",
              "gemm", "mscal", "symm", "syrk"]
@@ -225,18 +251,22 @@ def c_param(p: str, a: Arg) -> str:
     raise ValueError(a.kind)
 
 
-def symbol(proc: Proc, p: str, variant: str) -> str:
+def symbol(proc: Proc, p: str, variant: Optional[str]) -> str:
     prefix = p if proc.fixed_prefix is None else proc.fixed_prefix
+    if variant is None:
+        return f"exoblas_{prefix}{proc.name}"
     return f"exo_{prefix}{proc.name}_{variant}"
 
 
-def async_symbol(proc: Proc, p: str, variant: str) -> str:
+def async_symbol(proc: Proc, p: str, variant: Optional[str]) -> str:
     prefix = p if proc.fixed_prefix is None else proc.fixed_prefix
+    if variant is None:
+        return f"exoblas_{prefix}{proc.name}_async"
     return f"exoblas_{prefix}{proc.name}_{variant}_async"
 
 
 def op_name(proc: Proc, p: str) -> str:
-    if proc.fixed_prefix is not None:
+    if proc.fixed_prefix is not None or proc.lib == "ext":
         return f"exoblas::ops::{proc.name}"
     return f"exoblas::ops::{proc.name}<{PRECISIONS[p][0]}>"
 
@@ -246,6 +276,12 @@ def variants():
         for p in proc.precisions:
             for v in ("stride_any", "stride_1"):
                 yield proc, p, v
+
+
+def ext_variants():
+    for proc in EXT_PROCS:
+        for p in proc.precisions:
+            yield proc, p, None
 
 
 def all_symbols() -> List[str]:
@@ -281,6 +317,26 @@ def emit_lib_header(lib: str) -> str:
     return "".join(out)
 
 
+def emit_ext_header() -> str:
+    out = [HEADER_BANNER,
+           "/* Native entry points of exoblas_amd with no counterpart in the reference\n"
+           " * (the reference's exo_* functions are plain-precision only).  The plain\n"
+           " * form takes host or device pointers and returns once results are\n"
+           " * visible; the _async form takes device pointers and enqueues on\n"
+           " * `stream`.  Both return 0 or an EXOBLAS_STATUS_* code (exoblas_amd.h). */\n",
+           "#ifndef EXOBLAS_AMD_EXT_H\n#define EXOBLAS_AMD_EXT_H\n\n",
+           "#include <hip/hip_runtime_api.h>\n#include \"exo_types.h\"\n\n",
+           "#ifdef __cplusplus\nextern \"C\" {\n#endif\n\n"]
+    for proc, p, v in ext_variants():
+        out.append(f"/* {proc.ref} */\n")
+        params = ", ".join(["void *ctxt"] + [c_param(p, a) for a in proc.args])
+        out.append(f"int {symbol(proc, p, v)}({params});\n")
+        params = ", ".join(["hipStream_t stream"] + [c_param(p, a) for a in proc.args])
+        out.append(f"int {async_symbol(proc, p, v)}({params});\n\n")
+    out.append("#ifdef __cplusplus\n}\n#endif\n\n#endif\n")
+    return "".join(out)
+
+
 def emit_async_header() -> str:
     out = [HEADER_BANNER,
            "/* Device-pointer, stream-ordered forms of every exo_* entry point.\n"
@@ -312,14 +368,15 @@ def emit_body(proc: Proc, p: str, v: str, is_async: bool) -> str:
     lines = []
     lead = "hipStream_t stream" if is_async else "void *ctxt"
     params = ", ".join([lead] + [c_param(p, a) for a in proc.args])
-    ret = "int" if is_async else "void"
+    returns_int = is_async or v is None
+    ret = "int" if returns_int else "void"
     lines.append(f"extern \"C\" {ret} {sym}({params}) {{")
     mode = "exoblas::Call::kAsync" if is_async else "exoblas::Call::kSync"
     src = "stream" if is_async else "ctxt"
     lines.append(f"  exoblas::Call call(\"{sym}\", {mode}, {src});")
 
     def R(expr):
-        return f"return {expr};" if is_async else f"return (void){expr};"
+        return f"return {expr};" if returns_int else f"return (void){expr};"
     # validation
     for a in proc.args:
         q = '"' + a.name + '"'
@@ -393,8 +450,8 @@ def emit_exports() -> str:
            "#include \"runtime.hpp\"\n#include \"ops.hpp\"\n"]
     for lib in LIB_ORDER:
         out.append(f"#include \"exo_{lib}.h\"\n")
-    out.append("#include \"exoblas_amd_async.h\"\n\n")
-    for proc, p, v in variants():
+    out.append("#include \"exoblas_amd_async.h\"\n#include \"exoblas_amd_ext.h\"\n\n")
+    for proc, p, v in list(variants()) + list(ext_variants()):
         out.append(f"// {proc.ref}\n")
         out.append(emit_body(proc, p, v, False))
         out.append(emit_body(proc, p, v, True))
@@ -403,7 +460,8 @@ def emit_exports() -> str:
 
 
 def emit_symbol_list() -> str:
-    return "\n".join(all_symbols() + all_async_symbols()) + "\n"
+    ext = [f(proc, p, v) for proc, p, v in ext_variants() for f in (symbol, async_symbol)]
+    return "\n".join(all_symbols() + all_async_symbols() + ext) + "\n"
 
 
 def outputs():
@@ -411,6 +469,7 @@ def outputs():
     for lib in LIB_ORDER:
         files[os.path.join(ROOT, "include", f"exo_{lib}.h")] = emit_lib_header(lib)
     files[os.path.join(ROOT, "include", "exoblas_amd_async.h")] = emit_async_header()
+    files[os.path.join(ROOT, "include", "exoblas_amd_ext.h")] = emit_ext_header()
     files[os.path.join(ROOT, "exoblas_amd", "csrc", "gen_exports.cpp")] = emit_exports()
     files[os.path.join(ROOT, "exoblas_amd", "csrc", "symbols.txt")] = emit_symbol_list()
     return files

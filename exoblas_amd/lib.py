@@ -66,6 +66,15 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
         raise LibraryMissing(
             f"{path} is not built; run `python -m exoblas_amd.build` (the exo_* entry "
             "points have no CPU fallback)")
+    # torch wheels bundle their own HIP runtime (libamdhip64.so.7,
+    # libhsa-runtime64.so.1).  Loading torch first makes this library bind to
+    # that same runtime by SONAME, so torch tensors and streams are valid here;
+    # loaded the other way round the process would hold two HIP runtimes and
+    # the second one finds no GPU.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     lib = ctypes.CDLL(path)
     for proc, p, v in generator.variants():
         argt = [_ctype_of(p, a) for a in proc.args]
@@ -75,6 +84,12 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
         g = getattr(lib, generator.async_symbol(proc, p, v))
         g.argtypes = [ctypes.c_void_p] + argt
         g.restype = ctypes.c_int
+    for proc, p, v in generator.ext_variants():
+        argt = [_ctype_of(p, a) for a in proc.args]
+        for name in (generator.symbol(proc, p, v), generator.async_symbol(proc, p, v)):
+            f = getattr(lib, name)
+            f.argtypes = [ctypes.c_void_p] + argt
+            f.restype = ctypes.c_int
     lib.exoblas_last_status.restype = ctypes.c_int
     lib.exoblas_last_error.restype = ctypes.c_char_p
     lib.exoblas_clear_error.restype = None

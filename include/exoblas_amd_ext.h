@@ -1,0 +1,29 @@
+/* Generated by exoblas_amd/generator.py -- do not edit. */
+/* Native entry points of exoblas_amd with no counterpart in the reference
+ * (the reference's exo_* functions are plain-precision only).  The plain
+ * form takes host or device pointers and returns once results are
+ * visible; the _async form takes device pointers and enqueues on
+ * `stream`.  Both return 0 or an EXOBLAS_STATUS_* code (exoblas_amd.h). */
+#ifndef EXOBLAS_AMD_EXT_H
+#define EXOBLAS_AMD_EXT_H
+
+#include <hip/hip_runtime_api.h>
+#include "exo_types.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* C += alpha op(A) op(B) with FP32 operands split into BF16 terms on the BF16 MFMA; arguments as exo_sgemm_stride_1 (src/level3/gemm.py:17-47) plus the number of BF16 products */
+int exoblas_sgemm_split(void *ctxt, int_fast32_t TransA, int_fast32_t TransB, int_fast32_t M, int_fast32_t N, int_fast32_t K, const float *alpha, struct exo_win_2f32c A, struct exo_win_2f32c B, struct exo_win_2f32c AT, struct exo_win_2f32c BT, struct exo_win_2f32 C, int_fast32_t terms);
+int exoblas_sgemm_split_async(hipStream_t stream, int_fast32_t TransA, int_fast32_t TransB, int_fast32_t M, int_fast32_t N, int_fast32_t K, const float *alpha, struct exo_win_2f32c A, struct exo_win_2f32c B, struct exo_win_2f32c AT, struct exo_win_2f32c BT, struct exo_win_2f32 C, int_fast32_t terms);
+
+/* one triangle of C += alpha A A^T (or A^T A) in the split BF16 form; arguments as exo_ssyrk_rm_stride_1 (src/level3/syrk.py:17-32) without the duplicate windows */
+int exoblas_ssyrk_split(void *ctxt, int_fast32_t Uplo, int_fast32_t Trans, int_fast32_t N, int_fast32_t K, const float *alpha, struct exo_win_2f32c A, struct exo_win_2f32c AT, struct exo_win_2f32 C, int_fast32_t terms);
+int exoblas_ssyrk_split_async(hipStream_t stream, int_fast32_t Uplo, int_fast32_t Trans, int_fast32_t N, int_fast32_t K, const float *alpha, struct exo_win_2f32c A, struct exo_win_2f32c AT, struct exo_win_2f32 C, int_fast32_t terms);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

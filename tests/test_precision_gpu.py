@@ -1,0 +1,113 @@
+"""Split-precision sgemm / ssyrk (FP32 operands as BF16 pieces on the BF16
+MFMA) against the oracle and against float64.
+
+Tolerances are written per form, from the representation error of the split:
+  terms = 3 (two pieces):   |x - h - m| <= 2^-17 |x|, the dropped m*m term
+                            <= 2^-18 |ab|: per product ~ 2^-16 |a b|
+  terms = 6 (three pieces): per product ~ 2^-24 |a b|, the FP32 level
+on top of the FP32 accumulation bound (K+2) eps_32 sum|a b|.
+"""
+from __future__ import annotations
+
+import ctypes
+import itertools
+
+import numpy as np
+import pytest
+
+from oracle import LOWER, NO_TRANS, TRANS, UPPER, Oracle
+from util import uniform
+
+pytestmark = pytest.mark.gpu
+
+EPS32 = float(np.finfo(np.float32).eps)
+PER_PRODUCT = {3: 2.0 ** -16, 6: 2.0 ** -23}
+
+
+@pytest.fixture(scope="module")
+def so():
+    from exoblas_amd import lib
+    return lib.load()
+
+
+def check(got, want, mag, K, terms, what):
+    tol = 4 * (K + 2) * (EPS32 + PER_PRODUCT[terms]) * mag + 1e-30
+    err = np.abs(np.asarray(got, np.float64) - want)
+    bad = err > tol
+    assert not bad.any(), f"{what}: {int(bad.sum())} out of bound, worst ratio {float((err / tol).max()):.3g}"
+
+
+@pytest.mark.parametrize("terms", [3, 6])
+@pytest.mark.parametrize("ta,tb", list(itertools.product([NO_TRANS, TRANS], [NO_TRANS, TRANS])))
+@pytest.mark.parametrize("M,N,K", [(1, 7, 32), (7, 257, 64), (257, 64, 257), (130, 129, 33), (256, 256, 256)])
+def test_gemm_split_vs_oracle(so, terms, ta, tb, M, N, K, rng):
+    from exoblas_amd import lib as L
+    A = uniform(rng, (M, K) if ta == NO_TRANS else (K, M), np.float32)
+    B = uniform(rng, (K, N) if tb == NO_TRANS else (N, K), np.float32)
+    Cm = uniform(rng, (M, N), np.float32)
+    Co = Cm.copy()
+    Oracle(np.float32).gemm(ta, tb, M, N, K, 1.5, A, A.shape[1], B, B.shape[1], Co, N)
+    C0 = Cm.copy()
+    a = L.win2(A.ctypes.data, A.shape[1])
+    b = L.win2(B.ctypes.data, B.shape[1])
+    st = so.exoblas_sgemm_split(None, ta, tb, M, N, K, ctypes.byref(ctypes.c_float(1.5)), a, b, a, b,
+                                L.win2(Cm.ctypes.data, N), terms)
+    assert st == 0, L.last_error()
+    opA = np.abs(A if ta == NO_TRANS else A.T).astype(np.float64)
+    opB = np.abs(B if tb == NO_TRANS else B.T).astype(np.float64)
+    check(Cm, Co, np.abs(C0) + 1.5 * (opA @ opB), K, terms, "gemm_split")
+
+
+@pytest.mark.parametrize("terms", [3, 6])
+@pytest.mark.parametrize("uplo", [UPPER, LOWER])
+@pytest.mark.parametrize("trans", [NO_TRANS, TRANS])
+@pytest.mark.parametrize("N,K", [(7, 33), (257, 64), (300, 129)])
+def test_syrk_split_vs_oracle(so, terms, uplo, trans, N, K, rng):
+    from exoblas_amd import lib as L
+    A = uniform(rng, (N, K) if trans == NO_TRANS else (K, N), np.float32)
+    Cm = uniform(rng, (N, N), np.float32)
+    Co = Cm.copy()
+    Oracle(np.float32).syrk(uplo, trans, N, K, 2.0, A, A.shape[1], Co, N)
+    C0 = Cm.copy()
+    a = L.win2(A.ctypes.data, A.shape[1])
+    st = so.exoblas_ssyrk_split(None, uplo, trans, N, K, ctypes.byref(ctypes.c_float(2.0)), a, a,
+                                L.win2(Cm.ctypes.data, N), terms)
+    assert st == 0, L.last_error()
+    Aa = np.abs(A if trans == NO_TRANS else A.T).astype(np.float64)
+    check(Cm, Co, np.abs(C0) + 2.0 * (Aa @ Aa.T), K, terms, "syrk_split")
+    mask = np.tril(np.ones((N, N), bool)) if uplo == LOWER else np.triu(np.ones((N, N), bool))
+    assert np.array_equal(Cm[~mask], C0[~mask])
+
+
+def test_split_error_vs_plain_fp32(so):
+    """2048^3 NN on the uniform and the row-scaled (2^U[-8,8]) sets: error of
+    plain FP32 MFMA, 3-term and 6-term split against float64, normalised by
+    sum|a b|.  The 6-term form must not be less accurate than plain FP32."""
+    import torch
+    from exoblas_amd import lib as L
+    n = 2048
+    g = torch.Generator(device="cuda")
+    g.manual_seed(0)
+    stream = torch.cuda.current_stream().cuda_stream
+    for scaled in (False, True):
+        A = torch.empty((n, n), device="cuda").uniform_(-1, 1, generator=g)
+        B = torch.empty((n, n), device="cuda").uniform_(-1, 1, generator=g)
+        if scaled:
+            A *= torch.exp2(torch.empty((n, 1), device="cuda").uniform_(-8, 8, generator=g))
+            B *= torch.exp2(torch.empty((n, 1), device="cuda").uniform_(-8, 8, generator=g))
+        truth = A.double() @ B.double()
+        mag = A.double().abs() @ B.double().abs()
+        errs = {}
+        for form in ("plain", 3, 6):
+            Cm = torch.zeros((n, n), device="cuda")
+            a, b, c = L.win2(A.data_ptr(), n), L.win2(B.data_ptr(), n), L.win2(Cm.data_ptr(), n)
+            one = ctypes.byref(ctypes.c_float(1.0))
+            if form == "plain":
+                st = so.exoblas_sgemm_stride_1_async(stream, NO_TRANS, NO_TRANS, n, n, n, one, a, b, a, b, c)
+            else:
+                st = so.exoblas_sgemm_split_async(stream, NO_TRANS, NO_TRANS, n, n, n, one, a, b, a, b, c, form)
+            assert st == 0
+            torch.cuda.synchronize()
+            errs[form] = ((Cm.double() - truth).abs() / mag).max().item()
+        assert errs[6] <= errs["plain"] * 1.05, errs
+        assert errs[3] <= 4 * n * (EPS32 + PER_PRODUCT[3]), errs
