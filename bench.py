@@ -150,26 +150,7 @@ def cpu_baseline(n1, n2, budget_s=15.0):
                       f" -O3 -ffast-math build of the oracle, 1 thread, host-resident operands)"}
 
 
-def main(argv=None):
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--n1", type=int, default=1 << 26)
-    ap.add_argument("--n2", type=int, default=8192)
-    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
-    ap.add_argument("--cpu-budget", type=float, default=15.0)
-    a = ap.parse_args(argv)
-    if a.gpus != 1:
-        print(f"bench.py: this build runs on one GPU; --gpus {a.gpus} is not supported "
-              "(the multi-GPU sharding is a design note, not a measured path)", file=sys.stderr)
-        return 2
-
-    import torch
-
-    from exoblas_amd import lib as L
-    so = L.load()
-    torch.cuda.set_device(0)
+def bench_bandwidth(a, torch, so, L):
     stream = torch.cuda.current_stream()
     sh = stream.cuda_stream
     sets = [Operands(torch, a.n1, a.n2, seed) for seed in (0, 1)]
@@ -230,6 +211,244 @@ def main(argv=None):
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(a.n1, a.n2, a.cpu_budget)
         out["cpu_baseline"]["cores_on_host"] = os.cpu_count()
+    return out
+
+
+# ---------------------------------------------------------------- other workloads
+# Each returns one JSON object with the same top-level keys; they measure the
+# other configurations (level-3 products, triangular solves, split precision,
+# serving) and are run with --workload <name>.
+
+FP32_PEAK_TF, FP64_PEAK_TF, BF16_PEAK_TF = 157.3, 78.6, 2500.0
+
+
+def _timed(torch, stream, fn, reps, before=None):
+    """Average ms of fn() over reps launches, HIP events on `stream`."""
+    ts = []
+    for _ in range(reps):
+        if before is not None:
+            before()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        rc = fn()
+        e1.record(stream)
+        if rc not in (0, None):
+            from exoblas_amd import lib as L
+            raise RuntimeError(L.last_error())
+        ts.append((e0, e1))
+    torch.cuda.synchronize()
+    return sum(a.elapsed_time(b) for a, b in ts) / reps
+
+
+def _line(metric, value, unit, a, ms, dtype, workload, roof, kernels, higher=True, **extra):
+    out = {"metric": metric, "value": value, "unit": unit, "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+           "ms_per_step": round(ms, 4), "higher_is_better": higher, "scaling": "weak", "vs_baseline": None,
+           "dtype": dtype, "data": "synthetic U[-1,1], device-resident",
+           "config": {"workload": workload, "parallelism": "single GPU"}, "roofline": roof, "kernels": kernels}
+    out.update(extra)
+    return out
+
+
+def bench_level3(a, torch, so, L):
+    """sgemm / dgemm NN 8192^3, s/d syrk lower-N N=4096 K=1024, s/d symm
+    left-lower 4096^2, sgemm NN 512^3 (alpha 1.5, beta applied by the
+    separate mscal pass and not timed here)."""
+    stream = torch.cuda.current_stream()
+    sh = stream.cuda_stream
+    g = torch.Generator(device="cuda")
+    g.manual_seed(0)
+    kernels = {}
+    total_flop, total_ms = 0.0, 0.0
+
+    def u(shape, dt):
+        return torch.empty(shape, dtype=dt, device="cuda").uniform_(-1, 1, generator=g)
+    cases = []
+    for p, dt, peak in (("s", torch.float32, FP32_PEAK_TF), ("d", torch.float64, FP64_PEAK_TF)):
+        T = ctypes.c_float if p == "s" else ctypes.c_double
+        al = ctypes.byref(T(1.5))
+        n = 8192
+        A, B, C = u((n, n), dt), u((n, n), dt), u((n, n), dt)
+        f = getattr(so, f"exoblas_{p}gemm_stride_1_async")
+        cases.append((f"{p}gemm_NN_8192", peak, 2.0 * n ** 3,
+                      (lambda f=f, A=A, B=B, C=C, al=al, n=n: f(sh, 111, 111, n, n, n, al, L.win2(A.data_ptr(), n),
+                                                                 L.win2(B.data_ptr(), n), L.win2(A.data_ptr(), n),
+                                                                 L.win2(B.data_ptr(), n), L.win2(C.data_ptr(), n))),
+                      (A, B, C)))
+        N, K = 4096, 1024
+        As, Cs = u((N, K), dt), u((N, N), dt)
+        f = getattr(so, f"exoblas_{p}syrk_rm_stride_1_async")
+        w = L.win2(As.data_ptr(), K)
+        cases.append((f"{p}syrk_LN_4096x1024", peak, float(K) * N * (N + 1),
+                      (lambda f=f, w=w, Cs=Cs, al=al: f(sh, 122, 111, N, K, al, w, w, w, w, L.win2(Cs.data_ptr(), N))),
+                      (As, Cs)))
+        M = 4096
+        Sa, Bm, Cm = u((M, M), dt), u((M, M), dt), u((M, M), dt)
+        f = getattr(so, f"exoblas_{p}symm_rm_stride_1_async")
+        cases.append((f"{p}symm_LL_4096", peak, 2.0 * M * M * M,
+                      (lambda f=f, Sa=Sa, Bm=Bm, Cm=Cm, al=al: f(sh, 141, 122, M, M, al, L.win2(Sa.data_ptr(), M),
+                                                                  L.win2(Sa.data_ptr(), M), L.win2(Bm.data_ptr(), M),
+                                                                  L.win2(Cm.data_ptr(), M))),
+                      (Sa, Bm, Cm)))
+    n = 512
+    A5, B5, C5 = u((n, n), torch.float32), u((n, n), torch.float32), u((n, n), torch.float32)
+    cases.append(("sgemm_NN_512", FP32_PEAK_TF, 2.0 * n ** 3,
+                  (lambda: so.exoblas_sgemm_stride_1_async(sh, 111, 111, n, n, n, ctypes.byref(ctypes.c_float(1.5)),
+                                                           L.win2(A5.data_ptr(), n), L.win2(B5.data_ptr(), n),
+                                                           L.win2(A5.data_ptr(), n), L.win2(B5.data_ptr(), n),
+                                                           L.win2(C5.data_ptr(), n))), (A5, B5, C5)))
+    for name, peak, flop, fn, _ in cases:
+        for _ in range(max(1, a.warmup // 2)):
+            fn()
+        reps = 20 if flop < 1e10 else max(2, min(a.steps, 5))
+        ms = _timed(torch, stream, fn, reps)
+        tf = flop / (ms * 1e-3) / 1e12
+        kernels[name] = {"ms": round(ms, 4), "TFLOP/s": round(tf, 2), "frac": round(tf / peak, 4),
+                         "peak": peak, "flop": flop}
+        total_flop += flop
+        total_ms += ms
+    dom = "sgemm_NN_8192"
+    k = kernels[dom]
+    roof = {"bound": "mfma", "kernel": dom, "achieved": k["TFLOP/s"], "peak": FP32_PEAK_TF, "unit": "TFLOP/s",
+            "frac": k["frac"], "traffic": None}
+    return _line("TFLOP/s per level-3 kernel (MFMA roofline)", round(total_flop / (total_ms * 1e-3) / 1e12, 2),
+                 "TFLOP/s", a, total_ms, "f32+f64",
+                 "level-3: s/d gemm NN 8192^3, s/d syrk L-N 4096x1024, s/d symm L-L 4096^2, sgemm NN 512^3",
+                 roof, kernels)
+
+
+def bench_trsv(a, torch, so, L):
+    """s/d trsv lower, no-trans, non-unit at n = 4096 and 8192 on the
+    diagonally dominant matrices (off-diagonal U[-1,1]/n, diagonal
+    n + U[0,1]), b ~ U[-1,1]; x reset before every timed solve."""
+    stream = torch.cuda.current_stream()
+    sh = stream.cuda_stream
+    g = torch.Generator(device="cuda")
+    g.manual_seed(0)
+    kernels = {}
+    for p, dt, s in (("s", torch.float32, 4), ("d", torch.float64, 8)):
+        for n in (4096, 8192):
+            A = torch.empty((n, n), dtype=dt, device="cuda").uniform_(-1, 1, generator=g) / n
+            A.diagonal().copy_(n + torch.empty(n, dtype=dt, device="cuda").uniform_(0, 1, generator=g))
+            b = torch.empty(n, dtype=dt, device="cuda").uniform_(-1, 1, generator=g)
+            x = b.clone()
+            f = getattr(so, f"exoblas_{p}trsv_rm_stride_1_async")
+            fn = (lambda f=f, A=A, x=x, n=n: f(sh, 122, 111, 131, n, L.win1(x.data_ptr(), 1), L.win2(A.data_ptr(), n)))
+            fn()
+            ms = _timed(torch, stream, fn, 20, before=lambda x=x, b=b: x.copy_(b))
+            nbytes = (n * (n + 1) // 2 + 2 * n) * s
+            gbs = nbytes / (ms * 1e-3) / 1e9
+            kernels[f"{p}trsv_LNN_{n}"] = {"us": round(ms * 1e3, 2), "GB/s": round(gbs, 1),
+                                           "frac": round(gbs / HBM_PEAK_GBS, 4), "bytes": nbytes}
+    k = kernels["strsv_LNN_4096"]
+    roof = {"bound": "hbm", "kernel": "strsv_LNN_4096", "achieved": k["GB/s"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": k["frac"], "traffic": None}
+    tot_b = sum(v["bytes"] for v in kernels.values())
+    tot_ms = sum(v["us"] for v in kernels.values()) / 1e3
+    return _line("GB/s per triangular solve (HBM roofline; latency-bound chain)", round(tot_b / tot_ms / 1e6, 1),
+                 "GB/s", a, tot_ms, "f32+f64", "trsv: s/d lower no-trans non-unit, n = 4096 and 8192", roof, kernels)
+
+
+def bench_precision(a, torch, so, L):
+    """sgemm NN 8192^3: plain FP32 MFMA vs split BF16 (3 and 6 products);
+    max error against float64 on U[-1,1] and on rows scaled by 2^U[-8,8],
+    as max |C - C64| / (|A| |B|) (normwise per element) and max relative."""
+    stream = torch.cuda.current_stream()
+    sh = stream.cuda_stream
+    g = torch.Generator(device="cuda")
+    g.manual_seed(0)
+    n = a.n_gemm
+    kernels = {}
+    one = ctypes.byref(ctypes.c_float(1.0))
+    for scaled in (False, True):
+        A = torch.empty((n, n), device="cuda").uniform_(-1, 1, generator=g)
+        B = torch.empty((n, n), device="cuda").uniform_(-1, 1, generator=g)
+        if scaled:
+            A *= torch.exp2(torch.empty((n, 1), device="cuda").uniform_(-8, 8, generator=g))
+            B *= torch.exp2(torch.empty((n, 1), device="cuda").uniform_(-8, 8, generator=g))
+        truth = A.double() @ B.double()
+        mag = A.double().abs() @ B.double().abs()
+        C = torch.zeros((n, n), device="cuda")
+        wa, wb, wc = L.win2(A.data_ptr(), n), L.win2(B.data_ptr(), n), L.win2(C.data_ptr(), n)
+        forms = {"plain": lambda: so.exoblas_sgemm_stride_1_async(sh, 111, 111, n, n, n, one, wa, wb, wa, wb, wc),
+                 "split3": lambda: so.exoblas_sgemm_split_async(sh, 111, 111, n, n, n, one, wa, wb, wa, wb, wc, 3),
+                 "split6": lambda: so.exoblas_sgemm_split_async(sh, 111, 111, n, n, n, one, wa, wb, wa, wb, wc, 6)}
+        for name, fn in forms.items():
+            C.zero_()
+            fn()
+            torch.cuda.synchronize()
+            err = (C.double() - truth).abs()
+            nw = (err / mag).max().item()
+            rel = (err / truth.abs().clamp_min(1e-300)).max().item()
+            key = f"sgemm_{name}_{'scaled' if scaled else 'uniform'}"
+            ms = _timed(torch, stream, fn, 3) if not scaled else None
+            kernels[key] = {"max_err_normwise": nw, "max_rel_err": rel}
+            if ms is not None:
+                tf = 2.0 * n ** 3 / (ms * 1e-3) / 1e12
+                peak = FP32_PEAK_TF if name == "plain" else BF16_PEAK_TF / (3 if name == "split3" else 6)
+                kernels[key].update({"ms": round(ms, 3), "TFLOP/s": round(tf, 2), "frac": round(tf / peak, 4),
+                                     "peak_equiv": round(peak, 1)})
+        del truth, mag, A, B, C
+        torch.cuda.empty_cache()
+    sp = kernels["sgemm_plain_uniform"]["ms"] / kernels["sgemm_split3_uniform"]["ms"]
+    k3 = kernels["sgemm_split3_uniform"]
+    roof = {"bound": "mfma", "kernel": "sgemm_split3", "achieved": k3["TFLOP/s"], "peak": k3["peak_equiv"],
+            "unit": "TFLOP/s (FP32-equivalent; BF16 peak / products)", "frac": k3["frac"], "traffic": None}
+    return _line("split-BF16 sgemm speedup over plain FP32 MFMA sgemm", round(sp, 3), "x", a,
+                 kernels["sgemm_split3_uniform"]["ms"], "f32 (bf16 x3 / x6 products, f32 accumulate)",
+                 f"sgemm NN {n}^3, uniform and row-scaled sets", roof, kernels)
+
+
+def bench_serving(a, torch, so, L):
+    """The mixed Poisson stream (seeded): batched persistent kernels vs one
+    launch per call, same stream, same operands; 64 GB admission budget."""
+    from exoblas_amd import serving
+    s = serving.make_stream(a.requests, rate=a.rate, seed=0)
+    w = serving.Workload(s)
+    res = {}
+    for mode, name in ((1, "per_call"), (0, "batched")):
+        w.reset()
+        torch.cuda.synchronize()
+        st, lat = serving.run(w, mode=mode, budget=64 << 30, streams=4, max_batch=a.max_batch)
+        res[name] = {k: (round(v, 3) if isinstance(v, float) else v) for k, v in st.items()}
+    ratio = res["batched"]["req_per_s"] / max(res["per_call"]["req_per_s"], 1e-9)
+    roof = {"bound": "mixed", "kernel": "batch_kernel", "achieved": None, "peak": None, "unit": None, "frac": None,
+            "traffic": None}
+    return _line("mixed-size BLAS req/s (batched)", round(res["batched"]["req_per_s"], 1), "req/s", a,
+                 res["batched"]["wall_s"] * 1e3, "f32",
+                 f"serving: {a.requests} requests, Poisson {a.rate:.0f}/s, 60% sgemm / 25% strsv / 15% sgemv",
+                 roof, res, speedup_vs_per_call=round(ratio, 3),
+                 p50_us=res["batched"]["p50_us"], p99_us=res["batched"]["p99_us"])
+
+
+WORKLOADS = {"bandwidth": bench_bandwidth, "level3": bench_level3, "trsv": bench_trsv,
+             "precision": bench_precision, "serving": bench_serving}
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--workload", choices=sorted(WORKLOADS), default="bandwidth")
+    ap.add_argument("--n1", type=int, default=1 << 26)
+    ap.add_argument("--n2", type=int, default=8192)
+    ap.add_argument("--n-gemm", type=int, default=8192)
+    ap.add_argument("--requests", type=int, default=20000)
+    ap.add_argument("--rate", type=float, default=20000.0)
+    ap.add_argument("--max-batch", type=int, default=256)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
+    ap.add_argument("--cpu-budget", type=float, default=15.0)
+    a = ap.parse_args(argv)
+    if a.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {a.gpus} is not supported "
+              "(the multi-GPU sharding is a design note, not a measured path)", file=sys.stderr)
+        return 2
+    import torch
+
+    from exoblas_amd import lib as L
+    so = L.load()
+    torch.cuda.set_device(0)
+    out = WORKLOADS[a.workload](a, torch, so, L)
     print(json.dumps(out), flush=True)
     return 0
 

@@ -239,11 +239,13 @@ __device__ __forceinline__ void tile_mma(const Operand<typename C::type>& A, con
 }
 
 // C[m, n] += alpha * acc for the block tile at (m0, n0), restricted to the
-// TRI triangle of C (n <= m for kLower, n >= m for kUpper).
-template <class C, int TRI>
+// TRI triangle of C (n <= m for kLower, n >= m for kUpper).  With BETA the
+// update is C = alpha * acc + beta * C instead, C not read when beta == 0.
+template <class C, int TRI, bool BETA = false>
 __device__ __forceinline__ void tile_store(typename C::type* Cp, int64_t ldc, int cvec, int64_t M, int64_t N,
                                            int64_t m0, int64_t n0, typename C::type alpha,
-                                           const typename Mfma<typename C::type>::acc (&acc)[C::SM][C::SN]) {
+                                           const typename Mfma<typename C::type>::acc (&acc)[C::SM][C::SN],
+                                           typename C::type beta = 0) {
   using T = typename C::type;
   const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
   const int wm0 = (wave / C::WAVES_N) * C::WM, wn0 = (wave % C::WAVES_N) * C::WN;
@@ -264,9 +266,15 @@ __device__ __forceinline__ void tile_store(typename C::type* Cp, int64_t ldc, in
         if (TRI == kLower) all_in = all_in && col0 + E - 1 <= row;
         if (TRI == kUpper) all_in = all_in && col0 >= row;
         if (all_in) {
-          VecN<T, E> v = *reinterpret_cast<VecN<T, E>*>(cp);
+          VecN<T, E> v;
+          if (!BETA || beta != T(0)) v = *reinterpret_cast<VecN<T, E>*>(cp);
 #pragma unroll
-          for (int e = 0; e < E; ++e) v.v[e] += alpha * acc[ta][gb * E + e][q];
+          for (int e = 0; e < E; ++e) {
+            if (!BETA)
+              v.v[e] += alpha * acc[ta][gb * E + e][q];
+            else
+              v.v[e] = alpha * acc[ta][gb * E + e][q] + (beta != T(0) ? beta * v.v[e] : T(0));
+          }
           *reinterpret_cast<VecN<T, E>*>(cp) = v;
         } else {
 #pragma unroll
@@ -275,7 +283,12 @@ __device__ __forceinline__ void tile_store(typename C::type* Cp, int64_t ldc, in
             bool in = col < N;
             if (TRI == kLower) in = in && col <= row;
             if (TRI == kUpper) in = in && col >= row;
-            if (in) cp[e] += alpha * acc[ta][gb * E + e][q];
+            if (in) {
+              if (!BETA)
+                cp[e] += alpha * acc[ta][gb * E + e][q];
+              else
+                cp[e] = alpha * acc[ta][gb * E + e][q] + (beta != T(0) ? beta * cp[e] : T(0));
+            }
           }
         }
       }
