@@ -3199,3 +3199,39 @@ extern "C" int exoblas_ssyrk_split_async(hipStream_t stream, int_fast32_t Uplo, 
   return call.finish();
 }
 
+// op(A) x = b with BF16 (update 16) or FP32 (update 32) off-diagonal updates, FP64 diagonal-block solves and `refine` steps of iterative refinement on an FP64 residual; arguments as exo_strsv_rm_stride_any (src/level2/trsv.py:10-52) plus update and refine
+extern "C" int exoblas_strsv_mixed(void *ctxt, int_fast32_t Uplo, int_fast32_t TransA, int_fast32_t Diag, int_fast32_t n, struct exo_win_1f32 x, struct exo_win_2f32c A, int_fast32_t update, int_fast32_t refine) {
+  exoblas::Call call("exoblas_strsv_mixed", exoblas::Call::kSync, ctxt);
+  if (!(Uplo == 121 || Uplo == 122)) return call.fail_enum("Uplo", Uplo);
+  if (!(TransA == 111 || TransA == 112 || TransA == 113)) return call.fail_enum("TransA", TransA);
+  if (!(Diag == 131 || Diag == 132)) return call.fail_enum("Diag", Diag);
+  if (n < 0) return call.fail_size("n", n);
+  if (!(update == 16 || update == 32)) return call.fail_enum("update", update);
+  if (refine < 0) return call.fail_size("refine", refine);
+  if ((true) && A.strides[1] != 1) return call.fail_stride("A", A.strides[1]);
+  if ((true) && n > 1 && A.strides[0] < n) return call.fail_ld("A", A.strides[0]);
+  call.window(0, x.data, exoblas::span_lo(n, x.strides[0]), exoblas::span_hi(n, x.strides[0]), sizeof(float), true, true);
+  call.window(1, A.data, 0, exoblas::span2_hi(n, n, A.strides[0]), sizeof(float), false, true);
+  if (!call.stage()) return call.status();
+  if (!call.ok()) return call.status();
+  call.run([&] { exoblas::ops::trsv_mixed(call.ctx(), (int64_t)Uplo, (int64_t)TransA, (int64_t)Diag, (int64_t)n, call.dev<float>(0), (int64_t)x.strides[0], call.dev<const float>(1), (int64_t)A.strides[0], (int64_t)update, (int64_t)refine); });
+  return call.finish();
+}
+extern "C" int exoblas_strsv_mixed_async(hipStream_t stream, int_fast32_t Uplo, int_fast32_t TransA, int_fast32_t Diag, int_fast32_t n, struct exo_win_1f32 x, struct exo_win_2f32c A, int_fast32_t update, int_fast32_t refine) {
+  exoblas::Call call("exoblas_strsv_mixed_async", exoblas::Call::kAsync, stream);
+  if (!(Uplo == 121 || Uplo == 122)) return call.fail_enum("Uplo", Uplo);
+  if (!(TransA == 111 || TransA == 112 || TransA == 113)) return call.fail_enum("TransA", TransA);
+  if (!(Diag == 131 || Diag == 132)) return call.fail_enum("Diag", Diag);
+  if (n < 0) return call.fail_size("n", n);
+  if (!(update == 16 || update == 32)) return call.fail_enum("update", update);
+  if (refine < 0) return call.fail_size("refine", refine);
+  if ((true) && A.strides[1] != 1) return call.fail_stride("A", A.strides[1]);
+  if ((true) && n > 1 && A.strides[0] < n) return call.fail_ld("A", A.strides[0]);
+  call.window(0, x.data, exoblas::span_lo(n, x.strides[0]), exoblas::span_hi(n, x.strides[0]), sizeof(float), true, true);
+  call.window(1, A.data, 0, exoblas::span2_hi(n, n, A.strides[0]), sizeof(float), false, true);
+  if (!call.stage()) return call.status();
+  if (!call.ok()) return call.status();
+  call.run([&] { exoblas::ops::trsv_mixed(call.ctx(), (int64_t)Uplo, (int64_t)TransA, (int64_t)Diag, (int64_t)n, call.dev<float>(0), (int64_t)x.strides[0], call.dev<const float>(1), (int64_t)A.strides[0], (int64_t)update, (int64_t)refine); });
+  return call.finish();
+}
+

@@ -75,5 +75,10 @@ void gemm_split(Ctx&, int64_t TransA, int64_t TransB, int64_t M, int64_t N, int6
 void syrk_split(Ctx&, int64_t Uplo, int64_t Trans, int64_t N, int64_t K, float alpha, const float* A, int64_t lda,
                 const float* AT, int64_t ldat, float* C, int64_t ldc, int64_t terms);
 
+// op(A) x = b in FP32 with BF16 (update 16) / FP32 (update 32) off-diagonal
+// updates, FP64 diagonal blocks and `refine` FP64-residual refinement steps.
+void trsv_mixed(Ctx&, int64_t Uplo, int64_t TransA, int64_t Diag, int64_t n, float* x, int64_t incx, const float* A,
+                int64_t lda, int64_t update, int64_t refine);
+
 }  // namespace ops
 }  // namespace exoblas

@@ -1,0 +1,238 @@
+// Mixed-precision strsv with iterative refinement.
+//
+//   solve   op(A) x0 = b   off-diagonal block updates in FP32 arithmetic on
+//                          BF16 or FP32 copies of A (update = 16 / 32),
+//                          64 x 64 diagonal blocks solved in FP64
+//   repeat `refine` times:
+//           r = b - op(A) x   FP64 accumulation over the FP32 matrix
+//           solve op(A) d = r (as above),  x += d  (FP64)
+//   x (FP32 out) = x
+//
+// The solve is the look-back block solve of trsv_core.hpp with a different
+// element pipeline: right-hand side and solution in FP64, an FP32 shadow of
+// the solution feeding the off-diagonal updates.
+#include "device.hpp"
+#include "ops.hpp"
+#include "trsv_core.hpp"
+
+namespace exoblas {
+namespace {
+
+using trsv::kB;
+using trsv::kNT;
+constexpr int kFlagBase = 2;
+
+struct MixedSmem {
+  float xs[kB];
+  float red[kNT / kB][kB];
+  double diag[kB][kB + 1];
+};
+
+template <class TO>
+__device__ __forceinline__ float to_f(TO v) { return (float)v; }
+
+template <class TO, bool FWD, bool TRANS, bool UNIT>
+__global__ __launch_bounds__(kNT) void trsv_mixed_kernel(int64_t n, const float* __restrict__ A, int64_t lda,
+                                                         const TO* __restrict__ Aoff, int64_t ldo,
+                                                         const double* __restrict__ rhs, double* x, float* xf,
+                                                         unsigned* flags, unsigned epoch) {
+  __shared__ MixedSmem sm;
+  __shared__ int s_p;
+  const int nb = (int)((n + kB - 1) / kB);
+  const int t = threadIdx.x;
+  if (t == 0) {
+    const unsigned p = atomicAdd(flags, 1u);
+    if ((int)p == nb - 1) atomicExch(flags, 0u);
+    s_p = (int)p;
+  }
+  __syncthreads();
+  const int p = s_p;
+  unsigned* done = flags + kFlagBase;
+  const int64_t bi = FWD ? p : nb - 1 - p;
+  const int64_t r0 = bi * kB;
+  const int r = TRANS ? t % kB : t / 4;
+  const int cq = TRANS ? t / kB : t % 4;
+  const int64_t gi = r0 + r;
+  float acc = 0.f;
+  for (int q = 0; q < p; ++q) {
+    const int64_t bj = FWD ? q : nb - 1 - q;
+    const int64_t c0 = bj * kB + cq * 16;
+    float m[16];
+#pragma unroll
+    for (int e = 0; e < 16; ++e) {
+      const int64_t gj = c0 + e;
+      m[e] = (gi < n && gj < n) ? to_f(TRANS ? Aoff[gj * ldo + gi] : Aoff[gi * ldo + gj]) : 0.f;
+    }
+    if (t == 0) trsv::wait_marker(done + bj, epoch, flags + 1);
+    __syncthreads();
+    if (t < kB) sm.xs[t] = (bj * kB + t < n) ? xf[bj * kB + t] : 0.f;
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < 16; ++e) acc += m[e] * sm.xs[cq * 16 + e];
+  }
+  sm.red[cq][r] = acc;
+  for (int idx = t; idx < kB * kB; idx += kNT) {
+    const int i = TRANS ? idx % kB : idx / kB;
+    const int j = TRANS ? idx / kB : idx % kB;
+    const int64_t ai = r0 + i, aj = r0 + j;
+    sm.diag[i][j] = (ai < n && aj < n) ? (double)(TRANS ? A[aj * lda + ai] : A[ai * lda + aj]) : (i == j ? 1.0 : 0.0);
+  }
+  __syncthreads();
+  if (t < kB) {
+    const int lane = t;
+    double v = 0.0;
+    if (r0 + lane < n) {
+      float s = 0.f;
+#pragma unroll
+      for (int w = 0; w < kNT / kB; ++w) s += sm.red[w][lane];
+      v = rhs[r0 + lane] - (double)s;
+    }
+    for (int s = 0; s < kB; ++s) {
+      const int k = FWD ? s : kB - 1 - s;
+      double xk = __shfl(v, k, kB);
+      if (!UNIT) xk = xk / sm.diag[k][k];
+      if (lane == k)
+        v = xk;
+      else if (FWD ? lane > k : lane < k)
+        v -= sm.diag[lane][k] * xk;
+    }
+    if (r0 + lane < n) {
+      x[r0 + lane] = v;
+      xf[r0 + lane] = (float)v;
+    }
+    __threadfence();
+    if (lane == 0) __hip_atomic_store(done + bi, epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
+// r[i] = b[i] - sum_j M(i, j) x[j] over the triangle (FP64 accumulation), one
+// wave per row; NoTrans rows are contiguous, Trans rows are columns of A.
+template <bool LOWER, bool TRANS, bool UNIT>
+__global__ __launch_bounds__(256) void residual_kernel(int64_t n, const float* __restrict__ A, int64_t lda,
+                                                       const double* __restrict__ b, const double* __restrict__ x,
+                                                       double* r) {
+  const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+  const int64_t i = (int64_t)blockIdx.x * 4 + wave;
+  if (i >= n) return;
+  // M = op(tri(A)): lower-notrans / upper-trans touch j <= i, the others j >= i
+  const bool left = LOWER != TRANS;
+  const int64_t jl = left ? 0 : i, jh = left ? i + 1 : n;
+  double s = 0.0;
+  for (int64_t j = jl + lane; j < jh; j += 64) {
+    double m = (double)(TRANS ? A[j * lda + i] : A[i * lda + j]);
+    if (UNIT && j == i) m = 1.0;
+    s += m * x[j];
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+  if (lane == 0) r[i] = b[i] - s;
+}
+
+__global__ void widen_kernel(int64_t n, const float* x, int64_t incx, double* y) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i < n) y[i] = (double)x[i * incx];
+}
+__global__ void add_kernel(int64_t n, double* x, const double* d) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i < n) x[i] += d[i];
+}
+__global__ void narrow_kernel(int64_t n, const double* x, float* y, int64_t incy) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i < n) y[i * incy] = (float)x[i];
+}
+__global__ void to_bf16_kernel(int64_t rows, int64_t cols, const float* A, int64_t lda, __bf16* out) {
+  const int64_t i = blockIdx.y;
+  for (int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x; j < cols; j += (int64_t)gridDim.x * 256)
+    out[i * cols + j] = (__bf16)A[i * lda + j];
+}
+
+template <class TO, bool FWD, bool TRANS>
+void solve(Ctx& c, bool unit, int64_t n, const float* A, int64_t lda, const TO* Aoff, int64_t ldo,
+           const double* rhs, double* x, float* xf) {
+  const int64_t nb = ceil_div(n, kB);
+  unsigned epoch = 0;
+  unsigned* flags = c.epoch_flags((size_t)nb + kFlagBase, &epoch);
+  if (unit)
+    hipLaunchKernelGGL((trsv_mixed_kernel<TO, FWD, TRANS, true>), dim3((unsigned)nb), dim3(kNT), 0, c.stream, n, A,
+                       lda, Aoff, ldo, rhs, x, xf, flags, epoch);
+  else
+    hipLaunchKernelGGL((trsv_mixed_kernel<TO, FWD, TRANS, false>), dim3((unsigned)nb), dim3(kNT), 0, c.stream, n, A,
+                       lda, Aoff, ldo, rhs, x, xf, flags, epoch);
+  EXO_LAUNCH_CHECK();
+}
+
+template <class TO>
+void solve_any(Ctx& c, bool fwd, bool trans, bool unit, int64_t n, const float* A, int64_t lda, const TO* Aoff,
+               int64_t ldo, const double* rhs, double* x, float* xf) {
+  if (fwd && !trans) solve<TO, true, false>(c, unit, n, A, lda, Aoff, ldo, rhs, x, xf);
+  if (fwd && trans) solve<TO, true, true>(c, unit, n, A, lda, Aoff, ldo, rhs, x, xf);
+  if (!fwd && !trans) solve<TO, false, false>(c, unit, n, A, lda, Aoff, ldo, rhs, x, xf);
+  if (!fwd && trans) solve<TO, false, true>(c, unit, n, A, lda, Aoff, ldo, rhs, x, xf);
+}
+
+void residual(Ctx& c, bool lower, bool trans, bool unit, int64_t n, const float* A, int64_t lda, const double* b,
+              const double* x, double* r) {
+  const dim3 g((unsigned)ceil_div(n, 4));
+#define EXO_RES(L, T, U) \
+  hipLaunchKernelGGL((residual_kernel<L, T, U>), g, dim3(256), 0, c.stream, n, A, lda, b, x, r)
+  if (lower && !trans && unit) EXO_RES(true, false, true);
+  if (lower && !trans && !unit) EXO_RES(true, false, false);
+  if (lower && trans && unit) EXO_RES(true, true, true);
+  if (lower && trans && !unit) EXO_RES(true, true, false);
+  if (!lower && !trans && unit) EXO_RES(false, false, true);
+  if (!lower && !trans && !unit) EXO_RES(false, false, false);
+  if (!lower && trans && unit) EXO_RES(false, true, true);
+  if (!lower && trans && !unit) EXO_RES(false, true, false);
+#undef EXO_RES
+  EXO_LAUNCH_CHECK();
+}
+
+}  // namespace
+
+namespace ops {
+
+void trsv_mixed(Ctx& c, int64_t Uplo, int64_t TransA, int64_t Diag, int64_t n, float* x, int64_t incx,
+                const float* A, int64_t lda, int64_t update, int64_t refine) {
+  if (n <= 0) return;
+  const bool trans = TransA != 111, lower = Uplo == 122, unit = Diag == 132;
+  const bool fwd = lower != trans;
+  // scratch slot 2: b, x, d, r (FP64) and the FP32 shadow of the solution
+  char* ws = static_cast<char*>(c.scratch((size_t)n * (4 * sizeof(double) + sizeof(float)) + 1024, 2));
+  double* b = reinterpret_cast<double*>(ws);
+  double* xs = b + n;
+  double* d = xs + n;
+  double* r = d + n;
+  float* xf = reinterpret_cast<float*>(r + n);
+  const dim3 g1((unsigned)ceil_div(n, 256));
+  hipLaunchKernelGGL(widen_kernel, g1, dim3(256), 0, c.stream, n, x, incx, b);
+  EXO_LAUNCH_CHECK();
+  const __bf16* Abf = nullptr;
+  if (update == 16) {  // BF16 copy of the matrix for the off-diagonal updates (scratch slot 3)
+    __bf16* p = static_cast<__bf16*>(c.scratch((size_t)n * n * sizeof(__bf16), 3));
+    for (int64_t r0 = 0; r0 < n; r0 += 65535) {
+      const int64_t rows = std::min<int64_t>(65535, n - r0);
+      hipLaunchKernelGGL(to_bf16_kernel, dim3((unsigned)std::min<int64_t>(ceil_div(n, 256), 64), (unsigned)rows),
+                         dim3(256), 0, c.stream, rows, n, A + r0 * lda, lda, p + r0 * n);
+      EXO_LAUNCH_CHECK();
+    }
+    Abf = p;
+  }
+  auto run_solve = [&](const double* rhs, double* out) {
+    if (Abf)
+      solve_any<__bf16>(c, fwd, trans, unit, n, A, lda, Abf, n, rhs, out, xf);
+    else
+      solve_any<float>(c, fwd, trans, unit, n, A, lda, A, lda, rhs, out, xf);
+  };
+  run_solve(b, xs);
+  for (int64_t it = 0; it < refine; ++it) {
+    residual(c, lower, trans, unit, n, A, lda, b, xs, r);
+    run_solve(r, d);
+    hipLaunchKernelGGL(add_kernel, g1, dim3(256), 0, c.stream, n, xs, d);
+    EXO_LAUNCH_CHECK();
+  }
+  hipLaunchKernelGGL(narrow_kernel, g1, dim3(256), 0, c.stream, n, xs, x, incx);
+  EXO_LAUNCH_CHECK();
+}
+
+}  // namespace ops
+}  // namespace exoblas

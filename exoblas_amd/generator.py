@@ -218,6 +218,13 @@ EXT_PROCS: List[Proc] = [
          "one triangle of C += alpha A A^T (or A^T A) in the split BF16 form; "
          "arguments as exo_ssyrk_rm_stride_1 (src/level3/syrk.py:17-32) without the duplicate windows",
          "exoblas_amd.precision", precisions=("s",)),
+    Proc("trsv_mixed", "ext",
+         [enum("Uplo", UPLO_VALUES), enum("TransA", TRANS_VALUES), enum("Diag", DIAG_VALUES), size("n"),
+          win("x", "n", const=False), win("A", "n", "n"), enum("update", (16, 32)), size("refine")],
+         "op(A) x = b with BF16 (update 16) or FP32 (update 32) off-diagonal updates, FP64 diagonal-block "
+         "solves and `refine` steps of iterative refinement on an FP64 residual; arguments as "
+         "exo_strsv_rm_stride_any (src/level2/trsv.py:10-52) plus update and refine",
+         "exoblas_amd.precision", precisions=("s",)),
 ]
 
 LIB_ORDER = ["asum", "axpy", "dot", "dsdot", "copy", "swap", "rot", "rotm", "scal",

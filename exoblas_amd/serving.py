@@ -142,7 +142,9 @@ def run(w: Workload, mode: int, budget: int = 64 << 30, streams: int = 4, max_ba
         ops: Optional[np.ndarray] = None, arrival_us: Optional[np.ndarray] = None):
     """Replays the stream (or `ops` at `arrival_us`) through a fresh server;
     returns (stats dict, per-request latencies in us)."""
+    from . import autotune
     so = L.load()
+    autotune.apply()  # the tuned gemm tile table, when one has been written
     ops = w.ops if ops is None else ops
     arr = w.s.arrival_us if arrival_us is None else arrival_us
     arr = np.ascontiguousarray(arr, dtype=np.float64)

@@ -111,3 +111,36 @@ def test_split_error_vs_plain_fp32(so):
             errs[form] = ((Cm.double() - truth).abs() / mag).max().item()
         assert errs[6] <= errs["plain"] * 1.05, errs
         assert errs[3] <= 4 * n * (EPS32 + PER_PRODUCT[3]), errs
+
+
+@pytest.mark.parametrize("update", [16, 32])
+@pytest.mark.parametrize("refine", [0, 1])
+@pytest.mark.parametrize("uplo,trans,diag", list(itertools.product([UPPER, LOWER], [NO_TRANS, TRANS], [131, 132])))
+@pytest.mark.parametrize("n", [65, 300])
+def test_trsv_mixed(so, update, refine, uplo, trans, diag, n, rng):
+    """Against a float64 solve of the same FP32 matrix: one refinement step
+    brings the solution to FP32 rounding of the exact one (|err| <= 4 eps_32
+    |x|); without refinement the BF16 off-diagonal updates leave ~2^-8 of the
+    off-diagonal contribution."""
+    from exoblas_amd import lib as L
+    A = (rng.uniform(-1, 1, (n, n)) / n).astype(np.float32)
+    A[np.arange(n), np.arange(n)] = (n + rng.uniform(0, 1, n)).astype(np.float32)
+    b = rng.uniform(-1, 1, n).astype(np.float32)
+    T = np.tril(A) if uplo == LOWER else np.triu(A)
+    T = T.astype(np.float64)
+    if diag == 132:
+        np.fill_diagonal(T, 1.0)
+    opT = T if trans == NO_TRANS else T.T
+    exact = np.linalg.solve(opT, b.astype(np.float64))
+    x = b.copy()
+    st = so.exoblas_strsv_mixed(None, uplo, trans, diag, n, L.win1(x.ctypes.data, 1), L.win2(A.ctypes.data, n),
+                                update, refine)
+    assert st == 0, L.last_error()
+    err = np.abs(x.astype(np.float64) - exact)
+    if refine:
+        tol = 4 * EPS32 * np.abs(exact) + 1e-30
+    else:
+        off = np.abs(opT - np.diag(np.diag(opT))) @ np.abs(exact)
+        per = 2.0 ** -8 if update == 16 else 4 * n * EPS32
+        tol = 4 * EPS32 * np.abs(exact) + per * off * (3.0 if diag == 132 else 3.0 / n) + 1e-30
+    assert np.all(err <= tol), float((err / tol).max())
