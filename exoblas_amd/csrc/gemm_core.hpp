@@ -45,18 +45,25 @@ struct Operand {
 typedef float v4f __attribute__((ext_vector_type(4)));
 typedef double v4d __attribute__((ext_vector_type(4)));
 
+// One 16 x 16 x 4 MFMA step and its output layout: lane (lm = lane % 16,
+// lg = lane / 16) holds column lm of D in its 4 accumulator registers q, at
+// row out_row(lg, q).  The FP32 form keeps 4 consecutive rows per lane group;
+// the FP64 form interleaves the lane groups (row 4q + lg), as measured on
+// gfx950.
 template <class T> struct Mfma;
 template <> struct Mfma<float> {
   using acc = v4f;
   static __device__ __forceinline__ acc step(float a, float b, acc c) {
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
   }
+  static __device__ __forceinline__ int out_row(int lg, int q) { return 4 * lg + q; }
 };
 template <> struct Mfma<double> {
   using acc = v4d;
   static __device__ __forceinline__ acc step(double a, double b, acc c) {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
   }
+  static __device__ __forceinline__ int out_row(int lg, int q) { return 4 * q + lg; }
 };
 
 template <class T, int N> struct alignas(N * sizeof(T)) VecN { T v[N]; };
@@ -256,7 +263,7 @@ __device__ __forceinline__ void tile_store(typename C::type* Cp, int64_t ldc, in
     const int ga = ta / C::EA, ea = ta % C::EA;
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-      const int64_t row = m0 + wm0 + ga * 16 * C::EA + C::EA * (4 * lg + q) + ea;
+      const int64_t row = m0 + wm0 + ga * 16 * C::EA + C::EA * Mfma<T>::out_row(lg, q) + ea;
       if (row >= M) continue;
 #pragma unroll
       for (int gb = 0; gb < C::RB; ++gb) {

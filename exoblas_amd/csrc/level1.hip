@@ -195,29 +195,32 @@ __global__ __launch_bounds__(kNT) void reduce_kernel(int64_t n, const T* __restr
   for (int u = 0; u < U; ++u) acc[u] = Acc(0);
 
   if (vec) {
+    // The grid sweeps the vector front to back: in round j workgroup b reads
+    // the kNT*U vectors starting at (j * gridDim.x + b) * kNT*U, so the whole
+    // chip streams one contiguous region at a time.
     const int64_t nvec = n / VN;
-    const int64_t chunk = ceil_div(ceil_div(nvec, gridDim.x), kNT * U) * (kNT * U);
-    const int64_t beg = (int64_t)blockIdx.x * chunk;
-    const int64_t end = beg + chunk < nvec ? beg + chunk : nvec;
-    int64_t i = beg + threadIdx.x;
-    for (; i + (U - 1) * kNT < end; i += U * kNT) {
+    const int64_t span = (int64_t)kNT * U;
+    int64_t base = (int64_t)blockIdx.x * span;
+    for (; base + span <= nvec; base += (int64_t)gridDim.x * span) {
       V xv[U], yv[U];
 #pragma unroll
       for (int u = 0; u < U; ++u) {
-        xv[u] = reinterpret_cast<const V*>(x)[i + u * kNT];
-        if (Term::ry) yv[u] = reinterpret_cast<const V*>(y)[i + u * kNT];
+        xv[u] = reinterpret_cast<const V*>(x)[base + u * kNT + threadIdx.x];
+        if (Term::ry) yv[u] = reinterpret_cast<const V*>(y)[base + u * kNT + threadIdx.x];
       }
 #pragma unroll
       for (int u = 0; u < U; ++u)
 #pragma unroll
         for (int k = 0; k < VN; ++k) acc[u] += term(vget<T>(xv[u], k), Term::ry ? vget<T>(yv[u], k) : T(0));
     }
-    for (; i < end; i += kNT) {
-      V xv = reinterpret_cast<const V*>(x)[i];
-      V yv;
-      if (Term::ry) yv = reinterpret_cast<const V*>(y)[i];
+    if (base < nvec) {  // the partial last round
+      for (int64_t i = base + threadIdx.x; i < nvec; i += kNT) {
+        const V xv = reinterpret_cast<const V*>(x)[i];
+        V yv;
+        if (Term::ry) yv = reinterpret_cast<const V*>(y)[i];
 #pragma unroll
-      for (int k = 0; k < VN; ++k) acc[0] += term(vget<T>(xv, k), Term::ry ? vget<T>(yv, k) : T(0));
+        for (int k = 0; k < VN; ++k) acc[0] += term(vget<T>(xv, k), Term::ry ? vget<T>(yv, k) : T(0));
+      }
     }
     if (blockIdx.x == 0) {
       const int64_t t = nvec * VN + threadIdx.x;
@@ -256,10 +259,10 @@ void launch_reduce(Ctx& c, int64_t n, const T* x, int64_t incx, const T* y, int6
                    typename Term::Acc add) {
   using Acc = typename Term::Acc;
   constexpr int VN = Vec16<T>::n;
-  constexpr int U = 4;
+  constexpr int U = 8;
   const bool vec = incx == 1 && aligned16(x) && (!Term::ry || (incy == 1 && aligned16(y)));
   int64_t blocks = std::max<int64_t>(1, ceil_div(n, (int64_t)kNT * U * VN));
-  blocks = std::min<int64_t>(blocks, 2048);
+  blocks = std::min<int64_t>(blocks, (int64_t)c.num_cus * 8);
   Acc* partials = static_cast<Acc*>(c.scratch(sizeof(Acc) * blocks));
   hipLaunchKernelGGL((reduce_kernel<T, Term, Out, U>), dim3((unsigned)blocks), dim3(kNT), 0, c.stream, n, x, incx,
                      y, incy, vec ? 1 : 0, partials, c.counters(), result, Finish<Acc, Out>{add}, Term{});

@@ -218,7 +218,7 @@ template <class T, int TRI, bool UNIT>
 void launch_gemv_n(Ctx& c, int64_t M, int64_t N, T alpha, T beta, const T* A, int64_t lda, const T* x,
                    int64_t incx, T* y, int64_t incy) {
   constexpr int VN = Vec16<T>::n;
-  constexpr int R = 4;
+  constexpr int R = 8;
   constexpr int P = 2048;
   const bool vec = (lda % VN) == 0 && aligned16(A);
   const int64_t row_blocks = ceil_div(M, (int64_t)(kNT / kWave) * R);

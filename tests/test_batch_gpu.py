@@ -66,8 +66,8 @@ def _to_device(torch, native, probs):
             keep += [dA, dB, dC]
             o["kind"], o["trans_a"], o["trans_b"] = native.OP_GEMM, p["ta"], p["tb"]
             o["m"], o["n"], o["k"] = p["M"], p["N"], p["K"]
-            o["a"], o["lda"] = dA.data_ptr(), p["A"].shape[1]
-            o["b"], o["ldb"] = dB.data_ptr(), p["B"].shape[1]
+            o["a"], o["lda"] = dA.data_ptr(), max(1, p["A"].shape[1])
+            o["b"], o["ldb"] = dB.data_ptr(), max(1, p["B"].shape[1])
             o["c"], o["ldc"] = dC.data_ptr(), p["N"]
             o["alpha"], o["beta"] = p["alpha"], p["beta"]
             p["out"] = dC
@@ -96,8 +96,8 @@ def _check(probs):
         if kind == "gemm":
             want = p["C"].copy()
             o.mscal(p["M"], p["N"], p["beta"], want, p["N"])
-            o.gemm(p["ta"], p["tb"], p["M"], p["N"], p["K"], p["alpha"], p["A"], p["A"].shape[1], p["B"],
-                   p["B"].shape[1], want, p["N"])
+            o.gemm(p["ta"], p["tb"], p["M"], p["N"], p["K"], p["alpha"], p["A"], max(1, p["A"].shape[1]), p["B"],
+                   max(1, p["B"].shape[1]), want, p["N"])
             opA = np.abs(p["A"] if p["ta"] == NO_TRANS else p["A"].T).astype(np.float64)
             opB = np.abs(p["B"] if p["tb"] == NO_TRANS else p["B"].T).astype(np.float64)
             mag = abs(p["beta"]) * np.abs(p["C"]) + p["alpha"] * (opA @ opB)
