@@ -3,10 +3,10 @@
 // All of these are HBM-bound.  The unit-stride path moves 16 bytes per lane
 // per access (global_load_dwordx4), unrolled so each lane keeps several loads
 // in flight; a wave64 touches 1 KiB contiguous per access.  Strided and
-// misaligned windows use a grid-stride scalar kernel.  Reductions are one
-// launch: per-workgroup partial sums (wave shuffle + LDS), written to the
-// stream's scratch, and the last workgroup to finish (atomic ticket) sums the
-// partials in a fixed order, so results are deterministic.
+// misaligned windows use a grid-stride scalar kernel.  Reductions are two
+// launches: per-workgroup partial sums (wave shuffle + LDS) written to the
+// stream's scratch, then one workgroup sums the partials in a fixed order, so
+// results are deterministic.
 #include "device.hpp"
 #include "ops.hpp"
 
@@ -182,14 +182,11 @@ template <class Acc, class Out> struct Finish {
 template <class T, class Term, class Out, int U>
 __global__ __launch_bounds__(kNT) void reduce_kernel(int64_t n, const T* __restrict__ x, int64_t incx,
                                                      const T* __restrict__ y, int64_t incy, int vec,
-                                                     typename Term::Acc* partials, unsigned* ticket,
-                                                     Out* result, Finish<typename Term::Acc, Out> fin,
-                                                     Term term) {
+                                                     typename Term::Acc* partials, Term term) {
   using Acc = typename Term::Acc;
   using V = typename Vec16<T>::type;
   constexpr int VN = Vec16<T>::n;
   __shared__ Acc red[kNT / kWave];
-  __shared__ int is_last;
   Acc acc[U];
 #pragma unroll
   for (int u = 0; u < U; ++u) acc[u] = Acc(0);
@@ -235,23 +232,21 @@ __global__ __launch_bounds__(kNT) void reduce_kernel(int64_t n, const T* __restr
 #pragma unroll
   for (int u = 1; u < U; ++u) s += acc[u];
   s = block_sum<kNT>(s, red);
-  if (threadIdx.x == 0) {
-    partials[blockIdx.x] = s;
-    __threadfence();
-    const unsigned prev = atomicAdd(ticket, 1u);
-    is_last = (prev == gridDim.x - 1);
-  }
-  __syncthreads();
-  if (is_last) {
-    __threadfence();
-    Acc t = Acc(0);
-    for (int i = threadIdx.x; i < (int)gridDim.x; i += kNT) t += partials[i];
-    t = block_sum<kNT>(t, red);
-    if (threadIdx.x == 0) {
-      *result = fin(t);
-      *ticket = 0u;
-    }
-  }
+  if (threadIdx.x == 0) partials[blockIdx.x] = s;
+}
+
+// Sums the per-workgroup partials in a fixed order and writes the result.
+// A second launch rather than a last-workgroup ticket: the ticket needs a
+// device-scope release per workgroup, which writes back the XCD's L2 each
+// time and costs more than this launch.
+template <class Acc, class Out>
+__global__ __launch_bounds__(kNT) void reduce_finish_kernel(int nparts, const Acc* partials, Out* result,
+                                                            Finish<Acc, Out> fin) {
+  __shared__ Acc red[kNT / kWave];
+  Acc t = Acc(0);
+  for (int i = threadIdx.x; i < nparts; i += kNT) t += partials[i];
+  t = block_sum<kNT>(t, red);
+  if (threadIdx.x == 0) *result = fin(t);
 }
 
 template <class T, class Term, class Out>
@@ -265,7 +260,10 @@ void launch_reduce(Ctx& c, int64_t n, const T* x, int64_t incx, const T* y, int6
   blocks = std::min<int64_t>(blocks, (int64_t)c.num_cus * 8);
   Acc* partials = static_cast<Acc*>(c.scratch(sizeof(Acc) * blocks));
   hipLaunchKernelGGL((reduce_kernel<T, Term, Out, U>), dim3((unsigned)blocks), dim3(kNT), 0, c.stream, n, x, incx,
-                     y, incy, vec ? 1 : 0, partials, c.counters(), result, Finish<Acc, Out>{add}, Term{});
+                     y, incy, vec ? 1 : 0, partials, Term{});
+  EXO_LAUNCH_CHECK();
+  hipLaunchKernelGGL((reduce_finish_kernel<Acc, Out>), dim3(1), dim3(kNT), 0, c.stream, (int)blocks, partials, result,
+                     Finish<Acc, Out>{add});
   EXO_LAUNCH_CHECK();
 }
 

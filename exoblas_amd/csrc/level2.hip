@@ -123,19 +123,35 @@ __global__ __launch_bounds__(kNT) void gemv_n_kernel(int64_t M, int64_t N, T alp
   }
 }
 
-// y[r] = beta*y[r] + alpha * sum_s part[s*M + r]  (or the triangular overwrite)
+// y[r] = beta*y[r] + alpha * sum_s part[s*M + r]  (or the triangular overwrite).
+// A workgroup finishes 64 rows; its 4 waves take every 4th split each and
+// combine through LDS in a fixed order (deterministic).
 template <class T, bool TRI, bool UNIT>
 __global__ __launch_bounds__(kNT) void gemv_finish_kernel(int64_t M, int splits, T alpha, T beta,
                                                           const T* part, const T* x, int64_t incx, T* y,
                                                           int64_t incy) {
-  const int64_t step = (int64_t)gridDim.x * kNT;
-  for (int64_t r = (int64_t)blockIdx.x * kNT + threadIdx.x; r < M; r += step) {
-    T s = T(0);
-    for (int k = 0; k < splits; ++k) s += part[(int64_t)k * M + r];
+  constexpr int NW = kNT / kWave;
+  __shared__ T red[NW][kWave];
+  const int rl = threadIdx.x % kWave, g = threadIdx.x / kWave;
+  const int64_t r = (int64_t)blockIdx.x * kWave + rl;
+  T s = T(0);
+  if (r < M) {
+    int k = g;
+    for (; k + 3 * NW < splits; k += 4 * NW)
+      s += (part[(int64_t)k * M + r] + part[(int64_t)(k + NW) * M + r]) +
+           (part[(int64_t)(k + 2 * NW) * M + r] + part[(int64_t)(k + 3 * NW) * M + r]);
+    for (; k < splits; k += NW) s += part[(int64_t)k * M + r];
+  }
+  red[g][rl] = s;
+  __syncthreads();
+  if (g == 0 && r < M) {
+    T t = red[0][rl];
+#pragma unroll
+    for (int w = 1; w < NW; ++w) t += red[w][rl];
     if (TRI)
-      y[r * incy] = UNIT ? s + x[r * incx] : s;
+      y[r * incy] = UNIT ? t + x[r * incx] : t;
     else
-      y[r * incy] = beta * y[r * incy] + alpha * s;
+      y[r * incy] = beta * y[r * incy] + alpha * t;
   }
 }
 
@@ -237,9 +253,8 @@ void launch_gemv_n(Ctx& c, int64_t M, int64_t N, T alpha, T beta, const T* A, in
                        beta, A, lda, x, incx, y, incy, cols_per, part);
   EXO_LAUNCH_CHECK();
   if (splits > 1) {
-    const int64_t fb = std::min<int64_t>(ceil_div(M, kNT), 1024);
-    hipLaunchKernelGGL((gemv_finish_kernel<T, TRI != kFull, UNIT>), dim3((unsigned)fb), dim3(kNT), 0, c.stream, M,
-                       (int)splits, alpha, beta, part, x, incx, y, incy);
+    hipLaunchKernelGGL((gemv_finish_kernel<T, TRI != kFull, UNIT>), dim3((unsigned)ceil_div(M, kWave)), dim3(kNT), 0,
+                       c.stream, M, (int)splits, alpha, beta, part, x, incx, y, incy);
     EXO_LAUNCH_CHECK();
   }
 }
@@ -265,9 +280,8 @@ void launch_gemv_t(Ctx& c, int64_t Mc, int64_t Nr, T alpha, T beta, const T* A, 
                        beta, A, lda, x, incx, y, incy, rows_per, part);
   EXO_LAUNCH_CHECK();
   if (splits > 1) {
-    const int64_t fb = std::min<int64_t>(ceil_div(Mc, kNT), 1024);
-    hipLaunchKernelGGL((gemv_finish_kernel<T, TRI != kFull, UNIT>), dim3((unsigned)fb), dim3(kNT), 0, c.stream,
-                       Mc, (int)splits, alpha, beta, part, x, incx, y, incy);
+    hipLaunchKernelGGL((gemv_finish_kernel<T, TRI != kFull, UNIT>), dim3((unsigned)ceil_div(Mc, kWave)), dim3(kNT), 0,
+                       c.stream, Mc, (int)splits, alpha, beta, part, x, incx, y, incy);
     EXO_LAUNCH_CHECK();
   }
 }

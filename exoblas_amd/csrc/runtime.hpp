@@ -76,7 +76,7 @@ struct Ctx {
   // next scratch() call for that slot on this stream.  Growing a slot
   // synchronises the stream first, so earlier kernels never lose theirs.
   void* scratch(size_t bytes, int slot = 0);
-  static constexpr int kSlots = 4;
+  static constexpr int kSlots = 6;
   // kCounters zero-initialised 32-bit counters owned by this stream.  A kernel
   // that uses them must leave them zero when it ends.
   unsigned* counters();

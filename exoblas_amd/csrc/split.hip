@@ -3,17 +3,21 @@
 // Each FP32 operand element x is split into BF16 pieces by repeated rounding
 // of the exact remainder:  x = h + m + l + r  with h = bf16(x),
 // m = bf16(x - h), l = bf16(x - h - m) (every difference is exact in FP32).
-//   terms = 3:  two pieces (h, m):     h*h + h*m + m*h        |r| ~ 2^-17 |x|
-//   terms = 6:  three pieces (h, m, l): adds m*m + h*l + l*h  |r| ~ 2^-26 |x|
+//   terms = 3:  two pieces (h, m):      h*h + h*m + m*h        |r| ~ 2^-17 |x|
+//   terms = 6:  three pieces (h, m, l): adds m*m + h*l + l*h   |r| ~ 2^-26 |x|
 // Products of BF16 pieces are exact in FP32 and the MFMA accumulates in FP32;
 // the small terms are issued first so they enter the accumulator before the
 // dominant h*h term of the same K step.
 //
-// Tile: 128 x 128 per workgroup of 4 waves (64 x 64 each, 4 x 4 MFMA tiles of
-// 16 x 16), K step 32.  Operand tiles are read from global memory as FP32 with
-// the level-3 tile loader (any trans form), split while being written to LDS
-// in row-major [row][k] BF16 layout, so a lane's MFMA fragment (8 consecutive
-// k of one row) is one 16-byte LDS read per piece.
+// Two passes:
+//  1. pack: each operand, in any trans form, is split once into P BF16 planes
+//     in the MFMA-fragment layout [plane][K/8][rows][8] (rows padded to 128,
+//     K to 32, zero filled), so 8 consecutive k of one row are 16 contiguous
+//     bytes;
+//  2. product: 128 x 128 workgroup tiles, 4 waves of 64 x 64 (4 x 4 MFMA tiles
+//     of 16 x 16), K step 32, planes copied global -> LDS with 16-byte loads
+//     and stores (double-buffered), one 16-byte LDS read per fragment, no
+//     bounds checks inside the K loop.
 #include "device.hpp"
 #include "gemm_core.hpp"
 #include "ops.hpp"
@@ -23,14 +27,16 @@ namespace split {
 
 using gemm::Operand;
 typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
-typedef __bf16 v4bf __attribute__((ext_vector_type(4)));
 typedef float v4f __attribute__((ext_vector_type(4)));
 
-constexpr int BM = 128, BN = 128, BK = 32, NT = 256, WM = 64, WN = 64;
-constexpr int SM = WM / 16, SN = WN / 16;
-constexpr int KS = BK + 8;  // LDS row stride in BF16 elements (80 bytes)
+constexpr int BT = 128;  // tile rows (M and N)
+constexpr int KT = 32;   // K per tile = one MFMA K step
+constexpr int G = KT / 8;
+constexpr int NT = 256;
+constexpr int WT = 64;   // wave tile
+constexpr int S = WT / 16;
 
-using Loader = gemm::TileLoader<float, 128, BK, NT>;
+__host__ __device__ inline int64_t pad(int64_t v, int64_t m) { return (v + m - 1) / m * m; }
 
 template <int P>
 __device__ __forceinline__ void pieces(float x, __bf16 (&out)[P]) {
@@ -42,104 +48,112 @@ __device__ __forceinline__ void pieces(float x, __bf16 (&out)[P]) {
   }
 }
 
-// Writes the loader's FP32 registers to LDS as P BF16 planes [P][R][KS].
-template <int P, int R>
-__device__ __forceinline__ void store_split(const Loader& L, __bf16* lds) {
-  constexpr int VN = 4;
-  constexpr int NE = Loader::NE;
-  const int tid = threadIdx.x;
-  if (L.mode == 0) {  // 4 consecutive k of one row
-#pragma unroll
-    for (int i = 0; i < NE / VN; ++i) {
-      const int v = tid + NT * i;
-      const int r = v / (BK / VN), kv = v % (BK / VN);
-      v4bf q[P];
-#pragma unroll
-      for (int e = 0; e < VN; ++e) {
-        __bf16 pc[P];
-        pieces<P>(L.reg[i * VN + e], pc);
-#pragma unroll
-        for (int p = 0; p < P; ++p) q[p][e] = pc[p];
-      }
-#pragma unroll
-      for (int p = 0; p < P; ++p) *reinterpret_cast<v4bf*>(lds + (p * R + r) * KS + kv * VN) = q[p];
-    }
-  } else if (L.mode == 1) {  // 4 consecutive rows of one k
-#pragma unroll
-    for (int i = 0; i < NE / VN; ++i) {
-      const int v = tid + NT * i;
-      const int k = v / (R / VN), rv = v % (R / VN);
-#pragma unroll
-      for (int e = 0; e < VN; ++e) {
-        __bf16 pc[P];
-        pieces<P>(L.reg[i * VN + e], pc);
-#pragma unroll
-        for (int p = 0; p < P; ++p) lds[(p * R + rv * VN + e) * KS + k] = pc[p];
-      }
-    }
+// out[((p * (Kp/8) + g) * Rp + r) * 8 + e] = piece p of X(r, 8g + e).
+template <int P>
+__global__ __launch_bounds__(NT) void pack_kernel(Operand<float> X, int64_t R, int64_t K, int64_t Rp, int64_t Kp,
+                                                  __bf16* out) {
+  const int64_t groups = Kp / 8;
+  const int64_t idx = (int64_t)blockIdx.x * NT + threadIdx.x;
+  if (idx >= groups * Rp) return;
+  const int64_t g = idx / Rp, r = idx % Rp;
+  float v[8];
+  if (X.form == gemm::kRK && X.vec && r < R && 8 * g + 8 <= K) {
+    const float4 a = *reinterpret_cast<const float4*>(X.p + r * X.ld + 8 * g);
+    const float4 b = *reinterpret_cast<const float4*>(X.p + r * X.ld + 8 * g + 4);
+    v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w; v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
   } else {
 #pragma unroll
-    for (int j = 0; j < NE; ++j) {
-      const int idx = tid + NT * j;
-      const int k = idx / R, r = idx % R;
-      __bf16 pc[P];
-      pieces<P>(L.reg[j], pc);
-#pragma unroll
-      for (int p = 0; p < P; ++p) lds[(p * R + r) * KS + k] = pc[p];
+    for (int e = 0; e < 8; ++e) {
+      const int64_t k = 8 * g + e;
+      v[e] = (r < R && k < K) ? (X.form == gemm::kRK ? X.p[r * X.ld + k] : X.p[k * X.ld + r]) : 0.f;
     }
   }
+  v8bf q[P];
+#pragma unroll
+  for (int e = 0; e < 8; ++e) {
+    __bf16 pc[P];
+    pieces<P>(v[e], pc);
+#pragma unroll
+    for (int p = 0; p < P; ++p) q[p][e] = pc[p];
+  }
+#pragma unroll
+  for (int p = 0; p < P; ++p) *reinterpret_cast<v8bf*>(out + (((int64_t)p * groups + g) * Rp + r) * 8) = q[p];
 }
 
 __device__ __forceinline__ v4f mma(v8bf a, v8bf b, v4f c) {
   return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
 }
 
+// LDS tile of one operand: [P][G][BT] 16-byte fragments (8 BF16 each).
+template <int P>
+struct Stage {
+  static constexpr int NV = P * G * BT;  // 16-byte vectors per tile
+  static constexpr int PER = NV / NT;    // per thread
+  uint4 reg[PER];
+  __device__ __forceinline__ void load(const __bf16* planes, int64_t groups, int64_t Rp, int64_t r0, int64_t kt) {
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+      const int v = threadIdx.x + NT * i;
+      const int p = v / (G * BT), rem = v % (G * BT), gl = rem / BT, r = rem % BT;
+      reg[i] = *reinterpret_cast<const uint4*>(planes + (((int64_t)p * groups + kt * G + gl) * Rp + r0 + r) * 8);
+    }
+  }
+  __device__ __forceinline__ void store(uint4* lds) const {
+#pragma unroll
+    for (int i = 0; i < PER; ++i) lds[threadIdx.x + NT * i] = reg[i];
+  }
+};
+
 template <int P, int TRI>
-__global__ __launch_bounds__(NT) void split_kernel(int64_t M, int64_t N, int64_t K, float alpha, Operand<float> A,
-                                                   Operand<float> B, float* Cp, int64_t ldc, int64_t tiles_m,
-                                                   int64_t tiles_n) {
-  __shared__ __align__(16) __bf16 As[P * BM * KS];
-  __shared__ __align__(16) __bf16 Bs[P * BN * KS];
+__global__ __launch_bounds__(NT) void product_kernel(int64_t M, int64_t N, int64_t Kp, float alpha, const __bf16* Ap,
+                                                     int64_t Mp, const __bf16* Bp, int64_t Np, float* Cp, int64_t ldc,
+                                                     int64_t tiles_m, int64_t tiles_n) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  constexpr int TV = P * G * BT;  // vectors per operand tile
+  uint4* As = reinterpret_cast<uint4*>(smem);
+  uint4* Bs = As + 2 * TV;
   int64_t tm, tn;
   gemm::tile_coords(blockIdx.x, gridDim.x, tiles_m, tiles_n, tm, tn);
-  const int64_t m0 = tm * BM, n0 = tn * BN;
-  if (TRI == gemm::kLower && n0 > m0 + BM - 1) return;
-  if (TRI == gemm::kUpper && n0 + BN - 1 < m0) return;
+  const int64_t m0 = tm * BT, n0 = tn * BT;
+  if (TRI == gemm::kLower && n0 > m0 + BT - 1) return;
+  if (TRI == gemm::kUpper && n0 + BT - 1 < m0) return;
   const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
-  const int wm0 = (wave / 2) * WM, wn0 = (wave % 2) * WN;
+  const int wm0 = (wave / 2) * WT, wn0 = (wave % 2) * WT;
   const int lm = lane % 16, lg = lane / 16;
-  v4f acc[SM][SN];
+  const int64_t groups = Kp / 8;
+  const int64_t nk = Kp / KT;
+  v4f acc[S][S];
 #pragma unroll
-  for (int a = 0; a < SM; ++a)
+  for (int a = 0; a < S; ++a)
 #pragma unroll
-    for (int b = 0; b < SN; ++b) acc[a][b] = v4f{0.f, 0.f, 0.f, 0.f};
-  Loader la, lb;
-  const int64_t nk = (K + BK - 1) / BK;
-  la.load(A, m0, 0, M, K);
-  lb.load(B, n0, 0, N, K);
+    for (int b = 0; b < S; ++b) acc[a][b] = v4f{0.f, 0.f, 0.f, 0.f};
+  Stage<P> la, lb;
+  la.load(Ap, groups, Mp, m0, 0);
+  lb.load(Bp, groups, Np, n0, 0);
+  la.store(As);
+  lb.store(Bs);
+  __syncthreads();
   for (int64_t t = 0; t < nk; ++t) {
-    __syncthreads();
-    store_split<P, BM>(la, As);
-    store_split<P, BN>(lb, Bs);
-    __syncthreads();
+    const int cur = (int)(t & 1);
     if (t + 1 < nk) {
-      la.load(A, m0, (t + 1) * BK, M, K);
-      lb.load(B, n0, (t + 1) * BK, N, K);
+      la.load(Ap, groups, Mp, m0, t + 1);
+      lb.load(Bp, groups, Np, n0, t + 1);
     }
-    v8bf a[P][SM], b[P][SN];
+    const v8bf* as = reinterpret_cast<const v8bf*>(As + cur * TV);
+    const v8bf* bs = reinterpret_cast<const v8bf*>(Bs + cur * TV);
+    v8bf a[P][S], b[P][S];
 #pragma unroll
     for (int p = 0; p < P; ++p) {
 #pragma unroll
-      for (int ta = 0; ta < SM; ++ta)
-        a[p][ta] = *reinterpret_cast<const v8bf*>(As + (p * BM + wm0 + ta * 16 + lm) * KS + 8 * lg);
-#pragma unroll
-      for (int tb = 0; tb < SN; ++tb)
-        b[p][tb] = *reinterpret_cast<const v8bf*>(Bs + (p * BN + wn0 + tb * 16 + lm) * KS + 8 * lg);
+      for (int i = 0; i < S; ++i) {
+        a[p][i] = as[(p * G + lg) * BT + wm0 + i * 16 + lm];
+        b[p][i] = bs[(p * G + lg) * BT + wn0 + i * 16 + lm];
+      }
     }
 #pragma unroll
-    for (int ta = 0; ta < SM; ++ta)
+    for (int ta = 0; ta < S; ++ta)
 #pragma unroll
-      for (int tb = 0; tb < SN; ++tb) {
+      for (int tb = 0; tb < S; ++tb) {
         v4f c = acc[ta][tb];
         if (P == 3) {
           c = mma(a[2][ta], b[0][tb], c);
@@ -151,16 +165,22 @@ __global__ __launch_bounds__(NT) void split_kernel(int64_t M, int64_t N, int64_t
         c = mma(a[0][ta], b[0][tb], c);
         acc[ta][tb] = c;
       }
+    if (t + 1 < nk) {
+      la.store(As + (cur ^ 1) * TV);
+      lb.store(Bs + (cur ^ 1) * TV);
+    }
+    __syncthreads();
   }
-  // D layout of the 16 x 16 MFMA: lane (lm, lg) holds rows 4*lg + q of column lm.
+  // D layout of the FP32-accumulating 16 x 16 MFMA: lane (lm, lg) holds rows
+  // 4*lg + q of column lm.
 #pragma unroll
-  for (int ta = 0; ta < SM; ++ta)
+  for (int ta = 0; ta < S; ++ta)
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
       const int64_t row = m0 + wm0 + ta * 16 + 4 * lg + q;
       if (row >= M) continue;
 #pragma unroll
-      for (int tb = 0; tb < SN; ++tb) {
+      for (int tb = 0; tb < S; ++tb) {
         const int64_t col = n0 + wn0 + tb * 16 + lm;
         bool in = col < N;
         if (TRI == gemm::kLower) in = in && col <= row;
@@ -170,29 +190,57 @@ __global__ __launch_bounds__(NT) void split_kernel(int64_t M, int64_t N, int64_t
     }
 }
 
-template <int P, int TRI>
-void launch(Ctx& c, int64_t M, int64_t N, int64_t K, float alpha, const Operand<float>& A, const Operand<float>& B,
-            float* Cp, int64_t ldc) {
-  const int64_t tm = ceil_div(M, BM), tn = ceil_div(N, BN);
-  hipLaunchKernelGGL((split_kernel<P, TRI>), dim3((unsigned)(tm * tn)), dim3(NT), 0, c.stream, M, N, K, alpha, A, B,
-                     Cp, ldc, tm, tn);
+template <int P>
+void pack(Ctx& c, const Operand<float>& X, int64_t R, int64_t K, int64_t Rp, int64_t Kp, __bf16* out) {
+  const int64_t items = (Kp / 8) * Rp;
+  hipLaunchKernelGGL((pack_kernel<P>), dim3((unsigned)ceil_div(items, NT)), dim3(NT), 0, c.stream, X, R, K, Rp, Kp,
+                     out);
   EXO_LAUNCH_CHECK();
 }
 
-void run(Ctx& c, int tri, int64_t terms, int64_t M, int64_t N, int64_t K, float alpha, const Operand<float>& A,
-         const Operand<float>& B, float* Cp, int64_t ldc) {
-  if (M <= 0 || N <= 0 || K <= 0) return;
-  const bool six = terms == 6;
-  if (tri == gemm::kLower) {
-    if (six) launch<3, gemm::kLower>(c, M, N, K, alpha, A, B, Cp, ldc);
-    else launch<2, gemm::kLower>(c, M, N, K, alpha, A, B, Cp, ldc);
-  } else if (tri == gemm::kUpper) {
-    if (six) launch<3, gemm::kUpper>(c, M, N, K, alpha, A, B, Cp, ldc);
-    else launch<2, gemm::kUpper>(c, M, N, K, alpha, A, B, Cp, ldc);
-  } else {
-    if (six) launch<3, gemm::kFull>(c, M, N, K, alpha, A, B, Cp, ldc);
-    else launch<2, gemm::kFull>(c, M, N, K, alpha, A, B, Cp, ldc);
+template <int P, int TRI>
+void product(Ctx& c, int64_t M, int64_t N, int64_t Kp, float alpha, const __bf16* Ap, int64_t Mp, const __bf16* Bp,
+             int64_t Np, float* Cp, int64_t ldc) {
+  const int64_t tm = Mp / BT, tn = Np / BT;
+  const size_t lds = 2 * 2 * (size_t)P * G * BT * 16;
+  static bool attr_set = false;
+  if (!attr_set) {
+    EXO_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&product_kernel<P, TRI>),
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    attr_set = true;
   }
+  hipLaunchKernelGGL((product_kernel<P, TRI>), dim3((unsigned)(tm * tn)), dim3(NT), lds, c.stream, M, N, Kp, alpha, Ap,
+                     Mp, Bp, Np, Cp, ldc, tm, tn);
+  EXO_LAUNCH_CHECK();
+}
+
+// C (tri part) += alpha * X_A X_B^T, X_A M x K, X_B N x K; same_operand when
+// X_B is X_A (syrk), which is then packed once.
+template <int P>
+void run_p(Ctx& c, int tri, int64_t M, int64_t N, int64_t K, float alpha, const Operand<float>& A,
+           const Operand<float>& B, bool same_operand, float* Cp, int64_t ldc) {
+  const int64_t Mp = pad(M, BT), Np = pad(N, BT), Kp = pad(K, KT);
+  const size_t a_elems = (size_t)P * Kp * Mp, b_elems = same_operand ? 0 : (size_t)P * Kp * Np;
+  __bf16* ws = static_cast<__bf16*>(c.scratch((a_elems + b_elems) * sizeof(__bf16), 5));
+  __bf16* Ap = ws;
+  __bf16* Bp = same_operand ? ws : ws + a_elems;
+  pack<P>(c, A, M, K, Mp, Kp, Ap);
+  if (!same_operand) pack<P>(c, B, N, K, Np, Kp, Bp);
+  if (tri == gemm::kLower)
+    product<P, gemm::kLower>(c, M, N, Kp, alpha, Ap, Mp, Bp, same_operand ? Mp : Np, Cp, ldc);
+  else if (tri == gemm::kUpper)
+    product<P, gemm::kUpper>(c, M, N, Kp, alpha, Ap, Mp, Bp, same_operand ? Mp : Np, Cp, ldc);
+  else
+    product<P, gemm::kFull>(c, M, N, Kp, alpha, Ap, Mp, Bp, same_operand ? Mp : Np, Cp, ldc);
+}
+
+void run(Ctx& c, int tri, int64_t terms, int64_t M, int64_t N, int64_t K, float alpha, const Operand<float>& A,
+         const Operand<float>& B, bool same_operand, float* Cp, int64_t ldc) {
+  if (M <= 0 || N <= 0 || K <= 0) return;
+  if (terms == 6)
+    run_p<3>(c, tri, M, N, K, alpha, A, B, same_operand, Cp, ldc);
+  else
+    run_p<2>(c, tri, M, N, K, alpha, A, B, same_operand, Cp, ldc);
 }
 
 Operand<float> operand(const float* p, int64_t ld, int form) {
@@ -214,20 +262,15 @@ void gemm_split(Ctx& c, int64_t TransA, int64_t TransB, int64_t M, int64_t N, in
   using namespace gemm;
   const Operand<float> a = TransA == 111 ? split::operand(A, lda, kRK) : split::operand(AT, ldat, kKR);
   const Operand<float> b = TransB == 111 ? split::operand(B, ldb, kKR) : split::operand(BT, ldbt, kRK);
-  split::run(c, kFull, terms, M, N, K, alpha, a, b, Cp, ldc);
+  split::run(c, kFull, terms, M, N, K, alpha, a, b, false, Cp, ldc);
 }
 
 void syrk_split(Ctx& c, int64_t Uplo, int64_t Trans, int64_t N, int64_t K, float alpha, const float* A, int64_t lda,
                 const float* AT, int64_t ldat, float* Cp, int64_t ldc, int64_t terms) {
   using namespace gemm;
   const int tri = Uplo == 121 ? kUpper : kLower;
-  if (Trans == 111) {
-    const Operand<float> a = split::operand(A, lda, kRK);
-    split::run(c, tri, terms, N, N, K, alpha, a, a, Cp, ldc);
-  } else {
-    const Operand<float> a = split::operand(AT, ldat, kKR);
-    split::run(c, tri, terms, N, N, K, alpha, a, a, Cp, ldc);
-  }
+  const Operand<float> a = Trans == 111 ? split::operand(A, lda, kRK) : split::operand(AT, ldat, kKR);
+  split::run(c, tri, terms, N, N, K, alpha, a, a, true, Cp, ldc);
 }
 
 }  // namespace ops

@@ -5,13 +5,18 @@
 // upper/Trans) or a backward (upper/NoTrans, lower/Trans) substitution over
 // M(i, j) = TransA ? A[j, i] : A[i, j], in 64-row blocks taken in processing
 // order p = 0 .. nb-1 (block p for forward, nb-1-p for backward).  The
-// workgroup that runs step p streams the blocks of its row panel on the
-// solved side of the diagonal, each after the step that owns it has published
-// its part of x (a release store of `marker` into done[block]), so every
-// workgroup reads its panel while the chain of diagonal solves is still
-// running; then wave 0 solves the 64 x 64 diagonal block by substitution and
-// publishes its x block.  Waits are bounded: past the spin limit the step
-// records `marker` in *err and finishes.
+// workgroup that runs step p
+//   1. loads its diagonal block (in processing order it is lower triangular)
+//      and inverts it in LDS, before waiting on anything, so the inversion
+//      overlaps the earlier steps of the chain;
+//   2. streams the blocks of its row panel on the solved side of the
+//      diagonal, each after the step that owns it has published its part of x
+//      (a release store of `marker` into done[block], polled with relaxed
+//      loads and one acquire fence);
+//   3. forms v = b - (panel . x) and x_block = inv(D) v with all its threads,
+//      so the serial part of the chain is one small mat-vec per block.
+// Waits are bounded: past the spin limit the step records `marker` in *err
+// and finishes.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -25,11 +30,13 @@ constexpr int kB = 64;    // rows per block
 constexpr int kNT = 256;  // threads per workgroup
 constexpr long long kSpinLimit = 1ll << 22;
 
-template <class T>
+// TI: precision of the diagonal-block inverse and of the block mat-vec.
+template <class T, class TI = T>
 struct Smem {
-  T xs[kB];
+  TI inv[kB][kB + 1];
   T red[kNT / kB][kB];
-  T diag[kB][kB + 1];
+  T xs[kB];
+  TI vs[kB];
 };
 
 template <class T, bool TRANS>
@@ -39,12 +46,55 @@ __device__ __forceinline__ T mat(const T* A, int64_t lda, int64_t i, int64_t j) 
 
 __device__ __forceinline__ void wait_marker(const unsigned* f, unsigned marker, unsigned* err) {
   long long it = 0;
-  while (__hip_atomic_load(f, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) != marker) {
+  while (__hip_atomic_load(f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != marker) {
     if (++it > kSpinLimit) {
       __hip_atomic_store(err, marker, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      return;
+      break;
     }
     __builtin_amdgcn_s_sleep(1);
+  }
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+}
+
+// Natural row of local index i of block bi (local order = processing order).
+template <bool FWD>
+__device__ __forceinline__ int64_t row_of(int64_t r0, int i) {
+  return FWD ? r0 + i : r0 + (kB - 1 - i);
+}
+
+// Loads the diagonal block of block bi (rows r0 ..) in local order, converted
+// to TI, and inverts it in place; all kNT threads call this.
+template <class T, class TI, bool FWD, bool TRANS, bool UNIT>
+__device__ __forceinline__ void invert_diag(int64_t n, const T* A, int64_t lda, int64_t r0, TI (&L)[kB][kB + 1]) {
+  const int t = threadIdx.x;
+  for (int idx = t; idx < kB * kB; idx += kNT) {
+    // NoTrans reads rows of A, Trans reads columns: keep lanes on consecutive addresses
+    const int a = idx / kB, b = idx % kB;
+    const int i = TRANS ? b : a;  // local row
+    const int j = TRANS ? a : b;  // local column
+    const int64_t gi = row_of<FWD>(r0, i), gj = row_of<FWD>(r0, j);
+    TI v = TI(0);
+    if (j == i)
+      v = (UNIT || gi >= n) ? TI(1) : (TI)mat<T, TRANS>(A, lda, gi, gj);
+    else if (j < i && gi < n && gj < n)
+      v = (TI)mat<T, TRANS>(A, lda, gi, gj);
+    L[i][j] = v;
+  }
+  __syncthreads();
+  // Column j of the inverse from the already inverted trailing block:
+  // inv[t][j] = -(sum_{k=j+1..t} inv[t][k] L[k][j]) / L[j][j].
+  for (int j = kB - 1; j >= 0; --j) {
+    const TI djj = TI(1) / L[j][j];
+    TI nv = TI(0);
+    if (t > j && t < kB) {
+      TI s = TI(0);
+      for (int k = j + 1; k <= t; ++k) s += L[t][k] * L[k][j];
+      nv = -s * djj;
+    }
+    __syncthreads();
+    if (t > j && t < kB) L[t][j] = nv;
+    if (t == j) L[j][j] = djj;
+    __syncthreads();
   }
 }
 
@@ -56,6 +106,7 @@ __device__ __forceinline__ void step(int p, int64_t n, const T* __restrict__ A, 
   const int t = threadIdx.x;
   const int64_t bi = FWD ? p : nb - 1 - p;
   const int64_t r0 = bi * kB;
+  invert_diag<T, T, FWD, TRANS, UNIT>(n, A, lda, r0, sm.inv);
   // NoTrans: 4 threads per row, 16 consecutive columns each (contiguous
   // loads); Trans: one column of 64 rows per wave, 16 columns per wave
   // (consecutive lanes read consecutive addresses).
@@ -80,36 +131,33 @@ __device__ __forceinline__ void step(int p, int64_t n, const T* __restrict__ A, 
     for (int e = 0; e < 16; ++e) acc += m[e] * sm.xs[cq * 16 + e];
   }
   sm.red[cq][r] = acc;
-  for (int idx = t; idx < kB * kB; idx += kNT) {
-    // NoTrans reads rows of A, Trans reads columns: keep lanes on consecutive addresses
-    const int i = TRANS ? idx % kB : idx / kB;
-    const int j = TRANS ? idx / kB : idx % kB;
-    const int64_t ai = r0 + i, aj = r0 + j;
-    sm.diag[i][j] = (ai < n && aj < n) ? mat<T, TRANS>(A, lda, ai, aj) : T(i == j ? 1 : 0);
-  }
   __syncthreads();
-  if (t < kB) {
-    const int lane = t;
+  if (t < kB) {  // right-hand side in local order
+    const int64_t g = row_of<FWD>(r0, t);
+    const int nr = (int)(g - r0);
     T v = T(0);
-    if (r0 + lane < n) {
+    if (g < n) {
       T s = T(0);
 #pragma unroll
-      for (int w = 0; w < kNT / kB; ++w) s += sm.red[w][lane];
-      v = x[r0 + lane] - s;
+      for (int w = 0; w < kNT / kB; ++w) s += sm.red[w][nr];
+      v = x[g] - s;
     }
-    for (int s = 0; s < kB; ++s) {
-      const int k = FWD ? s : kB - 1 - s;
-      T xk = __shfl(v, k, kB);
-      if (!UNIT) xk = xk / sm.diag[k][k];
-      if (lane == k)
-        v = xk;
-      else if (FWD ? lane > k : lane < k)
-        v -= sm.diag[lane][k] * xk;
-    }
-    if (r0 + lane < n) x[r0 + lane] = v;
-    __threadfence();
-    if (lane == 0) __hip_atomic_store(done + bi, marker, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    sm.vs[t] = v;
   }
+  __syncthreads();
+  {  // x_local = inv(D) v: 4 threads per row, 16 columns each
+    const int i = t / 4, q4 = t % 4;
+    T s = T(0);
+#pragma unroll
+    for (int e = 0; e < 16; ++e) s += sm.inv[i][q4 * 16 + e] * sm.vs[q4 * 16 + e];
+    s += __shfl_xor(s, 1, 64);
+    s += __shfl_xor(s, 2, 64);
+    const int64_t g = row_of<FWD>(r0, i);
+    if (q4 == 0 && g < n) x[g] = s;
+  }
+  __threadfence();
+  __syncthreads();
+  if (t == 0) __hip_atomic_store(done + bi, marker, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
   __syncthreads();  // the shared buffers are reused by the caller's next step
 }
 

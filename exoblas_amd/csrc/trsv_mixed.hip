@@ -22,21 +22,19 @@ using trsv::kB;
 using trsv::kNT;
 constexpr int kFlagBase = 2;
 
-struct MixedSmem {
-  float xs[kB];
-  float red[kNT / kB][kB];
-  double diag[kB][kB + 1];
-};
-
 template <class TO>
 __device__ __forceinline__ float to_f(TO v) { return (float)v; }
 
+// One step of the mixed solve: the trsv_core step with FP32 off-diagonal
+// updates (on TO-typed copies of A), the diagonal block inverted and applied
+// in FP64, an FP64 right-hand side and solution, and an FP32 shadow of the
+// solution feeding later off-diagonal updates.
 template <class TO, bool FWD, bool TRANS, bool UNIT>
 __global__ __launch_bounds__(kNT) void trsv_mixed_kernel(int64_t n, const float* __restrict__ A, int64_t lda,
                                                          const TO* __restrict__ Aoff, int64_t ldo,
                                                          const double* __restrict__ rhs, double* x, float* xf,
                                                          unsigned* flags, unsigned epoch) {
-  __shared__ MixedSmem sm;
+  __shared__ trsv::Smem<float, double> sm;
   __shared__ int s_p;
   const int nb = (int)((n + kB - 1) / kB);
   const int t = threadIdx.x;
@@ -50,6 +48,7 @@ __global__ __launch_bounds__(kNT) void trsv_mixed_kernel(int64_t n, const float*
   unsigned* done = flags + kFlagBase;
   const int64_t bi = FWD ? p : nb - 1 - p;
   const int64_t r0 = bi * kB;
+  trsv::invert_diag<float, double, FWD, TRANS, UNIT>(n, A, lda, r0, sm.inv);
   const int r = TRANS ? t % kB : t / 4;
   const int cq = TRANS ? t / kB : t % 4;
   const int64_t gi = r0 + r;
@@ -71,38 +70,35 @@ __global__ __launch_bounds__(kNT) void trsv_mixed_kernel(int64_t n, const float*
     for (int e = 0; e < 16; ++e) acc += m[e] * sm.xs[cq * 16 + e];
   }
   sm.red[cq][r] = acc;
-  for (int idx = t; idx < kB * kB; idx += kNT) {
-    const int i = TRANS ? idx % kB : idx / kB;
-    const int j = TRANS ? idx / kB : idx % kB;
-    const int64_t ai = r0 + i, aj = r0 + j;
-    sm.diag[i][j] = (ai < n && aj < n) ? (double)(TRANS ? A[aj * lda + ai] : A[ai * lda + aj]) : (i == j ? 1.0 : 0.0);
-  }
   __syncthreads();
   if (t < kB) {
-    const int lane = t;
+    const int64_t g = trsv::row_of<FWD>(r0, t);
     double v = 0.0;
-    if (r0 + lane < n) {
+    if (g < n) {
       float s = 0.f;
 #pragma unroll
-      for (int w = 0; w < kNT / kB; ++w) s += sm.red[w][lane];
-      v = rhs[r0 + lane] - (double)s;
+      for (int w = 0; w < kNT / kB; ++w) s += sm.red[w][g - r0];
+      v = rhs[g] - (double)s;
     }
-    for (int s = 0; s < kB; ++s) {
-      const int k = FWD ? s : kB - 1 - s;
-      double xk = __shfl(v, k, kB);
-      if (!UNIT) xk = xk / sm.diag[k][k];
-      if (lane == k)
-        v = xk;
-      else if (FWD ? lane > k : lane < k)
-        v -= sm.diag[lane][k] * xk;
-    }
-    if (r0 + lane < n) {
-      x[r0 + lane] = v;
-      xf[r0 + lane] = (float)v;
-    }
-    __threadfence();
-    if (lane == 0) __hip_atomic_store(done + bi, epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    sm.vs[t] = v;
   }
+  __syncthreads();
+  {
+    const int i = t / 4, q4 = t % 4;
+    double s = 0.0;
+#pragma unroll
+    for (int e = 0; e < 16; ++e) s += sm.inv[i][q4 * 16 + e] * sm.vs[q4 * 16 + e];
+    s += __shfl_xor(s, 1, 64);
+    s += __shfl_xor(s, 2, 64);
+    const int64_t g = trsv::row_of<FWD>(r0, i);
+    if (q4 == 0 && g < n) {
+      x[g] = s;
+      xf[g] = (float)s;
+    }
+  }
+  __threadfence();
+  __syncthreads();
+  if (t == 0) __hip_atomic_store(done + bi, epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // r[i] = b[i] - sum_j M(i, j) x[j] over the triangle (FP64 accumulation), one
