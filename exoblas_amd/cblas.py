@@ -40,7 +40,9 @@ def _rebase(addr: int, n: int, inc: int, esize: int) -> int:
 
 
 def _call(name: str, *args) -> None:
-    getattr(L.load(), name)(None, *args)
+    lib = L.load()
+    lib.exoblas_clear_error()  # the status is sticky: check reports this call only
+    getattr(lib, name)(None, *args)
     L.check()
 
 

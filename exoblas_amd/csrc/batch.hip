@@ -286,6 +286,7 @@ int build(const exoblas_batch_op* ops, int64_t count, Built& out, std::string& e
       if (!o.a || !o.c) { err = at + "null pointer"; return kInvalidPointer; }
       d.m = o.m; d.n = o.m; d.k = 0;
       const uint32_t nb = (uint32_t)((o.m + kRows - 1) / kRows);
+      d.k = trsv::pub_words<float>(o.m);  // publication words this problem needs
       v.push_back({d, nb, 0, (double)nb, i});
     } else {
       err = at + "unknown kind " + std::to_string(o.kind);
@@ -303,8 +304,8 @@ int build(const exoblas_batch_op* ops, int64_t count, Built& out, std::string& e
   for (auto& it : v) {
     it.d.first_item = (uint32_t)total;
     if (it.d.kind == EXOBLAS_OP_TRSV) {
-      it.d.flag_off = (uint32_t)flags;
-      flags += it.items;
+      it.d.flag_off = (uint32_t)flags;  // even: every problem's word count is
+      flags += (uint64_t)it.d.k;        // even and the region starts 16-byte aligned
     }
     total += it.items;
     if (total > 0xffffff00ull) {

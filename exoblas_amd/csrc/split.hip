@@ -11,13 +11,16 @@
 //
 // Two passes:
 //  1. pack: each operand, in any trans form, is split once into P BF16 planes
-//     in the MFMA-fragment layout [plane][K/8][rows][8] (rows padded to 128,
-//     K to 32, zero filled), so 8 consecutive k of one row are 16 contiguous
-//     bytes;
-//  2. product: 128 x 128 workgroup tiles, 4 waves of 64 x 64 (4 x 4 MFMA tiles
-//     of 16 x 16), K step 32, planes copied global -> LDS with 16-byte loads
-//     and stores (double-buffered), one 16-byte LDS read per fragment, no
-//     bounds checks inside the K loop.
+//     in the MFMA-fragment layout [plane][K/8][rows][8] (rows padded to the
+//     tile, K to 32, zero filled), so 8 consecutive k of one row are 16
+//     contiguous bytes;
+//  2. product: 256 x 256 workgroup tiles of 4 waves (128 x 128 each, 8 x 8
+//     MFMA tiles of 16 x 16) when there are enough of them, else 128 x 128
+//     (waves of 64 x 64); K step 32, planes copied global -> LDS with 16-byte
+//     loads and stores (double-buffered), one 16-byte LDS read per fragment,
+//     no bounds checks inside the K loop.  The BF16 MFMA runs 16x the FP32
+//     MFMA rate, so the big tile is what keeps operand traffic below the L2
+//     bandwidth.
 #include "device.hpp"
 #include "gemm_core.hpp"
 #include "ops.hpp"
@@ -29,12 +32,9 @@ using gemm::Operand;
 typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
 typedef float v4f __attribute__((ext_vector_type(4)));
 
-constexpr int BT = 128;  // tile rows (M and N)
 constexpr int KT = 32;   // K per tile = one MFMA K step
 constexpr int G = KT / 8;
-constexpr int NT = 256;
-constexpr int WT = 64;   // wave tile
-constexpr int S = WT / 16;
+constexpr int NT = 256;  // pack kernel threads
 
 __host__ __device__ inline int64_t pad(int64_t v, int64_t m) { return (v + m - 1) / m * m; }
 
@@ -84,50 +84,73 @@ __device__ __forceinline__ v4f mma(v8bf a, v8bf b, v4f c) {
   return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
 }
 
-// LDS tile of one operand: [P][G][BT] 16-byte fragments (8 BF16 each).
-template <int P>
+// LDS tile of one operand: [P][G][R + PADV] 16-byte fragments (8 BF16
+// each); with PADV = 1 the extra vector per group row shifts the 4 lane
+// groups of a fragment read onto different banks.
+template <int P, int R, int NTH, int PADV>
 struct Stage {
-  static constexpr int NV = P * G * BT;  // 16-byte vectors per tile
-  static constexpr int PER = NV / NT;    // per thread
+  static constexpr int GS = R + PADV;
+  static constexpr int TV = P * G * GS;  // vectors per tile in LDS (padded)
+  static constexpr int NV = P * G * R;   // 16-byte vectors per tile
+  static constexpr int PER = NV / NTH;   // per thread
+  static_assert(NV % NTH == 0, "tile / thread count mismatch");
   uint4 reg[PER];
   __device__ __forceinline__ void load(const __bf16* planes, int64_t groups, int64_t Rp, int64_t r0, int64_t kt) {
 #pragma unroll
     for (int i = 0; i < PER; ++i) {
-      const int v = threadIdx.x + NT * i;
-      const int p = v / (G * BT), rem = v % (G * BT), gl = rem / BT, r = rem % BT;
+      const int v = threadIdx.x + NTH * i;
+      const int p = v / (G * R), rem = v % (G * R), gl = rem / R, r = rem % R;
       reg[i] = *reinterpret_cast<const uint4*>(planes + (((int64_t)p * groups + kt * G + gl) * Rp + r0 + r) * 8);
     }
   }
   __device__ __forceinline__ void store(uint4* lds) const {
 #pragma unroll
-    for (int i = 0; i < PER; ++i) lds[threadIdx.x + NT * i] = reg[i];
+    for (int i = 0; i < PER; ++i) {
+      const int v = threadIdx.x + NTH * i;
+      lds[(v / R) * GS + v % R] = reg[i];
+    }
   }
 };
 
-template <int P, int TRI>
-__global__ __launch_bounds__(NT) void product_kernel(int64_t M, int64_t N, int64_t Kp, float alpha, const __bf16* Ap,
-                                                     int64_t Mp, const __bf16* Bp, int64_t Np, float* Cp, int64_t ldc,
-                                                     int64_t tiles_m, int64_t tiles_n) {
+// Workgroup tile BM x BN, wave tile WM x WN, LDS row padding PADV (the
+// 256-wide tile drops it to stay within 128 KiB of LDS).
+template <int BM_, int BN_, int WM_, int WN_, int PADV_>
+struct Shape {
+  static constexpr int BM = BM_, BN = BN_, WM = WM_, WN = WN_, PADV = PADV_;
+  static constexpr int WAVES_N = BN / WN;
+  static constexpr int NTH = 64 * (BM / WM) * (BN / WN);
+  static constexpr int SM = WM / 16, SN = WN / 16;
+};
+using Small = Shape<128, 128, 64, 64, 1>;
+using Large = Shape<256, 256, 128, 128, 0>;
+
+template <int P, int TRI, class SH>
+__global__ __launch_bounds__(SH::NTH) void product_kernel(int64_t M, int64_t N, int64_t Kp, float alpha,
+                                                          const __bf16* Ap, int64_t Mp, const __bf16* Bp, int64_t Np,
+                                                          float* Cp, int64_t ldc, int64_t tiles_m, int64_t tiles_n) {
+  using LA = Stage<P, SH::BM, SH::NTH, SH::PADV>;
+  using LB = Stage<P, SH::BN, SH::NTH, SH::PADV>;
+  constexpr int SM = SH::SM, SN = SH::SN;
   extern __shared__ __align__(16) unsigned char smem[];
-  constexpr int TV = P * G * BT;  // vectors per operand tile
   uint4* As = reinterpret_cast<uint4*>(smem);
-  uint4* Bs = As + 2 * TV;
+  uint4* Bs = As + 2 * LA::TV;
   int64_t tm, tn;
   gemm::tile_coords(blockIdx.x, gridDim.x, tiles_m, tiles_n, tm, tn);
-  const int64_t m0 = tm * BT, n0 = tn * BT;
-  if (TRI == gemm::kLower && n0 > m0 + BT - 1) return;
-  if (TRI == gemm::kUpper && n0 + BT - 1 < m0) return;
+  const int64_t m0 = tm * SH::BM, n0 = tn * SH::BN;
+  if (TRI == gemm::kLower && n0 > m0 + SH::BM - 1) return;
+  if (TRI == gemm::kUpper && n0 + SH::BN - 1 < m0) return;
   const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
-  const int wm0 = (wave / 2) * WT, wn0 = (wave % 2) * WT;
+  const int wm0 = (wave / SH::WAVES_N) * SH::WM, wn0 = (wave % SH::WAVES_N) * SH::WN;
   const int lm = lane % 16, lg = lane / 16;
   const int64_t groups = Kp / 8;
   const int64_t nk = Kp / KT;
-  v4f acc[S][S];
+  v4f acc[SM][SN];
 #pragma unroll
-  for (int a = 0; a < S; ++a)
+  for (int a = 0; a < SM; ++a)
 #pragma unroll
-    for (int b = 0; b < S; ++b) acc[a][b] = v4f{0.f, 0.f, 0.f, 0.f};
-  Stage<P> la, lb;
+    for (int b = 0; b < SN; ++b) acc[a][b] = v4f{0.f, 0.f, 0.f, 0.f};
+  LA la;
+  LB lb;
   la.load(Ap, groups, Mp, m0, 0);
   lb.load(Bp, groups, Np, n0, 0);
   la.store(As);
@@ -139,48 +162,49 @@ __global__ __launch_bounds__(NT) void product_kernel(int64_t M, int64_t N, int64
       la.load(Ap, groups, Mp, m0, t + 1);
       lb.load(Bp, groups, Np, n0, t + 1);
     }
-    const v8bf* as = reinterpret_cast<const v8bf*>(As + cur * TV);
-    const v8bf* bs = reinterpret_cast<const v8bf*>(Bs + cur * TV);
-    v8bf a[P][S], b[P][S];
+    const v8bf* as = reinterpret_cast<const v8bf*>(As + cur * LA::TV);
+    const v8bf* bs = reinterpret_cast<const v8bf*>(Bs + cur * LB::TV);
+    v8bf a[P][SM], b[P][SN];
 #pragma unroll
     for (int p = 0; p < P; ++p) {
 #pragma unroll
-      for (int i = 0; i < S; ++i) {
-        a[p][i] = as[(p * G + lg) * BT + wm0 + i * 16 + lm];
-        b[p][i] = bs[(p * G + lg) * BT + wn0 + i * 16 + lm];
-      }
+      for (int i = 0; i < SM; ++i) a[p][i] = as[(p * G + lg) * LA::GS + wm0 + i * 16 + lm];
+#pragma unroll
+      for (int i = 0; i < SN; ++i) b[p][i] = bs[(p * G + lg) * LB::GS + wn0 + i * 16 + lm];
     }
+    // Term-major order: the accumulators are independent, so each term's
+    // MFMAs issue back to back and a tile's next term never waits on the
+    // previous MFMA into the same accumulator.
+    auto term = [&](int pa, int pb) {
 #pragma unroll
-    for (int ta = 0; ta < S; ++ta)
+      for (int ta = 0; ta < SM; ++ta)
 #pragma unroll
-      for (int tb = 0; tb < S; ++tb) {
-        v4f c = acc[ta][tb];
-        if (P == 3) {
-          c = mma(a[2][ta], b[0][tb], c);
-          c = mma(a[0][ta], b[2][tb], c);
-          c = mma(a[1][ta], b[1][tb], c);
-        }
-        c = mma(a[1][ta], b[0][tb], c);
-        c = mma(a[0][ta], b[1][tb], c);
-        c = mma(a[0][ta], b[0][tb], c);
-        acc[ta][tb] = c;
-      }
+        for (int tb = 0; tb < SN; ++tb) acc[ta][tb] = mma(a[pa][ta], b[pb][tb], acc[ta][tb]);
+    };
+    if (P == 3) {
+      term(2 % P, 0);
+      term(0, 2 % P);
+      term(1, 1);
+    }
+    term(1, 0);
+    term(0, 1);
+    term(0, 0);
     if (t + 1 < nk) {
-      la.store(As + (cur ^ 1) * TV);
-      lb.store(Bs + (cur ^ 1) * TV);
+      la.store(As + (cur ^ 1) * LA::TV);
+      lb.store(Bs + (cur ^ 1) * LB::TV);
     }
     __syncthreads();
   }
   // D layout of the FP32-accumulating 16 x 16 MFMA: lane (lm, lg) holds rows
   // 4*lg + q of column lm.
 #pragma unroll
-  for (int ta = 0; ta < S; ++ta)
+  for (int ta = 0; ta < SM; ++ta)
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
       const int64_t row = m0 + wm0 + ta * 16 + 4 * lg + q;
       if (row >= M) continue;
 #pragma unroll
-      for (int tb = 0; tb < S; ++tb) {
+      for (int tb = 0; tb < SN; ++tb) {
         const int64_t col = n0 + wn0 + tb * 16 + lm;
         bool in = col < N;
         if (TRI == gemm::kLower) in = in && col <= row;
@@ -198,20 +222,32 @@ void pack(Ctx& c, const Operand<float>& X, int64_t R, int64_t K, int64_t Rp, int
   EXO_LAUNCH_CHECK();
 }
 
-template <int P, int TRI>
+template <int P, int TRI, class SH>
 void product(Ctx& c, int64_t M, int64_t N, int64_t Kp, float alpha, const __bf16* Ap, int64_t Mp, const __bf16* Bp,
              int64_t Np, float* Cp, int64_t ldc) {
-  const int64_t tm = Mp / BT, tn = Np / BT;
-  const size_t lds = 2 * 2 * (size_t)P * G * BT * 16;
+  const int64_t tm = Mp / SH::BM, tn = Np / SH::BN;
+  const size_t lds =
+      2 * 16 * (size_t)(Stage<P, SH::BM, SH::NTH, SH::PADV>::TV + Stage<P, SH::BN, SH::NTH, SH::PADV>::TV);
   static bool attr_set = false;
   if (!attr_set) {
-    EXO_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&product_kernel<P, TRI>),
+    EXO_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&product_kernel<P, TRI, SH>),
                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     attr_set = true;
   }
-  hipLaunchKernelGGL((product_kernel<P, TRI>), dim3((unsigned)(tm * tn)), dim3(NT), lds, c.stream, M, N, Kp, alpha, Ap,
-                     Mp, Bp, Np, Cp, ldc, tm, tn);
+  hipLaunchKernelGGL((product_kernel<P, TRI, SH>), dim3((unsigned)(tm * tn)), dim3(SH::NTH), lds, c.stream, M, N, Kp,
+                     alpha, Ap, Mp, Bp, Np, Cp, ldc, tm, tn);
   EXO_LAUNCH_CHECK();
+}
+
+template <int P, class SH>
+void product_tri(Ctx& c, int tri, int64_t M, int64_t N, int64_t Kp, float alpha, const __bf16* Ap, int64_t Mp,
+                 const __bf16* Bp, int64_t Np, float* Cp, int64_t ldc) {
+  if (tri == gemm::kLower)
+    product<P, gemm::kLower, SH>(c, M, N, Kp, alpha, Ap, Mp, Bp, Np, Cp, ldc);
+  else if (tri == gemm::kUpper)
+    product<P, gemm::kUpper, SH>(c, M, N, Kp, alpha, Ap, Mp, Bp, Np, Cp, ldc);
+  else
+    product<P, gemm::kFull, SH>(c, M, N, Kp, alpha, Ap, Mp, Bp, Np, Cp, ldc);
 }
 
 // C (tri part) += alpha * X_A X_B^T, X_A M x K, X_B N x K; same_operand when
@@ -219,19 +255,23 @@ void product(Ctx& c, int64_t M, int64_t N, int64_t Kp, float alpha, const __bf16
 template <int P>
 void run_p(Ctx& c, int tri, int64_t M, int64_t N, int64_t K, float alpha, const Operand<float>& A,
            const Operand<float>& B, bool same_operand, float* Cp, int64_t ldc) {
-  const int64_t Mp = pad(M, BT), Np = pad(N, BT), Kp = pad(K, KT);
+  // 256 x 256 tiles (half the operand traffic per flop) while they still give
+  // two workgroups per CU, else 128 x 128; three pieces always use 128 x 128
+  // (their 256-wide tile would not fit in LDS).
+  const bool large = P == 2 && (ceil_div(M, 256) * ceil_div(N, 256)) >= 2 * (int64_t)c.num_cus;
+  const int64_t T = large ? 256 : 128;
+  const int64_t Mp = pad(M, T), Np = pad(N, T), Kp = pad(K, KT);
   const size_t a_elems = (size_t)P * Kp * Mp, b_elems = same_operand ? 0 : (size_t)P * Kp * Np;
   __bf16* ws = static_cast<__bf16*>(c.scratch((a_elems + b_elems) * sizeof(__bf16), 5));
   __bf16* Ap = ws;
   __bf16* Bp = same_operand ? ws : ws + a_elems;
   pack<P>(c, A, M, K, Mp, Kp, Ap);
   if (!same_operand) pack<P>(c, B, N, K, Np, Kp, Bp);
-  if (tri == gemm::kLower)
-    product<P, gemm::kLower>(c, M, N, Kp, alpha, Ap, Mp, Bp, same_operand ? Mp : Np, Cp, ldc);
-  else if (tri == gemm::kUpper)
-    product<P, gemm::kUpper>(c, M, N, Kp, alpha, Ap, Mp, Bp, same_operand ? Mp : Np, Cp, ldc);
+  const int64_t Bn = same_operand ? Mp : Np;
+  if (large)
+    product_tri<P, Large>(c, tri, M, N, Kp, alpha, Ap, Mp, Bp, Bn, Cp, ldc);
   else
-    product<P, gemm::kFull>(c, M, N, Kp, alpha, Ap, Mp, Bp, same_operand ? Mp : Np, Cp, ldc);
+    product_tri<P, Small>(c, tri, M, N, Kp, alpha, Ap, Mp, Bp, Bn, Cp, ldc);
 }
 
 void run(Ctx& c, int tri, int64_t terms, int64_t M, int64_t N, int64_t K, float alpha, const Operand<float>& A,

@@ -17,7 +17,8 @@ constexpr int64_t kUnitV = 132;
 using trsv::kB;
 using trsv::kNT;
 
-// flags[0]: ticket, flags[1]: timeout (epoch on timeout), flags[2 + b]: block b done
+// flags[0]: ticket, flags[1]: timeout (epoch on timeout), flags[2 ..]: the
+// publication words of trsv_core.hpp (8-byte aligned)
 constexpr int kFlagBase = 2;
 
 template <class T, bool FWD, bool TRANS, bool UNIT>
@@ -40,7 +41,7 @@ void launch_trsv(Ctx& c, bool unit, int64_t n, const T* A, int64_t lda, T* x) {
   const int64_t nb = ceil_div(n, kB);
   if (nb > 0x7fffffff) throw Error(kInvalidSize, "trsv: n too large");
   unsigned epoch = 0;
-  unsigned* flags = c.epoch_flags((size_t)nb + kFlagBase, &epoch);
+  unsigned* flags = c.epoch_flags((size_t)trsv::pub_words<T>(n) + kFlagBase, &epoch);
   if (unit)
     hipLaunchKernelGGL((trsv_kernel<T, FWD, TRANS, true>), dim3((unsigned)nb), dim3(kNT), 0, c.stream, n, A, lda, x,
                        flags, epoch);

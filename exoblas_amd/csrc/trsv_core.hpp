@@ -10,11 +10,15 @@
 //      and inverts it in LDS, before waiting on anything, so the inversion
 //      overlaps the earlier steps of the chain;
 //   2. streams the blocks of its row panel on the solved side of the
-//      diagonal, each after the step that owns it has published its part of x
-//      (a release store of `marker` into done[block], polled with relaxed
-//      loads and one acquire fence);
+//      diagonal, each after the step that owns it has published its part of x;
 //   3. forms v = b - (panel . x) and x_block = inv(D) v with all its threads,
 //      so the serial part of the chain is one small mat-vec per block.
+// Publication: an FP32 value travels with the call's marker in one 64-bit
+// word (marker << 32 | bits), stored and polled with relaxed device-scope
+// atomics, so a reader that sees the marker has the value and no fence or
+// separate flag is needed (`pub` holds one word per row).  FP64 values do
+// not fit beside a marker: the block is stored, fenced, and a flag word per
+// block is released (`pub` holds one 32-bit flag per block).
 // Waits are bounded: past the spin limit the step records `marker` in *err
 // and finishes.
 #pragma once
@@ -30,20 +34,6 @@ constexpr int kB = 64;    // rows per block
 constexpr int kNT = 256;  // threads per workgroup
 constexpr long long kSpinLimit = 1ll << 22;
 
-// TI: precision of the diagonal-block inverse and of the block mat-vec.
-template <class T, class TI = T>
-struct Smem {
-  TI inv[kB][kB + 1];
-  T red[kNT / kB][kB];
-  T xs[kB];
-  TI vs[kB];
-};
-
-template <class T, bool TRANS>
-__device__ __forceinline__ T mat(const T* A, int64_t lda, int64_t i, int64_t j) {
-  return TRANS ? A[j * lda + i] : A[i * lda + j];
-}
-
 __device__ __forceinline__ void wait_marker(const unsigned* f, unsigned marker, unsigned* err) {
   long long it = 0;
   while (__hip_atomic_load(f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != marker) {
@@ -55,6 +45,77 @@ __device__ __forceinline__ void wait_marker(const unsigned* f, unsigned marker, 
   }
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
 }
+
+// TI: precision of the diagonal-block inverse and of the block mat-vec.
+template <class T, class TI = T>
+struct Smem {
+  TI inv[kB][kB + 1];
+  T red[kNT / kB][kB];
+  T xs[2][kB];
+  TI vs[kB];
+};
+
+// Words of publication space a solve of order n needs (see above).
+template <class T>
+__host__ __device__ inline int64_t pub_words(int64_t n) {
+  return sizeof(T) == 4 ? 2 * n : (n + kB - 1) / kB;
+}
+
+// Wave 0 of the workgroup fetches the published x block bj into xs.
+template <class T>
+__device__ __forceinline__ void fetch_block(int64_t n, int64_t bj, unsigned* pub, unsigned marker, unsigned* err,
+                                            T* xs, const T* x) {
+  const int t = threadIdx.x;
+  if constexpr (sizeof(T) == 4) {
+    if (t < kB) {
+      const int64_t g = bj * kB + t;
+      T v = T(0);
+      if (g < n) {
+        uint64_t* w64 = reinterpret_cast<uint64_t*>(pub) + g;
+        uint64_t w;
+        long long it = 0;
+        while (((w = __hip_atomic_load(w64, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) >> 32) != marker) {
+          if (++it > kSpinLimit) {
+            __hip_atomic_store(err, marker, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            break;
+          }
+          __builtin_amdgcn_s_sleep(1);
+        }
+        v = __uint_as_float((unsigned)w);
+      }
+      xs[t] = v;
+    }
+  } else {
+    if (t == 0) wait_marker(pub + bj, marker, err);
+    __syncthreads();
+    if (t < kB) xs[t] = (bj * kB + t < n) ? x[bj * kB + t] : T(0);
+  }
+}
+
+// Publishes x[g] = v (called by the thread that owns row g).
+template <class T>
+__device__ __forceinline__ void publish_value(int64_t g, T v, T* x, unsigned* pub, unsigned marker) {
+  x[g] = v;
+  if constexpr (sizeof(T) == 4)
+    __hip_atomic_store(reinterpret_cast<uint64_t*>(pub) + g, ((uint64_t)marker << 32) | __float_as_uint(v),
+                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// After every thread has published its values: the FP64 block flag.
+template <class T>
+__device__ __forceinline__ void publish_block(int64_t bi, unsigned* pub, unsigned marker) {
+  if constexpr (sizeof(T) != 4) {
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) __hip_atomic_store(pub + bi, marker, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
+template <class T, bool TRANS>
+__device__ __forceinline__ T mat(const T* A, int64_t lda, int64_t i, int64_t j) {
+  return TRANS ? A[j * lda + i] : A[i * lda + j];
+}
+
 
 // Natural row of local index i of block bi (local order = processing order).
 template <bool FWD>
@@ -100,7 +161,7 @@ __device__ __forceinline__ void invert_diag(int64_t n, const T* A, int64_t lda, 
 
 // One processing step p of the solve; called by all kNT threads of a workgroup.
 template <class T, bool FWD, bool TRANS, bool UNIT>
-__device__ __forceinline__ void step(int p, int64_t n, const T* __restrict__ A, int64_t lda, T* x, unsigned* done,
+__device__ __forceinline__ void step(int p, int64_t n, const T* __restrict__ A, int64_t lda, T* x, unsigned* pub,
                                      unsigned marker, unsigned* err, Smem<T>& sm) {
   const int nb = (int)((n + kB - 1) / kB);
   const int t = threadIdx.x;
@@ -123,12 +184,11 @@ __device__ __forceinline__ void step(int p, int64_t n, const T* __restrict__ A, 
       const int64_t gj = c0 + e;
       m[e] = (gi < n && gj < n) ? mat<T, TRANS>(A, lda, gi, gj) : T(0);
     }
-    if (t == 0) wait_marker(done + bj, marker, err);
-    __syncthreads();
-    if (t < kB) sm.xs[t] = (bj * kB + t < n) ? x[bj * kB + t] : T(0);
+    T* xs = sm.xs[q & 1];  // double-buffered: one barrier per block
+    fetch_block<T>(n, bj, pub, marker, err, xs, x);
     __syncthreads();
 #pragma unroll
-    for (int e = 0; e < 16; ++e) acc += m[e] * sm.xs[cq * 16 + e];
+    for (int e = 0; e < 16; ++e) acc += m[e] * xs[cq * 16 + e];
   }
   sm.red[cq][r] = acc;
   __syncthreads();
@@ -153,11 +213,9 @@ __device__ __forceinline__ void step(int p, int64_t n, const T* __restrict__ A, 
     s += __shfl_xor(s, 1, 64);
     s += __shfl_xor(s, 2, 64);
     const int64_t g = row_of<FWD>(r0, i);
-    if (q4 == 0 && g < n) x[g] = s;
+    if (q4 == 0 && g < n) publish_value<T>(g, s, x, pub, marker);
   }
-  __threadfence();
-  __syncthreads();
-  if (t == 0) __hip_atomic_store(done + bi, marker, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+  publish_block<T>(bi, pub, marker);
   __syncthreads();  // the shared buffers are reused by the caller's next step
 }
 

@@ -62,12 +62,11 @@ __global__ __launch_bounds__(kNT) void trsv_mixed_kernel(int64_t n, const float*
       const int64_t gj = c0 + e;
       m[e] = (gi < n && gj < n) ? to_f(TRANS ? Aoff[gj * ldo + gi] : Aoff[gi * ldo + gj]) : 0.f;
     }
-    if (t == 0) trsv::wait_marker(done + bj, epoch, flags + 1);
-    __syncthreads();
-    if (t < kB) sm.xs[t] = (bj * kB + t < n) ? xf[bj * kB + t] : 0.f;
+    float* xs = sm.xs[q & 1];
+    trsv::fetch_block<float>(n, bj, done, epoch, flags + 1, xs, xf);
     __syncthreads();
 #pragma unroll
-    for (int e = 0; e < 16; ++e) acc += m[e] * sm.xs[cq * 16 + e];
+    for (int e = 0; e < 16; ++e) acc += m[e] * xs[cq * 16 + e];
   }
   sm.red[cq][r] = acc;
   __syncthreads();
@@ -93,12 +92,9 @@ __global__ __launch_bounds__(kNT) void trsv_mixed_kernel(int64_t n, const float*
     const int64_t g = trsv::row_of<FWD>(r0, i);
     if (q4 == 0 && g < n) {
       x[g] = s;
-      xf[g] = (float)s;
+      trsv::publish_value<float>(g, (float)s, xf, done, epoch);
     }
   }
-  __threadfence();
-  __syncthreads();
-  if (t == 0) __hip_atomic_store(done + bi, epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // r[i] = b[i] - sum_j M(i, j) x[j] over the triangle (FP64 accumulation), one
@@ -147,7 +143,7 @@ void solve(Ctx& c, bool unit, int64_t n, const float* A, int64_t lda, const TO* 
            const double* rhs, double* x, float* xf) {
   const int64_t nb = ceil_div(n, kB);
   unsigned epoch = 0;
-  unsigned* flags = c.epoch_flags((size_t)nb + kFlagBase, &epoch);
+  unsigned* flags = c.epoch_flags((size_t)trsv::pub_words<float>(n) + kFlagBase, &epoch);
   if (unit)
     hipLaunchKernelGGL((trsv_mixed_kernel<TO, FWD, TRANS, true>), dim3((unsigned)nb), dim3(kNT), 0, c.stream, n, A,
                        lda, Aoff, ldo, rhs, x, xf, flags, epoch);

@@ -256,3 +256,7 @@ def test_server_budget(env):
     stats, lat = serving.run(w, mode=0, budget=big - 1, streams=2)
     assert stats["rejected"] == int((nbytes > big - 1).sum()) >= 1
     assert stats["completed"] == len(s.kind) - stats["rejected"]
+    # the rejection is reported through the thread's sticky status
+    from exoblas_amd import lib
+    assert lib.last_status() == 6 and "budget" in lib.last_error()
+    lib.clear_error()
