@@ -32,6 +32,24 @@ __device__ __forceinline__ const T& vget(const typename Vec16<T>::type& v, int i
   return reinterpret_cast<const T*>(&v)[i];
 }
 
+// Streaming 16-byte access (global_load/store_dwordx4 with the non-temporal
+// cache policy) for data a call touches once: it keeps the lines from
+// displacing reused ones in L2 and MALL, and measured on MI355X it moves
+// read streams from 6.45 to 7.15 TB/s and axpy from 5.8 to 6.65 TB/s.
+template <class T> struct Native16;
+template <> struct Native16<float> { typedef float type __attribute__((ext_vector_type(4))); };
+template <> struct Native16<double> { typedef double type __attribute__((ext_vector_type(2))); };
+template <class T>
+__device__ __forceinline__ typename Vec16<T>::type ld_stream(const T* p) {
+  using N = typename Native16<T>::type;
+  return __builtin_bit_cast(typename Vec16<T>::type, __builtin_nontemporal_load(reinterpret_cast<const N*>(p)));
+}
+template <class T>
+__device__ __forceinline__ void st_stream(T* p, const typename Vec16<T>::type& v) {
+  using N = typename Native16<T>::type;
+  __builtin_nontemporal_store(__builtin_bit_cast(N, v), reinterpret_cast<N*>(p));
+}
+
 template <class T>
 __device__ __forceinline__ T wave_sum(T v) {
 #pragma unroll

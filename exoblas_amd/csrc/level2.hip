@@ -61,21 +61,14 @@ __global__ __launch_bounds__(kNT) void gemv_n_kernel(int64_t M, int64_t N, T alp
     __syncthreads();
     if (VEC) {
       const int nv = pn / VN;
-      for (int v = lane; v < nv; v += kWave) {
-        const V xv = reinterpret_cast<const V*>(xs)[v];
-        V av[R];
-#pragma unroll
-        for (int r = 0; r < R; ++r) {
-          const int64_t row = row0 + r;
-          if (row < M) av[r] = *reinterpret_cast<const V*>(A + row * lda + p0 + (int64_t)v * VN);
-        }
+      auto accum = [&](const V (&av)[R], const V& xv, int64_t cb) {
 #pragma unroll
         for (int r = 0; r < R; ++r) {
           const int64_t row = row0 + r;
           if (row < M) {
 #pragma unroll
             for (int k = 0; k < VN; ++k) {
-              const int64_t col = p0 + (int64_t)v * VN + k;
+              const int64_t col = cb + k;
               bool in = true;
               if (TRI == kLowerTri) in = UNIT ? col < row : col <= row;
               if (TRI == kUpperTri) in = UNIT ? col > row : col >= row;
@@ -83,6 +76,31 @@ __global__ __launch_bounds__(kNT) void gemv_n_kernel(int64_t M, int64_t N, T alp
             }
           }
         }
+      };
+      // UV vectors per lane per round: R * UV = 8 streaming loads in flight
+      constexpr int UV = R >= 8 ? 1 : 8 / R;
+      int v = lane;
+      for (; v + (UV - 1) * kWave < nv; v += UV * kWave) {
+        V av[UV][R];
+#pragma unroll
+        for (int u = 0; u < UV; ++u)
+#pragma unroll
+          for (int r = 0; r < R; ++r) {
+            const int64_t row = row0 + r;
+            if (row < M) av[u][r] = ld_stream<T>(A + row * lda + p0 + (int64_t)(v + u * kWave) * VN);
+          }
+#pragma unroll
+        for (int u = 0; u < UV; ++u)
+          accum(av[u], reinterpret_cast<const V*>(xs)[v + u * kWave], p0 + (int64_t)(v + u * kWave) * VN);
+      }
+      for (; v < nv; v += kWave) {
+        V av[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          const int64_t row = row0 + r;
+          if (row < M) av[r] = ld_stream<T>(A + row * lda + p0 + (int64_t)v * VN);
+        }
+        accum(av, reinterpret_cast<const V*>(xs)[v], p0 + (int64_t)v * VN);
       }
       for (int i = nv * VN + lane; i < pn; i += kWave) {
 #pragma unroll
@@ -183,7 +201,7 @@ __global__ __launch_bounds__(kNT) void gemv_t_kernel(int64_t Mc, int64_t Nr, T a
   auto body = [&](int64_t r) {
     const T xr = x[r * incx];
     if (VEC && c0 + VN <= Mc) {
-      const V av = *reinterpret_cast<const V*>(A + r * lda + c0);
+      const V av = ld_stream<T>(A + r * lda + c0);
 #pragma unroll
       for (int k = 0; k < VN; ++k) {
         const int64_t c = c0 + k;
@@ -234,23 +252,31 @@ template <class T, int TRI, bool UNIT>
 void launch_gemv_n(Ctx& c, int64_t M, int64_t N, T alpha, T beta, const T* A, int64_t lda, const T* x,
                    int64_t incx, T* y, int64_t incy) {
   constexpr int VN = Vec16<T>::n;
-  constexpr int R = 8;
   constexpr int P = 2048;
   const bool vec = (lda % VN) == 0 && aligned16(A);
+  const int64_t want = (int64_t)c.num_cus * 4;
+  // Tall matrices: 2 rows per wave over the whole row, one launch.  Shorter
+  // ones: 8 rows per wave and the columns split across workgroups, summed by
+  // a second launch.
+  const bool tall = ceil_div(M, (int64_t)(kNT / kWave) * 2) >= want;
+  const int R = tall ? 2 : 8;
   const int64_t row_blocks = ceil_div(M, (int64_t)(kNT / kWave) * R);
   int64_t splits = 1;
-  const int64_t want = (int64_t)c.num_cus * 4;
-  if (row_blocks < want && N > 2 * P) splits = std::min<int64_t>(ceil_div(want, row_blocks), ceil_div(N, P));
+  if (!tall && row_blocks < want && N > 2 * P)
+    splits = std::min<int64_t>(ceil_div(want, row_blocks), ceil_div(N, P));
   int64_t cols_per = N > 0 ? ceil_div(ceil_div(N, splits), P) * P : P;
   splits = N > 0 ? ceil_div(N, cols_per) : 1;
   T* part = splits > 1 ? static_cast<T*>(c.scratch(sizeof(T) * splits * M)) : nullptr;
   dim3 grid((unsigned)row_blocks, (unsigned)splits);
-  if (vec)
-    hipLaunchKernelGGL((gemv_n_kernel<T, R, P, true, TRI, UNIT>), grid, dim3(kNT), 0, c.stream, M, N, alpha,
-                       beta, A, lda, x, incx, y, incy, cols_per, part);
-  else
-    hipLaunchKernelGGL((gemv_n_kernel<T, R, P, false, TRI, UNIT>), grid, dim3(kNT), 0, c.stream, M, N, alpha,
-                       beta, A, lda, x, incx, y, incy, cols_per, part);
+#define EXO_GEMV_N(RR, V)                                                                                   \
+  hipLaunchKernelGGL((gemv_n_kernel<T, RR, P, V, TRI, UNIT>), grid, dim3(kNT), 0, c.stream, M, N, alpha, beta, \
+                     A, lda, x, incx, y, incy, cols_per, part)
+  if (tall) {
+    if (vec) EXO_GEMV_N(2, true); else EXO_GEMV_N(2, false);
+  } else {
+    if (vec) EXO_GEMV_N(8, true); else EXO_GEMV_N(8, false);
+  }
+#undef EXO_GEMV_N
   EXO_LAUNCH_CHECK();
   if (splits > 1) {
     hipLaunchKernelGGL((gemv_finish_kernel<T, TRI != kFull, UNIT>), dim3((unsigned)ceil_div(M, kWave)), dim3(kNT), 0,
@@ -318,7 +344,7 @@ __global__ __launch_bounds__(kNT) void rank_update_kernel(int64_t m, int64_t n, 
     const T r2 = TWO ? alpha * u2[i * inc_u2] : T(0);
     T* row = A + i * lda;
     if (VEC && j0 >= jl && j0 + VN <= jh) {
-      V a = *reinterpret_cast<V*>(row + j0);
+      V a = ld_stream<T>(row + j0);
 #pragma unroll
       for (int k = 0; k < VN; ++k) {
         if (TWO)
@@ -326,7 +352,7 @@ __global__ __launch_bounds__(kNT) void rank_update_kernel(int64_t m, int64_t n, 
         else
           vget<T>(a, k) += r1 * cv1[k];
       }
-      *reinterpret_cast<V*>(row + j0) = a;
+      st_stream<T>(row + j0, a);
     } else {
 #pragma unroll
       for (int k = 0; k < VN; ++k) {

@@ -16,8 +16,8 @@
 //     contiguous bytes;
 //  2. product: 256 x 256 workgroup tiles of 4 waves (128 x 128 each, 8 x 8
 //     MFMA tiles of 16 x 16) when there are enough of them, else 128 x 128
-//     (waves of 64 x 64); K step 32, planes copied global -> LDS with 16-byte
-//     loads and stores (double-buffered), one 16-byte LDS read per fragment,
+//     (waves of 64 x 64); K step 32, planes copied global -> LDS by the
+//     load-to-LDS path (double-buffered), one 16-byte LDS read per fragment,
 //     no bounds checks inside the K loop.  The BF16 MFMA runs 16x the FP32
 //     MFMA rate, so the big tile is what keeps operand traffic below the L2
 //     bandwidth.
@@ -86,31 +86,35 @@ __device__ __forceinline__ v4f mma(v8bf a, v8bf b, v4f c) {
 
 // LDS tile of one operand: [P][G][R + PADV] 16-byte fragments (8 BF16
 // each); with PADV = 1 the extra vector per group row shifts the 4 lane
-// groups of a fragment read onto different banks.
+// groups of a fragment read onto different banks.  The packed planes hold
+// the R rows of one (plane, k group) contiguously, so the tile is copied
+// global -> LDS directly (global_load_lds_dwordx4: each lane moves 16 bytes
+// and a wave fills 1 KiB of contiguous LDS), without passing through VGPRs.
 template <int P, int R, int NTH, int PADV>
 struct Stage {
   static constexpr int GS = R + PADV;
-  static constexpr int TV = P * G * GS;  // vectors per tile in LDS (padded)
-  static constexpr int NV = P * G * R;   // 16-byte vectors per tile
-  static constexpr int PER = NV / NTH;   // per thread
-  static_assert(NV % NTH == 0, "tile / thread count mismatch");
-  uint4 reg[PER];
-  __device__ __forceinline__ void load(const __bf16* planes, int64_t groups, int64_t Rp, int64_t r0, int64_t kt) {
+  static constexpr int TV = P * G * GS;     // vectors per tile in LDS (padded)
+  static constexpr int CH = P * G * R / 64;  // 64-row chunks: one wave instruction each
+  static constexpr int NW = NTH / 64;
+  static_assert(R % 64 == 0 && CH % NW == 0, "tile / wave count mismatch");
+  // Issues the copy of K tile kt into lds; the caller waits (vmcnt) and
+  // synchronises the workgroup before reading it.
+  __device__ __forceinline__ static void copy(const __bf16* planes, int64_t groups, int64_t Rp, int64_t r0,
+                                              int64_t kt, uint4* lds) {
+    const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
 #pragma unroll
-    for (int i = 0; i < PER; ++i) {
-      const int v = threadIdx.x + NTH * i;
-      const int p = v / (G * R), rem = v % (G * R), gl = rem / R, r = rem % R;
-      reg[i] = *reinterpret_cast<const uint4*>(planes + (((int64_t)p * groups + kt * G + gl) * Rp + r0 + r) * 8);
-    }
-  }
-  __device__ __forceinline__ void store(uint4* lds) const {
-#pragma unroll
-    for (int i = 0; i < PER; ++i) {
-      const int v = threadIdx.x + NTH * i;
-      lds[(v / R) * GS + v % R] = reg[i];
+    for (int i = 0; i < CH / NW; ++i) {
+      const int c = wave + NW * i;
+      const int pg = c / (R / 64), rr = c % (R / 64);
+      const int p = pg / G, gl = pg % G;
+      const __bf16* src = planes + (((int64_t)p * groups + kt * G + gl) * Rp + r0 + rr * 64 + lane) * 8;
+      __builtin_amdgcn_global_load_lds(src, (__attribute__((address_space(3))) void*)(lds + pg * GS + rr * 64), 16,
+                                       0, 0);
     }
   }
 };
+
+__device__ __forceinline__ void wait_copies() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
 
 // Workgroup tile BM x BN, wave tile WM x WN, LDS row padding PADV (the
 // 256-wide tile drops it to stay within 128 KiB of LDS).
@@ -149,19 +153,12 @@ __global__ __launch_bounds__(SH::NTH) void product_kernel(int64_t M, int64_t N, 
   for (int a = 0; a < SM; ++a)
 #pragma unroll
     for (int b = 0; b < SN; ++b) acc[a][b] = v4f{0.f, 0.f, 0.f, 0.f};
-  LA la;
-  LB lb;
-  la.load(Ap, groups, Mp, m0, 0);
-  lb.load(Bp, groups, Np, n0, 0);
-  la.store(As);
-  lb.store(Bs);
+  LA::copy(Ap, groups, Mp, m0, 0, As);
+  LB::copy(Bp, groups, Np, n0, 0, Bs);
+  wait_copies();
   __syncthreads();
   for (int64_t t = 0; t < nk; ++t) {
     const int cur = (int)(t & 1);
-    if (t + 1 < nk) {
-      la.load(Ap, groups, Mp, m0, t + 1);
-      lb.load(Bp, groups, Np, n0, t + 1);
-    }
     const v8bf* as = reinterpret_cast<const v8bf*>(As + cur * LA::TV);
     const v8bf* bs = reinterpret_cast<const v8bf*>(Bs + cur * LB::TV);
     v8bf a[P][SM], b[P][SN];
@@ -171,6 +168,13 @@ __global__ __launch_bounds__(SH::NTH) void product_kernel(int64_t M, int64_t N, 
       for (int i = 0; i < SM; ++i) a[p][i] = as[(p * G + lg) * LA::GS + wm0 + i * 16 + lm];
 #pragma unroll
       for (int i = 0; i < SN; ++i) b[p][i] = bs[(p * G + lg) * LB::GS + wn0 + i * 16 + lm];
+    }
+    // The other buffer was last read in step t - 1, before the barrier that
+    // ended it: the copy of tile t + 1 overlaps this step's MFMAs.  It is
+    // issued after this step's fragment reads so those never wait on it.
+    if (t + 1 < nk) {
+      LA::copy(Ap, groups, Mp, m0, t + 1, As + (cur ^ 1) * LA::TV);
+      LB::copy(Bp, groups, Np, n0, t + 1, Bs + (cur ^ 1) * LB::TV);
     }
     // Term-major order: the accumulators are independent, so each term's
     // MFMAs issue back to back and a tile's next term never waits on the
@@ -189,10 +193,7 @@ __global__ __launch_bounds__(SH::NTH) void product_kernel(int64_t M, int64_t N, 
     term(1, 0);
     term(0, 1);
     term(0, 0);
-    if (t + 1 < nk) {
-      la.store(As + (cur ^ 1) * LA::TV);
-      lb.store(Bs + (cur ^ 1) * LB::TV);
-    }
+    wait_copies();
     __syncthreads();
   }
   // D layout of the FP32-accumulating 16 x 16 MFMA: lane (lm, lg) holds rows

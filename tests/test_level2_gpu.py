@@ -150,6 +150,12 @@ def test_trmv(C, dt, uplo, trans, diag, n, incx, ld_diff, rng):
     assert_bound(x, xo, mag, n, dt, what="trmv")
 
 
+@pytest.mark.parametrize("uplo,diag", [(UPPER, UNIT), (LOWER, NON_UNIT)])
+def test_trmv_tall(C, uplo, diag, rng):
+    """n = 8200: the one-launch, two-rows-per-wave row path of the triangular mat-vec."""
+    test_trmv(C, np.float32, uplo, NO_TRANS, diag, 8200, 1, 3, rng)
+
+
 def dd_matrix(rng, n, lda, dt):
     """Diagonally dominant: off-diagonal U[-1,1]/n, diagonal n + U[0,1]."""
     A = (rng.uniform(-1, 1, (n, lda)) / n).astype(dt)
