@@ -351,9 +351,12 @@ def bench_trsv(a, torch, so, L):
 
 
 def bench_precision(a, torch, so, L):
-    """sgemm NN 8192^3: plain FP32 MFMA vs split BF16 (3 and 6 products);
-    max error against float64 on U[-1,1] and on rows scaled by 2^U[-8,8],
-    as max |C - C64| / (|A| |B|) (normwise per element) and max relative."""
+    """sgemm NN 8192^3 and ssyrk L-N 8192 x 2048: plain FP32 MFMA vs split
+    BF16 (3 and 6 products); max error against float64 on U[-1,1] and on rows
+    scaled by 2^U[-8,8], as max |C - C64| / (|A| |B|) (normwise per element)
+    and max relative.  strsv L-N-N n = 8192: plain FP32 solve vs the mixed
+    solve (BF16 / FP32 updates, FP64 diagonal blocks, one refinement step),
+    error against the FP64 solution."""
     stream = torch.cuda.current_stream()
     sh = stream.cuda_stream
     g = torch.Generator(device="cuda")
@@ -391,13 +394,73 @@ def bench_precision(a, torch, so, L):
                                      "peak_equiv": round(peak, 1)})
         del truth, mag, A, B, C
         torch.cuda.empty_cache()
+    # ssyrk lower-N, N = 8192, K = 2048: plain vs split, error on the stored triangle
+    N, K = a.n_gemm, a.n_gemm // 4
+    for scaled in (False, True):
+        A = torch.empty((N, K), device="cuda").uniform_(-1, 1, generator=g)
+        if scaled:
+            A *= torch.exp2(torch.empty((N, 1), device="cuda").uniform_(-8, 8, generator=g))
+        A64 = A.double()
+        tri = torch.ones((N, N), dtype=torch.bool, device="cuda").tril_()
+        truth = A64 @ A64.T
+        mag = A64.abs() @ A64.abs().T
+        C = torch.zeros((N, N), device="cuda")
+        wa, wc = L.win2(A.data_ptr(), K), L.win2(C.data_ptr(), N)
+        forms = {"plain": lambda: so.exoblas_ssyrk_rm_stride_1_async(sh, 122, 111, N, K, one, wa, wa, wa, wa, wc),
+                 "split3": lambda: so.exoblas_ssyrk_split_async(sh, 122, 111, N, K, one, wa, wa, wc, 3),
+                 "split6": lambda: so.exoblas_ssyrk_split_async(sh, 122, 111, N, K, one, wa, wa, wc, 6)}
+        for name, fn in forms.items():
+            C.zero_()
+            fn()
+            torch.cuda.synchronize()
+            err = (C.double() - truth).abs()
+            nw = (err / mag)[tri].max().item()
+            key = f"ssyrk_{name}_{'scaled' if scaled else 'uniform'}"
+            kernels[key] = {"max_err_normwise": nw}
+            if not scaled:
+                ms = _timed(torch, stream, fn, 3)
+                flop = float(K) * N * (N + 1)
+                tf = flop / (ms * 1e-3) / 1e12
+                peak = FP32_PEAK_TF if name == "plain" else BF16_PEAK_TF / (3 if name == "split3" else 6)
+                kernels[key].update({"ms": round(ms, 3), "TFLOP/s": round(tf, 2), "frac": round(tf / peak, 4),
+                                     "peak_equiv": round(peak, 1)})
+        del A, A64, tri, truth, mag, C
+        torch.cuda.empty_cache()
+    # strsv lower-N-nonunit, n = 8192 (diagonally dominant, as the trsv set):
+    # plain FP32 solve vs BF16 / FP32 off-diagonal updates + FP64 diagonal
+    # blocks + one refinement step; error against the FP64 solution.
+    n = a.n_gemm
+    A = torch.empty((n, n), device="cuda").uniform_(-1, 1, generator=g) / n
+    A.diagonal().copy_(n + torch.empty(n, device="cuda").uniform_(0, 1, generator=g))
+    b = torch.empty(n, device="cuda").uniform_(-1, 1, generator=g)
+    x64 = torch.linalg.solve_triangular(A.double().tril(), b.double()[:, None], upper=False)[:, 0]
+    x = b.clone()
+    wA = L.win2(A.data_ptr(), n)
+    forms = {"plain": lambda: so.exoblas_strsv_rm_stride_1_async(sh, 122, 111, 131, n, L.win1(x.data_ptr(), 1), wA),
+             "mixed_bf16": lambda: so.exoblas_strsv_mixed_async(sh, 122, 111, 131, n, L.win1(x.data_ptr(), 1), wA,
+                                                                16, 1),
+             "mixed_fp32": lambda: so.exoblas_strsv_mixed_async(sh, 122, 111, 131, n, L.win1(x.data_ptr(), 1), wA,
+                                                                32, 1)}
+    for name, fn in forms.items():
+        x.copy_(b)
+        fn()
+        torch.cuda.synchronize()
+        err = (x.double() - x64).abs()
+        ms = _timed(torch, stream, fn, 10, before=lambda: x.copy_(b))
+        nbytes = (n * (n + 1) // 2 + 2 * n) * 4
+        kernels[f"strsv_{name}_{n}"] = {"max_err_normwise": (err.max() / x64.abs().max()).item(),
+                                         "max_rel_err": (err / x64.abs().clamp_min(1e-300)).max().item(),
+                                         "us": round(ms * 1e3, 2), "GB/s": round(nbytes / (ms * 1e-3) / 1e9, 1)}
+    del A, b, x, x64
+    torch.cuda.empty_cache()
     sp = kernels["sgemm_plain_uniform"]["ms"] / kernels["sgemm_split3_uniform"]["ms"]
     k3 = kernels["sgemm_split3_uniform"]
     roof = {"bound": "mfma", "kernel": "sgemm_split3", "achieved": k3["TFLOP/s"], "peak": k3["peak_equiv"],
             "unit": "TFLOP/s (FP32-equivalent; BF16 peak / products)", "frac": k3["frac"], "traffic": None}
     return _line("split-BF16 sgemm speedup over plain FP32 MFMA sgemm", round(sp, 3), "x", a,
                  kernels["sgemm_split3_uniform"]["ms"], "f32 (bf16 x3 / x6 products, f32 accumulate)",
-                 f"sgemm NN {n}^3, uniform and row-scaled sets", roof, kernels)
+                 f"sgemm NN {a.n_gemm}^3 and ssyrk L-N {a.n_gemm}x{a.n_gemm // 4}, uniform and row-scaled sets; "
+                 f"strsv L-N-N n={a.n_gemm} plain vs mixed", roof, kernels)
 
 
 def bench_serving(a, torch, so, L):

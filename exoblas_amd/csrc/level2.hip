@@ -141,6 +141,60 @@ __global__ __launch_bounds__(kNT) void gemv_n_kernel(int64_t M, int64_t N, T alp
   }
 }
 
+// Tall matrices with a contiguous, aligned x: one row per wave, x read
+// straight from global memory (it stays in L2) and 8 streaming A vectors per
+// lane in flight; no LDS and no barriers.  (Best of a sweep on MI355X over
+// rows per wave, vectors in flight, block size and LDS-staged x: 6.4 TB/s
+// for 8192 x 8192 FP32.)  Same TRI / UNIT meaning as gemv_n_kernel.
+template <class T, int TRI, bool UNIT>
+__global__ __launch_bounds__(kNT) void gemv_n_row_kernel(int64_t M, int64_t N, T alpha, T beta,
+                                                         const T* __restrict__ A, int64_t lda,
+                                                         const T* __restrict__ x, T* y, int64_t incy) {
+  using V = typename Vec16<T>::type;
+  constexpr int VN = Vec16<T>::n;
+  constexpr int UV = 8;
+  const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+  const int64_t row = (int64_t)blockIdx.x * (kNT / kWave) + wave;
+  if (row >= M) return;
+  int64_t jl = 0, jh = N;  // columns [jl, jh) of this row
+  if (TRI == kLowerTri) jh = UNIT ? row : row + 1;
+  if (TRI == kUpperTri) jl = UNIT ? row + 1 : row;
+  const int64_t vl = ceil_div(jl, VN);
+  const int64_t vh = std::max<int64_t>(jh / VN, vl);  // whole vectors [vl, vh)
+  const T* Ar = A + row * lda;
+  T acc = T(0);
+  int64_t v = vl + lane;
+  for (; v + (UV - 1) * kWave < vh; v += UV * kWave) {
+    V av[UV], xv[UV];
+#pragma unroll
+    for (int u = 0; u < UV; ++u) {
+      av[u] = ld_stream<T>(Ar + (v + u * kWave) * VN);
+      xv[u] = *reinterpret_cast<const V*>(x + (v + u * kWave) * VN);
+    }
+#pragma unroll
+    for (int u = 0; u < UV; ++u)
+#pragma unroll
+      for (int k = 0; k < VN; ++k) acc += vget<T>(av[u], k) * vget<T>(xv[u], k);
+  }
+  for (; v < vh; v += kWave) {
+    const V av = ld_stream<T>(Ar + v * VN);
+    const V xv = *reinterpret_cast<const V*>(x + v * VN);
+#pragma unroll
+    for (int k = 0; k < VN; ++k) acc += vget<T>(av, k) * vget<T>(xv, k);
+  }
+  // the scalar columns before and after the whole vectors
+  const int64_t hend = std::min<int64_t>(vl * VN, jh);
+  for (int64_t j = jl + lane; j < hend; j += kWave) acc += Ar[j] * x[j];
+  for (int64_t j = std::max<int64_t>(vh * VN, hend) + lane; j < jh; j += kWave) acc += Ar[j] * x[j];
+  const T s = wave_sum(acc);
+  if (lane == 0) {
+    if (TRI != kFull)
+      y[row * incy] = UNIT ? s + x[row] : s;
+    else
+      y[row * incy] = beta * y[row * incy] + alpha * s;
+  }
+}
+
 // y[r] = beta*y[r] + alpha * sum_s part[s*M + r]  (or the triangular overwrite).
 // A workgroup finishes 64 rows; its 4 waves take every 4th split each and
 // combine through LDS in a fixed order (deterministic).
@@ -222,11 +276,36 @@ __global__ __launch_bounds__(kNT) void gemv_t_kernel(int64_t Mc, int64_t Nr, T a
     }
   };
   int64_t r = r_beg + wave;
-  for (; r + (RU - 1) * NW < r_end; r += RU * NW) {
+  if (TRI == kFull && VEC && cb1 <= Mc) {
+    // Interior column block of a full matrix: the same loop without the
+    // per-lane edge and triangle tests (which the compiler keeps per row).
+    const T* Ac = A + c0;
+    for (; r + (RU - 1) * NW < r_end; r += RU * NW) {
+      V av[RU];
+      T xr[RU];
 #pragma unroll
-    for (int u = 0; u < RU; ++u) body(r + u * NW);
+      for (int u = 0; u < RU; ++u) {
+        av[u] = ld_stream<T>(Ac + (r + u * NW) * lda);
+        xr[u] = x[(r + u * NW) * incx];
+      }
+#pragma unroll
+      for (int u = 0; u < RU; ++u)
+#pragma unroll
+        for (int k = 0; k < VN; ++k) acc[k] += vget<T>(av[u], k) * xr[u];
+    }
+    for (; r < r_end; r += NW) {
+      const V av = ld_stream<T>(Ac + r * lda);
+      const T xr = x[r * incx];
+#pragma unroll
+      for (int k = 0; k < VN; ++k) acc[k] += vget<T>(av, k) * xr;
+    }
+  } else {
+    for (; r + (RU - 1) * NW < r_end; r += RU * NW) {
+#pragma unroll
+      for (int u = 0; u < RU; ++u) body(r + u * NW);
+    }
+    for (; r < r_end; r += NW) body(r);
   }
-  for (; r < r_end; r += NW) body(r);
 
 #pragma unroll
   for (int k = 0; k < VN; ++k) red[wave][lane * VN + k] = acc[k];
@@ -252,19 +331,26 @@ template <class T, int TRI, bool UNIT>
 void launch_gemv_n(Ctx& c, int64_t M, int64_t N, T alpha, T beta, const T* A, int64_t lda, const T* x,
                    int64_t incx, T* y, int64_t incy) {
   constexpr int VN = Vec16<T>::n;
-  constexpr int P = 2048;
+  constexpr int P = 2048;               // x panel staged in LDS
+  constexpr int64_t SG = 2048;          // column-split granularity
   const bool vec = (lda % VN) == 0 && aligned16(A);
   const int64_t want = (int64_t)c.num_cus * 4;
   // Tall matrices: 2 rows per wave over the whole row, one launch.  Shorter
   // ones: 8 rows per wave and the columns split across workgroups, summed by
   // a second launch.
+  if (vec && incx == 1 && aligned16(x) && ceil_div(M, (int64_t)(kNT / kWave)) >= want) {
+    hipLaunchKernelGGL((gemv_n_row_kernel<T, TRI, UNIT>), dim3((unsigned)ceil_div(M, kNT / kWave)), dim3(kNT), 0,
+                       c.stream, M, N, alpha, beta, A, lda, x, y, incy);
+    EXO_LAUNCH_CHECK();
+    return;
+  }
   const bool tall = ceil_div(M, (int64_t)(kNT / kWave) * 2) >= want;
   const int R = tall ? 2 : 8;
   const int64_t row_blocks = ceil_div(M, (int64_t)(kNT / kWave) * R);
   int64_t splits = 1;
-  if (!tall && row_blocks < want && N > 2 * P)
-    splits = std::min<int64_t>(ceil_div(want, row_blocks), ceil_div(N, P));
-  int64_t cols_per = N > 0 ? ceil_div(ceil_div(N, splits), P) * P : P;
+  if (!tall && row_blocks < want && N > 2 * SG)
+    splits = std::min<int64_t>(ceil_div(want, row_blocks), ceil_div(N, SG));
+  int64_t cols_per = N > 0 ? ceil_div(ceil_div(N, splits), SG) * SG : SG;
   splits = N > 0 ? ceil_div(N, cols_per) : 1;
   T* part = splits > 1 ? static_cast<T*>(c.scratch(sizeof(T) * splits * M)) : nullptr;
   dim3 grid((unsigned)row_blocks, (unsigned)splits);
@@ -292,7 +378,9 @@ void launch_gemv_t(Ctx& c, int64_t Mc, int64_t Nr, T alpha, T beta, const T* A, 
   constexpr int RU = 8;
   const bool vec = (lda % VN) == 0 && aligned16(A);
   const int64_t col_blocks = ceil_div(Mc, (int64_t)kWave * VN);
-  const int64_t want = (int64_t)c.num_cus * 8;
+  // About one workgroup per CU: long row ranges per workgroup and few
+  // partial rows for the second launch (measured best on MI355X).
+  const int64_t want = (int64_t)c.num_cus;
   int64_t splits = std::max<int64_t>(1, std::min<int64_t>(ceil_div(want, col_blocks), ceil_div(Nr, 64)));
   const int64_t rows_per = Nr > 0 ? ceil_div(ceil_div(Nr, splits), 32) * 32 : 32;
   splits = std::max<int64_t>(1, ceil_div(Nr, rows_per));

@@ -56,26 +56,36 @@ void launch_cfg(Ctx& c, int64_t M, int64_t N, int64_t K, typename C::type alpha,
   EXO_LAUNCH_CHECK();
 }
 
-// Tile shapes, largest first (measured on 8192^3 NN: FP32 256 x 256 x 32
-// 123 TFLOP/s vs 114 for 128 x 128 x 16; FP64 64 x 128 x 16 at two waves per
-// SIMD 46 TFLOP/s vs 39 for 128 x 128 x 16 at one).
+// Tile shapes, largest first (measured on 8192^3 NN: FP32 256 x 256 x 32 in
+// 8 waves of 128 x 64 127.6 TFLOP/s, 4 waves of 128 x 128 123.8, 128 x 128 x
+// 16 114; FP64 128 x 256 x 16 in 8 waves of 64 x 64 57.9 TFLOP/s, 64 x 128 x
+// 16 in 4 waves 46.3, 128 x 128 x 16 in 4 waves 39).
 template <class T> struct Tiles;
 template <> struct Tiles<float> {
-  using Huge = Cfg<float, 256, 256, 32, 128, 128>;
+  using Huge = Cfg<float, 256, 256, 32, 128, 64>;
   using Big = Cfg<float, 128, 128, 16, 64, 64>;
   using Small = Cfg<float, 64, 64, 16, 32, 32>;
 };
 template <> struct Tiles<double> {
-  using Huge = Cfg<double, 64, 128, 16, 32, 64>;
+  using Huge = Cfg<double, 128, 256, 16, 64, 64>;
   using Big = Cfg<double, 64, 128, 16, 32, 64>;
   using Small = Cfg<double, 64, 64, 16, 32, 32>;
 };
 
-// Workgroup tiles an M x N product needs at tile size t (about half for a
-// triangular update).
-inline int64_t tile_count(int tri, int64_t M, int64_t N, int64_t t) {
-  const int64_t a = ceil_div(M, t), b = ceil_div(N, t);
-  return tri == kFull ? a * b : a * (a + 1) / 2;
+// Workgroup tiles of BM x BN an M x N product runs (for a triangular update
+// only those that meet the triangle).
+template <class C>
+int64_t tile_count(int tri, int64_t M, int64_t N) {
+  const int64_t a = ceil_div(M, C::BM), b = ceil_div(N, C::BN);
+  if (tri == kFull) return a * b;
+  int64_t n = 0;
+  for (int64_t tm = 0; tm < a; ++tm) {
+    const int64_t lo = tm * C::BM, hi = lo + C::BM - 1;  // rows of the tile
+    const int64_t first = tri == kLower ? 0 : lo / C::BN;
+    const int64_t last = tri == kLower ? std::min<int64_t>(b - 1, hi / C::BN) : b - 1;
+    n += last >= first ? last - first + 1 : 0;
+  }
+  return n;
 }
 
 template <class T, int TRI>
@@ -83,13 +93,14 @@ void run(Ctx& c, int64_t M, int64_t N, int64_t K, T alpha, const Operand<T>& A, 
          int64_t ldc) {
   if (M <= 0 || N <= 0 || K <= 0) return;
   // The largest tiles that still give every CU work (two tiles per CU for
-  // the 256-wide ones, which run one workgroup per CU).
-  if (tile_count(TRI, M, N, Tiles<T>::Huge::BM) >= 2 * c.num_cus)
-    launch_cfg<typename Tiles<T>::Huge, TRI>(c, M, N, K, alpha, A, B, Cp, ldc);
-  else if (tile_count(TRI, M, N, Tiles<T>::Big::BM) >= c.num_cus)
-    launch_cfg<typename Tiles<T>::Big, TRI>(c, M, N, K, alpha, A, B, Cp, ldc);
+  // the largest, which run one workgroup per CU).
+  using Tl = Tiles<T>;
+  if (tile_count<typename Tl::Huge>(TRI, M, N) >= 2 * c.num_cus)
+    launch_cfg<typename Tl::Huge, TRI>(c, M, N, K, alpha, A, B, Cp, ldc);
+  else if (tile_count<typename Tl::Big>(TRI, M, N) >= c.num_cus)
+    launch_cfg<typename Tl::Big, TRI>(c, M, N, K, alpha, A, B, Cp, ldc);
   else
-    launch_cfg<typename Tiles<T>::Small, TRI>(c, M, N, K, alpha, A, B, Cp, ldc);
+    launch_cfg<typename Tl::Small, TRI>(c, M, N, K, alpha, A, B, Cp, ldc);
 }
 
 template <class T>

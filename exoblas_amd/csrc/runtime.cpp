@@ -109,8 +109,12 @@ DeviceState& device_state(int device) {
   hipDeviceProp_t prop;
   EXO_HIP_CHECK(hipGetDeviceProperties(&prop, device));
   ds->num_cus = prop.multiProcessorCount;
+  // A blocking stream: the synchronous entry points then order after work
+  // already queued on the legacy default stream (a caller that filled its
+  // device buffers there, as a torch program does by default), and work
+  // queued there later waits for them, as with a BLAS call on that stream.
   hipStream_t s;
-  EXO_HIP_CHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  EXO_HIP_CHECK(hipStreamCreateWithFlags(&s, hipStreamDefault));
   ds->own = make_stream_state(device, s);
   DeviceState& ref = *ds;
   g_devices.emplace(device, std::move(ds));

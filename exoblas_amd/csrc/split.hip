@@ -4,6 +4,7 @@
 // of the exact remainder:  x = h + m + l + r  with h = bf16(x),
 // m = bf16(x - h), l = bf16(x - h - m) (every difference is exact in FP32).
 //   terms = 3:  two pieces (h, m):      h*h + h*m + m*h        |r| ~ 2^-17 |x|
+//               (+ m*m on the syrk tiles that meet the diagonal)
 //   terms = 6:  three pieces (h, m, l): adds m*m + h*l + l*h   |r| ~ 2^-26 |x|
 // Products of BF16 pieces are exact in FP32 and the MFMA accumulates in FP32;
 // the small terms are issued first so they enter the accumulator before the
@@ -148,6 +149,10 @@ __global__ __launch_bounds__(SH::NTH) void product_kernel(int64_t M, int64_t N, 
   const int lm = lane % 16, lg = lane / 16;
   const int64_t groups = Kp / 8;
   const int64_t nk = Kp / KT;
+  // A syrk tile that meets the diagonal multiplies rows by themselves there:
+  // the dropped m*m term of a 3-product split is then m^2 >= 0, a bias that
+  // grows with K rather than noise, so those tiles add it as a fourth product.
+  const bool diag = TRI != gemm::kFull && m0 < n0 + SH::BN && n0 < m0 + SH::BM;
   v4f acc[SM][SN];
 #pragma unroll
   for (int a = 0; a < SM; ++a)
@@ -188,6 +193,8 @@ __global__ __launch_bounds__(SH::NTH) void product_kernel(int64_t M, int64_t N, 
     if (P == 3) {
       term(2 % P, 0);
       term(0, 2 % P);
+      term(1, 1);
+    } else if (diag) {
       term(1, 1);
     }
     term(1, 0);

@@ -179,3 +179,85 @@ def test_mscal_trmscal(C, dt, uplo, rng):
                                                        L.win2(A.ctypes.data, ld))
             L.check()
             assert bits_equal(A, Ao)
+
+
+# Shapes that reach the largest tile configuration (at least two workgroup
+# tiles per CU) for each form: FP32 256 x 256 tiles, FP64 128 x 256.
+HUGE = {np.float32: dict(mn=(5800, 5784), k=700, syrk=(8192, 260)),
+        np.float64: dict(mn=(4100, 4104), k=300, syrk=(6000, 200))}
+
+
+def _dev(torch, rows, cols, dt, g):
+    tdt = torch.float32 if dt == np.float32 else torch.float64
+    return torch.empty((rows, cols), dtype=tdt, device="cuda").uniform_(-1, 1, generator=g)
+
+
+def _check_dev(torch, got, truth, mag, K, dt):
+    eps = float(np.finfo(dt).eps)
+    # mag is 0 only where C0 is exactly 0 and nothing is added (err is 0 there)
+    ratio = ((got.double() - truth).abs() / (mag * (K + 2) * eps).clamp_min(1e-300)).max().item()
+    assert ratio <= 1.0, ratio
+
+
+@pytest.mark.parametrize("dt", DT)
+@pytest.mark.parametrize("ta,tb,ld_diff", [(NO_TRANS, NO_TRANS, 0), (NO_TRANS, TRANS, 0), (TRANS, NO_TRANS, 0),
+                                           (TRANS, TRANS, 0), (NO_TRANS, NO_TRANS, 3)])
+def test_gemm_huge_tiles(C, dt, ta, tb, ld_diff):
+    import torch
+    g = torch.Generator(device="cuda")
+    g.manual_seed(1)
+    (M, N), K = HUGE[dt]["mn"], HUGE[dt]["k"]
+    A = _dev(torch, M if ta == NO_TRANS else K, (K if ta == NO_TRANS else M) + ld_diff, dt, g)
+    B = _dev(torch, K if tb == NO_TRANS else N, (N if tb == NO_TRANS else K) + ld_diff, dt, g)
+    Cm = _dev(torch, M, N + ld_diff, dt, g)
+    C0 = Cm.clone()
+    fn(C, dt, "gemm")(RM, ta, tb, M, N, K, 1.5, A, A.shape[1], B, B.shape[1], 0.5, Cm, Cm.shape[1])
+    torch.cuda.synchronize()
+    opA = A[:, :K] if ta == NO_TRANS else A[:, :M].T
+    opB = B[:, :N] if tb == NO_TRANS else B[:, :K].T
+    truth = 0.5 * C0[:, :N].double() + 1.5 * (opA.double() @ opB.double())
+    mag = 0.5 * C0[:, :N].double().abs() + 1.5 * (opA.double().abs() @ opB.double().abs())
+    _check_dev(torch, Cm[:, :N], truth, mag, K, dt)
+    assert torch.equal(Cm[:, N:], C0[:, N:])  # padding columns untouched
+
+
+@pytest.mark.parametrize("dt", DT)
+@pytest.mark.parametrize("side,uplo", list(itertools.product([LEFT, RIGHT], [UPPER, LOWER])))
+def test_symm_huge_tiles(C, dt, side, uplo):
+    import torch
+    g = torch.Generator(device="cuda")
+    g.manual_seed(2)
+    M, N = HUGE[dt]["mn"]
+    na = M if side == LEFT else N
+    A = _dev(torch, na, na, dt, g)
+    B = _dev(torch, M, N, dt, g)
+    Cm = _dev(torch, M, N, dt, g)
+    C0 = Cm.clone()
+    fn(C, dt, "symm")(RM, side, uplo, M, N, 1.5, A, na, B, N, 0.5, Cm, N)
+    torch.cuda.synchronize()
+    T = A.double().tril() if uplo == LOWER else A.double().triu()
+    S = T + T.T - torch.diag(torch.diag(T))
+    prod = S @ B.double() if side == LEFT else B.double() @ S
+    mag_p = S.abs() @ B.double().abs() if side == LEFT else B.double().abs() @ S.abs()
+    _check_dev(torch, Cm, 0.5 * C0.double() + 1.5 * prod, 0.5 * C0.double().abs() + 1.5 * mag_p, na, dt)
+
+
+@pytest.mark.parametrize("dt", DT)
+@pytest.mark.parametrize("uplo,trans", list(itertools.product([UPPER, LOWER], [NO_TRANS, TRANS])))
+def test_syrk_huge_tiles(C, dt, uplo, trans):
+    import torch
+    g = torch.Generator(device="cuda")
+    g.manual_seed(3)
+    N, K = HUGE[dt]["syrk"]
+    A = _dev(torch, N if trans == NO_TRANS else K, K if trans == NO_TRANS else N, dt, g)
+    Cm = _dev(torch, N, N, dt, g)
+    C0 = Cm.clone()
+    fn(C, dt, "syrk")(RM, uplo, trans, N, K, 1.5, A, A.shape[1], 0.5, Cm, N)
+    torch.cuda.synchronize()
+    X = (A if trans == NO_TRANS else A.T).double()
+    mask = torch.ones((N, N), dtype=torch.bool, device="cuda")
+    mask = mask.tril() if uplo == LOWER else mask.triu()
+    truth = torch.where(mask, 0.5 * C0.double() + 1.5 * (X @ X.T), C0.double())
+    mag = torch.where(mask, 0.5 * C0.double().abs() + 1.5 * (X.abs() @ X.abs().T), C0.double().abs())
+    _check_dev(torch, Cm, truth, mag, K, dt)
+    assert torch.equal(Cm[~mask], C0[~mask])  # other triangle untouched
