@@ -38,6 +38,69 @@ __global__ __launch_bounds__(C::NT) void gemm_kernel(int64_t M, int64_t N, int64
   tile_store<C, TRI>(Cp, ldc, cvec, M, N, m0, n0, alpha, acc);
 }
 
+// Split-K: workgroup (s, tile) runs the K slice [s * ksplit, (s + 1) * ksplit)
+// of its tile and writes the raw sums to P[s] (M x N, leading dimension N);
+// splitk_reduce_kernel then adds alpha * sum_s P[s] to C in a fixed order.
+template <class C, int TRI>
+__global__ __launch_bounds__(C::NT) void gemm_splitk_kernel(int64_t M, int64_t N, int64_t K, int64_t ksplit,
+                                                            Operand<typename C::type> A, Operand<typename C::type> B,
+                                                            typename C::type* P, int pvec, int64_t tiles_m,
+                                                            int64_t tiles_n) {
+  using T = typename C::type;
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int64_t tiles = tiles_m * tiles_n;
+  const int64_t s = blockIdx.x / tiles;
+  int64_t tm, tn;
+  tile_coords(blockIdx.x % tiles, tiles, tiles_m, tiles_n, tm, tn);
+  const int64_t m0 = tm * C::BM, n0 = tn * C::BN;
+  if (TRI == kLower && n0 > m0 + C::BM - 1) return;
+  if (TRI == kUpper && n0 + C::BN - 1 < m0) return;
+  typename Mfma<T>::acc acc[C::SM][C::SN];
+#pragma unroll
+  for (int a = 0; a < C::SM; ++a)
+#pragma unroll
+    for (int b = 0; b < C::SN; ++b)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) acc[a][b][q] = T(0);
+  const int64_t kb = s * ksplit, ke = kb + ksplit < K ? kb + ksplit : K;
+  tile_mma<C>(A, B, M, N, K, m0, n0, kb, ke, acc, reinterpret_cast<T*>(smem));
+  tile_store<C, kFull, true>(P + s * M * N, N, pvec, M, N, m0, n0, T(1), acc, T(0));
+}
+
+template <class T, int TRI>
+__global__ __launch_bounds__(256) void splitk_reduce_kernel(int64_t M, int64_t N, int S, T alpha, const T* P,
+                                                            T* Cp, int64_t ldc) {
+  const int64_t i = blockIdx.y, j = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (j >= N || (TRI == kLower && j > i) || (TRI == kUpper && j < i)) return;
+  T sum = T(0);
+  for (int k = 0; k < S; ++k) sum += P[(int64_t)k * M * N + i * N + j];
+  Cp[i * ldc + j] += alpha * sum;
+}
+
+template <class C, int TRI>
+void launch_splitk(Ctx& c, int64_t S, int64_t M, int64_t N, int64_t K, typename C::type alpha,
+                   const Operand<typename C::type>& A, const Operand<typename C::type>& B, typename C::type* Cp,
+                   int64_t ldc) {
+  using T = typename C::type;
+  const int64_t tm = ceil_div(M, C::BM), tn = ceil_div(N, C::BN);
+  const int64_t ksplit = ceil_div(ceil_div(K, S), C::BK) * C::BK;
+  S = ceil_div(K, ksplit);
+  T* P = static_cast<T*>(c.scratch(sizeof(T) * (size_t)(S * M * N), 4));
+  const int pvec = (N % C::EB == 0) ? 1 : 0;
+  static bool attr_set = false;
+  if (!attr_set) {
+    EXO_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&gemm_splitk_kernel<C, TRI>),
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)C::LDS_BYTES));
+    attr_set = true;
+  }
+  hipLaunchKernelGGL((gemm_splitk_kernel<C, TRI>), dim3((unsigned)(S * tm * tn)), dim3(C::NT), C::LDS_BYTES, c.stream,
+                     M, N, K, ksplit, A, B, P, pvec, tm, tn);
+  EXO_LAUNCH_CHECK();
+  hipLaunchKernelGGL((splitk_reduce_kernel<T, TRI>), dim3((unsigned)ceil_div(N, 256), (unsigned)M), dim3(256), 0,
+                     c.stream, M, N, (int)S, alpha, P, Cp, ldc);
+  EXO_LAUNCH_CHECK();
+}
+
 template <class C, int TRI>
 void launch_cfg(Ctx& c, int64_t M, int64_t N, int64_t K, typename C::type alpha,
                 const Operand<typename C::type>& A, const Operand<typename C::type>& B, typename C::type* Cp,
@@ -95,6 +158,13 @@ void run(Ctx& c, int64_t M, int64_t N, int64_t K, T alpha, const Operand<T>& A, 
   // The largest tiles that still give every CU work (two tiles per CU for
   // the largest, which run one workgroup per CU).
   using Tl = Tiles<T>;
+  // Few output tiles and a long K (sgemm 512^3: 64 tiles): split K so every
+  // CU gets several workgroups, each K slice at least 32 deep.
+  const int64_t small_tiles = tile_count<typename Tl::Small>(TRI, M, N);
+  if (small_tiles < c.num_cus && M <= 65535) {
+    const int64_t S = std::min<int64_t>({ceil_div((int64_t)c.num_cus, small_tiles), K / 128, 16});
+    if (S >= 2) return launch_splitk<typename Tl::Small, TRI>(c, S, M, N, K, alpha, A, B, Cp, ldc);
+  }
   if (tile_count<typename Tl::Huge>(TRI, M, N) >= 2 * c.num_cus)
     launch_cfg<typename Tl::Huge, TRI>(c, M, N, K, alpha, A, B, Cp, ldc);
   else if (tile_count<typename Tl::Big>(TRI, M, N) >= c.num_cus)
