@@ -152,6 +152,65 @@ def cpu_baseline(n1, n2, budget_s=15.0):
                       f" -O3 -ffast-math build of the oracle, 1 thread, host-resident operands)"}
 
 
+def vendor_cblas(n1, n2, budget_s, threads):
+    """The same set through the host's vendor CBLAS (MKL's libmkl_rt, as the
+    reference's own benchmarks compare against), `threads` threads; None
+    when no such library is present."""
+    import ctypes.util
+
+    import numpy as np
+    path = next((q for q in ("/opt/conda/lib/libmkl_rt.so", ctypes.util.find_library("mkl_rt") or "")
+                 if q and os.path.exists(q)), None)
+    if path is None:
+        return None
+    mkl = ctypes.CDLL(path)
+    mkl.MKL_Set_Num_Threads(ctypes.c_int(threads))
+    rng = np.random.default_rng(0)
+    P = ctypes.c_void_p
+    total_bytes = total_s = 0.0
+    reps = 0
+    t_start = time.perf_counter()
+    data = {}
+    for p, dt, T in (("s", np.float32, ctypes.c_float), ("d", np.float64, ctypes.c_double)):
+        data[p] = (dt, T, rng.uniform(-1, 1, n1).astype(dt), rng.uniform(-1, 1, n1).astype(dt),
+                   rng.uniform(-1, 1, (n2, n2)).astype(dt), rng.uniform(-1, 1, n2).astype(dt),
+                   rng.uniform(-1, 1, n2).astype(dt))
+    ptr = lambda a: a.ctypes.data_as(P)
+    first = True  # one untimed pass: library initialisation and page faults
+    while True:
+        for name, p, kind, nbytes in kernel_set(n1, n2):
+            dt, T, x, y, A, gx, gy = data[p]
+            f = lambda k: getattr(mkl, f"cblas_{p}{k}")
+            t0 = time.perf_counter()
+            if kind == "axpy":
+                f("axpy")(ctypes.c_int(n1), T(0.75), ptr(x), 1, ptr(y), 1)
+            elif kind == "dot":
+                fd = f("dot")
+                fd.restype = T
+                fd(ctypes.c_int(n1), ptr(x), 1, ptr(y), 1)
+            elif kind in ("gemv_n", "gemv_t"):
+                f("gemv")(101, 111 if kind == "gemv_n" else 112, n2, n2, T(1.5), ptr(A), n2, ptr(gx), 1, T(0.5),
+                          ptr(gy), 1)
+            elif kind == "copy":
+                f("copy")(ctypes.c_int(n1), ptr(x), 1, ptr(y), 1)
+            elif kind == "swap":
+                f("swap")(ctypes.c_int(n1), ptr(x), 1, ptr(y), 1)
+            elif kind == "rot":
+                f("rot")(ctypes.c_int(n1), ptr(x), 1, ptr(y), 1, T(0.6), T(0.8))
+            if not first:
+                total_s += time.perf_counter() - t0
+                total_bytes += nbytes
+        if first:
+            first = False
+            t_start = time.perf_counter()
+            continue
+        reps += 1
+        if time.perf_counter() - t_start > budget_s:
+            break
+    return {"value": round(total_bytes / total_s / 1e9, 3), "unit": "GB/s", "threads": threads,
+            "sample": f"{reps} pass(es) over the full bandwidth set, {os.path.basename(path)}"}
+
+
 def bench_bandwidth(a, torch, so, L):
     stream = torch.cuda.current_stream()
     sh = stream.cuda_stream
@@ -215,6 +274,9 @@ def bench_bandwidth(a, torch, so, L):
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(a.n1, a.n2, a.cpu_budget)
         out["cpu_baseline"]["cores_on_host"] = os.cpu_count()
+        vend = [vendor_cblas(a.n1, a.n2, a.cpu_budget / 3, t) for t in (1, os.cpu_count() or 1)]
+        if vend[0] is not None:
+            out["cpu_baseline"]["vendor_cblas"] = vend
     return out
 
 
