@@ -211,6 +211,16 @@ def vendor_cblas(n1, n2, budget_s, threads):
             "sample": f"{reps} pass(es) over the full bandwidth set, {os.path.basename(path)}"}
 
 
+def _gbench_emit(a, groups):
+    """Writes {kernel file: [records]} under --gbench-dir (Google Benchmark
+    layout read by the reference's analytics_tools/graphing)."""
+    if not a.gbench_dir:
+        return
+    from exoblas_amd import gbench
+    for kernel, recs in groups.items():
+        gbench.write(a.gbench_dir, "exoblas_amd", kernel, recs)
+
+
 def bench_bandwidth(a, torch, so, L):
     stream = torch.cuda.current_stream()
     sh = stream.cuda_stream
@@ -250,6 +260,17 @@ def bench_bandwidth(a, torch, so, L):
         gbs = b / (ms * 1e-3) / 1e9
         kernels[name] = {"us": round(ms * 1e3, 2), "GB/s": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4),
                          "bytes": b}
+    if a.gbench_dir:
+        from exoblas_amd import gbench
+        groups = {}
+        for (name, p, kind, b), ms in zip(ks, per_k_ms):
+            if kind in ("gemv_n", "gemv_t"):
+                groups.setdefault("gemv", []).append(
+                    gbench.gemv_record(p, 111 if kind == "gemv_n" else 112, a.n2, a.n2, ms * 1e3, a.steps))
+            else:
+                f, rec = gbench.level1_records(p, kind, a.n1, ms * 1e3, a.steps)
+                groups.setdefault(f, []).append(rec)
+        _gbench_emit(a, groups)
     dom = max(range(len(ks)), key=lambda j: per_k_ms[j])
     dname, _, _, dbytes = ks[dom]
     dach = dbytes / (per_k_ms[dom] * 1e-3) / 1e9
@@ -358,10 +379,11 @@ def bench_level3(a, torch, so, L):
     n = 512
     A5, B5, C5 = u((n, n), torch.float32), u((n, n), torch.float32), u((n, n), torch.float32)
     cases.append(("sgemm_NN_512", FP32_PEAK_TF, 2.0 * n ** 3,
-                  (lambda: so.exoblas_sgemm_stride_1_async(sh, 111, 111, n, n, n, ctypes.byref(ctypes.c_float(1.5)),
+                  (lambda n=n: so.exoblas_sgemm_stride_1_async(sh, 111, 111, n, n, n, ctypes.byref(ctypes.c_float(1.5)),
                                                            L.win2(A5.data_ptr(), n), L.win2(B5.data_ptr(), n),
                                                            L.win2(A5.data_ptr(), n), L.win2(B5.data_ptr(), n),
                                                            L.win2(C5.data_ptr(), n))), (A5, B5, C5)))
+    gb_groups = {}
     for name, peak, flop, fn, _ in cases:
         for _ in range(max(1, a.warmup // 2)):
             fn()
@@ -372,6 +394,18 @@ def bench_level3(a, torch, so, L):
                          "peak": peak, "flop": flop}
         total_flop += flop
         total_ms += ms
+        if a.gbench_dir:
+            from exoblas_amd import gbench
+            p, op = name[0], name[1:5]
+            if op == "gemm":
+                gn = int(name.rsplit("_", 1)[1])
+                rec = gbench.gemm_record(p, 111, 111, gn, gn, gn, ms * 1e3, reps)
+            elif op == "syrk":
+                rec = gbench.syrk_record(p, 122, 111, 4096, 1024, ms * 1e3, reps)
+            else:
+                rec = gbench.symm_record(p, 141, 122, 4096, 4096, ms * 1e3, reps)
+            gb_groups.setdefault(op, []).append(rec)
+    _gbench_emit(a, gb_groups)
     dom = "sgemm_NN_8192"
     k = kernels[dom]
     roof = {"bound": "mfma", "kernel": dom, "achieved": k["TFLOP/s"], "peak": FP32_PEAK_TF, "unit": "TFLOP/s",
@@ -391,6 +425,7 @@ def bench_trsv(a, torch, so, L):
     g = torch.Generator(device="cuda")
     g.manual_seed(0)
     kernels = {}
+    gb = []
     for p, dt, s in (("s", torch.float32, 4), ("d", torch.float64, 8)):
         for n in (4096, 8192):
             A = torch.empty((n, n), dtype=dt, device="cuda").uniform_(-1, 1, generator=g) / n
@@ -405,6 +440,10 @@ def bench_trsv(a, torch, so, L):
             gbs = nbytes / (ms * 1e-3) / 1e9
             kernels[f"{p}trsv_LNN_{n}"] = {"us": round(ms * 1e3, 2), "GB/s": round(gbs, 1),
                                            "frac": round(gbs / HBM_PEAK_GBS, 4), "bytes": nbytes}
+            if a.gbench_dir:
+                from exoblas_amd import gbench
+                gb.append(gbench.trsv_record(p, 122, 111, 131, n, ms * 1e3, 20))
+    _gbench_emit(a, {"trsv": gb} if gb else {})
     k = kernels["strsv_LNN_4096"]
     roof = {"bound": "hbm", "kernel": "strsv_LNN_4096", "achieved": k["GB/s"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": k["frac"], "traffic": None}
@@ -556,8 +595,82 @@ def bench_serving(a, torch, so, L):
                  p50_us_per_call=res["poisson_per_call"]["p50_us"], p99_us_per_call=res["poisson_per_call"]["p99_us"])
 
 
+def bench_hostpath(a, torch, so, L):
+    """The drop-in path as the reference's harness drives it: synchronous
+    exo_* calls on host (numpy) buffers, so every call stages its windows
+    over PCIe and returns with the results in host memory.  Rates here are
+    PCIe-inclusive (algorithmic bytes or flops / wall time per call); the
+    oracle's avx512 build on one host thread is timed on the same buffers.
+    sgemm 512^3 is config 1, with beta applied by the separate mscal call as
+    the wrapper does (test/level3/gemm/exo_gemm_wrapper.h:18-20)."""
+    import numpy as np
+
+    from oracle import Oracle
+    rng = np.random.default_rng(0)
+    f32 = np.float32
+    n1, n2, n3, nt = 1 << 26, 8192, 512, 4096
+    x, y = rng.uniform(-1, 1, n1).astype(f32), rng.uniform(-1, 1, n1).astype(f32)
+    A2, gx, gy = rng.uniform(-1, 1, (n2, n2)).astype(f32), rng.uniform(-1, 1, n2).astype(f32), np.zeros(n2, f32)
+    A3, B3, C3 = (rng.uniform(-1, 1, (n3, n3)).astype(f32) for _ in range(3))
+    T = (rng.uniform(-1, 1, (nt, nt)) / nt).astype(f32)
+    T[np.arange(nt), np.arange(nt)] = (nt + rng.uniform(0, 1, nt)).astype(f32)
+    b = rng.uniform(-1, 1, nt).astype(f32)
+    xt = b.copy()
+    res = np.zeros(1, f32)
+    P = lambda arr: arr.ctypes.data
+    fl = lambda v: ctypes.byref(ctypes.c_float(v))
+    cases = {
+        "saxpy_2^26": (3 * n1 * 4, "B",
+                       lambda: so.exo_saxpy_stride_1(None, n1, fl(0.75), L.win1(P(x), 1), L.win1(P(y), 1)),
+                       lambda o: o.axpy(n1, 0.75, x, 1, y, 1)),
+        "sdot_2^26": (2 * n1 * 4, "B",
+                      lambda: so.exo_sdot_stride_1(None, n1, L.win1(P(x), 1), L.win1(P(y), 1),
+                                                   ctypes.c_void_p(P(res))),
+                      lambda o: o.dot(n1, x, 1, y, 1)),
+        "sgemv_N_8192": ((n2 * n2 + 3 * n2) * 4, "B",
+                         lambda: so.exo_sgemv_rm_stride_1(None, 111, n2, n2, fl(1.5), fl(0.5), L.win2(P(A2), n2),
+                                                          L.win2(P(A2), n2), L.win1(P(gx), 1), L.win1(P(gy), 1)),
+                         lambda o: o.gemv(111, n2, n2, 1.5, 0.5, A2, n2, gx, 1, gy, 1)),
+        "sgemm_NN_512": (2.0 * n3 ** 3, "flop",
+                         lambda: (so.exo_smscal_rm_stride_1(None, n3, n3, fl(0.5), L.win2(P(C3), n3)),
+                                  so.exo_sgemm_stride_1(None, 111, 111, n3, n3, n3, fl(1.5), L.win2(P(A3), n3),
+                                                        L.win2(P(B3), n3), L.win2(P(A3), n3), L.win2(P(B3), n3),
+                                                        L.win2(P(C3), n3))),
+                         lambda o: (o.mscal(n3, n3, 0.5, C3, n3), o.gemm(111, 111, n3, n3, n3, 1.5, A3, n3, B3, n3,
+                                                                         C3, n3))),
+        "strsv_LNN_4096": ((nt * (nt + 1) // 2 + 2 * nt) * 4, "B",
+                           lambda: so.exo_strsv_rm_stride_1(None, 122, 111, 131, nt, L.win1(P(xt), 1),
+                                                            L.win2(P(T), nt)),
+                           lambda o: o.trsv(122, 111, 131, nt, xt, 1, T, nt)),
+    }
+    o = Oracle(f32, kind="fast")
+    kernels = {}
+    for name, (work, unit, gpu, cpu) in cases.items():
+        for _ in range(2):
+            gpu()
+        L.check()
+        reps = 5 if work > 1e8 else 20
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            gpu()
+        g_us = (time.perf_counter() - t0) / reps * 1e6
+        cpu(o)
+        c_reps = 1 if work > 1e8 else 3
+        t0 = time.perf_counter()
+        for _ in range(c_reps):
+            cpu(o)
+        c_us = (time.perf_counter() - t0) / c_reps * 1e6
+        rate = (lambda us: round(work / (us * 1e-6) / 1e9, 2))
+        kernels[name] = {"us": round(g_us, 1), f"G{unit}/s": rate(g_us), "cpu_us": round(c_us, 1),
+                         f"cpu_G{unit}/s": rate(c_us), "speedup_vs_cpu": round(c_us / g_us, 2)}
+    k = kernels["sgemm_NN_512"]
+    return _line("drop-in host-pointer path: PCIe-inclusive rate per call", k["Gflop/s"], "GFLOP/s", a,
+                 k["us"] / 1e3, "f32", "host buffers through the exo_* C-ABI: saxpy/sdot 2^26, sgemv N 8192^2, "
+                 "sgemm NN 512^3 (mscal + gemm), strsv L-N-N 4096", None, kernels)
+
+
 WORKLOADS = {"bandwidth": bench_bandwidth, "level3": bench_level3, "trsv": bench_trsv,
-             "precision": bench_precision, "serving": bench_serving}
+             "precision": bench_precision, "serving": bench_serving, "hostpath": bench_hostpath}
 
 
 def main(argv=None):
@@ -574,6 +687,8 @@ def main(argv=None):
     ap.add_argument("--max-batch", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--gbench-dir", default=None,
+                    help="also write the measured runs as Google Benchmark JSON (<dir>/exoblas_amd/<kernel>.json)")
     a = ap.parse_args(argv)
     if a.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {a.gpus} is not supported "
