@@ -13,12 +13,11 @@
 //      diagonal, each after the step that owns it has published its part of x;
 //   3. forms v = b - (panel . x) and x_block = inv(D) v with all its threads,
 //      so the serial part of the chain is one small mat-vec per block.
-// Publication: an FP32 value travels with the call's marker in one 64-bit
-// word (marker << 32 | bits), stored and polled with relaxed device-scope
-// atomics, so a reader that sees the marker has the value and no fence or
-// separate flag is needed (`pub` holds one word per row).  FP64 values do
-// not fit beside a marker: the block is stored, fenced, and a flag word per
-// block is released (`pub` holds one 32-bit flag per block).
+// Publication: every 32 bits of a value travel with the call's marker in one
+// 64-bit word (marker << 32 | bits), stored and polled with relaxed
+// device-scope atomics, so a reader that sees the marker has those bits and
+// no fence or separate flag is needed: one word per FP32 row, two (low and
+// high halves) per FP64 row.
 // Waits are bounded: past the spin limit the step records `marker` in *err
 // and finishes.
 #pragma once
@@ -34,18 +33,6 @@ constexpr int kB = 64;    // rows per block
 constexpr int kNT = 256;  // threads per workgroup
 constexpr long long kSpinLimit = 1ll << 22;
 
-__device__ __forceinline__ void wait_marker(const unsigned* f, unsigned marker, unsigned* err) {
-  long long it = 0;
-  while (__hip_atomic_load(f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != marker) {
-    if (++it > kSpinLimit) {
-      __hip_atomic_store(err, marker, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      break;
-    }
-    __builtin_amdgcn_s_sleep(1);
-  }
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-}
-
 // TI: precision of the diagonal-block inverse and of the block mat-vec.
 template <class T, class TI = T>
 struct Smem {
@@ -55,40 +42,46 @@ struct Smem {
   TI vs[kB];
 };
 
-// Words of publication space a solve of order n needs (see above).
+// 32-bit words of publication space a solve of order n needs (see above).
 template <class T>
 __host__ __device__ inline int64_t pub_words(int64_t n) {
-  return sizeof(T) == 4 ? 2 * n : (n + kB - 1) / kB;
+  return 2 * n * (int64_t)(sizeof(T) / 4);
+}
+
+// Polls one tagged word until it carries `marker` (bounded) and returns its
+// low 32 bits.
+__device__ __forceinline__ unsigned poll_word(uint64_t* w64, unsigned marker, unsigned* err) {
+  uint64_t w;
+  long long it = 0;
+  while (((w = __hip_atomic_load(w64, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) >> 32) != marker) {
+    if (++it > kSpinLimit) {
+      __hip_atomic_store(err, marker, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      break;
+    }
+    __builtin_amdgcn_s_sleep(1);
+  }
+  return (unsigned)w;
 }
 
 // Wave 0 of the workgroup fetches the published x block bj into xs.
 template <class T>
 __device__ __forceinline__ void fetch_block(int64_t n, int64_t bj, unsigned* pub, unsigned marker, unsigned* err,
-                                            T* xs, const T* x) {
+                                            T* xs) {
   const int t = threadIdx.x;
-  if constexpr (sizeof(T) == 4) {
-    if (t < kB) {
-      const int64_t g = bj * kB + t;
-      T v = T(0);
-      if (g < n) {
-        uint64_t* w64 = reinterpret_cast<uint64_t*>(pub) + g;
-        uint64_t w;
-        long long it = 0;
-        while (((w = __hip_atomic_load(w64, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) >> 32) != marker) {
-          if (++it > kSpinLimit) {
-            __hip_atomic_store(err, marker, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            break;
-          }
-          __builtin_amdgcn_s_sleep(1);
-        }
-        v = __uint_as_float((unsigned)w);
+  if (t < kB) {
+    const int64_t g = bj * kB + t;
+    T v = T(0);
+    if (g < n) {
+      uint64_t* w64 = reinterpret_cast<uint64_t*>(pub);
+      if constexpr (sizeof(T) == 4) {
+        v = __uint_as_float(poll_word(w64 + g, marker, err));
+      } else {
+        const unsigned lo = poll_word(w64 + 2 * g, marker, err);
+        const unsigned hi = poll_word(w64 + 2 * g + 1, marker, err);
+        v = __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
       }
-      xs[t] = v;
     }
-  } else {
-    if (t == 0) wait_marker(pub + bj, marker, err);
-    __syncthreads();
-    if (t < kB) xs[t] = (bj * kB + t < n) ? x[bj * kB + t] : T(0);
+    xs[t] = v;
   }
 }
 
@@ -96,18 +89,14 @@ __device__ __forceinline__ void fetch_block(int64_t n, int64_t bj, unsigned* pub
 template <class T>
 __device__ __forceinline__ void publish_value(int64_t g, T v, T* x, unsigned* pub, unsigned marker) {
   x[g] = v;
-  if constexpr (sizeof(T) == 4)
-    __hip_atomic_store(reinterpret_cast<uint64_t*>(pub) + g, ((uint64_t)marker << 32) | __float_as_uint(v),
-                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// After every thread has published its values: the FP64 block flag.
-template <class T>
-__device__ __forceinline__ void publish_block(int64_t bi, unsigned* pub, unsigned marker) {
-  if constexpr (sizeof(T) != 4) {
-    __threadfence();
-    __syncthreads();
-    if (threadIdx.x == 0) __hip_atomic_store(pub + bi, marker, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+  uint64_t* w64 = reinterpret_cast<uint64_t*>(pub);
+  const uint64_t tag = (uint64_t)marker << 32;
+  if constexpr (sizeof(T) == 4) {
+    __hip_atomic_store(w64 + g, tag | __float_as_uint(v), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  } else {
+    const uint64_t bits = (uint64_t)__double_as_longlong(v);
+    __hip_atomic_store(w64 + 2 * g, tag | (bits & 0xffffffffu), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(w64 + 2 * g + 1, tag | (bits >> 32), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
 }
 
@@ -185,7 +174,7 @@ __device__ __forceinline__ void step(int p, int64_t n, const T* __restrict__ A, 
       m[e] = (gi < n && gj < n) ? mat<T, TRANS>(A, lda, gi, gj) : T(0);
     }
     T* xs = sm.xs[q & 1];  // double-buffered: one barrier per block
-    fetch_block<T>(n, bj, pub, marker, err, xs, x);
+    fetch_block<T>(n, bj, pub, marker, err, xs);
     __syncthreads();
 #pragma unroll
     for (int e = 0; e < 16; ++e) acc += m[e] * xs[cq * 16 + e];
@@ -215,7 +204,6 @@ __device__ __forceinline__ void step(int p, int64_t n, const T* __restrict__ A, 
     const int64_t g = row_of<FWD>(r0, i);
     if (q4 == 0 && g < n) publish_value<T>(g, s, x, pub, marker);
   }
-  publish_block<T>(bi, pub, marker);
   __syncthreads();  // the shared buffers are reused by the caller's next step
 }
 

@@ -63,7 +63,7 @@ __global__ __launch_bounds__(kNT) void trsv_mixed_kernel(int64_t n, const float*
       m[e] = (gi < n && gj < n) ? to_f(TRANS ? Aoff[gj * ldo + gi] : Aoff[gi * ldo + gj]) : 0.f;
     }
     float* xs = sm.xs[q & 1];
-    trsv::fetch_block<float>(n, bj, done, epoch, flags + 1, xs, xf);
+    trsv::fetch_block<float>(n, bj, done, epoch, flags + 1, xs);
     __syncthreads();
 #pragma unroll
     for (int e = 0; e < 16; ++e) acc += m[e] * xs[cq * 16 + e];
