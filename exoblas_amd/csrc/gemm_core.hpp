@@ -319,5 +319,47 @@ __device__ __forceinline__ void tile_coords(int64_t bid, int64_t nblocks, int64_
   tn = (bid % per_group) / gsz;
 }
 
+// Tiles of row tm that meet the TRI triangle (lower: n0 <= m0 + BM - 1,
+// upper: n0 + BN - 1 >= m0), as the column range [first, first + count).
+__host__ __device__ inline void tri_row(int tri, int64_t tm, int64_t tiles_n, int BM, int BN, int64_t& first,
+                                        int64_t& count) {
+  const int64_t lo = tm * BM, hi = lo + BM - 1;
+  first = tri == kLower ? 0 : lo / BN;
+  const int64_t last = tri == kLower ? (hi / BN < tiles_n - 1 ? hi / BN : tiles_n - 1) : tiles_n - 1;
+  count = last >= first ? last - first + 1 : 0;
+}
+
+// Workgroup tiles of a TRI-restricted product (tri_tile_coords' list length).
+__host__ __device__ inline int64_t tri_tile_count(int tri, int64_t tiles_m, int64_t tiles_n, int BM, int BN) {
+  if (tri == kFull) return tiles_m * tiles_n;
+  int64_t n = 0, first, count;
+  for (int64_t tm = 0; tm < tiles_m; ++tm) {
+    tri_row(tri, tm, tiles_n, BM, BN, first, count);
+    n += count;
+  }
+  return n;
+}
+
+// Block id -> (tile_m, tile_n) over only the tiles that meet the triangle,
+// listed row by row, ids first regrouped per XCD as in tile_coords so every
+// XCD gets a contiguous, equal share.  (Launching the full grid and letting
+// off-triangle workgroups exit leaves whole XCDs idle: with the XCD-major
+// order their shares lie entirely on one side of the diagonal.)
+__device__ __forceinline__ void tri_tile_coords(int tri, int64_t bid, int64_t nblocks, int64_t tiles_m,
+                                                int64_t tiles_n, int BM, int BN, int64_t& tm, int64_t& tn) {
+  constexpr int kXcds = 8;
+  if (nblocks % kXcds == 0) bid = (bid % kXcds) * (nblocks / kXcds) + bid / kXcds;
+  int64_t acc = 0, first, count;
+  for (tm = 0; tm < tiles_m; ++tm) {
+    tri_row(tri, tm, tiles_n, BM, BN, first, count);
+    if (bid < acc + count) {
+      tn = first + (bid - acc);
+      return;
+    }
+    acc += count;
+  }
+  tn = tiles_n;  // past the list: the caller's bounds check skips it
+}
+
 }  // namespace gemm
 }  // namespace exoblas

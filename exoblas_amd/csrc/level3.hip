@@ -23,10 +23,12 @@ __global__ __launch_bounds__(C::NT) void gemm_kernel(int64_t M, int64_t N, int64
   using T = typename C::type;
   extern __shared__ __align__(16) unsigned char smem[];
   int64_t tm, tn;
-  tile_coords(blockIdx.x, gridDim.x, tiles_m, tiles_n, tm, tn);
+  if (TRI == kFull)
+    tile_coords(blockIdx.x, gridDim.x, tiles_m, tiles_n, tm, tn);
+  else
+    tri_tile_coords(TRI, blockIdx.x, gridDim.x, tiles_m, tiles_n, C::BM, C::BN, tm, tn);
+  if (tm >= tiles_m || tn >= tiles_n) return;
   const int64_t m0 = tm * C::BM, n0 = tn * C::BN;
-  if (TRI == kLower && n0 > m0 + C::BM - 1) return;
-  if (TRI == kUpper && n0 + C::BN - 1 < m0) return;
   typename Mfma<T>::acc acc[C::SM][C::SN];
 #pragma unroll
   for (int a = 0; a < C::SM; ++a)
@@ -48,13 +50,15 @@ __global__ __launch_bounds__(C::NT) void gemm_splitk_kernel(int64_t M, int64_t N
                                                             int64_t tiles_n) {
   using T = typename C::type;
   extern __shared__ __align__(16) unsigned char smem[];
-  const int64_t tiles = tiles_m * tiles_n;
+  const int64_t tiles = tri_tile_count(TRI, tiles_m, tiles_n, C::BM, C::BN);
   const int64_t s = blockIdx.x / tiles;
   int64_t tm, tn;
-  tile_coords(blockIdx.x % tiles, tiles, tiles_m, tiles_n, tm, tn);
+  if (TRI == kFull)
+    tile_coords(blockIdx.x % tiles, tiles, tiles_m, tiles_n, tm, tn);
+  else
+    tri_tile_coords(TRI, blockIdx.x % tiles, tiles, tiles_m, tiles_n, C::BM, C::BN, tm, tn);
+  if (tm >= tiles_m || tn >= tiles_n) return;
   const int64_t m0 = tm * C::BM, n0 = tn * C::BN;
-  if (TRI == kLower && n0 > m0 + C::BM - 1) return;
-  if (TRI == kUpper && n0 + C::BN - 1 < m0) return;
   typename Mfma<T>::acc acc[C::SM][C::SN];
 #pragma unroll
   for (int a = 0; a < C::SM; ++a)
@@ -93,7 +97,8 @@ void launch_splitk(Ctx& c, int64_t S, int64_t M, int64_t N, int64_t K, typename 
                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)C::LDS_BYTES));
     attr_set = true;
   }
-  hipLaunchKernelGGL((gemm_splitk_kernel<C, TRI>), dim3((unsigned)(S * tm * tn)), dim3(C::NT), C::LDS_BYTES, c.stream,
+  hipLaunchKernelGGL((gemm_splitk_kernel<C, TRI>), dim3((unsigned)(S * tri_tile_count(TRI, tm, tn, C::BM, C::BN))),
+                     dim3(C::NT), C::LDS_BYTES, c.stream,
                      M, N, K, ksplit, A, B, P, pvec, tm, tn);
   EXO_LAUNCH_CHECK();
   hipLaunchKernelGGL((splitk_reduce_kernel<T, TRI>), dim3((unsigned)ceil_div(N, 256), (unsigned)M), dim3(256), 0,
@@ -114,8 +119,8 @@ void launch_cfg(Ctx& c, int64_t M, int64_t N, int64_t K, typename C::type alpha,
                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)C::LDS_BYTES));
     attr_set = true;
   }
-  hipLaunchKernelGGL((gemm_kernel<C, TRI>), dim3((unsigned)(tm * tn)), dim3(C::NT), C::LDS_BYTES, c.stream, M, N, K,
-                     alpha, A, B, Cp, ldc, cvec, tm, tn);
+  hipLaunchKernelGGL((gemm_kernel<C, TRI>), dim3((unsigned)tri_tile_count(TRI, tm, tn, C::BM, C::BN)), dim3(C::NT),
+                     C::LDS_BYTES, c.stream, M, N, K, alpha, A, B, Cp, ldc, cvec, tm, tn);
   EXO_LAUNCH_CHECK();
 }
 
@@ -139,16 +144,7 @@ template <> struct Tiles<double> {
 // only those that meet the triangle).
 template <class C>
 int64_t tile_count(int tri, int64_t M, int64_t N) {
-  const int64_t a = ceil_div(M, C::BM), b = ceil_div(N, C::BN);
-  if (tri == kFull) return a * b;
-  int64_t n = 0;
-  for (int64_t tm = 0; tm < a; ++tm) {
-    const int64_t lo = tm * C::BM, hi = lo + C::BM - 1;  // rows of the tile
-    const int64_t first = tri == kLower ? 0 : lo / C::BN;
-    const int64_t last = tri == kLower ? std::min<int64_t>(b - 1, hi / C::BN) : b - 1;
-    n += last >= first ? last - first + 1 : 0;
-  }
-  return n;
+  return tri_tile_count(tri, ceil_div(M, C::BM), ceil_div(N, C::BN), C::BM, C::BN);
 }
 
 template <class T, int TRI>

@@ -140,10 +140,12 @@ __global__ __launch_bounds__(SH::NTH) void product_kernel(int64_t M, int64_t N, 
   uint4* As = reinterpret_cast<uint4*>(smem);
   uint4* Bs = As + 2 * LA::TV;
   int64_t tm, tn;
-  gemm::tile_coords(blockIdx.x, gridDim.x, tiles_m, tiles_n, tm, tn);
+  if (TRI == gemm::kFull)
+    gemm::tile_coords(blockIdx.x, gridDim.x, tiles_m, tiles_n, tm, tn);
+  else
+    gemm::tri_tile_coords(TRI, blockIdx.x, gridDim.x, tiles_m, tiles_n, SH::BM, SH::BN, tm, tn);
+  if (tm >= tiles_m || tn >= tiles_n) return;
   const int64_t m0 = tm * SH::BM, n0 = tn * SH::BN;
-  if (TRI == gemm::kLower && n0 > m0 + SH::BM - 1) return;
-  if (TRI == gemm::kUpper && n0 + SH::BN - 1 < m0) return;
   const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
   const int wm0 = (wave / SH::WAVES_N) * SH::WM, wn0 = (wave % SH::WAVES_N) * SH::WN;
   const int lm = lane % 16, lg = lane / 16;
@@ -242,7 +244,8 @@ void product(Ctx& c, int64_t M, int64_t N, int64_t Kp, float alpha, const __bf16
                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     attr_set = true;
   }
-  hipLaunchKernelGGL((product_kernel<P, TRI, SH>), dim3((unsigned)(tm * tn)), dim3(SH::NTH), lds, c.stream, M, N, Kp,
+  hipLaunchKernelGGL((product_kernel<P, TRI, SH>), dim3((unsigned)gemm::tri_tile_count(TRI, tm, tn, SH::BM, SH::BN)),
+                     dim3(SH::NTH), lds, c.stream, M, N, Kp,
                      alpha, Ap, Mp, Bp, Np, Cp, ldc, tm, tn);
   EXO_LAUNCH_CHECK();
 }
@@ -266,7 +269,8 @@ void run_p(Ctx& c, int tri, int64_t M, int64_t N, int64_t K, float alpha, const 
   // 256 x 256 tiles (half the operand traffic per flop) while they still give
   // two workgroups per CU, else 128 x 128; three pieces always use 128 x 128
   // (their 256-wide tile would not fit in LDS).
-  const bool large = P == 2 && (ceil_div(M, 256) * ceil_div(N, 256)) >= 2 * (int64_t)c.num_cus;
+  const bool large = P == 2 && gemm::tri_tile_count(tri, ceil_div(M, 256), ceil_div(N, 256), 256, 256) >=
+                                   2 * (int64_t)c.num_cus;
   const int64_t T = large ? 256 : 128;
   const int64_t Mp = pad(M, T), Np = pad(N, T), Kp = pad(K, KT);
   const size_t a_elems = (size_t)P * Kp * Mp, b_elems = same_operand ? 0 : (size_t)P * Kp * Np;
