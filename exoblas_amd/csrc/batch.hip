@@ -116,8 +116,8 @@ __device__ __forceinline__ void trsv_item(const Desc& d, uint32_t local, unsigne
                                       ctrl + 3, sm);
 }
 
-__global__ __launch_bounds__(kNT) void batch_kernel(const Desc* __restrict__ descs, const uint32_t* __restrict__ first,
-                                                    int32_t ndesc, uint32_t total, unsigned* ctrl) {
+__global__ __launch_bounds__(kNT) void batch_kernel(const Desc* __restrict__ descs, const uint64_t* __restrict__ order,
+                                                    uint32_t total, unsigned* ctrl) {
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ uint32_t s_item;
   const unsigned marker = __hip_atomic_load(ctrl + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) + 1u;
@@ -127,13 +127,9 @@ __global__ __launch_bounds__(kNT) void batch_kernel(const Desc* __restrict__ des
     const uint32_t item = s_item;
     __syncthreads();
     if (item >= total) break;
-    int lo = 0, hi = ndesc - 1;  // last problem whose first item is <= item
-    while (lo < hi) {
-      const int mid = (lo + hi + 1) >> 1;
-      if (first[mid] <= item) lo = mid; else hi = mid - 1;
-    }
-    const Desc d = descs[lo];
-    const uint32_t local = item - d.first_item;
+    const uint64_t e = order[item];
+    const Desc d = descs[e >> 32];
+    const uint32_t local = (uint32_t)e;
     if (d.kind == EXOBLAS_OP_GEMM) {
       float* lds = reinterpret_cast<float*>(smem);
       if (d.tile == 0) gemm_item<Tile0>(d, local, lds);
@@ -293,56 +289,90 @@ int build(const exoblas_batch_op* ops, int64_t count, Built& out, std::string& e
       return kInvalidEnum;
     }
   }
-  // trsv chains first (their steps are serial, so they should start early),
-  // the longest of them first; then gemm tiles and gemv blocks, the most
-  // expensive items first.  Stable on the submission index.
+  // Problems: trsv chains first, the longest first; then gemm and gemv, the
+  // most expensive items first.  Stable on the submission index.
   std::stable_sort(v.begin(), v.end(), [](const Item& a, const Item& b) {
     if (a.group != b.group) return a.group < b.group;
     return a.cost > b.cost;
   });
-  uint64_t total = 0, flags = 0;
+  uint64_t flags = 0, total = 0;
+  uint32_t rounds = 0;
+  size_t ntrsv = 0;
   for (auto& it : v) {
-    it.d.first_item = (uint32_t)total;
     if (it.d.kind == EXOBLAS_OP_TRSV) {
       it.d.flag_off = (uint32_t)flags;  // even: every problem's word count is
       flags += (uint64_t)it.d.k;        // even and the region starts 16-byte aligned
+      rounds = std::max(rounds, it.items);
+      ++ntrsv;
     }
     total += it.items;
-    if (total > 0xffffff00ull) {
-      err = "batch too large";
-      return kInvalidSize;
-    }
     out.descs.push_back(it.d);
-    out.first.push_back(it.d.first_item);
   }
-  out.total_items = (uint32_t)total;
+  if (total > 0xffffff00ull) {
+    err = "batch too large";
+    return kInvalidSize;
+  }
+  // The queue.  A trsv step waits only on the step before it in its chain,
+  // which sits in an earlier round, so every wait is on a queue entry some
+  // running workgroup already took and the queue cannot deadlock.  Spreading
+  // the steps over rounds keeps at most about one waiting workgroup per
+  // chain; the gemm / gemv items between rounds keep the rest of the grid
+  // busy while the chains advance.
+  out.order.reserve((size_t)total);
+  auto entry = [](size_t di, uint32_t local) { return ((uint64_t)di << 32) | local; };
+  size_t nd = ntrsv, nl = 0;  // next non-trsv problem and item
+  uint64_t rest = total;
+  for (size_t di = 0; di < ntrsv; ++di) rest -= v[di].items;
+  const uint64_t share = rounds ? (rest + rounds - 1) / rounds : rest;
+  auto take_rest = [&](uint64_t k) {
+    for (; k > 0 && nd < v.size(); --k) {
+      out.order.push_back(entry(nd, (uint32_t)nl));
+      if (++nl == v[nd].items) {
+        ++nd;
+        nl = 0;
+      }
+    }
+  };
+  for (uint32_t r = 0; r < rounds; ++r) {
+    for (size_t di = 0; di < ntrsv; ++di)
+      if (v[di].items > r) out.order.push_back(entry(di, r));
+    take_rest(share);
+  }
+  take_rest(rest);
+  out.total_items = (uint32_t)out.order.size();
   out.flag_words = (uint32_t)flags;
   return kOk;
 }
 
 void DeviceBatch::release() {
   if (desc) (void)hipFree(desc);
-  if (first) (void)hipFree(first);
+  if (order) (void)hipFree(order);
   if (ctrl) (void)hipFree(ctrl);
   desc = nullptr;
-  first = nullptr;
+  order = nullptr;
   ctrl = nullptr;
-  cap_desc = cap_flags = 0;
+  cap_desc = cap_order = cap_flags = 0;
 }
 
 void DeviceBatch::upload(const Built& b, hipStream_t stream) {
   const size_t nd = std::max<size_t>(1, b.descs.size());
+  const size_t no = std::max<size_t>(1, b.order.size());
   const size_t nf = (size_t)b.flag_words + kCtrlWords;
-  if (nd > cap_desc || nf > cap_flags || !ctrl) {
+  if (nd > cap_desc || no > cap_order || nf > cap_flags || !ctrl) {
     EXO_HIP_CHECK(hipStreamSynchronize(stream));
     if (nd > cap_desc) {
       if (desc) EXO_HIP_CHECK(hipFree(desc));
-      if (first) EXO_HIP_CHECK(hipFree(first));
       size_t c = 64;
       while (c < nd) c <<= 1;
       EXO_HIP_CHECK(hipMalloc(&desc, c * sizeof(Desc)));
-      EXO_HIP_CHECK(hipMalloc(&first, c * sizeof(uint32_t)));
       cap_desc = c;
+    }
+    if (no > cap_order) {
+      if (order) EXO_HIP_CHECK(hipFree(order));
+      size_t c = 1024;
+      while (c < no) c <<= 1;
+      EXO_HIP_CHECK(hipMalloc(&order, c * sizeof(uint64_t)));
+      cap_order = c;
     }
     if (nf > cap_flags || !ctrl) {
       if (ctrl) EXO_HIP_CHECK(hipFree(ctrl));
@@ -356,7 +386,7 @@ void DeviceBatch::upload(const Built& b, hipStream_t stream) {
   }
   if (!b.descs.empty()) {
     EXO_HIP_CHECK(hipMemcpyAsync(desc, b.descs.data(), b.descs.size() * sizeof(Desc), hipMemcpyHostToDevice, stream));
-    EXO_HIP_CHECK(hipMemcpyAsync(first, b.first.data(), b.first.size() * sizeof(uint32_t), hipMemcpyHostToDevice,
+    EXO_HIP_CHECK(hipMemcpyAsync(order, b.order.data(), b.order.size() * sizeof(uint64_t), hipMemcpyHostToDevice,
                                  stream));
   }
 }
@@ -364,8 +394,7 @@ void DeviceBatch::upload(const Built& b, hipStream_t stream) {
 void launch(const DeviceBatch& d, const Built& b, hipStream_t stream) {
   if (b.total_items == 0) return;
   const int grid = (int)std::min<int64_t>(b.total_items, max_grid());
-  hipLaunchKernelGGL(batch_kernel, dim3(grid), dim3(kNT), kSmem, stream, d.desc, d.first, (int32_t)b.descs.size(),
-                     b.total_items, d.ctrl);
+  hipLaunchKernelGGL(batch_kernel, dim3(grid), dim3(kNT), kSmem, stream, d.desc, d.order, b.total_items, d.ctrl);
   EXO_LAUNCH_CHECK();
 }
 
@@ -407,6 +436,26 @@ extern "C" int exoblas_batch_run(hipStream_t stream, const exoblas_batch_op* ops
     batch::launch(d, b, stream);
   } catch (const Error& e) {
     return batch::fail(e.code, std::string("exoblas_batch_run: ") + e.what());
+  }
+  return kOk;
+}
+
+extern "C" int exoblas_batch_queue(const exoblas_batch_op* ops, int64_t count, uint64_t* order, int32_t* kinds,
+                                   int64_t cap, int64_t* n_items, int64_t* n_problems) {
+  if (!n_items || !n_problems) return batch::fail(kInvalidPointer, "exoblas_batch_queue: null count pointer");
+  try {
+    batch::Built b;
+    std::string err;
+    const int st = batch::build(ops, count, b, err);
+    if (st != kOk) return batch::fail(st, "exoblas_batch_queue: " + err);
+    *n_items = (int64_t)b.order.size();
+    *n_problems = (int64_t)b.descs.size();
+    if (*n_items <= cap && *n_problems <= cap && order && kinds) {
+      std::copy(b.order.begin(), b.order.end(), order);
+      for (size_t i = 0; i < b.descs.size(); ++i) kinds[i] = b.descs[i].kind;
+    }
+  } catch (const Error& e) {
+    return batch::fail(e.code, std::string("exoblas_batch_queue: ") + e.what());
   }
   return kOk;
 }

@@ -26,8 +26,8 @@ struct Desc {
   float alpha, beta;
   int32_t tile;        // gemm tile id
   int32_t tiles_n;     // gemm tiles along N
-  uint32_t first_item; // index of the problem's first work item
-  uint32_t flag_off;   // trsv: offset of its block flags
+  uint32_t flag_off;   // trsv: offset of its publication words
+  uint32_t pad_;
 };
 
 // ctrl[0] work queue, ctrl[1] finished workgroups, ctrl[2] launch count,
@@ -35,15 +35,20 @@ struct Desc {
 constexpr int kCtrlWords = 4;
 constexpr int kRows = 64;  // gemv / trsv rows per work item
 
+// order[q] = (problem index << 32) | item index within the problem: the
+// work queue, taken front to back by the persistent kernel.
 struct Built {
   std::vector<Desc> descs;
-  std::vector<uint32_t> first;
+  std::vector<uint64_t> order;
   uint32_t total_items = 0;
   uint32_t flag_words = 0;
 };
 
-// Validates, orders (trsv chains first, then the longest items) and splits
-// `count` problems into work items.  Returns 0 or a status, with `err` set.
+// Validates `count` problems, splits them into work items and orders the
+// queue: the trsv chains' steps round by round (step r of every chain, the
+// longest chains first), with an even share of the gemm / gemv items (the
+// most expensive first) after each round.  Returns 0 or a status, with `err`
+// set.
 int build(const exoblas_batch_op* ops, int64_t count, Built& out, std::string& err);
 
 // Bytes of device memory a problem's operands occupy (admission accounting).
@@ -52,9 +57,9 @@ int64_t operand_bytes(const exoblas_batch_op& op);
 // Device-side buffers of one batch slot.
 struct DeviceBatch {
   Desc* desc = nullptr;
-  uint32_t* first = nullptr;
+  uint64_t* order = nullptr;
   unsigned* ctrl = nullptr;
-  size_t cap_desc = 0, cap_flags = 0;
+  size_t cap_desc = 0, cap_order = 0, cap_flags = 0;
   void release();
   // Grows the buffers (synchronising `stream` first if they are in use) and
   // enqueues the copies of `b` on `stream`.

@@ -113,7 +113,9 @@ __device__ __forceinline__ int64_t row_of(int64_t r0, int i) {
 }
 
 // Loads the diagonal block of block bi (rows r0 ..) in local order, converted
-// to TI, and inverts it in place; all kNT threads call this.
+// to TI, and inverts it in place (16 x 16 diagonal blocks, then two pairwise
+// merges: 40 barriers instead of the 128 of a column-by-column inverse); all
+// kNT threads call this.
 template <class T, class TI, bool FWD, bool TRANS, bool UNIT>
 __device__ __forceinline__ void invert_diag(int64_t n, const T* A, int64_t lda, int64_t r0, TI (&L)[kB][kB + 1]) {
   const int t = threadIdx.x;
@@ -131,19 +133,81 @@ __device__ __forceinline__ void invert_diag(int64_t n, const T* A, int64_t lda, 
     L[i][j] = v;
   }
   __syncthreads();
-  // Column j of the inverse from the already inverted trailing block:
-  // inv[t][j] = -(sum_{k=j+1..t} inv[t][k] L[k][j]) / L[j][j].
-  for (int j = kB - 1; j >= 0; --j) {
-    const TI djj = TI(1) / L[j][j];
-    TI nv = TI(0);
-    if (t > j && t < kB) {
-      TI s = TI(0);
-      for (int k = j + 1; k <= t; ++k) s += L[t][k] * L[k][j];
-      nv = -s * djj;
+  // 1. The four 16 x 16 diagonal blocks, column by column from the last (one
+  //    thread per row of each block): with X = inv(D),
+  //    X[i][j] = -(sum_{k=j+1..i} X[i][k] D[k][j]) / D[j][j], X[j][j] = 1 / D[j][j].
+  {
+    const int b = (t / 16) * 16, i = t % 16;
+    for (int j = 15; j >= 0; --j) {
+      TI djj = TI(0), nv = TI(0);
+      if (t < kB) {
+        djj = TI(1) / L[b + j][b + j];
+        if (i > j) {
+          TI acc = TI(0);
+          for (int k = j + 1; k <= i; ++k) acc += L[b + i][b + k] * L[b + k][b + j];
+          nv = -acc * djj;
+        }
+      }
+      __syncthreads();
+      if (t < kB) {
+        if (i > j) L[b + i][b + j] = nv;
+        if (i == j) L[b + j][b + j] = djj;
+      }
+      __syncthreads();
+    }
+  }
+  // 2. Merge inverted diagonal blocks pairwise, 16 -> 32 -> 64:
+  //    inv([A 0; B C]) = [inv(A) 0; -inv(C) B inv(A), inv(C)], the product
+  //    formed in registers and written over B.
+  for (int sz = 16; sz < kB; sz *= 2) {
+    constexpr int kMaxPer = kB * kB / 4 / kNT;  // outputs per thread at the last merge
+    const int per = (kB / (2 * sz)) * sz * sz / kNT;
+    TI pv[kMaxPer];
+    auto coords = [&](int e, int& base, int& r, int& c) {
+      const int idx = t + kNT * e;
+      base = (idx / (sz * sz)) * 2 * sz;
+      r = (idx % (sz * sz)) / sz;
+      c = idx % sz;
+    };
+#pragma unroll
+    for (int e = 0; e < kMaxPer; ++e) {
+      if (e < per) {
+        int base, r, c;
+        coords(e, base, r, c);
+        TI acc = TI(0);  // (B inv(A))[r][c], inv(A) lower triangular
+        for (int k = c; k < sz; ++k) acc += L[base + sz + r][base + k] * L[base + k][base + c];
+        pv[e] = acc;
+      }
     }
     __syncthreads();
-    if (t > j && t < kB) L[t][j] = nv;
-    if (t == j) L[j][j] = djj;
+#pragma unroll
+    for (int e = 0; e < kMaxPer; ++e) {
+      if (e < per) {
+        int base, r, c;
+        coords(e, base, r, c);
+        L[base + sz + r][base + c] = pv[e];
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < kMaxPer; ++e) {
+      if (e < per) {
+        int base, r, c;
+        coords(e, base, r, c);
+        TI acc = TI(0);  // -(inv(C) P)[r][c], inv(C) lower triangular
+        for (int k = 0; k <= r; ++k) acc += L[base + sz + r][base + sz + k] * L[base + sz + k][base + c];
+        pv[e] = -acc;
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < kMaxPer; ++e) {
+      if (e < per) {
+        int base, r, c;
+        coords(e, base, r, c);
+        L[base + sz + r][base + c] = pv[e];
+      }
+    }
     __syncthreads();
   }
 }

@@ -44,6 +44,7 @@ class ServerStats(ctypes.Structure):
 
 SYMBOLS = [
     ("exoblas_batch_run", _i, (_P, _P, _I64)),
+    ("exoblas_batch_queue", _i, (_P, _I64, _P, _P, _I64, _P, _P)),
     ("exoblas_batch_plan_create", _i, (ctypes.POINTER(_P), _P, _I64)),
     ("exoblas_batch_plan_launch", _i, (_P, _P)),
     ("exoblas_batch_plan_destroy", _i, (_P,)),

@@ -38,6 +38,14 @@ typedef struct exoblas_batch_op {
  * descriptors are copied before the call returns). */
 int exoblas_batch_run(hipStream_t stream, const exoblas_batch_op *ops, int64_t count);
 
+/* The work queue a batch of these problems would run, without running it
+ * (host only; the operand pointers are not dereferenced): *n_items entries
+ * order[q] = (problem << 32) | item of that problem, and kinds[p] the
+ * EXOBLAS_OP_* of problem p of the queue's numbering (*n_problems of them).
+ * The arrays are filled when both counts fit in cap. */
+int exoblas_batch_queue(const exoblas_batch_op *ops, int64_t count, uint64_t *order, int32_t *kinds, int64_t cap,
+                        int64_t *n_items, int64_t *n_problems);
+
 /* A fixed batch with device-resident descriptors: launching it is one kernel
  * launch with no copies or memsets, so it can be captured in a hipGraph and
  * replayed.  Launches of one plan must not overlap in time. */
