@@ -3101,7 +3101,7 @@ extern "C" int exoblas_sgemm_split(void *ctxt, int_fast32_t TransA, int_fast32_t
   if (M < 0) return call.fail_size("M", M);
   if (N < 0) return call.fail_size("N", N);
   if (K < 0) return call.fail_size("K", K);
-  if (!(terms == 3 || terms == 6)) return call.fail_enum("terms", terms);
+  if (!(terms == 3 || terms == 4 || terms == 6)) return call.fail_enum("terms", terms);
   if ((TransA == 111) && A.strides[1] != 1) return call.fail_stride("A", A.strides[1]);
   if ((TransA == 111) && M > 1 && A.strides[0] < K) return call.fail_ld("A", A.strides[0]);
   if ((TransB == 111) && B.strides[1] != 1) return call.fail_stride("B", B.strides[1]);
@@ -3130,7 +3130,7 @@ extern "C" int exoblas_sgemm_split_async(hipStream_t stream, int_fast32_t TransA
   if (M < 0) return call.fail_size("M", M);
   if (N < 0) return call.fail_size("N", N);
   if (K < 0) return call.fail_size("K", K);
-  if (!(terms == 3 || terms == 6)) return call.fail_enum("terms", terms);
+  if (!(terms == 3 || terms == 4 || terms == 6)) return call.fail_enum("terms", terms);
   if ((TransA == 111) && A.strides[1] != 1) return call.fail_stride("A", A.strides[1]);
   if ((TransA == 111) && M > 1 && A.strides[0] < K) return call.fail_ld("A", A.strides[0]);
   if ((TransB == 111) && B.strides[1] != 1) return call.fail_stride("B", B.strides[1]);
@@ -3160,7 +3160,7 @@ extern "C" int exoblas_ssyrk_split(void *ctxt, int_fast32_t Uplo, int_fast32_t T
   if (!(Trans == 111 || Trans == 112 || Trans == 113)) return call.fail_enum("Trans", Trans);
   if (N < 0) return call.fail_size("N", N);
   if (K < 0) return call.fail_size("K", K);
-  if (!(terms == 3 || terms == 6)) return call.fail_enum("terms", terms);
+  if (!(terms == 3 || terms == 4 || terms == 6)) return call.fail_enum("terms", terms);
   if ((Trans == 111) && A.strides[1] != 1) return call.fail_stride("A", A.strides[1]);
   if ((Trans == 111) && N > 1 && A.strides[0] < K) return call.fail_ld("A", A.strides[0]);
   if ((Trans != 111) && AT.strides[1] != 1) return call.fail_stride("AT", AT.strides[1]);
@@ -3182,7 +3182,7 @@ extern "C" int exoblas_ssyrk_split_async(hipStream_t stream, int_fast32_t Uplo, 
   if (!(Trans == 111 || Trans == 112 || Trans == 113)) return call.fail_enum("Trans", Trans);
   if (N < 0) return call.fail_size("N", N);
   if (K < 0) return call.fail_size("K", K);
-  if (!(terms == 3 || terms == 6)) return call.fail_enum("terms", terms);
+  if (!(terms == 3 || terms == 4 || terms == 6)) return call.fail_enum("terms", terms);
   if ((Trans == 111) && A.strides[1] != 1) return call.fail_stride("A", A.strides[1]);
   if ((Trans == 111) && N > 1 && A.strides[0] < K) return call.fail_ld("A", A.strides[0]);
   if ((Trans != 111) && AT.strides[1] != 1) return call.fail_stride("AT", AT.strides[1]);

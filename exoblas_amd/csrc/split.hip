@@ -5,23 +5,24 @@
 // m = bf16(x - h), l = bf16(x - h - m) (every difference is exact in FP32).
 //   terms = 3:  two pieces (h, m):      h*h + h*m + m*h        |r| ~ 2^-17 |x|
 //               (+ m*m on the syrk tiles that meet the diagonal)
+//   terms = 4:  two pieces, adds m*m everywhere
 //   terms = 6:  three pieces (h, m, l): adds m*m + h*l + l*h   |r| ~ 2^-26 |x|
-// Products of BF16 pieces are exact in FP32 and the MFMA accumulates in FP32;
-// the small terms are issued first so they enter the accumulator before the
-// dominant h*h term of the same K step.
+// Products of BF16 pieces are exact in FP32 and the MFMA accumulates in FP32.
+// With wave tiles of 64 x 64 the small products go to a second set of
+// accumulators, added once at the end, so the main accumulator takes one
+// rounding per K step instead of one per product (the accumulation error of
+// the main sum is what the small terms would otherwise add to).
 //
 // Two passes:
 //  1. pack: each operand, in any trans form, is split once into P BF16 planes
 //     in the MFMA-fragment layout [plane][K/8][rows][8] (rows padded to the
 //     tile, K to 32, zero filled), so 8 consecutive k of one row are 16
 //     contiguous bytes;
-//  2. product: 256 x 256 workgroup tiles of 4 waves (128 x 128 each, 8 x 8
-//     MFMA tiles of 16 x 16) when there are enough of them, else 128 x 128
-//     (waves of 64 x 64); K step 32, planes copied global -> LDS by the
+//  2. product: 256 x 128 workgroup tiles of 8 waves (64 x 64 each, 4 x 4
+//     MFMA tiles of 16 x 16) when they give every CU a workgroup, else
+//     128 x 128 of 4 waves; K step 32, planes copied global -> LDS by the
 //     load-to-LDS path (double-buffered), one 16-byte LDS read per fragment,
-//     no bounds checks inside the K loop.  The BF16 MFMA runs 16x the FP32
-//     MFMA rate, so the big tile is what keeps operand traffic below the L2
-//     bandwidth.
+//     no bounds checks inside the K loop.
 #include "device.hpp"
 #include "gemm_core.hpp"
 #include "ops.hpp"
@@ -127,12 +128,13 @@ struct Shape {
   static constexpr int SM = WM / 16, SN = WN / 16;
 };
 using Small = Shape<128, 128, 64, 64, 1>;
-using Large = Shape<256, 256, 128, 128, 0>;
+using Wide = Shape<256, 128, 64, 64, 0>;
 
 template <int P, int TRI, class SH>
 __global__ __launch_bounds__(SH::NTH) void product_kernel(int64_t M, int64_t N, int64_t Kp, float alpha,
                                                           const __bf16* Ap, int64_t Mp, const __bf16* Bp, int64_t Np,
-                                                          float* Cp, int64_t ldc, int64_t tiles_m, int64_t tiles_n) {
+                                                          float* Cp, int64_t ldc, int64_t tiles_m, int64_t tiles_n,
+                                                          int all_mm) {
   using LA = Stage<P, SH::BM, SH::NTH, SH::PADV>;
   using LB = Stage<P, SH::BN, SH::NTH, SH::PADV>;
   constexpr int SM = SH::SM, SN = SH::SN;
@@ -151,15 +153,26 @@ __global__ __launch_bounds__(SH::NTH) void product_kernel(int64_t M, int64_t N, 
   const int lm = lane % 16, lg = lane / 16;
   const int64_t groups = Kp / 8;
   const int64_t nk = Kp / KT;
-  // A syrk tile that meets the diagonal multiplies rows by themselves there:
-  // the dropped m*m term of a 3-product split is then m^2 >= 0, a bias that
-  // grows with K rather than noise, so those tiles add it as a fourth product.
-  const bool diag = TRI != gemm::kFull && m0 < n0 + SH::BN && n0 < m0 + SH::BM;
-  v4f acc[SM][SN];
+  // The m*m product: always with all_mm (the 4-product form); with 3 products
+  // still on a syrk tile that meets the diagonal, which multiplies rows by
+  // themselves there, so the dropped term would be m^2 >= 0, a bias that
+  // grows with K rather than noise.
+  const bool diag = all_mm || (TRI != gemm::kFull && m0 < n0 + SH::BN && n0 < m0 + SH::BM);
+  // Wave tiles of at most 64 x 64 keep the small products in a second set of
+  // accumulators, added once at the end: the main accumulator then takes one
+  // rounding per K step (the h*h product) instead of one per product.
+  constexpr bool DUAL = SM * SN <= 16;
+  v4f acc[SM][SN], acc2[DUAL ? SM : 1][DUAL ? SN : 1];
 #pragma unroll
   for (int a = 0; a < SM; ++a)
 #pragma unroll
     for (int b = 0; b < SN; ++b) acc[a][b] = v4f{0.f, 0.f, 0.f, 0.f};
+  if constexpr (DUAL) {
+#pragma unroll
+    for (int a = 0; a < SM; ++a)
+#pragma unroll
+      for (int b = 0; b < SN; ++b) acc2[a][b] = v4f{0.f, 0.f, 0.f, 0.f};
+  }
   LA::copy(Ap, groups, Mp, m0, 0, As);
   LB::copy(Bp, groups, Np, n0, 0, Bs);
   wait_copies();
@@ -190,7 +203,13 @@ __global__ __launch_bounds__(SH::NTH) void product_kernel(int64_t M, int64_t N, 
 #pragma unroll
       for (int ta = 0; ta < SM; ++ta)
 #pragma unroll
-        for (int tb = 0; tb < SN; ++tb) acc[ta][tb] = mma(a[pa][ta], b[pb][tb], acc[ta][tb]);
+        for (int tb = 0; tb < SN; ++tb) {
+          if (DUAL && (pa | pb))
+            acc2[ta % (DUAL ? SM : 1)][tb % (DUAL ? SN : 1)] =
+                mma(a[pa][ta], b[pb][tb], acc2[ta % (DUAL ? SM : 1)][tb % (DUAL ? SN : 1)]);
+          else
+            acc[ta][tb] = mma(a[pa][ta], b[pb][tb], acc[ta][tb]);
+        }
     };
     if (P == 3) {
       term(2 % P, 0);
@@ -204,6 +223,12 @@ __global__ __launch_bounds__(SH::NTH) void product_kernel(int64_t M, int64_t N, 
     term(0, 0);
     wait_copies();
     __syncthreads();
+  }
+  if constexpr (DUAL) {
+#pragma unroll
+    for (int a = 0; a < SM; ++a)
+#pragma unroll
+      for (int b = 0; b < SN; ++b) acc[a][b] += acc2[a][b];
   }
   // D layout of the FP32-accumulating 16 x 16 MFMA: lane (lm, lg) holds rows
   // 4*lg + q of column lm.
@@ -234,7 +259,7 @@ void pack(Ctx& c, const Operand<float>& X, int64_t R, int64_t K, int64_t Rp, int
 
 template <int P, int TRI, class SH>
 void product(Ctx& c, int64_t M, int64_t N, int64_t Kp, float alpha, const __bf16* Ap, int64_t Mp, const __bf16* Bp,
-             int64_t Np, float* Cp, int64_t ldc) {
+             int64_t Np, float* Cp, int64_t ldc, int all_mm) {
   const int64_t tm = Mp / SH::BM, tn = Np / SH::BN;
   const size_t lds =
       2 * 16 * (size_t)(Stage<P, SH::BM, SH::NTH, SH::PADV>::TV + Stage<P, SH::BN, SH::NTH, SH::PADV>::TV);
@@ -246,53 +271,55 @@ void product(Ctx& c, int64_t M, int64_t N, int64_t Kp, float alpha, const __bf16
   }
   hipLaunchKernelGGL((product_kernel<P, TRI, SH>), dim3((unsigned)gemm::tri_tile_count(TRI, tm, tn, SH::BM, SH::BN)),
                      dim3(SH::NTH), lds, c.stream, M, N, Kp,
-                     alpha, Ap, Mp, Bp, Np, Cp, ldc, tm, tn);
+                     alpha, Ap, Mp, Bp, Np, Cp, ldc, tm, tn, all_mm);
   EXO_LAUNCH_CHECK();
 }
 
 template <int P, class SH>
 void product_tri(Ctx& c, int tri, int64_t M, int64_t N, int64_t Kp, float alpha, const __bf16* Ap, int64_t Mp,
-                 const __bf16* Bp, int64_t Np, float* Cp, int64_t ldc) {
+                 const __bf16* Bp, int64_t Np, float* Cp, int64_t ldc, int all_mm) {
   if (tri == gemm::kLower)
-    product<P, gemm::kLower, SH>(c, M, N, Kp, alpha, Ap, Mp, Bp, Np, Cp, ldc);
+    product<P, gemm::kLower, SH>(c, M, N, Kp, alpha, Ap, Mp, Bp, Np, Cp, ldc, all_mm);
   else if (tri == gemm::kUpper)
-    product<P, gemm::kUpper, SH>(c, M, N, Kp, alpha, Ap, Mp, Bp, Np, Cp, ldc);
+    product<P, gemm::kUpper, SH>(c, M, N, Kp, alpha, Ap, Mp, Bp, Np, Cp, ldc, all_mm);
   else
-    product<P, gemm::kFull, SH>(c, M, N, Kp, alpha, Ap, Mp, Bp, Np, Cp, ldc);
+    product<P, gemm::kFull, SH>(c, M, N, Kp, alpha, Ap, Mp, Bp, Np, Cp, ldc, all_mm);
 }
 
 // C (tri part) += alpha * X_A X_B^T, X_A M x K, X_B N x K; same_operand when
 // X_B is X_A (syrk), which is then packed once.
 template <int P>
 void run_p(Ctx& c, int tri, int64_t M, int64_t N, int64_t K, float alpha, const Operand<float>& A,
-           const Operand<float>& B, bool same_operand, float* Cp, int64_t ldc) {
-  // 256 x 256 tiles (half the operand traffic per flop) while they still give
-  // two workgroups per CU, else 128 x 128; three pieces always use 128 x 128
-  // (their 256-wide tile would not fit in LDS).
-  const bool large = P == 2 && gemm::tri_tile_count(tri, ceil_div(M, 256), ceil_div(N, 256), 256, 256) >=
-                                   2 * (int64_t)c.num_cus;
-  const int64_t T = large ? 256 : 128;
-  const int64_t Mp = pad(M, T), Np = pad(N, T), Kp = pad(K, KT);
+           const Operand<float>& B, bool same_operand, float* Cp, int64_t ldc, int all_mm) {
+  // 256 x 128 workgroup tiles (8 waves of 64 x 64, small products in their
+  // own accumulators) while they give every CU a workgroup, else 128 x 128.
+  // Rows are padded to the row tile, columns to the column tile; a syrk
+  // operand, packed once for both sides, to the larger.
+  const bool wide = gemm::tri_tile_count(tri, ceil_div(M, Wide::BM), ceil_div(N, Wide::BN), Wide::BM, Wide::BN) >=
+                    (int64_t)c.num_cus;
+  const int64_t TM = wide ? Wide::BM : Small::BM, TN = wide ? Wide::BN : Small::BN;
+  const int64_t Kp = pad(K, KT);
+  const int64_t Mp = same_operand ? pad(M, std::max(TM, TN)) : pad(M, TM);
+  const int64_t Np = same_operand ? Mp : pad(N, TN);
   const size_t a_elems = (size_t)P * Kp * Mp, b_elems = same_operand ? 0 : (size_t)P * Kp * Np;
   __bf16* ws = static_cast<__bf16*>(c.scratch((a_elems + b_elems) * sizeof(__bf16), 5));
   __bf16* Ap = ws;
   __bf16* Bp = same_operand ? ws : ws + a_elems;
   pack<P>(c, A, M, K, Mp, Kp, Ap);
   if (!same_operand) pack<P>(c, B, N, K, Np, Kp, Bp);
-  const int64_t Bn = same_operand ? Mp : Np;
-  if (large)
-    product_tri<P, Large>(c, tri, M, N, Kp, alpha, Ap, Mp, Bp, Bn, Cp, ldc);
+  if (wide)
+    product_tri<P, Wide>(c, tri, M, N, Kp, alpha, Ap, Mp, Bp, Np, Cp, ldc, all_mm);
   else
-    product_tri<P, Small>(c, tri, M, N, Kp, alpha, Ap, Mp, Bp, Bn, Cp, ldc);
+    product_tri<P, Small>(c, tri, M, N, Kp, alpha, Ap, Mp, Bp, Np, Cp, ldc, all_mm);
 }
 
 void run(Ctx& c, int tri, int64_t terms, int64_t M, int64_t N, int64_t K, float alpha, const Operand<float>& A,
          const Operand<float>& B, bool same_operand, float* Cp, int64_t ldc) {
   if (M <= 0 || N <= 0 || K <= 0) return;
   if (terms == 6)
-    run_p<3>(c, tri, M, N, K, alpha, A, B, same_operand, Cp, ldc);
+    run_p<3>(c, tri, M, N, K, alpha, A, B, same_operand, Cp, ldc, 0);
   else
-    run_p<2>(c, tri, M, N, K, alpha, A, B, same_operand, Cp, ldc);
+    run_p<2>(c, tri, M, N, K, alpha, A, B, same_operand, Cp, ldc, terms == 4 ? 1 : 0);
 }
 
 Operand<float> operand(const float* p, int64_t ld, int form) {

@@ -206,7 +206,7 @@ EXT_PROCS: List[Proc] = [
           win("B", "K", "N", used_when="TransB == 111"),
           win("AT", "K", "M", used_when="TransA != 111"),
           win("BT", "N", "K", used_when="TransB != 111"),
-          win("C", "M", "N", const=False), enum("terms", (3, 6))],
+          win("C", "M", "N", const=False), enum("terms", (3, 4, 6))],
          "C += alpha op(A) op(B) with FP32 operands split into BF16 terms on the BF16 MFMA; "
          "arguments as exo_sgemm_stride_1 (src/level3/gemm.py:17-47) plus the number of BF16 products",
          "exoblas_amd.precision", precisions=("s",)),
@@ -214,7 +214,7 @@ EXT_PROCS: List[Proc] = [
          [enum("Uplo", UPLO_VALUES), enum("Trans", TRANS_VALUES), size("N"), size("K"), scalar("alpha"),
           win("A", "N", "K", used_when="Trans == 111"),
           win("AT", "K", "N", used_when="Trans != 111"),
-          win("C", "N", "N", const=False), enum("terms", (3, 6))],
+          win("C", "N", "N", const=False), enum("terms", (3, 4, 6))],
          "one triangle of C += alpha A A^T (or A^T A) in the split BF16 form; "
          "arguments as exo_ssyrk_rm_stride_1 (src/level3/syrk.py:17-32) without the duplicate windows",
          "exoblas_amd.precision", precisions=("s",)),
