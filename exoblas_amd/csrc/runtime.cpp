@@ -3,12 +3,15 @@
 #include "runtime.hpp"
 
 #include <algorithm>
+#include <condition_variable>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
 #include <map>
 #include <memory>
+#include <thread>
 
+#include "copy_pool.hpp"
 #include "exoblas_amd.h"
 
 namespace exoblas {
@@ -74,6 +77,71 @@ struct Pool {
   }
 };
 
+// Pinned, chunked staging of large host ranges.  The host side is copied by
+// the copy threads into a ring of pinned chunks while the DMA engine moves
+// the previous chunk, so a transfer runs near the pinned PCIe rate (57 GB/s
+// per direction on MI355X) instead of the pageable path's 17-27 GB/s.  Used
+// only under the device's call mutex.
+class Stager {
+ public:
+  static constexpr size_t kChunk = size_t(8) << 20;
+  static constexpr int kRing = 4;
+  static constexpr size_t kMinBytes = size_t(4) << 20;  // smaller ranges: one plain copy
+
+  ~Stager() {
+    // the HIP runtime may already be gone at process exit: leave the pinned
+    // chunks and events to it
+  }
+  void h2d(char* dev, const char* host, size_t bytes, hipStream_t s) {
+    ensure();
+    for (size_t i = 0, off = 0; off < bytes; ++i, off += kChunk) {
+      const int slot = (int)(i % kRing);
+      const size_t len = std::min(kChunk, bytes - off);
+      EXO_HIP_CHECK(hipEventSynchronize(ev_[slot]));  // its previous DMA is done
+      pool().copy(ring_[slot], host + off, len);
+      EXO_HIP_CHECK(hipMemcpyAsync(dev + off, ring_[slot], len, hipMemcpyHostToDevice, s));
+      EXO_HIP_CHECK(hipEventRecord(ev_[slot], s));
+    }
+  }
+  // Returns after host[0, bytes) holds dev[0, bytes) as of the work queued on s.
+  void d2h(char* host, const char* dev, size_t bytes, hipStream_t s) {
+    ensure();
+    const size_t n = (bytes + kChunk - 1) / kChunk;
+    auto issue = [&](size_t i) {
+      const int slot = (int)(i % kRing);
+      const size_t off = i * kChunk, len = std::min(kChunk, bytes - off);
+      EXO_HIP_CHECK(hipEventSynchronize(ev_[slot]));
+      EXO_HIP_CHECK(hipMemcpyAsync(ring_[slot], dev + off, len, hipMemcpyDeviceToHost, s));
+      EXO_HIP_CHECK(hipEventRecord(ev_[slot], s));
+    };
+    for (size_t i = 0; i < n && i < (size_t)kRing; ++i) issue(i);
+    for (size_t i = 0; i < n; ++i) {
+      const int slot = (int)(i % kRing);
+      const size_t off = i * kChunk, len = std::min(kChunk, bytes - off);
+      EXO_HIP_CHECK(hipEventSynchronize(ev_[slot]));
+      pool().copy(host + off, ring_[slot], len);
+      if (i + kRing < n) issue(i + kRing);
+    }
+  }
+
+ private:
+  void ensure() {
+    if (ready_) return;
+    for (int i = 0; i < kRing; ++i) {
+      EXO_HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&ring_[i]), kChunk, hipHostMallocDefault));
+      EXO_HIP_CHECK(hipEventCreateWithFlags(&ev_[i], hipEventDisableTiming));
+    }
+    ready_ = true;
+  }
+  static CopyPool& pool() {
+    static CopyPool p(4);  // 4 threads: 76 GB/s host copies measured, above PCIe
+    return p;
+  }
+  char* ring_[kRing] = {};
+  hipEvent_t ev_[kRing] = {};
+  bool ready_ = false;
+};
+
 struct DeviceState {
   std::mutex call_mutex;  // serialises kSync calls on this device
   std::mutex map_mutex;   // guards `streams`
@@ -82,6 +150,7 @@ struct DeviceState {
   StreamState* own = nullptr;  // the library's stream for kSync calls
   std::map<hipStream_t, std::unique_ptr<StreamState>> streams;
   Pool pool;
+  Stager stager;
 };
 
 std::mutex g_devices_mutex;
@@ -296,8 +365,12 @@ bool Call::stage() {
     for (auto& r : ranges_) {
       size_t bytes = (size_t)(r.end - r.begin);
       r.dev = ds.pool.get(bytes, &r.cap);
-      if (r.read)
-        EXO_HIP_CHECK(hipMemcpyAsync(r.dev, r.begin, bytes, hipMemcpyHostToDevice, ctx_.stream));
+      if (r.read) {
+        if (bytes >= Stager::kMinBytes)
+          ds.stager.h2d(static_cast<char*>(r.dev), r.begin, bytes, ctx_.stream);
+        else
+          EXO_HIP_CHECK(hipMemcpyAsync(r.dev, r.begin, bytes, hipMemcpyHostToDevice, ctx_.stream));
+      }
     }
     for (int i = 0; i < nwin_; ++i) {
       Win& w = wins_[i];
@@ -332,11 +405,18 @@ int Call::finish() {
   try {
     if (status_ == kOk) EXO_LAUNCH_CHECK();
     if (mode_ == kSync && ctx_.stream) {
-      if (status_ == kOk)
-        for (auto& r : ranges_)
-          if (r.write)
-            EXO_HIP_CHECK(hipMemcpyAsync(const_cast<char*>(r.begin), r.dev, (size_t)(r.end - r.begin),
-                                         hipMemcpyDeviceToHost, ctx_.stream));
+      if (status_ == kOk) {
+        DeviceState& ds = device_state(ctx_.device);
+        for (auto& r : ranges_) {
+          if (!r.write) continue;
+          const size_t bytes = (size_t)(r.end - r.begin);
+          if (bytes >= Stager::kMinBytes)
+            ds.stager.d2h(const_cast<char*>(r.begin), static_cast<const char*>(r.dev), bytes, ctx_.stream);
+          else
+            EXO_HIP_CHECK(hipMemcpyAsync(const_cast<char*>(r.begin), r.dev, bytes, hipMemcpyDeviceToHost,
+                                         ctx_.stream));
+        }
+      }
       EXO_HIP_CHECK(hipStreamSynchronize(ctx_.stream));
     }
   } catch (const Error& e) {
