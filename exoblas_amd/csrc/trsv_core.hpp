@@ -29,17 +29,18 @@
 namespace exoblas {
 namespace trsv {
 
-constexpr int kB = 64;    // rows per block
-constexpr int kNT = 256;  // threads per workgroup
+constexpr int kB = 64;    // default rows per block (batched and mixed solves)
+constexpr int kNT = 256;  // threads per workgroup at kB
 constexpr long long kSpinLimit = 1ll << 22;
 
 // TI: precision of the diagonal-block inverse and of the block mat-vec.
-template <class T, class TI = T>
+// KB: rows per block; a block's workgroup has 4 * KB threads (4 per row).
+template <class T, class TI = T, int KB = kB>
 struct Smem {
-  TI inv[kB][kB + 1];
-  T red[kNT / kB][kB];
-  T xs[2][kB];
-  TI vs[kB];
+  TI inv[KB][KB + 1];
+  T red[4][KB];
+  T xs[2][KB];
+  TI vs[KB];
 };
 
 // 32-bit words of publication space a solve of order n needs (see above).
@@ -64,12 +65,12 @@ __device__ __forceinline__ unsigned poll_word(uint64_t* w64, unsigned marker, un
 }
 
 // Wave 0 of the workgroup fetches the published x block bj into xs.
-template <class T>
+template <class T, int KB = kB>
 __device__ __forceinline__ void fetch_block(int64_t n, int64_t bj, unsigned* pub, unsigned marker, unsigned* err,
                                             T* xs) {
   const int t = threadIdx.x;
-  if (t < kB) {
-    const int64_t g = bj * kB + t;
+  if (t < KB) {
+    const int64_t g = bj * KB + t;
     T v = T(0);
     if (g < n) {
       uint64_t* w64 = reinterpret_cast<uint64_t*>(pub);
@@ -107,24 +108,25 @@ __device__ __forceinline__ T mat(const T* A, int64_t lda, int64_t i, int64_t j) 
 
 
 // Natural row of local index i of block bi (local order = processing order).
-template <bool FWD>
+template <bool FWD, int KB = kB>
 __device__ __forceinline__ int64_t row_of(int64_t r0, int i) {
-  return FWD ? r0 + i : r0 + (kB - 1 - i);
+  return FWD ? r0 + i : r0 + (KB - 1 - i);
 }
 
 // Loads the diagonal block of block bi (rows r0 ..) in local order, converted
 // to TI, and inverts it in place (16 x 16 diagonal blocks, then two pairwise
-// merges: 40 barriers instead of the 128 of a column-by-column inverse); all
-// kNT threads call this.
-template <class T, class TI, bool FWD, bool TRANS, bool UNIT>
-__device__ __forceinline__ void invert_diag(int64_t n, const T* A, int64_t lda, int64_t r0, TI (&L)[kB][kB + 1]) {
+// merges: 40 barriers instead of the 128 of a column-by-column inverse at
+// KB = 64); all 4 * KB threads call this.
+template <class T, class TI, bool FWD, bool TRANS, bool UNIT, int KB = kB>
+__device__ __forceinline__ void invert_diag(int64_t n, const T* A, int64_t lda, int64_t r0, TI (&L)[KB][KB + 1]) {
+  constexpr int NT = 4 * KB;
   const int t = threadIdx.x;
-  for (int idx = t; idx < kB * kB; idx += kNT) {
+  for (int idx = t; idx < KB * KB; idx += NT) {
     // NoTrans reads rows of A, Trans reads columns: keep lanes on consecutive addresses
-    const int a = idx / kB, b = idx % kB;
+    const int a = idx / KB, b = idx % KB;
     const int i = TRANS ? b : a;  // local row
     const int j = TRANS ? a : b;  // local column
-    const int64_t gi = row_of<FWD>(r0, i), gj = row_of<FWD>(r0, j);
+    const int64_t gi = row_of<FWD, KB>(r0, i), gj = row_of<FWD, KB>(r0, j);
     TI v = TI(0);
     if (j == i)
       v = (UNIT || gi >= n) ? TI(1) : (TI)mat<T, TRANS>(A, lda, gi, gj);
@@ -140,7 +142,7 @@ __device__ __forceinline__ void invert_diag(int64_t n, const T* A, int64_t lda, 
     const int b = (t / 16) * 16, i = t % 16;
     for (int j = 15; j >= 0; --j) {
       TI djj = TI(0), nv = TI(0);
-      if (t < kB) {
+      if (t < KB) {
         djj = TI(1) / L[b + j][b + j];
         if (i > j) {
           TI acc = TI(0);
@@ -149,7 +151,7 @@ __device__ __forceinline__ void invert_diag(int64_t n, const T* A, int64_t lda, 
         }
       }
       __syncthreads();
-      if (t < kB) {
+      if (t < KB) {
         if (i > j) L[b + i][b + j] = nv;
         if (i == j) L[b + j][b + j] = djj;
       }
@@ -159,12 +161,12 @@ __device__ __forceinline__ void invert_diag(int64_t n, const T* A, int64_t lda, 
   // 2. Merge inverted diagonal blocks pairwise, 16 -> 32 -> 64:
   //    inv([A 0; B C]) = [inv(A) 0; -inv(C) B inv(A), inv(C)], the product
   //    formed in registers and written over B.
-  for (int sz = 16; sz < kB; sz *= 2) {
-    constexpr int kMaxPer = kB * kB / 4 / kNT;  // outputs per thread at the last merge
-    const int per = (kB / (2 * sz)) * sz * sz / kNT;
+  for (int sz = 16; sz < KB; sz *= 2) {
+    constexpr int kMaxPer = KB * KB / 4 / NT;  // outputs per thread at the last merge
+    const int per = (KB / (2 * sz)) * sz * sz / NT;
     TI pv[kMaxPer];
     auto coords = [&](int e, int& base, int& r, int& c) {
-      const int idx = t + kNT * e;
+      const int idx = t + NT * e;
       base = (idx / (sz * sz)) * 2 * sz;
       r = (idx % (sz * sz)) / sz;
       c = idx % sz;
@@ -212,60 +214,62 @@ __device__ __forceinline__ void invert_diag(int64_t n, const T* A, int64_t lda, 
   }
 }
 
-// One processing step p of the solve; called by all kNT threads of a workgroup.
-template <class T, bool FWD, bool TRANS, bool UNIT>
+// One processing step p of the solve; called by all 4 * KB threads of a
+// workgroup.
+template <class T, bool FWD, bool TRANS, bool UNIT, int KB = kB>
 __device__ __forceinline__ void step(int p, int64_t n, const T* __restrict__ A, int64_t lda, T* x, unsigned* pub,
-                                     unsigned marker, unsigned* err, Smem<T>& sm) {
-  const int nb = (int)((n + kB - 1) / kB);
+                                     unsigned marker, unsigned* err, Smem<T, T, KB>& sm) {
+  constexpr int CW = KB / 4;  // columns per thread in the block mat-vecs
+  const int nb = (int)((n + KB - 1) / KB);
   const int t = threadIdx.x;
   const int64_t bi = FWD ? p : nb - 1 - p;
-  const int64_t r0 = bi * kB;
-  invert_diag<T, T, FWD, TRANS, UNIT>(n, A, lda, r0, sm.inv);
-  // NoTrans: 4 threads per row, 16 consecutive columns each (contiguous
-  // loads); Trans: one column of 64 rows per wave, 16 columns per wave
-  // (consecutive lanes read consecutive addresses).
-  const int r = TRANS ? t % kB : t / 4;
-  const int cq = TRANS ? t / kB : t % 4;
+  const int64_t r0 = bi * KB;
+  invert_diag<T, T, FWD, TRANS, UNIT, KB>(n, A, lda, r0, sm.inv);
+  // NoTrans: 4 threads per row, CW consecutive columns each (contiguous
+  // loads); Trans: one column of KB rows per KB threads, CW columns per
+  // group (consecutive lanes read consecutive addresses).
+  const int r = TRANS ? t % KB : t / 4;
+  const int cq = TRANS ? t / KB : t % 4;
   const int64_t gi = r0 + r;
   T acc = T(0);
   for (int q = 0; q < p; ++q) {
     const int64_t bj = FWD ? q : nb - 1 - q;
-    const int64_t c0 = bj * kB + cq * 16;
-    T m[16];
+    const int64_t c0 = bj * KB + cq * CW;
+    T m[CW];
 #pragma unroll
-    for (int e = 0; e < 16; ++e) {
+    for (int e = 0; e < CW; ++e) {
       const int64_t gj = c0 + e;
       m[e] = (gi < n && gj < n) ? mat<T, TRANS>(A, lda, gi, gj) : T(0);
     }
     T* xs = sm.xs[q & 1];  // double-buffered: one barrier per block
-    fetch_block<T>(n, bj, pub, marker, err, xs);
+    fetch_block<T, KB>(n, bj, pub, marker, err, xs);
     __syncthreads();
 #pragma unroll
-    for (int e = 0; e < 16; ++e) acc += m[e] * xs[cq * 16 + e];
+    for (int e = 0; e < CW; ++e) acc += m[e] * xs[cq * CW + e];
   }
   sm.red[cq][r] = acc;
   __syncthreads();
-  if (t < kB) {  // right-hand side in local order
-    const int64_t g = row_of<FWD>(r0, t);
+  if (t < KB) {  // right-hand side in local order
+    const int64_t g = row_of<FWD, KB>(r0, t);
     const int nr = (int)(g - r0);
     T v = T(0);
     if (g < n) {
       T s = T(0);
 #pragma unroll
-      for (int w = 0; w < kNT / kB; ++w) s += sm.red[w][nr];
+      for (int w = 0; w < 4; ++w) s += sm.red[w][nr];
       v = x[g] - s;
     }
     sm.vs[t] = v;
   }
   __syncthreads();
-  {  // x_local = inv(D) v: 4 threads per row, 16 columns each
+  {  // x_local = inv(D) v: 4 threads per row, CW columns each
     const int i = t / 4, q4 = t % 4;
     T s = T(0);
 #pragma unroll
-    for (int e = 0; e < 16; ++e) s += sm.inv[i][q4 * 16 + e] * sm.vs[q4 * 16 + e];
+    for (int e = 0; e < CW; ++e) s += sm.inv[i][q4 * CW + e] * sm.vs[q4 * CW + e];
     s += __shfl_xor(s, 1, 64);
     s += __shfl_xor(s, 2, 64);
-    const int64_t g = row_of<FWD>(r0, i);
+    const int64_t g = row_of<FWD, KB>(r0, i);
     if (q4 == 0 && g < n) publish_value<T>(g, s, x, pub, marker);
   }
   __syncthreads();  // the shared buffers are reused by the caller's next step

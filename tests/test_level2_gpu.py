@@ -176,7 +176,16 @@ def test_trsv(C, dt, uplo, trans, diag, n, incx, ld_diff, rng):
 
 
 @pytest.mark.parametrize("dt", DT)
-@pytest.mark.parametrize("n", [2, 257, 1000])
+@pytest.mark.parametrize("uplo,trans,diag", list(itertools.product([UPPER, LOWER], [NO_TRANS, TRANS],
+                                                                  [UNIT, NON_UNIT])))
+@pytest.mark.parametrize("n,incx,ld_diff", [(1024, 1, 0), (1500, 1, 3), (2600, 2, 5)])
+def test_trsv_long(C, dt, uplo, trans, diag, n, incx, ld_diff, rng):
+    """Long solves: chains of 16 to 41 dependent 64-row blocks."""
+    test_trsv(C, dt, uplo, trans, diag, n, incx, ld_diff, rng)
+
+
+@pytest.mark.parametrize("dt", DT)
+@pytest.mark.parametrize("n", [2, 257, 1000, 1100, 2049])
 def test_trsv_known_answer(C, dt, n):
     # test/level2/trsv/correctness.cpp:20-25: every A[i][j] = 2.0, x[i] = i
     A = np.full((n, n), 2.0, dt)
