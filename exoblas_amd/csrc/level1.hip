@@ -17,21 +17,35 @@ constexpr int kNT = 256;
 
 // ------------------------------------------------------------ elementwise
 
-// One 16-byte vector per lane per round, NT lanes per workgroup; the grid
-// sweeps the vectors (one round per workgroup when the grid covers them all).
-template <class T, class Op, int NT>
+// U 16-byte vectors per lane per round (lane l of the workgroup takes
+// vectors l, l + NT, ..., l + (U-1) NT of the workgroup's NT*U), all loads
+// issued before the first store; the grid sweeps the vectors (one round per
+// workgroup when the grid covers them all).
+template <class T, class Op, int NT, int U>
 __global__ __launch_bounds__(NT) void ew_vec_kernel(int64_t nvec, int64_t n, T* __restrict__ x, T* __restrict__ y,
                                                     Op op) {
   using V = typename Vec16<T>::type;
   constexpr int VN = Vec16<T>::n;
-  for (int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x; i < nvec; i += (int64_t)gridDim.x * NT) {
-    V xv, yv;
-    if (Op::rx) xv = ld_stream<T>(x + i * VN);
-    if (Op::ry) yv = ld_stream<T>(y + i * VN);
+  for (int64_t i0 = (int64_t)blockIdx.x * NT * U + threadIdx.x; i0 < nvec; i0 += (int64_t)gridDim.x * NT * U) {
+    V xv[U], yv[U];
 #pragma unroll
-    for (int k = 0; k < VN; ++k) op(vget<T>(xv, k), vget<T>(yv, k));
-    if (Op::wx) st_stream<T>(x + i * VN, xv);
-    if (Op::wy) st_stream<T>(y + i * VN, yv);
+    for (int u = 0; u < U; ++u) {
+      const int64_t i = i0 + u * NT;
+      if (U == 1 || i < nvec) {
+        if (Op::rx) xv[u] = ld_stream<T>(x + i * VN);
+        if (Op::ry) yv[u] = ld_stream<T>(y + i * VN);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int64_t i = i0 + u * NT;
+      if (U == 1 || i < nvec) {
+#pragma unroll
+        for (int k = 0; k < VN; ++k) op(vget<T>(xv[u], k), vget<T>(yv[u], k));
+        if (Op::wx) st_stream<T>(x + i * VN, xv[u]);
+        if (Op::wy) st_stream<T>(y + i * VN, yv[u]);
+      }
+    }
   }
   if (blockIdx.x == gridDim.x - 1) {  // the < VN trailing elements
     const int64_t t = nvec * VN + threadIdx.x;
@@ -65,20 +79,16 @@ void launch_ew(Ctx& c, int64_t n, T* x, int64_t incx, T* y, int64_t incy, Op op)
   const bool unit = (!(Op::rx || Op::wx) || (incx == 1 && aligned16(x))) &&
                     (!(Op::ry || Op::wy) || (incy == 1 && aligned16(y)));
   if (unit) {
-    // One round per workgroup; axpy (two reads, one write) measured fastest
-    // with 1024 lanes per workgroup on MI355X (80.6 % of HBM peak against
-    // 79 % with 256), the other updates with 256.
+    // One round per workgroup of 256 lanes with four 16-byte vectors per lane
+    // in flight: the best of a sweep over 64-1024 lanes, 1-8 vectors per lane
+    // and persistent grids on MI355X for every update in the set (axpy 80.6 %,
+    // copy 77.7 %, swap and rot 77 % of HBM peak, against 67-69 % for swap
+    // and rot with one vector per lane).
+    constexpr int NT = kNT, U = 4;
     const int64_t nvec = n / VN;
-    constexpr int streams = Op::rx + Op::ry + Op::wx + Op::wy;
-    if (streams == 3) {
-      const int64_t blocks = std::max<int64_t>(1, ceil_div(nvec, 1024));
-      hipLaunchKernelGGL((ew_vec_kernel<T, Op, 1024>), dim3((unsigned)blocks), dim3(1024), 0, c.stream, nvec, n, x,
-                         y, op);
-    } else {
-      const int64_t blocks = std::max<int64_t>(1, ceil_div(nvec, kNT));
-      hipLaunchKernelGGL((ew_vec_kernel<T, Op, kNT>), dim3((unsigned)blocks), dim3(kNT), 0, c.stream, nvec, n, x, y,
-                         op);
-    }
+    const int64_t blocks = std::max<int64_t>(1, ceil_div(nvec, (int64_t)NT * U));
+    hipLaunchKernelGGL((ew_vec_kernel<T, Op, NT, U>), dim3((unsigned)blocks), dim3(NT), 0, c.stream, nvec, n, x, y,
+                       op);
   } else {
     const int64_t blocks = std::min<int64_t>(ceil_div(n, kNT), (int64_t)c.num_cus * 16);
     hipLaunchKernelGGL((ew_strided_kernel<T, Op>), dim3((unsigned)blocks), dim3(kNT), 0, c.stream, n, x, incx,
