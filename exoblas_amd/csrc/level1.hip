@@ -3,12 +3,13 @@
 // All of these are HBM-bound.  The unit-stride path moves 16 bytes per lane
 // per access (global_load_dwordx4), unrolled so each lane keeps several loads
 // in flight; a wave64 touches 1 KiB contiguous per access.  Strided and
-// misaligned windows use a grid-stride scalar kernel.  Reductions are two
-// launches: per-workgroup partial sums (wave shuffle + LDS) written to the
-// stream's scratch, then one workgroup sums the partials in a fixed order, so
-// results are deterministic.
+// misaligned windows use a grid-stride scalar kernel.  Reductions are one
+// launch: per-workgroup partial sums (wave shuffle + LDS) published as tagged
+// words (tagged.hpp), summed in a fixed order by the workgroup that finishes
+// last, so results are deterministic.
 #include "device.hpp"
 #include "ops.hpp"
+#include "tagged.hpp"
 
 namespace exoblas {
 namespace {
@@ -184,15 +185,24 @@ struct DsdotTerm {
   static constexpr bool ry = true;
 };
 
+constexpr int kReduceFlagBase = 2;  // [1] timeout; the 8-byte aligned partials from [2]
+
 template <class Acc, class Out> struct Finish {
   Acc add;  // sdsdot's sb
   __device__ Out operator()(Acc s) const { return (Out)(s + add); }
 };
 
+// One launch: every workgroup reduces its share and publishes its partial
+// as tagged words; the workgroup that takes the last ticket waits for all
+// partials and sums them in a fixed order (thread t takes partials t,
+// t + kNT, ..., then the workgroup tree), so the result does not depend on
+// which workgroup finishes last.  *ticket: the stream's first counter;
+// flags: [1] timeout, [2 ..] the tagged partials.
 template <class T, class Term, class Out, int U>
 __global__ __launch_bounds__(kNT) void reduce_kernel(int64_t n, const T* __restrict__ x, int64_t incx,
                                                      const T* __restrict__ y, int64_t incy, int vec,
-                                                     typename Term::Acc* partials, Term term) {
+                                                     unsigned* ticket, unsigned* flags, unsigned epoch, Out* result,
+                                                     Finish<typename Term::Acc, Out> fin, Term term) {
   using Acc = typename Term::Acc;
   using V = typename Vec16<T>::type;
   constexpr int VN = Vec16<T>::n;
@@ -242,19 +252,22 @@ __global__ __launch_bounds__(kNT) void reduce_kernel(int64_t n, const T* __restr
 #pragma unroll
   for (int u = 1; u < U; ++u) s += acc[u];
   s = block_sum<kNT>(s, red);
-  if (threadIdx.x == 0) partials[blockIdx.x] = s;
-}
-
-// Sums the per-workgroup partials in a fixed order and writes the result.
-// A second launch rather than a last-workgroup ticket: the ticket needs a
-// device-scope release per workgroup, which writes back the XCD's L2 each
-// time and costs more than this launch.
-template <class Acc, class Out>
-__global__ __launch_bounds__(kNT) void reduce_finish_kernel(int nparts, const Acc* partials, Out* result,
-                                                            Finish<Acc, Out> fin) {
-  __shared__ Acc red[kNT / kWave];
+  unsigned* pub = flags + kReduceFlagBase;
+  __shared__ int s_last;
+  if (threadIdx.x == 0) {
+    tagged::publish<Acc>(pub, blockIdx.x, s, epoch);
+    s_last = tagged::last_ticket(ticket, gridDim.x);
+  }
+  __syncthreads();
+  if (!s_last) return;
   Acc t = Acc(0);
-  for (int i = threadIdx.x; i < nparts; i += kNT) t += partials[i];
+  for (int i = threadIdx.x; i < (int)gridDim.x; i += 4 * kNT) {
+    Acc v[4];
+    tagged::read_n<Acc, 4>(pub, i, kNT, (int)ceil_div((int)gridDim.x - i, kNT), v, epoch, flags + 1);
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+      if (i + k * kNT < (int)gridDim.x) t += v[k];
+  }
   t = block_sum<kNT>(t, red);
   if (threadIdx.x == 0) *result = fin(t);
 }
@@ -271,12 +284,10 @@ void launch_reduce(Ctx& c, int64_t n, const T* x, int64_t incx, const T* y, int6
   const bool vec = incx == 1 && aligned16(x) && (!Term::ry || (incy == 1 && aligned16(y)));
   int64_t blocks = std::max<int64_t>(1, ceil_div(n, (int64_t)kNT * U * VN));
   blocks = std::min<int64_t>(blocks, (int64_t)c.num_cus * 2);
-  Acc* partials = static_cast<Acc*>(c.scratch(sizeof(Acc) * blocks));
+  unsigned epoch = 0;
+  unsigned* flags = c.epoch_flags((size_t)(kReduceFlagBase + tagged::words<Acc>(blocks)), &epoch);
   hipLaunchKernelGGL((reduce_kernel<T, Term, Out, U>), dim3((unsigned)blocks), dim3(kNT), 0, c.stream, n, x, incx,
-                     y, incy, vec ? 1 : 0, partials, Term{});
-  EXO_LAUNCH_CHECK();
-  hipLaunchKernelGGL((reduce_finish_kernel<Acc, Out>), dim3(1), dim3(kNT), 0, c.stream, (int)blocks, partials, result,
-                     Finish<Acc, Out>{add});
+                     y, incy, vec ? 1 : 0, c.counters(), flags, epoch, result, Finish<Acc, Out>{add}, Term{});
   EXO_LAUNCH_CHECK();
 }
 

@@ -13,6 +13,7 @@
 // trmv    the triangular gemv-N / gemv-T kernels over a copy of x.
 #include "device.hpp"
 #include "ops.hpp"
+#include "tagged.hpp"
 
 namespace exoblas {
 namespace {
@@ -231,11 +232,16 @@ __global__ __launch_bounds__(kNT) void gemv_finish_kernel(int64_t M, int splits,
 
 // y[c] = beta*y[c] + alpha * sum_r A[r,c] x[r], A is Nr x Mc (rows x cols).
 // Workgroup (bx, by): columns [bx*64*VN, +64*VN), rows [by*rows_per, +rows_per).
+// With row splits (gridDim.y > 1) each workgroup publishes its column sums
+// as tagged words in `pub` (tagged.hpp) and the last of a column block's
+// workgroups (ticket tickets[blockIdx.x]) adds the splits in order 0, 1, ...
+// and writes y; *err is the timeout word.
 template <class T, int RU, bool VEC, int TRI, bool UNIT>
 __global__ __launch_bounds__(kNT) void gemv_t_kernel(int64_t Mc, int64_t Nr, T alpha, T beta,
                                                      const T* __restrict__ A, int64_t lda,
                                                      const T* __restrict__ x, int64_t incx, T* y,
-                                                     int64_t incy, int64_t rows_per, T* part) {
+                                                     int64_t incy, int64_t rows_per, unsigned* tickets,
+                                                     unsigned* err, unsigned* pub, unsigned epoch) {
   using V = typename Vec16<T>::type;
   constexpr int VN = Vec16<T>::n;
   constexpr int NW = kNT / kWave;
@@ -310,22 +316,46 @@ __global__ __launch_bounds__(kNT) void gemv_t_kernel(int64_t Mc, int64_t Nr, T a
 #pragma unroll
   for (int k = 0; k < VN; ++k) red[wave][lane * VN + k] = acc[k];
   __syncthreads();
+  auto write_y = [&](int64_t c, T s) {
+    if (TRI != kFull)
+      y[c * incy] = UNIT ? s + x[c * incx] : s;
+    else
+      y[c * incy] = beta * y[c * incy] + alpha * s;
+  };
   for (int i = threadIdx.x; i < kWave * VN; i += kNT) {
     T s = T(0);
 #pragma unroll
     for (int w = 0; w < NW; ++w) s += red[w][i];
     const int64_t c = cb0 + i;
     if (c < Mc) {
-      if (gridDim.y > 1) {
-        part[(int64_t)blockIdx.y * Mc + c] = s;
-      } else if (TRI != kFull) {
-        y[c * incy] = UNIT ? s + x[c * incx] : s;
-      } else {
-        y[c * incy] = beta * y[c * incy] + alpha * s;
-      }
+      if (gridDim.y > 1)
+        tagged::publish<T>(pub, (int64_t)blockIdx.y * Mc + c, s, epoch);
+      else
+        write_y(c, s);
     }
   }
+  if (gridDim.y == 1) return;
+  __shared__ int s_last;
+  __syncthreads();
+  if (threadIdx.x == 0) s_last = tagged::last_ticket(tickets + blockIdx.x, gridDim.y);
+  __syncthreads();
+  if (!s_last) return;
+  const int splits = (int)gridDim.y;
+  for (int i = threadIdx.x; i < kWave * VN; i += kNT) {
+    const int64_t c = cb0 + i;
+    if (c >= Mc) break;
+    T t = T(0);
+    for (int s0 = 0; s0 < splits; s0 += 8) {
+      T v[8];
+      tagged::read_n<T, 8>(pub, (int64_t)s0 * Mc + c, Mc, splits - s0, v, epoch, err);
+#pragma unroll
+      for (int k = 0; k < 8; ++k)
+        if (s0 + k < splits) t += v[k];
+    }
+    write_y(c, t);
+  }
 }
+
 
 template <class T, int TRI, bool UNIT>
 void launch_gemv_n(Ctx& c, int64_t M, int64_t N, T alpha, T beta, const T* A, int64_t lda, const T* x,
@@ -386,20 +416,25 @@ void launch_gemv_t(Ctx& c, int64_t Mc, int64_t Nr, T alpha, T beta, const T* A, 
   int64_t splits = std::max<int64_t>(1, std::min<int64_t>(ceil_div(want, col_blocks), ceil_div(Nr, 64)));
   const int64_t rows_per = Nr > 0 ? ceil_div(ceil_div(Nr, splits), 32) * 32 : 32;
   splits = std::max<int64_t>(1, ceil_div(Nr, rows_per));
-  T* part = splits > 1 ? static_cast<T*>(c.scratch(sizeof(T) * splits * Mc)) : nullptr;
+  // splits > 1 only when col_blocks < want <= Ctx::kCounters: one ticket per
+  // column block from the stream's counters
+  unsigned epoch = 0;
+  unsigned *tickets = nullptr, *err = nullptr, *pub = nullptr;
+  if (splits > 1) {
+    if (col_blocks > Ctx::kCounters) throw Error(kInvalidSize, "gemv: too many column blocks for split rows");
+    tickets = c.counters();
+    unsigned* flags = c.epoch_flags((size_t)(2 + tagged::words<T>(splits * Mc)), &epoch);
+    err = flags + 1;
+    pub = flags + 2;  // 8-byte aligned
+  }
   dim3 grid((unsigned)col_blocks, (unsigned)splits);
   if (vec)
     hipLaunchKernelGGL((gemv_t_kernel<T, RU, true, TRI, UNIT>), grid, dim3(kNT), 0, c.stream, Mc, Nr, alpha, beta,
-                       A, lda, x, incx, y, incy, rows_per, part);
+                       A, lda, x, incx, y, incy, rows_per, tickets, err, pub, epoch);
   else
     hipLaunchKernelGGL((gemv_t_kernel<T, RU, false, TRI, UNIT>), grid, dim3(kNT), 0, c.stream, Mc, Nr, alpha,
-                       beta, A, lda, x, incx, y, incy, rows_per, part);
+                       beta, A, lda, x, incx, y, incy, rows_per, tickets, err, pub, epoch);
   EXO_LAUNCH_CHECK();
-  if (splits > 1) {
-    hipLaunchKernelGGL((gemv_finish_kernel<T, TRI != kFull, UNIT>), dim3((unsigned)ceil_div(Mc, kWave)), dim3(kNT), 0,
-                       c.stream, Mc, (int)splits, alpha, beta, part, x, incx, y, incy);
-    EXO_LAUNCH_CHECK();
-  }
 }
 
 // ------------------------------------------------------------ rank updates

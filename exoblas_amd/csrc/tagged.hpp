@@ -1,0 +1,114 @@
+// Tagged publication between the workgroups of one launch.
+//
+// Every 32 bits of a published value travel with the call's marker in one
+// 64-bit word (marker << 32 | bits), stored and polled with relaxed
+// device-scope atomics: a reader that sees the marker has those bits, so no
+// fence or separate ready flag is needed.  One word per FP32 value, two (low
+// and high halves) per FP64 value.  Markers are the per-stream call epochs
+// of Ctx::epoch_flags, so words left by earlier calls never match.
+//
+// Waits are bounded: past kSpinLimit polls the reader records its marker in
+// *err and goes on, so no grid can hang the device.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace exoblas {
+namespace tagged {
+
+constexpr long long kSpinLimit = 1ll << 22;
+
+// 32-bit units of publication space for n values of T.
+template <class T>
+__host__ __device__ inline int64_t words(int64_t n) {
+  return 2 * n * (int64_t)(sizeof(T) / 4);
+}
+
+// Polls one tagged word until it carries `marker` (bounded) and returns its
+// low 32 bits.
+__device__ __forceinline__ unsigned poll_word(uint64_t* w64, unsigned marker, unsigned* err) {
+  uint64_t w;
+  long long it = 0;
+  while (((w = __hip_atomic_load(w64, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) >> 32) != marker) {
+    if (++it > kSpinLimit) {
+      __hip_atomic_store(err, marker, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      break;
+    }
+    __builtin_amdgcn_s_sleep(1);
+  }
+  return (unsigned)w;
+}
+
+// Value g of the publication area `pub` (8-byte aligned), once published.
+template <class T>
+__device__ __forceinline__ T read(unsigned* pub, int64_t g, unsigned marker, unsigned* err) {
+  uint64_t* w64 = reinterpret_cast<uint64_t*>(pub);
+  if constexpr (sizeof(T) == 4) {
+    return __uint_as_float(poll_word(w64 + g, marker, err));
+  } else {
+    const unsigned lo = poll_word(w64 + 2 * g, marker, err);
+    const unsigned hi = poll_word(w64 + 2 * g + 1, marker, err);
+    return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+  }
+}
+
+// Values g0, g0 + stride, ..., g0 + (K-1) stride of `pub` (those with index
+// k < count) into v[k]: all words are loaded before any is checked, so the
+// K reads overlap instead of forming a chain of memory round trips.
+template <class T, int K>
+__device__ __forceinline__ void read_n(unsigned* pub, int64_t g0, int64_t stride, int count, T (&v)[K],
+                                       unsigned marker, unsigned* err) {
+  uint64_t* w64 = reinterpret_cast<uint64_t*>(pub);
+  constexpr int W = sizeof(T) / 4;  // words per value
+  uint64_t w[K][W];
+#pragma unroll
+  for (int k = 0; k < K; ++k)
+#pragma unroll
+    for (int h = 0; h < W; ++h)
+      if (k < count) w[k][h] = __hip_atomic_load(w64 + W * (g0 + k * stride) + h, __ATOMIC_RELAXED,
+                                                 __HIP_MEMORY_SCOPE_AGENT);
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    if (k >= count) continue;
+    unsigned lo[W];
+#pragma unroll
+    for (int h = 0; h < W; ++h)
+      lo[h] = (w[k][h] >> 32) == marker ? (unsigned)w[k][h]
+                                        : poll_word(w64 + W * (g0 + k * stride) + h, marker, err);
+    if constexpr (W == 1)
+      v[k] = __uint_as_float(lo[0]);
+    else
+      v[k] = __longlong_as_double((long long)(((uint64_t)lo[1] << 32) | lo[0]));
+  }
+}
+
+// Publishes value g of `pub`.
+template <class T>
+__device__ __forceinline__ void publish(unsigned* pub, int64_t g, T v, unsigned marker) {
+  uint64_t* w64 = reinterpret_cast<uint64_t*>(pub);
+  const uint64_t tag = (uint64_t)marker << 32;
+  if constexpr (sizeof(T) == 4) {
+    __hip_atomic_store(w64 + g, tag | __float_as_uint(v), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  } else {
+    const uint64_t bits = (uint64_t)__double_as_longlong(v);
+    __hip_atomic_store(w64 + 2 * g, tag | (bits & 0xffffffffu), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(w64 + 2 * g + 1, tag | (bits >> 32), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
+// Takes a ticket from counter *t shared by `count` workgroups; true for the
+// last one, which also resets the counter for the next call on the stream.
+// Call from one thread.
+__device__ __forceinline__ bool last_ticket(unsigned* t, unsigned count) {
+  const unsigned p = atomicAdd(t, 1u);
+  if (p == count - 1) {
+    atomicExch(t, 0u);
+    return true;
+  }
+  return false;
+}
+
+}  // namespace tagged
+}  // namespace exoblas

@@ -13,11 +13,7 @@
 //      diagonal, each after the step that owns it has published its part of x;
 //   3. forms v = b - (panel . x) and x_block = inv(D) v with all its threads,
 //      so the serial part of the chain is one small mat-vec per block.
-// Publication: every 32 bits of a value travel with the call's marker in one
-// 64-bit word (marker << 32 | bits), stored and polled with relaxed
-// device-scope atomics, so a reader that sees the marker has those bits and
-// no fence or separate flag is needed: one word per FP32 row, two (low and
-// high halves) per FP64 row.
+// Publication of x: the tagged words of tagged.hpp, one value per row.
 // Waits are bounded: past the spin limit the step records `marker` in *err
 // and finishes.
 #pragma once
@@ -26,12 +22,13 @@
 
 #include <cstdint>
 
+#include "tagged.hpp"
+
 namespace exoblas {
 namespace trsv {
 
 constexpr int kB = 64;    // default rows per block (batched and mixed solves)
 constexpr int kNT = 256;  // threads per workgroup at kB
-constexpr long long kSpinLimit = 1ll << 22;
 
 // TI: precision of the diagonal-block inverse and of the block mat-vec.
 // KB: rows per block; a block's workgroup has 4 * KB threads (4 per row).
@@ -46,22 +43,7 @@ struct Smem {
 // 32-bit words of publication space a solve of order n needs (see above).
 template <class T>
 __host__ __device__ inline int64_t pub_words(int64_t n) {
-  return 2 * n * (int64_t)(sizeof(T) / 4);
-}
-
-// Polls one tagged word until it carries `marker` (bounded) and returns its
-// low 32 bits.
-__device__ __forceinline__ unsigned poll_word(uint64_t* w64, unsigned marker, unsigned* err) {
-  uint64_t w;
-  long long it = 0;
-  while (((w = __hip_atomic_load(w64, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) >> 32) != marker) {
-    if (++it > kSpinLimit) {
-      __hip_atomic_store(err, marker, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      break;
-    }
-    __builtin_amdgcn_s_sleep(1);
-  }
-  return (unsigned)w;
+  return tagged::words<T>(n);
 }
 
 // Wave 0 of the workgroup fetches the published x block bj into xs.
@@ -71,18 +53,7 @@ __device__ __forceinline__ void fetch_block(int64_t n, int64_t bj, unsigned* pub
   const int t = threadIdx.x;
   if (t < KB) {
     const int64_t g = bj * KB + t;
-    T v = T(0);
-    if (g < n) {
-      uint64_t* w64 = reinterpret_cast<uint64_t*>(pub);
-      if constexpr (sizeof(T) == 4) {
-        v = __uint_as_float(poll_word(w64 + g, marker, err));
-      } else {
-        const unsigned lo = poll_word(w64 + 2 * g, marker, err);
-        const unsigned hi = poll_word(w64 + 2 * g + 1, marker, err);
-        v = __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
-      }
-    }
-    xs[t] = v;
+    xs[t] = g < n ? tagged::read<T>(pub, g, marker, err) : T(0);
   }
 }
 
@@ -90,15 +61,7 @@ __device__ __forceinline__ void fetch_block(int64_t n, int64_t bj, unsigned* pub
 template <class T>
 __device__ __forceinline__ void publish_value(int64_t g, T v, T* x, unsigned* pub, unsigned marker) {
   x[g] = v;
-  uint64_t* w64 = reinterpret_cast<uint64_t*>(pub);
-  const uint64_t tag = (uint64_t)marker << 32;
-  if constexpr (sizeof(T) == 4) {
-    __hip_atomic_store(w64 + g, tag | __float_as_uint(v), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  } else {
-    const uint64_t bits = (uint64_t)__double_as_longlong(v);
-    __hip_atomic_store(w64 + 2 * g, tag | (bits & 0xffffffffu), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __hip_atomic_store(w64 + 2 * g + 1, tag | (bits >> 32), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  }
+  tagged::publish<T>(pub, g, v, marker);
 }
 
 template <class T, bool TRANS>
