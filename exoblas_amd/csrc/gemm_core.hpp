@@ -303,20 +303,12 @@ __device__ __forceinline__ void tile_store(typename C::type* Cp, int64_t ldc, in
   }
 }
 
-// Block id -> (tile_m, tile_n): blocks are dealt round-robin over the 8 XCDs,
-// so consecutive ids are first regrouped per XCD, then walked in groups of
-// GROUP_M tile rows so that the tiles one XCD runs together share A row panels
-// and B column panels in its L2.
-__device__ __forceinline__ void tile_coords(int64_t bid, int64_t nblocks, int64_t tiles_m, int64_t tiles_n,
-                                            int64_t& tm, int64_t& tn) {
-  constexpr int kXcds = 8, kGroupM = 8;
-  if (nblocks % kXcds == 0) bid = (bid % kXcds) * (nblocks / kXcds) + bid / kXcds;
-  const int64_t per_group = (int64_t)kGroupM * tiles_n;
-  const int64_t group = bid / per_group;
-  const int64_t first_m = group * kGroupM;
-  const int64_t gsz = tiles_m - first_m < kGroupM ? tiles_m - first_m : kGroupM;
-  tm = first_m + (bid % per_group) % gsz;
-  tn = (bid % per_group) / gsz;
+// Workgroup ids are dealt round-robin over the 8 XCDs; this regroups the ids
+// of an nblocks grid so that each XCD runs a contiguous share of the tile
+// list (identity when nblocks is not a multiple of 8).
+__host__ __device__ inline int64_t xcd_regroup(int64_t bid, int64_t nblocks) {
+  constexpr int kXcds = 8;
+  return nblocks % kXcds == 0 ? (bid % kXcds) * (nblocks / kXcds) + bid / kXcds : bid;
 }
 
 // Tiles of row tm that meet the TRI triangle (lower: n0 <= m0 + BM - 1,
@@ -329,7 +321,7 @@ __host__ __device__ inline void tri_row(int tri, int64_t tm, int64_t tiles_n, in
   count = last >= first ? last - first + 1 : 0;
 }
 
-// Workgroup tiles of a TRI-restricted product (tri_tile_coords' list length).
+// Length of the tile list of a TRI-restricted product (list_tile's domain).
 __host__ __device__ inline int64_t tri_tile_count(int tri, int64_t tiles_m, int64_t tiles_n, int BM, int BN) {
   if (tri == kFull) return tiles_m * tiles_n;
   int64_t n = 0, first, count;
@@ -340,25 +332,46 @@ __host__ __device__ inline int64_t tri_tile_count(int tri, int64_t tiles_m, int6
   return n;
 }
 
-// Block id -> (tile_m, tile_n) over only the tiles that meet the triangle,
-// listed row by row, ids first regrouped per XCD as in tile_coords so every
-// XCD gets a contiguous, equal share.  (Launching the full grid and letting
-// off-triangle workgroups exit leaves whole XCDs idle: with the XCD-major
-// order their shares lie entirely on one side of the diagonal.)
-__device__ __forceinline__ void tri_tile_coords(int tri, int64_t bid, int64_t nblocks, int64_t tiles_m,
-                                                int64_t tiles_n, int BM, int BN, int64_t& tm, int64_t& tn) {
-  constexpr int kXcds = 8;
-  if (nblocks % kXcds == 0) bid = (bid % kXcds) * (nblocks / kXcds) + bid / kXcds;
+// Position idx of the tile list -> (tile_m, tile_n).  Full products walk the
+// tiles in groups of GROUP_M tile rows, column by column inside a group, so
+// the tiles one XCD runs together share A row panels and B column panels in
+// its L2.  Triangular updates list only the tiles that meet the triangle, row
+// by row (launching the full grid and letting off-triangle workgroups exit
+// leaves whole XCDs idle: with the XCD-major order their shares lie entirely
+// on one side of the diagonal).  Past the end of the list tn = tiles_n, which
+// the callers' bounds checks skip.
+__host__ __device__ inline void list_tile(int tri, int64_t idx, int64_t tiles_m, int64_t tiles_n, int BM, int BN,
+                                          int64_t& tm, int64_t& tn) {
+  if (tri == kFull) {
+    constexpr int kGroupM = 8;
+    const int64_t per_group = (int64_t)kGroupM * tiles_n;
+    const int64_t group = idx / per_group;
+    const int64_t first_m = group * kGroupM;
+    const int64_t gsz = tiles_m - first_m < kGroupM ? tiles_m - first_m : kGroupM;
+    tm = first_m + (idx % per_group) % gsz;
+    tn = (idx % per_group) / gsz;
+    return;
+  }
   int64_t acc = 0, first, count;
   for (tm = 0; tm < tiles_m; ++tm) {
     tri_row(tri, tm, tiles_n, BM, BN, first, count);
-    if (bid < acc + count) {
-      tn = first + (bid - acc);
+    if (idx < acc + count) {
+      tn = first + (idx - acc);
       return;
     }
     acc += count;
   }
-  tn = tiles_n;  // past the list: the caller's bounds check skips it
+  tn = tiles_n;
+}
+
+// Block id of an nblocks grid that covers the whole list -> tile.
+__device__ __forceinline__ void tile_coords(int64_t bid, int64_t nblocks, int64_t tiles_m, int64_t tiles_n,
+                                            int64_t& tm, int64_t& tn) {
+  list_tile(kFull, xcd_regroup(bid, nblocks), tiles_m, tiles_n, 1, 1, tm, tn);
+}
+__device__ __forceinline__ void tri_tile_coords(int tri, int64_t bid, int64_t nblocks, int64_t tiles_m,
+                                                int64_t tiles_n, int BM, int BN, int64_t& tm, int64_t& tn) {
+  list_tile(tri, xcd_regroup(bid, nblocks), tiles_m, tiles_n, BM, BN, tm, tn);
 }
 
 }  // namespace gemm

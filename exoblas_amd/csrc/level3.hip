@@ -15,18 +15,17 @@
 namespace exoblas {
 namespace gemm {
 
+// Tiles [tile0, tile0 + gridDim.x) of the tile list (list_tile), one per
+// workgroup.
 template <class C, int TRI>
 __global__ __launch_bounds__(C::NT) void gemm_kernel(int64_t M, int64_t N, int64_t K, typename C::type alpha,
                                                      Operand<typename C::type> A, Operand<typename C::type> B,
                                                      typename C::type* Cp, int64_t ldc, int cvec, int64_t tiles_m,
-                                                     int64_t tiles_n) {
+                                                     int64_t tiles_n, int64_t tile0) {
   using T = typename C::type;
   extern __shared__ __align__(16) unsigned char smem[];
   int64_t tm, tn;
-  if (TRI == kFull)
-    tile_coords(blockIdx.x, gridDim.x, tiles_m, tiles_n, tm, tn);
-  else
-    tri_tile_coords(TRI, blockIdx.x, gridDim.x, tiles_m, tiles_n, C::BM, C::BN, tm, tn);
+  list_tile(TRI, tile0 + xcd_regroup(blockIdx.x, gridDim.x), tiles_m, tiles_n, C::BM, C::BN, tm, tn);
   if (tm >= tiles_m || tn >= tiles_n) return;
   const int64_t m0 = tm * C::BM, n0 = tn * C::BN;
   typename Mfma<T>::acc acc[C::SM][C::SN];
@@ -40,23 +39,21 @@ __global__ __launch_bounds__(C::NT) void gemm_kernel(int64_t M, int64_t N, int64
   tile_store<C, TRI>(Cp, ldc, cvec, M, N, m0, n0, alpha, acc);
 }
 
-// Split-K: workgroup (s, tile) runs the K slice [s * ksplit, (s + 1) * ksplit)
-// of its tile and writes the raw sums to P[s] (M x N, leading dimension N);
-// splitk_reduce_kernel then adds alpha * sum_s P[s] to C in a fixed order.
+// Split-K over tiles [tile0, tile0 + ntiles) of the list: workgroup
+// s * ntiles + l runs the K slice [s * ksplit, (s + 1) * ksplit) of tile
+// tile0 + l and writes its raw sums as a dense BM x BN block
+// P[(s * ntiles + l) * BM * BN ...]; splitk_reduce_kernel then adds
+// alpha * sum_s of the blocks to C in a fixed order.
 template <class C, int TRI>
 __global__ __launch_bounds__(C::NT) void gemm_splitk_kernel(int64_t M, int64_t N, int64_t K, int64_t ksplit,
                                                             Operand<typename C::type> A, Operand<typename C::type> B,
-                                                            typename C::type* P, int pvec, int64_t tiles_m,
-                                                            int64_t tiles_n) {
+                                                            typename C::type* P, int64_t tiles_m, int64_t tiles_n,
+                                                            int64_t tile0, int64_t ntiles) {
   using T = typename C::type;
   extern __shared__ __align__(16) unsigned char smem[];
-  const int64_t tiles = tri_tile_count(TRI, tiles_m, tiles_n, C::BM, C::BN);
-  const int64_t s = blockIdx.x / tiles;
+  const int64_t s = blockIdx.x / ntiles, l = blockIdx.x % ntiles;
   int64_t tm, tn;
-  if (TRI == kFull)
-    tile_coords(blockIdx.x % tiles, tiles, tiles_m, tiles_n, tm, tn);
-  else
-    tri_tile_coords(TRI, blockIdx.x % tiles, tiles, tiles_m, tiles_n, C::BM, C::BN, tm, tn);
+  list_tile(TRI, tile0 + l, tiles_m, tiles_n, C::BM, C::BN, tm, tn);
   if (tm >= tiles_m || tn >= tiles_n) return;
   const int64_t m0 = tm * C::BM, n0 = tn * C::BN;
   typename Mfma<T>::acc acc[C::SM][C::SN];
@@ -68,59 +65,86 @@ __global__ __launch_bounds__(C::NT) void gemm_splitk_kernel(int64_t M, int64_t N
       for (int q = 0; q < 4; ++q) acc[a][b][q] = T(0);
   const int64_t kb = s * ksplit, ke = kb + ksplit < K ? kb + ksplit : K;
   tile_mma<C>(A, B, M, N, K, m0, n0, kb, ke, acc, reinterpret_cast<T*>(smem));
-  tile_store<C, kFull, true>(P + s * M * N, N, pvec, M, N, m0, n0, T(1), acc, T(0));
+  tile_store<C, kFull, true>(P + (s * ntiles + l) * C::BM * C::BN, C::BN, 1, C::BM, C::BN, 0, 0, T(1), acc, T(0));
 }
 
-template <class T, int TRI>
-__global__ __launch_bounds__(256) void splitk_reduce_kernel(int64_t M, int64_t N, int S, T alpha, const T* P,
-                                                            T* Cp, int64_t ldc) {
-  const int64_t i = blockIdx.y, j = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (j >= N || (TRI == kLower && j > i) || (TRI == kUpper && j < i)) return;
-  T sum = T(0);
-  for (int k = 0; k < S; ++k) sum += P[(int64_t)k * M * N + i * N + j];
-  Cp[i * ldc + j] += alpha * sum;
+// Workgroup (l, r) adds rows [r * 256 / (BN / VN), ...) of tile tile0 + l:
+// each thread one 16-byte column group of one row, summed over the S slices
+// in order.
+template <class T, int TRI, int BM, int BN>
+__global__ __launch_bounds__(256) void splitk_reduce_kernel(int64_t M, int64_t N, int S, int64_t tiles_m,
+                                                            int64_t tiles_n, int64_t tile0, int64_t ntiles, T alpha,
+                                                            const T* __restrict__ P, T* Cp, int64_t ldc) {
+  using V = typename Vec16<T>::type;
+  constexpr int VN = Vec16<T>::n, GPR = BN / VN, RPW = 256 / GPR;  // groups per row, rows per workgroup
+  const int64_t l = blockIdx.x;
+  int64_t tm, tn;
+  list_tile(TRI, tile0 + l, tiles_m, tiles_n, BM, BN, tm, tn);
+  if (tm >= tiles_m || tn >= tiles_n) return;
+  const int r = blockIdx.y * RPW + threadIdx.x / GPR, g = threadIdx.x % GPR;
+  const int64_t row = tm * BM + r, col0 = tn * BN + g * VN;
+  if (r >= BM || row >= M) return;
+  T sum[VN];
+#pragma unroll
+  for (int e = 0; e < VN; ++e) sum[e] = T(0);
+  const T* p = P + l * BM * BN + (int64_t)r * BN + g * VN;
+  for (int k = 0; k < S; ++k) {
+    const V v = *reinterpret_cast<const V*>(p + (int64_t)k * ntiles * BM * BN);
+#pragma unroll
+    for (int e = 0; e < VN; ++e) sum[e] += vget<T>(v, e);
+  }
+#pragma unroll
+  for (int e = 0; e < VN; ++e) {
+    const int64_t col = col0 + e;
+    if (col >= N || (TRI == kLower && col > row) || (TRI == kUpper && col < row)) continue;
+    Cp[row * ldc + col] += alpha * sum[e];
+  }
 }
 
 template <class C, int TRI>
-void launch_splitk(Ctx& c, int64_t S, int64_t M, int64_t N, int64_t K, typename C::type alpha,
-                   const Operand<typename C::type>& A, const Operand<typename C::type>& B, typename C::type* Cp,
-                   int64_t ldc) {
+void set_lds_attr(const void* fn) {
+  EXO_HIP_CHECK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)C::LDS_BYTES));
+}
+
+// Split-K over tiles [tile0, tile0 + ntiles) of the list, S slices.
+template <class C, int TRI>
+void launch_splitk(Ctx& c, int64_t S, int64_t tile0, int64_t ntiles, int64_t M, int64_t N, int64_t K,
+                   typename C::type alpha, const Operand<typename C::type>& A, const Operand<typename C::type>& B,
+                   typename C::type* Cp, int64_t ldc) {
   using T = typename C::type;
   const int64_t tm = ceil_div(M, C::BM), tn = ceil_div(N, C::BN);
   const int64_t ksplit = ceil_div(ceil_div(K, S), C::BK) * C::BK;
   S = ceil_div(K, ksplit);
-  T* P = static_cast<T*>(c.scratch(sizeof(T) * (size_t)(S * M * N), 4));
-  const int pvec = (N % C::EB == 0) ? 1 : 0;
-  static bool attr_set = false;
+  T* P = static_cast<T*>(c.scratch(sizeof(T) * (size_t)(S * ntiles * C::BM * C::BN), 4));
+  static bool attr_set = false;  // dynamic LDS above 64 KiB needs the attribute
   if (!attr_set) {
-    EXO_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&gemm_splitk_kernel<C, TRI>),
-                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)C::LDS_BYTES));
+    set_lds_attr<C, TRI>(reinterpret_cast<const void*>(&gemm_splitk_kernel<C, TRI>));
     attr_set = true;
   }
-  hipLaunchKernelGGL((gemm_splitk_kernel<C, TRI>), dim3((unsigned)(S * tri_tile_count(TRI, tm, tn, C::BM, C::BN))),
-                     dim3(C::NT), C::LDS_BYTES, c.stream,
-                     M, N, K, ksplit, A, B, P, pvec, tm, tn);
+  hipLaunchKernelGGL((gemm_splitk_kernel<C, TRI>), dim3((unsigned)(S * ntiles)), dim3(C::NT), C::LDS_BYTES,
+                     c.stream, M, N, K, ksplit, A, B, P, tm, tn, tile0, ntiles);
   EXO_LAUNCH_CHECK();
-  hipLaunchKernelGGL((splitk_reduce_kernel<T, TRI>), dim3((unsigned)ceil_div(N, 256), (unsigned)M), dim3(256), 0,
-                     c.stream, M, N, (int)S, alpha, P, Cp, ldc);
+  constexpr int RPW = 256 / (C::BN / Vec16<T>::n);
+  hipLaunchKernelGGL((splitk_reduce_kernel<T, TRI, C::BM, C::BN>), dim3((unsigned)ntiles, (unsigned)ceil_div(C::BM, RPW)),
+                     dim3(256), 0, c.stream, M, N, (int)S, tm, tn, tile0, ntiles, alpha, P, Cp, ldc);
   EXO_LAUNCH_CHECK();
 }
 
+// Tiles [tile0, tile0 + ntiles) of the list, one workgroup each.
 template <class C, int TRI>
-void launch_cfg(Ctx& c, int64_t M, int64_t N, int64_t K, typename C::type alpha,
+void launch_cfg(Ctx& c, int64_t tile0, int64_t ntiles, int64_t M, int64_t N, int64_t K, typename C::type alpha,
                 const Operand<typename C::type>& A, const Operand<typename C::type>& B, typename C::type* Cp,
                 int64_t ldc) {
   using T = typename C::type;
   const int64_t tm = ceil_div(M, C::BM), tn = ceil_div(N, C::BN);
   const int cvec = (ldc % C::EB == 0) && (reinterpret_cast<uintptr_t>(Cp) % (C::EB * sizeof(T)) == 0);
-  static bool attr_set = false;  // dynamic LDS above 64 KiB needs the attribute
+  static bool attr_set = false;
   if (!attr_set) {
-    EXO_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&gemm_kernel<C, TRI>),
-                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)C::LDS_BYTES));
+    set_lds_attr<C, TRI>(reinterpret_cast<const void*>(&gemm_kernel<C, TRI>));
     attr_set = true;
   }
-  hipLaunchKernelGGL((gemm_kernel<C, TRI>), dim3((unsigned)tri_tile_count(TRI, tm, tn, C::BM, C::BN)), dim3(C::NT),
-                     C::LDS_BYTES, c.stream, M, N, K, alpha, A, B, Cp, ldc, cvec, tm, tn);
+  hipLaunchKernelGGL((gemm_kernel<C, TRI>), dim3((unsigned)ntiles), dim3(C::NT), C::LDS_BYTES, c.stream, M, N, K,
+                     alpha, A, B, Cp, ldc, cvec, tm, tn, tile0);
   EXO_LAUNCH_CHECK();
 }
 
@@ -147,26 +171,81 @@ int64_t tile_count(int tri, int64_t M, int64_t N) {
   return tri_tile_count(tri, ceil_div(M, C::BM), ceil_div(N, C::BN), C::BM, C::BN);
 }
 
+// One configuration's launches: tiles [0, dp) one workgroup each, the rest
+// split S ways along K (S = 1: dp is every tile).
+template <class C, int TRI>
+void launch_plan(Ctx& c, int64_t dp, int64_t S, int64_t M, int64_t N, int64_t K, typename C::type alpha,
+                 const Operand<typename C::type>& A, const Operand<typename C::type>& B, typename C::type* Cp,
+                 int64_t ldc) {
+  const int64_t total = tile_count<C>(TRI, M, N);
+  if (dp > 0) launch_cfg<C, TRI>(c, 0, dp, M, N, K, alpha, A, B, Cp, ldc);
+  if (dp < total) launch_splitk<C, TRI>(c, S, dp, total - dp, M, N, K, alpha, A, B, Cp, ldc);
+}
+
+// Split factor in [2, smax] that fills the last round of S * tiles units
+// over P CUs best (the smallest on ties); 0 when no S >= 2 fills at least
+// `need` of it.
+inline int64_t best_split(int64_t tiles, int64_t P, int64_t smax, double need) {
+  int64_t best = 0;
+  double best_fill = need;
+  for (int64_t S = 2; S <= smax; ++S) {
+    const int64_t units = S * tiles;
+    const double fill = (double)units / (double)(ceil_div(units, P) * P);
+    if (fill > best_fill + 1e-9) {
+      best = S;
+      best_fill = fill;
+    }
+  }
+  return best;
+}
+
+// Tiles [0, dp) of C's tile list run whole, the remaining ones split along K
+// into the stream's free CUs: the full rounds keep every CU busy and the
+// last, partial round is spread over all of them.
+template <class C, int TRI>
+void launch_rounds(Ctx& c, int64_t M, int64_t N, int64_t K, typename C::type alpha,
+                   const Operand<typename C::type>& A, const Operand<typename C::type>& B, typename C::type* Cp,
+                   int64_t ldc) {
+  const int64_t total = tile_count<C>(TRI, M, N), P = c.num_cus;
+  const int64_t dp = total / P * P, rem = total - dp;
+  const int64_t S = rem ? std::min<int64_t>({P / rem, K / (4 * C::BK), 16}) : 1;
+  if (S < 2)
+    launch_plan<C, TRI>(c, total, 1, M, N, K, alpha, A, B, Cp, ldc);
+  else
+    launch_plan<C, TRI>(c, dp, S, M, N, K, alpha, A, B, Cp, ldc);
+}
+
+// Plan choice, from measurements on MI355X (FP32 TFLOP/s, old rule -> this):
+// the largest tiles sustain about 80 % of the MFMA peak even at one tile per
+// CU (ssymm 4096^2: 105.8 with 128 x 128 tiles -> 125.8), so they run
+// whenever they make a full round; a partial last round is split along K
+// (ssyrk 8192 x 2048 84.2 -> 111.5, dsyrk 4096 x 1024 34.4 -> 48.5 FP64).
+// Fewer large tiles than CUs: 128 x 128 tiles when they give two per CU
+// (ssyrk 4096 x 1024 71.1 -> 84.8), else the large tiles split along K when
+// that fills the rounds (sgemm 2048^3 81.2 -> 100.2), else 128 x 128 tiles,
+// else 64 x 64 tiles split along K for small outputs with a long K (sgemm
+// 512^3).
 template <class T, int TRI>
 void run(Ctx& c, int64_t M, int64_t N, int64_t K, T alpha, const Operand<T>& A, const Operand<T>& B, T* Cp,
          int64_t ldc) {
   if (M <= 0 || N <= 0 || K <= 0) return;
-  // The largest tiles that still give every CU work (two tiles per CU for
-  // the largest, which run one workgroup per CU).
   using Tl = Tiles<T>;
-  // Few output tiles and a long K (sgemm 512^3: 64 tiles): split K so every
-  // CU gets several workgroups, each K slice at least 32 deep.
-  const int64_t small_tiles = tile_count<typename Tl::Small>(TRI, M, N);
-  if (small_tiles < c.num_cus && M <= 65535) {
-    const int64_t S = std::min<int64_t>({ceil_div((int64_t)c.num_cus, small_tiles), K / 128, 16});
-    if (S >= 2) return launch_splitk<typename Tl::Small, TRI>(c, S, M, N, K, alpha, A, B, Cp, ldc);
-  }
-  if (tile_count<typename Tl::Huge>(TRI, M, N) >= 2 * c.num_cus)
-    launch_cfg<typename Tl::Huge, TRI>(c, M, N, K, alpha, A, B, Cp, ldc);
-  else if (tile_count<typename Tl::Big>(TRI, M, N) >= c.num_cus)
-    launch_cfg<typename Tl::Big, TRI>(c, M, N, K, alpha, A, B, Cp, ldc);
-  else
-    launch_cfg<typename Tl::Small, TRI>(c, M, N, K, alpha, A, B, Cp, ldc);
+  using Huge = typename Tl::Huge;
+  using Big = typename Tl::Big;
+  using Small = typename Tl::Small;
+  const int64_t P = c.num_cus;
+  const int64_t huge = tile_count<Huge>(TRI, M, N), big = tile_count<Big>(TRI, M, N);
+  if (huge >= P) return launch_rounds<Huge, TRI>(c, M, N, K, alpha, A, B, Cp, ldc);
+  if (big >= 2 * P) return launch_rounds<Big, TRI>(c, M, N, K, alpha, A, B, Cp, ldc);
+  const int64_t sh = best_split(huge, P, std::min<int64_t>(16, K / (4 * Huge::BK)), 0.75);
+  if (sh >= 2) return launch_plan<Huge, TRI>(c, 0, sh, M, N, K, alpha, A, B, Cp, ldc);
+  if (big >= P) return launch_rounds<Big, TRI>(c, M, N, K, alpha, A, B, Cp, ldc);
+  // Few output tiles and a long K: split K so every CU gets a workgroup,
+  // each K slice at least 128 deep.
+  const int64_t small = tile_count<Small>(TRI, M, N);
+  const int64_t S = std::min<int64_t>({ceil_div(P, small), K / 128, 16});
+  if (S >= 2) return launch_plan<Small, TRI>(c, 0, S, M, N, K, alpha, A, B, Cp, ldc);
+  launch_rounds<Small, TRI>(c, M, N, K, alpha, A, B, Cp, ldc);
 }
 
 template <class T>
