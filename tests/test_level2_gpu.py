@@ -213,3 +213,20 @@ def test_gemv_bandwidth_size(C, dt, trans, rng):
     want = 1.5 * y + 0.5 * (opA @ x.astype(np.float64))
     mag = 1.5 * np.abs(y) + 0.5 * (np.abs(opA) @ np.abs(x.astype(np.float64)))
     assert_bound(dy.cpu().numpy(), want, mag, n, dt, c=1.0, what="gemv 8192")
+
+
+@pytest.mark.parametrize("dt", DT)
+@pytest.mark.parametrize("M,N,lda", [(4099, 3001, 3004), (4100, 5, 8), (5000, 4097, 4100), (4096, 8191, 8192)])
+def test_gemv_n_long_rows(C, dt, M, N, lda, rng):
+    """M >= 4096 rows with a contiguous x: whole-workgroup rows with x held in
+    registers, an odd row count and a partial last vector of columns."""
+    import torch
+    A = uniform(rng, (M, lda), dt)
+    x = uniform(rng, N, dt)
+    y = uniform(rng, M, dt)
+    dA, dx, dy = (torch.from_numpy(a).cuda() for a in (A, x, y))
+    fn(C, dt, "gemv")(RM, NO_TRANS, M, N, -0.75, dA, lda, dx, 1, 1.25, dy, 1)
+    A64 = A[:, :N].astype(np.float64)
+    want = 1.25 * y + -0.75 * (A64 @ x.astype(np.float64))
+    mag = 1.25 * np.abs(y) + 0.75 * (np.abs(A64) @ np.abs(x.astype(np.float64)))
+    assert_bound(dy.cpu().numpy(), want, mag, N, dt, c=1.0, what="gemv long rows")
