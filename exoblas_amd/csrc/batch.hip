@@ -114,7 +114,7 @@ __device__ __forceinline__ void gemv_item(const Desc& d, uint32_t local) {
 template <bool FWD, bool TRANS, bool UNIT>
 __device__ __forceinline__ void trsv_item(const Desc& d, uint32_t local, unsigned* ctrl, unsigned marker,
                                           trsv::Smem<float>& sm) {
-  trsv::step<float, FWD, TRANS, UNIT>((int)local, d.n, d.a, d.lda, d.c, ctrl + kCtrlWords + d.flag_off, marker,
+  trsv::step<float, FWD, TRANS, UNIT, trsv::kB, false>((int)local, d.n, d.a, d.lda, d.c, ctrl + kCtrlWords + d.flag_off, marker,
                                       ctrl + 3, sm);
 }
 

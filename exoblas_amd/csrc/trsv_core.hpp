@@ -179,7 +179,9 @@ __device__ __forceinline__ void invert_diag(int64_t n, const T* A, int64_t lda, 
 
 // One processing step p of the solve; called by all 4 * KB threads of a
 // workgroup.
-template <class T, bool FWD, bool TRANS, bool UNIT, int KB = kB>
+// PF: load each panel block one iteration ahead (a kernel that keeps other
+// work items in its registers passes false).
+template <class T, bool FWD, bool TRANS, bool UNIT, int KB = kB, bool PF = true>
 __device__ __forceinline__ void step(int p, int64_t n, const T* __restrict__ A, int64_t lda, T* x, unsigned* pub,
                                      unsigned marker, unsigned* err, Smem<T, T, KB>& sm) {
   constexpr int CW = KB / 4;  // columns per thread in the block mat-vecs
@@ -194,21 +196,51 @@ __device__ __forceinline__ void step(int p, int64_t n, const T* __restrict__ A, 
   const int r = TRANS ? t % KB : t / 4;
   const int cq = TRANS ? t / KB : t % 4;
   const int64_t gi = r0 + r;
-  T acc = T(0);
-  for (int q = 0; q < p; ++q) {
+  // This thread's CW matrix entries of block q, one bounds test for the
+  // whole run when it is inside the matrix (so NoTrans loads merge into
+  // 16-byte accesses).
+  auto load_m = [&](int q, T (&m)[CW]) {
     const int64_t bj = FWD ? q : nb - 1 - q;
     const int64_t c0 = bj * KB + cq * CW;
-    T m[CW];
+    if (gi < n && c0 + CW <= n) {
 #pragma unroll
-    for (int e = 0; e < CW; ++e) {
-      const int64_t gj = c0 + e;
-      m[e] = (gi < n && gj < n) ? mat<T, TRANS>(A, lda, gi, gj) : T(0);
+      for (int e = 0; e < CW; ++e) m[e] = mat<T, TRANS>(A, lda, gi, c0 + e);
+    } else {
+#pragma unroll
+      for (int e = 0; e < CW; ++e) m[e] = (gi < n && c0 + e < n) ? mat<T, TRANS>(A, lda, gi, c0 + e) : T(0);
     }
-    T* xs = sm.xs[q & 1];  // double-buffered: one barrier per block
-    fetch_block<T, KB>(n, bj, pub, marker, err, xs);
-    __syncthreads();
+  };
+  // Block q's entries are loaded one iteration ahead, so an iteration costs
+  // the wait for x_q and not also a matrix load's latency: the panel loop
+  // then keeps ahead of the chain of published x blocks.
+  T acc = T(0);
+  if constexpr (!PF) {
+    for (int q = 0; q < p; ++q) {
+      T m[CW];
+      load_m(q, m);
+      T* xs = sm.xs[q & 1];
+      fetch_block<T, KB>(n, FWD ? q : nb - 1 - q, pub, marker, err, xs);
+      __syncthreads();
 #pragma unroll
-    for (int e = 0; e < CW; ++e) acc += m[e] * xs[cq * CW + e];
+      for (int e = 0; e < CW; ++e) acc += m[e] * xs[cq * CW + e];
+    }
+  }
+  T m[2][CW];
+  if (PF && p > 0) load_m(0, m[0]);
+  for (int q = 0; PF && q < p; q += 2) {
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int qq = q + h;
+      if (qq < p) {
+        if (qq + 1 < p) load_m(qq + 1, m[h ^ 1]);
+        const int64_t bj = FWD ? qq : nb - 1 - qq;
+        T* xs = sm.xs[h];  // double-buffered: one barrier per block
+        fetch_block<T, KB>(n, bj, pub, marker, err, xs);
+        __syncthreads();
+#pragma unroll
+        for (int e = 0; e < CW; ++e) acc += m[h][e] * xs[cq * CW + e];
+      }
+    }
   }
   sm.red[cq][r] = acc;
   __syncthreads();
