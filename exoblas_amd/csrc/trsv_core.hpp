@@ -76,6 +76,51 @@ __device__ __forceinline__ int64_t row_of(int64_t r0, int i) {
   return FWD ? r0 + i : r0 + (KB - 1 - i);
 }
 
+// One level of the pairwise merges of invert_diag (block size SZ -> 2 SZ),
+// then the next level.
+template <class TI, int KB, int SZ>
+__device__ __forceinline__ void merge_level(TI (&L)[KB][KB + 1]) {
+  if constexpr (SZ < KB) {
+    constexpr int NT = 4 * KB;
+    constexpr int PER = (KB / (2 * SZ)) * SZ * SZ / NT;  // outputs per thread
+    static_assert(PER >= 1 && (KB / (2 * SZ)) * SZ * SZ % NT == 0, "merge layout");
+    const int t = threadIdx.x;
+    int base[PER], r[PER], c[PER];
+#pragma unroll
+    for (int e = 0; e < PER; ++e) {
+      const int idx = t + NT * e;
+      base[e] = (idx / (SZ * SZ)) * 2 * SZ;
+      r[e] = (idx % (SZ * SZ)) / SZ;
+      c[e] = idx % SZ;
+    }
+    TI pv[PER];
+#pragma unroll
+    for (int e = 0; e < PER; ++e) {  // (B inv(A))[r][c]
+      TI acc = TI(0);
+#pragma unroll
+      for (int k = 0; k < SZ; ++k) acc += L[base[e] + SZ + r[e]][base[e] + k] * L[base[e] + k][base[e] + c[e]];
+      pv[e] = acc;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < PER; ++e) L[base[e] + SZ + r[e]][base[e] + c[e]] = pv[e];
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < PER; ++e) {  // -(inv(C) P)[r][c]
+      TI acc = TI(0);
+#pragma unroll
+      for (int k = 0; k < SZ; ++k)
+        acc += L[base[e] + SZ + r[e]][base[e] + SZ + k] * L[base[e] + SZ + k][base[e] + c[e]];
+      pv[e] = -acc;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < PER; ++e) L[base[e] + SZ + r[e]][base[e] + c[e]] = pv[e];
+    __syncthreads();
+    merge_level<TI, KB, 2 * SZ>(L);
+  }
+}
+
 // Loads the diagonal block of block bi (rows r0 ..) in local order, converted
 // to TI, and inverts it in place (16 x 16 diagonal blocks, then two pairwise
 // merges: 40 barriers instead of the 128 of a column-by-column inverse at
@@ -98,83 +143,38 @@ __device__ __forceinline__ void invert_diag(int64_t n, const T* A, int64_t lda, 
     L[i][j] = v;
   }
   __syncthreads();
-  // 1. The four 16 x 16 diagonal blocks, column by column from the last (one
-  //    thread per row of each block): with X = inv(D),
-  //    X[i][j] = -(sum_{k=j+1..i} X[i][k] D[k][j]) / D[j][j], X[j][j] = 1 / D[j][j].
+  // 1. The KB/16 diagonal 16 x 16 blocks by forward substitution, no
+  //    barriers: thread 16 * blk + j computes column j of inv(D_blk) in
+  //    registers, X[i][j] = (delta_ij - sum_{k<i} D[i][k] X[k][j]) / D[i][i]
+  //    (so X[k][j] = 0 for k < j), reading D by broadcast from LDS; the
+  //    columns are written over D once every thread has read it.
   {
-    const int b = (t / 16) * 16, i = t % 16;
-    for (int j = 15; j >= 0; --j) {
-      TI djj = TI(0), nv = TI(0);
-      if (t < KB) {
-        djj = TI(1) / L[b + j][b + j];
-        if (i > j) {
-          TI acc = TI(0);
-          for (int k = j + 1; k <= i; ++k) acc += L[b + i][b + k] * L[b + k][b + j];
-          nv = -acc * djj;
-        }
+    TI xc[16];
+    const int b = (t / 16) * 16, j = t % 16;
+    if (t < KB) {
+#pragma unroll
+      for (int i = 0; i < 16; ++i) {
+        TI s = i == j ? TI(1) : TI(0);
+#pragma unroll
+        for (int k = 0; k < i; ++k) s -= L[b + i][b + k] * xc[k];
+        xc[i] = s / L[b + i][b + i];
       }
-      __syncthreads();
-      if (t < KB) {
-        if (i > j) L[b + i][b + j] = nv;
-        if (i == j) L[b + j][b + j] = djj;
-      }
-      __syncthreads();
     }
+    __syncthreads();
+    if (t < KB) {
+#pragma unroll
+      for (int i = 0; i < 16; ++i)
+        if (i >= j) L[b + i][b + j] = xc[i];
+    }
+    __syncthreads();
   }
-  // 2. Merge inverted diagonal blocks pairwise, 16 -> 32 -> 64:
+  // 2. Merge inverted diagonal blocks pairwise, 16 -> 32 -> ... -> KB:
   //    inv([A 0; B C]) = [inv(A) 0; -inv(C) B inv(A), inv(C)], the product
-  //    formed in registers and written over B.
-  for (int sz = 16; sz < KB; sz *= 2) {
-    constexpr int kMaxPer = KB * KB / 4 / NT;  // outputs per thread at the last merge
-    const int per = (KB / (2 * sz)) * sz * sz / NT;
-    TI pv[kMaxPer];
-    auto coords = [&](int e, int& base, int& r, int& c) {
-      const int idx = t + NT * e;
-      base = (idx / (sz * sz)) * 2 * sz;
-      r = (idx % (sz * sz)) / sz;
-      c = idx % sz;
-    };
-#pragma unroll
-    for (int e = 0; e < kMaxPer; ++e) {
-      if (e < per) {
-        int base, r, c;
-        coords(e, base, r, c);
-        TI acc = TI(0);  // (B inv(A))[r][c], inv(A) lower triangular
-        for (int k = c; k < sz; ++k) acc += L[base + sz + r][base + k] * L[base + k][base + c];
-        pv[e] = acc;
-      }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int e = 0; e < kMaxPer; ++e) {
-      if (e < per) {
-        int base, r, c;
-        coords(e, base, r, c);
-        L[base + sz + r][base + c] = pv[e];
-      }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int e = 0; e < kMaxPer; ++e) {
-      if (e < per) {
-        int base, r, c;
-        coords(e, base, r, c);
-        TI acc = TI(0);  // -(inv(C) P)[r][c], inv(C) lower triangular
-        for (int k = 0; k <= r; ++k) acc += L[base + sz + r][base + sz + k] * L[base + sz + k][base + c];
-        pv[e] = -acc;
-      }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int e = 0; e < kMaxPer; ++e) {
-      if (e < per) {
-        int base, r, c;
-        coords(e, base, r, c);
-        L[base + sz + r][base + c] = pv[e];
-      }
-    }
-    __syncthreads();
-  }
+  //    formed in registers and written over B.  The sums run over whole
+  //    block rows / columns with compile-time bounds (the strictly upper
+  //    parts of inv(A) and inv(C) hold zeros), so they unroll and the LDS
+  //    reads overlap.
+  merge_level<TI, KB, 16>(L);
 }
 
 // One processing step p of the solve; called by all 4 * KB threads of a
@@ -189,6 +189,13 @@ __device__ __forceinline__ void step(int p, int64_t n, const T* __restrict__ A, 
   const int t = threadIdx.x;
   const int64_t bi = FWD ? p : nb - 1 - p;
   const int64_t r0 = bi * KB;
+  // With PF, this block's right-hand side is read before anything waits
+  // (only this workgroup writes these rows of x, at the end of the step).
+  T b = T(0);
+  if (PF && t < KB) {
+    const int64_t g = row_of<FWD, KB>(r0, t);
+    if (g < n) b = x[g];
+  }
   invert_diag<T, T, FWD, TRANS, UNIT, KB>(n, A, lda, r0, sm.inv);
   // NoTrans: 4 threads per row, CW consecutive columns each (contiguous
   // loads); Trans: one column of KB rows per KB threads, CW columns per
@@ -252,7 +259,7 @@ __device__ __forceinline__ void step(int p, int64_t n, const T* __restrict__ A, 
       T s = T(0);
 #pragma unroll
       for (int w = 0; w < 4; ++w) s += sm.red[w][nr];
-      v = x[g] - s;
+      v = (PF ? b : x[g]) - s;
     }
     sm.vs[t] = v;
   }

@@ -52,21 +52,34 @@ __global__ __launch_bounds__(kNT) void trsv_mixed_kernel(int64_t n, const float*
   const int r = TRANS ? t % kB : t / 4;
   const int cq = TRANS ? t / kB : t % 4;
   const int64_t gi = r0 + r;
-  float acc = 0.f;
-  for (int q = 0; q < p; ++q) {
+  // Off-diagonal entries of block q for this thread, one bounds test per run;
+  // each block is loaded one iteration ahead of its x block's wait.
+  auto load_m = [&](int q, float (&m)[16]) {
     const int64_t bj = FWD ? q : nb - 1 - q;
     const int64_t c0 = bj * kB + cq * 16;
-    float m[16];
+    const bool in = gi < n && c0 + 16 <= n;
 #pragma unroll
     for (int e = 0; e < 16; ++e) {
       const int64_t gj = c0 + e;
-      m[e] = (gi < n && gj < n) ? to_f(TRANS ? Aoff[gj * ldo + gi] : Aoff[gi * ldo + gj]) : 0.f;
+      m[e] = (in || (gi < n && gj < n)) ? to_f(TRANS ? Aoff[gj * ldo + gi] : Aoff[gi * ldo + gj]) : 0.f;
     }
-    float* xs = sm.xs[q & 1];
-    trsv::fetch_block<float>(n, bj, done, epoch, flags + 1, xs);
-    __syncthreads();
+  };
+  float acc = 0.f;
+  float m[2][16];
+  if (p > 0) load_m(0, m[0]);
+  for (int q = 0; q < p; q += 2) {
 #pragma unroll
-    for (int e = 0; e < 16; ++e) acc += m[e] * xs[cq * 16 + e];
+    for (int h = 0; h < 2; ++h) {
+      const int qq = q + h;
+      if (qq < p) {
+        if (qq + 1 < p) load_m(qq + 1, m[h ^ 1]);
+        float* xs = sm.xs[h];
+        trsv::fetch_block<float>(n, FWD ? qq : nb - 1 - qq, done, epoch, flags + 1, xs);
+        __syncthreads();
+#pragma unroll
+        for (int e = 0; e < 16; ++e) acc += m[h][e] * xs[cq * 16 + e];
+      }
+    }
   }
   sm.red[cq][r] = acc;
   __syncthreads();
