@@ -153,15 +153,19 @@ void launch_cfg(Ctx& c, int64_t tile0, int64_t ntiles, int64_t M, int64_t N, int
 // 16 114; FP64 128 x 256 x 16 in 8 waves of 64 x 64 57.9 TFLOP/s, 64 x 128 x
 // 16 in 4 waves 46.3, 128 x 128 x 16 in 4 waves 39).
 template <class T> struct Tiles;
+// The 64 x 64 tiles serve small outputs, where a workgroup runs only a few K
+// steps and each step costs about one memory latency: deep K tiles (64 FP32
+// / 32 FP64) halve or quarter the steps (sgemm 512^3 13 -> 11 us, 1024^3
+// 40 -> 60 TFLOP/s on MI355X against 16-deep tiles).
 template <> struct Tiles<float> {
   using Huge = Cfg<float, 256, 256, 32, 128, 64>;
   using Big = Cfg<float, 128, 128, 16, 64, 64>;
-  using Small = Cfg<float, 64, 64, 16, 32, 32>;
+  using Small = Cfg<float, 64, 64, 64, 32, 32>;
 };
 template <> struct Tiles<double> {
   using Huge = Cfg<double, 128, 256, 16, 64, 64>;
   using Big = Cfg<double, 64, 128, 16, 32, 64>;
-  using Small = Cfg<double, 64, 64, 16, 32, 32>;
+  using Small = Cfg<double, 64, 64, 32, 32, 32>;
 };
 
 // Workgroup tiles of BM x BN an M x N product runs (for a triangular update
@@ -224,7 +228,7 @@ void launch_rounds(Ctx& c, int64_t M, int64_t N, int64_t K, typename C::type alp
 // (ssyrk 4096 x 1024 71.1 -> 84.8), else the large tiles split along K when
 // that fills the rounds (sgemm 2048^3 81.2 -> 100.2), else 128 x 128 tiles,
 // else 64 x 64 tiles split along K for small outputs with a long K (sgemm
-// 512^3).
+// 512^3, 1024^3).
 template <class T, int TRI>
 void run(Ctx& c, int64_t M, int64_t N, int64_t K, T alpha, const Operand<T>& A, const Operand<T>& B, T* Cp,
          int64_t ldc) {
@@ -240,10 +244,10 @@ void run(Ctx& c, int64_t M, int64_t N, int64_t K, T alpha, const Operand<T>& A, 
   const int64_t sh = best_split(huge, P, std::min<int64_t>(16, K / (4 * Huge::BK)), 0.75);
   if (sh >= 2) return launch_plan<Huge, TRI>(c, 0, sh, M, N, K, alpha, A, B, Cp, ldc);
   if (big >= P) return launch_rounds<Big, TRI>(c, M, N, K, alpha, A, B, Cp, ldc);
-  // Few output tiles and a long K: split K so every CU gets a workgroup,
+  // Few output tiles and a long K: split K so every CU gets two workgroups,
   // each K slice at least 128 deep.
   const int64_t small = tile_count<Small>(TRI, M, N);
-  const int64_t S = std::min<int64_t>({ceil_div(P, small), K / 128, 16});
+  const int64_t S = std::min<int64_t>({ceil_div(2 * P, small), K / 128, 16});
   if (S >= 2) return launch_plan<Small, TRI>(c, 0, S, M, N, K, alpha, A, B, Cp, ldc);
   launch_rounds<Small, TRI>(c, M, N, K, alpha, A, B, Cp, ldc);
 }
