@@ -88,10 +88,16 @@ __global__ __launch_bounds__(256) void splitk_reduce_kernel(int64_t M, int64_t N
 #pragma unroll
   for (int e = 0; e < VN; ++e) sum[e] = T(0);
   const T* p = P + l * BM * BN + (int64_t)r * BN + g * VN;
-  for (int k = 0; k < S; ++k) {
-    const V v = *reinterpret_cast<const V*>(p + (int64_t)k * ntiles * BM * BN);
+  for (int k0 = 0; k0 < S; k0 += 8) {  // 8 slices' loads in flight, summed in slice order
+    V v[8];
 #pragma unroll
-    for (int e = 0; e < VN; ++e) sum[e] += vget<T>(v, e);
+    for (int j = 0; j < 8; ++j)
+      if (k0 + j < S) v[j] = ld_stream<T>(p + (int64_t)(k0 + j) * ntiles * BM * BN);
+#pragma unroll
+    for (int j = 0; j < 8; ++j)
+      if (k0 + j < S)
+#pragma unroll
+        for (int e = 0; e < VN; ++e) sum[e] += vget<T>(v[j], e);
   }
 #pragma unroll
   for (int e = 0; e < VN; ++e) {
