@@ -25,6 +25,7 @@ template <class T, bool FWD, bool TRANS, bool UNIT, int KB>
 __global__ __launch_bounds__(4 * KB) void trsv_kernel(int64_t n, const T* __restrict__ A, int64_t lda, T* x,
                                                       unsigned* flags, unsigned epoch) {
   __shared__ trsv::Smem<T, T, KB> sm;
+  __shared__ T mb[KB][KB + 1];
   __shared__ int s_p;
   const int nb = (int)((n + KB - 1) / KB);
   if (threadIdx.x == 0) {
@@ -33,7 +34,7 @@ __global__ __launch_bounds__(4 * KB) void trsv_kernel(int64_t n, const T* __rest
     s_p = (int)p;
   }
   __syncthreads();
-  trsv::step<T, FWD, TRANS, UNIT, KB>(s_p, n, A, lda, x, flags + kFlagBase, epoch, flags + 1, sm);
+  trsv::step<T, FWD, TRANS, UNIT, KB>(s_p, n, A, lda, x, flags + kFlagBase, epoch, flags + 1, sm, mb);
 }
 
 // 64-row blocks.  (128-row blocks halve the number of dependent steps but

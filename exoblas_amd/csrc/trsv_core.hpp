@@ -183,7 +183,8 @@ __device__ __forceinline__ void invert_diag(int64_t n, const T* A, int64_t lda, 
 // work items in its registers passes false).
 template <class T, bool FWD, bool TRANS, bool UNIT, int KB = kB, bool PF = true>
 __device__ __forceinline__ void step(int p, int64_t n, const T* __restrict__ A, int64_t lda, T* x, unsigned* pub,
-                                     unsigned marker, unsigned* err, Smem<T, T, KB>& sm) {
+                                     unsigned marker, unsigned* err, Smem<T, T, KB>& sm,
+                                     T (*mb)[KB + 1] = nullptr) {
   constexpr int CW = KB / 4;  // columns per thread in the block mat-vecs
   const int nb = (int)((n + KB - 1) / KB);
   const int t = threadIdx.x;
@@ -203,6 +204,34 @@ __device__ __forceinline__ void step(int p, int64_t n, const T* __restrict__ A, 
   const int r = TRANS ? t % KB : t / 4;
   const int cq = TRANS ? t / KB : t % 4;
   const int64_t gi = r0 + r;
+  const int i4 = t / 4, q4 = t % 4;  // row (local order) and column quarter of the inv(D) products
+  // PF: the previous block's update is folded into inv(D) ahead of time,
+  // G = inv(D) M(this block, block p-1), so once x_{p-1} appears the step is
+  // one block mat-vec x = y0 - G x_{p-1} (y0 = inv(D)(b - the other blocks)).
+  // mb holds M(row_of(k), block p-1 column c); this thread keeps
+  // G[i4][q4*CW .. +CW] in registers.
+  T gq[CW];
+  const int pp = PF && p > 0 ? p - 1 : p;  // blocks the panel loop covers
+  if constexpr (PF) {
+    if (p > 0) {
+      const int64_t cb0 = (FWD ? (int64_t)(p - 1) : (int64_t)(nb - p)) * KB;
+      for (int idx = t; idx < KB * KB; idx += 4 * KB) {
+        const int a = idx / KB, c2 = idx % KB;
+        const int k = TRANS ? c2 : a, c = TRANS ? a : c2;  // lanes on consecutive addresses
+        const int64_t gk = row_of<FWD, KB>(r0, k), gc = cb0 + c;
+        mb[k][c] = (gk < n && gc < n) ? mat<T, TRANS>(A, lda, gk, gc) : T(0);
+      }
+      __syncthreads();
+#pragma unroll
+      for (int e = 0; e < CW; ++e) gq[e] = T(0);
+#pragma unroll 8
+      for (int k = 0; k < KB; ++k) {  // inv(D) is lower triangular in local order: zeros above
+        const T iv = sm.inv[i4][k];
+#pragma unroll
+        for (int e = 0; e < CW; ++e) gq[e] += iv * mb[k][q4 * CW + e];
+      }
+    }
+  }
   // This thread's CW matrix entries of block q, one bounds test for the
   // whole run when it is inside the matrix (so NoTrans loads merge into
   // 16-byte accesses).
@@ -233,13 +262,13 @@ __device__ __forceinline__ void step(int p, int64_t n, const T* __restrict__ A, 
     }
   }
   T m[2][CW];
-  if (PF && p > 0) load_m(0, m[0]);
-  for (int q = 0; PF && q < p; q += 2) {
+  if (PF && pp > 0) load_m(0, m[0]);
+  for (int q = 0; PF && q < pp; q += 2) {
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
       const int qq = q + h;
-      if (qq < p) {
-        if (qq + 1 < p) load_m(qq + 1, m[h ^ 1]);
+      if (qq < pp) {
+        if (qq + 1 < pp) load_m(qq + 1, m[h ^ 1]);
         const int64_t bj = FWD ? qq : nb - 1 - qq;
         T* xs = sm.xs[h];  // double-buffered: one barrier per block
         fetch_block<T, KB>(n, bj, pub, marker, err, xs);
@@ -265,13 +294,23 @@ __device__ __forceinline__ void step(int p, int64_t n, const T* __restrict__ A, 
   }
   __syncthreads();
   {  // x_local = inv(D) v: 4 threads per row, CW columns each
-    const int i = t / 4, q4 = t % 4;
     T s = T(0);
 #pragma unroll
-    for (int e = 0; e < CW; ++e) s += sm.inv[i][q4 * CW + e] * sm.vs[q4 * CW + e];
+    for (int e = 0; e < CW; ++e) s += sm.inv[i4][q4 * CW + e] * sm.vs[q4 * CW + e];
     s += __shfl_xor(s, 1, 64);
     s += __shfl_xor(s, 2, 64);
-    const int64_t g = row_of<FWD, KB>(r0, i);
+    if (PF && p > 0) {  // x = y0 - G x_{p-1}
+      T* xs = sm.xs[0];
+      fetch_block<T, KB>(n, FWD ? (int64_t)(p - 1) : (int64_t)(nb - p), pub, marker, err, xs);
+      __syncthreads();
+      T u = T(0);
+#pragma unroll
+      for (int e = 0; e < CW; ++e) u += gq[e] * xs[q4 * CW + e];
+      u += __shfl_xor(u, 1, 64);
+      u += __shfl_xor(u, 2, 64);
+      s -= u;
+    }
+    const int64_t g = row_of<FWD, KB>(r0, i4);
     if (q4 == 0 && g < n) publish_value<T>(g, s, x, pub, marker);
   }
   __syncthreads();  // the shared buffers are reused by the caller's next step
