@@ -4,6 +4,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cstdint>
 
 #include "runtime.hpp"
@@ -73,6 +74,16 @@ __device__ __forceinline__ T block_sum(T v, T* red) {
 }
 
 inline bool aligned16(const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15) == 0; }
+
+// Raises kernel fn's dynamic-LDS limit to `bytes` (needed above 64 KiB) once
+// per device; `done` is the kernel's own mask of devices already set.  Safe
+// to call from several host threads (a repeated set is harmless).
+inline void ensure_lds_limit(const void* fn, size_t bytes, int device, std::atomic<uint64_t>& done) {
+  const uint64_t bit = 1ull << (device & 63);
+  if (done.load(std::memory_order_acquire) & bit) return;
+  EXO_HIP_CHECK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+  done.fetch_or(bit, std::memory_order_acq_rel);
+}
 
 __host__ __device__ inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
 

@@ -107,11 +107,6 @@ __global__ __launch_bounds__(256) void splitk_reduce_kernel(int64_t M, int64_t N
   }
 }
 
-template <class C, int TRI>
-void set_lds_attr(const void* fn) {
-  EXO_HIP_CHECK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)C::LDS_BYTES));
-}
-
 // Split-K over tiles [tile0, tile0 + ntiles) of the list, S slices.
 template <class C, int TRI>
 void launch_splitk(Ctx& c, int64_t S, int64_t tile0, int64_t ntiles, int64_t M, int64_t N, int64_t K,
@@ -122,11 +117,8 @@ void launch_splitk(Ctx& c, int64_t S, int64_t tile0, int64_t ntiles, int64_t M, 
   const int64_t ksplit = ceil_div(ceil_div(K, S), C::BK) * C::BK;
   S = ceil_div(K, ksplit);
   T* P = static_cast<T*>(c.scratch(sizeof(T) * (size_t)(S * ntiles * C::BM * C::BN), 4));
-  static bool attr_set = false;  // dynamic LDS above 64 KiB needs the attribute
-  if (!attr_set) {
-    set_lds_attr<C, TRI>(reinterpret_cast<const void*>(&gemm_splitk_kernel<C, TRI>));
-    attr_set = true;
-  }
+  static std::atomic<uint64_t> lds_set{0};
+  ensure_lds_limit(reinterpret_cast<const void*>(&gemm_splitk_kernel<C, TRI>), C::LDS_BYTES, c.device, lds_set);
   hipLaunchKernelGGL((gemm_splitk_kernel<C, TRI>), dim3((unsigned)(S * ntiles)), dim3(C::NT), C::LDS_BYTES,
                      c.stream, M, N, K, ksplit, A, B, P, tm, tn, tile0, ntiles);
   EXO_LAUNCH_CHECK();
@@ -144,11 +136,8 @@ void launch_cfg(Ctx& c, int64_t tile0, int64_t ntiles, int64_t M, int64_t N, int
   using T = typename C::type;
   const int64_t tm = ceil_div(M, C::BM), tn = ceil_div(N, C::BN);
   const int cvec = (ldc % C::EB == 0) && (reinterpret_cast<uintptr_t>(Cp) % (C::EB * sizeof(T)) == 0);
-  static bool attr_set = false;
-  if (!attr_set) {
-    set_lds_attr<C, TRI>(reinterpret_cast<const void*>(&gemm_kernel<C, TRI>));
-    attr_set = true;
-  }
+  static std::atomic<uint64_t> lds_set{0};
+  ensure_lds_limit(reinterpret_cast<const void*>(&gemm_kernel<C, TRI>), C::LDS_BYTES, c.device, lds_set);
   hipLaunchKernelGGL((gemm_kernel<C, TRI>), dim3((unsigned)ntiles), dim3(C::NT), C::LDS_BYTES, c.stream, M, N, K,
                      alpha, A, B, Cp, ldc, cvec, tm, tn, tile0);
   EXO_LAUNCH_CHECK();

@@ -263,12 +263,8 @@ void product(Ctx& c, int64_t M, int64_t N, int64_t Kp, float alpha, const __bf16
   const int64_t tm = Mp / SH::BM, tn = Np / SH::BN;
   const size_t lds =
       2 * 16 * (size_t)(Stage<P, SH::BM, SH::NTH, SH::PADV>::TV + Stage<P, SH::BN, SH::NTH, SH::PADV>::TV);
-  static bool attr_set = false;
-  if (!attr_set) {
-    EXO_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&product_kernel<P, TRI, SH>),
-                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    attr_set = true;
-  }
+  static std::atomic<uint64_t> lds_set{0};
+  ensure_lds_limit(reinterpret_cast<const void*>(&product_kernel<P, TRI, SH>), lds, c.device, lds_set);
   hipLaunchKernelGGL((product_kernel<P, TRI, SH>), dim3((unsigned)gemm::tri_tile_count(TRI, tm, tn, SH::BM, SH::BN)),
                      dim3(SH::NTH), lds, c.stream, M, N, Kp,
                      alpha, Ap, Mp, Bp, Np, Cp, ldc, tm, tn, all_mm);
