@@ -26,8 +26,10 @@ class CopyPool {
     cv_.notify_all();
     for (auto& t : threads_) t.join();
   }
-  // dst[0, bytes) = src[0, bytes), the calling thread taking part 0.
+  // dst[0, bytes) = src[0, bytes), the calling thread taking part 0.  Copies
+  // from several host threads (calls on different devices) run one at a time.
   void copy(void* dst, const void* src, size_t bytes) {
+    std::lock_guard<std::mutex> serial(serial_);
     {
       std::lock_guard<std::mutex> g(m_);
       dst_ = static_cast<char*>(dst);
@@ -66,6 +68,7 @@ class CopyPool {
   }
   int n_;
   std::vector<std::thread> threads_;
+  std::mutex serial_;  // one copy at a time
   std::mutex m_;
   std::condition_variable cv_, done_;
   char* dst_ = nullptr;
