@@ -351,7 +351,12 @@ def emit_async_header() -> str:
            " * hipStream_t the work is enqueued on in place of `void *ctxt`.  All window\n"
            " * data and result pointers must be device (or managed) memory; scalar\n"
            " * inputs are read on the host before the call returns.  Returns 0 on\n"
-           " * success, otherwise an EXOBLAS_STATUS_* code (exoblas_amd.h). */\n",
+           " * success, otherwise an EXOBLAS_STATUS_* code (exoblas_amd.h).\n"
+           " * Calls whose workgroups exchange results through memory (asum, dot,\n"
+           " * dsdot, sdsdot, trsv, and gemv with TransA = Trans on few columns)\n"
+           " * tag those results with a marker the host picks per call, so a hipGraph\n"
+           " * that captured one of them may not be replayed; the batch plans of\n"
+           " * exoblas_amd_batch.h are the graph-replayable form. */\n",
            "#ifndef EXOBLAS_AMD_ASYNC_H\n#define EXOBLAS_AMD_ASYNC_H\n\n",
            "#include <hip/hip_runtime_api.h>\n#include \"exo_types.h\"\n\n",
            "#ifdef __cplusplus\nextern \"C\" {\n#endif\n\n"]
