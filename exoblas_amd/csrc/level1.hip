@@ -196,17 +196,24 @@ template <class Acc, class Out> struct Finish {
 // as tagged words; the workgroup that takes the last ticket waits for all
 // partials and sums them in a fixed order (thread t takes partials t,
 // t + kNT, ..., then the workgroup tree), so the result does not depend on
-// which workgroup finishes last.  *ticket: the stream's first counter;
-// flags: [1] timeout, [2 ..] the tagged partials.
+// which workgroup finishes last.  *ticket: the stream's first counter; the
+// marker comes from the stream's launch generation (tagged.hpp), so the
+// launch can be replayed from a hipGraph; flags: [1] timeout, [2 ..] the
+// tagged partials.
 template <class T, class Term, class Out, int U>
 __global__ __launch_bounds__(kNT) void reduce_kernel(int64_t n, const T* __restrict__ x, int64_t incx,
                                                      const T* __restrict__ y, int64_t incy, int vec,
-                                                     unsigned* ticket, unsigned* flags, unsigned epoch, Out* result,
+                                                     unsigned* ticket, unsigned* gen, unsigned* flags, Out* result,
                                                      Finish<typename Term::Acc, Out> fin, Term term) {
   using Acc = typename Term::Acc;
   using V = typename Vec16<T>::type;
   constexpr int VN = Vec16<T>::n;
   __shared__ Acc red[kNT / kWave];
+  // The generation is loaded now and stored for the workgroup only after the
+  // sweep, so no wave waits for it before its streaming loads; thread 0 uses
+  // it (publication) before taking its ticket.
+  __shared__ unsigned s_marker;
+  const unsigned mk = threadIdx.x == 0 ? tagged::read_generation(gen) : 0u;
   Acc acc[U];
 #pragma unroll
   for (int u = 0; u < U; ++u) acc[u] = Acc(0);
@@ -251,11 +258,13 @@ __global__ __launch_bounds__(kNT) void reduce_kernel(int64_t n, const T* __restr
   Acc s = acc[0];
 #pragma unroll
   for (int u = 1; u < U; ++u) s += acc[u];
+  if (threadIdx.x == 0) s_marker = mk;
   s = block_sum<kNT>(s, red);
   unsigned* pub = flags + kReduceFlagBase;
+  const unsigned marker = s_marker;  // written before block_sum's barriers
   __shared__ int s_last;
   if (threadIdx.x == 0) {
-    tagged::publish<Acc>(pub, blockIdx.x, s, epoch);
+    tagged::publish<Acc>(pub, blockIdx.x, s, marker);
     s_last = tagged::last_ticket(ticket, gridDim.x);
   }
   __syncthreads();
@@ -263,13 +272,16 @@ __global__ __launch_bounds__(kNT) void reduce_kernel(int64_t n, const T* __restr
   Acc t = Acc(0);
   for (int i = threadIdx.x; i < (int)gridDim.x; i += 4 * kNT) {
     Acc v[4];
-    tagged::read_n<Acc, 4>(pub, i, kNT, (int)ceil_div((int)gridDim.x - i, kNT), v, epoch, flags + 1);
+    tagged::read_n<Acc, 4>(pub, i, kNT, (int)ceil_div((int)gridDim.x - i, kNT), v, marker, flags + 1);
 #pragma unroll
     for (int k = 0; k < 4; ++k)
       if (i + k * kNT < (int)gridDim.x) t += v[k];
   }
   t = block_sum<kNT>(t, red);
-  if (threadIdx.x == 0) *result = fin(t);
+  if (threadIdx.x == 0) {
+    *result = fin(t);
+    tagged::advance_generation(gen);  // every workgroup has read it: all took tickets
+  }
 }
 
 template <class T, class Term, class Out>
@@ -284,10 +296,11 @@ void launch_reduce(Ctx& c, int64_t n, const T* x, int64_t incx, const T* y, int6
   const bool vec = incx == 1 && aligned16(x) && (!Term::ry || (incy == 1 && aligned16(y)));
   int64_t blocks = std::max<int64_t>(1, ceil_div(n, (int64_t)kNT * U * VN));
   blocks = std::min<int64_t>(blocks, (int64_t)c.num_cus * 2);
-  unsigned epoch = 0;
+  unsigned epoch = 0;  // (the host epoch is not used: the marker comes from the launch generation)
   unsigned* flags = c.epoch_flags((size_t)(kReduceFlagBase + tagged::words<Acc>(blocks)), &epoch);
   hipLaunchKernelGGL((reduce_kernel<T, Term, Out, U>), dim3((unsigned)blocks), dim3(kNT), 0, c.stream, n, x, incx,
-                     y, incy, vec ? 1 : 0, c.counters(), flags, epoch, result, Finish<Acc, Out>{add}, Term{});
+                     y, incy, vec ? 1 : 0, c.counters(), c.counters() + Ctx::kGeneration, flags, result,
+                     Finish<Acc, Out>{add}, Term{});
   EXO_LAUNCH_CHECK();
 }
 

@@ -305,17 +305,23 @@ __global__ __launch_bounds__(kNT) void gemv_finish_kernel(int64_t M, int splits,
 // With row splits (gridDim.y > 1) each workgroup publishes its column sums
 // as tagged words in `pub` (tagged.hpp) and the last of a column block's
 // workgroups (ticket tickets[blockIdx.x]) adds the splits in order 0, 1, ...
-// and writes y; *err is the timeout word.
+// and writes y; *err is the timeout word.  The marker comes from the
+// stream's launch generation *gen, advanced by the last column block to
+// complete (ticket *lt), so the launch can be replayed from a hipGraph.
 template <class T, int RU, bool VEC, int TRI, bool UNIT>
 __global__ __launch_bounds__(kNT) void gemv_t_kernel(int64_t Mc, int64_t Nr, T alpha, T beta,
                                                      const T* __restrict__ A, int64_t lda,
                                                      const T* __restrict__ x, int64_t incx, T* y,
                                                      int64_t incy, int64_t rows_per, unsigned* tickets,
-                                                     unsigned* err, unsigned* pub, unsigned epoch) {
+                                                     unsigned* gen, unsigned* lt, unsigned* err, unsigned* pub) {
   using V = typename Vec16<T>::type;
   constexpr int VN = Vec16<T>::n;
   constexpr int NW = kNT / kWave;
   __shared__ T red[NW][kWave * VN];
+  // The generation is loaded now and stored for the workgroup only at the
+  // reduction below, so no wave waits for it before its streaming loads.
+  __shared__ unsigned s_marker;
+  const unsigned mk = (gridDim.y > 1 && threadIdx.x == 0) ? tagged::read_generation(gen) : 0u;
   const int wave = threadIdx.x / kWave;
   const int lane = threadIdx.x % kWave;
   const int64_t c0 = (int64_t)blockIdx.x * kWave * VN + (int64_t)lane * VN;  // this lane's first column
@@ -385,6 +391,7 @@ __global__ __launch_bounds__(kNT) void gemv_t_kernel(int64_t Mc, int64_t Nr, T a
 
 #pragma unroll
   for (int k = 0; k < VN; ++k) red[wave][lane * VN + k] = acc[k];
+  if (threadIdx.x == 0) s_marker = mk;
   __syncthreads();
   auto write_y = [&](int64_t c, T s) {
     if (TRI != kFull)
@@ -399,7 +406,7 @@ __global__ __launch_bounds__(kNT) void gemv_t_kernel(int64_t Mc, int64_t Nr, T a
     const int64_t c = cb0 + i;
     if (c < Mc) {
       if (gridDim.y > 1)
-        tagged::publish<T>(pub, (int64_t)blockIdx.y * Mc + c, s, epoch);
+        tagged::publish<T>(pub, (int64_t)blockIdx.y * Mc + c, s, s_marker);
       else
         write_y(c, s);
     }
@@ -407,7 +414,13 @@ __global__ __launch_bounds__(kNT) void gemv_t_kernel(int64_t Mc, int64_t Nr, T a
   if (gridDim.y == 1) return;
   __shared__ int s_last;
   __syncthreads();
-  if (threadIdx.x == 0) s_last = tagged::last_ticket(tickets + blockIdx.x, gridDim.y);
+  if (threadIdx.x == 0) {
+    // thread 0 published column cb0 with the marker it read: its ticket
+    // follows that read.  The last column block to complete advances the
+    // generation (every workgroup has read it by then).
+    s_last = tagged::last_ticket(tickets + blockIdx.x, gridDim.y);
+    if (s_last && tagged::last_ticket(lt, gridDim.x)) tagged::advance_generation(gen);
+  }
   __syncthreads();
   if (!s_last) return;
   const int splits = (int)gridDim.y;
@@ -417,7 +430,7 @@ __global__ __launch_bounds__(kNT) void gemv_t_kernel(int64_t Mc, int64_t Nr, T a
     T t = T(0);
     for (int s0 = 0; s0 < splits; s0 += 8) {
       T v[8];
-      tagged::read_n<T, 8>(pub, (int64_t)s0 * Mc + c, Mc, splits - s0, v, epoch, err);
+      tagged::read_n<T, 8>(pub, (int64_t)s0 * Mc + c, Mc, splits - s0, v, s_marker, err);
 #pragma unroll
       for (int k = 0; k < 8; ++k)
         if (s0 + k < splits) t += v[k];
@@ -503,13 +516,15 @@ void launch_gemv_t(Ctx& c, int64_t Mc, int64_t Nr, T alpha, T beta, const T* A, 
   int64_t splits = std::max<int64_t>(1, std::min<int64_t>(ceil_div(want, col_blocks), ceil_div(Nr, 64)));
   const int64_t rows_per = Nr > 0 ? ceil_div(ceil_div(Nr, splits), 32) * 32 : 32;
   splits = std::max<int64_t>(1, ceil_div(Nr, rows_per));
-  // splits > 1 only when col_blocks < want <= Ctx::kCounters: one ticket per
+  // splits > 1 only when col_blocks < want <= Ctx::kLaunchTicket: one ticket per
   // column block from the stream's counters
-  unsigned epoch = 0;
-  unsigned *tickets = nullptr, *err = nullptr, *pub = nullptr;
+  unsigned epoch = 0;  // (not used: the marker comes from the launch generation)
+  unsigned *tickets = nullptr, *gen = nullptr, *lt = nullptr, *err = nullptr, *pub = nullptr;
   if (splits > 1) {
-    if (col_blocks > Ctx::kCounters) throw Error(kInvalidSize, "gemv: too many column blocks for split rows");
+    if (col_blocks > Ctx::kLaunchTicket) throw Error(kInvalidSize, "gemv: too many column blocks for split rows");
     tickets = c.counters();
+    gen = c.counters() + Ctx::kGeneration;
+    lt = c.counters() + Ctx::kLaunchTicket;
     unsigned* flags = c.epoch_flags((size_t)(2 + tagged::words<T>(splits * Mc)), &epoch);
     err = flags + 1;
     pub = flags + 2;  // 8-byte aligned
@@ -517,10 +532,10 @@ void launch_gemv_t(Ctx& c, int64_t Mc, int64_t Nr, T alpha, T beta, const T* A, 
   dim3 grid((unsigned)col_blocks, (unsigned)splits);
   if (vec)
     hipLaunchKernelGGL((gemv_t_kernel<T, RU, true, TRI, UNIT>), grid, dim3(kNT), 0, c.stream, Mc, Nr, alpha, beta,
-                       A, lda, x, incx, y, incy, rows_per, tickets, err, pub, epoch);
+                       A, lda, x, incx, y, incy, rows_per, tickets, gen, lt, err, pub);
   else
     hipLaunchKernelGGL((gemv_t_kernel<T, RU, false, TRI, UNIT>), grid, dim3(kNT), 0, c.stream, Mc, Nr, alpha,
-                       beta, A, lda, x, incx, y, incy, rows_per, tickets, err, pub, epoch);
+                       beta, A, lda, x, incx, y, incy, rows_per, tickets, gen, lt, err, pub);
   EXO_LAUNCH_CHECK();
 }
 

@@ -238,7 +238,7 @@ void* Ctx::scratch(size_t bytes, int slot) {
 unsigned* Ctx::counters() { return ss->counters; }
 
 unsigned* Ctx::epoch_flags(size_t n, unsigned* epoch) {
-  if (ss->flags_cap < n || ss->epoch == 0xffffffffu) {
+  if (ss->flags_cap < n || ss->epoch >= 0x7fffffffu) {
     EXO_HIP_CHECK(hipStreamSynchronize(stream));
     if (ss->flags) EXO_HIP_CHECK(hipFree(ss->flags));
     ss->flags = nullptr;

@@ -78,13 +78,20 @@ struct Ctx {
   void* scratch(size_t bytes, int slot = 0);
   static constexpr int kSlots = 6;
   // kCounters zero-initialised 32-bit counters owned by this stream.  A kernel
-  // that uses them must leave them zero when it ends.
+  // that uses them must leave them zero when it ends, except counter
+  // kGeneration: the stream's launch generation, which single-launch kernels
+  // read for their publication marker and advance once per launch (so their
+  // markers change from launch to launch also when a hipGraph replays them).
   unsigned* counters();
   static constexpr int kCounters = 4096;
+  static constexpr int kGeneration = kCounters - 1;
+  static constexpr int kLaunchTicket = kCounters - 2;  // a launch-wide ticket
   // At least n 32-bit completion flags owned by this stream, plus a fresh
   // epoch value.  A kernel marks item i done by storing the epoch into
   // flags[i]; since every call gets a new epoch the flags never need
-  // clearing.  Zeroed when (re)allocated; the epoch restarts then.
+  // clearing.  Zeroed when (re)allocated; the epoch restarts then.  Host
+  // epochs stay below 2^31; markers drawn from the launch generation have
+  // the top bit set, so the two never match each other's words.
   unsigned* epoch_flags(size_t n, unsigned* epoch);
 };
 

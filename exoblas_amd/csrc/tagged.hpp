@@ -98,6 +98,17 @@ __device__ __forceinline__ void publish(unsigned* pub, int64_t g, T v, unsigned 
   }
 }
 
+// Marker of a launch drawn from the stream's launch generation *gen (top
+// bit set, so it never equals a host epoch).  Every workgroup of the launch
+// reads it and takes its launch-wide ticket only after a store that uses the
+// value (its own publication), so the read is complete before the ticket;
+// the workgroup that takes the last ticket advances the generation, so all
+// workgroups see one value and the next launch (or graph replay) another.
+__device__ __forceinline__ unsigned read_generation(unsigned* gen) {
+  return 0x80000000u | (__hip_atomic_load(gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & 0x7fffffffu);
+}
+__device__ __forceinline__ void advance_generation(unsigned* gen) { atomicAdd(gen, 1u); }
+
 // Takes a ticket from counter *t shared by `count` workgroups; true for the
 // last one, which also resets the counter for the next call on the stream.
 // Call from one thread.

@@ -5,11 +5,11 @@
  * data and result pointers must be device (or managed) memory; scalar
  * inputs are read on the host before the call returns.  Returns 0 on
  * success, otherwise an EXOBLAS_STATUS_* code (exoblas_amd.h).
- * Calls whose workgroups exchange results through memory (asum, dot,
- * dsdot, sdsdot, trsv, and gemv with TransA = Trans on few columns)
- * tag those results with a marker the host picks per call, so a hipGraph
- * that captured one of them may not be replayed; the batch plans of
- * exoblas_amd_batch.h are the graph-replayable form. */
+ * trsv tags the values its workgroups exchange with a marker the host
+ * picks per call, so a hipGraph that captured it may not be replayed
+ * (the batch plans of exoblas_amd_batch.h are the graph-replayable form
+ * of a solve); the single-launch reductions and split gemv draw their
+ * markers from a per-stream counter on the device and replay. */
 #ifndef EXOBLAS_AMD_ASYNC_H
 #define EXOBLAS_AMD_ASYNC_H
 

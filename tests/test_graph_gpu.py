@@ -1,0 +1,89 @@
+"""Single-launch reductions and split gemv-T captured in a hipGraph: their
+publication markers come from a per-stream counter on the device, so every
+replay must reproduce the eager result for the inputs of that replay (the
+inputs are rewritten eagerly between replays)."""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def env():
+    import torch
+    from exoblas_amd import lib as L
+    so = L.load()
+    return torch, so, L
+
+
+def test_dot_and_gemv_t_graph_replay(env):
+    torch, so, L = env
+    g = torch.Generator(device="cuda")
+    g.manual_seed(7)
+    n = (1 << 20) + 37
+    M = N = 4096  # gemv T over 16 column blocks: row splits, last-workgroup sums
+    x = torch.empty(n, device="cuda")
+    y = torch.empty(n, device="cuda")
+    A = torch.empty((M, N), device="cuda")
+    gx = torch.empty(M, device="cuda")
+    gy = torch.empty(N, device="cuda")
+    res = torch.zeros(1, device="cuda")
+    ga, gb = ctypes.byref(ctypes.c_float(0.5)), ctypes.byref(ctypes.c_float(1.5))
+    s = torch.cuda.Stream()
+
+    def fill(seed):
+        g.manual_seed(seed)
+        for t in (x, y, A, gx, gy):
+            t.uniform_(-1, 1, generator=g)
+
+    def launch(stream):
+        assert so.exoblas_sdot_stride_1_async(stream, n, L.win1(x.data_ptr(), 1), L.win1(y.data_ptr(), 1),
+                                              ctypes.c_void_p(res.data_ptr())) == 0
+        assert so.exoblas_sgemv_rm_stride_1_async(stream, 112, N, M, ga, gb, L.win2(A.data_ptr(), N),
+                                                  L.win2(A.data_ptr(), N), L.win1(gx.data_ptr(), 1),
+                                                  L.win1(gy.data_ptr(), 1)) == 0
+
+    def expect():
+        x64, y64 = x.double().cpu().numpy(), y.double().cpu().numpy()
+        dot = float(x64 @ y64)
+        gy_want = 1.5 * gy.double().cpu().numpy() + 0.5 * (A.double().cpu().numpy().T @ gx.double().cpu().numpy())
+        return dot, gy_want
+
+    # eager warm-up (grows the stream's buffers outside the capture)
+    fill(0)
+    torch.cuda.synchronize()
+    with torch.cuda.stream(s):
+        launch(s.cuda_stream)
+    s.synchronize()
+    graph = torch.cuda.CUDAGraph()
+    fill(1)
+    torch.cuda.synchronize()
+    with torch.cuda.graph(graph, stream=s):
+        launch(torch.cuda.current_stream().cuda_stream)
+    for rep in range(4):
+        fill(100 + rep)
+        gy0 = gy.clone()
+        torch.cuda.synchronize()
+        graph.replay()
+        torch.cuda.synchronize()
+        got_gy = gy.double().cpu().numpy()
+        gy.copy_(gy0)
+        dot, gy_want = expect()
+        eps = float(np.finfo(np.float32).eps)
+        mag = float(np.abs(x.double().cpu().numpy() * y.double().cpu().numpy()).sum())
+        assert abs(res.item() - dot) <= 4 * n * eps * mag, f"replay {rep}: dot"
+        gmag = 1.5 * np.abs(gy0.double().cpu().numpy()) + 0.5 * (np.abs(A.double().cpu().numpy()).T @
+                                                              np.abs(gx.double().cpu().numpy()))
+        assert np.all(np.abs(got_gy - gy_want) <= 2 * M * eps * gmag + 1e-30), f"replay {rep}: gemv T"
+        # and the replay equals an eager launch on the same inputs
+        res_g = res.clone()
+        with torch.cuda.stream(s):
+            launch(s.cuda_stream)
+        s.synchronize()
+        assert torch.equal(res, res_g), f"replay {rep}: dot differs from the eager launch"
+        assert np.array_equal(gy.double().cpu().numpy(), got_gy), f"replay {rep}: gemv T differs from eager"
+        gy.copy_(gy0)
