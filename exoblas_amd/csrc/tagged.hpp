@@ -109,6 +109,24 @@ __device__ __forceinline__ unsigned read_generation(unsigned* gen) {
 }
 __device__ __forceinline__ void advance_generation(unsigned* gen) { atomicAdd(gen, 1u); }
 
+// Reads the launch generation, then takes a ticket from counter *t shared
+// by `count` workgroups (call from one thread).  The empty asm needs the
+// loaded value in its register and orders memory operations, so the read is
+// complete before the ticket is taken; the workgroup with the last ticket
+// resets *t and advances the generation (every workgroup has read it).
+// Returns the ticket; *marker receives the launch's marker.
+__device__ __forceinline__ unsigned generation_ticket(unsigned* t, unsigned count, unsigned* gen, unsigned* marker) {
+  unsigned g = __hip_atomic_load(gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  asm volatile("" : "+v"(g)::"memory");
+  *marker = 0x80000000u | (g & 0x7fffffffu);
+  const unsigned p = atomicAdd(t, 1u);
+  if (p == count - 1) {
+    atomicExch(t, 0u);
+    advance_generation(gen);
+  }
+  return p;
+}
+
 // Takes a ticket from counter *t shared by `count` workgroups; true for the
 // last one, which also resets the counter for the next call on the stream.
 // Call from one thread.

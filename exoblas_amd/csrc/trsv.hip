@@ -17,24 +17,26 @@ constexpr int64_t kUnitV = 132;
 using trsv::kB;
 using trsv::kNT;
 
-// flags[0]: ticket, flags[1]: timeout (epoch on timeout), flags[2 ..]: the
-// publication words of trsv_core.hpp (8-byte aligned)
+// flags[0]: ticket, flags[1]: timeout (marker on timeout), flags[2 ..]: the
+// publication words of trsv_core.hpp (8-byte aligned).  The marker is the
+// stream's launch generation (tagged.hpp), so a captured solve replays.
 constexpr int kFlagBase = 2;
 
 template <class T, bool FWD, bool TRANS, bool UNIT, int KB>
 __global__ __launch_bounds__(4 * KB) void trsv_kernel(int64_t n, const T* __restrict__ A, int64_t lda, T* x,
-                                                      unsigned* flags, unsigned epoch) {
+                                                      unsigned* flags, unsigned* gen) {
   __shared__ trsv::Smem<T, T, KB> sm;
   __shared__ T mb[KB][KB + 1];
   __shared__ int s_p;
+  __shared__ unsigned s_marker;
   const int nb = (int)((n + KB - 1) / KB);
   if (threadIdx.x == 0) {
-    const unsigned p = atomicAdd(flags, 1u);
-    if ((int)p == nb - 1) atomicExch(flags, 0u);  // the last ticket: every ticket is taken
-    s_p = (int)p;
+    unsigned marker;
+    s_p = (int)tagged::generation_ticket(flags, (unsigned)nb, gen, &marker);
+    s_marker = marker;
   }
   __syncthreads();
-  trsv::step<T, FWD, TRANS, UNIT, KB>(s_p, n, A, lda, x, flags + kFlagBase, epoch, flags + 1, sm, mb);
+  trsv::step<T, FWD, TRANS, UNIT, KB>(s_p, n, A, lda, x, flags + kFlagBase, s_marker, flags + 1, sm, mb);
 }
 
 // 64-row blocks.  (128-row blocks halve the number of dependent steps but
@@ -44,14 +46,15 @@ template <class T, bool FWD, bool TRANS, int KB>
 void launch_trsv_kb(Ctx& c, bool unit, int64_t n, const T* A, int64_t lda, T* x) {
   const int64_t nb = ceil_div(n, KB);
   if (nb > 0x7fffffff) throw Error(kInvalidSize, "trsv: n too large");
-  unsigned epoch = 0;
+  unsigned epoch = 0;  // (not used: the marker comes from the launch generation)
   unsigned* flags = c.epoch_flags((size_t)trsv::pub_words<T>(n) + kFlagBase, &epoch);
+  unsigned* gen = c.counters() + Ctx::kGeneration;
   if (unit)
     hipLaunchKernelGGL((trsv_kernel<T, FWD, TRANS, true, KB>), dim3((unsigned)nb), dim3(4 * KB), 0, c.stream, n, A,
-                       lda, x, flags, epoch);
+                       lda, x, flags, gen);
   else
     hipLaunchKernelGGL((trsv_kernel<T, FWD, TRANS, false, KB>), dim3((unsigned)nb), dim3(4 * KB), 0, c.stream, n,
-                       A, lda, x, flags, epoch);
+                       A, lda, x, flags, gen);
   EXO_LAUNCH_CHECK();
 }
 

@@ -33,18 +33,20 @@ template <class TO, bool FWD, bool TRANS, bool UNIT>
 __global__ __launch_bounds__(kNT) void trsv_mixed_kernel(int64_t n, const float* __restrict__ A, int64_t lda,
                                                          const TO* __restrict__ Aoff, int64_t ldo,
                                                          const double* __restrict__ rhs, double* x, float* xf,
-                                                         unsigned* flags, unsigned epoch) {
+                                                         unsigned* flags, unsigned* gen) {
   __shared__ trsv::Smem<float, double> sm;
   __shared__ int s_p;
   const int nb = (int)((n + kB - 1) / kB);
   const int t = threadIdx.x;
-  if (t == 0) {
-    const unsigned p = atomicAdd(flags, 1u);
-    if ((int)p == nb - 1) atomicExch(flags, 0u);
-    s_p = (int)p;
+  __shared__ unsigned s_marker;
+  if (t == 0) {  // ticket and marker from the stream's launch generation (graph-replayable)
+    unsigned marker;
+    s_p = (int)tagged::generation_ticket(flags, (unsigned)nb, gen, &marker);
+    s_marker = marker;
   }
   __syncthreads();
   const int p = s_p;
+  const unsigned epoch = s_marker;
   unsigned* done = flags + kFlagBase;
   const int64_t bi = FWD ? p : nb - 1 - p;
   const int64_t r0 = bi * kB;
@@ -155,14 +157,15 @@ template <class TO, bool FWD, bool TRANS>
 void solve(Ctx& c, bool unit, int64_t n, const float* A, int64_t lda, const TO* Aoff, int64_t ldo,
            const double* rhs, double* x, float* xf) {
   const int64_t nb = ceil_div(n, kB);
-  unsigned epoch = 0;
+  unsigned epoch = 0;  // (not used: the marker comes from the launch generation)
   unsigned* flags = c.epoch_flags((size_t)trsv::pub_words<float>(n) + kFlagBase, &epoch);
+  unsigned* gen = c.counters() + Ctx::kGeneration;
   if (unit)
     hipLaunchKernelGGL((trsv_mixed_kernel<TO, FWD, TRANS, true>), dim3((unsigned)nb), dim3(kNT), 0, c.stream, n, A,
-                       lda, Aoff, ldo, rhs, x, xf, flags, epoch);
+                       lda, Aoff, ldo, rhs, x, xf, flags, gen);
   else
     hipLaunchKernelGGL((trsv_mixed_kernel<TO, FWD, TRANS, false>), dim3((unsigned)nb), dim3(kNT), 0, c.stream, n, A,
-                       lda, Aoff, ldo, rhs, x, xf, flags, epoch);
+                       lda, Aoff, ldo, rhs, x, xf, flags, gen);
   EXO_LAUNCH_CHECK();
 }
 

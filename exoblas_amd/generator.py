@@ -352,11 +352,11 @@ def emit_async_header() -> str:
            " * data and result pointers must be device (or managed) memory; scalar\n"
            " * inputs are read on the host before the call returns.  Returns 0 on\n"
            " * success, otherwise an EXOBLAS_STATUS_* code (exoblas_amd.h).\n"
-           " * trsv tags the values its workgroups exchange with a marker the host\n"
-           " * picks per call, so a hipGraph that captured it may not be replayed\n"
-           " * (the batch plans of exoblas_amd_batch.h are the graph-replayable form\n"
-           " * of a solve); the single-launch reductions and split gemv draw their\n"
-           " * markers from a per-stream counter on the device and replay. */\n",
+           " * Values that workgroups exchange through memory (reductions, split\n"
+           " * gemv, trsv) carry a marker drawn from a per-stream counter on the\n"
+           " * device, so a call captured in a hipGraph replays correctly.  The first\n"
+           " * call of a kind on a stream may allocate its buffers: make it once\n"
+           " * before capturing. */\n",
            "#ifndef EXOBLAS_AMD_ASYNC_H\n#define EXOBLAS_AMD_ASYNC_H\n\n",
            "#include <hip/hip_runtime_api.h>\n#include \"exo_types.h\"\n\n",
            "#ifdef __cplusplus\nextern \"C\" {\n#endif\n\n"]

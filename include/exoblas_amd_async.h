@@ -5,11 +5,11 @@
  * data and result pointers must be device (or managed) memory; scalar
  * inputs are read on the host before the call returns.  Returns 0 on
  * success, otherwise an EXOBLAS_STATUS_* code (exoblas_amd.h).
- * trsv tags the values its workgroups exchange with a marker the host
- * picks per call, so a hipGraph that captured it may not be replayed
- * (the batch plans of exoblas_amd_batch.h are the graph-replayable form
- * of a solve); the single-launch reductions and split gemv draw their
- * markers from a per-stream counter on the device and replay. */
+ * Values that workgroups exchange through memory (reductions, split
+ * gemv, trsv) carry a marker drawn from a per-stream counter on the
+ * device, so a call captured in a hipGraph replays correctly.  The first
+ * call of a kind on a stream may allocate its buffers: make it once
+ * before capturing. */
 #ifndef EXOBLAS_AMD_ASYNC_H
 #define EXOBLAS_AMD_ASYNC_H
 

@@ -87,3 +87,50 @@ def test_dot_and_gemv_t_graph_replay(env):
         assert torch.equal(res, res_g), f"replay {rep}: dot differs from the eager launch"
         assert np.array_equal(gy.double().cpu().numpy(), got_gy), f"replay {rep}: gemv T differs from eager"
         gy.copy_(gy0)
+
+
+@pytest.mark.parametrize("dt", ["float32", "float64"])
+def test_trsv_graph_replay(env, dt):
+    """A captured triangular solve (look-back chain with tagged x blocks)
+    replayed with a new right-hand side each time equals the eager solve of
+    that right-hand side, bit for bit, and solves the system."""
+    torch, so, L = env
+    tdt = getattr(torch, dt)
+    p = "s" if dt == "float32" else "d"
+    n = 1000
+    g = torch.Generator(device="cuda")
+    g.manual_seed(3)
+    A = torch.empty((n, n), dtype=tdt, device="cuda").uniform_(-1, 1, generator=g) / n
+    A.diagonal().copy_(n + torch.empty(n, dtype=tdt, device="cuda").uniform_(0, 1, generator=g))
+    x = torch.empty(n, dtype=tdt, device="cuda")
+    f = getattr(so, f"exoblas_{p}trsv_rm_stride_1_async")
+    s = torch.cuda.Stream()
+
+    def launch(stream):
+        assert f(stream, 122, 111, 131, n, L.win1(x.data_ptr(), 1), L.win2(A.data_ptr(), n)) == 0
+
+    x.uniform_(-1, 1, generator=g)
+    torch.cuda.synchronize()
+    with torch.cuda.stream(s):
+        launch(s.cuda_stream)  # eager warm-up outside the capture
+    s.synchronize()
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph, stream=s):
+        launch(torch.cuda.current_stream().cuda_stream)
+    L64 = torch.tril(A).double().cpu().numpy()
+    for rep in range(4):
+        b = torch.empty(n, dtype=tdt, device="cuda").uniform_(-1, 1, generator=g)
+        x.copy_(b)
+        torch.cuda.synchronize()
+        graph.replay()
+        torch.cuda.synchronize()
+        got = x.clone()
+        x.copy_(b)
+        torch.cuda.synchronize()
+        with torch.cuda.stream(s):
+            launch(s.cuda_stream)
+        s.synchronize()
+        assert torch.equal(got, x), f"replay {rep} differs from the eager solve"
+        r = L64 @ got.double().cpu().numpy() - b.double().cpu().numpy()
+        eps = float(np.finfo(np.dtype(dt)).eps)
+        assert np.abs(r).max() <= 8 * n * eps, f"replay {rep}: residual {np.abs(r).max()}"
