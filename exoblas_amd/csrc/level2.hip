@@ -1,10 +1,13 @@
 // Level-2 kernels for gfx950 (all HBM-bound on the matrix stream).
 //
-// gemv-N  one wave per R rows, x staged through LDS in panels, 16-byte loads
-//         of R rows in flight per lane, wave-shuffle row sums; wide matrices
-//         split their columns over workgroups (partials + a finishing pass).
+// gemv-N  long rows with a contiguous x: whole-workgroup rows, x held in
+//         registers, two rows at a time; other tall matrices: one row per
+//         wave; short ones: R rows per wave, x staged through LDS in panels,
+//         wide matrices split their columns over workgroups (partials + a
+//         finishing pass).
 // gemv-T  a workgroup owns 64*VN consecutive columns and a chunk of rows;
-//         lanes hold column sums, waves split the rows and reduce through LDS.
+//         lanes hold column sums, waves split the rows and reduce through LDS;
+//         row splits are summed by the last workgroup of each column block.
 // ger, syr, syr2   row-wise rank-1/2 updates (read+write each element once),
 //         restricted to one triangle for syr/syr2.
 // symv    64x64 tiles of the stored triangle, each read once: a tile adds

@@ -13,6 +13,10 @@
 //      diagonal, each after the step that owns it has published its part of x;
 //   3. forms v = b - (panel . x) and x_block = inv(D) v with all its threads,
 //      so the serial part of the chain is one small mat-vec per block.
+// The standalone solve (PF) also folds the previous block into the inverse
+// ahead of time, G = inv(D) M(p, p-1): the panel covers blocks 0 .. p-2,
+// y0 = inv(D)(b - panel . x), and once x_{p-1} appears the step is
+// x_block = y0 - G x_{p-1}.
 // Publication of x: the tagged words of tagged.hpp, one value per row.
 // Waits are bounded: past the spin limit the step records `marker` in *err
 // and finishes.
