@@ -17,6 +17,7 @@
 namespace exoblas {
 
 struct StreamState {
+  std::mutex mutex;  // held by an _async call for its duration: calls on one stream from several threads take turns
   hipStream_t stream = nullptr;
   int device = 0;
   void* scratch[Ctx::kSlots] = {};
@@ -266,6 +267,7 @@ Call::Call(const char* name, Mode mode, void* src) : name_(name), mode_(mode) {
       ctx_.ss = ds.own;
     } else {
       ctx_.ss = stream_state(ds, static_cast<hipStream_t>(src));
+      lock_ = std::unique_lock<std::mutex>(ctx_.ss->mutex);
     }
     ctx_.stream = ctx_.ss->stream;
   } catch (const Error& e) {
