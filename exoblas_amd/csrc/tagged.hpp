@@ -48,8 +48,11 @@ __device__ __forceinline__ T read(unsigned* pub, int64_t g, unsigned marker, uns
   if constexpr (sizeof(T) == 4) {
     return __uint_as_float(poll_word(w64 + g, marker, err));
   } else {
-    const unsigned lo = poll_word(w64 + 2 * g, marker, err);
-    const unsigned hi = poll_word(w64 + 2 * g + 1, marker, err);
+    // both halves loaded before either is checked: one round trip, not two
+    const uint64_t w0 = __hip_atomic_load(w64 + 2 * g, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const uint64_t w1 = __hip_atomic_load(w64 + 2 * g + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const unsigned lo = (w0 >> 32) == marker ? (unsigned)w0 : poll_word(w64 + 2 * g, marker, err);
+    const unsigned hi = (w1 >> 32) == marker ? (unsigned)w1 : poll_word(w64 + 2 * g + 1, marker, err);
     return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
   }
 }
