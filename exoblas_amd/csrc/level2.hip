@@ -311,15 +311,15 @@ __global__ __launch_bounds__(kNT) void gemv_finish_kernel(int64_t M, int splits,
 // and writes y; *err is the timeout word.  The marker comes from the
 // stream's launch generation *gen, advanced by the last column block to
 // complete (ticket *lt), so the launch can be replayed from a hipGraph.
-template <class T, int RU, bool VEC, int TRI, bool UNIT>
-__global__ __launch_bounds__(kNT) void gemv_t_kernel(int64_t Mc, int64_t Nr, T alpha, T beta,
+template <class T, int RU, bool VEC, int TRI, bool UNIT, int NTT = kNT>
+__global__ __launch_bounds__(NTT) void gemv_t_kernel(int64_t Mc, int64_t Nr, T alpha, T beta,
                                                      const T* __restrict__ A, int64_t lda,
                                                      const T* __restrict__ x, int64_t incx, T* y,
                                                      int64_t incy, int64_t rows_per, unsigned* tickets,
                                                      unsigned* gen, unsigned* lt, unsigned* err, unsigned* pub) {
   using V = typename Vec16<T>::type;
   constexpr int VN = Vec16<T>::n;
-  constexpr int NW = kNT / kWave;
+  constexpr int NW = NTT / kWave;
   __shared__ T red[NW][kWave * VN];
   // The generation is loaded now and stored for the workgroup only at the
   // reduction below, so no wave waits for it before its streaming loads.
@@ -402,7 +402,7 @@ __global__ __launch_bounds__(kNT) void gemv_t_kernel(int64_t Mc, int64_t Nr, T a
     else
       y[c * incy] = beta * y[c * incy] + alpha * s;
   };
-  for (int i = threadIdx.x; i < kWave * VN; i += kNT) {
+  for (int i = threadIdx.x; i < kWave * VN; i += NTT) {
     T s = T(0);
 #pragma unroll
     for (int w = 0; w < NW; ++w) s += red[w][i];
@@ -427,7 +427,7 @@ __global__ __launch_bounds__(kNT) void gemv_t_kernel(int64_t Mc, int64_t Nr, T a
   __syncthreads();
   if (!s_last) return;
   const int splits = (int)gridDim.y;
-  for (int i = threadIdx.x; i < kWave * VN; i += kNT) {
+  for (int i = threadIdx.x; i < kWave * VN; i += NTT) {
     const int64_t c = cb0 + i;
     if (c >= Mc) break;
     T t = T(0);
@@ -533,7 +533,13 @@ void launch_gemv_t(Ctx& c, int64_t Mc, int64_t Nr, T alpha, T beta, const T* A, 
     pub = flags + 2;  // 8-byte aligned
   }
   dim3 grid((unsigned)col_blocks, (unsigned)splits);
-  if (vec)
+  // Full FP32 matrices: 16 waves along the rows with 8 rows in flight each
+  // (73 % of HBM peak at 8192 x 8192 against 70 % with 4 waves of 16 rows);
+  // FP64 keeps 4 waves of 16 rows (81 % against 75 %).
+  if (sizeof(T) == 4 && vec && TRI == kFull)
+    hipLaunchKernelGGL((gemv_t_kernel<T, 8, true, TRI, UNIT, 1024>), grid, dim3(1024), 0, c.stream, Mc, Nr, alpha,
+                       beta, A, lda, x, incx, y, incy, rows_per, tickets, gen, lt, err, pub);
+  else if (vec)
     hipLaunchKernelGGL((gemv_t_kernel<T, RU, true, TRI, UNIT>), grid, dim3(kNT), 0, c.stream, Mc, Nr, alpha, beta,
                        A, lda, x, incx, y, incy, rows_per, tickets, gen, lt, err, pub);
   else
