@@ -112,11 +112,24 @@ def prefix(dtype, truth: bool = False) -> str:
 
 def vptr(a: np.ndarray, n: int, inc: int) -> int:
     """Address of element 0 of a vector of n elements, stride inc, stored in a
-    (the lowest-addressed element for inc < 0 is a[0])."""
+    (the lowest-addressed element for inc < 0 is a[0]).  Raises if a is too
+    short to hold the vector: the C restatement would write past its end."""
+    need = (n - 1) * abs(inc) + 1 if n > 0 else 0
+    if a.size < need:
+        raise ValueError(f"vector of {n} at stride {inc} needs {need} elements, buffer has {a.size}")
     base = a.ctypes.data
     if inc < 0 and n > 0:
         base += (n - 1) * (-inc) * a.itemsize
     return base
+
+
+def mptr(a: np.ndarray, rows: int, cols: int, ld: int) -> int:
+    """Address of a row-major rows x cols matrix with leading dimension ld
+    stored in a; raises if a is too short to hold it."""
+    need = (rows - 1) * ld + cols if rows > 0 and cols > 0 else 0
+    if a.size < need:
+        raise ValueError(f"{rows} x {cols} matrix with ld {ld} needs {need} elements, buffer has {a.size}")
+    return a.ctypes.data
 
 
 class Oracle:
@@ -188,54 +201,61 @@ class Oracle:
     def gemv(self, trans, m, n, alpha, beta, A, lda, x, incx, y, incy):
         """Callee view: m, n after the wrapper's swap (A is n x m for Trans)."""
         self._chk(A, x, y)
-        self.f("gemv")(trans, m, n, alpha, beta, A.ctypes.data, lda, vptr(x, n, incx), incx,
+        rows, cols = (m, n) if trans == NO_TRANS else (n, m)
+        self.f("gemv")(trans, m, n, alpha, beta, mptr(A, rows, cols, lda), lda, vptr(x, n, incx), incx,
                        vptr(y, m, incy), incy)
 
     def ger(self, m, n, alpha, x, incx, y, incy, A, lda):
         self._chk(A, x, y)
-        self.f("ger")(m, n, alpha, vptr(x, m, incx), incx, vptr(y, n, incy), incy, A.ctypes.data, lda)
+        self.f("ger")(m, n, alpha, vptr(x, m, incx), incx, vptr(y, n, incy), incy, mptr(A, m, n, lda), lda)
 
     def symv(self, uplo, n, alpha, A, lda, x, incx, beta, y, incy):
         self._chk(A, x, y)
-        self.f("symv")(uplo, n, alpha, A.ctypes.data, lda, vptr(x, n, incx), incx, beta, vptr(y, n, incy),
+        self.f("symv")(uplo, n, alpha, mptr(A, n, n, lda), lda, vptr(x, n, incx), incx, beta, vptr(y, n, incy),
                        incy)
 
     def syr(self, uplo, n, alpha, x, incx, A, lda):
         self._chk(A, x)
-        self.f("syr")(uplo, n, alpha, vptr(x, n, incx), incx, A.ctypes.data, lda)
+        self.f("syr")(uplo, n, alpha, vptr(x, n, incx), incx, mptr(A, n, n, lda), lda)
 
     def syr2(self, uplo, n, alpha, x, incx, y, incy, A, lda):
         self._chk(A, x, y)
-        self.f("syr2")(uplo, n, alpha, vptr(x, n, incx), incx, vptr(y, n, incy), incy, A.ctypes.data, lda)
+        self.f("syr2")(uplo, n, alpha, vptr(x, n, incx), incx, vptr(y, n, incy), incy, mptr(A, n, n, lda), lda)
 
     def trmv(self, uplo, trans, diag, n, x, incx, A, lda):
         self._chk(A, x)
-        self.f("trmv")(uplo, trans, diag, n, vptr(x, n, incx), incx, A.ctypes.data, lda)
+        self.f("trmv")(uplo, trans, diag, n, vptr(x, n, incx), incx, mptr(A, n, n, lda), lda)
 
     def trsv(self, uplo, trans, diag, n, x, incx, A, lda):
         self._chk(A, x)
-        self.f("trsv")(uplo, trans, diag, n, vptr(x, n, incx), incx, A.ctypes.data, lda)
+        self.f("trsv")(uplo, trans, diag, n, vptr(x, n, incx), incx, mptr(A, n, n, lda), lda)
 
     # level 3 -----------------------------------------------------------
     def gemm(self, ta, tb, M, N, K, alpha, A, lda, B, ldb, C, ldc):
         self._chk(A, B, C)
-        self.f("gemm")(ta, tb, M, N, K, alpha, A.ctypes.data, lda, B.ctypes.data, ldb, C.ctypes.data, ldc)
+        ra, ca = (M, K) if ta == NO_TRANS else (K, M)
+        rb, cb = (K, N) if tb == NO_TRANS else (N, K)
+        self.f("gemm")(ta, tb, M, N, K, alpha, mptr(A, ra, ca, lda), lda, mptr(B, rb, cb, ldb), ldb,
+                       mptr(C, M, N, ldc), ldc)
 
     def mscal(self, M, N, alpha, A, lda):
         self._chk(A)
-        self.f("mscal")(M, N, alpha, A.ctypes.data, lda)
+        self.f("mscal")(M, N, alpha, mptr(A, M, N, lda), lda)
 
     def trmscal(self, uplo, N, alpha, A, lda):
         self._chk(A)
-        self.f("trmscal")(uplo, N, alpha, A.ctypes.data, lda)
+        self.f("trmscal")(uplo, N, alpha, mptr(A, N, N, lda), lda)
 
     def symm(self, side, uplo, M, N, alpha, A, lda, B, ldb, C, ldc):
         self._chk(A, B, C)
-        self.f("symm")(side, uplo, M, N, alpha, A.ctypes.data, lda, B.ctypes.data, ldb, C.ctypes.data, ldc)
+        na = M if side == LEFT else N
+        self.f("symm")(side, uplo, M, N, alpha, mptr(A, na, na, lda), lda, mptr(B, M, N, ldb), ldb,
+                       mptr(C, M, N, ldc), ldc)
 
     def syrk(self, uplo, trans, N, K, alpha, A, lda, C, ldc):
         self._chk(A, C)
-        self.f("syrk")(uplo, trans, N, K, alpha, A.ctypes.data, lda, C.ctypes.data, ldc)
+        ra, ca = (N, K) if trans == NO_TRANS else (K, N)
+        self.f("syrk")(uplo, trans, N, K, alpha, mptr(A, ra, ca, lda), lda, mptr(C, N, N, ldc), ldc)
 
 
 def strided(n: int, inc: int, rng: np.random.Generator, dtype, fill: Optional[float] = None) -> np.ndarray:

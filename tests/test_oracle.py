@@ -193,6 +193,23 @@ def test_gemv(dt, trans, m, n, lda_diff, rng):
                 assert_bound(y1, wrong, np.abs(B.astype(np.float64)).T @ np.abs(x[:sq]), sq, dt, c=1.0)
 
 
+def test_oracle_rejects_short_buffers():
+    # CBLAS sizes passed where the callee's (swapped for Trans) are expected:
+    # y would receive 30 values in a buffer of 20.  The binding refuses
+    # before the C restatement writes past the array.
+    o = O(np.float32)
+    A = np.zeros((30, 20), np.float32)
+    x = np.zeros(30, np.float32)
+    y = np.zeros(20, np.float32)
+    with pytest.raises(ValueError):
+        o.gemv(TRANS, 30, 20, 1.0, 0.0, A, 20, x, 1, y, 1)
+    o.gemv(TRANS, 20, 30, 1.0, 0.0, A, 20, x, 1, y, 1)  # the callee view fits
+    with pytest.raises(ValueError):
+        o.axpy(5, 1.0, x, 1, y[:4].copy(), -1)
+    with pytest.raises(ValueError):
+        o.gemm(NO_TRANS, NO_TRANS, 30, 20, 21, 1.0, A, 20, A, 20, A, 20)
+
+
 @pytest.mark.parametrize("dt", DT)
 def test_ger(dt, rng):
     m, n, lda = 103, 41, 46
