@@ -230,9 +230,32 @@ def _make(p: str):
         a = L.win2(_addr(A), lda)
         _call(f"exo_{p}syrk_rm_stride_1", Uplo, Trans, N, K, ref(alpha), a, a, a, a, c)
 
+    def syr2k(Order, Uplo, Trans, N, K, alpha, A, lda, B, ldb, beta, C, ldc):
+        # test/level3/syr2k/exo_syr2k_wrapper.h:9-32: trmscal(beta) then syr2k
+        if Order != ROW_MAJOR:
+            raise UnsupportedParameterException("syr2k::Order must be Row Major")
+        c = L.win2(_addr(C), ldc)
+        _call(f"exo_{p}trmscal_rm_stride_1", Uplo, N, ref(beta), c)
+        a, b = L.win2(_addr(A), lda), L.win2(_addr(B), ldb)
+        _call(f"exo_{p}syr2k_rm_stride_1", Uplo, Trans, N, K, ref(alpha), a, a, b, b, a, a, b, b, c)
+
+    def trmm(Order, Side, Uplo, TransA, Diag, M, N, alpha, A, lda, B, ldb):
+        # test/level3/strmm/exo_strmm.h:7-56: Left side only (Right: no call),
+        # Diag passed as (Diag == CblasUnit); Trans and ConjTrans both transpose
+        if Order != ROW_MAJOR:
+            raise UnsupportedParameterException("Unsupported Exception")
+        if Side != L.LEFT:
+            return
+        u = "Upper" if Uplo == L.UPPER else "Lower"
+        t = "NonTrans" if TransA == L.NO_TRANS else "Trans"
+        _call(f"exo_{p}trmm_row_major_Left_{u}_{t}", M, N, ref(alpha), L.win2(_addr(A), lda),
+              L.win2(_addr(B), ldb), int(Diag == L.UNIT))
+
     ns.update(asum=asum, axpy=axpy, dot=dot, copy=_pair("copy"), swap=_pair("swap"), rot=rot, rotm=rotm,
               scal=scal, gemv=gemv, ger=ger, symv=symv, syr=syr, syr2=syr2, trmv=_tr("trmv"),
-              trsv=_tr("trsv"), gemm=gemm, symm=symm, syrk=syrk)
+              trsv=_tr("trsv"), gemm=gemm, symm=symm, syrk=syrk, syr2k=syr2k)
+    if p == "s":  # the reference exports strmm only (src/level3/trmm.py:133-142)
+        ns.update(trmm=trmm)
     return {f"exo_{p}{k}": v for k, v in ns.items()}
 
 

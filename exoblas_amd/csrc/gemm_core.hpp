@@ -8,9 +8,12 @@
 //   kKR        X(r, k) = p[k * ld + r]      (r contiguous: A^T, B NoTrans)
 //   kSymLower  X(r, k) = S(r, k), S symmetric, lower triangle stored
 //   kSymUpper  X(r, k) = S(r, k), upper triangle stored
-// A symmetric operand resolves per K tile to a plain kRK or kKR copy when the
-// tile lies on one side of the diagonal, and to an element-wise select on the
-// diagonal.
+//   tri_form(base, upper, unit)  X(r, k) = the kRK / kKR value where k <= r
+//              (lower) or k >= r (upper), 0 elsewhere, 1 on the diagonal when
+//              unit (triangular operands of trmm)
+// A symmetric or triangular operand resolves per K tile to a plain kRK or kKR
+// copy when the tile lies on one side of the diagonal (or to zeros outside a
+// triangle), and to an element-wise select on the diagonal.
 //
 // Block tile BM x BN, BK-deep K tiles double-buffered in LDS in k-major layout
 // ([BK][BM + pad]).  A workgroup of NT = 64 * (BM/WM) * (BN/WN) threads; each
@@ -31,7 +34,12 @@
 namespace exoblas {
 namespace gemm {
 
-enum Form : int { kRK = 0, kKR = 1, kSymLower = 2, kSymUpper = 3 };
+enum Form : int { kRK = 0, kKR = 1, kSymLower = 2, kSymUpper = 3, kTri = 4 };
+
+// Triangular form codes kTri + base + 2 * upper + 4 * unit (base kRK / kKR).
+__host__ __device__ constexpr int tri_form(int base, bool upper, bool unit) {
+  return kTri + base + (upper ? 2 : 0) + (unit ? 4 : 0);
+}
 enum Tri : int { kFull = 0, kLower = 1, kUpper = 2 };
 
 template <class T>
@@ -81,7 +89,13 @@ struct TileLoader {
   int mode;  // 0: kRK vector, 1: kKR vector, 2: element-wise
 
   __device__ __forceinline__ static T elem(const Operand<T>& op, int form, int64_t r, int64_t k) {
-    // form is kRK or kKR, or symmetric (select by triangle)
+    // form is kRK or kKR, symmetric (select by triangle) or triangular
+    if (form >= kTri) {
+      const int f = form - kTri;
+      if (r == k && (f & 4)) return T(1);
+      if ((f & 2) ? k < r : k > r) return T(0);
+      form = f & 1;
+    }
     if (form == kSymLower) form = r >= k ? kRK : kKR;
     if (form == kSymUpper) form = r <= k ? kRK : kKR;
     return form == kRK ? op.p[r * op.ld + k] : op.p[k * op.ld + r];
@@ -95,6 +109,18 @@ struct TileLoader {
     } else if (form == kSymUpper) {
       if (r0 + R - 1 <= k0) form = kRK;
       else if (r0 > k0 + BK - 1) form = kKR;
+    } else if (form >= kTri) {
+      const int f = form - kTri;
+      // strictly inside the triangle: the plain form; wholly outside: zeros
+      const bool inside = (f & 2) ? k0 > r0 + R - 1 : k0 + BK - 1 < r0;
+      const bool outside = (f & 2) ? k0 + BK - 1 < r0 : k0 > r0 + R - 1;
+      if (inside) form = f & 1;
+      if (outside) {
+        mode = 2;
+#pragma unroll
+        for (int j = 0; j < NE; ++j) reg[j] = T(0);
+        return;
+      }
     }
     const bool full = r0 + R <= rows && k0 + BK <= K;
     const int tid = threadIdx.x;
@@ -181,6 +207,18 @@ struct Cfg {
   static constexpr size_t LDS_BYTES = sizeof(T) * 2 * (LDS_A + LDS_B);
   static_assert(BK % 4 == 0, "BK is a multiple of the MFMA K (4)");
 };
+
+// The part of [kb, ke) where a triangular A operand can be nonzero for rows
+// [m0, m0 + BM): k <= r (lower) ends at m0 + BM, k >= r (upper) starts at m0.
+template <class T>
+__device__ __forceinline__ void tri_k_window(const Operand<T>& A, int64_t m0, int BM, int64_t& kb, int64_t& ke) {
+  if (A.form < kTri) return;
+  if ((A.form - kTri) & 2) {
+    if (kb < m0) kb = m0;
+  } else if (ke > m0 + BM) {
+    ke = m0 + BM;
+  }
+}
 
 // Accumulates one block tile of  sum_k X_A(m, k) X_B(n, k)  for k in [kb, ke)
 // into acc; lds points at LDS_BYTES of shared memory.

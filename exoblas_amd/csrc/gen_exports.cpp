@@ -22,6 +22,8 @@
 #include "exo_mscal.h"
 #include "exo_symm.h"
 #include "exo_syrk.h"
+#include "exo_syr2k.h"
+#include "exo_trmm.h"
 #include "exoblas_amd_async.h"
 #include "exoblas_amd_ext.h"
 
@@ -3090,6 +3092,470 @@ extern "C" int exoblas_dsyrk_rm_stride_1_async(hipStream_t stream, int_fast32_t 
   const double alpha_v = call.scalar(alpha);
   if (!call.ok()) return call.status();
   call.run([&] { exoblas::ops::syrk_rm<double>(call.ctx(), (int64_t)Uplo, (int64_t)Trans, (int64_t)N, (int64_t)K, alpha_v, call.dev<const double>(0), (int64_t)A.strides[0], call.dev<const double>(1), (int64_t)A_.strides[0], call.dev<const double>(2), (int64_t)AT.strides[0], call.dev<const double>(3), (int64_t)AT_.strides[0], call.dev<double>(4), (int64_t)C.strides[0]); });
+  return call.finish();
+}
+
+// src/level3/syr2k.py:14-39
+extern "C" void exo_ssyr2k_rm_stride_any(void *ctxt, int_fast32_t Uplo, int_fast32_t Trans, int_fast32_t N, int_fast32_t K, const float *alpha, struct exo_win_2f32c A, struct exo_win_2f32c A_, struct exo_win_2f32c B, struct exo_win_2f32c B_, struct exo_win_2f32c AT, struct exo_win_2f32c AT_, struct exo_win_2f32c BT, struct exo_win_2f32c BT_, struct exo_win_2f32 C) {
+  exoblas::Call call("exo_ssyr2k_rm_stride_any", exoblas::Call::kSync, ctxt);
+  if (!(Uplo == 121 || Uplo == 122)) return (void)call.fail_enum("Uplo", Uplo);
+  if (!(Trans == 111 || Trans == 112 || Trans == 113)) return (void)call.fail_enum("Trans", Trans);
+  if (N < 0) return (void)call.fail_size("N", N);
+  if (K < 0) return (void)call.fail_size("K", K);
+  if ((Trans == 111) && A.strides[1] != 1) return (void)call.fail_stride("A", A.strides[1]);
+  if ((Trans == 111) && N > 1 && A.strides[0] < K) return (void)call.fail_ld("A", A.strides[0]);
+  if ((Trans == 111) && A_.strides[1] != 1) return (void)call.fail_stride("A_", A_.strides[1]);
+  if ((Trans == 111) && N > 1 && A_.strides[0] < K) return (void)call.fail_ld("A_", A_.strides[0]);
+  if ((Trans == 111) && B.strides[1] != 1) return (void)call.fail_stride("B", B.strides[1]);
+  if ((Trans == 111) && N > 1 && B.strides[0] < K) return (void)call.fail_ld("B", B.strides[0]);
+  if ((Trans == 111) && B_.strides[1] != 1) return (void)call.fail_stride("B_", B_.strides[1]);
+  if ((Trans == 111) && N > 1 && B_.strides[0] < K) return (void)call.fail_ld("B_", B_.strides[0]);
+  if ((Trans != 111) && AT.strides[1] != 1) return (void)call.fail_stride("AT", AT.strides[1]);
+  if ((Trans != 111) && K > 1 && AT.strides[0] < N) return (void)call.fail_ld("AT", AT.strides[0]);
+  if ((Trans != 111) && AT_.strides[1] != 1) return (void)call.fail_stride("AT_", AT_.strides[1]);
+  if ((Trans != 111) && K > 1 && AT_.strides[0] < N) return (void)call.fail_ld("AT_", AT_.strides[0]);
+  if ((Trans != 111) && BT.strides[1] != 1) return (void)call.fail_stride("BT", BT.strides[1]);
+  if ((Trans != 111) && K > 1 && BT.strides[0] < N) return (void)call.fail_ld("BT", BT.strides[0]);
+  if ((Trans != 111) && BT_.strides[1] != 1) return (void)call.fail_stride("BT_", BT_.strides[1]);
+  if ((Trans != 111) && K > 1 && BT_.strides[0] < N) return (void)call.fail_ld("BT_", BT_.strides[0]);
+  if ((true) && C.strides[1] != 1) return (void)call.fail_stride("C", C.strides[1]);
+  if ((true) && N > 1 && C.strides[0] < N) return (void)call.fail_ld("C", C.strides[0]);
+  call.window(0, A.data, 0, exoblas::span2_hi(N, K, A.strides[0]), sizeof(float), false, Trans == 111);
+  call.window(1, A_.data, 0, exoblas::span2_hi(N, K, A_.strides[0]), sizeof(float), false, Trans == 111);
+  call.window(2, B.data, 0, exoblas::span2_hi(N, K, B.strides[0]), sizeof(float), false, Trans == 111);
+  call.window(3, B_.data, 0, exoblas::span2_hi(N, K, B_.strides[0]), sizeof(float), false, Trans == 111);
+  call.window(4, AT.data, 0, exoblas::span2_hi(K, N, AT.strides[0]), sizeof(float), false, Trans != 111);
+  call.window(5, AT_.data, 0, exoblas::span2_hi(K, N, AT_.strides[0]), sizeof(float), false, Trans != 111);
+  call.window(6, BT.data, 0, exoblas::span2_hi(K, N, BT.strides[0]), sizeof(float), false, Trans != 111);
+  call.window(7, BT_.data, 0, exoblas::span2_hi(K, N, BT_.strides[0]), sizeof(float), false, Trans != 111);
+  call.window(8, C.data, 0, exoblas::span2_hi(N, N, C.strides[0]), sizeof(float), true, true);
+  if (!call.stage()) return (void)call.status();
+  const float alpha_v = call.scalar(alpha);
+  if (!call.ok()) return (void)call.status();
+  call.run([&] { exoblas::ops::syr2k_rm<float>(call.ctx(), (int64_t)Uplo, (int64_t)Trans, (int64_t)N, (int64_t)K, alpha_v, call.dev<const float>(0), (int64_t)A.strides[0], call.dev<const float>(1), (int64_t)A_.strides[0], call.dev<const float>(2), (int64_t)B.strides[0], call.dev<const float>(3), (int64_t)B_.strides[0], call.dev<const float>(4), (int64_t)AT.strides[0], call.dev<const float>(5), (int64_t)AT_.strides[0], call.dev<const float>(6), (int64_t)BT.strides[0], call.dev<const float>(7), (int64_t)BT_.strides[0], call.dev<float>(8), (int64_t)C.strides[0]); });
+  return (void)call.finish();
+}
+extern "C" int exoblas_ssyr2k_rm_stride_any_async(hipStream_t stream, int_fast32_t Uplo, int_fast32_t Trans, int_fast32_t N, int_fast32_t K, const float *alpha, struct exo_win_2f32c A, struct exo_win_2f32c A_, struct exo_win_2f32c B, struct exo_win_2f32c B_, struct exo_win_2f32c AT, struct exo_win_2f32c AT_, struct exo_win_2f32c BT, struct exo_win_2f32c BT_, struct exo_win_2f32 C) {
+  exoblas::Call call("exoblas_ssyr2k_rm_stride_any_async", exoblas::Call::kAsync, stream);
+  if (!(Uplo == 121 || Uplo == 122)) return call.fail_enum("Uplo", Uplo);
+  if (!(Trans == 111 || Trans == 112 || Trans == 113)) return call.fail_enum("Trans", Trans);
+  if (N < 0) return call.fail_size("N", N);
+  if (K < 0) return call.fail_size("K", K);
+  if ((Trans == 111) && A.strides[1] != 1) return call.fail_stride("A", A.strides[1]);
+  if ((Trans == 111) && N > 1 && A.strides[0] < K) return call.fail_ld("A", A.strides[0]);
+  if ((Trans == 111) && A_.strides[1] != 1) return call.fail_stride("A_", A_.strides[1]);
+  if ((Trans == 111) && N > 1 && A_.strides[0] < K) return call.fail_ld("A_", A_.strides[0]);
+  if ((Trans == 111) && B.strides[1] != 1) return call.fail_stride("B", B.strides[1]);
+  if ((Trans == 111) && N > 1 && B.strides[0] < K) return call.fail_ld("B", B.strides[0]);
+  if ((Trans == 111) && B_.strides[1] != 1) return call.fail_stride("B_", B_.strides[1]);
+  if ((Trans == 111) && N > 1 && B_.strides[0] < K) return call.fail_ld("B_", B_.strides[0]);
+  if ((Trans != 111) && AT.strides[1] != 1) return call.fail_stride("AT", AT.strides[1]);
+  if ((Trans != 111) && K > 1 && AT.strides[0] < N) return call.fail_ld("AT", AT.strides[0]);
+  if ((Trans != 111) && AT_.strides[1] != 1) return call.fail_stride("AT_", AT_.strides[1]);
+  if ((Trans != 111) && K > 1 && AT_.strides[0] < N) return call.fail_ld("AT_", AT_.strides[0]);
+  if ((Trans != 111) && BT.strides[1] != 1) return call.fail_stride("BT", BT.strides[1]);
+  if ((Trans != 111) && K > 1 && BT.strides[0] < N) return call.fail_ld("BT", BT.strides[0]);
+  if ((Trans != 111) && BT_.strides[1] != 1) return call.fail_stride("BT_", BT_.strides[1]);
+  if ((Trans != 111) && K > 1 && BT_.strides[0] < N) return call.fail_ld("BT_", BT_.strides[0]);
+  if ((true) && C.strides[1] != 1) return call.fail_stride("C", C.strides[1]);
+  if ((true) && N > 1 && C.strides[0] < N) return call.fail_ld("C", C.strides[0]);
+  call.window(0, A.data, 0, exoblas::span2_hi(N, K, A.strides[0]), sizeof(float), false, Trans == 111);
+  call.window(1, A_.data, 0, exoblas::span2_hi(N, K, A_.strides[0]), sizeof(float), false, Trans == 111);
+  call.window(2, B.data, 0, exoblas::span2_hi(N, K, B.strides[0]), sizeof(float), false, Trans == 111);
+  call.window(3, B_.data, 0, exoblas::span2_hi(N, K, B_.strides[0]), sizeof(float), false, Trans == 111);
+  call.window(4, AT.data, 0, exoblas::span2_hi(K, N, AT.strides[0]), sizeof(float), false, Trans != 111);
+  call.window(5, AT_.data, 0, exoblas::span2_hi(K, N, AT_.strides[0]), sizeof(float), false, Trans != 111);
+  call.window(6, BT.data, 0, exoblas::span2_hi(K, N, BT.strides[0]), sizeof(float), false, Trans != 111);
+  call.window(7, BT_.data, 0, exoblas::span2_hi(K, N, BT_.strides[0]), sizeof(float), false, Trans != 111);
+  call.window(8, C.data, 0, exoblas::span2_hi(N, N, C.strides[0]), sizeof(float), true, true);
+  if (!call.stage()) return call.status();
+  const float alpha_v = call.scalar(alpha);
+  if (!call.ok()) return call.status();
+  call.run([&] { exoblas::ops::syr2k_rm<float>(call.ctx(), (int64_t)Uplo, (int64_t)Trans, (int64_t)N, (int64_t)K, alpha_v, call.dev<const float>(0), (int64_t)A.strides[0], call.dev<const float>(1), (int64_t)A_.strides[0], call.dev<const float>(2), (int64_t)B.strides[0], call.dev<const float>(3), (int64_t)B_.strides[0], call.dev<const float>(4), (int64_t)AT.strides[0], call.dev<const float>(5), (int64_t)AT_.strides[0], call.dev<const float>(6), (int64_t)BT.strides[0], call.dev<const float>(7), (int64_t)BT_.strides[0], call.dev<float>(8), (int64_t)C.strides[0]); });
+  return call.finish();
+}
+
+// src/level3/syr2k.py:14-39
+extern "C" void exo_ssyr2k_rm_stride_1(void *ctxt, int_fast32_t Uplo, int_fast32_t Trans, int_fast32_t N, int_fast32_t K, const float *alpha, struct exo_win_2f32c A, struct exo_win_2f32c A_, struct exo_win_2f32c B, struct exo_win_2f32c B_, struct exo_win_2f32c AT, struct exo_win_2f32c AT_, struct exo_win_2f32c BT, struct exo_win_2f32c BT_, struct exo_win_2f32 C) {
+  exoblas::Call call("exo_ssyr2k_rm_stride_1", exoblas::Call::kSync, ctxt);
+  if (!(Uplo == 121 || Uplo == 122)) return (void)call.fail_enum("Uplo", Uplo);
+  if (!(Trans == 111 || Trans == 112 || Trans == 113)) return (void)call.fail_enum("Trans", Trans);
+  if (N < 0) return (void)call.fail_size("N", N);
+  if (K < 0) return (void)call.fail_size("K", K);
+  if ((Trans == 111) && A.strides[1] != 1) return (void)call.fail_stride("A", A.strides[1]);
+  if ((Trans == 111) && N > 1 && A.strides[0] < K) return (void)call.fail_ld("A", A.strides[0]);
+  if ((Trans == 111) && A_.strides[1] != 1) return (void)call.fail_stride("A_", A_.strides[1]);
+  if ((Trans == 111) && N > 1 && A_.strides[0] < K) return (void)call.fail_ld("A_", A_.strides[0]);
+  if ((Trans == 111) && B.strides[1] != 1) return (void)call.fail_stride("B", B.strides[1]);
+  if ((Trans == 111) && N > 1 && B.strides[0] < K) return (void)call.fail_ld("B", B.strides[0]);
+  if ((Trans == 111) && B_.strides[1] != 1) return (void)call.fail_stride("B_", B_.strides[1]);
+  if ((Trans == 111) && N > 1 && B_.strides[0] < K) return (void)call.fail_ld("B_", B_.strides[0]);
+  if ((Trans != 111) && AT.strides[1] != 1) return (void)call.fail_stride("AT", AT.strides[1]);
+  if ((Trans != 111) && K > 1 && AT.strides[0] < N) return (void)call.fail_ld("AT", AT.strides[0]);
+  if ((Trans != 111) && AT_.strides[1] != 1) return (void)call.fail_stride("AT_", AT_.strides[1]);
+  if ((Trans != 111) && K > 1 && AT_.strides[0] < N) return (void)call.fail_ld("AT_", AT_.strides[0]);
+  if ((Trans != 111) && BT.strides[1] != 1) return (void)call.fail_stride("BT", BT.strides[1]);
+  if ((Trans != 111) && K > 1 && BT.strides[0] < N) return (void)call.fail_ld("BT", BT.strides[0]);
+  if ((Trans != 111) && BT_.strides[1] != 1) return (void)call.fail_stride("BT_", BT_.strides[1]);
+  if ((Trans != 111) && K > 1 && BT_.strides[0] < N) return (void)call.fail_ld("BT_", BT_.strides[0]);
+  if ((true) && C.strides[1] != 1) return (void)call.fail_stride("C", C.strides[1]);
+  if ((true) && N > 1 && C.strides[0] < N) return (void)call.fail_ld("C", C.strides[0]);
+  call.window(0, A.data, 0, exoblas::span2_hi(N, K, A.strides[0]), sizeof(float), false, Trans == 111);
+  call.window(1, A_.data, 0, exoblas::span2_hi(N, K, A_.strides[0]), sizeof(float), false, Trans == 111);
+  call.window(2, B.data, 0, exoblas::span2_hi(N, K, B.strides[0]), sizeof(float), false, Trans == 111);
+  call.window(3, B_.data, 0, exoblas::span2_hi(N, K, B_.strides[0]), sizeof(float), false, Trans == 111);
+  call.window(4, AT.data, 0, exoblas::span2_hi(K, N, AT.strides[0]), sizeof(float), false, Trans != 111);
+  call.window(5, AT_.data, 0, exoblas::span2_hi(K, N, AT_.strides[0]), sizeof(float), false, Trans != 111);
+  call.window(6, BT.data, 0, exoblas::span2_hi(K, N, BT.strides[0]), sizeof(float), false, Trans != 111);
+  call.window(7, BT_.data, 0, exoblas::span2_hi(K, N, BT_.strides[0]), sizeof(float), false, Trans != 111);
+  call.window(8, C.data, 0, exoblas::span2_hi(N, N, C.strides[0]), sizeof(float), true, true);
+  if (!call.stage()) return (void)call.status();
+  const float alpha_v = call.scalar(alpha);
+  if (!call.ok()) return (void)call.status();
+  call.run([&] { exoblas::ops::syr2k_rm<float>(call.ctx(), (int64_t)Uplo, (int64_t)Trans, (int64_t)N, (int64_t)K, alpha_v, call.dev<const float>(0), (int64_t)A.strides[0], call.dev<const float>(1), (int64_t)A_.strides[0], call.dev<const float>(2), (int64_t)B.strides[0], call.dev<const float>(3), (int64_t)B_.strides[0], call.dev<const float>(4), (int64_t)AT.strides[0], call.dev<const float>(5), (int64_t)AT_.strides[0], call.dev<const float>(6), (int64_t)BT.strides[0], call.dev<const float>(7), (int64_t)BT_.strides[0], call.dev<float>(8), (int64_t)C.strides[0]); });
+  return (void)call.finish();
+}
+extern "C" int exoblas_ssyr2k_rm_stride_1_async(hipStream_t stream, int_fast32_t Uplo, int_fast32_t Trans, int_fast32_t N, int_fast32_t K, const float *alpha, struct exo_win_2f32c A, struct exo_win_2f32c A_, struct exo_win_2f32c B, struct exo_win_2f32c B_, struct exo_win_2f32c AT, struct exo_win_2f32c AT_, struct exo_win_2f32c BT, struct exo_win_2f32c BT_, struct exo_win_2f32 C) {
+  exoblas::Call call("exoblas_ssyr2k_rm_stride_1_async", exoblas::Call::kAsync, stream);
+  if (!(Uplo == 121 || Uplo == 122)) return call.fail_enum("Uplo", Uplo);
+  if (!(Trans == 111 || Trans == 112 || Trans == 113)) return call.fail_enum("Trans", Trans);
+  if (N < 0) return call.fail_size("N", N);
+  if (K < 0) return call.fail_size("K", K);
+  if ((Trans == 111) && A.strides[1] != 1) return call.fail_stride("A", A.strides[1]);
+  if ((Trans == 111) && N > 1 && A.strides[0] < K) return call.fail_ld("A", A.strides[0]);
+  if ((Trans == 111) && A_.strides[1] != 1) return call.fail_stride("A_", A_.strides[1]);
+  if ((Trans == 111) && N > 1 && A_.strides[0] < K) return call.fail_ld("A_", A_.strides[0]);
+  if ((Trans == 111) && B.strides[1] != 1) return call.fail_stride("B", B.strides[1]);
+  if ((Trans == 111) && N > 1 && B.strides[0] < K) return call.fail_ld("B", B.strides[0]);
+  if ((Trans == 111) && B_.strides[1] != 1) return call.fail_stride("B_", B_.strides[1]);
+  if ((Trans == 111) && N > 1 && B_.strides[0] < K) return call.fail_ld("B_", B_.strides[0]);
+  if ((Trans != 111) && AT.strides[1] != 1) return call.fail_stride("AT", AT.strides[1]);
+  if ((Trans != 111) && K > 1 && AT.strides[0] < N) return call.fail_ld("AT", AT.strides[0]);
+  if ((Trans != 111) && AT_.strides[1] != 1) return call.fail_stride("AT_", AT_.strides[1]);
+  if ((Trans != 111) && K > 1 && AT_.strides[0] < N) return call.fail_ld("AT_", AT_.strides[0]);
+  if ((Trans != 111) && BT.strides[1] != 1) return call.fail_stride("BT", BT.strides[1]);
+  if ((Trans != 111) && K > 1 && BT.strides[0] < N) return call.fail_ld("BT", BT.strides[0]);
+  if ((Trans != 111) && BT_.strides[1] != 1) return call.fail_stride("BT_", BT_.strides[1]);
+  if ((Trans != 111) && K > 1 && BT_.strides[0] < N) return call.fail_ld("BT_", BT_.strides[0]);
+  if ((true) && C.strides[1] != 1) return call.fail_stride("C", C.strides[1]);
+  if ((true) && N > 1 && C.strides[0] < N) return call.fail_ld("C", C.strides[0]);
+  call.window(0, A.data, 0, exoblas::span2_hi(N, K, A.strides[0]), sizeof(float), false, Trans == 111);
+  call.window(1, A_.data, 0, exoblas::span2_hi(N, K, A_.strides[0]), sizeof(float), false, Trans == 111);
+  call.window(2, B.data, 0, exoblas::span2_hi(N, K, B.strides[0]), sizeof(float), false, Trans == 111);
+  call.window(3, B_.data, 0, exoblas::span2_hi(N, K, B_.strides[0]), sizeof(float), false, Trans == 111);
+  call.window(4, AT.data, 0, exoblas::span2_hi(K, N, AT.strides[0]), sizeof(float), false, Trans != 111);
+  call.window(5, AT_.data, 0, exoblas::span2_hi(K, N, AT_.strides[0]), sizeof(float), false, Trans != 111);
+  call.window(6, BT.data, 0, exoblas::span2_hi(K, N, BT.strides[0]), sizeof(float), false, Trans != 111);
+  call.window(7, BT_.data, 0, exoblas::span2_hi(K, N, BT_.strides[0]), sizeof(float), false, Trans != 111);
+  call.window(8, C.data, 0, exoblas::span2_hi(N, N, C.strides[0]), sizeof(float), true, true);
+  if (!call.stage()) return call.status();
+  const float alpha_v = call.scalar(alpha);
+  if (!call.ok()) return call.status();
+  call.run([&] { exoblas::ops::syr2k_rm<float>(call.ctx(), (int64_t)Uplo, (int64_t)Trans, (int64_t)N, (int64_t)K, alpha_v, call.dev<const float>(0), (int64_t)A.strides[0], call.dev<const float>(1), (int64_t)A_.strides[0], call.dev<const float>(2), (int64_t)B.strides[0], call.dev<const float>(3), (int64_t)B_.strides[0], call.dev<const float>(4), (int64_t)AT.strides[0], call.dev<const float>(5), (int64_t)AT_.strides[0], call.dev<const float>(6), (int64_t)BT.strides[0], call.dev<const float>(7), (int64_t)BT_.strides[0], call.dev<float>(8), (int64_t)C.strides[0]); });
+  return call.finish();
+}
+
+// src/level3/syr2k.py:14-39
+extern "C" void exo_dsyr2k_rm_stride_any(void *ctxt, int_fast32_t Uplo, int_fast32_t Trans, int_fast32_t N, int_fast32_t K, const double *alpha, struct exo_win_2f64c A, struct exo_win_2f64c A_, struct exo_win_2f64c B, struct exo_win_2f64c B_, struct exo_win_2f64c AT, struct exo_win_2f64c AT_, struct exo_win_2f64c BT, struct exo_win_2f64c BT_, struct exo_win_2f64 C) {
+  exoblas::Call call("exo_dsyr2k_rm_stride_any", exoblas::Call::kSync, ctxt);
+  if (!(Uplo == 121 || Uplo == 122)) return (void)call.fail_enum("Uplo", Uplo);
+  if (!(Trans == 111 || Trans == 112 || Trans == 113)) return (void)call.fail_enum("Trans", Trans);
+  if (N < 0) return (void)call.fail_size("N", N);
+  if (K < 0) return (void)call.fail_size("K", K);
+  if ((Trans == 111) && A.strides[1] != 1) return (void)call.fail_stride("A", A.strides[1]);
+  if ((Trans == 111) && N > 1 && A.strides[0] < K) return (void)call.fail_ld("A", A.strides[0]);
+  if ((Trans == 111) && A_.strides[1] != 1) return (void)call.fail_stride("A_", A_.strides[1]);
+  if ((Trans == 111) && N > 1 && A_.strides[0] < K) return (void)call.fail_ld("A_", A_.strides[0]);
+  if ((Trans == 111) && B.strides[1] != 1) return (void)call.fail_stride("B", B.strides[1]);
+  if ((Trans == 111) && N > 1 && B.strides[0] < K) return (void)call.fail_ld("B", B.strides[0]);
+  if ((Trans == 111) && B_.strides[1] != 1) return (void)call.fail_stride("B_", B_.strides[1]);
+  if ((Trans == 111) && N > 1 && B_.strides[0] < K) return (void)call.fail_ld("B_", B_.strides[0]);
+  if ((Trans != 111) && AT.strides[1] != 1) return (void)call.fail_stride("AT", AT.strides[1]);
+  if ((Trans != 111) && K > 1 && AT.strides[0] < N) return (void)call.fail_ld("AT", AT.strides[0]);
+  if ((Trans != 111) && AT_.strides[1] != 1) return (void)call.fail_stride("AT_", AT_.strides[1]);
+  if ((Trans != 111) && K > 1 && AT_.strides[0] < N) return (void)call.fail_ld("AT_", AT_.strides[0]);
+  if ((Trans != 111) && BT.strides[1] != 1) return (void)call.fail_stride("BT", BT.strides[1]);
+  if ((Trans != 111) && K > 1 && BT.strides[0] < N) return (void)call.fail_ld("BT", BT.strides[0]);
+  if ((Trans != 111) && BT_.strides[1] != 1) return (void)call.fail_stride("BT_", BT_.strides[1]);
+  if ((Trans != 111) && K > 1 && BT_.strides[0] < N) return (void)call.fail_ld("BT_", BT_.strides[0]);
+  if ((true) && C.strides[1] != 1) return (void)call.fail_stride("C", C.strides[1]);
+  if ((true) && N > 1 && C.strides[0] < N) return (void)call.fail_ld("C", C.strides[0]);
+  call.window(0, A.data, 0, exoblas::span2_hi(N, K, A.strides[0]), sizeof(double), false, Trans == 111);
+  call.window(1, A_.data, 0, exoblas::span2_hi(N, K, A_.strides[0]), sizeof(double), false, Trans == 111);
+  call.window(2, B.data, 0, exoblas::span2_hi(N, K, B.strides[0]), sizeof(double), false, Trans == 111);
+  call.window(3, B_.data, 0, exoblas::span2_hi(N, K, B_.strides[0]), sizeof(double), false, Trans == 111);
+  call.window(4, AT.data, 0, exoblas::span2_hi(K, N, AT.strides[0]), sizeof(double), false, Trans != 111);
+  call.window(5, AT_.data, 0, exoblas::span2_hi(K, N, AT_.strides[0]), sizeof(double), false, Trans != 111);
+  call.window(6, BT.data, 0, exoblas::span2_hi(K, N, BT.strides[0]), sizeof(double), false, Trans != 111);
+  call.window(7, BT_.data, 0, exoblas::span2_hi(K, N, BT_.strides[0]), sizeof(double), false, Trans != 111);
+  call.window(8, C.data, 0, exoblas::span2_hi(N, N, C.strides[0]), sizeof(double), true, true);
+  if (!call.stage()) return (void)call.status();
+  const double alpha_v = call.scalar(alpha);
+  if (!call.ok()) return (void)call.status();
+  call.run([&] { exoblas::ops::syr2k_rm<double>(call.ctx(), (int64_t)Uplo, (int64_t)Trans, (int64_t)N, (int64_t)K, alpha_v, call.dev<const double>(0), (int64_t)A.strides[0], call.dev<const double>(1), (int64_t)A_.strides[0], call.dev<const double>(2), (int64_t)B.strides[0], call.dev<const double>(3), (int64_t)B_.strides[0], call.dev<const double>(4), (int64_t)AT.strides[0], call.dev<const double>(5), (int64_t)AT_.strides[0], call.dev<const double>(6), (int64_t)BT.strides[0], call.dev<const double>(7), (int64_t)BT_.strides[0], call.dev<double>(8), (int64_t)C.strides[0]); });
+  return (void)call.finish();
+}
+extern "C" int exoblas_dsyr2k_rm_stride_any_async(hipStream_t stream, int_fast32_t Uplo, int_fast32_t Trans, int_fast32_t N, int_fast32_t K, const double *alpha, struct exo_win_2f64c A, struct exo_win_2f64c A_, struct exo_win_2f64c B, struct exo_win_2f64c B_, struct exo_win_2f64c AT, struct exo_win_2f64c AT_, struct exo_win_2f64c BT, struct exo_win_2f64c BT_, struct exo_win_2f64 C) {
+  exoblas::Call call("exoblas_dsyr2k_rm_stride_any_async", exoblas::Call::kAsync, stream);
+  if (!(Uplo == 121 || Uplo == 122)) return call.fail_enum("Uplo", Uplo);
+  if (!(Trans == 111 || Trans == 112 || Trans == 113)) return call.fail_enum("Trans", Trans);
+  if (N < 0) return call.fail_size("N", N);
+  if (K < 0) return call.fail_size("K", K);
+  if ((Trans == 111) && A.strides[1] != 1) return call.fail_stride("A", A.strides[1]);
+  if ((Trans == 111) && N > 1 && A.strides[0] < K) return call.fail_ld("A", A.strides[0]);
+  if ((Trans == 111) && A_.strides[1] != 1) return call.fail_stride("A_", A_.strides[1]);
+  if ((Trans == 111) && N > 1 && A_.strides[0] < K) return call.fail_ld("A_", A_.strides[0]);
+  if ((Trans == 111) && B.strides[1] != 1) return call.fail_stride("B", B.strides[1]);
+  if ((Trans == 111) && N > 1 && B.strides[0] < K) return call.fail_ld("B", B.strides[0]);
+  if ((Trans == 111) && B_.strides[1] != 1) return call.fail_stride("B_", B_.strides[1]);
+  if ((Trans == 111) && N > 1 && B_.strides[0] < K) return call.fail_ld("B_", B_.strides[0]);
+  if ((Trans != 111) && AT.strides[1] != 1) return call.fail_stride("AT", AT.strides[1]);
+  if ((Trans != 111) && K > 1 && AT.strides[0] < N) return call.fail_ld("AT", AT.strides[0]);
+  if ((Trans != 111) && AT_.strides[1] != 1) return call.fail_stride("AT_", AT_.strides[1]);
+  if ((Trans != 111) && K > 1 && AT_.strides[0] < N) return call.fail_ld("AT_", AT_.strides[0]);
+  if ((Trans != 111) && BT.strides[1] != 1) return call.fail_stride("BT", BT.strides[1]);
+  if ((Trans != 111) && K > 1 && BT.strides[0] < N) return call.fail_ld("BT", BT.strides[0]);
+  if ((Trans != 111) && BT_.strides[1] != 1) return call.fail_stride("BT_", BT_.strides[1]);
+  if ((Trans != 111) && K > 1 && BT_.strides[0] < N) return call.fail_ld("BT_", BT_.strides[0]);
+  if ((true) && C.strides[1] != 1) return call.fail_stride("C", C.strides[1]);
+  if ((true) && N > 1 && C.strides[0] < N) return call.fail_ld("C", C.strides[0]);
+  call.window(0, A.data, 0, exoblas::span2_hi(N, K, A.strides[0]), sizeof(double), false, Trans == 111);
+  call.window(1, A_.data, 0, exoblas::span2_hi(N, K, A_.strides[0]), sizeof(double), false, Trans == 111);
+  call.window(2, B.data, 0, exoblas::span2_hi(N, K, B.strides[0]), sizeof(double), false, Trans == 111);
+  call.window(3, B_.data, 0, exoblas::span2_hi(N, K, B_.strides[0]), sizeof(double), false, Trans == 111);
+  call.window(4, AT.data, 0, exoblas::span2_hi(K, N, AT.strides[0]), sizeof(double), false, Trans != 111);
+  call.window(5, AT_.data, 0, exoblas::span2_hi(K, N, AT_.strides[0]), sizeof(double), false, Trans != 111);
+  call.window(6, BT.data, 0, exoblas::span2_hi(K, N, BT.strides[0]), sizeof(double), false, Trans != 111);
+  call.window(7, BT_.data, 0, exoblas::span2_hi(K, N, BT_.strides[0]), sizeof(double), false, Trans != 111);
+  call.window(8, C.data, 0, exoblas::span2_hi(N, N, C.strides[0]), sizeof(double), true, true);
+  if (!call.stage()) return call.status();
+  const double alpha_v = call.scalar(alpha);
+  if (!call.ok()) return call.status();
+  call.run([&] { exoblas::ops::syr2k_rm<double>(call.ctx(), (int64_t)Uplo, (int64_t)Trans, (int64_t)N, (int64_t)K, alpha_v, call.dev<const double>(0), (int64_t)A.strides[0], call.dev<const double>(1), (int64_t)A_.strides[0], call.dev<const double>(2), (int64_t)B.strides[0], call.dev<const double>(3), (int64_t)B_.strides[0], call.dev<const double>(4), (int64_t)AT.strides[0], call.dev<const double>(5), (int64_t)AT_.strides[0], call.dev<const double>(6), (int64_t)BT.strides[0], call.dev<const double>(7), (int64_t)BT_.strides[0], call.dev<double>(8), (int64_t)C.strides[0]); });
+  return call.finish();
+}
+
+// src/level3/syr2k.py:14-39
+extern "C" void exo_dsyr2k_rm_stride_1(void *ctxt, int_fast32_t Uplo, int_fast32_t Trans, int_fast32_t N, int_fast32_t K, const double *alpha, struct exo_win_2f64c A, struct exo_win_2f64c A_, struct exo_win_2f64c B, struct exo_win_2f64c B_, struct exo_win_2f64c AT, struct exo_win_2f64c AT_, struct exo_win_2f64c BT, struct exo_win_2f64c BT_, struct exo_win_2f64 C) {
+  exoblas::Call call("exo_dsyr2k_rm_stride_1", exoblas::Call::kSync, ctxt);
+  if (!(Uplo == 121 || Uplo == 122)) return (void)call.fail_enum("Uplo", Uplo);
+  if (!(Trans == 111 || Trans == 112 || Trans == 113)) return (void)call.fail_enum("Trans", Trans);
+  if (N < 0) return (void)call.fail_size("N", N);
+  if (K < 0) return (void)call.fail_size("K", K);
+  if ((Trans == 111) && A.strides[1] != 1) return (void)call.fail_stride("A", A.strides[1]);
+  if ((Trans == 111) && N > 1 && A.strides[0] < K) return (void)call.fail_ld("A", A.strides[0]);
+  if ((Trans == 111) && A_.strides[1] != 1) return (void)call.fail_stride("A_", A_.strides[1]);
+  if ((Trans == 111) && N > 1 && A_.strides[0] < K) return (void)call.fail_ld("A_", A_.strides[0]);
+  if ((Trans == 111) && B.strides[1] != 1) return (void)call.fail_stride("B", B.strides[1]);
+  if ((Trans == 111) && N > 1 && B.strides[0] < K) return (void)call.fail_ld("B", B.strides[0]);
+  if ((Trans == 111) && B_.strides[1] != 1) return (void)call.fail_stride("B_", B_.strides[1]);
+  if ((Trans == 111) && N > 1 && B_.strides[0] < K) return (void)call.fail_ld("B_", B_.strides[0]);
+  if ((Trans != 111) && AT.strides[1] != 1) return (void)call.fail_stride("AT", AT.strides[1]);
+  if ((Trans != 111) && K > 1 && AT.strides[0] < N) return (void)call.fail_ld("AT", AT.strides[0]);
+  if ((Trans != 111) && AT_.strides[1] != 1) return (void)call.fail_stride("AT_", AT_.strides[1]);
+  if ((Trans != 111) && K > 1 && AT_.strides[0] < N) return (void)call.fail_ld("AT_", AT_.strides[0]);
+  if ((Trans != 111) && BT.strides[1] != 1) return (void)call.fail_stride("BT", BT.strides[1]);
+  if ((Trans != 111) && K > 1 && BT.strides[0] < N) return (void)call.fail_ld("BT", BT.strides[0]);
+  if ((Trans != 111) && BT_.strides[1] != 1) return (void)call.fail_stride("BT_", BT_.strides[1]);
+  if ((Trans != 111) && K > 1 && BT_.strides[0] < N) return (void)call.fail_ld("BT_", BT_.strides[0]);
+  if ((true) && C.strides[1] != 1) return (void)call.fail_stride("C", C.strides[1]);
+  if ((true) && N > 1 && C.strides[0] < N) return (void)call.fail_ld("C", C.strides[0]);
+  call.window(0, A.data, 0, exoblas::span2_hi(N, K, A.strides[0]), sizeof(double), false, Trans == 111);
+  call.window(1, A_.data, 0, exoblas::span2_hi(N, K, A_.strides[0]), sizeof(double), false, Trans == 111);
+  call.window(2, B.data, 0, exoblas::span2_hi(N, K, B.strides[0]), sizeof(double), false, Trans == 111);
+  call.window(3, B_.data, 0, exoblas::span2_hi(N, K, B_.strides[0]), sizeof(double), false, Trans == 111);
+  call.window(4, AT.data, 0, exoblas::span2_hi(K, N, AT.strides[0]), sizeof(double), false, Trans != 111);
+  call.window(5, AT_.data, 0, exoblas::span2_hi(K, N, AT_.strides[0]), sizeof(double), false, Trans != 111);
+  call.window(6, BT.data, 0, exoblas::span2_hi(K, N, BT.strides[0]), sizeof(double), false, Trans != 111);
+  call.window(7, BT_.data, 0, exoblas::span2_hi(K, N, BT_.strides[0]), sizeof(double), false, Trans != 111);
+  call.window(8, C.data, 0, exoblas::span2_hi(N, N, C.strides[0]), sizeof(double), true, true);
+  if (!call.stage()) return (void)call.status();
+  const double alpha_v = call.scalar(alpha);
+  if (!call.ok()) return (void)call.status();
+  call.run([&] { exoblas::ops::syr2k_rm<double>(call.ctx(), (int64_t)Uplo, (int64_t)Trans, (int64_t)N, (int64_t)K, alpha_v, call.dev<const double>(0), (int64_t)A.strides[0], call.dev<const double>(1), (int64_t)A_.strides[0], call.dev<const double>(2), (int64_t)B.strides[0], call.dev<const double>(3), (int64_t)B_.strides[0], call.dev<const double>(4), (int64_t)AT.strides[0], call.dev<const double>(5), (int64_t)AT_.strides[0], call.dev<const double>(6), (int64_t)BT.strides[0], call.dev<const double>(7), (int64_t)BT_.strides[0], call.dev<double>(8), (int64_t)C.strides[0]); });
+  return (void)call.finish();
+}
+extern "C" int exoblas_dsyr2k_rm_stride_1_async(hipStream_t stream, int_fast32_t Uplo, int_fast32_t Trans, int_fast32_t N, int_fast32_t K, const double *alpha, struct exo_win_2f64c A, struct exo_win_2f64c A_, struct exo_win_2f64c B, struct exo_win_2f64c B_, struct exo_win_2f64c AT, struct exo_win_2f64c AT_, struct exo_win_2f64c BT, struct exo_win_2f64c BT_, struct exo_win_2f64 C) {
+  exoblas::Call call("exoblas_dsyr2k_rm_stride_1_async", exoblas::Call::kAsync, stream);
+  if (!(Uplo == 121 || Uplo == 122)) return call.fail_enum("Uplo", Uplo);
+  if (!(Trans == 111 || Trans == 112 || Trans == 113)) return call.fail_enum("Trans", Trans);
+  if (N < 0) return call.fail_size("N", N);
+  if (K < 0) return call.fail_size("K", K);
+  if ((Trans == 111) && A.strides[1] != 1) return call.fail_stride("A", A.strides[1]);
+  if ((Trans == 111) && N > 1 && A.strides[0] < K) return call.fail_ld("A", A.strides[0]);
+  if ((Trans == 111) && A_.strides[1] != 1) return call.fail_stride("A_", A_.strides[1]);
+  if ((Trans == 111) && N > 1 && A_.strides[0] < K) return call.fail_ld("A_", A_.strides[0]);
+  if ((Trans == 111) && B.strides[1] != 1) return call.fail_stride("B", B.strides[1]);
+  if ((Trans == 111) && N > 1 && B.strides[0] < K) return call.fail_ld("B", B.strides[0]);
+  if ((Trans == 111) && B_.strides[1] != 1) return call.fail_stride("B_", B_.strides[1]);
+  if ((Trans == 111) && N > 1 && B_.strides[0] < K) return call.fail_ld("B_", B_.strides[0]);
+  if ((Trans != 111) && AT.strides[1] != 1) return call.fail_stride("AT", AT.strides[1]);
+  if ((Trans != 111) && K > 1 && AT.strides[0] < N) return call.fail_ld("AT", AT.strides[0]);
+  if ((Trans != 111) && AT_.strides[1] != 1) return call.fail_stride("AT_", AT_.strides[1]);
+  if ((Trans != 111) && K > 1 && AT_.strides[0] < N) return call.fail_ld("AT_", AT_.strides[0]);
+  if ((Trans != 111) && BT.strides[1] != 1) return call.fail_stride("BT", BT.strides[1]);
+  if ((Trans != 111) && K > 1 && BT.strides[0] < N) return call.fail_ld("BT", BT.strides[0]);
+  if ((Trans != 111) && BT_.strides[1] != 1) return call.fail_stride("BT_", BT_.strides[1]);
+  if ((Trans != 111) && K > 1 && BT_.strides[0] < N) return call.fail_ld("BT_", BT_.strides[0]);
+  if ((true) && C.strides[1] != 1) return call.fail_stride("C", C.strides[1]);
+  if ((true) && N > 1 && C.strides[0] < N) return call.fail_ld("C", C.strides[0]);
+  call.window(0, A.data, 0, exoblas::span2_hi(N, K, A.strides[0]), sizeof(double), false, Trans == 111);
+  call.window(1, A_.data, 0, exoblas::span2_hi(N, K, A_.strides[0]), sizeof(double), false, Trans == 111);
+  call.window(2, B.data, 0, exoblas::span2_hi(N, K, B.strides[0]), sizeof(double), false, Trans == 111);
+  call.window(3, B_.data, 0, exoblas::span2_hi(N, K, B_.strides[0]), sizeof(double), false, Trans == 111);
+  call.window(4, AT.data, 0, exoblas::span2_hi(K, N, AT.strides[0]), sizeof(double), false, Trans != 111);
+  call.window(5, AT_.data, 0, exoblas::span2_hi(K, N, AT_.strides[0]), sizeof(double), false, Trans != 111);
+  call.window(6, BT.data, 0, exoblas::span2_hi(K, N, BT.strides[0]), sizeof(double), false, Trans != 111);
+  call.window(7, BT_.data, 0, exoblas::span2_hi(K, N, BT_.strides[0]), sizeof(double), false, Trans != 111);
+  call.window(8, C.data, 0, exoblas::span2_hi(N, N, C.strides[0]), sizeof(double), true, true);
+  if (!call.stage()) return call.status();
+  const double alpha_v = call.scalar(alpha);
+  if (!call.ok()) return call.status();
+  call.run([&] { exoblas::ops::syr2k_rm<double>(call.ctx(), (int64_t)Uplo, (int64_t)Trans, (int64_t)N, (int64_t)K, alpha_v, call.dev<const double>(0), (int64_t)A.strides[0], call.dev<const double>(1), (int64_t)A_.strides[0], call.dev<const double>(2), (int64_t)B.strides[0], call.dev<const double>(3), (int64_t)B_.strides[0], call.dev<const double>(4), (int64_t)AT.strides[0], call.dev<const double>(5), (int64_t)AT_.strides[0], call.dev<const double>(6), (int64_t)BT.strides[0], call.dev<const double>(7), (int64_t)BT_.strides[0], call.dev<double>(8), (int64_t)C.strides[0]); });
+  return call.finish();
+}
+
+// src/level3/trmm.py:12-23
+extern "C" void exo_strmm_row_major_Left_Upper_NonTrans(void *ctxt, int_fast32_t m, int_fast32_t n, const float *alpha, struct exo_win_2f32c A, struct exo_win_2f32 B, int_fast32_t Diag) {
+  exoblas::Call call("exo_strmm_row_major_Left_Upper_NonTrans", exoblas::Call::kSync, ctxt);
+  if (m < 0) return (void)call.fail_size("m", m);
+  if (n < 0) return (void)call.fail_size("n", n);
+  if (Diag < 0) return (void)call.fail_size("Diag", Diag);
+  if ((true) && A.strides[1] != 1) return (void)call.fail_stride("A", A.strides[1]);
+  if ((true) && m > 1 && A.strides[0] < m) return (void)call.fail_ld("A", A.strides[0]);
+  if ((true) && B.strides[1] != 1) return (void)call.fail_stride("B", B.strides[1]);
+  if ((true) && m > 1 && B.strides[0] < n) return (void)call.fail_ld("B", B.strides[0]);
+  call.window(0, A.data, 0, exoblas::span2_hi(m, m, A.strides[0]), sizeof(float), false, true);
+  call.window(1, B.data, 0, exoblas::span2_hi(m, n, B.strides[0]), sizeof(float), true, true);
+  if (!call.stage()) return (void)call.status();
+  const float alpha_v = call.scalar(alpha);
+  if (!call.ok()) return (void)call.status();
+  call.run([&] { exoblas::ops::trmm_row_major_Left_Upper_NonTrans<float>(call.ctx(), (int64_t)m, (int64_t)n, alpha_v, call.dev<const float>(0), (int64_t)A.strides[0], call.dev<float>(1), (int64_t)B.strides[0], (int64_t)Diag); });
+  return (void)call.finish();
+}
+extern "C" int exoblas_strmm_row_major_Left_Upper_NonTrans_async(hipStream_t stream, int_fast32_t m, int_fast32_t n, const float *alpha, struct exo_win_2f32c A, struct exo_win_2f32 B, int_fast32_t Diag) {
+  exoblas::Call call("exoblas_strmm_row_major_Left_Upper_NonTrans_async", exoblas::Call::kAsync, stream);
+  if (m < 0) return call.fail_size("m", m);
+  if (n < 0) return call.fail_size("n", n);
+  if (Diag < 0) return call.fail_size("Diag", Diag);
+  if ((true) && A.strides[1] != 1) return call.fail_stride("A", A.strides[1]);
+  if ((true) && m > 1 && A.strides[0] < m) return call.fail_ld("A", A.strides[0]);
+  if ((true) && B.strides[1] != 1) return call.fail_stride("B", B.strides[1]);
+  if ((true) && m > 1 && B.strides[0] < n) return call.fail_ld("B", B.strides[0]);
+  call.window(0, A.data, 0, exoblas::span2_hi(m, m, A.strides[0]), sizeof(float), false, true);
+  call.window(1, B.data, 0, exoblas::span2_hi(m, n, B.strides[0]), sizeof(float), true, true);
+  if (!call.stage()) return call.status();
+  const float alpha_v = call.scalar(alpha);
+  if (!call.ok()) return call.status();
+  call.run([&] { exoblas::ops::trmm_row_major_Left_Upper_NonTrans<float>(call.ctx(), (int64_t)m, (int64_t)n, alpha_v, call.dev<const float>(0), (int64_t)A.strides[0], call.dev<float>(1), (int64_t)B.strides[0], (int64_t)Diag); });
+  return call.finish();
+}
+
+// src/level3/trmm.py:26-46
+extern "C" void exo_strmm_row_major_Left_Lower_NonTrans(void *ctxt, int_fast32_t m, int_fast32_t n, const float *alpha, struct exo_win_2f32c A, struct exo_win_2f32 B, int_fast32_t Diag) {
+  exoblas::Call call("exo_strmm_row_major_Left_Lower_NonTrans", exoblas::Call::kSync, ctxt);
+  if (m < 0) return (void)call.fail_size("m", m);
+  if (n < 0) return (void)call.fail_size("n", n);
+  if (Diag < 0) return (void)call.fail_size("Diag", Diag);
+  if ((true) && A.strides[1] != 1) return (void)call.fail_stride("A", A.strides[1]);
+  if ((true) && m > 1 && A.strides[0] < m) return (void)call.fail_ld("A", A.strides[0]);
+  if ((true) && B.strides[1] != 1) return (void)call.fail_stride("B", B.strides[1]);
+  if ((true) && m > 1 && B.strides[0] < n) return (void)call.fail_ld("B", B.strides[0]);
+  call.window(0, A.data, 0, exoblas::span2_hi(m, m, A.strides[0]), sizeof(float), false, true);
+  call.window(1, B.data, 0, exoblas::span2_hi(m, n, B.strides[0]), sizeof(float), true, true);
+  if (!call.stage()) return (void)call.status();
+  const float alpha_v = call.scalar(alpha);
+  if (!call.ok()) return (void)call.status();
+  call.run([&] { exoblas::ops::trmm_row_major_Left_Lower_NonTrans<float>(call.ctx(), (int64_t)m, (int64_t)n, alpha_v, call.dev<const float>(0), (int64_t)A.strides[0], call.dev<float>(1), (int64_t)B.strides[0], (int64_t)Diag); });
+  return (void)call.finish();
+}
+extern "C" int exoblas_strmm_row_major_Left_Lower_NonTrans_async(hipStream_t stream, int_fast32_t m, int_fast32_t n, const float *alpha, struct exo_win_2f32c A, struct exo_win_2f32 B, int_fast32_t Diag) {
+  exoblas::Call call("exoblas_strmm_row_major_Left_Lower_NonTrans_async", exoblas::Call::kAsync, stream);
+  if (m < 0) return call.fail_size("m", m);
+  if (n < 0) return call.fail_size("n", n);
+  if (Diag < 0) return call.fail_size("Diag", Diag);
+  if ((true) && A.strides[1] != 1) return call.fail_stride("A", A.strides[1]);
+  if ((true) && m > 1 && A.strides[0] < m) return call.fail_ld("A", A.strides[0]);
+  if ((true) && B.strides[1] != 1) return call.fail_stride("B", B.strides[1]);
+  if ((true) && m > 1 && B.strides[0] < n) return call.fail_ld("B", B.strides[0]);
+  call.window(0, A.data, 0, exoblas::span2_hi(m, m, A.strides[0]), sizeof(float), false, true);
+  call.window(1, B.data, 0, exoblas::span2_hi(m, n, B.strides[0]), sizeof(float), true, true);
+  if (!call.stage()) return call.status();
+  const float alpha_v = call.scalar(alpha);
+  if (!call.ok()) return call.status();
+  call.run([&] { exoblas::ops::trmm_row_major_Left_Lower_NonTrans<float>(call.ctx(), (int64_t)m, (int64_t)n, alpha_v, call.dev<const float>(0), (int64_t)A.strides[0], call.dev<float>(1), (int64_t)B.strides[0], (int64_t)Diag); });
+  return call.finish();
+}
+
+// src/level3/trmm.py:49-69
+extern "C" void exo_strmm_row_major_Left_Upper_Trans(void *ctxt, int_fast32_t m, int_fast32_t n, const float *alpha, struct exo_win_2f32c A, struct exo_win_2f32 B, int_fast32_t Diag) {
+  exoblas::Call call("exo_strmm_row_major_Left_Upper_Trans", exoblas::Call::kSync, ctxt);
+  if (m < 0) return (void)call.fail_size("m", m);
+  if (n < 0) return (void)call.fail_size("n", n);
+  if (Diag < 0) return (void)call.fail_size("Diag", Diag);
+  if ((true) && A.strides[1] != 1) return (void)call.fail_stride("A", A.strides[1]);
+  if ((true) && m > 1 && A.strides[0] < m) return (void)call.fail_ld("A", A.strides[0]);
+  if ((true) && B.strides[1] != 1) return (void)call.fail_stride("B", B.strides[1]);
+  if ((true) && m > 1 && B.strides[0] < n) return (void)call.fail_ld("B", B.strides[0]);
+  call.window(0, A.data, 0, exoblas::span2_hi(m, m, A.strides[0]), sizeof(float), false, true);
+  call.window(1, B.data, 0, exoblas::span2_hi(m, n, B.strides[0]), sizeof(float), true, true);
+  if (!call.stage()) return (void)call.status();
+  const float alpha_v = call.scalar(alpha);
+  if (!call.ok()) return (void)call.status();
+  call.run([&] { exoblas::ops::trmm_row_major_Left_Upper_Trans<float>(call.ctx(), (int64_t)m, (int64_t)n, alpha_v, call.dev<const float>(0), (int64_t)A.strides[0], call.dev<float>(1), (int64_t)B.strides[0], (int64_t)Diag); });
+  return (void)call.finish();
+}
+extern "C" int exoblas_strmm_row_major_Left_Upper_Trans_async(hipStream_t stream, int_fast32_t m, int_fast32_t n, const float *alpha, struct exo_win_2f32c A, struct exo_win_2f32 B, int_fast32_t Diag) {
+  exoblas::Call call("exoblas_strmm_row_major_Left_Upper_Trans_async", exoblas::Call::kAsync, stream);
+  if (m < 0) return call.fail_size("m", m);
+  if (n < 0) return call.fail_size("n", n);
+  if (Diag < 0) return call.fail_size("Diag", Diag);
+  if ((true) && A.strides[1] != 1) return call.fail_stride("A", A.strides[1]);
+  if ((true) && m > 1 && A.strides[0] < m) return call.fail_ld("A", A.strides[0]);
+  if ((true) && B.strides[1] != 1) return call.fail_stride("B", B.strides[1]);
+  if ((true) && m > 1 && B.strides[0] < n) return call.fail_ld("B", B.strides[0]);
+  call.window(0, A.data, 0, exoblas::span2_hi(m, m, A.strides[0]), sizeof(float), false, true);
+  call.window(1, B.data, 0, exoblas::span2_hi(m, n, B.strides[0]), sizeof(float), true, true);
+  if (!call.stage()) return call.status();
+  const float alpha_v = call.scalar(alpha);
+  if (!call.ok()) return call.status();
+  call.run([&] { exoblas::ops::trmm_row_major_Left_Upper_Trans<float>(call.ctx(), (int64_t)m, (int64_t)n, alpha_v, call.dev<const float>(0), (int64_t)A.strides[0], call.dev<float>(1), (int64_t)B.strides[0], (int64_t)Diag); });
+  return call.finish();
+}
+
+// src/level3/trmm.py:72-83
+extern "C" void exo_strmm_row_major_Left_Lower_Trans(void *ctxt, int_fast32_t m, int_fast32_t n, const float *alpha, struct exo_win_2f32c A, struct exo_win_2f32 B, int_fast32_t Diag) {
+  exoblas::Call call("exo_strmm_row_major_Left_Lower_Trans", exoblas::Call::kSync, ctxt);
+  if (m < 0) return (void)call.fail_size("m", m);
+  if (n < 0) return (void)call.fail_size("n", n);
+  if (Diag < 0) return (void)call.fail_size("Diag", Diag);
+  if ((true) && A.strides[1] != 1) return (void)call.fail_stride("A", A.strides[1]);
+  if ((true) && m > 1 && A.strides[0] < m) return (void)call.fail_ld("A", A.strides[0]);
+  if ((true) && B.strides[1] != 1) return (void)call.fail_stride("B", B.strides[1]);
+  if ((true) && m > 1 && B.strides[0] < n) return (void)call.fail_ld("B", B.strides[0]);
+  call.window(0, A.data, 0, exoblas::span2_hi(m, m, A.strides[0]), sizeof(float), false, true);
+  call.window(1, B.data, 0, exoblas::span2_hi(m, n, B.strides[0]), sizeof(float), true, true);
+  if (!call.stage()) return (void)call.status();
+  const float alpha_v = call.scalar(alpha);
+  if (!call.ok()) return (void)call.status();
+  call.run([&] { exoblas::ops::trmm_row_major_Left_Lower_Trans<float>(call.ctx(), (int64_t)m, (int64_t)n, alpha_v, call.dev<const float>(0), (int64_t)A.strides[0], call.dev<float>(1), (int64_t)B.strides[0], (int64_t)Diag); });
+  return (void)call.finish();
+}
+extern "C" int exoblas_strmm_row_major_Left_Lower_Trans_async(hipStream_t stream, int_fast32_t m, int_fast32_t n, const float *alpha, struct exo_win_2f32c A, struct exo_win_2f32 B, int_fast32_t Diag) {
+  exoblas::Call call("exoblas_strmm_row_major_Left_Lower_Trans_async", exoblas::Call::kAsync, stream);
+  if (m < 0) return call.fail_size("m", m);
+  if (n < 0) return call.fail_size("n", n);
+  if (Diag < 0) return call.fail_size("Diag", Diag);
+  if ((true) && A.strides[1] != 1) return call.fail_stride("A", A.strides[1]);
+  if ((true) && m > 1 && A.strides[0] < m) return call.fail_ld("A", A.strides[0]);
+  if ((true) && B.strides[1] != 1) return call.fail_stride("B", B.strides[1]);
+  if ((true) && m > 1 && B.strides[0] < n) return call.fail_ld("B", B.strides[0]);
+  call.window(0, A.data, 0, exoblas::span2_hi(m, m, A.strides[0]), sizeof(float), false, true);
+  call.window(1, B.data, 0, exoblas::span2_hi(m, n, B.strides[0]), sizeof(float), true, true);
+  if (!call.stage()) return call.status();
+  const float alpha_v = call.scalar(alpha);
+  if (!call.ok()) return call.status();
+  call.run([&] { exoblas::ops::trmm_row_major_Left_Lower_Trans<float>(call.ctx(), (int64_t)m, (int64_t)n, alpha_v, call.dev<const float>(0), (int64_t)A.strides[0], call.dev<float>(1), (int64_t)B.strides[0], (int64_t)Diag); });
   return call.finish();
 }
 

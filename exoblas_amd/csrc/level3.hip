@@ -1,5 +1,5 @@
-// Level-3 kernels for gfx950: gemm / symm / syrk on the MFMA tile engine
-// (gemm_core.hpp) and the mscal / trmscal beta passes.
+// Level-3 kernels for gfx950: gemm / symm / syrk / syr2k / trmm on the MFMA
+// tile engine (gemm_core.hpp) and the mscal / trmscal beta passes.
 //
 // Every product is  C[i, j] += alpha * sum_k X_A(i, k) X_B(j, k)  with the
 // operand forms chosen on the host from TransA/TransB, Side/Uplo or
@@ -35,7 +35,9 @@ __global__ __launch_bounds__(C::NT) void gemm_kernel(int64_t M, int64_t N, int64
     for (int b = 0; b < C::SN; ++b)
 #pragma unroll
       for (int q = 0; q < 4; ++q) acc[a][b][q] = T(0);
-  tile_mma<C>(A, B, M, N, K, m0, n0, 0, K, acc, reinterpret_cast<T*>(smem));
+  int64_t kb = 0, ke = K;
+  tri_k_window(A, m0, C::BM, kb, ke);
+  tile_mma<C>(A, B, M, N, K, m0, n0, kb, ke, acc, reinterpret_cast<T*>(smem));
   tile_store<C, TRI>(Cp, ldc, cvec, M, N, m0, n0, alpha, acc);
 }
 
@@ -63,7 +65,8 @@ __global__ __launch_bounds__(C::NT) void gemm_splitk_kernel(int64_t M, int64_t N
     for (int b = 0; b < C::SN; ++b)
 #pragma unroll
       for (int q = 0; q < 4; ++q) acc[a][b][q] = T(0);
-  const int64_t kb = s * ksplit, ke = kb + ksplit < K ? kb + ksplit : K;
+  int64_t kb = s * ksplit, ke = kb + ksplit < K ? kb + ksplit : K;
+  tri_k_window(A, m0, C::BM, kb, ke);  // an empty window leaves the block's sums zero
   tile_mma<C>(A, B, M, N, K, m0, n0, kb, ke, acc, reinterpret_cast<T*>(smem));
   tile_store<C, kFull, true>(P + (s * ntiles + l) * C::BM * C::BN, C::BN, 1, C::BM, C::BN, 0, 0, T(1), acc, T(0));
 }
@@ -281,6 +284,20 @@ constexpr int64_t kNoTrans = 111;
 constexpr int64_t kUpperV = 121;
 constexpr int64_t kLeftV = 141;
 
+// Bc[i, 0:n) = B[i, 0:n) and B[i, 0:n) = 0 for row i = blockIdx.y: trmm
+// reads its right-hand operand from the copy and accumulates into B.
+template <class T>
+__global__ __launch_bounds__(kNT) void copy_clear_kernel(int64_t i0, int64_t n, T* B, int64_t ldb, T* Bc,
+                                                         int64_t ldc) {
+  const int64_t i = i0 + blockIdx.y;
+  T* row = B + i * ldb;
+  T* out = Bc + i * ldc;
+  for (int64_t j = (int64_t)blockIdx.x * kNT + threadIdx.x; j < n; j += (int64_t)gridDim.x * kNT) {
+    out[j] = row[j];
+    row[j] = T(0);
+  }
+}
+
 // A[i, j] *= alpha over the full M x N matrix (TRI 0) or the lower (1) /
 // upper (2) triangle of an N x N matrix.  blockIdx.y picks the row (i0 +
 // blockIdx.y); VEC uses 16-byte accesses for the aligned body of the row's
@@ -375,6 +392,77 @@ void syrk_rm(Ctx& c, int64_t Uplo, int64_t Trans, int64_t N, int64_t K, T alpha,
     product<T>(c, tri, N, N, K, alpha, operand(AT, ldat, kKR), operand(AT_, ldat_, kKR), Cp, ldc);
 }
 
+// src/level3/syr2k.py:14-39.  NoTrans: C += alpha (A B^T + B A^T); Trans:
+// C += alpha (A^T B + B^T A), on the Uplo triangle of C: two triangular
+// products (A with B, then B with A).
+template <class T>
+void syr2k_rm(Ctx& c, int64_t Uplo, int64_t Trans, int64_t N, int64_t K, T alpha, const T* A, int64_t lda,
+              const T* A_, int64_t lda_, const T* B, int64_t ldb, const T* B_, int64_t ldb_, const T* AT,
+              int64_t ldat, const T* AT_, int64_t ldat_, const T* BT, int64_t ldbt, const T* BT_, int64_t ldbt_,
+              T* Cp, int64_t ldc) {
+  using namespace gemm;
+  const int tri = Uplo == kUpperV ? kUpper : kLower;
+  if (Trans == kNoTrans) {
+    product<T>(c, tri, N, N, K, alpha, operand(A, lda, kRK), operand(B, ldb, kRK), Cp, ldc);
+    product<T>(c, tri, N, N, K, alpha, operand(B_, ldb_, kRK), operand(A_, lda_, kRK), Cp, ldc);
+  } else {
+    product<T>(c, tri, N, N, K, alpha, operand(AT, ldat, kKR), operand(BT, ldbt, kKR), Cp, ldc);
+    product<T>(c, tri, N, N, K, alpha, operand(BT_, ldbt_, kKR), operand(AT_, ldat_, kKR), Cp, ldc);
+  }
+}
+
+// src/level3/trmm.py:12-83.  B <- alpha op(A) B, A triangular m x m (unit
+// diagonal when Diag != 0), B m x n in place: B is copied to scratch (row
+// length padded to 16 bytes) and cleared in one pass, then the product with
+// the triangular operand form accumulates into it.  K tiles outside the
+// triangle are skipped.
+template <class T>
+void trmm_left(Ctx& c, bool upper, bool trans, int64_t m, int64_t n, T alpha, const T* A, int64_t lda, T* B,
+               int64_t ldb, int64_t Diag) {
+  using namespace gemm;
+  if (m <= 0 || n <= 0) return;
+  constexpr int VN = Vec16<T>::n;
+  const int64_t ldp = ceil_div(n, VN) * VN;
+  T* Bc = static_cast<T*>(c.scratch(sizeof(T) * (size_t)(m * ldp), 1));
+  const unsigned gx = (unsigned)std::max<int64_t>(1, std::min<int64_t>(ceil_div(n, kNT), 64));
+  for (int64_t r0 = 0; r0 < m; r0 += 65535) {
+    const int64_t rows = std::min<int64_t>(65535, m - r0);
+    hipLaunchKernelGGL((copy_clear_kernel<T>), dim3(gx, (unsigned)rows), dim3(kNT), 0, c.stream, r0, n, B, ldb, Bc,
+                       ldp);
+    EXO_LAUNCH_CHECK();
+  }
+  // op(A) is lower for (Lower, NoTrans) and (Upper, Trans); X_A(i, k) reads
+  // A[i, k] without a transpose and A[k, i] with one.
+  const bool lower_op = upper == trans;
+  const Operand<T> a = operand(A, lda, tri_form(trans ? kKR : kRK, !lower_op, Diag != 0));
+  product<T>(c, kFull, m, n, m, alpha, a, operand<T>(Bc, ldp, kKR), B, ldb);
+}
+
+// src/level3/trmm.py:12-23
+template <class T>
+void trmm_row_major_Left_Upper_NonTrans(Ctx& c, int64_t m, int64_t n, T alpha, const T* A, int64_t lda, T* B,
+                                        int64_t ldb, int64_t Diag) {
+  trmm_left<T>(c, true, false, m, n, alpha, A, lda, B, ldb, Diag);
+}
+// src/level3/trmm.py:26-46
+template <class T>
+void trmm_row_major_Left_Lower_NonTrans(Ctx& c, int64_t m, int64_t n, T alpha, const T* A, int64_t lda, T* B,
+                                        int64_t ldb, int64_t Diag) {
+  trmm_left<T>(c, false, false, m, n, alpha, A, lda, B, ldb, Diag);
+}
+// src/level3/trmm.py:49-69
+template <class T>
+void trmm_row_major_Left_Upper_Trans(Ctx& c, int64_t m, int64_t n, T alpha, const T* A, int64_t lda, T* B,
+                                     int64_t ldb, int64_t Diag) {
+  trmm_left<T>(c, true, true, m, n, alpha, A, lda, B, ldb, Diag);
+}
+// src/level3/trmm.py:72-83
+template <class T>
+void trmm_row_major_Left_Lower_Trans(Ctx& c, int64_t m, int64_t n, T alpha, const T* A, int64_t lda, T* B,
+                                     int64_t ldb, int64_t Diag) {
+  trmm_left<T>(c, false, true, m, n, alpha, A, lda, B, ldb, Diag);
+}
+
 // src/level3/mscal.py:12-17
 template <class T>
 void mscal_rm(Ctx& c, int64_t M, int64_t N, T alpha, T* A, int64_t lda) {
@@ -398,9 +486,20 @@ void trmscal_rm(Ctx& c, int64_t Uplo, int64_t N, T alpha, T* A, int64_t lda) {
   template void symm_rm<T>(Ctx&, int64_t, int64_t, int64_t, int64_t, T, const T*, int64_t, const T*, int64_t, \
                            const T*, int64_t, T*, int64_t);                                                   \
   template void syrk_rm<T>(Ctx&, int64_t, int64_t, int64_t, int64_t, T, const T*, int64_t, const T*, int64_t, \
-                           const T*, int64_t, const T*, int64_t, T*, int64_t);
+                           const T*, int64_t, const T*, int64_t, T*, int64_t);                                \
+  template void syr2k_rm<T>(Ctx&, int64_t, int64_t, int64_t, int64_t, T, const T*, int64_t, const T*, int64_t,\
+                            const T*, int64_t, const T*, int64_t, const T*, int64_t, const T*, int64_t,       \
+                            const T*, int64_t, const T*, int64_t, T*, int64_t);
 EXO_L3_INST(float)
 EXO_L3_INST(double)
+template void trmm_row_major_Left_Upper_NonTrans<float>(Ctx&, int64_t, int64_t, float, const float*, int64_t,
+                                                        float*, int64_t, int64_t);
+template void trmm_row_major_Left_Lower_NonTrans<float>(Ctx&, int64_t, int64_t, float, const float*, int64_t,
+                                                        float*, int64_t, int64_t);
+template void trmm_row_major_Left_Upper_Trans<float>(Ctx&, int64_t, int64_t, float, const float*, int64_t,
+                                                     float*, int64_t, int64_t);
+template void trmm_row_major_Left_Lower_Trans<float>(Ctx&, int64_t, int64_t, float, const float*, int64_t,
+                                                     float*, int64_t, int64_t);
 
 }  // namespace ops
 

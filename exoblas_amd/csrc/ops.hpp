@@ -66,6 +66,24 @@ void syrk_rm(Ctx&, int64_t Uplo, int64_t Trans, int64_t N, int64_t K, T alpha, c
              const T* A_, int64_t lda_, const T* AT, int64_t ldat, const T* AT_, int64_t ldat_, T* C,
              int64_t ldc);
 
+template <class T>
+void syr2k_rm(Ctx&, int64_t Uplo, int64_t Trans, int64_t N, int64_t K, T alpha, const T* A, int64_t lda,
+              const T* A_, int64_t lda_, const T* B, int64_t ldb, const T* B_, int64_t ldb_, const T* AT,
+              int64_t ldat, const T* AT_, int64_t ldat_, const T* BT, int64_t ldbt, const T* BT_, int64_t ldbt_,
+              T* C, int64_t ldc);
+// B <- alpha op(A) B, A triangular (unit diagonal when Diag != 0), in place.
+template <class T>
+void trmm_row_major_Left_Upper_NonTrans(Ctx&, int64_t m, int64_t n, T alpha, const T* A, int64_t lda, T* B,
+                                        int64_t ldb, int64_t Diag);
+template <class T>
+void trmm_row_major_Left_Lower_NonTrans(Ctx&, int64_t m, int64_t n, T alpha, const T* A, int64_t lda, T* B,
+                                        int64_t ldb, int64_t Diag);
+template <class T>
+void trmm_row_major_Left_Upper_Trans(Ctx&, int64_t m, int64_t n, T alpha, const T* A, int64_t lda, T* B,
+                                     int64_t ldb, int64_t Diag);
+template <class T>
+void trmm_row_major_Left_Lower_Trans(Ctx&, int64_t m, int64_t n, T alpha, const T* A, int64_t lda, T* B,
+                                     int64_t ldb, int64_t Diag);
 // ---------------------------------------------------------------- split precision
 // C += alpha op(A) op(B) (or one triangle of alpha A A^T / A^T A) with each
 // FP32 operand split into `terms`-product BF16 pieces on the BF16 MFMA.

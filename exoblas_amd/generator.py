@@ -88,6 +88,9 @@ class Proc:
     callers: str  # reference wrapper call sites
     precisions: Sequence[str] = ("s", "d")
     fixed_prefix: Optional[str] = None  # dsdot/sdsdot carry no s/d prefix
+    # name suffixes the reference exports; "" = the bare name (trmm's procs are
+    # exported without the stride_1 / stride_any pair)
+    variants: Sequence[str] = ("stride_any", "stride_1")
 
 
 W = "int_fast32_t"
@@ -192,6 +195,26 @@ PROCS: List[Proc] = [
           win("AT_", "K", "N", used_when="Trans != 111"),
           win("C", "N", "N", const=False)],
          "src/level3/syrk.py:17-32", "test/level3/syrk/exo_syrk_wrapper.h:21-27"),
+    Proc("syr2k_rm", "syr2k",
+         [enum("Uplo", UPLO_VALUES), enum("Trans", TRANS_VALUES), size("N"), size("K"), scalar("alpha"),
+          win("A", "N", "K", used_when="Trans == 111"),
+          win("A_", "N", "K", used_when="Trans == 111"),
+          win("B", "N", "K", used_when="Trans == 111"),
+          win("B_", "N", "K", used_when="Trans == 111"),
+          win("AT", "K", "N", used_when="Trans != 111"),
+          win("AT_", "K", "N", used_when="Trans != 111"),
+          win("BT", "K", "N", used_when="Trans != 111"),
+          win("BT_", "K", "N", used_when="Trans != 111"),
+          win("C", "N", "N", const=False)],
+         "src/level3/syr2k.py:14-39", "test/level3/syr2k/exo_syr2k_wrapper.h:21-31"),
+] + [
+    Proc(f"trmm_row_major_Left_{uplo}_{tr}", "trmm",
+         [size("m"), size("n"), scalar("alpha"), win("A", "m", "m"), win("B", "m", "n", const=False),
+          size("Diag")],
+         f"src/level3/trmm.py:{lines}", f"test/level3/strmm/exo_strmm.h:{callers}",
+         precisions=("s",), variants=("",))
+    for uplo, tr, lines, callers in (("Upper", "NonTrans", "12-23", "18-22"), ("Lower", "NonTrans", "26-46", "32-36"),
+                                     ("Upper", "Trans", "49-69", "24-28"), ("Lower", "Trans", "72-83", "38-42"))
 ]
 
 # Native entry points with no counterpart in the reference: the split-
@@ -229,7 +252,7 @@ EXT_PROCS: List[Proc] = [
 
 LIB_ORDER = ["asum", "axpy", "dot", "dsdot", "copy", "swap", "rot", "rotm", "scal",
              "gemv", "ger", "symv", "syr", "syr2", "trmv", "trsv",
-             "gemm", "mscal", "symm", "syrk"]
+             "gemm", "mscal", "symm", "syrk", "syr2k", "trmm"]
 
 
 # ---------------------------------------------------------------- helpers
@@ -262,12 +285,14 @@ def symbol(proc: Proc, p: str, variant: Optional[str]) -> str:
     prefix = p if proc.fixed_prefix is None else proc.fixed_prefix
     if variant is None:
         return f"exoblas_{prefix}{proc.name}"
+    if variant == "":
+        return f"exo_{prefix}{proc.name}"
     return f"exo_{prefix}{proc.name}_{variant}"
 
 
 def async_symbol(proc: Proc, p: str, variant: Optional[str]) -> str:
     prefix = p if proc.fixed_prefix is None else proc.fixed_prefix
-    if variant is None:
+    if not variant:
         return f"exoblas_{prefix}{proc.name}_async"
     return f"exoblas_{prefix}{proc.name}_{variant}_async"
 
@@ -281,7 +306,7 @@ def op_name(proc: Proc, p: str) -> str:
 def variants():
     for proc in PROCS:
         for p in proc.precisions:
-            for v in ("stride_any", "stride_1"):
+            for v in proc.variants:
                 yield proc, p, v
 
 
@@ -316,7 +341,7 @@ def emit_lib_header(lib: str) -> str:
     for proc in procs:
         out.append(f"/* {proc.name}: spec {proc.ref}; called from {proc.callers} */\n")
         for p in proc.precisions:
-            for v in ("stride_any", "stride_1"):
+            for v in proc.variants:
                 params = ", ".join(["void *ctxt"] + [c_param(p, a) for a in proc.args])
                 out.append(f"void {symbol(proc, p, v)}({params});\n")
         out.append("\n")

@@ -120,6 +120,14 @@ int exoblas_ssyrk_rm_stride_any_async(hipStream_t stream, int_fast32_t Uplo, int
 int exoblas_ssyrk_rm_stride_1_async(hipStream_t stream, int_fast32_t Uplo, int_fast32_t Trans, int_fast32_t N, int_fast32_t K, const float *alpha, struct exo_win_2f32c A, struct exo_win_2f32c A_, struct exo_win_2f32c AT, struct exo_win_2f32c AT_, struct exo_win_2f32 C);
 int exoblas_dsyrk_rm_stride_any_async(hipStream_t stream, int_fast32_t Uplo, int_fast32_t Trans, int_fast32_t N, int_fast32_t K, const double *alpha, struct exo_win_2f64c A, struct exo_win_2f64c A_, struct exo_win_2f64c AT, struct exo_win_2f64c AT_, struct exo_win_2f64 C);
 int exoblas_dsyrk_rm_stride_1_async(hipStream_t stream, int_fast32_t Uplo, int_fast32_t Trans, int_fast32_t N, int_fast32_t K, const double *alpha, struct exo_win_2f64c A, struct exo_win_2f64c A_, struct exo_win_2f64c AT, struct exo_win_2f64c AT_, struct exo_win_2f64 C);
+int exoblas_ssyr2k_rm_stride_any_async(hipStream_t stream, int_fast32_t Uplo, int_fast32_t Trans, int_fast32_t N, int_fast32_t K, const float *alpha, struct exo_win_2f32c A, struct exo_win_2f32c A_, struct exo_win_2f32c B, struct exo_win_2f32c B_, struct exo_win_2f32c AT, struct exo_win_2f32c AT_, struct exo_win_2f32c BT, struct exo_win_2f32c BT_, struct exo_win_2f32 C);
+int exoblas_ssyr2k_rm_stride_1_async(hipStream_t stream, int_fast32_t Uplo, int_fast32_t Trans, int_fast32_t N, int_fast32_t K, const float *alpha, struct exo_win_2f32c A, struct exo_win_2f32c A_, struct exo_win_2f32c B, struct exo_win_2f32c B_, struct exo_win_2f32c AT, struct exo_win_2f32c AT_, struct exo_win_2f32c BT, struct exo_win_2f32c BT_, struct exo_win_2f32 C);
+int exoblas_dsyr2k_rm_stride_any_async(hipStream_t stream, int_fast32_t Uplo, int_fast32_t Trans, int_fast32_t N, int_fast32_t K, const double *alpha, struct exo_win_2f64c A, struct exo_win_2f64c A_, struct exo_win_2f64c B, struct exo_win_2f64c B_, struct exo_win_2f64c AT, struct exo_win_2f64c AT_, struct exo_win_2f64c BT, struct exo_win_2f64c BT_, struct exo_win_2f64 C);
+int exoblas_dsyr2k_rm_stride_1_async(hipStream_t stream, int_fast32_t Uplo, int_fast32_t Trans, int_fast32_t N, int_fast32_t K, const double *alpha, struct exo_win_2f64c A, struct exo_win_2f64c A_, struct exo_win_2f64c B, struct exo_win_2f64c B_, struct exo_win_2f64c AT, struct exo_win_2f64c AT_, struct exo_win_2f64c BT, struct exo_win_2f64c BT_, struct exo_win_2f64 C);
+int exoblas_strmm_row_major_Left_Upper_NonTrans_async(hipStream_t stream, int_fast32_t m, int_fast32_t n, const float *alpha, struct exo_win_2f32c A, struct exo_win_2f32 B, int_fast32_t Diag);
+int exoblas_strmm_row_major_Left_Lower_NonTrans_async(hipStream_t stream, int_fast32_t m, int_fast32_t n, const float *alpha, struct exo_win_2f32c A, struct exo_win_2f32 B, int_fast32_t Diag);
+int exoblas_strmm_row_major_Left_Upper_Trans_async(hipStream_t stream, int_fast32_t m, int_fast32_t n, const float *alpha, struct exo_win_2f32c A, struct exo_win_2f32 B, int_fast32_t Diag);
+int exoblas_strmm_row_major_Left_Lower_Trans_async(hipStream_t stream, int_fast32_t m, int_fast32_t n, const float *alpha, struct exo_win_2f32c A, struct exo_win_2f32 B, int_fast32_t Diag);
 
 #ifdef __cplusplus
 }

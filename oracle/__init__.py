@@ -94,6 +94,8 @@ def load(kind: str = "strict") -> ctypes.CDLL:
         d("trmscal", None, _I, _I, T, _P, _I)
         d("symm", None, _I, _I, _I, _I, T, _P, _I, _P, _I, _P, _I)
         d("syrk", None, _I, _I, _I, _I, T, _P, _I, _P, _I)
+        d("syr2k", None, _I, _I, _I, _I, T, _P, _I, _P, _I, _P, _I)
+        d("trmm", None, _I, _I, _I, _I, T, _P, _I, _P, _I, _I)
     lib.oracle_dsdot.restype = ctypes.c_double
     lib.oracle_dsdot.argtypes = [_I, _P, _I, _P, _I]
     lib.oracle_xdsdot.restype = ctypes.c_double
@@ -256,6 +258,19 @@ class Oracle:
         self._chk(A, C)
         ra, ca = (N, K) if trans == NO_TRANS else (K, N)
         self.f("syrk")(uplo, trans, N, K, alpha, mptr(A, ra, ca, lda), lda, mptr(C, N, N, ldc), ldc)
+
+    def syr2k(self, uplo, trans, N, K, alpha, A, lda, B, ldb, C, ldc):
+        self._chk(A, B, C)
+        ra, ca = (N, K) if trans == NO_TRANS else (K, N)
+        self.f("syr2k")(uplo, trans, N, K, alpha, mptr(A, ra, ca, lda), lda, mptr(B, ra, ca, ldb), ldb,
+                        mptr(C, N, N, ldc), ldc)
+
+    def trmm(self, uplo, trans, diag, m, n, alpha, A, lda, B, ldb):
+        """Left side: B <- alpha op(A) B (trmm.py's four Left procs); diag is
+        the CBLAS value (UNIT reads no diagonal)."""
+        self._chk(A, B)
+        self.f("trmm")(int(uplo == UPPER), int(trans != NO_TRANS), m, n, alpha, mptr(A, m, m, lda), lda,
+                       mptr(B, m, n, ldb), ldb, int(diag == UNIT))
 
 
 def strided(n: int, inc: int, rng: np.random.Generator, dtype, fill: Optional[float] = None) -> np.ndarray:

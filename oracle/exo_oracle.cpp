@@ -462,6 +462,59 @@ void syrk(int64_t uplo, int64_t trans, int64_t N, int64_t K, T alpha_, const T* 
 
 }  // namespace
 
+// src/level3/syr2k.py:14-39.  A and B are N x K (NoTrans) or K x N (Trans).
+template <class T, class W>
+void syr2k(int64_t uplo, int64_t trans, int64_t N, int64_t K, T alpha_, const T* A_, int64_t lda, const T* B_,
+           int64_t ldb, T* C_, int64_t ldc) {
+  const int64_t r = trans == kNoTrans ? N : K, c = trans == kNoTrans ? K : N;
+  M2<T, W> A(A_, r, c, lda, false);
+  M2<T, W> B(B_, r, c, ldb, false);
+  M2<T, W> C(C_, N, N, ldc, true);
+  W alpha = alpha_;
+  for (int64_t i = 0; i < N; ++i)
+    for (int64_t j = 0; j < N; ++j)
+      if ((uplo == kUpper && j >= i) || (uplo != kUpper && j < i + 1))
+        for (int64_t k = 0; k < K; ++k) {
+          if (trans == kNoTrans)
+            C(i, j) += alpha * (A(i, k) * B(j, k) + B(i, k) * A(j, k));
+          else
+            C(i, j) += alpha * (A(k, i) * B(k, j) + B(k, i) * A(k, j));
+        }
+}
+
+// src/level3/trmm.py:12-83, the four Left procs, in place on B (m x n); the
+// diagonal of A is read only when diag == 0 (the wrapper passes Diag == Unit).
+// Upper NoTrans (:12-23) and Lower Trans (:72-83) walk the rows downwards and
+// read rows below i, which are not yet updated; Lower NoTrans (:26-46) and
+// Upper Trans (:49-69) walk upwards from the last row and read rows above it.
+template <class T, class W>
+void trmm(int64_t upper, int64_t trans, int64_t m, int64_t n, T alpha_, const T* A_, int64_t lda, T* B_,
+          int64_t ldb, int64_t diag) {
+  M2<T, W> A(A_, m, m, lda, false);
+  M2<T, W> B(B_, m, n, ldb, true);
+  W alpha = alpha_;
+  const bool down = (upper != 0) != (trans != 0);  // Upper NoTrans or Lower Trans
+  for (int64_t j = 0; j < n; ++j)
+    for (int64_t i = 0; i < m; ++i) {
+      if (down) {
+        if (diag == 0) B(i, j) = A(i, i) * B(i, j);
+        for (int64_t k = 0; k < m - i - 1; ++k)
+          B(i, j) += (trans ? A(i + k + 1, i) : A(i, i + k + 1)) * B(i + k + 1, j);
+        B(i, j) = alpha * B(i, j);
+      } else {
+        const int64_t r = m - i - 1;
+        const W org = B(r, j);
+        B(r, j) = 0;
+        for (int64_t k = 0; k < m - i - 1; ++k) B(r, j) += (trans ? A(k, r) : A(r, k)) * B(k, j);
+        if (diag == 0)
+          B(r, j) += A(r, r) * org;
+        else
+          B(r, j) += org;
+        B(r, j) = alpha * B(r, j);
+      }
+    }
+}
+
 // ------------------------------------------------------------ C exports
 // oracle_s*: float strict, oracle_d*: double strict, oracle_x*: double
 // storage with long-double arithmetic (the truth oracle).
@@ -540,6 +593,14 @@ void syrk(int64_t uplo, int64_t trans, int64_t N, int64_t K, T alpha_, const T* 
   extern "C" void oracle_##P##syrk(int64_t up, int64_t tr, int64_t N, int64_t K, T a, const T* A,   \
                                    int64_t lda, T* C, int64_t ldc) {                                \
     syrk<T, W>(up, tr, N, K, a, A, lda, C, ldc);                                                    \
+  }                                                                                                 \
+  extern "C" void oracle_##P##syr2k(int64_t up, int64_t tr, int64_t N, int64_t K, T a, const T* A,  \
+                                    int64_t lda, const T* B, int64_t ldb, T* C, int64_t ldc) {      \
+    syr2k<T, W>(up, tr, N, K, a, A, lda, B, ldb, C, ldc);                                           \
+  }                                                                                                 \
+  extern "C" void oracle_##P##trmm(int64_t upper, int64_t trans, int64_t m, int64_t n, T a,         \
+                                   const T* A, int64_t lda, T* B, int64_t ldb, int64_t diag) {      \
+    trmm<T, W>(upper, trans, m, n, a, A, lda, B, ldb, diag);                                        \
   }
 
 ORACLE_TYPED(s, float, float)

@@ -27,7 +27,8 @@ WRAPPERS = {
                ("syr2", "exo_syr2_wrapper.h", "exo_ssyr2"), ("trmv", "exo_trmv_wrapper.h", "exo_dtrmv"),
                ("trsv", "exo_trsv_wrapper.h", "exo_strsv")],
     "level3": [("gemm", "exo_gemm_wrapper.h", "exo_sgemm"), ("symm", "exo_symm_wrapper.h", "exo_dsymm"),
-               ("syrk", "exo_syrk_wrapper.h", "exo_ssyrk")],
+               ("syrk", "exo_syrk_wrapper.h", "exo_ssyrk"), ("syr2k", "exo_syr2k_wrapper.h", "exo_dsyr2k"),
+               ("strmm", "exo_strmm.h", "exo_strmm")],
 }
 
 CASES = [(lvl, k, h, fn) for lvl, v in WRAPPERS.items() for (k, h, fn) in v]

@@ -375,3 +375,67 @@ def test_syrk_trmscal(dt, uplo, trans, rng):
     ref = np.where(mask, 18.0 * C + 13.0 * G, C)
     assert_bound(CC, ref, 18.0 * np.abs(C) + 13.0 * Ga, K, dt, what="syrk")
     assert bits_equal(CC[~mask], C[~mask])
+
+
+@pytest.mark.parametrize("dt", DT)
+@pytest.mark.parametrize("uplo", [LOWER, UPPER])
+@pytest.mark.parametrize("trans", [NO_TRANS, TRANS])
+def test_syr2k(dt, uplo, trans, rng):
+    N, K = 33, 21
+    shape = (N, K) if trans == NO_TRANS else (K, N)
+    A, B = uniform(rng, shape, dt), uniform(rng, shape, dt)
+    C = uniform(rng, (N, N), dt)
+    CC = C.copy()
+    O(dt).syr2k(uplo, trans, N, K, 13.0, A, A.shape[1], B, B.shape[1], CC, N)
+    mask = np.tril(np.ones((N, N), bool)) if uplo == LOWER else np.triu(np.ones((N, N), bool))
+    A64, B64 = A.astype(np.float64), B.astype(np.float64)
+    G = A64 @ B64.T + B64 @ A64.T if trans == NO_TRANS else A64.T @ B64 + B64.T @ A64
+    aA, aB = np.abs(A64), np.abs(B64)
+    Ga = aA @ aB.T + aB @ aA.T if trans == NO_TRANS else aA.T @ aB + aB.T @ aA
+    ref = np.where(mask, C + 13.0 * G, C)
+    assert_bound(CC, ref, np.abs(C) + 13.0 * Ga, 2 * K, dt, what="syr2k")
+    assert bits_equal(CC[~mask], C[~mask])
+    # library routine: scipy's syr2k on the same (logical) operands
+    p = "s" if dt == np.float32 else "d"
+    lib = getattr(sb, p + "syr2k")(13.0, A, B, beta=1.0, c=C.copy(), trans=int(trans != NO_TRANS),
+                                   lower=int(uplo == LOWER))
+    assert_bound(CC[mask], lib[mask], (np.abs(C) + 13.0 * Ga)[mask], 2 * K, dt, what="syr2k/scipy")
+    # planted mistake: one product only (A B^T without B A^T) fails the bound
+    G1 = A64 @ B64.T if trans == NO_TRANS else A64.T @ B64
+    with pytest.raises(AssertionError):
+        assert_bound(CC, np.where(mask, C + 13.0 * (G1 + G1), C), np.abs(C) + 13.0 * Ga, 2 * K, dt, c=1.0)
+
+
+@pytest.mark.parametrize("dt", DT)
+@pytest.mark.parametrize("uplo", [LOWER, UPPER])
+@pytest.mark.parametrize("trans", [NO_TRANS, TRANS])
+@pytest.mark.parametrize("diag", [NON_UNIT, UNIT])
+@pytest.mark.parametrize("m,n,ld_diff", [(4, 8, 0), (29, 15, 5)])
+def test_trmm(dt, uplo, trans, diag, m, n, ld_diff, rng):
+    lda, ldb = m + ld_diff, n + ld_diff
+    A = uniform(rng, (m, lda), dt)
+    B = uniform(rng, (m, ldb), dt)
+    BB = B.copy()
+    O(dt).trmm(uplo, trans, diag, m, n, 1.4, A, lda, BB, ldb)
+    T = np.tril(A[:, :m].astype(np.float64)) if uplo == LOWER else np.triu(A[:, :m].astype(np.float64))
+    if diag == UNIT:
+        np.fill_diagonal(T, 1.0)
+    opT = T if trans == NO_TRANS else T.T
+    B64 = B[:, :n].astype(np.float64)
+    ref = 1.4 * (opT @ B64)
+    mag = 1.4 * (np.abs(opT) @ np.abs(B64))
+    assert_bound(BB[:, :n], ref, mag, m, dt, what="trmm")
+    assert bits_equal(BB[:, n:], B[:, n:])  # padding columns untouched
+    # library routine: scipy's trmm (logical operands, Left side)
+    p = "s" if dt == np.float32 else "d"
+    lib = getattr(sb, p + "trmm")(1.4, np.ascontiguousarray(A[:, :m]), np.ascontiguousarray(B[:, :n]), side=0,
+                                  lower=int(uplo == LOWER), trans_a=int(trans != NO_TRANS), diag=int(diag == UNIT))
+    assert_bound(BB[:, :n], lib, mag, m, dt, what="trmm/scipy")
+    # planted mistake: the other triangle fails the bound
+    Tw = np.triu(A[:, :m].astype(np.float64)) if uplo == LOWER else np.tril(A[:, :m].astype(np.float64))
+    if diag == UNIT:
+        np.fill_diagonal(Tw, 1.0)
+    wrong = 1.4 * ((Tw if trans == NO_TRANS else Tw.T) @ B64)
+    if m > 1:
+        with pytest.raises(AssertionError):
+            assert_bound(BB[:, :n], wrong, mag, m, dt, c=1.0)
