@@ -202,6 +202,42 @@ def _make(p: str):
             _call(f"exo_{p}{kind}_rm_{v}", Uplo, TransA, Diag, N, L.win1(x, incX), L.win2(_addr(A), lda))
         return f
 
+    # ---------------------------------------------------------------- banded level 2
+    def gbmv(M, N, KL, KU, alpha, A, lda, X, incX, beta, Y, incY):
+        # test/level2/sgbmv/exo_sgbmv.h:5-20 (no Order argument, no re-basing)
+        v = "stride_1" if incX == 1 and incY == 1 else "stride_any"
+        _call(f"exo_{p}gbmv_row_major_NonTrans_{v}", M, N, KL, KU, ref(alpha), ref(beta), L.win2(_addr(A), lda),
+              L.win1(_addr(X), incX), L.win1(_addr(Y), incY))
+
+    def sbmv(order, Uplo, N, K, alpha, A, lda, X, incX, beta, Y, incY):
+        # test/level2/ssbmv/exo_ssbmv.h:7-53: beta through scal_y when != 1, then the product
+        if order != ROW_MAJOR:
+            raise UnsupportedParameterException("Unsupported Exception")
+        x = _rebase(_addr(X), N, incX, es)
+        y = _rebase(_addr(Y), N, incY, es)
+        if beta != 1.0:
+            v = "stride_1" if incY == 1 else "stride_any"
+            _call(f"exo_{p}sbmv_scal_y_{v}", N, ref(beta), L.win1(y, incY))
+        u = "Upper" if Uplo == L.UPPER else "Lower"
+        v = "stride_1" if incX == 1 and incY == 1 else "stride_any"
+        _call(f"exo_{p}sbmv_row_major_{u}_{v}", N, K, ref(alpha), L.win2(_addr(A), lda), L.win1(x, incX),
+              L.win1(y, incY))
+
+    def _tb(kind):
+        def f(order, Uplo, TransA, Diag, N, K, A, lda, X, incX):
+            # test/level2/stbmv/exo_stbmv.h:6-70, stbsv/exo_stbsv.h:6-56; tbsv's
+            # Trans forms exist as _stride_any only, whatever incX
+            if order != ROW_MAJOR:
+                raise UnsupportedParameterException("Unsupported Exception")
+            x = _rebase(_addr(X), N, incX, es)
+            u = "Upper" if Uplo == L.UPPER else "Lower"
+            t = "NonTrans" if TransA == L.NO_TRANS else "Trans"
+            one = incX == 1 and not (kind == "tbsv" and t == "Trans")
+            v = "stride_1" if one else "stride_any"
+            _call(f"exo_{p}{kind}_row_major_{u}_{t}_{v}", N, K, L.win1(x, incX), L.win2(_addr(A), lda),
+                  int(Diag == L.UNIT))
+        return f
+
     # ---------------------------------------------------------------- level 3
     def gemm(Order, TransA, TransB, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc):
         # test/level3/gemm/exo_gemm_wrapper.h:10-28: C *= beta, then C += alpha op(A) op(B)
@@ -253,7 +289,8 @@ def _make(p: str):
 
     ns.update(asum=asum, axpy=axpy, dot=dot, copy=_pair("copy"), swap=_pair("swap"), rot=rot, rotm=rotm,
               scal=scal, gemv=gemv, ger=ger, symv=symv, syr=syr, syr2=syr2, trmv=_tr("trmv"),
-              trsv=_tr("trsv"), gemm=gemm, symm=symm, syrk=syrk, syr2k=syr2k)
+              trsv=_tr("trsv"), gemm=gemm, symm=symm, syrk=syrk, syr2k=syr2k, gbmv=gbmv, sbmv=sbmv,
+              tbmv=_tb("tbmv"), tbsv=_tb("tbsv"))
     if p == "s":  # the reference exports strmm only (src/level3/trmm.py:133-142)
         ns.update(trmm=trmm)
     return {f"exo_{p}{k}": v for k, v in ns.items()}

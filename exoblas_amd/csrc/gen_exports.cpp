@@ -18,6 +18,10 @@
 #include "exo_syr2.h"
 #include "exo_trmv.h"
 #include "exo_trsv.h"
+#include "exo_gbmv.h"
+#include "exo_sbmv.h"
+#include "exo_tbmv.h"
+#include "exo_tbsv.h"
 #include "exo_gemm.h"
 #include "exo_mscal.h"
 #include "exo_symm.h"
@@ -2212,6 +2216,1342 @@ extern "C" int exoblas_dtrsv_rm_stride_1_async(hipStream_t stream, int_fast32_t 
   if (!call.stage()) return call.status();
   if (!call.ok()) return call.status();
   call.run([&] { exoblas::ops::trsv_rm<double>(call.ctx(), (int64_t)Uplo, (int64_t)TransA, (int64_t)Diag, (int64_t)n, call.dev<double>(0), (int64_t)x.strides[0], call.dev<const double>(1), (int64_t)A.strides[0]); });
+  return call.finish();
+}
+
+// src/level2/gbmv.py:13-52
+extern "C" void exo_sgbmv_row_major_NonTrans_stride_any(void *ctxt, int_fast32_t m, int_fast32_t n, int_fast32_t kl, int_fast32_t ku, const float *alpha, const float *beta, struct exo_win_2f32c a, struct exo_win_1f32c x, struct exo_win_1f32 y) {
+  exoblas::Call call("exo_sgbmv_row_major_NonTrans_stride_any", exoblas::Call::kSync, ctxt);
+  if (m < 0) return (void)call.fail_size("m", m);
+  if (n < 0) return (void)call.fail_size("n", n);
+  if (kl < 0) return (void)call.fail_size("kl", kl);
+  if (ku < 0) return (void)call.fail_size("ku", ku);
+  if ((true) && a.strides[1] != 1) return (void)call.fail_stride("a", a.strides[1]);
+  if ((true) && m > 1 && a.strides[0] < kl + ku + 1) return (void)call.fail_ld("a", a.strides[0]);
+  call.window(0, a.data, 0, exoblas::span2_hi(m, kl + ku + 1, a.strides[0]), sizeof(float), false, true);
+  call.window(1, x.data, exoblas::span_lo(n, x.strides[0]), exoblas::span_hi(n, x.strides[0]), sizeof(float), false, true);
+  call.window(2, y.data, exoblas::span_lo(m, y.strides[0]), exoblas::span_hi(m, y.strides[0]), sizeof(float), true, true);
+  if (!call.stage()) return (void)call.status();
+  const float alpha_v = call.scalar(alpha);
+  const float beta_v = call.scalar(beta);
+  if (!call.ok()) return (void)call.status();
+  call.run([&] { exoblas::ops::gbmv_row_major_NonTrans<float>(call.ctx(), (int64_t)m, (int64_t)n, (int64_t)kl, (int64_t)ku, alpha_v, beta_v, call.dev<const float>(0), (int64_t)a.strides[0], call.dev<const float>(1), (int64_t)x.strides[0], call.dev<float>(2), (int64_t)y.strides[0]); });
+  return (void)call.finish();
+}
+extern "C" int exoblas_sgbmv_row_major_NonTrans_stride_any_async(hipStream_t stream, int_fast32_t m, int_fast32_t n, int_fast32_t kl, int_fast32_t ku, const float *alpha, const float *beta, struct exo_win_2f32c a, struct exo_win_1f32c x, struct exo_win_1f32 y) {
+  exoblas::Call call("exoblas_sgbmv_row_major_NonTrans_stride_any_async", exoblas::Call::kAsync, stream);
+  if (m < 0) return call.fail_size("m", m);
+  if (n < 0) return call.fail_size("n", n);
+  if (kl < 0) return call.fail_size("kl", kl);
+  if (ku < 0) return call.fail_size("ku", ku);
+  if ((true) && a.strides[1] != 1) return call.fail_stride("a", a.strides[1]);
+  if ((true) && m > 1 && a.strides[0] < kl + ku + 1) return call.fail_ld("a", a.strides[0]);
+  call.window(0, a.data, 0, exoblas::span2_hi(m, kl + ku + 1, a.strides[0]), sizeof(float), false, true);
+  call.window(1, x.data, exoblas::span_lo(n, x.strides[0]), exoblas::span_hi(n, x.strides[0]), sizeof(float), false, true);
+  call.window(2, y.data, exoblas::span_lo(m, y.strides[0]), exoblas::span_hi(m, y.strides[0]), sizeof(float), true, true);
+  if (!call.stage()) return call.status();
+  const float alpha_v = call.scalar(alpha);
+  const float beta_v = call.scalar(beta);
+  if (!call.ok()) return call.status();
+  call.run([&] { exoblas::ops::gbmv_row_major_NonTrans<float>(call.ctx(), (int64_t)m, (int64_t)n, (int64_t)kl, (int64_t)ku, alpha_v, beta_v, call.dev<const float>(0), (int64_t)a.strides[0], call.dev<const float>(1), (int64_t)x.strides[0], call.dev<float>(2), (int64_t)y.strides[0]); });
+  return call.finish();
+}
+
+// src/level2/gbmv.py:13-52
+extern "C" void exo_sgbmv_row_major_NonTrans_stride_1(void *ctxt, int_fast32_t m, int_fast32_t n, int_fast32_t kl, int_fast32_t ku, const float *alpha, const float *beta, struct exo_win_2f32c a, struct exo_win_1f32c x, struct exo_win_1f32 y) {
+  exoblas::Call call("exo_sgbmv_row_major_NonTrans_stride_1", exoblas::Call::kSync, ctxt);
+  if (m < 0) return (void)call.fail_size("m", m);
+  if (n < 0) return (void)call.fail_size("n", n);
+  if (kl < 0) return (void)call.fail_size("kl", kl);
+  if (ku < 0) return (void)call.fail_size("ku", ku);
+  if ((true) && a.strides[1] != 1) return (void)call.fail_stride("a", a.strides[1]);
+  if ((true) && m > 1 && a.strides[0] < kl + ku + 1) return (void)call.fail_ld("a", a.strides[0]);
+  call.window(0, a.data, 0, exoblas::span2_hi(m, kl + ku + 1, a.strides[0]), sizeof(float), false, true);
+  call.window(1, x.data, exoblas::span_lo(n, x.strides[0]), exoblas::span_hi(n, x.strides[0]), sizeof(float), false, true);
+  call.window(2, y.data, exoblas::span_lo(m, y.strides[0]), exoblas::span_hi(m, y.strides[0]), sizeof(float), true, true);
+  if (!call.stage()) return (void)call.status();
+  const float alpha_v = call.scalar(alpha);
+  const float beta_v = call.scalar(beta);
+  if (!call.ok()) return (void)call.status();
+  call.run([&] { exoblas::ops::gbmv_row_major_NonTrans<float>(call.ctx(), (int64_t)m, (int64_t)n, (int64_t)kl, (int64_t)ku, alpha_v, beta_v, call.dev<const float>(0), (int64_t)a.strides[0], call.dev<const float>(1), (int64_t)x.strides[0], call.dev<float>(2), (int64_t)y.strides[0]); });
+  return (void)call.finish();
+}
+extern "C" int exoblas_sgbmv_row_major_NonTrans_stride_1_async(hipStream_t stream, int_fast32_t m, int_fast32_t n, int_fast32_t kl, int_fast32_t ku, const float *alpha, const float *beta, struct exo_win_2f32c a, struct exo_win_1f32c x, struct exo_win_1f32 y) {
+  exoblas::Call call("exoblas_sgbmv_row_major_NonTrans_stride_1_async", exoblas::Call::kAsync, stream);
+  if (m < 0) return call.fail_size("m", m);
+  if (n < 0) return call.fail_size("n", n);
+  if (kl < 0) return call.fail_size("kl", kl);
+  if (ku < 0) return call.fail_size("ku", ku);
+  if ((true) && a.strides[1] != 1) return call.fail_stride("a", a.strides[1]);
+  if ((true) && m > 1 && a.strides[0] < kl + ku + 1) return call.fail_ld("a", a.strides[0]);
+  call.window(0, a.data, 0, exoblas::span2_hi(m, kl + ku + 1, a.strides[0]), sizeof(float), false, true);
+  call.window(1, x.data, exoblas::span_lo(n, x.strides[0]), exoblas::span_hi(n, x.strides[0]), sizeof(float), false, true);
+  call.window(2, y.data, exoblas::span_lo(m, y.strides[0]), exoblas::span_hi(m, y.strides[0]), sizeof(float), true, true);
+  if (!call.stage()) return call.status();
+  const float alpha_v = call.scalar(alpha);
+  const float beta_v = call.scalar(beta);
+  if (!call.ok()) return call.status();
+  call.run([&] { exoblas::ops::gbmv_row_major_NonTrans<float>(call.ctx(), (int64_t)m, (int64_t)n, (int64_t)kl, (int64_t)ku, alpha_v, beta_v, call.dev<const float>(0), (int64_t)a.strides[0], call.dev<const float>(1), (int64_t)x.strides[0], call.dev<float>(2), (int64_t)y.strides[0]); });
+  return call.finish();
+}
+
+// src/level2/gbmv.py:13-52
+extern "C" void exo_dgbmv_row_major_NonTrans_stride_any(void *ctxt, int_fast32_t m, int_fast32_t n, int_fast32_t kl, int_fast32_t ku, const double *alpha, const double *beta, struct exo_win_2f64c a, struct exo_win_1f64c x, struct exo_win_1f64 y) {
+  exoblas::Call call("exo_dgbmv_row_major_NonTrans_stride_any", exoblas::Call::kSync, ctxt);
+  if (m < 0) return (void)call.fail_size("m", m);
+  if (n < 0) return (void)call.fail_size("n", n);
+  if (kl < 0) return (void)call.fail_size("kl", kl);
+  if (ku < 0) return (void)call.fail_size("ku", ku);
+  if ((true) && a.strides[1] != 1) return (void)call.fail_stride("a", a.strides[1]);
+  if ((true) && m > 1 && a.strides[0] < kl + ku + 1) return (void)call.fail_ld("a", a.strides[0]);
+  call.window(0, a.data, 0, exoblas::span2_hi(m, kl + ku + 1, a.strides[0]), sizeof(double), false, true);
+  call.window(1, x.data, exoblas::span_lo(n, x.strides[0]), exoblas::span_hi(n, x.strides[0]), sizeof(double), false, true);
+  call.window(2, y.data, exoblas::span_lo(m, y.strides[0]), exoblas::span_hi(m, y.strides[0]), sizeof(double), true, true);
+  if (!call.stage()) return (void)call.status();
+  const double alpha_v = call.scalar(alpha);
+  const double beta_v = call.scalar(beta);
+  if (!call.ok()) return (void)call.status();
+  call.run([&] { exoblas::ops::gbmv_row_major_NonTrans<double>(call.ctx(), (int64_t)m, (int64_t)n, (int64_t)kl, (int64_t)ku, alpha_v, beta_v, call.dev<const double>(0), (int64_t)a.strides[0], call.dev<const double>(1), (int64_t)x.strides[0], call.dev<double>(2), (int64_t)y.strides[0]); });
+  return (void)call.finish();
+}
+extern "C" int exoblas_dgbmv_row_major_NonTrans_stride_any_async(hipStream_t stream, int_fast32_t m, int_fast32_t n, int_fast32_t kl, int_fast32_t ku, const double *alpha, const double *beta, struct exo_win_2f64c a, struct exo_win_1f64c x, struct exo_win_1f64 y) {
+  exoblas::Call call("exoblas_dgbmv_row_major_NonTrans_stride_any_async", exoblas::Call::kAsync, stream);
+  if (m < 0) return call.fail_size("m", m);
+  if (n < 0) return call.fail_size("n", n);
+  if (kl < 0) return call.fail_size("kl", kl);
+  if (ku < 0) return call.fail_size("ku", ku);
+  if ((true) && a.strides[1] != 1) return call.fail_stride("a", a.strides[1]);
+  if ((true) && m > 1 && a.strides[0] < kl + ku + 1) return call.fail_ld("a", a.strides[0]);
+  call.window(0, a.data, 0, exoblas::span2_hi(m, kl + ku + 1, a.strides[0]), sizeof(double), false, true);
+  call.window(1, x.data, exoblas::span_lo(n, x.strides[0]), exoblas::span_hi(n, x.strides[0]), sizeof(double), false, true);
+  call.window(2, y.data, exoblas::span_lo(m, y.strides[0]), exoblas::span_hi(m, y.strides[0]), sizeof(double), true, true);
+  if (!call.stage()) return call.status();
+  const double alpha_v = call.scalar(alpha);
+  const double beta_v = call.scalar(beta);
+  if (!call.ok()) return call.status();
+  call.run([&] { exoblas::ops::gbmv_row_major_NonTrans<double>(call.ctx(), (int64_t)m, (int64_t)n, (int64_t)kl, (int64_t)ku, alpha_v, beta_v, call.dev<const double>(0), (int64_t)a.strides[0], call.dev<const double>(1), (int64_t)x.strides[0], call.dev<double>(2), (int64_t)y.strides[0]); });
+  return call.finish();
+}
+
+// src/level2/gbmv.py:13-52
+extern "C" void exo_dgbmv_row_major_NonTrans_stride_1(void *ctxt, int_fast32_t m, int_fast32_t n, int_fast32_t kl, int_fast32_t ku, const double *alpha, const double *beta, struct exo_win_2f64c a, struct exo_win_1f64c x, struct exo_win_1f64 y) {
+  exoblas::Call call("exo_dgbmv_row_major_NonTrans_stride_1", exoblas::Call::kSync, ctxt);
+  if (m < 0) return (void)call.fail_size("m", m);
+  if (n < 0) return (void)call.fail_size("n", n);
+  if (kl < 0) return (void)call.fail_size("kl", kl);
+  if (ku < 0) return (void)call.fail_size("ku", ku);
+  if ((true) && a.strides[1] != 1) return (void)call.fail_stride("a", a.strides[1]);
+  if ((true) && m > 1 && a.strides[0] < kl + ku + 1) return (void)call.fail_ld("a", a.strides[0]);
+  call.window(0, a.data, 0, exoblas::span2_hi(m, kl + ku + 1, a.strides[0]), sizeof(double), false, true);
+  call.window(1, x.data, exoblas::span_lo(n, x.strides[0]), exoblas::span_hi(n, x.strides[0]), sizeof(double), false, true);
+  call.window(2, y.data, exoblas::span_lo(m, y.strides[0]), exoblas::span_hi(m, y.strides[0]), sizeof(double), true, true);
+  if (!call.stage()) return (void)call.status();
+  const double alpha_v = call.scalar(alpha);
+  const double beta_v = call.scalar(beta);
+  if (!call.ok()) return (void)call.status();
+  call.run([&] { exoblas::ops::gbmv_row_major_NonTrans<double>(call.ctx(), (int64_t)m, (int64_t)n, (int64_t)kl, (int64_t)ku, alpha_v, beta_v, call.dev<const double>(0), (int64_t)a.strides[0], call.dev<const double>(1), (int64_t)x.strides[0], call.dev<double>(2), (int64_t)y.strides[0]); });
+  return (void)call.finish();
+}
+extern "C" int exoblas_dgbmv_row_major_NonTrans_stride_1_async(hipStream_t stream, int_fast32_t m, int_fast32_t n, int_fast32_t kl, int_fast32_t ku, const double *alpha, const double *beta, struct exo_win_2f64c a, struct exo_win_1f64c x, struct exo_win_1f64 y) {
+  exoblas::Call call("exoblas_dgbmv_row_major_NonTrans_stride_1_async", exoblas::Call::kAsync, stream);
+  if (m < 0) return call.fail_size("m", m);
+  if (n < 0) return call.fail_size("n", n);
+  if (kl < 0) return call.fail_size("kl", kl);
+  if (ku < 0) return call.fail_size("ku", ku);
+  if ((true) && a.strides[1] != 1) return call.fail_stride("a", a.strides[1]);
+  if ((true) && m > 1 && a.strides[0] < kl + ku + 1) return call.fail_ld("a", a.strides[0]);
+  call.window(0, a.data, 0, exoblas::span2_hi(m, kl + ku + 1, a.strides[0]), sizeof(double), false, true);
+  call.window(1, x.data, exoblas::span_lo(n, x.strides[0]), exoblas::span_hi(n, x.strides[0]), sizeof(double), false, true);
+  call.window(2, y.data, exoblas::span_lo(m, y.strides[0]), exoblas::span_hi(m, y.strides[0]), sizeof(double), true, true);
+  if (!call.stage()) return call.status();
+  const double alpha_v = call.scalar(alpha);
+  const double beta_v = call.scalar(beta);
+  if (!call.ok()) return call.status();
+  call.run([&] { exoblas::ops::gbmv_row_major_NonTrans<double>(call.ctx(), (int64_t)m, (int64_t)n, (int64_t)kl, (int64_t)ku, alpha_v, beta_v, call.dev<const double>(0), (int64_t)a.strides[0], call.dev<const double>(1), (int64_t)x.strides[0], call.dev<double>(2), (int64_t)y.strides[0]); });
+  return call.finish();
+}
+
+// src/level2/sbmv.py:12-16
+extern "C" void exo_ssbmv_scal_y_stride_any(void *ctxt, int_fast32_t n, const float *beta, struct exo_win_1f32 y) {
+  exoblas::Call call("exo_ssbmv_scal_y_stride_any", exoblas::Call::kSync, ctxt);
+  if (n < 0) return (void)call.fail_size("n", n);
+  call.window(0, y.data, exoblas::span_lo(n, y.strides[0]), exoblas::span_hi(n, y.strides[0]), sizeof(float), true, true);
+  if (!call.stage()) return (void)call.status();
+  const float beta_v = call.scalar(beta);
+  if (!call.ok()) return (void)call.status();
+  call.run([&] { exoblas::ops::sbmv_scal_y<float>(call.ctx(), (int64_t)n, beta_v, call.dev<float>(0), (int64_t)y.strides[0]); });
+  return (void)call.finish();
+}
+extern "C" int exoblas_ssbmv_scal_y_stride_any_async(hipStream_t stream, int_fast32_t n, const float *beta, struct exo_win_1f32 y) {
+  exoblas::Call call("exoblas_ssbmv_scal_y_stride_any_async", exoblas::Call::kAsync, stream);
+  if (n < 0) return call.fail_size("n", n);
+  call.window(0, y.data, exoblas::span_lo(n, y.strides[0]), exoblas::span_hi(n, y.strides[0]), sizeof(float), true, true);
+  if (!call.stage()) return call.status();
+  const float beta_v = call.scalar(beta);
+  if (!call.ok()) return call.status();
+  call.run([&] { exoblas::ops::sbmv_scal_y<float>(call.ctx(), (int64_t)n, beta_v, call.dev<float>(0), (int64_t)y.strides[0]); });
+  return call.finish();
+}
+
+// src/level2/sbmv.py:12-16
+extern "C" void exo_ssbmv_scal_y_stride_1(void *ctxt, int_fast32_t n, const float *beta, struct exo_win_1f32 y) {
+  exoblas::Call call("exo_ssbmv_scal_y_stride_1", exoblas::Call::kSync, ctxt);
+  if (n < 0) return (void)call.fail_size("n", n);
+  call.window(0, y.data, exoblas::span_lo(n, y.strides[0]), exoblas::span_hi(n, y.strides[0]), sizeof(float), true, true);
+  if (!call.stage()) return (void)call.status();
+  const float beta_v = call.scalar(beta);
+  if (!call.ok()) return (void)call.status();
+  call.run([&] { exoblas::ops::sbmv_scal_y<float>(call.ctx(), (int64_t)n, beta_v, call.dev<float>(0), (int64_t)y.strides[0]); });
+  return (void)call.finish();
+}
+extern "C" int exoblas_ssbmv_scal_y_stride_1_async(hipStream_t stream, int_fast32_t n, const float *beta, struct exo_win_1f32 y) {
+  exoblas::Call call("exoblas_ssbmv_scal_y_stride_1_async", exoblas::Call::kAsync, stream);
+  if (n < 0) return call.fail_size("n", n);
+  call.window(0, y.data, exoblas::span_lo(n, y.strides[0]), exoblas::span_hi(n, y.strides[0]), sizeof(float), true, true);
+  if (!call.stage()) return call.status();
+  const float beta_v = call.scalar(beta);
+  if (!call.ok()) return call.status();
+  call.run([&] { exoblas::ops::sbmv_scal_y<float>(call.ctx(), (int64_t)n, beta_v, call.dev<float>(0), (int64_t)y.strides[0]); });
+  return call.finish();
+}
+
+// src/level2/sbmv.py:12-16
+extern "C" void exo_dsbmv_scal_y_stride_any(void *ctxt, int_fast32_t n, const double *beta, struct exo_win_1f64 y) {
+  exoblas::Call call("exo_dsbmv_scal_y_stride_any", exoblas::Call::kSync, ctxt);
+  if (n < 0) return (void)call.fail_size("n", n);
+  call.window(0, y.data, exoblas::span_lo(n, y.strides[0]), exoblas::span_hi(n, y.strides[0]), sizeof(double), true, true);
+  if (!call.stage()) return (void)call.status();
+  const double beta_v = call.scalar(beta);
+  if (!call.ok()) return (void)call.status();
+  call.run([&] { exoblas::ops::sbmv_scal_y<double>(call.ctx(), (int64_t)n, beta_v, call.dev<double>(0), (int64_t)y.strides[0]); });
+  return (void)call.finish();
+}
+extern "C" int exoblas_dsbmv_scal_y_stride_any_async(hipStream_t stream, int_fast32_t n, const double *beta, struct exo_win_1f64 y) {
+  exoblas::Call call("exoblas_dsbmv_scal_y_stride_any_async", exoblas::Call::kAsync, stream);
+  if (n < 0) return call.fail_size("n", n);
+  call.window(0, y.data, exoblas::span_lo(n, y.strides[0]), exoblas::span_hi(n, y.strides[0]), sizeof(double), true, true);
+  if (!call.stage()) return call.status();
+  const double beta_v = call.scalar(beta);
+  if (!call.ok()) return call.status();
+  call.run([&] { exoblas::ops::sbmv_scal_y<double>(call.ctx(), (int64_t)n, beta_v, call.dev<double>(0), (int64_t)y.strides[0]); });
+  return call.finish();
+}
+
+// src/level2/sbmv.py:12-16
+extern "C" void exo_dsbmv_scal_y_stride_1(void *ctxt, int_fast32_t n, const double *beta, struct exo_win_1f64 y) {
+  exoblas::Call call("exo_dsbmv_scal_y_stride_1", exoblas::Call::kSync, ctxt);
+  if (n < 0) return (void)call.fail_size("n", n);
+  call.window(0, y.data, exoblas::span_lo(n, y.strides[0]), exoblas::span_hi(n, y.strides[0]), sizeof(double), true, true);
+  if (!call.stage()) return (void)call.status();
+  const double beta_v = call.scalar(beta);
+  if (!call.ok()) return (void)call.status();
+  call.run([&] { exoblas::ops::sbmv_scal_y<double>(call.ctx(), (int64_t)n, beta_v, call.dev<double>(0), (int64_t)y.strides[0]); });
+  return (void)call.finish();
+}
+extern "C" int exoblas_dsbmv_scal_y_stride_1_async(hipStream_t stream, int_fast32_t n, const double *beta, struct exo_win_1f64 y) {
+  exoblas::Call call("exoblas_dsbmv_scal_y_stride_1_async", exoblas::Call::kAsync, stream);
+  if (n < 0) return call.fail_size("n", n);
+  call.window(0, y.data, exoblas::span_lo(n, y.strides[0]), exoblas::span_hi(n, y.strides[0]), sizeof(double), true, true);
+  if (!call.stage()) return call.status();
+  const double beta_v = call.scalar(beta);
+  if (!call.ok()) return call.status();
+  call.run([&] { exoblas::ops::sbmv_scal_y<double>(call.ctx(), (int64_t)n, beta_v, call.dev<double>(0), (int64_t)y.strides[0]); });
+  return call.finish();
+}
+
+// src/level2/sbmv.py:19-33
+extern "C" void exo_ssbmv_row_major_Upper_stride_any(void *ctxt, int_fast32_t n, int_fast32_t k, const float *alpha, struct exo_win_2f32c A, struct exo_win_1f32c x, struct exo_win_1f32 y) {
+  exoblas::Call call("exo_ssbmv_row_major_Upper_stride_any", exoblas::Call::kSync, ctxt);
+  if (n < 0) return (void)call.fail_size("n", n);
+  if (k < 0) return (void)call.fail_size("k", k);
+  if ((true) && A.strides[1] != 1) return (void)call.fail_stride("A", A.strides[1]);
+  if ((true) && n > 1 && A.strides[0] < k + 1) return (void)call.fail_ld("A", A.strides[0]);
+  call.window(0, A.data, 0, exoblas::span2_hi(n, k + 1, A.strides[0]), sizeof(float), false, true);
+  call.window(1, x.data, exoblas::span_lo(n, x.strides[0]), exoblas::span_hi(n, x.strides[0]), sizeof(float), false, true);
+  call.window(2, y.data, exoblas::span_lo(n, y.strides[0]), exoblas::span_hi(n, y.strides[0]), sizeof(float), true, true);
+  if (!call.stage()) return (void)call.status();
+  const float alpha_v = call.scalar(alpha);
+  if (!call.ok()) return (void)call.status();
+  call.run([&] { exoblas::ops::sbmv_row_major_Upper<float>(call.ctx(), (int64_t)n, (int64_t)k, alpha_v, call.dev<const float>(0), (int64_t)A.strides[0], call.dev<const float>(1), (int64_t)x.strides[0], call.dev<float>(2), (int64_t)y.strides[0]); });
+  return (void)call.finish();
+}
+extern "C" int exoblas_ssbmv_row_major_Upper_stride_any_async(hipStream_t stream, int_fast32_t n, int_fast32_t k, const float *alpha, struct exo_win_2f32c A, struct exo_win_1f32c x, struct exo_win_1f32 y) {
+  exoblas::Call call("exoblas_ssbmv_row_major_Upper_stride_any_async", exoblas::Call::kAsync, stream);
+  if (n < 0) return call.fail_size("n", n);
+  if (k < 0) return call.fail_size("k", k);
+  if ((true) && A.strides[1] != 1) return call.fail_stride("A", A.strides[1]);
+  if ((true) && n > 1 && A.strides[0] < k + 1) return call.fail_ld("A", A.strides[0]);
+  call.window(0, A.data, 0, exoblas::span2_hi(n, k + 1, A.strides[0]), sizeof(float), false, true);
+  call.window(1, x.data, exoblas::span_lo(n, x.strides[0]), exoblas::span_hi(n, x.strides[0]), sizeof(float), false, true);
+  call.window(2, y.data, exoblas::span_lo(n, y.strides[0]), exoblas::span_hi(n, y.strides[0]), sizeof(float), true, true);
+  if (!call.stage()) return call.status();
+  const float alpha_v = call.scalar(alpha);
+  if (!call.ok()) return call.status();
+  call.run([&] { exoblas::ops::sbmv_row_major_Upper<float>(call.ctx(), (int64_t)n, (int64_t)k, alpha_v, call.dev<const float>(0), (int64_t)A.strides[0], call.dev<const float>(1), (int64_t)x.strides[0], call.dev<float>(2), (int64_t)y.strides[0]); });
+  return call.finish();
+}
+
+// src/level2/sbmv.py:19-33
+extern "C" void exo_ssbmv_row_major_Upper_stride_1(void *ctxt, int_fast32_t n, int_fast32_t k, const float *alpha, struct exo_win_2f32c A, struct exo_win_1f32c x, struct exo_win_1f32 y) {
+  exoblas::Call call("exo_ssbmv_row_major_Upper_stride_1", exoblas::Call::kSync, ctxt);
+  if (n < 0) return (void)call.fail_size("n", n);
+  if (k < 0) return (void)call.fail_size("k", k);
+  if ((true) && A.strides[1] != 1) return (void)call.fail_stride("A", A.strides[1]);
+  if ((true) && n > 1 && A.strides[0] < k + 1) return (void)call.fail_ld("A", A.strides[0]);
+  call.window(0, A.data, 0, exoblas::span2_hi(n, k + 1, A.strides[0]), sizeof(float), false, true);
+  call.window(1, x.data, exoblas::span_lo(n, x.strides[0]), exoblas::span_hi(n, x.strides[0]), sizeof(float), false, true);
+  call.window(2, y.data, exoblas::span_lo(n, y.strides[0]), exoblas::span_hi(n, y.strides[0]), sizeof(float), true, true);
+  if (!call.stage()) return (void)call.status();
+  const float alpha_v = call.scalar(alpha);
+  if (!call.ok()) return (void)call.status();
+  call.run([&] { exoblas::ops::sbmv_row_major_Upper<float>(call.ctx(), (int64_t)n, (int64_t)k, alpha_v, call.dev<const float>(0), (int64_t)A.strides[0], call.dev<const float>(1), (int64_t)x.strides[0], call.dev<float>(2), (int64_t)y.strides[0]); });
+  return (void)call.finish();
+}
+extern "C" int exoblas_ssbmv_row_major_Upper_stride_1_async(hipStream_t stream, int_fast32_t n, int_fast32_t k, const float *alpha, struct exo_win_2f32c A, struct exo_win_1f32c x, struct exo_win_1f32 y) {
+  exoblas::Call call("exoblas_ssbmv_row_major_Upper_stride_1_async", exoblas::Call::kAsync, stream);
+  if (n < 0) return call.fail_size("n", n);
+  if (k < 0) return call.fail_size("k", k);
+  if ((true) && A.strides[1] != 1) return call.fail_stride("A", A.strides[1]);
+  if ((true) && n > 1 && A.strides[0] < k + 1) return call.fail_ld("A", A.strides[0]);
+  call.window(0, A.data, 0, exoblas::span2_hi(n, k + 1, A.strides[0]), sizeof(float), false, true);
+  call.window(1, x.data, exoblas::span_lo(n, x.strides[0]), exoblas::span_hi(n, x.strides[0]), sizeof(float), false, true);
+  call.window(2, y.data, exoblas::span_lo(n, y.strides[0]), exoblas::span_hi(n, y.strides[0]), sizeof(float), true, true);
+  if (!call.stage()) return call.status();
+  const float alpha_v = call.scalar(alpha);
+  if (!call.ok()) return call.status();
+  call.run([&] { exoblas::ops::sbmv_row_major_Upper<float>(call.ctx(), (int64_t)n, (int64_t)k, alpha_v, call.dev<const float>(0), (int64_t)A.strides[0], call.dev<const float>(1), (int64_t)x.strides[0], call.dev<float>(2), (int64_t)y.strides[0]); });
+  return call.finish();
+}
+
+// src/level2/sbmv.py:19-33
+extern "C" void exo_dsbmv_row_major_Upper_stride_any(void *ctxt, int_fast32_t n, int_fast32_t k, const double *alpha, struct exo_win_2f64c A, struct exo_win_1f64c x, struct exo_win_1f64 y) {
+  exoblas::Call call("exo_dsbmv_row_major_Upper_stride_any", exoblas::Call::kSync, ctxt);
+  if (n < 0) return (void)call.fail_size("n", n);
+  if (k < 0) return (void)call.fail_size("k", k);
+  if ((true) && A.strides[1] != 1) return (void)call.fail_stride("A", A.strides[1]);
+  if ((true) && n > 1 && A.strides[0] < k + 1) return (void)call.fail_ld("A", A.strides[0]);
+  call.window(0, A.data, 0, exoblas::span2_hi(n, k + 1, A.strides[0]), sizeof(double), false, true);
+  call.window(1, x.data, exoblas::span_lo(n, x.strides[0]), exoblas::span_hi(n, x.strides[0]), sizeof(double), false, true);
+  call.window(2, y.data, exoblas::span_lo(n, y.strides[0]), exoblas::span_hi(n, y.strides[0]), sizeof(double), true, true);
+  if (!call.stage()) return (void)call.status();
+  const double alpha_v = call.scalar(alpha);
+  if (!call.ok()) return (void)call.status();
+  call.run([&] { exoblas::ops::sbmv_row_major_Upper<double>(call.ctx(), (int64_t)n, (int64_t)k, alpha_v, call.dev<const double>(0), (int64_t)A.strides[0], call.dev<const double>(1), (int64_t)x.strides[0], call.dev<double>(2), (int64_t)y.strides[0]); });
+  return (void)call.finish();
+}
+extern "C" int exoblas_dsbmv_row_major_Upper_stride_any_async(hipStream_t stream, int_fast32_t n, int_fast32_t k, const double *alpha, struct exo_win_2f64c A, struct exo_win_1f64c x, struct exo_win_1f64 y) {
+  exoblas::Call call("exoblas_dsbmv_row_major_Upper_stride_any_async", exoblas::Call::kAsync, stream);
+  if (n < 0) return call.fail_size("n", n);
+  if (k < 0) return call.fail_size("k", k);
+  if ((true) && A.strides[1] != 1) return call.fail_stride("A", A.strides[1]);
+  if ((true) && n > 1 && A.strides[0] < k + 1) return call.fail_ld("A", A.strides[0]);
+  call.window(0, A.data, 0, exoblas::span2_hi(n, k + 1, A.strides[0]), sizeof(double), false, true);
+  call.window(1, x.data, exoblas::span_lo(n, x.strides[0]), exoblas::span_hi(n, x.strides[0]), sizeof(double), false, true);
+  call.window(2, y.data, exoblas::span_lo(n, y.strides[0]), exoblas::span_hi(n, y.strides[0]), sizeof(double), true, true);
+  if (!call.stage()) return call.status();
+  const double alpha_v = call.scalar(alpha);
+  if (!call.ok()) return call.status();
+  call.run([&] { exoblas::ops::sbmv_row_major_Upper<double>(call.ctx(), (int64_t)n, (int64_t)k, alpha_v, call.dev<const double>(0), (int64_t)A.strides[0], call.dev<const double>(1), (int64_t)x.strides[0], call.dev<double>(2), (int64_t)y.strides[0]); });
+  return call.finish();
+}
+
+// src/level2/sbmv.py:19-33
+extern "C" void exo_dsbmv_row_major_Upper_stride_1(void *ctxt, int_fast32_t n, int_fast32_t k, const double *alpha, struct exo_win_2f64c A, struct exo_win_1f64c x, struct exo_win_1f64 y) {
+  exoblas::Call call("exo_dsbmv_row_major_Upper_stride_1", exoblas::Call::kSync, ctxt);
+  if (n < 0) return (void)call.fail_size("n", n);
+  if (k < 0) return (void)call.fail_size("k", k);
+  if ((true) && A.strides[1] != 1) return (void)call.fail_stride("A", A.strides[1]);
+  if ((true) && n > 1 && A.strides[0] < k + 1) return (void)call.fail_ld("A", A.strides[0]);
+  call.window(0, A.data, 0, exoblas::span2_hi(n, k + 1, A.strides[0]), sizeof(double), false, true);
+  call.window(1, x.data, exoblas::span_lo(n, x.strides[0]), exoblas::span_hi(n, x.strides[0]), sizeof(double), false, true);
+  call.window(2, y.data, exoblas::span_lo(n, y.strides[0]), exoblas::span_hi(n, y.strides[0]), sizeof(double), true, true);
+  if (!call.stage()) return (void)call.status();
+  const double alpha_v = call.scalar(alpha);
+  if (!call.ok()) return (void)call.status();
+  call.run([&] { exoblas::ops::sbmv_row_major_Upper<double>(call.ctx(), (int64_t)n, (int64_t)k, alpha_v, call.dev<const double>(0), (int64_t)A.strides[0], call.dev<const double>(1), (int64_t)x.strides[0], call.dev<double>(2), (int64_t)y.strides[0]); });
+  return (void)call.finish();
+}
+extern "C" int exoblas_dsbmv_row_major_Upper_stride_1_async(hipStream_t stream, int_fast32_t n, int_fast32_t k, const double *alpha, struct exo_win_2f64c A, struct exo_win_1f64c x, struct exo_win_1f64 y) {
+  exoblas::Call call("exoblas_dsbmv_row_major_Upper_stride_1_async", exoblas::Call::kAsync, stream);
+  if (n < 0) return call.fail_size("n", n);
+  if (k < 0) return call.fail_size("k", k);
+  if ((true) && A.strides[1] != 1) return call.fail_stride("A", A.strides[1]);
+  if ((true) && n > 1 && A.strides[0] < k + 1) return call.fail_ld("A", A.strides[0]);
+  call.window(0, A.data, 0, exoblas::span2_hi(n, k + 1, A.strides[0]), sizeof(double), false, true);
+  call.window(1, x.data, exoblas::span_lo(n, x.strides[0]), exoblas::span_hi(n, x.strides[0]), sizeof(double), false, true);
+  call.window(2, y.data, exoblas::span_lo(n, y.strides[0]), exoblas::span_hi(n, y.strides[0]), sizeof(double), true, true);
+  if (!call.stage()) return call.status();
+  const double alpha_v = call.scalar(alpha);
+  if (!call.ok()) return call.status();
+  call.run([&] { exoblas::ops::sbmv_row_major_Upper<double>(call.ctx(), (int64_t)n, (int64_t)k, alpha_v, call.dev<const double>(0), (int64_t)A.strides[0], call.dev<const double>(1), (int64_t)x.strides[0], call.dev<double>(2), (int64_t)y.strides[0]); });
+  return call.finish();
+}
+
+// src/level2/sbmv.py:36-50
+extern "C" void exo_ssbmv_row_major_Lower_stride_any(void *ctxt, int_fast32_t n, int_fast32_t k, const float *alpha, struct exo_win_2f32c A, struct exo_win_1f32c x, struct exo_win_1f32 y) {
+  exoblas::Call call("exo_ssbmv_row_major_Lower_stride_any", exoblas::Call::kSync, ctxt);
+  if (n < 0) return (void)call.fail_size("n", n);
+  if (k < 0) return (void)call.fail_size("k", k);
+  if ((true) && A.strides[1] != 1) return (void)call.fail_stride("A", A.strides[1]);
+  if ((true) && n > 1 && A.strides[0] < k + 1) return (void)call.fail_ld("A", A.strides[0]);
+  call.window(0, A.data, 0, exoblas::span2_hi(n, k + 1, A.strides[0]), sizeof(float), false, true);
+  call.window(1, x.data, exoblas::span_lo(n, x.strides[0]), exoblas::span_hi(n, x.strides[0]), sizeof(float), false, true);
+  call.window(2, y.data, exoblas::span_lo(n, y.strides[0]), exoblas::span_hi(n, y.strides[0]), sizeof(float), true, true);
+  if (!call.stage()) return (void)call.status();
+  const float alpha_v = call.scalar(alpha);
+  if (!call.ok()) return (void)call.status();
+  call.run([&] { exoblas::ops::sbmv_row_major_Lower<float>(call.ctx(), (int64_t)n, (int64_t)k, alpha_v, call.dev<const float>(0), (int64_t)A.strides[0], call.dev<const float>(1), (int64_t)x.strides[0], call.dev<float>(2), (int64_t)y.strides[0]); });
+  return (void)call.finish();
+}
+extern "C" int exoblas_ssbmv_row_major_Lower_stride_any_async(hipStream_t stream, int_fast32_t n, int_fast32_t k, const float *alpha, struct exo_win_2f32c A, struct exo_win_1f32c x, struct exo_win_1f32 y) {
+  exoblas::Call call("exoblas_ssbmv_row_major_Lower_stride_any_async", exoblas::Call::kAsync, stream);
+  if (n < 0) return call.fail_size("n", n);
+  if (k < 0) return call.fail_size("k", k);
+  if ((true) && A.strides[1] != 1) return call.fail_stride("A", A.strides[1]);
+  if ((true) && n > 1 && A.strides[0] < k + 1) return call.fail_ld("A", A.strides[0]);
+  call.window(0, A.data, 0, exoblas::span2_hi(n, k + 1, A.strides[0]), sizeof(float), false, true);
+  call.window(1, x.data, exoblas::span_lo(n, x.strides[0]), exoblas::span_hi(n, x.strides[0]), sizeof(float), false, true);
+  call.window(2, y.data, exoblas::span_lo(n, y.strides[0]), exoblas::span_hi(n, y.strides[0]), sizeof(float), true, true);
+  if (!call.stage()) return call.status();
+  const float alpha_v = call.scalar(alpha);
+  if (!call.ok()) return call.status();
+  call.run([&] { exoblas::ops::sbmv_row_major_Lower<float>(call.ctx(), (int64_t)n, (int64_t)k, alpha_v, call.dev<const float>(0), (int64_t)A.strides[0], call.dev<const float>(1), (int64_t)x.strides[0], call.dev<float>(2), (int64_t)y.strides[0]); });
+  return call.finish();
+}
+
+// src/level2/sbmv.py:36-50
+extern "C" void exo_ssbmv_row_major_Lower_stride_1(void *ctxt, int_fast32_t n, int_fast32_t k, const float *alpha, struct exo_win_2f32c A, struct exo_win_1f32c x, struct exo_win_1f32 y) {
+  exoblas::Call call("exo_ssbmv_row_major_Lower_stride_1", exoblas::Call::kSync, ctxt);
+  if (n < 0) return (void)call.fail_size("n", n);
+  if (k < 0) return (void)call.fail_size("k", k);
+  if ((true) && A.strides[1] != 1) return (void)call.fail_stride("A", A.strides[1]);
+  if ((true) && n > 1 && A.strides[0] < k + 1) return (void)call.fail_ld("A", A.strides[0]);
+  call.window(0, A.data, 0, exoblas::span2_hi(n, k + 1, A.strides[0]), sizeof(float), false, true);
+  call.window(1, x.data, exoblas::span_lo(n, x.strides[0]), exoblas::span_hi(n, x.strides[0]), sizeof(float), false, true);
+  call.window(2, y.data, exoblas::span_lo(n, y.strides[0]), exoblas::span_hi(n, y.strides[0]), sizeof(float), true, true);
+  if (!call.stage()) return (void)call.status();
+  const float alpha_v = call.scalar(alpha);
+  if (!call.ok()) return (void)call.status();
+  call.run([&] { exoblas::ops::sbmv_row_major_Lower<float>(call.ctx(), (int64_t)n, (int64_t)k, alpha_v, call.dev<const float>(0), (int64_t)A.strides[0], call.dev<const float>(1), (int64_t)x.strides[0], call.dev<float>(2), (int64_t)y.strides[0]); });
+  return (void)call.finish();
+}
+extern "C" int exoblas_ssbmv_row_major_Lower_stride_1_async(hipStream_t stream, int_fast32_t n, int_fast32_t k, const float *alpha, struct exo_win_2f32c A, struct exo_win_1f32c x, struct exo_win_1f32 y) {
+  exoblas::Call call("exoblas_ssbmv_row_major_Lower_stride_1_async", exoblas::Call::kAsync, stream);
+  if (n < 0) return call.fail_size("n", n);
+  if (k < 0) return call.fail_size("k", k);
+  if ((true) && A.strides[1] != 1) return call.fail_stride("A", A.strides[1]);
+  if ((true) && n > 1 && A.strides[0] < k + 1) return call.fail_ld("A", A.strides[0]);
+  call.window(0, A.data, 0, exoblas::span2_hi(n, k + 1, A.strides[0]), sizeof(float), false, true);
+  call.window(1, x.data, exoblas::span_lo(n, x.strides[0]), exoblas::span_hi(n, x.strides[0]), sizeof(float), false, true);
+  call.window(2, y.data, exoblas::span_lo(n, y.strides[0]), exoblas::span_hi(n, y.strides[0]), sizeof(float), true, true);
+  if (!call.stage()) return call.status();
+  const float alpha_v = call.scalar(alpha);
+  if (!call.ok()) return call.status();
+  call.run([&] { exoblas::ops::sbmv_row_major_Lower<float>(call.ctx(), (int64_t)n, (int64_t)k, alpha_v, call.dev<const float>(0), (int64_t)A.strides[0], call.dev<const float>(1), (int64_t)x.strides[0], call.dev<float>(2), (int64_t)y.strides[0]); });
+  return call.finish();
+}
+
+// src/level2/sbmv.py:36-50
+extern "C" void exo_dsbmv_row_major_Lower_stride_any(void *ctxt, int_fast32_t n, int_fast32_t k, const double *alpha, struct exo_win_2f64c A, struct exo_win_1f64c x, struct exo_win_1f64 y) {
+  exoblas::Call call("exo_dsbmv_row_major_Lower_stride_any", exoblas::Call::kSync, ctxt);
+  if (n < 0) return (void)call.fail_size("n", n);
+  if (k < 0) return (void)call.fail_size("k", k);
+  if ((true) && A.strides[1] != 1) return (void)call.fail_stride("A", A.strides[1]);
+  if ((true) && n > 1 && A.strides[0] < k + 1) return (void)call.fail_ld("A", A.strides[0]);
+  call.window(0, A.data, 0, exoblas::span2_hi(n, k + 1, A.strides[0]), sizeof(double), false, true);
+  call.window(1, x.data, exoblas::span_lo(n, x.strides[0]), exoblas::span_hi(n, x.strides[0]), sizeof(double), false, true);
+  call.window(2, y.data, exoblas::span_lo(n, y.strides[0]), exoblas::span_hi(n, y.strides[0]), sizeof(double), true, true);
+  if (!call.stage()) return (void)call.status();
+  const double alpha_v = call.scalar(alpha);
+  if (!call.ok()) return (void)call.status();
+  call.run([&] { exoblas::ops::sbmv_row_major_Lower<double>(call.ctx(), (int64_t)n, (int64_t)k, alpha_v, call.dev<const double>(0), (int64_t)A.strides[0], call.dev<const double>(1), (int64_t)x.strides[0], call.dev<double>(2), (int64_t)y.strides[0]); });
+  return (void)call.finish();
+}
+extern "C" int exoblas_dsbmv_row_major_Lower_stride_any_async(hipStream_t stream, int_fast32_t n, int_fast32_t k, const double *alpha, struct exo_win_2f64c A, struct exo_win_1f64c x, struct exo_win_1f64 y) {
+  exoblas::Call call("exoblas_dsbmv_row_major_Lower_stride_any_async", exoblas::Call::kAsync, stream);
+  if (n < 0) return call.fail_size("n", n);
+  if (k < 0) return call.fail_size("k", k);
+  if ((true) && A.strides[1] != 1) return call.fail_stride("A", A.strides[1]);
+  if ((true) && n > 1 && A.strides[0] < k + 1) return call.fail_ld("A", A.strides[0]);
+  call.window(0, A.data, 0, exoblas::span2_hi(n, k + 1, A.strides[0]), sizeof(double), false, true);
+  call.window(1, x.data, exoblas::span_lo(n, x.strides[0]), exoblas::span_hi(n, x.strides[0]), sizeof(double), false, true);
+  call.window(2, y.data, exoblas::span_lo(n, y.strides[0]), exoblas::span_hi(n, y.strides[0]), sizeof(double), true, true);
+  if (!call.stage()) return call.status();
+  const double alpha_v = call.scalar(alpha);
+  if (!call.ok()) return call.status();
+  call.run([&] { exoblas::ops::sbmv_row_major_Lower<double>(call.ctx(), (int64_t)n, (int64_t)k, alpha_v, call.dev<const double>(0), (int64_t)A.strides[0], call.dev<const double>(1), (int64_t)x.strides[0], call.dev<double>(2), (int64_t)y.strides[0]); });
+  return call.finish();
+}
+
+// src/level2/sbmv.py:36-50
+extern "C" void exo_dsbmv_row_major_Lower_stride_1(void *ctxt, int_fast32_t n, int_fast32_t k, const double *alpha, struct exo_win_2f64c A, struct exo_win_1f64c x, struct exo_win_1f64 y) {
+  exoblas::Call call("exo_dsbmv_row_major_Lower_stride_1", exoblas::Call::kSync, ctxt);
+  if (n < 0) return (void)call.fail_size("n", n);
+  if (k < 0) return (void)call.fail_size("k", k);
+  if ((true) && A.strides[1] != 1) return (void)call.fail_stride("A", A.strides[1]);
+  if ((true) && n > 1 && A.strides[0] < k + 1) return (void)call.fail_ld("A", A.strides[0]);
+  call.window(0, A.data, 0, exoblas::span2_hi(n, k + 1, A.strides[0]), sizeof(double), false, true);
+  call.window(1, x.data, exoblas::span_lo(n, x.strides[0]), exoblas::span_hi(n, x.strides[0]), sizeof(double), false, true);
+  call.window(2, y.data, exoblas::span_lo(n, y.strides[0]), exoblas::span_hi(n, y.strides[0]), sizeof(double), true, true);
+  if (!call.stage()) return (void)call.status();
+  const double alpha_v = call.scalar(alpha);
+  if (!call.ok()) return (void)call.status();
+  call.run([&] { exoblas::ops::sbmv_row_major_Lower<double>(call.ctx(), (int64_t)n, (int64_t)k, alpha_v, call.dev<const double>(0), (int64_t)A.strides[0], call.dev<const double>(1), (int64_t)x.strides[0], call.dev<double>(2), (int64_t)y.strides[0]); });
+  return (void)call.finish();
+}
+extern "C" int exoblas_dsbmv_row_major_Lower_stride_1_async(hipStream_t stream, int_fast32_t n, int_fast32_t k, const double *alpha, struct exo_win_2f64c A, struct exo_win_1f64c x, struct exo_win_1f64 y) {
+  exoblas::Call call("exoblas_dsbmv_row_major_Lower_stride_1_async", exoblas::Call::kAsync, stream);
+  if (n < 0) return call.fail_size("n", n);
+  if (k < 0) return call.fail_size("k", k);
+  if ((true) && A.strides[1] != 1) return call.fail_stride("A", A.strides[1]);
+  if ((true) && n > 1 && A.strides[0] < k + 1) return call.fail_ld("A", A.strides[0]);
+  call.window(0, A.data, 0, exoblas::span2_hi(n, k + 1, A.strides[0]), sizeof(double), false, true);
+  call.window(1, x.data, exoblas::span_lo(n, x.strides[0]), exoblas::span_hi(n, x.strides[0]), sizeof(double), false, true);
+  call.window(2, y.data, exoblas::span_lo(n, y.strides[0]), exoblas::span_hi(n, y.strides[0]), sizeof(double), true, true);
+  if (!call.stage()) return call.status();
+  const double alpha_v = call.scalar(alpha);
+  if (!call.ok()) return call.status();
+  call.run([&] { exoblas::ops::sbmv_row_major_Lower<double>(call.ctx(), (int64_t)n, (int64_t)k, alpha_v, call.dev<const double>(0), (int64_t)A.strides[0], call.dev<const double>(1), (int64_t)x.strides[0], call.dev<double>(2), (int64_t)y.strides[0]); });
+  return call.finish();
+}
+
+// src/level2/tbmv.py:12-43
+extern "C" void exo_stbmv_row_major_Upper_NonTrans_stride_any(void *ctxt, int_fast32_t n, int_fast32_t k, struct exo_win_1f32 x, struct exo_win_2f32c A, int_fast32_t Diag) {
+  exoblas::Call call("exo_stbmv_row_major_Upper_NonTrans_stride_any", exoblas::Call::kSync, ctxt);
+  if (n < 0) return (void)call.fail_size("n", n);
+  if (k < 0) return (void)call.fail_size("k", k);
+  if (Diag < 0) return (void)call.fail_size("Diag", Diag);
+  if ((true) && A.strides[1] != 1) return (void)call.fail_stride("A", A.strides[1]);
+  if ((true) && n > 1 && A.strides[0] < k + 1) return (void)call.fail_ld("A", A.strides[0]);
+  call.window(0, x.data, exoblas::span_lo(n, x.strides[0]), exoblas::span_hi(n, x.strides[0]), sizeof(float), true, true);
+  call.window(1, A.data, 0, exoblas::span2_hi(n, k + 1, A.strides[0]), sizeof(float), false, true);
+  if (!call.stage()) return (void)call.status();
+  if (!call.ok()) return (void)call.status();
+  call.run([&] { exoblas::ops::tbmv_row_major_Upper_NonTrans<float>(call.ctx(), (int64_t)n, (int64_t)k, call.dev<float>(0), (int64_t)x.strides[0], call.dev<const float>(1), (int64_t)A.strides[0], (int64_t)Diag); });
+  return (void)call.finish();
+}
+extern "C" int exoblas_stbmv_row_major_Upper_NonTrans_stride_any_async(hipStream_t stream, int_fast32_t n, int_fast32_t k, struct exo_win_1f32 x, struct exo_win_2f32c A, int_fast32_t Diag) {
+  exoblas::Call call("exoblas_stbmv_row_major_Upper_NonTrans_stride_any_async", exoblas::Call::kAsync, stream);
+  if (n < 0) return call.fail_size("n", n);
+  if (k < 0) return call.fail_size("k", k);
+  if (Diag < 0) return call.fail_size("Diag", Diag);
+  if ((true) && A.strides[1] != 1) return call.fail_stride("A", A.strides[1]);
+  if ((true) && n > 1 && A.strides[0] < k + 1) return call.fail_ld("A", A.strides[0]);
+  call.window(0, x.data, exoblas::span_lo(n, x.strides[0]), exoblas::span_hi(n, x.strides[0]), sizeof(float), true, true);
+  call.window(1, A.data, 0, exoblas::span2_hi(n, k + 1, A.strides[0]), sizeof(float), false, true);
+  if (!call.stage()) return call.status();
+  if (!call.ok()) return call.status();
+  call.run([&] { exoblas::ops::tbmv_row_major_Upper_NonTrans<float>(call.ctx(), (int64_t)n, (int64_t)k, call.dev<float>(0), (int64_t)x.strides[0], call.dev<const float>(1), (int64_t)A.strides[0], (int64_t)Diag); });
+  return call.finish();
+}
+
+// src/level2/tbmv.py:12-43
+extern "C" void exo_stbmv_row_major_Upper_NonTrans_stride_1(void *ctxt, int_fast32_t n, int_fast32_t k, struct exo_win_1f32 x, struct exo_win_2f32c A, int_fast32_t Diag) {
+  exoblas::Call call("exo_stbmv_row_major_Upper_NonTrans_stride_1", exoblas::Call::kSync, ctxt);
+  if (n < 0) return (void)call.fail_size("n", n);
+  if (k < 0) return (void)call.fail_size("k", k);
+  if (Diag < 0) return (void)call.fail_size("Diag", Diag);
+  if ((true) && A.strides[1] != 1) return (void)call.fail_stride("A", A.strides[1]);
+  if ((true) && n > 1 && A.strides[0] < k + 1) return (void)call.fail_ld("A", A.strides[0]);
+  call.window(0, x.data, exoblas::span_lo(n, x.strides[0]), exoblas::span_hi(n, x.strides[0]), sizeof(float), true, true);
+  call.window(1, A.data, 0, exoblas::span2_hi(n, k + 1, A.strides[0]), sizeof(float), false, true);
+  if (!call.stage()) return (void)call.status();
+  if (!call.ok()) return (void)call.status();
+  call.run([&] { exoblas::ops::tbmv_row_major_Upper_NonTrans<float>(call.ctx(), (int64_t)n, (int64_t)k, call.dev<float>(0), (int64_t)x.strides[0], call.dev<const float>(1), (int64_t)A.strides[0], (int64_t)Diag); });
+  return (void)call.finish();
+}
+extern "C" int exoblas_stbmv_row_major_Upper_NonTrans_stride_1_async(hipStream_t stream, int_fast32_t n, int_fast32_t k, struct exo_win_1f32 x, struct exo_win_2f32c A, int_fast32_t Diag) {
+  exoblas::Call call("exoblas_stbmv_row_major_Upper_NonTrans_stride_1_async", exoblas::Call::kAsync, stream);
+  if (n < 0) return call.fail_size("n", n);
+  if (k < 0) return call.fail_size("k", k);
+  if (Diag < 0) return call.fail_size("Diag", Diag);
+  if ((true) && A.strides[1] != 1) return call.fail_stride("A", A.strides[1]);
+  if ((true) && n > 1 && A.strides[0] < k + 1) return call.fail_ld("A", A.strides[0]);
+  call.window(0, x.data, exoblas::span_lo(n, x.strides[0]), exoblas::span_hi(n, x.strides[0]), sizeof(float), true, true);
+  call.window(1, A.data, 0, exoblas::span2_hi(n, k + 1, A.strides[0]), sizeof(float), false, true);
+  if (!call.stage()) return call.status();
+  if (!call.ok()) return call.status();
+  call.run([&] { exoblas::ops::tbmv_row_major_Upper_NonTrans<float>(call.ctx(), (int64_t)n, (int64_t)k, call.dev<float>(0), (int64_t)x.strides[0], call.dev<const float>(1), (int64_t)A.strides[0], (int64_t)Diag); });
+  return call.finish();
+}
+
+// src/level2/tbmv.py:12-43
+extern "C" void exo_dtbmv_row_major_Upper_NonTrans_stride_any(void *ctxt, int_fast32_t n, int_fast32_t k, struct exo_win_1f64 x, struct exo_win_2f64c A, int_fast32_t Diag) {
+  exoblas::Call call("exo_dtbmv_row_major_Upper_NonTrans_stride_any", exoblas::Call::kSync, ctxt);
+  if (n < 0) return (void)call.fail_size("n", n);
+  if (k < 0) return (void)call.fail_size("k", k);
+  if (Diag < 0) return (void)call.fail_size("Diag", Diag);
+  if ((true) && A.strides[1] != 1) return (void)call.fail_stride("A", A.strides[1]);
+  if ((true) && n > 1 && A.strides[0] < k + 1) return (void)call.fail_ld("A", A.strides[0]);
+  call.window(0, x.data, exoblas::span_lo(n, x.strides[0]), exoblas::span_hi(n, x.strides[0]), sizeof(double), true, true);
+  call.window(1, A.data, 0, exoblas::span2_hi(n, k + 1, A.strides[0]), sizeof(double), false, true);
+  if (!call.stage()) return (void)call.status();
+  if (!call.ok()) return (void)call.status();
+  call.run([&] { exoblas::ops::tbmv_row_major_Upper_NonTrans<double>(call.ctx(), (int64_t)n, (int64_t)k, call.dev<double>(0), (int64_t)x.strides[0], call.dev<const double>(1), (int64_t)A.strides[0], (int64_t)Diag); });
+  return (void)call.finish();
+}
+extern "C" int exoblas_dtbmv_row_major_Upper_NonTrans_stride_any_async(hipStream_t stream, int_fast32_t n, int_fast32_t k, struct exo_win_1f64 x, struct exo_win_2f64c A, int_fast32_t Diag) {
+  exoblas::Call call("exoblas_dtbmv_row_major_Upper_NonTrans_stride_any_async", exoblas::Call::kAsync, stream);
+  if (n < 0) return call.fail_size("n", n);
+  if (k < 0) return call.fail_size("k", k);
+  if (Diag < 0) return call.fail_size("Diag", Diag);
+  if ((true) && A.strides[1] != 1) return call.fail_stride("A", A.strides[1]);
+  if ((true) && n > 1 && A.strides[0] < k + 1) return call.fail_ld("A", A.strides[0]);
+  call.window(0, x.data, exoblas::span_lo(n, x.strides[0]), exoblas::span_hi(n, x.strides[0]), sizeof(double), true, true);
+  call.window(1, A.data, 0, exoblas::span2_hi(n, k + 1, A.strides[0]), sizeof(double), false, true);
+  if (!call.stage()) return call.status();
+  if (!call.ok()) return call.status();
+  call.run([&] { exoblas::ops::tbmv_row_major_Upper_NonTrans<double>(call.ctx(), (int64_t)n, (int64_t)k, call.dev<double>(0), (int64_t)x.strides[0], call.dev<const double>(1), (int64_t)A.strides[0], (int64_t)Diag); });
+  return call.finish();
+}
+
+// src/level2/tbmv.py:12-43
+extern "C" void exo_dtbmv_row_major_Upper_NonTrans_stride_1(void *ctxt, int_fast32_t n, int_fast32_t k, struct exo_win_1f64 x, struct exo_win_2f64c A, int_fast32_t Diag) {
+  exoblas::Call call("exo_dtbmv_row_major_Upper_NonTrans_stride_1", exoblas::Call::kSync, ctxt);
+  if (n < 0) return (void)call.fail_size("n", n);
+  if (k < 0) return (void)call.fail_size("k", k);
+  if (Diag < 0) return (void)call.fail_size("Diag", Diag);
+  if ((true) && A.strides[1] != 1) return (void)call.fail_stride("A", A.strides[1]);
+  if ((true) && n > 1 && A.strides[0] < k + 1) return (void)call.fail_ld("A", A.strides[0]);
+  call.window(0, x.data, exoblas::span_lo(n, x.strides[0]), exoblas::span_hi(n, x.strides[0]), sizeof(double), true, true);
+  call.window(1, A.data, 0, exoblas::span2_hi(n, k + 1, A.strides[0]), sizeof(double), false, true);
+  if (!call.stage()) return (void)call.status();
+  if (!call.ok()) return (void)call.status();
+  call.run([&] { exoblas::ops::tbmv_row_major_Upper_NonTrans<double>(call.ctx(), (int64_t)n, (int64_t)k, call.dev<double>(0), (int64_t)x.strides[0], call.dev<const double>(1), (int64_t)A.strides[0], (int64_t)Diag); });
+  return (void)call.finish();
+}
+extern "C" int exoblas_dtbmv_row_major_Upper_NonTrans_stride_1_async(hipStream_t stream, int_fast32_t n, int_fast32_t k, struct exo_win_1f64 x, struct exo_win_2f64c A, int_fast32_t Diag) {
+  exoblas::Call call("exoblas_dtbmv_row_major_Upper_NonTrans_stride_1_async", exoblas::Call::kAsync, stream);
+  if (n < 0) return call.fail_size("n", n);
+  if (k < 0) return call.fail_size("k", k);
+  if (Diag < 0) return call.fail_size("Diag", Diag);
+  if ((true) && A.strides[1] != 1) return call.fail_stride("A", A.strides[1]);
+  if ((true) && n > 1 && A.strides[0] < k + 1) return call.fail_ld("A", A.strides[0]);
+  call.window(0, x.data, exoblas::span_lo(n, x.strides[0]), exoblas::span_hi(n, x.strides[0]), sizeof(double), true, true);
+  call.window(1, A.data, 0, exoblas::span2_hi(n, k + 1, A.strides[0]), sizeof(double), false, true);
+  if (!call.stage()) return call.status();
+  if (!call.ok()) return call.status();
+  call.run([&] { exoblas::ops::tbmv_row_major_Upper_NonTrans<double>(call.ctx(), (int64_t)n, (int64_t)k, call.dev<double>(0), (int64_t)x.strides[0], call.dev<const double>(1), (int64_t)A.strides[0], (int64_t)Diag); });
+  return call.finish();
+}
+
+// src/level2/tbmv.py:46-77
+extern "C" void exo_stbmv_row_major_Lower_NonTrans_stride_any(void *ctxt, int_fast32_t n, int_fast32_t k, struct exo_win_1f32 x, struct exo_win_2f32c A, int_fast32_t Diag) {
+  exoblas::Call call("exo_stbmv_row_major_Lower_NonTrans_stride_any", exoblas::Call::kSync, ctxt);
+  if (n < 0) return (void)call.fail_size("n", n);
+  if (k < 0) return (void)call.fail_size("k", k);
+  if (Diag < 0) return (void)call.fail_size("Diag", Diag);
+  if ((true) && A.strides[1] != 1) return (void)call.fail_stride("A", A.strides[1]);
+  if ((true) && n > 1 && A.strides[0] < k + 1) return (void)call.fail_ld("A", A.strides[0]);
+  call.window(0, x.data, exoblas::span_lo(n, x.strides[0]), exoblas::span_hi(n, x.strides[0]), sizeof(float), true, true);
+  call.window(1, A.data, 0, exoblas::span2_hi(n, k + 1, A.strides[0]), sizeof(float), false, true);
+  if (!call.stage()) return (void)call.status();
+  if (!call.ok()) return (void)call.status();
+  call.run([&] { exoblas::ops::tbmv_row_major_Lower_NonTrans<float>(call.ctx(), (int64_t)n, (int64_t)k, call.dev<float>(0), (int64_t)x.strides[0], call.dev<const float>(1), (int64_t)A.strides[0], (int64_t)Diag); });
+  return (void)call.finish();
+}
+extern "C" int exoblas_stbmv_row_major_Lower_NonTrans_stride_any_async(hipStream_t stream, int_fast32_t n, int_fast32_t k, struct exo_win_1f32 x, struct exo_win_2f32c A, int_fast32_t Diag) {
+  exoblas::Call call("exoblas_stbmv_row_major_Lower_NonTrans_stride_any_async", exoblas::Call::kAsync, stream);
+  if (n < 0) return call.fail_size("n", n);
+  if (k < 0) return call.fail_size("k", k);
+  if (Diag < 0) return call.fail_size("Diag", Diag);
+  if ((true) && A.strides[1] != 1) return call.fail_stride("A", A.strides[1]);
+  if ((true) && n > 1 && A.strides[0] < k + 1) return call.fail_ld("A", A.strides[0]);
+  call.window(0, x.data, exoblas::span_lo(n, x.strides[0]), exoblas::span_hi(n, x.strides[0]), sizeof(float), true, true);
+  call.window(1, A.data, 0, exoblas::span2_hi(n, k + 1, A.strides[0]), sizeof(float), false, true);
+  if (!call.stage()) return call.status();
+  if (!call.ok()) return call.status();
+  call.run([&] { exoblas::ops::tbmv_row_major_Lower_NonTrans<float>(call.ctx(), (int64_t)n, (int64_t)k, call.dev<float>(0), (int64_t)x.strides[0], call.dev<const float>(1), (int64_t)A.strides[0], (int64_t)Diag); });
+  return call.finish();
+}
+
+// src/level2/tbmv.py:46-77
+extern "C" void exo_stbmv_row_major_Lower_NonTrans_stride_1(void *ctxt, int_fast32_t n, int_fast32_t k, struct exo_win_1f32 x, struct exo_win_2f32c A, int_fast32_t Diag) {
+  exoblas::Call call("exo_stbmv_row_major_Lower_NonTrans_stride_1", exoblas::Call::kSync, ctxt);
+  if (n < 0) return (void)call.fail_size("n", n);
+  if (k < 0) return (void)call.fail_size("k", k);
+  if (Diag < 0) return (void)call.fail_size("Diag", Diag);
+  if ((true) && A.strides[1] != 1) return (void)call.fail_stride("A", A.strides[1]);
+  if ((true) && n > 1 && A.strides[0] < k + 1) return (void)call.fail_ld("A", A.strides[0]);
+  call.window(0, x.data, exoblas::span_lo(n, x.strides[0]), exoblas::span_hi(n, x.strides[0]), sizeof(float), true, true);
+  call.window(1, A.data, 0, exoblas::span2_hi(n, k + 1, A.strides[0]), sizeof(float), false, true);
+  if (!call.stage()) return (void)call.status();
+  if (!call.ok()) return (void)call.status();
+  call.run([&] { exoblas::ops::tbmv_row_major_Lower_NonTrans<float>(call.ctx(), (int64_t)n, (int64_t)k, call.dev<float>(0), (int64_t)x.strides[0], call.dev<const float>(1), (int64_t)A.strides[0], (int64_t)Diag); });
+  return (void)call.finish();
+}
+extern "C" int exoblas_stbmv_row_major_Lower_NonTrans_stride_1_async(hipStream_t stream, int_fast32_t n, int_fast32_t k, struct exo_win_1f32 x, struct exo_win_2f32c A, int_fast32_t Diag) {
+  exoblas::Call call("exoblas_stbmv_row_major_Lower_NonTrans_stride_1_async", exoblas::Call::kAsync, stream);
+  if (n < 0) return call.fail_size("n", n);
+  if (k < 0) return call.fail_size("k", k);
+  if (Diag < 0) return call.fail_size("Diag", Diag);
+  if ((true) && A.strides[1] != 1) return call.fail_stride("A", A.strides[1]);
+  if ((true) && n > 1 && A.strides[0] < k + 1) return call.fail_ld("A", A.strides[0]);
+  call.window(0, x.data, exoblas::span_lo(n, x.strides[0]), exoblas::span_hi(n, x.strides[0]), sizeof(float), true, true);
+  call.window(1, A.data, 0, exoblas::span2_hi(n, k + 1, A.strides[0]), sizeof(float), false, true);
+  if (!call.stage()) return call.status();
+  if (!call.ok()) return call.status();
+  call.run([&] { exoblas::ops::tbmv_row_major_Lower_NonTrans<float>(call.ctx(), (int64_t)n, (int64_t)k, call.dev<float>(0), (int64_t)x.strides[0], call.dev<const float>(1), (int64_t)A.strides[0], (int64_t)Diag); });
+  return call.finish();
+}
+
+// src/level2/tbmv.py:46-77
+extern "C" void exo_dtbmv_row_major_Lower_NonTrans_stride_any(void *ctxt, int_fast32_t n, int_fast32_t k, struct exo_win_1f64 x, struct exo_win_2f64c A, int_fast32_t Diag) {
+  exoblas::Call call("exo_dtbmv_row_major_Lower_NonTrans_stride_any", exoblas::Call::kSync, ctxt);
+  if (n < 0) return (void)call.fail_size("n", n);
+  if (k < 0) return (void)call.fail_size("k", k);
+  if (Diag < 0) return (void)call.fail_size("Diag", Diag);
+  if ((true) && A.strides[1] != 1) return (void)call.fail_stride("A", A.strides[1]);
+  if ((true) && n > 1 && A.strides[0] < k + 1) return (void)call.fail_ld("A", A.strides[0]);
+  call.window(0, x.data, exoblas::span_lo(n, x.strides[0]), exoblas::span_hi(n, x.strides[0]), sizeof(double), true, true);
+  call.window(1, A.data, 0, exoblas::span2_hi(n, k + 1, A.strides[0]), sizeof(double), false, true);
+  if (!call.stage()) return (void)call.status();
+  if (!call.ok()) return (void)call.status();
+  call.run([&] { exoblas::ops::tbmv_row_major_Lower_NonTrans<double>(call.ctx(), (int64_t)n, (int64_t)k, call.dev<double>(0), (int64_t)x.strides[0], call.dev<const double>(1), (int64_t)A.strides[0], (int64_t)Diag); });
+  return (void)call.finish();
+}
+extern "C" int exoblas_dtbmv_row_major_Lower_NonTrans_stride_any_async(hipStream_t stream, int_fast32_t n, int_fast32_t k, struct exo_win_1f64 x, struct exo_win_2f64c A, int_fast32_t Diag) {
+  exoblas::Call call("exoblas_dtbmv_row_major_Lower_NonTrans_stride_any_async", exoblas::Call::kAsync, stream);
+  if (n < 0) return call.fail_size("n", n);
+  if (k < 0) return call.fail_size("k", k);
+  if (Diag < 0) return call.fail_size("Diag", Diag);
+  if ((true) && A.strides[1] != 1) return call.fail_stride("A", A.strides[1]);
+  if ((true) && n > 1 && A.strides[0] < k + 1) return call.fail_ld("A", A.strides[0]);
+  call.window(0, x.data, exoblas::span_lo(n, x.strides[0]), exoblas::span_hi(n, x.strides[0]), sizeof(double), true, true);
+  call.window(1, A.data, 0, exoblas::span2_hi(n, k + 1, A.strides[0]), sizeof(double), false, true);
+  if (!call.stage()) return call.status();
+  if (!call.ok()) return call.status();
+  call.run([&] { exoblas::ops::tbmv_row_major_Lower_NonTrans<double>(call.ctx(), (int64_t)n, (int64_t)k, call.dev<double>(0), (int64_t)x.strides[0], call.dev<const double>(1), (int64_t)A.strides[0], (int64_t)Diag); });
+  return call.finish();
+}
+
+// src/level2/tbmv.py:46-77
+extern "C" void exo_dtbmv_row_major_Lower_NonTrans_stride_1(void *ctxt, int_fast32_t n, int_fast32_t k, struct exo_win_1f64 x, struct exo_win_2f64c A, int_fast32_t Diag) {
+  exoblas::Call call("exo_dtbmv_row_major_Lower_NonTrans_stride_1", exoblas::Call::kSync, ctxt);
+  if (n < 0) return (void)call.fail_size("n", n);
+  if (k < 0) return (void)call.fail_size("k", k);
+  if (Diag < 0) return (void)call.fail_size("Diag", Diag);
+  if ((true) && A.strides[1] != 1) return (void)call.fail_stride("A", A.strides[1]);
+  if ((true) && n > 1 && A.strides[0] < k + 1) return (void)call.fail_ld("A", A.strides[0]);
+  call.window(0, x.data, exoblas::span_lo(n, x.strides[0]), exoblas::span_hi(n, x.strides[0]), sizeof(double), true, true);
+  call.window(1, A.data, 0, exoblas::span2_hi(n, k + 1, A.strides[0]), sizeof(double), false, true);
+  if (!call.stage()) return (void)call.status();
+  if (!call.ok()) return (void)call.status();
+  call.run([&] { exoblas::ops::tbmv_row_major_Lower_NonTrans<double>(call.ctx(), (int64_t)n, (int64_t)k, call.dev<double>(0), (int64_t)x.strides[0], call.dev<const double>(1), (int64_t)A.strides[0], (int64_t)Diag); });
+  return (void)call.finish();
+}
+extern "C" int exoblas_dtbmv_row_major_Lower_NonTrans_stride_1_async(hipStream_t stream, int_fast32_t n, int_fast32_t k, struct exo_win_1f64 x, struct exo_win_2f64c A, int_fast32_t Diag) {
+  exoblas::Call call("exoblas_dtbmv_row_major_Lower_NonTrans_stride_1_async", exoblas::Call::kAsync, stream);
+  if (n < 0) return call.fail_size("n", n);
+  if (k < 0) return call.fail_size("k", k);
+  if (Diag < 0) return call.fail_size("Diag", Diag);
+  if ((true) && A.strides[1] != 1) return call.fail_stride("A", A.strides[1]);
+  if ((true) && n > 1 && A.strides[0] < k + 1) return call.fail_ld("A", A.strides[0]);
+  call.window(0, x.data, exoblas::span_lo(n, x.strides[0]), exoblas::span_hi(n, x.strides[0]), sizeof(double), true, true);
+  call.window(1, A.data, 0, exoblas::span2_hi(n, k + 1, A.strides[0]), sizeof(double), false, true);
+  if (!call.stage()) return call.status();
+  if (!call.ok()) return call.status();
+  call.run([&] { exoblas::ops::tbmv_row_major_Lower_NonTrans<double>(call.ctx(), (int64_t)n, (int64_t)k, call.dev<double>(0), (int64_t)x.strides[0], call.dev<const double>(1), (int64_t)A.strides[0], (int64_t)Diag); });
+  return call.finish();
+}
+
+// src/level2/tbmv.py:80-108
+extern "C" void exo_stbmv_row_major_Upper_Trans_stride_any(void *ctxt, int_fast32_t n, int_fast32_t k, struct exo_win_1f32 x, struct exo_win_2f32c A, int_fast32_t Diag) {
+  exoblas::Call call("exo_stbmv_row_major_Upper_Trans_stride_any", exoblas::Call::kSync, ctxt);
+  if (n < 0) return (void)call.fail_size("n", n);
+  if (k < 0) return (void)call.fail_size("k", k);
+  if (Diag < 0) return (void)call.fail_size("Diag", Diag);
+  if ((true) && A.strides[1] != 1) return (void)call.fail_stride("A", A.strides[1]);
+  if ((true) && n > 1 && A.strides[0] < k + 1) return (void)call.fail_ld("A", A.strides[0]);
+  call.window(0, x.data, exoblas::span_lo(n, x.strides[0]), exoblas::span_hi(n, x.strides[0]), sizeof(float), true, true);
+  call.window(1, A.data, 0, exoblas::span2_hi(n, k + 1, A.strides[0]), sizeof(float), false, true);
+  if (!call.stage()) return (void)call.status();
+  if (!call.ok()) return (void)call.status();
+  call.run([&] { exoblas::ops::tbmv_row_major_Upper_Trans<float>(call.ctx(), (int64_t)n, (int64_t)k, call.dev<float>(0), (int64_t)x.strides[0], call.dev<const float>(1), (int64_t)A.strides[0], (int64_t)Diag); });
+  return (void)call.finish();
+}
+extern "C" int exoblas_stbmv_row_major_Upper_Trans_stride_any_async(hipStream_t stream, int_fast32_t n, int_fast32_t k, struct exo_win_1f32 x, struct exo_win_2f32c A, int_fast32_t Diag) {
+  exoblas::Call call("exoblas_stbmv_row_major_Upper_Trans_stride_any_async", exoblas::Call::kAsync, stream);
+  if (n < 0) return call.fail_size("n", n);
+  if (k < 0) return call.fail_size("k", k);
+  if (Diag < 0) return call.fail_size("Diag", Diag);
+  if ((true) && A.strides[1] != 1) return call.fail_stride("A", A.strides[1]);
+  if ((true) && n > 1 && A.strides[0] < k + 1) return call.fail_ld("A", A.strides[0]);
+  call.window(0, x.data, exoblas::span_lo(n, x.strides[0]), exoblas::span_hi(n, x.strides[0]), sizeof(float), true, true);
+  call.window(1, A.data, 0, exoblas::span2_hi(n, k + 1, A.strides[0]), sizeof(float), false, true);
+  if (!call.stage()) return call.status();
+  if (!call.ok()) return call.status();
+  call.run([&] { exoblas::ops::tbmv_row_major_Upper_Trans<float>(call.ctx(), (int64_t)n, (int64_t)k, call.dev<float>(0), (int64_t)x.strides[0], call.dev<const float>(1), (int64_t)A.strides[0], (int64_t)Diag); });
+  return call.finish();
+}
+
+// src/level2/tbmv.py:80-108
+extern "C" void exo_stbmv_row_major_Upper_Trans_stride_1(void *ctxt, int_fast32_t n, int_fast32_t k, struct exo_win_1f32 x, struct exo_win_2f32c A, int_fast32_t Diag) {
+  exoblas::Call call("exo_stbmv_row_major_Upper_Trans_stride_1", exoblas::Call::kSync, ctxt);
+  if (n < 0) return (void)call.fail_size("n", n);
+  if (k < 0) return (void)call.fail_size("k", k);
+  if (Diag < 0) return (void)call.fail_size("Diag", Diag);
+  if ((true) && A.strides[1] != 1) return (void)call.fail_stride("A", A.strides[1]);
+  if ((true) && n > 1 && A.strides[0] < k + 1) return (void)call.fail_ld("A", A.strides[0]);
+  call.window(0, x.data, exoblas::span_lo(n, x.strides[0]), exoblas::span_hi(n, x.strides[0]), sizeof(float), true, true);
+  call.window(1, A.data, 0, exoblas::span2_hi(n, k + 1, A.strides[0]), sizeof(float), false, true);
+  if (!call.stage()) return (void)call.status();
+  if (!call.ok()) return (void)call.status();
+  call.run([&] { exoblas::ops::tbmv_row_major_Upper_Trans<float>(call.ctx(), (int64_t)n, (int64_t)k, call.dev<float>(0), (int64_t)x.strides[0], call.dev<const float>(1), (int64_t)A.strides[0], (int64_t)Diag); });
+  return (void)call.finish();
+}
+extern "C" int exoblas_stbmv_row_major_Upper_Trans_stride_1_async(hipStream_t stream, int_fast32_t n, int_fast32_t k, struct exo_win_1f32 x, struct exo_win_2f32c A, int_fast32_t Diag) {
+  exoblas::Call call("exoblas_stbmv_row_major_Upper_Trans_stride_1_async", exoblas::Call::kAsync, stream);
+  if (n < 0) return call.fail_size("n", n);
+  if (k < 0) return call.fail_size("k", k);
+  if (Diag < 0) return call.fail_size("Diag", Diag);
+  if ((true) && A.strides[1] != 1) return call.fail_stride("A", A.strides[1]);
+  if ((true) && n > 1 && A.strides[0] < k + 1) return call.fail_ld("A", A.strides[0]);
+  call.window(0, x.data, exoblas::span_lo(n, x.strides[0]), exoblas::span_hi(n, x.strides[0]), sizeof(float), true, true);
+  call.window(1, A.data, 0, exoblas::span2_hi(n, k + 1, A.strides[0]), sizeof(float), false, true);
+  if (!call.stage()) return call.status();
+  if (!call.ok()) return call.status();
+  call.run([&] { exoblas::ops::tbmv_row_major_Upper_Trans<float>(call.ctx(), (int64_t)n, (int64_t)k, call.dev<float>(0), (int64_t)x.strides[0], call.dev<const float>(1), (int64_t)A.strides[0], (int64_t)Diag); });
+  return call.finish();
+}
+
+// src/level2/tbmv.py:80-108
+extern "C" void exo_dtbmv_row_major_Upper_Trans_stride_any(void *ctxt, int_fast32_t n, int_fast32_t k, struct exo_win_1f64 x, struct exo_win_2f64c A, int_fast32_t Diag) {
+  exoblas::Call call("exo_dtbmv_row_major_Upper_Trans_stride_any", exoblas::Call::kSync, ctxt);
+  if (n < 0) return (void)call.fail_size("n", n);
+  if (k < 0) return (void)call.fail_size("k", k);
+  if (Diag < 0) return (void)call.fail_size("Diag", Diag);
+  if ((true) && A.strides[1] != 1) return (void)call.fail_stride("A", A.strides[1]);
+  if ((true) && n > 1 && A.strides[0] < k + 1) return (void)call.fail_ld("A", A.strides[0]);
+  call.window(0, x.data, exoblas::span_lo(n, x.strides[0]), exoblas::span_hi(n, x.strides[0]), sizeof(double), true, true);
+  call.window(1, A.data, 0, exoblas::span2_hi(n, k + 1, A.strides[0]), sizeof(double), false, true);
+  if (!call.stage()) return (void)call.status();
+  if (!call.ok()) return (void)call.status();
+  call.run([&] { exoblas::ops::tbmv_row_major_Upper_Trans<double>(call.ctx(), (int64_t)n, (int64_t)k, call.dev<double>(0), (int64_t)x.strides[0], call.dev<const double>(1), (int64_t)A.strides[0], (int64_t)Diag); });
+  return (void)call.finish();
+}
+extern "C" int exoblas_dtbmv_row_major_Upper_Trans_stride_any_async(hipStream_t stream, int_fast32_t n, int_fast32_t k, struct exo_win_1f64 x, struct exo_win_2f64c A, int_fast32_t Diag) {
+  exoblas::Call call("exoblas_dtbmv_row_major_Upper_Trans_stride_any_async", exoblas::Call::kAsync, stream);
+  if (n < 0) return call.fail_size("n", n);
+  if (k < 0) return call.fail_size("k", k);
+  if (Diag < 0) return call.fail_size("Diag", Diag);
+  if ((true) && A.strides[1] != 1) return call.fail_stride("A", A.strides[1]);
+  if ((true) && n > 1 && A.strides[0] < k + 1) return call.fail_ld("A", A.strides[0]);
+  call.window(0, x.data, exoblas::span_lo(n, x.strides[0]), exoblas::span_hi(n, x.strides[0]), sizeof(double), true, true);
+  call.window(1, A.data, 0, exoblas::span2_hi(n, k + 1, A.strides[0]), sizeof(double), false, true);
+  if (!call.stage()) return call.status();
+  if (!call.ok()) return call.status();
+  call.run([&] { exoblas::ops::tbmv_row_major_Upper_Trans<double>(call.ctx(), (int64_t)n, (int64_t)k, call.dev<double>(0), (int64_t)x.strides[0], call.dev<const double>(1), (int64_t)A.strides[0], (int64_t)Diag); });
+  return call.finish();
+}
+
+// src/level2/tbmv.py:80-108
+extern "C" void exo_dtbmv_row_major_Upper_Trans_stride_1(void *ctxt, int_fast32_t n, int_fast32_t k, struct exo_win_1f64 x, struct exo_win_2f64c A, int_fast32_t Diag) {
+  exoblas::Call call("exo_dtbmv_row_major_Upper_Trans_stride_1", exoblas::Call::kSync, ctxt);
+  if (n < 0) return (void)call.fail_size("n", n);
+  if (k < 0) return (void)call.fail_size("k", k);
+  if (Diag < 0) return (void)call.fail_size("Diag", Diag);
+  if ((true) && A.strides[1] != 1) return (void)call.fail_stride("A", A.strides[1]);
+  if ((true) && n > 1 && A.strides[0] < k + 1) return (void)call.fail_ld("A", A.strides[0]);
+  call.window(0, x.data, exoblas::span_lo(n, x.strides[0]), exoblas::span_hi(n, x.strides[0]), sizeof(double), true, true);
+  call.window(1, A.data, 0, exoblas::span2_hi(n, k + 1, A.strides[0]), sizeof(double), false, true);
+  if (!call.stage()) return (void)call.status();
+  if (!call.ok()) return (void)call.status();
+  call.run([&] { exoblas::ops::tbmv_row_major_Upper_Trans<double>(call.ctx(), (int64_t)n, (int64_t)k, call.dev<double>(0), (int64_t)x.strides[0], call.dev<const double>(1), (int64_t)A.strides[0], (int64_t)Diag); });
+  return (void)call.finish();
+}
+extern "C" int exoblas_dtbmv_row_major_Upper_Trans_stride_1_async(hipStream_t stream, int_fast32_t n, int_fast32_t k, struct exo_win_1f64 x, struct exo_win_2f64c A, int_fast32_t Diag) {
+  exoblas::Call call("exoblas_dtbmv_row_major_Upper_Trans_stride_1_async", exoblas::Call::kAsync, stream);
+  if (n < 0) return call.fail_size("n", n);
+  if (k < 0) return call.fail_size("k", k);
+  if (Diag < 0) return call.fail_size("Diag", Diag);
+  if ((true) && A.strides[1] != 1) return call.fail_stride("A", A.strides[1]);
+  if ((true) && n > 1 && A.strides[0] < k + 1) return call.fail_ld("A", A.strides[0]);
+  call.window(0, x.data, exoblas::span_lo(n, x.strides[0]), exoblas::span_hi(n, x.strides[0]), sizeof(double), true, true);
+  call.window(1, A.data, 0, exoblas::span2_hi(n, k + 1, A.strides[0]), sizeof(double), false, true);
+  if (!call.stage()) return call.status();
+  if (!call.ok()) return call.status();
+  call.run([&] { exoblas::ops::tbmv_row_major_Upper_Trans<double>(call.ctx(), (int64_t)n, (int64_t)k, call.dev<double>(0), (int64_t)x.strides[0], call.dev<const double>(1), (int64_t)A.strides[0], (int64_t)Diag); });
+  return call.finish();
+}
+
+// src/level2/tbmv.py:111-141
+extern "C" void exo_stbmv_row_major_Lower_Trans_stride_any(void *ctxt, int_fast32_t n, int_fast32_t k, struct exo_win_1f32 x, struct exo_win_2f32c A, int_fast32_t Diag) {
+  exoblas::Call call("exo_stbmv_row_major_Lower_Trans_stride_any", exoblas::Call::kSync, ctxt);
+  if (n < 0) return (void)call.fail_size("n", n);
+  if (k < 0) return (void)call.fail_size("k", k);
+  if (Diag < 0) return (void)call.fail_size("Diag", Diag);
+  if ((true) && A.strides[1] != 1) return (void)call.fail_stride("A", A.strides[1]);
+  if ((true) && n > 1 && A.strides[0] < k + 1) return (void)call.fail_ld("A", A.strides[0]);
+  call.window(0, x.data, exoblas::span_lo(n, x.strides[0]), exoblas::span_hi(n, x.strides[0]), sizeof(float), true, true);
+  call.window(1, A.data, 0, exoblas::span2_hi(n, k + 1, A.strides[0]), sizeof(float), false, true);
+  if (!call.stage()) return (void)call.status();
+  if (!call.ok()) return (void)call.status();
+  call.run([&] { exoblas::ops::tbmv_row_major_Lower_Trans<float>(call.ctx(), (int64_t)n, (int64_t)k, call.dev<float>(0), (int64_t)x.strides[0], call.dev<const float>(1), (int64_t)A.strides[0], (int64_t)Diag); });
+  return (void)call.finish();
+}
+extern "C" int exoblas_stbmv_row_major_Lower_Trans_stride_any_async(hipStream_t stream, int_fast32_t n, int_fast32_t k, struct exo_win_1f32 x, struct exo_win_2f32c A, int_fast32_t Diag) {
+  exoblas::Call call("exoblas_stbmv_row_major_Lower_Trans_stride_any_async", exoblas::Call::kAsync, stream);
+  if (n < 0) return call.fail_size("n", n);
+  if (k < 0) return call.fail_size("k", k);
+  if (Diag < 0) return call.fail_size("Diag", Diag);
+  if ((true) && A.strides[1] != 1) return call.fail_stride("A", A.strides[1]);
+  if ((true) && n > 1 && A.strides[0] < k + 1) return call.fail_ld("A", A.strides[0]);
+  call.window(0, x.data, exoblas::span_lo(n, x.strides[0]), exoblas::span_hi(n, x.strides[0]), sizeof(float), true, true);
+  call.window(1, A.data, 0, exoblas::span2_hi(n, k + 1, A.strides[0]), sizeof(float), false, true);
+  if (!call.stage()) return call.status();
+  if (!call.ok()) return call.status();
+  call.run([&] { exoblas::ops::tbmv_row_major_Lower_Trans<float>(call.ctx(), (int64_t)n, (int64_t)k, call.dev<float>(0), (int64_t)x.strides[0], call.dev<const float>(1), (int64_t)A.strides[0], (int64_t)Diag); });
+  return call.finish();
+}
+
+// src/level2/tbmv.py:111-141
+extern "C" void exo_stbmv_row_major_Lower_Trans_stride_1(void *ctxt, int_fast32_t n, int_fast32_t k, struct exo_win_1f32 x, struct exo_win_2f32c A, int_fast32_t Diag) {
+  exoblas::Call call("exo_stbmv_row_major_Lower_Trans_stride_1", exoblas::Call::kSync, ctxt);
+  if (n < 0) return (void)call.fail_size("n", n);
+  if (k < 0) return (void)call.fail_size("k", k);
+  if (Diag < 0) return (void)call.fail_size("Diag", Diag);
+  if ((true) && A.strides[1] != 1) return (void)call.fail_stride("A", A.strides[1]);
+  if ((true) && n > 1 && A.strides[0] < k + 1) return (void)call.fail_ld("A", A.strides[0]);
+  call.window(0, x.data, exoblas::span_lo(n, x.strides[0]), exoblas::span_hi(n, x.strides[0]), sizeof(float), true, true);
+  call.window(1, A.data, 0, exoblas::span2_hi(n, k + 1, A.strides[0]), sizeof(float), false, true);
+  if (!call.stage()) return (void)call.status();
+  if (!call.ok()) return (void)call.status();
+  call.run([&] { exoblas::ops::tbmv_row_major_Lower_Trans<float>(call.ctx(), (int64_t)n, (int64_t)k, call.dev<float>(0), (int64_t)x.strides[0], call.dev<const float>(1), (int64_t)A.strides[0], (int64_t)Diag); });
+  return (void)call.finish();
+}
+extern "C" int exoblas_stbmv_row_major_Lower_Trans_stride_1_async(hipStream_t stream, int_fast32_t n, int_fast32_t k, struct exo_win_1f32 x, struct exo_win_2f32c A, int_fast32_t Diag) {
+  exoblas::Call call("exoblas_stbmv_row_major_Lower_Trans_stride_1_async", exoblas::Call::kAsync, stream);
+  if (n < 0) return call.fail_size("n", n);
+  if (k < 0) return call.fail_size("k", k);
+  if (Diag < 0) return call.fail_size("Diag", Diag);
+  if ((true) && A.strides[1] != 1) return call.fail_stride("A", A.strides[1]);
+  if ((true) && n > 1 && A.strides[0] < k + 1) return call.fail_ld("A", A.strides[0]);
+  call.window(0, x.data, exoblas::span_lo(n, x.strides[0]), exoblas::span_hi(n, x.strides[0]), sizeof(float), true, true);
+  call.window(1, A.data, 0, exoblas::span2_hi(n, k + 1, A.strides[0]), sizeof(float), false, true);
+  if (!call.stage()) return call.status();
+  if (!call.ok()) return call.status();
+  call.run([&] { exoblas::ops::tbmv_row_major_Lower_Trans<float>(call.ctx(), (int64_t)n, (int64_t)k, call.dev<float>(0), (int64_t)x.strides[0], call.dev<const float>(1), (int64_t)A.strides[0], (int64_t)Diag); });
+  return call.finish();
+}
+
+// src/level2/tbmv.py:111-141
+extern "C" void exo_dtbmv_row_major_Lower_Trans_stride_any(void *ctxt, int_fast32_t n, int_fast32_t k, struct exo_win_1f64 x, struct exo_win_2f64c A, int_fast32_t Diag) {
+  exoblas::Call call("exo_dtbmv_row_major_Lower_Trans_stride_any", exoblas::Call::kSync, ctxt);
+  if (n < 0) return (void)call.fail_size("n", n);
+  if (k < 0) return (void)call.fail_size("k", k);
+  if (Diag < 0) return (void)call.fail_size("Diag", Diag);
+  if ((true) && A.strides[1] != 1) return (void)call.fail_stride("A", A.strides[1]);
+  if ((true) && n > 1 && A.strides[0] < k + 1) return (void)call.fail_ld("A", A.strides[0]);
+  call.window(0, x.data, exoblas::span_lo(n, x.strides[0]), exoblas::span_hi(n, x.strides[0]), sizeof(double), true, true);
+  call.window(1, A.data, 0, exoblas::span2_hi(n, k + 1, A.strides[0]), sizeof(double), false, true);
+  if (!call.stage()) return (void)call.status();
+  if (!call.ok()) return (void)call.status();
+  call.run([&] { exoblas::ops::tbmv_row_major_Lower_Trans<double>(call.ctx(), (int64_t)n, (int64_t)k, call.dev<double>(0), (int64_t)x.strides[0], call.dev<const double>(1), (int64_t)A.strides[0], (int64_t)Diag); });
+  return (void)call.finish();
+}
+extern "C" int exoblas_dtbmv_row_major_Lower_Trans_stride_any_async(hipStream_t stream, int_fast32_t n, int_fast32_t k, struct exo_win_1f64 x, struct exo_win_2f64c A, int_fast32_t Diag) {
+  exoblas::Call call("exoblas_dtbmv_row_major_Lower_Trans_stride_any_async", exoblas::Call::kAsync, stream);
+  if (n < 0) return call.fail_size("n", n);
+  if (k < 0) return call.fail_size("k", k);
+  if (Diag < 0) return call.fail_size("Diag", Diag);
+  if ((true) && A.strides[1] != 1) return call.fail_stride("A", A.strides[1]);
+  if ((true) && n > 1 && A.strides[0] < k + 1) return call.fail_ld("A", A.strides[0]);
+  call.window(0, x.data, exoblas::span_lo(n, x.strides[0]), exoblas::span_hi(n, x.strides[0]), sizeof(double), true, true);
+  call.window(1, A.data, 0, exoblas::span2_hi(n, k + 1, A.strides[0]), sizeof(double), false, true);
+  if (!call.stage()) return call.status();
+  if (!call.ok()) return call.status();
+  call.run([&] { exoblas::ops::tbmv_row_major_Lower_Trans<double>(call.ctx(), (int64_t)n, (int64_t)k, call.dev<double>(0), (int64_t)x.strides[0], call.dev<const double>(1), (int64_t)A.strides[0], (int64_t)Diag); });
+  return call.finish();
+}
+
+// src/level2/tbmv.py:111-141
+extern "C" void exo_dtbmv_row_major_Lower_Trans_stride_1(void *ctxt, int_fast32_t n, int_fast32_t k, struct exo_win_1f64 x, struct exo_win_2f64c A, int_fast32_t Diag) {
+  exoblas::Call call("exo_dtbmv_row_major_Lower_Trans_stride_1", exoblas::Call::kSync, ctxt);
+  if (n < 0) return (void)call.fail_size("n", n);
+  if (k < 0) return (void)call.fail_size("k", k);
+  if (Diag < 0) return (void)call.fail_size("Diag", Diag);
+  if ((true) && A.strides[1] != 1) return (void)call.fail_stride("A", A.strides[1]);
+  if ((true) && n > 1 && A.strides[0] < k + 1) return (void)call.fail_ld("A", A.strides[0]);
+  call.window(0, x.data, exoblas::span_lo(n, x.strides[0]), exoblas::span_hi(n, x.strides[0]), sizeof(double), true, true);
+  call.window(1, A.data, 0, exoblas::span2_hi(n, k + 1, A.strides[0]), sizeof(double), false, true);
+  if (!call.stage()) return (void)call.status();
+  if (!call.ok()) return (void)call.status();
+  call.run([&] { exoblas::ops::tbmv_row_major_Lower_Trans<double>(call.ctx(), (int64_t)n, (int64_t)k, call.dev<double>(0), (int64_t)x.strides[0], call.dev<const double>(1), (int64_t)A.strides[0], (int64_t)Diag); });
+  return (void)call.finish();
+}
+extern "C" int exoblas_dtbmv_row_major_Lower_Trans_stride_1_async(hipStream_t stream, int_fast32_t n, int_fast32_t k, struct exo_win_1f64 x, struct exo_win_2f64c A, int_fast32_t Diag) {
+  exoblas::Call call("exoblas_dtbmv_row_major_Lower_Trans_stride_1_async", exoblas::Call::kAsync, stream);
+  if (n < 0) return call.fail_size("n", n);
+  if (k < 0) return call.fail_size("k", k);
+  if (Diag < 0) return call.fail_size("Diag", Diag);
+  if ((true) && A.strides[1] != 1) return call.fail_stride("A", A.strides[1]);
+  if ((true) && n > 1 && A.strides[0] < k + 1) return call.fail_ld("A", A.strides[0]);
+  call.window(0, x.data, exoblas::span_lo(n, x.strides[0]), exoblas::span_hi(n, x.strides[0]), sizeof(double), true, true);
+  call.window(1, A.data, 0, exoblas::span2_hi(n, k + 1, A.strides[0]), sizeof(double), false, true);
+  if (!call.stage()) return call.status();
+  if (!call.ok()) return call.status();
+  call.run([&] { exoblas::ops::tbmv_row_major_Lower_Trans<double>(call.ctx(), (int64_t)n, (int64_t)k, call.dev<double>(0), (int64_t)x.strides[0], call.dev<const double>(1), (int64_t)A.strides[0], (int64_t)Diag); });
+  return call.finish();
+}
+
+// src/level2/tbsv.py:12-45
+extern "C" void exo_stbsv_row_major_Upper_NonTrans_stride_any(void *ctxt, int_fast32_t n, int_fast32_t k, struct exo_win_1f32 x, struct exo_win_2f32c A, int_fast32_t Diag) {
+  exoblas::Call call("exo_stbsv_row_major_Upper_NonTrans_stride_any", exoblas::Call::kSync, ctxt);
+  if (n < 0) return (void)call.fail_size("n", n);
+  if (k < 0) return (void)call.fail_size("k", k);
+  if (Diag < 0) return (void)call.fail_size("Diag", Diag);
+  if ((true) && A.strides[1] != 1) return (void)call.fail_stride("A", A.strides[1]);
+  if ((true) && n > 1 && A.strides[0] < k + 1) return (void)call.fail_ld("A", A.strides[0]);
+  call.window(0, x.data, exoblas::span_lo(n, x.strides[0]), exoblas::span_hi(n, x.strides[0]), sizeof(float), true, true);
+  call.window(1, A.data, 0, exoblas::span2_hi(n, k + 1, A.strides[0]), sizeof(float), false, true);
+  if (!call.stage()) return (void)call.status();
+  if (!call.ok()) return (void)call.status();
+  call.run([&] { exoblas::ops::tbsv_row_major_Upper_NonTrans<float>(call.ctx(), (int64_t)n, (int64_t)k, call.dev<float>(0), (int64_t)x.strides[0], call.dev<const float>(1), (int64_t)A.strides[0], (int64_t)Diag); });
+  return (void)call.finish();
+}
+extern "C" int exoblas_stbsv_row_major_Upper_NonTrans_stride_any_async(hipStream_t stream, int_fast32_t n, int_fast32_t k, struct exo_win_1f32 x, struct exo_win_2f32c A, int_fast32_t Diag) {
+  exoblas::Call call("exoblas_stbsv_row_major_Upper_NonTrans_stride_any_async", exoblas::Call::kAsync, stream);
+  if (n < 0) return call.fail_size("n", n);
+  if (k < 0) return call.fail_size("k", k);
+  if (Diag < 0) return call.fail_size("Diag", Diag);
+  if ((true) && A.strides[1] != 1) return call.fail_stride("A", A.strides[1]);
+  if ((true) && n > 1 && A.strides[0] < k + 1) return call.fail_ld("A", A.strides[0]);
+  call.window(0, x.data, exoblas::span_lo(n, x.strides[0]), exoblas::span_hi(n, x.strides[0]), sizeof(float), true, true);
+  call.window(1, A.data, 0, exoblas::span2_hi(n, k + 1, A.strides[0]), sizeof(float), false, true);
+  if (!call.stage()) return call.status();
+  if (!call.ok()) return call.status();
+  call.run([&] { exoblas::ops::tbsv_row_major_Upper_NonTrans<float>(call.ctx(), (int64_t)n, (int64_t)k, call.dev<float>(0), (int64_t)x.strides[0], call.dev<const float>(1), (int64_t)A.strides[0], (int64_t)Diag); });
+  return call.finish();
+}
+
+// src/level2/tbsv.py:12-45
+extern "C" void exo_stbsv_row_major_Upper_NonTrans_stride_1(void *ctxt, int_fast32_t n, int_fast32_t k, struct exo_win_1f32 x, struct exo_win_2f32c A, int_fast32_t Diag) {
+  exoblas::Call call("exo_stbsv_row_major_Upper_NonTrans_stride_1", exoblas::Call::kSync, ctxt);
+  if (n < 0) return (void)call.fail_size("n", n);
+  if (k < 0) return (void)call.fail_size("k", k);
+  if (Diag < 0) return (void)call.fail_size("Diag", Diag);
+  if ((true) && A.strides[1] != 1) return (void)call.fail_stride("A", A.strides[1]);
+  if ((true) && n > 1 && A.strides[0] < k + 1) return (void)call.fail_ld("A", A.strides[0]);
+  call.window(0, x.data, exoblas::span_lo(n, x.strides[0]), exoblas::span_hi(n, x.strides[0]), sizeof(float), true, true);
+  call.window(1, A.data, 0, exoblas::span2_hi(n, k + 1, A.strides[0]), sizeof(float), false, true);
+  if (!call.stage()) return (void)call.status();
+  if (!call.ok()) return (void)call.status();
+  call.run([&] { exoblas::ops::tbsv_row_major_Upper_NonTrans<float>(call.ctx(), (int64_t)n, (int64_t)k, call.dev<float>(0), (int64_t)x.strides[0], call.dev<const float>(1), (int64_t)A.strides[0], (int64_t)Diag); });
+  return (void)call.finish();
+}
+extern "C" int exoblas_stbsv_row_major_Upper_NonTrans_stride_1_async(hipStream_t stream, int_fast32_t n, int_fast32_t k, struct exo_win_1f32 x, struct exo_win_2f32c A, int_fast32_t Diag) {
+  exoblas::Call call("exoblas_stbsv_row_major_Upper_NonTrans_stride_1_async", exoblas::Call::kAsync, stream);
+  if (n < 0) return call.fail_size("n", n);
+  if (k < 0) return call.fail_size("k", k);
+  if (Diag < 0) return call.fail_size("Diag", Diag);
+  if ((true) && A.strides[1] != 1) return call.fail_stride("A", A.strides[1]);
+  if ((true) && n > 1 && A.strides[0] < k + 1) return call.fail_ld("A", A.strides[0]);
+  call.window(0, x.data, exoblas::span_lo(n, x.strides[0]), exoblas::span_hi(n, x.strides[0]), sizeof(float), true, true);
+  call.window(1, A.data, 0, exoblas::span2_hi(n, k + 1, A.strides[0]), sizeof(float), false, true);
+  if (!call.stage()) return call.status();
+  if (!call.ok()) return call.status();
+  call.run([&] { exoblas::ops::tbsv_row_major_Upper_NonTrans<float>(call.ctx(), (int64_t)n, (int64_t)k, call.dev<float>(0), (int64_t)x.strides[0], call.dev<const float>(1), (int64_t)A.strides[0], (int64_t)Diag); });
+  return call.finish();
+}
+
+// src/level2/tbsv.py:12-45
+extern "C" void exo_dtbsv_row_major_Upper_NonTrans_stride_any(void *ctxt, int_fast32_t n, int_fast32_t k, struct exo_win_1f64 x, struct exo_win_2f64c A, int_fast32_t Diag) {
+  exoblas::Call call("exo_dtbsv_row_major_Upper_NonTrans_stride_any", exoblas::Call::kSync, ctxt);
+  if (n < 0) return (void)call.fail_size("n", n);
+  if (k < 0) return (void)call.fail_size("k", k);
+  if (Diag < 0) return (void)call.fail_size("Diag", Diag);
+  if ((true) && A.strides[1] != 1) return (void)call.fail_stride("A", A.strides[1]);
+  if ((true) && n > 1 && A.strides[0] < k + 1) return (void)call.fail_ld("A", A.strides[0]);
+  call.window(0, x.data, exoblas::span_lo(n, x.strides[0]), exoblas::span_hi(n, x.strides[0]), sizeof(double), true, true);
+  call.window(1, A.data, 0, exoblas::span2_hi(n, k + 1, A.strides[0]), sizeof(double), false, true);
+  if (!call.stage()) return (void)call.status();
+  if (!call.ok()) return (void)call.status();
+  call.run([&] { exoblas::ops::tbsv_row_major_Upper_NonTrans<double>(call.ctx(), (int64_t)n, (int64_t)k, call.dev<double>(0), (int64_t)x.strides[0], call.dev<const double>(1), (int64_t)A.strides[0], (int64_t)Diag); });
+  return (void)call.finish();
+}
+extern "C" int exoblas_dtbsv_row_major_Upper_NonTrans_stride_any_async(hipStream_t stream, int_fast32_t n, int_fast32_t k, struct exo_win_1f64 x, struct exo_win_2f64c A, int_fast32_t Diag) {
+  exoblas::Call call("exoblas_dtbsv_row_major_Upper_NonTrans_stride_any_async", exoblas::Call::kAsync, stream);
+  if (n < 0) return call.fail_size("n", n);
+  if (k < 0) return call.fail_size("k", k);
+  if (Diag < 0) return call.fail_size("Diag", Diag);
+  if ((true) && A.strides[1] != 1) return call.fail_stride("A", A.strides[1]);
+  if ((true) && n > 1 && A.strides[0] < k + 1) return call.fail_ld("A", A.strides[0]);
+  call.window(0, x.data, exoblas::span_lo(n, x.strides[0]), exoblas::span_hi(n, x.strides[0]), sizeof(double), true, true);
+  call.window(1, A.data, 0, exoblas::span2_hi(n, k + 1, A.strides[0]), sizeof(double), false, true);
+  if (!call.stage()) return call.status();
+  if (!call.ok()) return call.status();
+  call.run([&] { exoblas::ops::tbsv_row_major_Upper_NonTrans<double>(call.ctx(), (int64_t)n, (int64_t)k, call.dev<double>(0), (int64_t)x.strides[0], call.dev<const double>(1), (int64_t)A.strides[0], (int64_t)Diag); });
+  return call.finish();
+}
+
+// src/level2/tbsv.py:12-45
+extern "C" void exo_dtbsv_row_major_Upper_NonTrans_stride_1(void *ctxt, int_fast32_t n, int_fast32_t k, struct exo_win_1f64 x, struct exo_win_2f64c A, int_fast32_t Diag) {
+  exoblas::Call call("exo_dtbsv_row_major_Upper_NonTrans_stride_1", exoblas::Call::kSync, ctxt);
+  if (n < 0) return (void)call.fail_size("n", n);
+  if (k < 0) return (void)call.fail_size("k", k);
+  if (Diag < 0) return (void)call.fail_size("Diag", Diag);
+  if ((true) && A.strides[1] != 1) return (void)call.fail_stride("A", A.strides[1]);
+  if ((true) && n > 1 && A.strides[0] < k + 1) return (void)call.fail_ld("A", A.strides[0]);
+  call.window(0, x.data, exoblas::span_lo(n, x.strides[0]), exoblas::span_hi(n, x.strides[0]), sizeof(double), true, true);
+  call.window(1, A.data, 0, exoblas::span2_hi(n, k + 1, A.strides[0]), sizeof(double), false, true);
+  if (!call.stage()) return (void)call.status();
+  if (!call.ok()) return (void)call.status();
+  call.run([&] { exoblas::ops::tbsv_row_major_Upper_NonTrans<double>(call.ctx(), (int64_t)n, (int64_t)k, call.dev<double>(0), (int64_t)x.strides[0], call.dev<const double>(1), (int64_t)A.strides[0], (int64_t)Diag); });
+  return (void)call.finish();
+}
+extern "C" int exoblas_dtbsv_row_major_Upper_NonTrans_stride_1_async(hipStream_t stream, int_fast32_t n, int_fast32_t k, struct exo_win_1f64 x, struct exo_win_2f64c A, int_fast32_t Diag) {
+  exoblas::Call call("exoblas_dtbsv_row_major_Upper_NonTrans_stride_1_async", exoblas::Call::kAsync, stream);
+  if (n < 0) return call.fail_size("n", n);
+  if (k < 0) return call.fail_size("k", k);
+  if (Diag < 0) return call.fail_size("Diag", Diag);
+  if ((true) && A.strides[1] != 1) return call.fail_stride("A", A.strides[1]);
+  if ((true) && n > 1 && A.strides[0] < k + 1) return call.fail_ld("A", A.strides[0]);
+  call.window(0, x.data, exoblas::span_lo(n, x.strides[0]), exoblas::span_hi(n, x.strides[0]), sizeof(double), true, true);
+  call.window(1, A.data, 0, exoblas::span2_hi(n, k + 1, A.strides[0]), sizeof(double), false, true);
+  if (!call.stage()) return call.status();
+  if (!call.ok()) return call.status();
+  call.run([&] { exoblas::ops::tbsv_row_major_Upper_NonTrans<double>(call.ctx(), (int64_t)n, (int64_t)k, call.dev<double>(0), (int64_t)x.strides[0], call.dev<const double>(1), (int64_t)A.strides[0], (int64_t)Diag); });
+  return call.finish();
+}
+
+// src/level2/tbsv.py:48-81
+extern "C" void exo_stbsv_row_major_Lower_NonTrans_stride_any(void *ctxt, int_fast32_t n, int_fast32_t k, struct exo_win_1f32 x, struct exo_win_2f32c A, int_fast32_t Diag) {
+  exoblas::Call call("exo_stbsv_row_major_Lower_NonTrans_stride_any", exoblas::Call::kSync, ctxt);
+  if (n < 0) return (void)call.fail_size("n", n);
+  if (k < 0) return (void)call.fail_size("k", k);
+  if (Diag < 0) return (void)call.fail_size("Diag", Diag);
+  if ((true) && A.strides[1] != 1) return (void)call.fail_stride("A", A.strides[1]);
+  if ((true) && n > 1 && A.strides[0] < k + 1) return (void)call.fail_ld("A", A.strides[0]);
+  call.window(0, x.data, exoblas::span_lo(n, x.strides[0]), exoblas::span_hi(n, x.strides[0]), sizeof(float), true, true);
+  call.window(1, A.data, 0, exoblas::span2_hi(n, k + 1, A.strides[0]), sizeof(float), false, true);
+  if (!call.stage()) return (void)call.status();
+  if (!call.ok()) return (void)call.status();
+  call.run([&] { exoblas::ops::tbsv_row_major_Lower_NonTrans<float>(call.ctx(), (int64_t)n, (int64_t)k, call.dev<float>(0), (int64_t)x.strides[0], call.dev<const float>(1), (int64_t)A.strides[0], (int64_t)Diag); });
+  return (void)call.finish();
+}
+extern "C" int exoblas_stbsv_row_major_Lower_NonTrans_stride_any_async(hipStream_t stream, int_fast32_t n, int_fast32_t k, struct exo_win_1f32 x, struct exo_win_2f32c A, int_fast32_t Diag) {
+  exoblas::Call call("exoblas_stbsv_row_major_Lower_NonTrans_stride_any_async", exoblas::Call::kAsync, stream);
+  if (n < 0) return call.fail_size("n", n);
+  if (k < 0) return call.fail_size("k", k);
+  if (Diag < 0) return call.fail_size("Diag", Diag);
+  if ((true) && A.strides[1] != 1) return call.fail_stride("A", A.strides[1]);
+  if ((true) && n > 1 && A.strides[0] < k + 1) return call.fail_ld("A", A.strides[0]);
+  call.window(0, x.data, exoblas::span_lo(n, x.strides[0]), exoblas::span_hi(n, x.strides[0]), sizeof(float), true, true);
+  call.window(1, A.data, 0, exoblas::span2_hi(n, k + 1, A.strides[0]), sizeof(float), false, true);
+  if (!call.stage()) return call.status();
+  if (!call.ok()) return call.status();
+  call.run([&] { exoblas::ops::tbsv_row_major_Lower_NonTrans<float>(call.ctx(), (int64_t)n, (int64_t)k, call.dev<float>(0), (int64_t)x.strides[0], call.dev<const float>(1), (int64_t)A.strides[0], (int64_t)Diag); });
+  return call.finish();
+}
+
+// src/level2/tbsv.py:48-81
+extern "C" void exo_stbsv_row_major_Lower_NonTrans_stride_1(void *ctxt, int_fast32_t n, int_fast32_t k, struct exo_win_1f32 x, struct exo_win_2f32c A, int_fast32_t Diag) {
+  exoblas::Call call("exo_stbsv_row_major_Lower_NonTrans_stride_1", exoblas::Call::kSync, ctxt);
+  if (n < 0) return (void)call.fail_size("n", n);
+  if (k < 0) return (void)call.fail_size("k", k);
+  if (Diag < 0) return (void)call.fail_size("Diag", Diag);
+  if ((true) && A.strides[1] != 1) return (void)call.fail_stride("A", A.strides[1]);
+  if ((true) && n > 1 && A.strides[0] < k + 1) return (void)call.fail_ld("A", A.strides[0]);
+  call.window(0, x.data, exoblas::span_lo(n, x.strides[0]), exoblas::span_hi(n, x.strides[0]), sizeof(float), true, true);
+  call.window(1, A.data, 0, exoblas::span2_hi(n, k + 1, A.strides[0]), sizeof(float), false, true);
+  if (!call.stage()) return (void)call.status();
+  if (!call.ok()) return (void)call.status();
+  call.run([&] { exoblas::ops::tbsv_row_major_Lower_NonTrans<float>(call.ctx(), (int64_t)n, (int64_t)k, call.dev<float>(0), (int64_t)x.strides[0], call.dev<const float>(1), (int64_t)A.strides[0], (int64_t)Diag); });
+  return (void)call.finish();
+}
+extern "C" int exoblas_stbsv_row_major_Lower_NonTrans_stride_1_async(hipStream_t stream, int_fast32_t n, int_fast32_t k, struct exo_win_1f32 x, struct exo_win_2f32c A, int_fast32_t Diag) {
+  exoblas::Call call("exoblas_stbsv_row_major_Lower_NonTrans_stride_1_async", exoblas::Call::kAsync, stream);
+  if (n < 0) return call.fail_size("n", n);
+  if (k < 0) return call.fail_size("k", k);
+  if (Diag < 0) return call.fail_size("Diag", Diag);
+  if ((true) && A.strides[1] != 1) return call.fail_stride("A", A.strides[1]);
+  if ((true) && n > 1 && A.strides[0] < k + 1) return call.fail_ld("A", A.strides[0]);
+  call.window(0, x.data, exoblas::span_lo(n, x.strides[0]), exoblas::span_hi(n, x.strides[0]), sizeof(float), true, true);
+  call.window(1, A.data, 0, exoblas::span2_hi(n, k + 1, A.strides[0]), sizeof(float), false, true);
+  if (!call.stage()) return call.status();
+  if (!call.ok()) return call.status();
+  call.run([&] { exoblas::ops::tbsv_row_major_Lower_NonTrans<float>(call.ctx(), (int64_t)n, (int64_t)k, call.dev<float>(0), (int64_t)x.strides[0], call.dev<const float>(1), (int64_t)A.strides[0], (int64_t)Diag); });
+  return call.finish();
+}
+
+// src/level2/tbsv.py:48-81
+extern "C" void exo_dtbsv_row_major_Lower_NonTrans_stride_any(void *ctxt, int_fast32_t n, int_fast32_t k, struct exo_win_1f64 x, struct exo_win_2f64c A, int_fast32_t Diag) {
+  exoblas::Call call("exo_dtbsv_row_major_Lower_NonTrans_stride_any", exoblas::Call::kSync, ctxt);
+  if (n < 0) return (void)call.fail_size("n", n);
+  if (k < 0) return (void)call.fail_size("k", k);
+  if (Diag < 0) return (void)call.fail_size("Diag", Diag);
+  if ((true) && A.strides[1] != 1) return (void)call.fail_stride("A", A.strides[1]);
+  if ((true) && n > 1 && A.strides[0] < k + 1) return (void)call.fail_ld("A", A.strides[0]);
+  call.window(0, x.data, exoblas::span_lo(n, x.strides[0]), exoblas::span_hi(n, x.strides[0]), sizeof(double), true, true);
+  call.window(1, A.data, 0, exoblas::span2_hi(n, k + 1, A.strides[0]), sizeof(double), false, true);
+  if (!call.stage()) return (void)call.status();
+  if (!call.ok()) return (void)call.status();
+  call.run([&] { exoblas::ops::tbsv_row_major_Lower_NonTrans<double>(call.ctx(), (int64_t)n, (int64_t)k, call.dev<double>(0), (int64_t)x.strides[0], call.dev<const double>(1), (int64_t)A.strides[0], (int64_t)Diag); });
+  return (void)call.finish();
+}
+extern "C" int exoblas_dtbsv_row_major_Lower_NonTrans_stride_any_async(hipStream_t stream, int_fast32_t n, int_fast32_t k, struct exo_win_1f64 x, struct exo_win_2f64c A, int_fast32_t Diag) {
+  exoblas::Call call("exoblas_dtbsv_row_major_Lower_NonTrans_stride_any_async", exoblas::Call::kAsync, stream);
+  if (n < 0) return call.fail_size("n", n);
+  if (k < 0) return call.fail_size("k", k);
+  if (Diag < 0) return call.fail_size("Diag", Diag);
+  if ((true) && A.strides[1] != 1) return call.fail_stride("A", A.strides[1]);
+  if ((true) && n > 1 && A.strides[0] < k + 1) return call.fail_ld("A", A.strides[0]);
+  call.window(0, x.data, exoblas::span_lo(n, x.strides[0]), exoblas::span_hi(n, x.strides[0]), sizeof(double), true, true);
+  call.window(1, A.data, 0, exoblas::span2_hi(n, k + 1, A.strides[0]), sizeof(double), false, true);
+  if (!call.stage()) return call.status();
+  if (!call.ok()) return call.status();
+  call.run([&] { exoblas::ops::tbsv_row_major_Lower_NonTrans<double>(call.ctx(), (int64_t)n, (int64_t)k, call.dev<double>(0), (int64_t)x.strides[0], call.dev<const double>(1), (int64_t)A.strides[0], (int64_t)Diag); });
+  return call.finish();
+}
+
+// src/level2/tbsv.py:48-81
+extern "C" void exo_dtbsv_row_major_Lower_NonTrans_stride_1(void *ctxt, int_fast32_t n, int_fast32_t k, struct exo_win_1f64 x, struct exo_win_2f64c A, int_fast32_t Diag) {
+  exoblas::Call call("exo_dtbsv_row_major_Lower_NonTrans_stride_1", exoblas::Call::kSync, ctxt);
+  if (n < 0) return (void)call.fail_size("n", n);
+  if (k < 0) return (void)call.fail_size("k", k);
+  if (Diag < 0) return (void)call.fail_size("Diag", Diag);
+  if ((true) && A.strides[1] != 1) return (void)call.fail_stride("A", A.strides[1]);
+  if ((true) && n > 1 && A.strides[0] < k + 1) return (void)call.fail_ld("A", A.strides[0]);
+  call.window(0, x.data, exoblas::span_lo(n, x.strides[0]), exoblas::span_hi(n, x.strides[0]), sizeof(double), true, true);
+  call.window(1, A.data, 0, exoblas::span2_hi(n, k + 1, A.strides[0]), sizeof(double), false, true);
+  if (!call.stage()) return (void)call.status();
+  if (!call.ok()) return (void)call.status();
+  call.run([&] { exoblas::ops::tbsv_row_major_Lower_NonTrans<double>(call.ctx(), (int64_t)n, (int64_t)k, call.dev<double>(0), (int64_t)x.strides[0], call.dev<const double>(1), (int64_t)A.strides[0], (int64_t)Diag); });
+  return (void)call.finish();
+}
+extern "C" int exoblas_dtbsv_row_major_Lower_NonTrans_stride_1_async(hipStream_t stream, int_fast32_t n, int_fast32_t k, struct exo_win_1f64 x, struct exo_win_2f64c A, int_fast32_t Diag) {
+  exoblas::Call call("exoblas_dtbsv_row_major_Lower_NonTrans_stride_1_async", exoblas::Call::kAsync, stream);
+  if (n < 0) return call.fail_size("n", n);
+  if (k < 0) return call.fail_size("k", k);
+  if (Diag < 0) return call.fail_size("Diag", Diag);
+  if ((true) && A.strides[1] != 1) return call.fail_stride("A", A.strides[1]);
+  if ((true) && n > 1 && A.strides[0] < k + 1) return call.fail_ld("A", A.strides[0]);
+  call.window(0, x.data, exoblas::span_lo(n, x.strides[0]), exoblas::span_hi(n, x.strides[0]), sizeof(double), true, true);
+  call.window(1, A.data, 0, exoblas::span2_hi(n, k + 1, A.strides[0]), sizeof(double), false, true);
+  if (!call.stage()) return call.status();
+  if (!call.ok()) return call.status();
+  call.run([&] { exoblas::ops::tbsv_row_major_Lower_NonTrans<double>(call.ctx(), (int64_t)n, (int64_t)k, call.dev<double>(0), (int64_t)x.strides[0], call.dev<const double>(1), (int64_t)A.strides[0], (int64_t)Diag); });
+  return call.finish();
+}
+
+// src/level2/tbsv.py:84-105
+extern "C" void exo_stbsv_row_major_Upper_Trans_stride_any(void *ctxt, int_fast32_t n, int_fast32_t k, struct exo_win_1f32 x, struct exo_win_2f32c A, int_fast32_t Diag) {
+  exoblas::Call call("exo_stbsv_row_major_Upper_Trans_stride_any", exoblas::Call::kSync, ctxt);
+  if (n < 0) return (void)call.fail_size("n", n);
+  if (k < 0) return (void)call.fail_size("k", k);
+  if (Diag < 0) return (void)call.fail_size("Diag", Diag);
+  if ((true) && A.strides[1] != 1) return (void)call.fail_stride("A", A.strides[1]);
+  if ((true) && n > 1 && A.strides[0] < k + 1) return (void)call.fail_ld("A", A.strides[0]);
+  call.window(0, x.data, exoblas::span_lo(n, x.strides[0]), exoblas::span_hi(n, x.strides[0]), sizeof(float), true, true);
+  call.window(1, A.data, 0, exoblas::span2_hi(n, k + 1, A.strides[0]), sizeof(float), false, true);
+  if (!call.stage()) return (void)call.status();
+  if (!call.ok()) return (void)call.status();
+  call.run([&] { exoblas::ops::tbsv_row_major_Upper_Trans<float>(call.ctx(), (int64_t)n, (int64_t)k, call.dev<float>(0), (int64_t)x.strides[0], call.dev<const float>(1), (int64_t)A.strides[0], (int64_t)Diag); });
+  return (void)call.finish();
+}
+extern "C" int exoblas_stbsv_row_major_Upper_Trans_stride_any_async(hipStream_t stream, int_fast32_t n, int_fast32_t k, struct exo_win_1f32 x, struct exo_win_2f32c A, int_fast32_t Diag) {
+  exoblas::Call call("exoblas_stbsv_row_major_Upper_Trans_stride_any_async", exoblas::Call::kAsync, stream);
+  if (n < 0) return call.fail_size("n", n);
+  if (k < 0) return call.fail_size("k", k);
+  if (Diag < 0) return call.fail_size("Diag", Diag);
+  if ((true) && A.strides[1] != 1) return call.fail_stride("A", A.strides[1]);
+  if ((true) && n > 1 && A.strides[0] < k + 1) return call.fail_ld("A", A.strides[0]);
+  call.window(0, x.data, exoblas::span_lo(n, x.strides[0]), exoblas::span_hi(n, x.strides[0]), sizeof(float), true, true);
+  call.window(1, A.data, 0, exoblas::span2_hi(n, k + 1, A.strides[0]), sizeof(float), false, true);
+  if (!call.stage()) return call.status();
+  if (!call.ok()) return call.status();
+  call.run([&] { exoblas::ops::tbsv_row_major_Upper_Trans<float>(call.ctx(), (int64_t)n, (int64_t)k, call.dev<float>(0), (int64_t)x.strides[0], call.dev<const float>(1), (int64_t)A.strides[0], (int64_t)Diag); });
+  return call.finish();
+}
+
+// src/level2/tbsv.py:84-105
+extern "C" void exo_dtbsv_row_major_Upper_Trans_stride_any(void *ctxt, int_fast32_t n, int_fast32_t k, struct exo_win_1f64 x, struct exo_win_2f64c A, int_fast32_t Diag) {
+  exoblas::Call call("exo_dtbsv_row_major_Upper_Trans_stride_any", exoblas::Call::kSync, ctxt);
+  if (n < 0) return (void)call.fail_size("n", n);
+  if (k < 0) return (void)call.fail_size("k", k);
+  if (Diag < 0) return (void)call.fail_size("Diag", Diag);
+  if ((true) && A.strides[1] != 1) return (void)call.fail_stride("A", A.strides[1]);
+  if ((true) && n > 1 && A.strides[0] < k + 1) return (void)call.fail_ld("A", A.strides[0]);
+  call.window(0, x.data, exoblas::span_lo(n, x.strides[0]), exoblas::span_hi(n, x.strides[0]), sizeof(double), true, true);
+  call.window(1, A.data, 0, exoblas::span2_hi(n, k + 1, A.strides[0]), sizeof(double), false, true);
+  if (!call.stage()) return (void)call.status();
+  if (!call.ok()) return (void)call.status();
+  call.run([&] { exoblas::ops::tbsv_row_major_Upper_Trans<double>(call.ctx(), (int64_t)n, (int64_t)k, call.dev<double>(0), (int64_t)x.strides[0], call.dev<const double>(1), (int64_t)A.strides[0], (int64_t)Diag); });
+  return (void)call.finish();
+}
+extern "C" int exoblas_dtbsv_row_major_Upper_Trans_stride_any_async(hipStream_t stream, int_fast32_t n, int_fast32_t k, struct exo_win_1f64 x, struct exo_win_2f64c A, int_fast32_t Diag) {
+  exoblas::Call call("exoblas_dtbsv_row_major_Upper_Trans_stride_any_async", exoblas::Call::kAsync, stream);
+  if (n < 0) return call.fail_size("n", n);
+  if (k < 0) return call.fail_size("k", k);
+  if (Diag < 0) return call.fail_size("Diag", Diag);
+  if ((true) && A.strides[1] != 1) return call.fail_stride("A", A.strides[1]);
+  if ((true) && n > 1 && A.strides[0] < k + 1) return call.fail_ld("A", A.strides[0]);
+  call.window(0, x.data, exoblas::span_lo(n, x.strides[0]), exoblas::span_hi(n, x.strides[0]), sizeof(double), true, true);
+  call.window(1, A.data, 0, exoblas::span2_hi(n, k + 1, A.strides[0]), sizeof(double), false, true);
+  if (!call.stage()) return call.status();
+  if (!call.ok()) return call.status();
+  call.run([&] { exoblas::ops::tbsv_row_major_Upper_Trans<double>(call.ctx(), (int64_t)n, (int64_t)k, call.dev<double>(0), (int64_t)x.strides[0], call.dev<const double>(1), (int64_t)A.strides[0], (int64_t)Diag); });
+  return call.finish();
+}
+
+// src/level2/tbsv.py:108-129
+extern "C" void exo_stbsv_row_major_Lower_Trans_stride_any(void *ctxt, int_fast32_t n, int_fast32_t k, struct exo_win_1f32 x, struct exo_win_2f32c A, int_fast32_t Diag) {
+  exoblas::Call call("exo_stbsv_row_major_Lower_Trans_stride_any", exoblas::Call::kSync, ctxt);
+  if (n < 0) return (void)call.fail_size("n", n);
+  if (k < 0) return (void)call.fail_size("k", k);
+  if (Diag < 0) return (void)call.fail_size("Diag", Diag);
+  if ((true) && A.strides[1] != 1) return (void)call.fail_stride("A", A.strides[1]);
+  if ((true) && n > 1 && A.strides[0] < k + 1) return (void)call.fail_ld("A", A.strides[0]);
+  call.window(0, x.data, exoblas::span_lo(n, x.strides[0]), exoblas::span_hi(n, x.strides[0]), sizeof(float), true, true);
+  call.window(1, A.data, 0, exoblas::span2_hi(n, k + 1, A.strides[0]), sizeof(float), false, true);
+  if (!call.stage()) return (void)call.status();
+  if (!call.ok()) return (void)call.status();
+  call.run([&] { exoblas::ops::tbsv_row_major_Lower_Trans<float>(call.ctx(), (int64_t)n, (int64_t)k, call.dev<float>(0), (int64_t)x.strides[0], call.dev<const float>(1), (int64_t)A.strides[0], (int64_t)Diag); });
+  return (void)call.finish();
+}
+extern "C" int exoblas_stbsv_row_major_Lower_Trans_stride_any_async(hipStream_t stream, int_fast32_t n, int_fast32_t k, struct exo_win_1f32 x, struct exo_win_2f32c A, int_fast32_t Diag) {
+  exoblas::Call call("exoblas_stbsv_row_major_Lower_Trans_stride_any_async", exoblas::Call::kAsync, stream);
+  if (n < 0) return call.fail_size("n", n);
+  if (k < 0) return call.fail_size("k", k);
+  if (Diag < 0) return call.fail_size("Diag", Diag);
+  if ((true) && A.strides[1] != 1) return call.fail_stride("A", A.strides[1]);
+  if ((true) && n > 1 && A.strides[0] < k + 1) return call.fail_ld("A", A.strides[0]);
+  call.window(0, x.data, exoblas::span_lo(n, x.strides[0]), exoblas::span_hi(n, x.strides[0]), sizeof(float), true, true);
+  call.window(1, A.data, 0, exoblas::span2_hi(n, k + 1, A.strides[0]), sizeof(float), false, true);
+  if (!call.stage()) return call.status();
+  if (!call.ok()) return call.status();
+  call.run([&] { exoblas::ops::tbsv_row_major_Lower_Trans<float>(call.ctx(), (int64_t)n, (int64_t)k, call.dev<float>(0), (int64_t)x.strides[0], call.dev<const float>(1), (int64_t)A.strides[0], (int64_t)Diag); });
+  return call.finish();
+}
+
+// src/level2/tbsv.py:108-129
+extern "C" void exo_dtbsv_row_major_Lower_Trans_stride_any(void *ctxt, int_fast32_t n, int_fast32_t k, struct exo_win_1f64 x, struct exo_win_2f64c A, int_fast32_t Diag) {
+  exoblas::Call call("exo_dtbsv_row_major_Lower_Trans_stride_any", exoblas::Call::kSync, ctxt);
+  if (n < 0) return (void)call.fail_size("n", n);
+  if (k < 0) return (void)call.fail_size("k", k);
+  if (Diag < 0) return (void)call.fail_size("Diag", Diag);
+  if ((true) && A.strides[1] != 1) return (void)call.fail_stride("A", A.strides[1]);
+  if ((true) && n > 1 && A.strides[0] < k + 1) return (void)call.fail_ld("A", A.strides[0]);
+  call.window(0, x.data, exoblas::span_lo(n, x.strides[0]), exoblas::span_hi(n, x.strides[0]), sizeof(double), true, true);
+  call.window(1, A.data, 0, exoblas::span2_hi(n, k + 1, A.strides[0]), sizeof(double), false, true);
+  if (!call.stage()) return (void)call.status();
+  if (!call.ok()) return (void)call.status();
+  call.run([&] { exoblas::ops::tbsv_row_major_Lower_Trans<double>(call.ctx(), (int64_t)n, (int64_t)k, call.dev<double>(0), (int64_t)x.strides[0], call.dev<const double>(1), (int64_t)A.strides[0], (int64_t)Diag); });
+  return (void)call.finish();
+}
+extern "C" int exoblas_dtbsv_row_major_Lower_Trans_stride_any_async(hipStream_t stream, int_fast32_t n, int_fast32_t k, struct exo_win_1f64 x, struct exo_win_2f64c A, int_fast32_t Diag) {
+  exoblas::Call call("exoblas_dtbsv_row_major_Lower_Trans_stride_any_async", exoblas::Call::kAsync, stream);
+  if (n < 0) return call.fail_size("n", n);
+  if (k < 0) return call.fail_size("k", k);
+  if (Diag < 0) return call.fail_size("Diag", Diag);
+  if ((true) && A.strides[1] != 1) return call.fail_stride("A", A.strides[1]);
+  if ((true) && n > 1 && A.strides[0] < k + 1) return call.fail_ld("A", A.strides[0]);
+  call.window(0, x.data, exoblas::span_lo(n, x.strides[0]), exoblas::span_hi(n, x.strides[0]), sizeof(double), true, true);
+  call.window(1, A.data, 0, exoblas::span2_hi(n, k + 1, A.strides[0]), sizeof(double), false, true);
+  if (!call.stage()) return call.status();
+  if (!call.ok()) return call.status();
+  call.run([&] { exoblas::ops::tbsv_row_major_Lower_Trans<double>(call.ctx(), (int64_t)n, (int64_t)k, call.dev<double>(0), (int64_t)x.strides[0], call.dev<const double>(1), (int64_t)A.strides[0], (int64_t)Diag); });
   return call.finish();
 }
 

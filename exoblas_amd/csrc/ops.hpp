@@ -51,6 +51,29 @@ template <class T>
 void trsv_rm(Ctx&, int64_t Uplo, int64_t TransA, int64_t Diag, int64_t n, T* x, int64_t incx, const T* A,
              int64_t lda);
 
+// ---------------------------------------------------------------- banded level 2
+template <class T>
+void gbmv_row_major_NonTrans(Ctx&, int64_t m, int64_t n, int64_t kl, int64_t ku, T alpha, T beta, const T* a,
+                             int64_t lda, const T* x, int64_t incx, T* y, int64_t incy);
+template <class T> void sbmv_scal_y(Ctx&, int64_t n, T beta, T* y, int64_t incy);
+template <class T>
+void sbmv_row_major_Upper(Ctx&, int64_t n, int64_t k, T alpha, const T* A, int64_t lda, const T* x, int64_t incx,
+                          T* y, int64_t incy);
+template <class T>
+void sbmv_row_major_Lower(Ctx&, int64_t n, int64_t k, T alpha, const T* A, int64_t lda, const T* x, int64_t incx,
+                          T* y, int64_t incy);
+#define EXO_BAND_TRI_DECL(NAME) \
+  template <class T>            \
+  void NAME(Ctx&, int64_t n, int64_t k, T* x, int64_t incx, const T* A, int64_t lda, int64_t Diag);
+EXO_BAND_TRI_DECL(tbmv_row_major_Upper_NonTrans)
+EXO_BAND_TRI_DECL(tbmv_row_major_Lower_NonTrans)
+EXO_BAND_TRI_DECL(tbmv_row_major_Upper_Trans)
+EXO_BAND_TRI_DECL(tbmv_row_major_Lower_Trans)
+EXO_BAND_TRI_DECL(tbsv_row_major_Upper_NonTrans)
+EXO_BAND_TRI_DECL(tbsv_row_major_Lower_NonTrans)
+EXO_BAND_TRI_DECL(tbsv_row_major_Upper_Trans)
+EXO_BAND_TRI_DECL(tbsv_row_major_Lower_Trans)
+#undef EXO_BAND_TRI_DECL
 // ---------------------------------------------------------------- level 3
 template <class T>
 void gemm(Ctx&, int64_t TransA, int64_t TransB, int64_t M, int64_t N, int64_t K, T alpha, const T* A,

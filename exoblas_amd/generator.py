@@ -166,6 +166,39 @@ PROCS: List[Proc] = [
          [enum("Uplo", UPLO_VALUES), enum("TransA", TRANS_VALUES), enum("Diag", DIAG_VALUES), size("n"),
           win("x", "n", const=False), win("A", "n", "n")],
          "src/level2/trsv.py:10-52", "test/level2/trsv/exo_trsv_wrapper.h:19-27"),
+    # ------------------------------------------------------------ banded level 2
+    Proc("gbmv_row_major_NonTrans", "gbmv",
+         [size("m"), size("n"), size("kl"), size("ku"), scalar("alpha"), scalar("beta"),
+          win("a", "m", "kl + ku + 1"), win("x", "n"), win("y", "m", const=False)],
+         "src/level2/gbmv.py:13-52", "test/level2/sgbmv/exo_sgbmv.h:9-19"),
+    Proc("sbmv_scal_y", "sbmv", [size("n"), scalar("beta"), win("y", "n", const=False)],
+         "src/level2/sbmv.py:12-16", "test/level2/ssbmv/exo_ssbmv.h:22-27"),
+    Proc("sbmv_row_major_Upper", "sbmv",
+         [size("n"), size("k"), scalar("alpha"), win("A", "n", "k + 1"), win("x", "n"),
+          win("y", "n", const=False)],
+         "src/level2/sbmv.py:19-33", "test/level2/ssbmv/exo_ssbmv.h:31-39"),
+    Proc("sbmv_row_major_Lower", "sbmv",
+         [size("n"), size("k"), scalar("alpha"), win("A", "n", "k + 1"), win("x", "n"),
+          win("y", "n", const=False)],
+         "src/level2/sbmv.py:36-50", "test/level2/ssbmv/exo_ssbmv.h:43-51"),
+] + [
+    Proc(f"tbmv_row_major_{uplo}_{tr}", "tbmv",
+         [size("n"), size("k"), win("x", "n", const=False), win("A", "n", "k + 1"), size("Diag")],
+         f"src/level2/tbmv.py:{lines}", f"test/level2/stbmv/exo_stbmv.h:{callers}")
+    for uplo, tr, lines, callers in (("Upper", "NonTrans", "12-43", "20-28"), ("Lower", "NonTrans", "46-77", "46-54"),
+                                     ("Upper", "Trans", "80-108", "32-40"), ("Lower", "Trans", "111-141", "58-66"))
+] + [
+    # the spec declares A as [n, n]; the band occupies n rows of k + 1 columns
+    Proc(f"tbsv_row_major_{uplo}_{tr}", "tbsv",
+         [size("n"), size("k"), win("x", "n", const=False), win("A", "n", "k + 1"), size("Diag")],
+         f"src/level2/tbsv.py:{lines}", f"test/level2/stbsv/exo_stbsv.h:{callers}",
+         variants=variants)
+    for uplo, tr, lines, callers, variants in (
+        ("Upper", "NonTrans", "12-45", "20-28", ("stride_any", "stride_1")),
+        ("Lower", "NonTrans", "48-81", "39-47", ("stride_any", "stride_1")),
+        ("Upper", "Trans", "84-105", "31-34", ("stride_any",)),
+        ("Lower", "Trans", "108-129", "50-53", ("stride_any",)))
+] + [
     # ------------------------------------------------------------ level 3
     Proc("gemm", "gemm",
          [enum("TransA", TRANS_VALUES), enum("TransB", TRANS_VALUES), size("M"), size("N"), size("K"),
@@ -251,7 +284,7 @@ EXT_PROCS: List[Proc] = [
 ]
 
 LIB_ORDER = ["asum", "axpy", "dot", "dsdot", "copy", "swap", "rot", "rotm", "scal",
-             "gemv", "ger", "symv", "syr", "syr2", "trmv", "trsv",
+             "gemv", "ger", "symv", "syr", "syr2", "trmv", "trsv", "gbmv", "sbmv", "tbmv", "tbsv",
              "gemm", "mscal", "symm", "syrk", "syr2k", "trmm"]
 
 

@@ -100,6 +100,50 @@ int exoblas_strsv_rm_stride_any_async(hipStream_t stream, int_fast32_t Uplo, int
 int exoblas_strsv_rm_stride_1_async(hipStream_t stream, int_fast32_t Uplo, int_fast32_t TransA, int_fast32_t Diag, int_fast32_t n, struct exo_win_1f32 x, struct exo_win_2f32c A);
 int exoblas_dtrsv_rm_stride_any_async(hipStream_t stream, int_fast32_t Uplo, int_fast32_t TransA, int_fast32_t Diag, int_fast32_t n, struct exo_win_1f64 x, struct exo_win_2f64c A);
 int exoblas_dtrsv_rm_stride_1_async(hipStream_t stream, int_fast32_t Uplo, int_fast32_t TransA, int_fast32_t Diag, int_fast32_t n, struct exo_win_1f64 x, struct exo_win_2f64c A);
+int exoblas_sgbmv_row_major_NonTrans_stride_any_async(hipStream_t stream, int_fast32_t m, int_fast32_t n, int_fast32_t kl, int_fast32_t ku, const float *alpha, const float *beta, struct exo_win_2f32c a, struct exo_win_1f32c x, struct exo_win_1f32 y);
+int exoblas_sgbmv_row_major_NonTrans_stride_1_async(hipStream_t stream, int_fast32_t m, int_fast32_t n, int_fast32_t kl, int_fast32_t ku, const float *alpha, const float *beta, struct exo_win_2f32c a, struct exo_win_1f32c x, struct exo_win_1f32 y);
+int exoblas_dgbmv_row_major_NonTrans_stride_any_async(hipStream_t stream, int_fast32_t m, int_fast32_t n, int_fast32_t kl, int_fast32_t ku, const double *alpha, const double *beta, struct exo_win_2f64c a, struct exo_win_1f64c x, struct exo_win_1f64 y);
+int exoblas_dgbmv_row_major_NonTrans_stride_1_async(hipStream_t stream, int_fast32_t m, int_fast32_t n, int_fast32_t kl, int_fast32_t ku, const double *alpha, const double *beta, struct exo_win_2f64c a, struct exo_win_1f64c x, struct exo_win_1f64 y);
+int exoblas_ssbmv_scal_y_stride_any_async(hipStream_t stream, int_fast32_t n, const float *beta, struct exo_win_1f32 y);
+int exoblas_ssbmv_scal_y_stride_1_async(hipStream_t stream, int_fast32_t n, const float *beta, struct exo_win_1f32 y);
+int exoblas_dsbmv_scal_y_stride_any_async(hipStream_t stream, int_fast32_t n, const double *beta, struct exo_win_1f64 y);
+int exoblas_dsbmv_scal_y_stride_1_async(hipStream_t stream, int_fast32_t n, const double *beta, struct exo_win_1f64 y);
+int exoblas_ssbmv_row_major_Upper_stride_any_async(hipStream_t stream, int_fast32_t n, int_fast32_t k, const float *alpha, struct exo_win_2f32c A, struct exo_win_1f32c x, struct exo_win_1f32 y);
+int exoblas_ssbmv_row_major_Upper_stride_1_async(hipStream_t stream, int_fast32_t n, int_fast32_t k, const float *alpha, struct exo_win_2f32c A, struct exo_win_1f32c x, struct exo_win_1f32 y);
+int exoblas_dsbmv_row_major_Upper_stride_any_async(hipStream_t stream, int_fast32_t n, int_fast32_t k, const double *alpha, struct exo_win_2f64c A, struct exo_win_1f64c x, struct exo_win_1f64 y);
+int exoblas_dsbmv_row_major_Upper_stride_1_async(hipStream_t stream, int_fast32_t n, int_fast32_t k, const double *alpha, struct exo_win_2f64c A, struct exo_win_1f64c x, struct exo_win_1f64 y);
+int exoblas_ssbmv_row_major_Lower_stride_any_async(hipStream_t stream, int_fast32_t n, int_fast32_t k, const float *alpha, struct exo_win_2f32c A, struct exo_win_1f32c x, struct exo_win_1f32 y);
+int exoblas_ssbmv_row_major_Lower_stride_1_async(hipStream_t stream, int_fast32_t n, int_fast32_t k, const float *alpha, struct exo_win_2f32c A, struct exo_win_1f32c x, struct exo_win_1f32 y);
+int exoblas_dsbmv_row_major_Lower_stride_any_async(hipStream_t stream, int_fast32_t n, int_fast32_t k, const double *alpha, struct exo_win_2f64c A, struct exo_win_1f64c x, struct exo_win_1f64 y);
+int exoblas_dsbmv_row_major_Lower_stride_1_async(hipStream_t stream, int_fast32_t n, int_fast32_t k, const double *alpha, struct exo_win_2f64c A, struct exo_win_1f64c x, struct exo_win_1f64 y);
+int exoblas_stbmv_row_major_Upper_NonTrans_stride_any_async(hipStream_t stream, int_fast32_t n, int_fast32_t k, struct exo_win_1f32 x, struct exo_win_2f32c A, int_fast32_t Diag);
+int exoblas_stbmv_row_major_Upper_NonTrans_stride_1_async(hipStream_t stream, int_fast32_t n, int_fast32_t k, struct exo_win_1f32 x, struct exo_win_2f32c A, int_fast32_t Diag);
+int exoblas_dtbmv_row_major_Upper_NonTrans_stride_any_async(hipStream_t stream, int_fast32_t n, int_fast32_t k, struct exo_win_1f64 x, struct exo_win_2f64c A, int_fast32_t Diag);
+int exoblas_dtbmv_row_major_Upper_NonTrans_stride_1_async(hipStream_t stream, int_fast32_t n, int_fast32_t k, struct exo_win_1f64 x, struct exo_win_2f64c A, int_fast32_t Diag);
+int exoblas_stbmv_row_major_Lower_NonTrans_stride_any_async(hipStream_t stream, int_fast32_t n, int_fast32_t k, struct exo_win_1f32 x, struct exo_win_2f32c A, int_fast32_t Diag);
+int exoblas_stbmv_row_major_Lower_NonTrans_stride_1_async(hipStream_t stream, int_fast32_t n, int_fast32_t k, struct exo_win_1f32 x, struct exo_win_2f32c A, int_fast32_t Diag);
+int exoblas_dtbmv_row_major_Lower_NonTrans_stride_any_async(hipStream_t stream, int_fast32_t n, int_fast32_t k, struct exo_win_1f64 x, struct exo_win_2f64c A, int_fast32_t Diag);
+int exoblas_dtbmv_row_major_Lower_NonTrans_stride_1_async(hipStream_t stream, int_fast32_t n, int_fast32_t k, struct exo_win_1f64 x, struct exo_win_2f64c A, int_fast32_t Diag);
+int exoblas_stbmv_row_major_Upper_Trans_stride_any_async(hipStream_t stream, int_fast32_t n, int_fast32_t k, struct exo_win_1f32 x, struct exo_win_2f32c A, int_fast32_t Diag);
+int exoblas_stbmv_row_major_Upper_Trans_stride_1_async(hipStream_t stream, int_fast32_t n, int_fast32_t k, struct exo_win_1f32 x, struct exo_win_2f32c A, int_fast32_t Diag);
+int exoblas_dtbmv_row_major_Upper_Trans_stride_any_async(hipStream_t stream, int_fast32_t n, int_fast32_t k, struct exo_win_1f64 x, struct exo_win_2f64c A, int_fast32_t Diag);
+int exoblas_dtbmv_row_major_Upper_Trans_stride_1_async(hipStream_t stream, int_fast32_t n, int_fast32_t k, struct exo_win_1f64 x, struct exo_win_2f64c A, int_fast32_t Diag);
+int exoblas_stbmv_row_major_Lower_Trans_stride_any_async(hipStream_t stream, int_fast32_t n, int_fast32_t k, struct exo_win_1f32 x, struct exo_win_2f32c A, int_fast32_t Diag);
+int exoblas_stbmv_row_major_Lower_Trans_stride_1_async(hipStream_t stream, int_fast32_t n, int_fast32_t k, struct exo_win_1f32 x, struct exo_win_2f32c A, int_fast32_t Diag);
+int exoblas_dtbmv_row_major_Lower_Trans_stride_any_async(hipStream_t stream, int_fast32_t n, int_fast32_t k, struct exo_win_1f64 x, struct exo_win_2f64c A, int_fast32_t Diag);
+int exoblas_dtbmv_row_major_Lower_Trans_stride_1_async(hipStream_t stream, int_fast32_t n, int_fast32_t k, struct exo_win_1f64 x, struct exo_win_2f64c A, int_fast32_t Diag);
+int exoblas_stbsv_row_major_Upper_NonTrans_stride_any_async(hipStream_t stream, int_fast32_t n, int_fast32_t k, struct exo_win_1f32 x, struct exo_win_2f32c A, int_fast32_t Diag);
+int exoblas_stbsv_row_major_Upper_NonTrans_stride_1_async(hipStream_t stream, int_fast32_t n, int_fast32_t k, struct exo_win_1f32 x, struct exo_win_2f32c A, int_fast32_t Diag);
+int exoblas_dtbsv_row_major_Upper_NonTrans_stride_any_async(hipStream_t stream, int_fast32_t n, int_fast32_t k, struct exo_win_1f64 x, struct exo_win_2f64c A, int_fast32_t Diag);
+int exoblas_dtbsv_row_major_Upper_NonTrans_stride_1_async(hipStream_t stream, int_fast32_t n, int_fast32_t k, struct exo_win_1f64 x, struct exo_win_2f64c A, int_fast32_t Diag);
+int exoblas_stbsv_row_major_Lower_NonTrans_stride_any_async(hipStream_t stream, int_fast32_t n, int_fast32_t k, struct exo_win_1f32 x, struct exo_win_2f32c A, int_fast32_t Diag);
+int exoblas_stbsv_row_major_Lower_NonTrans_stride_1_async(hipStream_t stream, int_fast32_t n, int_fast32_t k, struct exo_win_1f32 x, struct exo_win_2f32c A, int_fast32_t Diag);
+int exoblas_dtbsv_row_major_Lower_NonTrans_stride_any_async(hipStream_t stream, int_fast32_t n, int_fast32_t k, struct exo_win_1f64 x, struct exo_win_2f64c A, int_fast32_t Diag);
+int exoblas_dtbsv_row_major_Lower_NonTrans_stride_1_async(hipStream_t stream, int_fast32_t n, int_fast32_t k, struct exo_win_1f64 x, struct exo_win_2f64c A, int_fast32_t Diag);
+int exoblas_stbsv_row_major_Upper_Trans_stride_any_async(hipStream_t stream, int_fast32_t n, int_fast32_t k, struct exo_win_1f32 x, struct exo_win_2f32c A, int_fast32_t Diag);
+int exoblas_dtbsv_row_major_Upper_Trans_stride_any_async(hipStream_t stream, int_fast32_t n, int_fast32_t k, struct exo_win_1f64 x, struct exo_win_2f64c A, int_fast32_t Diag);
+int exoblas_stbsv_row_major_Lower_Trans_stride_any_async(hipStream_t stream, int_fast32_t n, int_fast32_t k, struct exo_win_1f32 x, struct exo_win_2f32c A, int_fast32_t Diag);
+int exoblas_dtbsv_row_major_Lower_Trans_stride_any_async(hipStream_t stream, int_fast32_t n, int_fast32_t k, struct exo_win_1f64 x, struct exo_win_2f64c A, int_fast32_t Diag);
 int exoblas_sgemm_stride_any_async(hipStream_t stream, int_fast32_t TransA, int_fast32_t TransB, int_fast32_t M, int_fast32_t N, int_fast32_t K, const float *alpha, struct exo_win_2f32c A, struct exo_win_2f32c B, struct exo_win_2f32c AT, struct exo_win_2f32c BT, struct exo_win_2f32 C);
 int exoblas_sgemm_stride_1_async(hipStream_t stream, int_fast32_t TransA, int_fast32_t TransB, int_fast32_t M, int_fast32_t N, int_fast32_t K, const float *alpha, struct exo_win_2f32c A, struct exo_win_2f32c B, struct exo_win_2f32c AT, struct exo_win_2f32c BT, struct exo_win_2f32 C);
 int exoblas_dgemm_stride_any_async(hipStream_t stream, int_fast32_t TransA, int_fast32_t TransB, int_fast32_t M, int_fast32_t N, int_fast32_t K, const double *alpha, struct exo_win_2f64c A, struct exo_win_2f64c B, struct exo_win_2f64c AT, struct exo_win_2f64c BT, struct exo_win_2f64 C);

@@ -95,6 +95,11 @@ def load(kind: str = "strict") -> ctypes.CDLL:
         d("symm", None, _I, _I, _I, _I, T, _P, _I, _P, _I, _P, _I)
         d("syrk", None, _I, _I, _I, _I, T, _P, _I, _P, _I)
         d("syr2k", None, _I, _I, _I, _I, T, _P, _I, _P, _I, _P, _I)
+        d("gbmv", None, _I, _I, _I, _I, T, T, _P, _I, _P, _I, _P, _I)
+        d("sbmv_scal_y", None, _I, T, _P, _I)
+        d("sbmv", None, _I, _I, _I, T, _P, _I, _P, _I, _P, _I)
+        d("tbmv", None, _I, _I, _I, _I, _P, _I, _P, _I, _I)
+        d("tbsv", None, _I, _I, _I, _I, _P, _I, _P, _I, _I)
         d("trmm", None, _I, _I, _I, _I, T, _P, _I, _P, _I, _I)
     lib.oracle_dsdot.restype = ctypes.c_double
     lib.oracle_dsdot.argtypes = [_I, _P, _I, _P, _I]
@@ -231,6 +236,32 @@ class Oracle:
     def trsv(self, uplo, trans, diag, n, x, incx, A, lda):
         self._chk(A, x)
         self.f("trsv")(uplo, trans, diag, n, vptr(x, n, incx), incx, mptr(A, n, n, lda), lda)
+
+    # banded level 2 ----------------------------------------------------
+    def gbmv(self, m, n, kl, ku, alpha, beta, A, lda, x, incx, y, incy):
+        self._chk(A, x, y)
+        self.f("gbmv")(m, n, kl, ku, alpha, beta, mptr(A, m, kl + ku + 1, lda), lda, vptr(x, n, incx), incx,
+                       vptr(y, m, incy), incy)
+
+    def sbmv_scal_y(self, n, beta, y, incy):
+        self._chk(y)
+        self.f("sbmv_scal_y")(n, beta, vptr(y, n, incy), incy)
+
+    def sbmv(self, uplo, n, k, alpha, A, lda, x, incx, y, incy):
+        """y += alpha S x (the proc after the wrapper's separate beta scaling)."""
+        self._chk(A, x, y)
+        self.f("sbmv")(int(uplo == UPPER), n, k, alpha, mptr(A, n, k + 1, lda), lda, vptr(x, n, incx), incx,
+                       vptr(y, n, incy), incy)
+
+    def tbmv(self, uplo, trans, diag, n, k, x, incx, A, lda):
+        self._chk(A, x)
+        self.f("tbmv")(int(uplo == UPPER), int(trans != NO_TRANS), n, k, vptr(x, n, incx), incx,
+                       mptr(A, n, k + 1, lda), lda, int(diag == UNIT))
+
+    def tbsv(self, uplo, trans, diag, n, k, x, incx, A, lda):
+        self._chk(A, x)
+        self.f("tbsv")(int(uplo == UPPER), int(trans != NO_TRANS), n, k, vptr(x, n, incx), incx,
+                       mptr(A, n, k + 1, lda), lda, int(diag == UNIT))
 
     # level 3 -----------------------------------------------------------
     def gemm(self, ta, tb, M, N, K, alpha, A, lda, B, ldb, C, ldc):

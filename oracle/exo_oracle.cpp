@@ -462,6 +462,153 @@ void syrk(int64_t uplo, int64_t trans, int64_t N, int64_t K, T alpha_, const T* 
 
 }  // namespace
 
+// ------------------------------------------------------------ banded level 2
+
+// src/level2/gbmv.py:13-52.  a[i, kl + (j - i)] holds A(i, j); the four
+// branches of the spec all sum the in-band columns of row i in increasing
+// order.  A row with no in-band column (possible only when m > n + kl, where
+// the spec leaves its sum unset) contributes 0.
+template <class T, class W>
+void gbmv(int64_t m, int64_t n, int64_t kl, int64_t ku, T alpha_, T beta_, const T* a_, int64_t lda,
+          const T* x_, int64_t incx, T* y_, int64_t incy) {
+  M2<T, W> a(a_, m, kl + ku + 1, lda, false);
+  V1<T, W> x(x_, n, incx, false);
+  V1<T, W> y(y_, m, incy, true);
+  W alpha = alpha_, beta = beta_;
+  for (int64_t i = 0; i < m; ++i) {
+    W result = 0;
+    const int64_t c0 = i - kl > 0 ? i - kl : 0, c1 = i + ku < n - 1 ? i + ku : n - 1;
+    for (int64_t c = c0; c <= c1; ++c) result += a(i, kl + c - i) * x[c];
+    y[i] = beta * y[i] + alpha * result;
+  }
+}
+
+// src/level2/sbmv.py:12-16
+template <class T, class W>
+void sbmv_scal_y(int64_t n, T beta_, T* y_, int64_t incy) {
+  V1<T, W> y(y_, n, incy, true);
+  W beta = beta_;
+  for (int64_t i = 0; i < n; ++i) y[i] = beta * y[i];
+}
+
+// src/level2/sbmv.py:19-33 (upper) and :36-50 (lower): y += alpha S x, S
+// symmetric with k off-diagonals, one triangle stored in band form.
+template <class T, class W>
+void sbmv(int64_t upper, int64_t n, int64_t k, T alpha_, const T* A_, int64_t lda, const T* x_, int64_t incx, T* y_,
+          int64_t incy) {
+  M2<T, W> A(A_, n, k + 1, lda, false);
+  V1<T, W> x(x_, n, incx, false);
+  V1<T, W> y(y_, n, incy, true);
+  W alpha = alpha_;
+  for (int64_t i = 0; i < n; ++i) {
+    const W temp = alpha * x[i];
+    W dot = 0;
+    for (int64_t j = 0; j < k; ++j) {
+      if (upper) {
+        if (i + j + 1 < n) {
+          y[i + j + 1] += temp * A(i, j + 1);
+          dot += A(i, j + 1) * x[i + j + 1];
+        }
+      } else if (i - j - 1 >= 0) {
+        y[i - j - 1] += temp * A(i, k - j - 1);
+        dot += A(i, k - j - 1) * x[i - j - 1];
+      }
+    }
+    y[i] += temp * A(i, upper ? 0 : k) + alpha * dot;
+  }
+}
+
+// src/level2/tbmv.py:12-141: x = op(A) x in place for a triangular band; the
+// diagonal (A[i, 0] upper, A[i, k] lower) is read only when diag == 0.
+template <class T, class W>
+void tbmv(int64_t upper, int64_t trans, int64_t n, int64_t k, T* x_, int64_t incx, const T* A_, int64_t lda,
+          int64_t diag) {
+  M2<T, W> A(A_, n, k + 1, lda, false);
+  V1<T, W> x(x_, n, incx, true);
+  if (!trans && upper) {  // :12-43, rows upwards from 0; reads rows below, not yet updated
+    for (int64_t i = 0; i < n - k; ++i) {
+      W dot = diag == 0 ? x[i] * A(i, 0) : x[i];
+      for (int64_t j = 0; j < k; ++j) dot += A(i, j + 1) * x[i + j + 1];
+      x[i] = dot;
+    }
+    for (int64_t i = 0; i < k; ++i) {
+      const int64_t r = n - k + i;
+      W dot = diag == 0 ? x[r] * A(r, 0) : x[r];
+      for (int64_t j = 0; j < k - i - 1; ++j) dot += A(r, j + 1) * x[r + j + 1];
+      x[r] = dot;
+    }
+  } else if (!trans) {  // :46-77, rows from n-1 down
+    for (int64_t i = 0; i < n; ++i) {
+      const int64_t r = n - i - 1;
+      const int64_t cnt = i < n - k ? k : k - (i - (n - k)) - 1;
+      W dot = 0;
+      for (int64_t j = 0; j < cnt; ++j) dot += A(r, k - j - 1) * x[r - j - 1];
+      dot += diag == 0 ? x[r] * A(r, k) : x[r];
+      x[r] = dot;
+    }
+  } else {  // :80-141, scatter into a result vector, then copy back
+    std::vector<W> res(n > 0 ? n : 1, W(0));
+    if (upper) {
+      for (int64_t i = 0; i < n; ++i) {
+        const int64_t cnt = i < n - k ? k : k - (i - (n - k)) - 1;
+        res[i] += diag == 0 ? x[i] * A(i, 0) : x[i];
+        for (int64_t j = 0; j < cnt; ++j) res[i + j + 1] += A(i, j + 1) * x[i];
+      }
+    } else {
+      for (int64_t i = 0; i < n; ++i) {
+        const int64_t r = n - i - 1;
+        const int64_t cnt = i < n - k ? k : k - (i - (n - k)) - 1;
+        for (int64_t j = 0; j < cnt; ++j) res[r - j - 1] += A(r, k - j - 1) * x[r];
+        res[r] += diag == 0 ? x[r] * A(r, k) : x[r];
+      }
+    }
+    for (int64_t i = 0; i < n; ++i) x[i] = res[i];
+  }
+}
+
+// src/level2/tbsv.py:12-129: op(A) x = b in place for a triangular band.
+template <class T, class W>
+void tbsv(int64_t upper, int64_t trans, int64_t n, int64_t k, T* x_, int64_t incx, const T* A_, int64_t lda,
+          int64_t diag) {
+  M2<T, W> A(A_, n, k + 1, lda, false);
+  V1<T, W> x(x_, n, incx, true);
+  if (!trans && upper) {  // :12-45, rows n-1 down to 0
+    for (int64_t s = 0; s < n; ++s) {
+      const int64_t r = n - s - 1;
+      const int64_t cnt = s < k ? s : k;
+      const W pivot = diag == 0 ? A(r, 0) : W(1);
+      W dot = 0;
+      for (int64_t j = 0; j < cnt; ++j) dot += A(r, j + 1) * x[r + 1 + j];
+      x[r] = (x[r] - dot) / pivot;
+    }
+  } else if (!trans) {  // :48-81, rows 0 up to n-1
+    for (int64_t i = 0; i < n; ++i) {
+      const int64_t cnt = i < k ? i : k;
+      W dot = 0;
+      for (int64_t j = 0; j < cnt; ++j) dot += A(i, k - j - 1) * x[i - j - 1];
+      const W pivot = diag == 0 ? A(i, k) : W(1);
+      x[i] = (x[i] - dot) / pivot;
+    }
+  } else if (upper) {  // :84-105, rows 0 up to n-1
+    for (int64_t i = 0; i < n; ++i) {
+      W dot = 0;
+      for (int64_t j = 0; j < k; ++j)
+        if (i - j - 1 >= 0) dot += A(i - j - 1, j + 1) * x[i - j - 1];
+      const W pivot = diag == 0 ? A(i, 0) : W(1);
+      x[i] = (x[i] - dot) / pivot;
+    }
+  } else {  // :108-129, rows n-1 down to 0
+    for (int64_t i = 0; i < n; ++i) {
+      const int64_t r = n - i - 1;
+      const W pivot = diag == 0 ? A(r, k) : W(1);
+      W dot = 0;
+      for (int64_t j = 0; j < k; ++j)
+        if (n - i + j < n) dot += A(n - i + j, k - j - 1) * x[n - i + j];
+      x[r] = (x[r] - dot) / pivot;
+    }
+  }
+}
+
 // src/level3/syr2k.py:14-39.  A and B are N x K (NoTrans) or K x N (Trans).
 template <class T, class W>
 void syr2k(int64_t uplo, int64_t trans, int64_t N, int64_t K, T alpha_, const T* A_, int64_t lda, const T* B_,
@@ -597,6 +744,25 @@ void trmm(int64_t upper, int64_t trans, int64_t m, int64_t n, T alpha_, const T*
   extern "C" void oracle_##P##syr2k(int64_t up, int64_t tr, int64_t N, int64_t K, T a, const T* A,  \
                                     int64_t lda, const T* B, int64_t ldb, T* C, int64_t ldc) {      \
     syr2k<T, W>(up, tr, N, K, a, A, lda, B, ldb, C, ldc);                                           \
+  }                                                                                                 \
+  extern "C" void oracle_##P##gbmv(int64_t m, int64_t n, int64_t kl, int64_t ku, T a, T b,          \
+                                   const T* A, int64_t lda, const T* x, int64_t ix, T* y, int64_t iy) { \
+    gbmv<T, W>(m, n, kl, ku, a, b, A, lda, x, ix, y, iy);                                            \
+  }                                                                                                 \
+  extern "C" void oracle_##P##sbmv_scal_y(int64_t n, T b, T* y, int64_t iy) {                       \
+    sbmv_scal_y<T, W>(n, b, y, iy);                                                                 \
+  }                                                                                                 \
+  extern "C" void oracle_##P##sbmv(int64_t up, int64_t n, int64_t k, T a, const T* A, int64_t lda,  \
+                                   const T* x, int64_t ix, T* y, int64_t iy) {                      \
+    sbmv<T, W>(up, n, k, a, A, lda, x, ix, y, iy);                                                  \
+  }                                                                                                 \
+  extern "C" void oracle_##P##tbmv(int64_t up, int64_t tr, int64_t n, int64_t k, T* x, int64_t ix,  \
+                                   const T* A, int64_t lda, int64_t diag) {                         \
+    tbmv<T, W>(up, tr, n, k, x, ix, A, lda, diag);                                                  \
+  }                                                                                                 \
+  extern "C" void oracle_##P##tbsv(int64_t up, int64_t tr, int64_t n, int64_t k, T* x, int64_t ix,  \
+                                   const T* A, int64_t lda, int64_t diag) {                         \
+    tbsv<T, W>(up, tr, n, k, x, ix, A, lda, diag);                                                  \
   }                                                                                                 \
   extern "C" void oracle_##P##trmm(int64_t upper, int64_t trans, int64_t m, int64_t n, T a,         \
                                    const T* A, int64_t lda, T* B, int64_t ldb, int64_t diag) {      \

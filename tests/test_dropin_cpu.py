@@ -25,7 +25,11 @@ WRAPPERS = {
     "level2": [("gemv", "exo_gemv_wrapper.h", "exo_sgemv"), ("ger", "exo_ger_wrapper.h", "exo_dger"),
                ("symv", "exo_symv_wrapper.h", "exo_ssymv"), ("syr", "exo_syr_wrapper.h", "exo_dsyr"),
                ("syr2", "exo_syr2_wrapper.h", "exo_ssyr2"), ("trmv", "exo_trmv_wrapper.h", "exo_dtrmv"),
-               ("trsv", "exo_trsv_wrapper.h", "exo_strsv")],
+               ("trsv", "exo_trsv_wrapper.h", "exo_strsv"), ("sgbmv", "exo_sgbmv.h", "exo_sgbmv"),
+               ("dgbmv", "exo_dgbmv.h", "exo_dgbmv"), ("ssbmv", "exo_ssbmv.h", "exo_ssbmv"),
+               ("dsbmv", "exo_dsbmv.h", "exo_dsbmv"), ("stbmv", "exo_stbmv.h", "exo_stbmv"),
+               ("dtbmv", "exo_dtbmv.h", "exo_dtbmv"), ("stbsv", "exo_stbsv.h", "exo_stbsv"),
+               ("dtbsv", "exo_dtbsv.h", "exo_dtbsv")],
     "level3": [("gemm", "exo_gemm_wrapper.h", "exo_sgemm"), ("symm", "exo_symm_wrapper.h", "exo_dsymm"),
                ("syrk", "exo_syrk_wrapper.h", "exo_ssyrk"), ("syr2k", "exo_syr2k_wrapper.h", "exo_dsyr2k"),
                ("strmm", "exo_strmm.h", "exo_strmm")],

@@ -439,3 +439,129 @@ def test_trmm(dt, uplo, trans, diag, m, n, ld_diff, rng):
     if m > 1:
         with pytest.raises(AssertionError):
             assert_bound(BB[:, :n], wrong, mag, m, dt, c=1.0)
+
+
+# ---------------------------------------------------------------- banded level 2
+# Dense matrices from the row-major band layout (CBLAS definition, written here
+# independently of the spec's loops): general A(i, j) = a[i, kl + j - i];
+# upper-stored A(i, j>=i) = a[i, j - i]; lower-stored A(i, j<=i) = a[i, k + j - i].
+
+def _band_dense(a, m, n, lo, hi, c0):
+    D = np.zeros((m, n))
+    for i in range(m):
+        for j in range(max(0, i - lo), min(n, i + hi + 1)):
+            D[i, j] = a[i, c0 + j - i]
+    return D
+
+
+def _fortran_band(D, kl, ku):
+    """Column-major band array of netlib BLAS: ab[ku + i - j, j] = D(i, j)."""
+    m, n = D.shape
+    ab = np.zeros((kl + ku + 1, n))
+    for j in range(n):
+        for i in range(max(0, j - ku), min(m, j + kl + 1)):
+            ab[ku + i - j, j] = D[i, j]
+    return ab
+
+
+@pytest.mark.parametrize("dt", DT)
+@pytest.mark.parametrize("m,n,kl,ku", [(2, 2, 1, 1), (15, 20, 0, 5), (40, 33, 7, 3), (29, 29, 28, 28)])
+def test_gbmv(dt, m, n, kl, ku, rng):
+    lda = kl + ku + 1
+    a = uniform(rng, (m, lda), dt)
+    x, y = uniform(rng, n, dt), uniform(rng, m, dt)
+    yy = y.copy()
+    O(dt).gbmv(m, n, kl, ku, 1.2, 1.2, a, lda, x, 1, yy, 1)
+    D = _band_dense(a.astype(np.float64), m, n, kl, ku, kl)
+    ref = 1.2 * y + 1.2 * (D @ x)
+    mag = 1.2 * np.abs(y) + 1.2 * (np.abs(D) @ np.abs(x.astype(np.float64)))
+    assert_bound(yy, ref, mag, kl + ku + 1, dt, what="gbmv")
+    p = "s" if dt == np.float32 else "d"
+    if m >= kl + ku + 1:  # scipy's binding requires it
+        lib = getattr(sb, p + "gbmv")(m, n, kl, ku, 1.2, _fortran_band(D, kl, ku).astype(dt), x, beta=1.2,
+                                      y=y.copy())
+        assert_bound(yy, lib, mag, kl + ku + 1, dt, what="gbmv/scipy")
+    with pytest.raises(AssertionError):  # planted: the band read one column off
+        Dw = _band_dense(np.roll(a.astype(np.float64), 1, axis=1), m, n, kl, ku, kl)
+        assert_bound(yy, 1.2 * y + 1.2 * (Dw @ x), mag, kl + ku + 1, dt, c=1.0)
+
+
+@pytest.mark.parametrize("dt", DT)
+@pytest.mark.parametrize("uplo", [UPPER, LOWER])
+@pytest.mark.parametrize("n,k,inc", [(2, 1, 1), (100, 0, 1), (100, 7, -2), (33, 32, 3)])
+def test_sbmv(dt, uplo, n, k, inc, rng):
+    A = uniform(rng, (n, k + 1 + 3), dt)
+    x, y = vec(rng, n, inc, dt), vec(rng, n, inc, dt)
+    yy = y.copy()
+    o = O(dt)
+    o.sbmv_scal_y(n, 0.5, yy, inc)
+    o.sbmv(uplo, n, k, 2.4, A, A.shape[1], x, inc, yy, inc)
+    A64 = A.astype(np.float64)
+    T = _band_dense(A64, n, n, 0, k, 0) if uplo == UPPER else _band_dense(A64, n, n, k, 0, k)
+    S = T + T.T - np.diag(np.diag(T))
+    xe, ye = elems(x, n, inc).astype(np.float64), elems(y, n, inc).astype(np.float64)
+    ref = 0.5 * ye + 2.4 * (S @ xe)
+    mag = 0.5 * np.abs(ye) + 2.4 * (np.abs(S) @ np.abs(xe))
+    assert_bound(elems(yy, n, inc), ref, mag, 2 * k + 2, dt, what="sbmv")
+    p = "s" if dt == np.float32 else "d"
+    ab = _fortran_band(T, 0, k)[:, :] if uplo == UPPER else _fortran_band(T, k, 0)
+    lib = getattr(sb, p + "sbmv")(k, 2.4, ab.astype(dt), xe.astype(dt), beta=0.5, y=ye.astype(dt),
+                                  lower=int(uplo == LOWER))
+    assert_bound(elems(yy, n, inc), lib, mag, 2 * k + 2, dt, what="sbmv/scipy")
+    if k > 0:
+        with pytest.raises(AssertionError):  # planted: the stored triangle only (no mirror)
+            assert_bound(elems(yy, n, inc), 0.5 * ye + 2.4 * (T @ xe), mag, 2 * k + 2, dt, c=1.0)
+
+
+def _tri_band(rng, n, k, uplo, dt, dominant):
+    A = uniform(rng, (n, k + 1 + 2), dt)
+    if dominant:
+        A[:, :k + 1] /= (k + 1)
+        dcol = 0 if uplo == UPPER else k
+        A[:, dcol] = (1.0 + rng.uniform(0, 1, n)).astype(dt)
+    A64 = A.astype(np.float64)
+    T = _band_dense(A64, n, n, 0, k, 0) if uplo == UPPER else _band_dense(A64, n, n, k, 0, k)
+    return A, T
+
+
+@pytest.mark.parametrize("dt", DT)
+@pytest.mark.parametrize("uplo", [UPPER, LOWER])
+@pytest.mark.parametrize("trans", [NO_TRANS, TRANS])
+@pytest.mark.parametrize("diag", [NON_UNIT, UNIT])
+@pytest.mark.parametrize("n,k,inc", [(2, 1, 1), (100, 6, 3), (57, 56, -1)])
+def test_tbmv_tbsv(dt, uplo, trans, diag, n, k, inc, rng):
+    p = "s" if dt == np.float32 else "d"
+    o = O(dt)
+    A, T = _tri_band(rng, n, k, uplo, dt, dominant=True)
+    Tu = T.copy()
+    if diag == UNIT:
+        np.fill_diagonal(Tu, 1.0)
+    opT = Tu if trans == NO_TRANS else Tu.T
+    lower = int(uplo == LOWER)
+    ab = (_fortran_band(T, 0, k) if uplo == UPPER else _fortran_band(T, k, 0)).astype(dt)
+    tr = int(trans != NO_TRANS)
+    # tbmv
+    x = vec(rng, n, inc, dt)
+    xx = x.copy()
+    o.tbmv(uplo, trans, diag, n, k, xx, inc, A, A.shape[1])
+    xe = elems(x, n, inc).astype(np.float64)
+    mag = np.abs(opT) @ np.abs(xe)
+    assert_bound(elems(xx, n, inc), opT @ xe, mag, k + 1, dt, what="tbmv")
+    lib = getattr(sb, p + "tbmv")(k, ab, xe.astype(dt), lower=lower, trans=tr, diag=int(diag == UNIT))
+    assert_bound(elems(xx, n, inc), lib, mag, k + 1, dt, what="tbmv/scipy")
+    if k > 0:
+        with pytest.raises(AssertionError):  # planted: the transposed band
+            assert_bound(elems(xx, n, inc), opT.T @ xe, mag, k + 1, dt, c=1.0)
+    # tbsv: solve, then the residual of the float64 system is small
+    b = vec(rng, n, inc, dt)
+    bb = b.copy()
+    o.tbsv(uplo, trans, diag, n, k, bb, inc, A, A.shape[1])
+    be = elems(b, n, inc).astype(np.float64)
+    want = np.linalg.solve(opT, be)
+    got = elems(bb, n, inc).astype(np.float64)
+    assert np.max(np.abs(got - want)) <= 8 * (k + 2) * eps(dt) * (np.max(np.abs(want)) + 1), "tbsv"
+    lib = getattr(sb, p + "tbsv")(k, ab, be.astype(dt), lower=lower, trans=tr, diag=int(diag == UNIT))
+    assert np.max(np.abs(got - lib)) <= 8 * (k + 2) * eps(dt) * (np.max(np.abs(want)) + 1), "tbsv/scipy"
+    if k > 0:  # planted: solving with the transposed band misses
+        wrong = np.linalg.solve(opT.T, be)
+        assert np.max(np.abs(got - wrong)) > 8 * (k + 2) * eps(dt) * (np.max(np.abs(want)) + 1)
