@@ -453,6 +453,89 @@ def bench_trsv(a, torch, so, L):
                  "GB/s", a, tot_ms, "f32+f64", "trsv: s/d lower no-trans non-unit, n = 4096 and 8192", roof, kernels)
 
 
+def bench_extended(a, torch, so, L):
+    """The reference's other library targets: s/d syr2k lower-N N=4096 K=1024
+    (2KN(N+1) flop), strmm Left-Lower-NoTrans-NonUnit 4096 x 4096 (M(M+1)N
+    flop; B restored before each timed call, outside the timed region), and
+    the banded level-2 kernels on n = 2^22 rows (s/d gbmv kl = ku = 16, sbmv
+    upper k = 16, tbmv upper no-trans k = 16; band + vector bytes) and tbsv
+    (upper no-trans, k = 16, n = 2^16, a latency-bound chain)."""
+    stream = torch.cuda.current_stream()
+    sh = stream.cuda_stream
+    g = torch.Generator(device="cuda")
+    g.manual_seed(0)
+    kernels = {}
+
+    def u(shape, dt):
+        return torch.empty(shape, dtype=dt, device="cuda").uniform_(-1, 1, generator=g)
+
+    def run(name, fn, work, unit, peak, reps, before=None):
+        for _ in range(2):
+            if before is not None:
+                before()
+            fn()
+        ms = _timed(torch, stream, fn, reps, before=before)
+        rate = work / (ms * 1e-3) / (1e12 if unit == "TFLOP/s" else 1e9)
+        kernels[name] = {"us": round(ms * 1e3, 2), unit: round(rate, 2 if unit == "TFLOP/s" else 1),
+                         "frac": round(rate / peak, 4), ("flop" if unit == "TFLOP/s" else "bytes"): work}
+
+    for p, dt, s, peak in (("s", torch.float32, 4, FP32_PEAK_TF), ("d", torch.float64, 8, FP64_PEAK_TF)):
+        T = ctypes.c_float if p == "s" else ctypes.c_double
+        al, be = ctypes.byref(T(1.5)), ctypes.byref(T(0.5))
+        N, K = 4096, 1024
+        A, B, Cm = u((N, K), dt), u((N, K), dt), u((N, N), dt)
+        f = getattr(so, f"exoblas_{p}syr2k_rm_stride_1_async")
+        wa, wb = L.win2(A.data_ptr(), K), L.win2(B.data_ptr(), K)
+        run(f"{p}syr2k_LN_4096x1024",
+            lambda f=f, wa=wa, wb=wb, Cm=Cm, al=al: f(sh, 122, 111, N, K, al, wa, wa, wb, wb, wa, wa, wb, wb,
+                                                      L.win2(Cm.data_ptr(), N)),
+            2.0 * K * N * (N + 1), "TFLOP/s", peak, 5)
+        n, k = 1 << 22, 16
+        Ab = u((n, 2 * k + 1), dt)
+        x, y = u((n,), dt), u((n,), dt)
+        f = getattr(so, f"exoblas_{p}gbmv_row_major_NonTrans_stride_1_async")
+        run(f"{p}gbmv_{n}_kl{k}_ku{k}",
+            lambda f=f, Ab=Ab, x=x, y=y: f(sh, n, n, k, k, al, be, L.win2(Ab.data_ptr(), 2 * k + 1),
+                                           L.win1(x.data_ptr(), 1), L.win1(y.data_ptr(), 1)),
+            (n * (2 * k + 1) + n + 2 * n) * s, "GB/s", HBM_PEAK_GBS, 20)
+        As = u((n, k + 1), dt)
+        f = getattr(so, f"exoblas_{p}sbmv_row_major_Upper_stride_1_async")
+        run(f"{p}sbmv_U_{n}_k{k}",
+            lambda f=f, As=As, x=x, y=y: f(sh, n, k, al, L.win2(As.data_ptr(), k + 1), L.win1(x.data_ptr(), 1),
+                                           L.win1(y.data_ptr(), 1)),
+            (n * (k + 1) + n + 2 * n) * s, "GB/s", HBM_PEAK_GBS, 20)
+        At = As.clone()
+        At[:, 0] = 1.0
+        xt = x.clone()
+        f = getattr(so, f"exoblas_{p}tbmv_row_major_Upper_NonTrans_stride_1_async")
+        run(f"{p}tbmv_UN_{n}_k{k}",
+            lambda f=f, At=At, xt=xt: f(sh, n, k, L.win1(xt.data_ptr(), 1), L.win2(At.data_ptr(), k + 1), 0),
+            (n * (k + 1) + 2 * n) * s, "GB/s", HBM_PEAK_GBS, 20, before=lambda xt=xt, x=x: xt.copy_(x))
+        ns = 1 << 16
+        Ad = u((ns, k + 1), dt)
+        Ad[:, 0] = k + 1 + torch.empty(ns, dtype=dt, device="cuda").uniform_(0, 1, generator=g)
+        b, xs = u((ns,), dt), torch.empty(ns, dtype=dt, device="cuda")
+        f = getattr(so, f"exoblas_{p}tbsv_row_major_Upper_NonTrans_stride_1_async")
+        run(f"{p}tbsv_UN_{ns}_k{k}",
+            lambda f=f, Ad=Ad, xs=xs: f(sh, ns, k, L.win1(xs.data_ptr(), 1), L.win2(Ad.data_ptr(), k + 1), 0),
+            (ns * (k + 1) + 2 * ns) * s, "GB/s", HBM_PEAK_GBS, 5, before=lambda xs=xs, b=b: xs.copy_(b))
+    M = 4096
+    At, B0 = u((M, M), torch.float32), u((M, M), torch.float32)
+    Bw = B0.clone()
+    run("strmm_LLNN_4096",
+        lambda: so.exoblas_strmm_row_major_Left_Lower_NonTrans_async(sh, M, M, ctypes.byref(ctypes.c_float(1.0)),
+                                                                     L.win2(At.data_ptr(), M),
+                                                                     L.win2(Bw.data_ptr(), M), 0),
+        float(M) * (M + 1) * M, "TFLOP/s", FP32_PEAK_TF, 5, before=lambda: Bw.copy_(B0))
+    k = kernels["ssyr2k_LN_4096x1024"]
+    roof = {"bound": "mfma", "kernel": "ssyr2k_LN_4096x1024", "achieved": k["TFLOP/s"], "peak": FP32_PEAK_TF,
+            "unit": "TFLOP/s", "frac": k["frac"], "traffic": None}
+    tot_ms = sum(v["us"] for v in kernels.values()) / 1e3
+    return _line("TFLOP/s (syr2k, trmm) and GB/s (banded) per kernel", k["TFLOP/s"], "TFLOP/s", a, tot_ms,
+                 "f32+f64", "extended: s/d syr2k L-N 4096x1024, strmm L-L-N-N 4096^2, s/d gbmv/sbmv/tbmv n=2^22 "
+                 "k=16, s/d tbsv n=2^16 k=16", roof, kernels)
+
+
 def bench_precision(a, torch, so, L):
     """sgemm NN 8192^3 and ssyrk L-N 8192 x 2048: plain FP32 MFMA vs split
     BF16 (3 and 6 products); max error against float64 on U[-1,1] and on rows
@@ -675,7 +758,7 @@ def bench_hostpath(a, torch, so, L):
                  "sgemm NN 512^3 (mscal + gemm), strsv L-N-N 4096", None, kernels)
 
 
-WORKLOADS = {"bandwidth": bench_bandwidth, "level3": bench_level3, "trsv": bench_trsv,
+WORKLOADS = {"bandwidth": bench_bandwidth, "level3": bench_level3, "trsv": bench_trsv, "extended": bench_extended,
              "precision": bench_precision, "serving": bench_serving, "hostpath": bench_hostpath}
 
 

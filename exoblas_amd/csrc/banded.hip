@@ -10,9 +10,13 @@
 // each read row-wise (A[i, c0 + d]: contiguous per row) or column-wise
 // (A[j, c0 - d]: one element per stored row).
 //
-// band_mv_kernel: a group of G lanes per output row (G = 1, 4, 16 or 64 by
-// band width) sums its row's products lane-strided and reduces with xor
-// shuffles in a fixed order (deterministic).
+// band_lds_kernel (bands whose rows fit in LDS): a workgroup owns 256
+// consecutive output rows, copies the band rows they touch (their own and, for
+// column-wise segments, the k rows either side) and the x window into LDS with
+// coalesced loads, then each thread sums one row from LDS in offset order.
+// band_mv_kernel (wider bands): a group of G lanes per output row (G = 1, 4,
+// 16 or 64 by band width) sums its row's products lane-strided and reduces
+// with xor shuffles in a fixed order.  Both are deterministic.
 // tbsv_kernel: one wave walks the rows in dependency order; lane l handles
 // offsets d = 1 + l, 1 + l + 64, ...; the last k solutions live in an LDS
 // ring, so each step is one wave reduction and one division.
@@ -77,10 +81,112 @@ __global__ __launch_bounds__(kNT) void band_mv_kernel(int64_t rows, int64_t ncol
   }
 }
 
+// Rows [i0, i0 + kNT) of the band product with the band rows [rs0, rs1) and
+// x[xs0 .. xs1) staged in LDS; band row r occupies LDS [(r - rs0) * ws, +wc).
+// When lda == wc the staged rows are one contiguous range copied as is (ws ==
+// wc, no index arithmetic); otherwise ws = wc rounded up to odd, which keeps
+// the row-per-thread reads free of bank conflicts.
 template <class T>
-void launch_band_mv(Ctx& c, int64_t rows, int64_t ncols, Seg s0, Seg s1, int nseg, const T* A, int64_t lda,
-                    const T* x, int64_t incx, T* y, int64_t incy, T alpha, T beta, int out, bool unit) {
+__global__ __launch_bounds__(kNT) void band_lds_kernel(int64_t rows, int64_t ncols, int64_t srows, Seg s0, Seg s1,
+                                                       int nseg, int64_t wc, int64_t ws, int64_t reach_lo,
+                                                       int64_t reach_hi, int64_t xlo, int64_t xhi,
+                                                       const T* __restrict__ A, int64_t lda,
+                                                       const T* __restrict__ x, int64_t incx, T* y, int64_t incy,
+                                                       T alpha, T beta, int out, int unit) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int64_t i0 = (int64_t)blockIdx.x * kNT;
+  const int64_t rs0 = i0 + reach_lo > 0 ? i0 + reach_lo : 0;
+  const int64_t rs1 = i0 + kNT + reach_hi < srows ? i0 + kNT + reach_hi : srows;
+  const int64_t nr = rs1 > rs0 ? rs1 - rs0 : 0;
+  T* band = reinterpret_cast<T*>(smem);
+  T* xs = band + (kNT + reach_hi - reach_lo) * ws;
+  const int64_t xs0 = i0 + xlo > 0 ? i0 + xlo : 0;
+  const int64_t xs1 = i0 + kNT + xhi < ncols ? i0 + kNT + xhi : ncols;
+  // global -> LDS, 8 loads in flight per thread (LDS offsets fit 32 bits)
+  const uint32_t E = (uint32_t)(nr * wc), wcu = (uint32_t)wc, wsu = (uint32_t)ws;
+  if (lda == wc) {
+    const T* src = A + rs0 * lda;
+    for (uint32_t e0 = threadIdx.x; e0 < E; e0 += 8 * kNT) {
+      T v[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u)
+        if (e0 + u * kNT < E) v[u] = src[e0 + u * kNT];
+#pragma unroll
+      for (int u = 0; u < 8; ++u)
+        if (e0 + u * kNT < E) band[e0 + u * kNT] = v[u];
+    }
+  } else {
+    for (uint32_t e0 = threadIdx.x; e0 < E; e0 += 8 * kNT) {
+      T v[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const uint32_t e = e0 + u * kNT, r = e / wcu;
+        if (e < E) v[u] = A[(rs0 + r) * lda + (e - r * wcu)];
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const uint32_t e = e0 + u * kNT, r = e / wcu;
+        if (e < E) band[r * wsu + (e - r * wcu)] = v[u];
+      }
+    }
+  }
+  for (int64_t j = xs0 + threadIdx.x; j < xs1; j += kNT) xs[j - xs0] = x[j * incx];
+  __syncthreads();
+  const int64_t i = i0 + threadIdx.x;
+  if (i >= rows) return;
+  const int ii = (int)(i - rs0), jx = (int)(i - xs0);  // this row's LDS row and x offset
+  T s = T(0);
+#pragma unroll
+  for (int q = 0; q < 2; ++q) {
+    if (q >= nseg) break;
+    const Seg sg = q == 0 ? s0 : s1;
+    const int dlo = (int)(sg.lo > -i ? sg.lo : -i);
+    const int dhi = (int)(sg.hi < ncols - 1 - i ? sg.hi : ncols - 1 - i);
+    const int c0 = (int)sg.c0, w = (int)ws;
+    for (int d = dlo; d <= dhi; ++d) {
+      T a;
+      if (unit && d == 0)
+        a = T(1);
+      else
+        a = sg.col ? band[(ii + d) * w + (c0 - d)] : band[ii * w + (c0 + d)];
+      s += a * xs[jx + d];
+    }
+  }
+  T* yi = y + i * incy;
+  if (out == kGbmv)
+    *yi = beta * *yi + alpha * s;
+  else if (out == kAccumulate)
+    *yi += alpha * s;
+  else
+    *yi = s;
+}
+
+// srows: rows of the band storage; wc: its columns in use.
+template <class T>
+void launch_band_mv(Ctx& c, int64_t rows, int64_t ncols, int64_t srows, int64_t wc, Seg s0, Seg s1, int nseg,
+                    const T* A, int64_t lda, const T* x, int64_t incx, T* y, int64_t incy, T alpha, T beta, int out,
+                    bool unit) {
   if (rows <= 0) return;
+  // rows the column-wise segments reach beyond the block, and the x window
+  int64_t reach_lo = 0, reach_hi = 0, xlo = s0.lo, xhi = s0.hi;
+  for (int q = 0; q < nseg; ++q) {
+    const Seg& sg = q == 0 ? s0 : s1;
+    if (sg.col) {
+      reach_lo = std::min(reach_lo, sg.lo);
+      reach_hi = std::max(reach_hi, sg.hi);
+    }
+    xlo = std::min(xlo, sg.lo);
+    xhi = std::max(xhi, sg.hi);
+  }
+  const int64_t ws = lda == wc ? wc : (wc | 1);
+  const size_t lds = sizeof(T) * (size_t)((kNT + reach_hi - reach_lo) * ws + (kNT + xhi - xlo));
+  if (lds <= (64 << 10)) {
+    hipLaunchKernelGGL((band_lds_kernel<T>), dim3((unsigned)ceil_div(rows, kNT)), dim3(kNT), lds, c.stream, rows,
+                       ncols, srows, s0, s1, nseg, wc, ws, reach_lo, reach_hi, xlo, xhi, A, lda, x, incx, y, incy,
+                       alpha, beta, out, (int)unit);
+    EXO_LAUNCH_CHECK();
+    return;
+  }
   int64_t w = s0.hi - s0.lo + 1 + (nseg > 1 ? s1.hi - s1.lo + 1 : 0);
 #define EXO_BAND(GG)                                                                                       \
   hipLaunchKernelGGL((band_mv_kernel<T, GG>), dim3((unsigned)ceil_div(rows, kNT / GG)), dim3(kNT), 0, c.stream, \
@@ -169,7 +275,7 @@ template <class T>
 void gbmv_row_major_NonTrans(Ctx& c, int64_t m, int64_t n, int64_t kl, int64_t ku, T alpha, T beta, const T* a,
                              int64_t lda, const T* x, int64_t incx, T* y, int64_t incy) {
   const Seg s{-kl, ku, kl, 0};
-  launch_band_mv<T>(c, m, n, s, s, 1, a, lda, x, incx, y, incy, alpha, beta, kGbmv, false);
+  launch_band_mv<T>(c, m, n, m, kl + ku + 1, s, s, 1, a, lda, x, incx, y, incy, alpha, beta, kGbmv, false);
 }
 
 // src/level2/sbmv.py:12-16  y = beta*y (the wrapper calls it when beta != 1)
@@ -183,16 +289,16 @@ void sbmv_scal_y(Ctx& c, int64_t n, T beta, T* y, int64_t incy) {
 template <class T>
 void sbmv_row_major_Upper(Ctx& c, int64_t n, int64_t k, T alpha, const T* A, int64_t lda, const T* x, int64_t incx,
                           T* y, int64_t incy) {
-  launch_band_mv<T>(c, n, n, Seg{0, k, 0, 0}, Seg{-k, -1, 0, 1}, k > 0 ? 2 : 1, A, lda, x, incx, y, incy, alpha,
-                    T(0), kAccumulate, false);
+  launch_band_mv<T>(c, n, n, n, k + 1, Seg{0, k, 0, 0}, Seg{-k, -1, 0, 1}, k > 0 ? 2 : 1, A, lda, x, incx, y,
+                    incy, alpha, T(0), kAccumulate, false);
 }
 
 // src/level2/sbmv.py:36-50.  S(i, j<=i) = A[i, k + (j - i)].
 template <class T>
 void sbmv_row_major_Lower(Ctx& c, int64_t n, int64_t k, T alpha, const T* A, int64_t lda, const T* x, int64_t incx,
                           T* y, int64_t incy) {
-  launch_band_mv<T>(c, n, n, Seg{-k, 0, k, 0}, Seg{1, k, k, 1}, k > 0 ? 2 : 1, A, lda, x, incx, y, incy, alpha,
-                    T(0), kAccumulate, false);
+  launch_band_mv<T>(c, n, n, n, k + 1, Seg{-k, 0, k, 0}, Seg{1, k, k, 1}, k > 0 ? 2 : 1, A, lda, x, incx, y,
+                    incy, alpha, T(0), kAccumulate, false);
 }
 
 // x = op(T) x for a triangular band (tbmv): gather the old x, then overwrite.
@@ -203,7 +309,7 @@ void tbmv(Ctx& c, int64_t n, int64_t k, T* x, int64_t incx, const T* A, int64_t 
   hipLaunchKernelGGL((gather_kernel<T>), dim3((unsigned)std::min<int64_t>(ceil_div(n, kNT), 1024)), dim3(kNT), 0,
                      c.stream, n, x, incx, xs);
   EXO_LAUNCH_CHECK();
-  launch_band_mv<T>(c, n, n, s, s, 1, A, lda, xs, 1, x, incx, T(1), T(0), kOverwrite, Diag != 0);
+  launch_band_mv<T>(c, n, n, n, k + 1, s, s, 1, A, lda, xs, 1, x, incx, T(1), T(0), kOverwrite, Diag != 0);
 }
 
 // src/level2/tbmv.py:12-43  x = U x, U(i, j>=i) = A[i, j - i]
