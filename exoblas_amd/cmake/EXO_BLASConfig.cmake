@@ -17,7 +17,8 @@ if(NOT TARGET EXO_BLAS::exoblas_amd)
     IMPORTED_LOCATION "${_exoblas_lib}"
     INTERFACE_INCLUDE_DIRECTORIES "${_exoblas_root}/include;/opt/rocm/include"
     INTERFACE_COMPILE_DEFINITIONS "__HIP_PLATFORM_AMD__")
-  foreach(_k asum axpy dot dsdot copy swap rot rotm scal gemv ger symv syr syr2 trmv trsv gemm mscal symm syrk)
+  foreach(_k asum axpy dot dsdot copy swap rot rotm scal gemv gbmv ger symv sbmv syr syr2 trmv tbmv trsv tbsv
+             gemm mscal symm syrk syr2k trmm)
     add_library(EXO_BLAS::exo_${_k} INTERFACE IMPORTED)
     set_target_properties(EXO_BLAS::exo_${_k} PROPERTIES INTERFACE_LINK_LIBRARIES EXO_BLAS::exoblas_amd)
   endforeach()

@@ -10,16 +10,18 @@
 // each read row-wise (A[i, c0 + d]: contiguous per row) or column-wise
 // (A[j, c0 - d]: one element per stored row).
 //
-// band_lds_kernel (bands whose rows fit in LDS): a workgroup owns 256
-// consecutive output rows, copies the band rows they touch (their own and, for
+// band_lds_kernel (bands whose rows fit in LDS): a workgroup owns 256, 128 or
+// 64 consecutive output rows (the most whose staging fits in 64 KiB), copies the band rows they touch (their own and, for
 // column-wise segments, the k rows either side) and the x window into LDS with
 // coalesced loads, then each thread sums one row from LDS in offset order.
 // band_mv_kernel (wider bands): a group of G lanes per output row (G = 1, 4,
 // 16 or 64 by band width) sums its row's products lane-strided and reduces
 // with xor shuffles in a fixed order.  Both are deterministic.
-// tbsv_kernel: one wave walks the rows in dependency order; lane l handles
-// offsets d = 1 + l, 1 + l + 64, ...; the last k solutions live in an LDS
-// ring, so each step is one wave reduction and one division.
+// tbsv_window_kernel (k <= 32): one wave solves blocks of 64 rows from LDS
+// while it prefetches the next block, with the last k solutions in registers.
+// tbsv_kernel (wider bands): one wave walks the rows in dependency order;
+// lane l handles offsets d = 1 + l, 1 + l + 64, ...; the last k solutions live
+// in an LDS ring, so each step is one wave reduction and one division.
 #include "device.hpp"
 #include "ops.hpp"
 
@@ -81,56 +83,56 @@ __global__ __launch_bounds__(kNT) void band_mv_kernel(int64_t rows, int64_t ncol
   }
 }
 
-// Rows [i0, i0 + kNT) of the band product with the band rows [rs0, rs1) and
+// Rows [i0, i0 + NT) of the band product with the band rows [rs0, rs1) and
 // x[xs0 .. xs1) staged in LDS; band row r occupies LDS [(r - rs0) * ws, +wc).
 // When lda == wc the staged rows are one contiguous range copied as is (ws ==
 // wc, no index arithmetic); otherwise ws = wc rounded up to odd, which keeps
 // the row-per-thread reads free of bank conflicts.
-template <class T>
-__global__ __launch_bounds__(kNT) void band_lds_kernel(int64_t rows, int64_t ncols, int64_t srows, Seg s0, Seg s1,
+template <class T, int NT>
+__global__ __launch_bounds__(NT) void band_lds_kernel(int64_t rows, int64_t ncols, int64_t srows, Seg s0, Seg s1,
                                                        int nseg, int64_t wc, int64_t ws, int64_t reach_lo,
                                                        int64_t reach_hi, int64_t xlo, int64_t xhi,
                                                        const T* __restrict__ A, int64_t lda,
                                                        const T* __restrict__ x, int64_t incx, T* y, int64_t incy,
                                                        T alpha, T beta, int out, int unit) {
   extern __shared__ __align__(16) unsigned char smem[];
-  const int64_t i0 = (int64_t)blockIdx.x * kNT;
+  const int64_t i0 = (int64_t)blockIdx.x * NT;
   const int64_t rs0 = i0 + reach_lo > 0 ? i0 + reach_lo : 0;
-  const int64_t rs1 = i0 + kNT + reach_hi < srows ? i0 + kNT + reach_hi : srows;
+  const int64_t rs1 = i0 + NT + reach_hi < srows ? i0 + NT + reach_hi : srows;
   const int64_t nr = rs1 > rs0 ? rs1 - rs0 : 0;
   T* band = reinterpret_cast<T*>(smem);
-  T* xs = band + (kNT + reach_hi - reach_lo) * ws;
+  T* xs = band + (NT + reach_hi - reach_lo) * ws;
   const int64_t xs0 = i0 + xlo > 0 ? i0 + xlo : 0;
-  const int64_t xs1 = i0 + kNT + xhi < ncols ? i0 + kNT + xhi : ncols;
+  const int64_t xs1 = i0 + NT + xhi < ncols ? i0 + NT + xhi : ncols;
   // global -> LDS, 8 loads in flight per thread (LDS offsets fit 32 bits)
   const uint32_t E = (uint32_t)(nr * wc), wcu = (uint32_t)wc, wsu = (uint32_t)ws;
   if (lda == wc) {
     const T* src = A + rs0 * lda;
-    for (uint32_t e0 = threadIdx.x; e0 < E; e0 += 8 * kNT) {
+    for (uint32_t e0 = threadIdx.x; e0 < E; e0 += 8 * NT) {
       T v[8];
 #pragma unroll
       for (int u = 0; u < 8; ++u)
-        if (e0 + u * kNT < E) v[u] = src[e0 + u * kNT];
+        if (e0 + u * NT < E) v[u] = src[e0 + u * NT];
 #pragma unroll
       for (int u = 0; u < 8; ++u)
-        if (e0 + u * kNT < E) band[e0 + u * kNT] = v[u];
+        if (e0 + u * NT < E) band[e0 + u * NT] = v[u];
     }
   } else {
-    for (uint32_t e0 = threadIdx.x; e0 < E; e0 += 8 * kNT) {
+    for (uint32_t e0 = threadIdx.x; e0 < E; e0 += 8 * NT) {
       T v[8];
 #pragma unroll
       for (int u = 0; u < 8; ++u) {
-        const uint32_t e = e0 + u * kNT, r = e / wcu;
+        const uint32_t e = e0 + u * NT, r = e / wcu;
         if (e < E) v[u] = A[(rs0 + r) * lda + (e - r * wcu)];
       }
 #pragma unroll
       for (int u = 0; u < 8; ++u) {
-        const uint32_t e = e0 + u * kNT, r = e / wcu;
+        const uint32_t e = e0 + u * NT, r = e / wcu;
         if (e < E) band[r * wsu + (e - r * wcu)] = v[u];
       }
     }
   }
-  for (int64_t j = xs0 + threadIdx.x; j < xs1; j += kNT) xs[j - xs0] = x[j * incx];
+  for (int64_t j = xs0 + threadIdx.x; j < xs1; j += NT) xs[j - xs0] = x[j * incx];
   __syncthreads();
   const int64_t i = i0 + threadIdx.x;
   if (i >= rows) return;
@@ -179,14 +181,20 @@ void launch_band_mv(Ctx& c, int64_t rows, int64_t ncols, int64_t srows, int64_t 
     xhi = std::max(xhi, sg.hi);
   }
   const int64_t ws = lda == wc ? wc : (wc | 1);
-  const size_t lds = sizeof(T) * (size_t)((kNT + reach_hi - reach_lo) * ws + (kNT + xhi - xlo));
-  if (lds <= (64 << 10)) {
-    hipLaunchKernelGGL((band_lds_kernel<T>), dim3((unsigned)ceil_div(rows, kNT)), dim3(kNT), lds, c.stream, rows,
-                       ncols, srows, s0, s1, nseg, wc, ws, reach_lo, reach_hi, xlo, xhi, A, lda, x, incx, y, incy,
-                       alpha, beta, out, (int)unit);
-    EXO_LAUNCH_CHECK();
-    return;
+  // the largest row block (256, 128 or 64 rows) whose staging fits in 64 KiB
+  auto lds_of = [&](int64_t nt) { return sizeof(T) * (size_t)((nt + reach_hi - reach_lo) * ws + (nt + xhi - xlo)); };
+#define EXO_BAND_LDS(NTT)                                                                                        \
+  if (lds_of(NTT) <= (64 << 10)) {                                                                               \
+    hipLaunchKernelGGL((band_lds_kernel<T, NTT>), dim3((unsigned)ceil_div(rows, NTT)), dim3(NTT), lds_of(NTT),   \
+                       c.stream, rows, ncols, srows, s0, s1, nseg, wc, ws, reach_lo, reach_hi, xlo, xhi, A, lda, \
+                       x, incx, y, incy, alpha, beta, out, (int)unit);                                           \
+    EXO_LAUNCH_CHECK();                                                                                          \
+    return;                                                                                                      \
   }
+  EXO_BAND_LDS(256)
+  EXO_BAND_LDS(128)
+  EXO_BAND_LDS(64)
+#undef EXO_BAND_LDS
   int64_t w = s0.hi - s0.lo + 1 + (nseg > 1 ? s1.hi - s1.lo + 1 : 0);
 #define EXO_BAND(GG)                                                                                       \
   hipLaunchKernelGGL((band_mv_kernel<T, GG>), dim3((unsigned)ceil_div(rows, kNT / GG)), dim3(kNT), 0, c.stream, \
@@ -245,10 +253,105 @@ __global__ __launch_bounds__(kWave) void tbsv_kernel(int64_t n, int64_t k, int f
   }
 }
 
+// Narrow bands (k <= KMAX <= 32): the rows are solved in blocks of 64 in
+// dependency order.  While block b is solved, every lane loads row l of
+// block b + 1 (its pivot, its KMAX offset entries, 0 past the band or the
+// matrix, and its right-hand side) into registers; after block b they go to
+// LDS.  The solve itself reads the block from LDS (uniform addresses) and
+// keeps the last KMAX solutions in a register window, so a step is KMAX
+// multiply-adds in four chains and one multiply by the pivot's reciprocal
+// (taken when the row is fetched), with no memory round trip.
+template <class T, int KMAX>
+__global__ __launch_bounds__(kWave) void tbsv_window_kernel(int64_t n, int64_t k, int fwd, int col, int64_t c0,
+                                                            int unit, const T* __restrict__ A, int64_t lda, T* x,
+                                                            int64_t incx) {
+  __shared__ T arow[kWave][KMAX + 1];  // [r][0] 1 / pivot, [r][d] entry of offset d
+  __shared__ T bv[kWave];
+  const int lane = threadIdx.x;
+  const int64_t nb = ceil_div(n, kWave);
+  T ra[KMAX + 1], rb = T(0);
+  auto row_of = [&](int64_t s) { return fwd ? s : n - 1 - s; };
+  // fetch only issues loads (every one unconditional, from the row's diagonal
+  // where the entry lies outside the band or the matrix), so none waits for
+  // another; stash, after the block's solve, masks them, takes the pivot's
+  // reciprocal and writes LDS.
+  auto fetch = [&](int64_t blk) {
+    const int64_t s = blk * kWave + lane;
+    const int64_t i = row_of(s < n ? s : n - 1);
+    const int64_t dg = i * lda + c0;
+    ra[0] = A[dg];
+#pragma unroll
+    for (int d = 1; d <= KMAX; ++d) {
+      const int64_t j = fwd ? i - d : i + d;
+      const bool in = d <= k && j >= 0 && j < n;
+      ra[d] = A[in ? (col ? j * lda + (c0 - (j - i)) : dg + (j - i)) : dg];
+    }
+    rb = x[i * incx];
+  };
+  auto stash = [&](int64_t blk) {
+    const int64_t s = blk * kWave + lane;
+    const int64_t i = row_of(s < n ? s : n - 1);
+    arow[lane][0] = unit ? T(1) : T(1) / ra[0];
+#pragma unroll
+    for (int d = 1; d <= KMAX; ++d) {
+      const int64_t j = fwd ? i - d : i + d;
+      arow[lane][d] = (d <= k && j >= 0 && j < n) ? ra[d] : T(0);
+    }
+    bv[lane] = rb;
+  };
+  T xw[KMAX];  // xw[d - 1] = solution of the row d steps back
+#pragma unroll
+  for (int d = 0; d < KMAX; ++d) xw[d] = T(0);
+  fetch(0);
+  stash(0);
+  __syncthreads();
+  for (int64_t blk = 0; blk < nb; ++blk) {
+    if (blk + 1 < nb) fetch(blk + 1);
+    const int cnt = (int)(n - blk * kWave < kWave ? n - blk * kWave : kWave);
+    // FP32: branch-free and unrolled, so the LDS reads of the next rows issue
+    // ahead of this row's chain; lane r keeps row r's solution and the block's
+    // solutions are stored together afterwards (11.1 -> 10.2 ms at n = 2^16,
+    // k = 16).  FP64 stores each row as it is solved (13.8 ms against 17.0
+    // with the block store, whose registers it runs short of).
+    constexpr bool kBlockStore = sizeof(T) == 4;
+    T mine = T(0);
+#pragma unroll 4
+    for (int r = 0; r < cnt; ++r) {
+      T acc[4] = {T(0), T(0), T(0), T(0)};
+#pragma unroll
+      for (int d = 1; d <= KMAX; ++d) acc[(d - 1) & 3] += arow[r][d] * xw[d - 1];
+      const T xi = (bv[r] - ((acc[0] + acc[1]) + (acc[2] + acc[3]))) * arow[r][0];
+#pragma unroll
+      for (int d = KMAX - 1; d > 0; --d) xw[d] = xw[d - 1];
+      xw[0] = xi;
+      if constexpr (kBlockStore)
+        mine = lane == r ? xi : mine;
+      else if (lane == 0)
+        x[row_of(blk * kWave + r) * incx] = xi;
+    }
+    if (kBlockStore && lane < cnt) x[row_of(blk * kWave + lane) * incx] = mine;
+    __syncthreads();
+    if (blk + 1 < nb) stash(blk + 1);
+    __syncthreads();
+  }
+}
+
 template <class T>
 void launch_tbsv(Ctx& c, int64_t n, int64_t k, bool fwd, bool col, int64_t c0, bool unit, const T* A, int64_t lda,
                  T* x, int64_t incx) {
   if (n <= 0) return;
+#define EXO_TBSV_W(KM)                                                                                          \
+  if (k <= KM) {                                                                                                \
+    hipLaunchKernelGGL((tbsv_window_kernel<T, KM>), dim3(1), dim3(kWave), 0, c.stream, n, k, (int)fwd, (int)col, \
+                       c0, (int)unit, A, lda, x, incx);                                                         \
+    EXO_LAUNCH_CHECK();                                                                                         \
+    return;                                                                                                     \
+  }
+  EXO_TBSV_W(4)
+  EXO_TBSV_W(8)
+  EXO_TBSV_W(16)
+  EXO_TBSV_W(32)
+#undef EXO_TBSV_W
   int64_t R = 1;
   while (R < k + 1) R <<= 1;
   const size_t bytes = sizeof(T) * (size_t)R;

@@ -253,6 +253,24 @@ void run(Ctx& c, int64_t M, int64_t N, int64_t K, T alpha, const Operand<T>& A, 
   launch_rounds<Small, TRI>(c, M, N, K, alpha, A, B, Cp, ldc);
 }
 
+// Triangular-operand products (trmm): row tile t of a lower op(A) runs t + 1
+// K tiles and of an upper one the rest, so whole-tile launches wait for the
+// longest rows.  With at least half a round of large tiles, K is split four
+// ways instead: the slices outside a tile's window exit at once and the
+// dispatcher spreads the uneven remainder over the CUs.
+template <class T>
+void run_triangular(Ctx& c, int64_t M, int64_t N, int64_t K, T alpha, const Operand<T>& A, const Operand<T>& B,
+                    T* Cp, int64_t ldc) {
+  if (M <= 0 || N <= 0 || K <= 0) return;
+  using Huge = typename Tiles<T>::Huge;
+  const int64_t tiles = tile_count<Huge>(kFull, M, N);
+  const int64_t S = std::min<int64_t>(4, K / (4 * Huge::BK));
+  if (2 * tiles >= c.num_cus && S >= 2)
+    launch_plan<Huge, kFull>(c, 0, S, M, N, K, alpha, A, B, Cp, ldc);
+  else
+    run<T, kFull>(c, M, N, K, alpha, A, B, Cp, ldc);
+}
+
 template <class T>
 Operand<T> operand(const T* p, int64_t ld, int form) {
   constexpr int VN = 16 / sizeof(T);
@@ -415,7 +433,7 @@ void syr2k_rm(Ctx& c, int64_t Uplo, int64_t Trans, int64_t N, int64_t K, T alpha
 // diagonal when Diag != 0), B m x n in place: B is copied to scratch (row
 // length padded to 16 bytes) and cleared in one pass, then the product with
 // the triangular operand form accumulates into it.  K tiles outside the
-// triangle are skipped.
+// triangle are skipped, and large products split K (run_triangular).
 template <class T>
 void trmm_left(Ctx& c, bool upper, bool trans, int64_t m, int64_t n, T alpha, const T* A, int64_t lda, T* B,
                int64_t ldb, int64_t Diag) {
@@ -435,7 +453,7 @@ void trmm_left(Ctx& c, bool upper, bool trans, int64_t m, int64_t n, T alpha, co
   // A[i, k] without a transpose and A[k, i] with one.
   const bool lower_op = upper == trans;
   const Operand<T> a = operand(A, lda, tri_form(trans ? kKR : kRK, !lower_op, Diag != 0));
-  product<T>(c, kFull, m, n, m, alpha, a, operand<T>(Bc, ldp, kKR), B, ldb);
+  run_triangular<T>(c, m, n, m, alpha, a, operand<T>(Bc, ldp, kKR), B, ldb);
 }
 
 // src/level3/trmm.py:12-23
