@@ -42,12 +42,18 @@ __host__ __device__ constexpr int tri_form(int base, bool upper, bool unit) {
 }
 enum Tri : int { kFull = 0, kLower = 1, kUpper = 2 };
 
+// An operand may be the concatenation of two along K: X(r, k) reads (p, ld)
+// for k < k1 and (p2, ld2) at k - k1 beyond (syr2k's [A | B]); both parts
+// share the form.
 template <class T>
 struct Operand {
   const T* p;
   int64_t ld;
   int form;
-  int vec;  // 16-byte loads allowed (aligned base, ld a multiple of the vector width)
+  int vec;  // 16-byte loads allowed (aligned bases, lds a multiple of the vector width)
+  const T* p2 = nullptr;
+  int64_t ld2 = 0;
+  int64_t k1 = INT64_MAX;
 };
 
 typedef float v4f __attribute__((ext_vector_type(4)));
@@ -88,19 +94,35 @@ struct TileLoader {
   T reg[NE];
   int mode;  // 0: kRK vector, 1: kKR vector, 2: element-wise
 
+  // EXT 1: the operand may be triangular (trmm); 2: concatenated along K
+  // (syr2k).  The plain kernels (EXT 0) compile without either, which would
+  // cost them registers.
+  template <int EXT>
   __device__ __forceinline__ static T elem(const Operand<T>& op, int form, int64_t r, int64_t k) {
     // form is kRK or kKR, symmetric (select by triangle) or triangular
-    if (form >= kTri) {
-      const int f = form - kTri;
-      if (r == k && (f & 4)) return T(1);
-      if ((f & 2) ? k < r : k > r) return T(0);
-      form = f & 1;
+    if constexpr (EXT == 1) {
+      if (form >= kTri) {
+        const int f = form - kTri;
+        if (r == k && (f & 4)) return T(1);
+        if ((f & 2) ? k < r : k > r) return T(0);
+        form = f & 1;
+      }
     }
     if (form == kSymLower) form = r >= k ? kRK : kKR;
     if (form == kSymUpper) form = r <= k ? kRK : kKR;
-    return form == kRK ? op.p[r * op.ld + k] : op.p[k * op.ld + r];
+    const T* p = op.p;
+    int64_t ld = op.ld;
+    if constexpr (EXT == 2) {
+      if (k >= op.k1) {
+        p = op.p2;
+        ld = op.ld2;
+        k -= op.k1;
+      }
+    }
+    return form == kRK ? p[r * ld + k] : p[k * ld + r];
   }
 
+  template <int EXT>
   __device__ __forceinline__ void load(const Operand<T>& op, int64_t r0, int64_t k0, int64_t rows, int64_t K) {
     int form = op.form;
     if (form == kSymLower) {
@@ -109,7 +131,7 @@ struct TileLoader {
     } else if (form == kSymUpper) {
       if (r0 + R - 1 <= k0) form = kRK;
       else if (r0 > k0 + BK - 1) form = kKR;
-    } else if (form >= kTri) {
+    } else if (EXT == 1 && form >= kTri) {
       const int f = form - kTri;
       // strictly inside the triangle: the plain form; wholly outside: zeros
       const bool inside = (f & 2) ? k0 > r0 + R - 1 : k0 + BK - 1 < r0;
@@ -122,7 +144,21 @@ struct TileLoader {
         return;
       }
     }
-    const bool full = r0 + R <= rows && k0 + BK <= K;
+    // a concatenated operand: tiles past k1 read the second part, a tile
+    // across k1 goes element-wise
+    const T* p = op.p;
+    int64_t ld = op.ld, kb = k0;
+    bool across = false;
+    if constexpr (EXT == 2) {
+      if (k0 >= op.k1) {
+        p = op.p2;
+        ld = op.ld2;
+        kb = k0 - op.k1;
+      } else if (k0 + BK > op.k1) {
+        across = true;
+      }
+    }
+    const bool full = r0 + R <= rows && k0 + BK <= K && !across;
     const int tid = threadIdx.x;
     if (full && op.vec && form == kRK) {
       mode = 0;
@@ -131,7 +167,7 @@ struct TileLoader {
       for (int i = 0; i < NE / VN; ++i) {
         const int v = tid + NT * i;
         const int r = v / (BK / VN), kv = v % (BK / VN);
-        const V x = *reinterpret_cast<const V*>(op.p + (r0 + r) * op.ld + k0 + kv * VN);
+        const V x = *reinterpret_cast<const V*>(p + (r0 + r) * ld + kb + kv * VN);
 #pragma unroll
         for (int e = 0; e < VN; ++e) reg[i * VN + e] = x.v[e];
       }
@@ -142,7 +178,7 @@ struct TileLoader {
       for (int i = 0; i < NE / VN; ++i) {
         const int v = tid + NT * i;
         const int k = v / (R / VN), rv = v % (R / VN);
-        const V x = *reinterpret_cast<const V*>(op.p + (k0 + k) * op.ld + r0 + rv * VN);
+        const V x = *reinterpret_cast<const V*>(p + (kb + k) * ld + r0 + rv * VN);
 #pragma unroll
         for (int e = 0; e < VN; ++e) reg[i * VN + e] = x.v[e];
       }
@@ -153,7 +189,7 @@ struct TileLoader {
         const int idx = tid + NT * j;
         const int k = idx / R, r = idx % R;
         const int64_t gr = r0 + r, gk = k0 + k;
-        reg[j] = (gr < rows && gk < K) ? elem(op, form, gr, gk) : T(0);
+        reg[j] = (gr < rows && gk < K) ? elem<EXT>(op, form, gr, gk) : T(0);
       }
     }
   }
@@ -222,7 +258,7 @@ __device__ __forceinline__ void tri_k_window(const Operand<T>& A, int64_t m0, in
 
 // Accumulates one block tile of  sum_k X_A(m, k) X_B(n, k)  for k in [kb, ke)
 // into acc; lds points at LDS_BYTES of shared memory.
-template <class C>
+template <class C, int EXT = 0>
 __device__ __forceinline__ void tile_mma(const Operand<typename C::type>& A, const Operand<typename C::type>& B,
                                          int64_t M, int64_t N, int64_t K, int64_t m0, int64_t n0, int64_t kb,
                                          int64_t ke, typename Mfma<typename C::type>::acc (&acc)[C::SM][C::SN],
@@ -239,16 +275,16 @@ __device__ __forceinline__ void tile_mma(const Operand<typename C::type>& A, con
   const int64_t nk = (ke - kb + C::BK - 1) / C::BK;
   if (nk <= 0) return;
   // the K extent the loaders bound-check against is ke (a split's end)
-  la.load(A, m0, kb, M, ke);
-  lb.load(B, n0, kb, N, ke);
+  la.template load<EXT>(A, m0, kb, M, ke);
+  lb.template load<EXT>(B, n0, kb, N, ke);
   la.store(As);
   lb.store(Bs);
   __syncthreads();
   for (int64_t t = 0; t < nk; ++t) {
     const int cur = (int)(t & 1);
     if (t + 1 < nk) {
-      la.load(A, m0, kb + (t + 1) * C::BK, M, ke);
-      lb.load(B, n0, kb + (t + 1) * C::BK, N, ke);
+      la.template load<EXT>(A, m0, kb + (t + 1) * C::BK, M, ke);
+      lb.template load<EXT>(B, n0, kb + (t + 1) * C::BK, N, ke);
     }
     const T* as = As + cur * C::LDS_A;
     const T* bs = Bs + cur * C::LDS_B;

@@ -17,7 +17,7 @@ namespace gemm {
 
 // Tiles [tile0, tile0 + gridDim.x) of the tile list (list_tile), one per
 // workgroup.
-template <class C, int TRI>
+template <class C, int TRI, int EXT = 0>
 __global__ __launch_bounds__(C::NT) void gemm_kernel(int64_t M, int64_t N, int64_t K, typename C::type alpha,
                                                      Operand<typename C::type> A, Operand<typename C::type> B,
                                                      typename C::type* Cp, int64_t ldc, int cvec, int64_t tiles_m,
@@ -36,8 +36,8 @@ __global__ __launch_bounds__(C::NT) void gemm_kernel(int64_t M, int64_t N, int64
 #pragma unroll
       for (int q = 0; q < 4; ++q) acc[a][b][q] = T(0);
   int64_t kb = 0, ke = K;
-  tri_k_window(A, m0, C::BM, kb, ke);
-  tile_mma<C>(A, B, M, N, K, m0, n0, kb, ke, acc, reinterpret_cast<T*>(smem));
+  if constexpr (EXT == 1) tri_k_window(A, m0, C::BM, kb, ke);
+  tile_mma<C, EXT>(A, B, M, N, K, m0, n0, kb, ke, acc, reinterpret_cast<T*>(smem));
   tile_store<C, TRI>(Cp, ldc, cvec, M, N, m0, n0, alpha, acc);
 }
 
@@ -46,7 +46,7 @@ __global__ __launch_bounds__(C::NT) void gemm_kernel(int64_t M, int64_t N, int64
 // tile0 + l and writes its raw sums as a dense BM x BN block
 // P[(s * ntiles + l) * BM * BN ...]; splitk_reduce_kernel then adds
 // alpha * sum_s of the blocks to C in a fixed order.
-template <class C, int TRI>
+template <class C, int TRI, int EXT = 0>
 __global__ __launch_bounds__(C::NT) void gemm_splitk_kernel(int64_t M, int64_t N, int64_t K, int64_t ksplit,
                                                             Operand<typename C::type> A, Operand<typename C::type> B,
                                                             typename C::type* P, int64_t tiles_m, int64_t tiles_n,
@@ -66,8 +66,8 @@ __global__ __launch_bounds__(C::NT) void gemm_splitk_kernel(int64_t M, int64_t N
 #pragma unroll
       for (int q = 0; q < 4; ++q) acc[a][b][q] = T(0);
   int64_t kb = s * ksplit, ke = kb + ksplit < K ? kb + ksplit : K;
-  tri_k_window(A, m0, C::BM, kb, ke);  // an empty window leaves the block's sums zero
-  tile_mma<C>(A, B, M, N, K, m0, n0, kb, ke, acc, reinterpret_cast<T*>(smem));
+  if constexpr (EXT == 1) tri_k_window(A, m0, C::BM, kb, ke);  // an empty window leaves the sums zero
+  tile_mma<C, EXT>(A, B, M, N, K, m0, n0, kb, ke, acc, reinterpret_cast<T*>(smem));
   tile_store<C, kFull, true>(P + (s * ntiles + l) * C::BM * C::BN, C::BN, 1, C::BM, C::BN, 0, 0, T(1), acc, T(0));
 }
 
@@ -111,7 +111,7 @@ __global__ __launch_bounds__(256) void splitk_reduce_kernel(int64_t M, int64_t N
 }
 
 // Split-K over tiles [tile0, tile0 + ntiles) of the list, S slices.
-template <class C, int TRI>
+template <class C, int TRI, int EXT = 0>
 void launch_splitk(Ctx& c, int64_t S, int64_t tile0, int64_t ntiles, int64_t M, int64_t N, int64_t K,
                    typename C::type alpha, const Operand<typename C::type>& A, const Operand<typename C::type>& B,
                    typename C::type* Cp, int64_t ldc) {
@@ -121,8 +121,8 @@ void launch_splitk(Ctx& c, int64_t S, int64_t tile0, int64_t ntiles, int64_t M, 
   S = ceil_div(K, ksplit);
   T* P = static_cast<T*>(c.scratch(sizeof(T) * (size_t)(S * ntiles * C::BM * C::BN), 4));
   static std::atomic<uint64_t> lds_set{0};
-  ensure_lds_limit(reinterpret_cast<const void*>(&gemm_splitk_kernel<C, TRI>), C::LDS_BYTES, c.device, lds_set);
-  hipLaunchKernelGGL((gemm_splitk_kernel<C, TRI>), dim3((unsigned)(S * ntiles)), dim3(C::NT), C::LDS_BYTES,
+  ensure_lds_limit(reinterpret_cast<const void*>(&gemm_splitk_kernel<C, TRI, EXT>), C::LDS_BYTES, c.device, lds_set);
+  hipLaunchKernelGGL((gemm_splitk_kernel<C, TRI, EXT>), dim3((unsigned)(S * ntiles)), dim3(C::NT), C::LDS_BYTES,
                      c.stream, M, N, K, ksplit, A, B, P, tm, tn, tile0, ntiles);
   EXO_LAUNCH_CHECK();
   constexpr int RPW = 256 / (C::BN / Vec16<T>::n);
@@ -132,7 +132,7 @@ void launch_splitk(Ctx& c, int64_t S, int64_t tile0, int64_t ntiles, int64_t M, 
 }
 
 // Tiles [tile0, tile0 + ntiles) of the list, one workgroup each.
-template <class C, int TRI>
+template <class C, int TRI, int EXT = 0>
 void launch_cfg(Ctx& c, int64_t tile0, int64_t ntiles, int64_t M, int64_t N, int64_t K, typename C::type alpha,
                 const Operand<typename C::type>& A, const Operand<typename C::type>& B, typename C::type* Cp,
                 int64_t ldc) {
@@ -140,8 +140,8 @@ void launch_cfg(Ctx& c, int64_t tile0, int64_t ntiles, int64_t M, int64_t N, int
   const int64_t tm = ceil_div(M, C::BM), tn = ceil_div(N, C::BN);
   const int cvec = (ldc % C::EB == 0) && (reinterpret_cast<uintptr_t>(Cp) % (C::EB * sizeof(T)) == 0);
   static std::atomic<uint64_t> lds_set{0};
-  ensure_lds_limit(reinterpret_cast<const void*>(&gemm_kernel<C, TRI>), C::LDS_BYTES, c.device, lds_set);
-  hipLaunchKernelGGL((gemm_kernel<C, TRI>), dim3((unsigned)ntiles), dim3(C::NT), C::LDS_BYTES, c.stream, M, N, K,
+  ensure_lds_limit(reinterpret_cast<const void*>(&gemm_kernel<C, TRI, EXT>), C::LDS_BYTES, c.device, lds_set);
+  hipLaunchKernelGGL((gemm_kernel<C, TRI, EXT>), dim3((unsigned)ntiles), dim3(C::NT), C::LDS_BYTES, c.stream, M, N, K,
                      alpha, A, B, Cp, ldc, cvec, tm, tn, tile0);
   EXO_LAUNCH_CHECK();
 }
@@ -178,13 +178,13 @@ int64_t tile_count(int tri, int64_t M, int64_t N) {
 
 // One configuration's launches: tiles [0, dp) one workgroup each, the rest
 // split S ways along K (S = 1: dp is every tile).
-template <class C, int TRI>
+template <class C, int TRI, int EXT = 0>
 void launch_plan(Ctx& c, int64_t dp, int64_t S, int64_t M, int64_t N, int64_t K, typename C::type alpha,
                  const Operand<typename C::type>& A, const Operand<typename C::type>& B, typename C::type* Cp,
                  int64_t ldc) {
   const int64_t total = tile_count<C>(TRI, M, N);
-  if (dp > 0) launch_cfg<C, TRI>(c, 0, dp, M, N, K, alpha, A, B, Cp, ldc);
-  if (dp < total) launch_splitk<C, TRI>(c, S, dp, total - dp, M, N, K, alpha, A, B, Cp, ldc);
+  if (dp > 0) launch_cfg<C, TRI, EXT>(c, 0, dp, M, N, K, alpha, A, B, Cp, ldc);
+  if (dp < total) launch_splitk<C, TRI, EXT>(c, S, dp, total - dp, M, N, K, alpha, A, B, Cp, ldc);
 }
 
 // Split factor in [2, smax] that fills the last round of S * tiles units
@@ -207,7 +207,7 @@ inline int64_t best_split(int64_t tiles, int64_t P, int64_t smax, double need) {
 // Tiles [0, dp) of C's tile list run whole, the remaining ones split along K
 // into the stream's free CUs: the full rounds keep every CU busy and the
 // last, partial round is spread over all of them.
-template <class C, int TRI>
+template <class C, int TRI, int EXT = 0>
 void launch_rounds(Ctx& c, int64_t M, int64_t N, int64_t K, typename C::type alpha,
                    const Operand<typename C::type>& A, const Operand<typename C::type>& B, typename C::type* Cp,
                    int64_t ldc) {
@@ -215,9 +215,9 @@ void launch_rounds(Ctx& c, int64_t M, int64_t N, int64_t K, typename C::type alp
   const int64_t dp = total / P * P, rem = total - dp;
   const int64_t S = rem ? std::min<int64_t>({P / rem, K / (4 * C::BK), 16}) : 1;
   if (S < 2)
-    launch_plan<C, TRI>(c, total, 1, M, N, K, alpha, A, B, Cp, ldc);
+    launch_plan<C, TRI, EXT>(c, total, 1, M, N, K, alpha, A, B, Cp, ldc);
   else
-    launch_plan<C, TRI>(c, dp, S, M, N, K, alpha, A, B, Cp, ldc);
+    launch_plan<C, TRI, EXT>(c, dp, S, M, N, K, alpha, A, B, Cp, ldc);
 }
 
 // Plan choice, from measurements on MI355X (FP32 TFLOP/s, old rule -> this):
@@ -230,7 +230,7 @@ void launch_rounds(Ctx& c, int64_t M, int64_t N, int64_t K, typename C::type alp
 // that fills the rounds (sgemm 2048^3 81.2 -> 100.2), else 128 x 128 tiles,
 // else 64 x 64 tiles split along K for small outputs with a long K (sgemm
 // 512^3, 1024^3).
-template <class T, int TRI>
+template <class T, int TRI, int EXT = 0>
 void run(Ctx& c, int64_t M, int64_t N, int64_t K, T alpha, const Operand<T>& A, const Operand<T>& B, T* Cp,
          int64_t ldc) {
   if (M <= 0 || N <= 0 || K <= 0) return;
@@ -240,17 +240,17 @@ void run(Ctx& c, int64_t M, int64_t N, int64_t K, T alpha, const Operand<T>& A, 
   using Small = typename Tl::Small;
   const int64_t P = c.num_cus;
   const int64_t huge = tile_count<Huge>(TRI, M, N), big = tile_count<Big>(TRI, M, N);
-  if (huge >= P) return launch_rounds<Huge, TRI>(c, M, N, K, alpha, A, B, Cp, ldc);
-  if (big >= 2 * P) return launch_rounds<Big, TRI>(c, M, N, K, alpha, A, B, Cp, ldc);
+  if (huge >= P) return launch_rounds<Huge, TRI, EXT>(c, M, N, K, alpha, A, B, Cp, ldc);
+  if (big >= 2 * P) return launch_rounds<Big, TRI, EXT>(c, M, N, K, alpha, A, B, Cp, ldc);
   const int64_t sh = best_split(huge, P, std::min<int64_t>(16, K / (4 * Huge::BK)), 0.75);
-  if (sh >= 2) return launch_plan<Huge, TRI>(c, 0, sh, M, N, K, alpha, A, B, Cp, ldc);
-  if (big >= P) return launch_rounds<Big, TRI>(c, M, N, K, alpha, A, B, Cp, ldc);
+  if (sh >= 2) return launch_plan<Huge, TRI, EXT>(c, 0, sh, M, N, K, alpha, A, B, Cp, ldc);
+  if (big >= P) return launch_rounds<Big, TRI, EXT>(c, M, N, K, alpha, A, B, Cp, ldc);
   // Few output tiles and a long K: split K so every CU gets two workgroups,
   // each K slice at least 128 deep.
   const int64_t small = tile_count<Small>(TRI, M, N);
   const int64_t S = std::min<int64_t>({ceil_div(2 * P, small), K / 128, 16});
-  if (S >= 2) return launch_plan<Small, TRI>(c, 0, S, M, N, K, alpha, A, B, Cp, ldc);
-  launch_rounds<Small, TRI>(c, M, N, K, alpha, A, B, Cp, ldc);
+  if (S >= 2) return launch_plan<Small, TRI, EXT>(c, 0, S, M, N, K, alpha, A, B, Cp, ldc);
+  launch_rounds<Small, TRI, EXT>(c, M, N, K, alpha, A, B, Cp, ldc);
 }
 
 // Triangular-operand products (trmm): row tile t of a lower op(A) runs t + 1
@@ -266,9 +266,9 @@ void run_triangular(Ctx& c, int64_t M, int64_t N, int64_t K, T alpha, const Oper
   const int64_t tiles = tile_count<Huge>(kFull, M, N);
   const int64_t S = std::min<int64_t>(4, K / (4 * Huge::BK));
   if (2 * tiles >= c.num_cus && S >= 2)
-    launch_plan<Huge, kFull>(c, 0, S, M, N, K, alpha, A, B, Cp, ldc);
+    launch_plan<Huge, kFull, 1>(c, 0, S, M, N, K, alpha, A, B, Cp, ldc);
   else
-    run<T, kFull>(c, M, N, K, alpha, A, B, Cp, ldc);
+    run<T, kFull, 1>(c, M, N, K, alpha, A, B, Cp, ldc);
 }
 
 template <class T>
@@ -285,12 +285,24 @@ Operand<T> operand(const T* p, int64_t ld, int form) {
 template <class T>
 void product(Ctx& c, int tri, int64_t M, int64_t N, int64_t K, T alpha, const Operand<T>& A, const Operand<T>& B,
              T* Cp, int64_t ldc) {
-  if (tri == kLower)
+  // triangular (1) or concatenated (2) operands take the kernels built with that code
+  const int ext = (A.form >= kTri || B.form >= kTri) ? 1 : (A.k1 != INT64_MAX || B.k1 != INT64_MAX) ? 2 : 0;
+  if (ext == 1) {
+    run<T, kFull, 1>(c, M, N, K, alpha, A, B, Cp, ldc);  // trmm: full output
+  } else if (ext == 2) {
+    if (tri == kLower)
+      run<T, kLower, 2>(c, M, N, K, alpha, A, B, Cp, ldc);
+    else if (tri == kUpper)
+      run<T, kUpper, 2>(c, M, N, K, alpha, A, B, Cp, ldc);
+    else
+      run<T, kFull, 2>(c, M, N, K, alpha, A, B, Cp, ldc);
+  } else if (tri == kLower) {
     run<T, kLower>(c, M, N, K, alpha, A, B, Cp, ldc);
-  else if (tri == kUpper)
+  } else if (tri == kUpper) {
     run<T, kUpper>(c, M, N, K, alpha, A, B, Cp, ldc);
-  else
+  } else {
     run<T, kFull>(c, M, N, K, alpha, A, B, Cp, ldc);
+  }
 }
 
 }  // namespace gemm
@@ -410,9 +422,19 @@ void syrk_rm(Ctx& c, int64_t Uplo, int64_t Trans, int64_t N, int64_t K, T alpha,
     product<T>(c, tri, N, N, K, alpha, operand(AT, ldat, kKR), operand(AT_, ldat_, kKR), Cp, ldc);
 }
 
+// [X | Y] along K: k < K reads x, k >= K reads y at k - K.
+template <class T>
+gemm::Operand<T> concat(gemm::Operand<T> x, const gemm::Operand<T>& y, int64_t K) {
+  x.p2 = y.p;
+  x.ld2 = y.ld;
+  x.k1 = K;
+  x.vec = x.vec && y.vec;
+  return x;
+}
+
 // src/level3/syr2k.py:14-39.  NoTrans: C += alpha (A B^T + B A^T); Trans:
-// C += alpha (A^T B + B^T A), on the Uplo triangle of C: two triangular
-// products (A with B, then B with A).
+// C += alpha (A^T B + B^T A), on the Uplo triangle of C: one triangular
+// product over 2K with X_A = [A | B] and X_B = [B | A].
 template <class T>
 void syr2k_rm(Ctx& c, int64_t Uplo, int64_t Trans, int64_t N, int64_t K, T alpha, const T* A, int64_t lda,
               const T* A_, int64_t lda_, const T* B, int64_t ldb, const T* B_, int64_t ldb_, const T* AT,
@@ -420,13 +442,13 @@ void syr2k_rm(Ctx& c, int64_t Uplo, int64_t Trans, int64_t N, int64_t K, T alpha
               T* Cp, int64_t ldc) {
   using namespace gemm;
   const int tri = Uplo == kUpperV ? kUpper : kLower;
-  if (Trans == kNoTrans) {
-    product<T>(c, tri, N, N, K, alpha, operand(A, lda, kRK), operand(B, ldb, kRK), Cp, ldc);
-    product<T>(c, tri, N, N, K, alpha, operand(B_, ldb_, kRK), operand(A_, lda_, kRK), Cp, ldc);
-  } else {
-    product<T>(c, tri, N, N, K, alpha, operand(AT, ldat, kKR), operand(BT, ldbt, kKR), Cp, ldc);
-    product<T>(c, tri, N, N, K, alpha, operand(BT_, ldbt_, kKR), operand(AT_, ldat_, kKR), Cp, ldc);
-  }
+  if (N <= 0 || K <= 0) return;
+  if (Trans == kNoTrans)
+    product<T>(c, tri, N, N, 2 * K, alpha, concat(operand(A, lda, kRK), operand(B_, ldb_, kRK), K),
+               concat(operand(B, ldb, kRK), operand(A_, lda_, kRK), K), Cp, ldc);
+  else
+    product<T>(c, tri, N, N, 2 * K, alpha, concat(operand(AT, ldat, kKR), operand(BT_, ldbt_, kKR), K),
+               concat(operand(BT, ldbt, kKR), operand(AT_, ldat_, kKR), K), Cp, ldc);
 }
 
 // src/level3/trmm.py:12-83.  B <- alpha op(A) B, A triangular m x m (unit
