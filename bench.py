@@ -527,6 +527,12 @@ def bench_extended(a, torch, so, L):
                                                                      L.win2(At.data_ptr(), M),
                                                                      L.win2(Bw.data_ptr(), M), 0),
         float(M) * (M + 1) * M, "TFLOP/s", FP32_PEAK_TF, 5, before=lambda: Bw.copy_(B0))
+    if a.gbench_dir:
+        from exoblas_amd import gbench
+        groups = {"syr2k": [gbench.syr2k_record(p, 122, 111, 4096, 1024, kernels[f"{p}syr2k_LN_4096x1024"]["us"], 5,
+                                                alpha=1.5, beta=1) for p in "sd"],
+                  "trmm": [gbench.strmm_record(122, 111, 131, M, M, 1, kernels["strmm_LLNN_4096"]["us"], 5)]}
+        _gbench_emit(a, groups)
     k = kernels["ssyr2k_LN_4096x1024"]
     roof = {"bound": "mfma", "kernel": "ssyr2k_LN_4096x1024", "achieved": k["TFLOP/s"], "peak": FP32_PEAK_TF,
             "unit": "TFLOP/s", "frac": k["frac"], "traffic": None}

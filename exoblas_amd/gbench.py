@@ -154,3 +154,22 @@ def symm_record(p: str, side: int, uplo: int, m: int, n: int, us: float, iterati
             ("alignmentB", 64), ("alignmentC", 64), ("bench_type", LEVEL_3_EQ), ("precision", _bits(p))]
     return record(kernel_name("exo", p, "symm", ROW_MAJOR, side=side, uplo=uplo, level3=True), args, us * 1e3,
                   iterations)
+
+
+def syr2k_record(p: str, uplo: int, trans: int, n: int, k: int, us: float, iterations: int, alpha=17,
+                 beta=18) -> dict:
+    """test/level3/syr2k/bench.cpp:43-64 ArgNames (the same kernel_name scheme as syrk)."""
+    args = [("N", n), ("K", k), ("order", ROW_MAJOR), ("Uplo", uplo), ("Trans", trans), ("alpha", alpha),
+            ("lda_diff", 0), ("ldb_diff", 0), ("beta", beta), ("ldc_diff", 0), ("alignmentA", 64),
+            ("alignmentB", 64), ("alignmentC", 64), ("bench_type", LEVEL_3_EQ), ("precision", _bits(p))]
+    return record(kernel_name("exo", p, "syr2k", ROW_MAJOR, uplo=uplo, trans_a=trans, level3=True), args, us * 1e3,
+                  iterations)
+
+
+def strmm_record(uplo: int, trans: int, diag: int, m: int, n: int, alpha: int, us: float, iterations: int) -> dict:
+    """test/level3/strmm/bench.cpp:103-109: the benchmark BM_exo_strmm, enums encoded 0/1 (Order 0 = row
+    major, Side 0 = Left, Uplo 0 = Lower, TransA 0 = NoTrans, Diag 0 = NonUnit), alpha 0 or 1."""
+    args = [("m", m), ("n", n), ("Order", 0), ("Side", 0), ("Uplo", 0 if uplo == 122 else 1),
+            ("TransA", 0 if trans == 111 else 1), ("Diag", 0 if diag == 131 else 1), ("alpha", alpha),
+            ("lda", m), ("ldb", n), ("alignmentA", 64), ("alignmentB", 64)]
+    return record("BM_exo_strmm", args, us * 1e3, iterations)

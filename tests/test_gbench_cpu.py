@@ -50,3 +50,16 @@ def test_records_and_files(tmp_path):
     doc = json.load(open(path))
     assert doc["benchmarks"][0]["run_name"].startswith("exo_sgemv_rm_n/M:8192/N:8192")
     assert path.endswith("exoblas_amd/gemv.json")
+
+
+def test_syr2k_and_strmm_records():
+    r = g.syr2k_record("s", g.LOWER, g.NO_TRANS, 4096, 1024, 390.2, 5, alpha=1.5, beta=1)
+    d = _parse(r["run_name"])
+    assert d["sub_kernel_name"] == "exo_ssyr2k_rm_ln"
+    assert (int(d["N"]), int(d["K"]), int(d["Uplo"]), int(d["Trans"])) == (4096, 1024, g.LOWER, g.NO_TRANS)
+    assert int(d["bench_type"]) == g.LEVEL_3_EQ and d["precision"] == "32"
+    t = g.strmm_record(g.LOWER, g.NO_TRANS, g.NON_UNIT, 4096, 4096, 1, 1070.0, 5)
+    # test/level3/strmm/bench.cpp:107-109: BM_exo_strmm with enums as 0/1 (0 = Lower / NoTrans / NonUnit)
+    assert t["run_name"] == ("BM_exo_strmm/m:4096/n:4096/Order:0/Side:0/Uplo:0/TransA:0/Diag:0/alpha:1/"
+                             "lda:4096/ldb:4096/alignmentA:64/alignmentB:64")
+    assert t["real_time"] == pytest.approx(1070000.0)

@@ -134,3 +134,70 @@ def test_trsv_graph_replay(env, dt):
         r = L64 @ got.double().cpu().numpy() - b.double().cpu().numpy()
         eps = float(np.finfo(np.dtype(dt)).eps)
         assert np.abs(r).max() <= 8 * n * eps, f"replay {rep}: residual {np.abs(r).max()}"
+
+
+def test_banded_syr2k_trmm_graph_replay(env):
+    """sbmv, tbmv, tbsv (both solvers), syr2k and strmm captured in one hipGraph
+    (their scratch is grown by an eager call first): each replay on freshly
+    written inputs equals the eager calls on the same inputs, bit for bit."""
+    torch, so, L = env
+    g = torch.Generator(device="cuda")
+    n, k, kw = 20000, 16, 40
+    N2, K2 = 512, 256
+    M3, N3 = 1024, 777
+    f = lambda: ctypes.byref(ctypes.c_float(1.25))
+    A = torch.empty((n, k + 1), device="cuda")
+    Aw = torch.empty((n, kw + 1), device="cuda")
+    x, y, xt, xs, xw = (torch.empty(n, device="cuda") for _ in range(5))
+    Bk, Ck = torch.empty((N2, K2), device="cuda"), torch.empty((N2, N2), device="cuda")
+    Ak = torch.empty((N2, K2), device="cuda")
+    At, Bt = torch.empty((M3, M3), device="cuda"), torch.empty((M3, N3), device="cuda")
+    outs = (y, xt, xs, xw, Ck, Bt)
+    s = torch.cuda.Stream()
+
+    def fill(seed):
+        g.manual_seed(seed)
+        for t in (A, Aw, x, y, xt, xs, xw, Bk, Ck, Ak, At, Bt):
+            t.uniform_(-1, 1, generator=g)
+        A[:, 0] += k + 1  # dominant diagonals for the solves (upper storage: column 0)
+        Aw[:, 0] += kw + 1
+
+    def launch(st):
+        assert so.exoblas_ssbmv_row_major_Upper_stride_1_async(st, n, k, f(), L.win2(A.data_ptr(), k + 1),
+                                                               L.win1(x.data_ptr(), 1), L.win1(y.data_ptr(), 1)) == 0
+        assert so.exoblas_stbmv_row_major_Upper_Trans_stride_1_async(st, n, k, L.win1(xt.data_ptr(), 1),
+                                                                     L.win2(A.data_ptr(), k + 1), 0) == 0
+        assert so.exoblas_stbsv_row_major_Upper_NonTrans_stride_1_async(st, n, k, L.win1(xs.data_ptr(), 1),
+                                                                        L.win2(A.data_ptr(), k + 1), 0) == 0
+        assert so.exoblas_stbsv_row_major_Upper_Trans_stride_any_async(st, n, kw, L.win1(xw.data_ptr(), 1),
+                                                                       L.win2(Aw.data_ptr(), kw + 1), 0) == 0
+        wa, wb = L.win2(Ak.data_ptr(), K2), L.win2(Bk.data_ptr(), K2)
+        assert so.exoblas_ssyr2k_rm_stride_1_async(st, 122, 111, N2, K2, f(), wa, wa, wb, wb, wa, wa, wb, wb,
+                                                   L.win2(Ck.data_ptr(), N2)) == 0
+        assert so.exoblas_strmm_row_major_Left_Lower_NonTrans_async(st, M3, N3, f(), L.win2(At.data_ptr(), M3),
+                                                                    L.win2(Bt.data_ptr(), N3), 0) == 0
+
+    fill(0)
+    torch.cuda.synchronize()
+    with torch.cuda.stream(s):
+        launch(s.cuda_stream)
+    s.synchronize()
+    graph = torch.cuda.CUDAGraph()
+    fill(1)
+    torch.cuda.synchronize()
+    with torch.cuda.graph(graph, stream=s):
+        launch(torch.cuda.current_stream().cuda_stream)
+    for rep in range(3):
+        fill(200 + rep)
+        torch.cuda.synchronize()
+        graph.replay()
+        torch.cuda.synchronize()
+        got = [t.clone() for t in outs]
+        fill(200 + rep)
+        torch.cuda.synchronize()
+        with torch.cuda.stream(s):
+            launch(s.cuda_stream)
+        s.synchronize()
+        for name, a, b in zip(("sbmv", "tbmv", "tbsv", "tbsv wide", "syr2k", "trmm"), got, outs):
+            assert torch.equal(a, b), f"replay {rep}: {name} differs from the eager call"
+        assert bool(torch.isfinite(xs).all()) and bool(torch.isfinite(xw).all())
