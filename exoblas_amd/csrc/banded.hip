@@ -107,15 +107,28 @@ __global__ __launch_bounds__(NT) void band_lds_kernel(int64_t rows, int64_t ncol
   // global -> LDS, 8 loads in flight per thread (LDS offsets fit 32 bits)
   const uint32_t E = (uint32_t)(nr * wc), wcu = (uint32_t)wc, wsu = (uint32_t)ws;
   if (lda == wc) {
+    // one contiguous range: its 16-byte-aligned interior as streaming 16-byte
+    // loads (8 in flight per thread), the ragged ends element by element; no
+    // load leaves [src, src + E)
+    using V = typename Vec16<T>::type;
+    constexpr int VN = Vec16<T>::n;
     const T* src = A + rs0 * lda;
-    for (uint32_t e0 = threadIdx.x; e0 < E; e0 += 8 * NT) {
-      T v[8];
+    const uint32_t mis = (uint32_t)((reinterpret_cast<uintptr_t>(src) & 15) / sizeof(T));
+    const uint32_t h = std::min<uint32_t>(mis ? (uint32_t)VN - mis : 0u, E);
+    const uint32_t nv = (E - h) / VN, t0 = h + nv * VN;
+    for (uint32_t e = threadIdx.x; e < h; e += NT) band[e] = src[e];
+    for (uint32_t e = t0 + threadIdx.x; e < E; e += NT) band[e] = src[e];
+    const T* body = src + h;
+    for (uint32_t v0 = threadIdx.x; v0 < nv; v0 += 8 * NT) {
+      V r[8];
 #pragma unroll
       for (int u = 0; u < 8; ++u)
-        if (e0 + u * NT < E) v[u] = src[e0 + u * NT];
+        if (v0 + u * NT < nv) r[u] = ld_stream<T>(body + (size_t)(v0 + u * NT) * VN);
 #pragma unroll
       for (int u = 0; u < 8; ++u)
-        if (e0 + u * NT < E) band[e0 + u * NT] = v[u];
+        if (v0 + u * NT < nv)
+#pragma unroll
+          for (int c = 0; c < VN; ++c) band[h + (v0 + u * NT) * VN + c] = vget<T>(r[u], c);
     }
   } else {
     for (uint32_t e0 = threadIdx.x; e0 < E; e0 += 8 * NT) {
