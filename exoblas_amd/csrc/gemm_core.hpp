@@ -94,13 +94,17 @@ struct TileLoader {
   T reg[NE];
   int mode;  // 0: kRK vector, 1: kKR vector, 2: element-wise
 
-  // EXT 1: the operand may be triangular (trmm); 2: concatenated along K
-  // (syr2k).  The plain kernels (EXT 0) compile without either, which would
-  // cost them registers.
+  // EXT 1 and 4: the operand may be triangular (trmm), checked per K tile and
+  // on the diagonal per element; 3: a triangular operand read only strictly
+  // inside its triangle (the kernel clips K to that), so plain loads; 2:
+  // concatenated along K (syr2k).  The plain kernels (EXT 0) compile without
+  // any of it, which would cost them registers.
   template <int EXT>
   __device__ __forceinline__ static T elem(const Operand<T>& op, int form, int64_t r, int64_t k) {
     // form is kRK or kKR, symmetric (select by triangle) or triangular
-    if constexpr (EXT == 1) {
+    if constexpr (EXT == 3)
+      if (form >= kTri) form = (form - kTri) & 1;
+    if constexpr (EXT == 1 || EXT == 4) {
       if (form >= kTri) {
         const int f = form - kTri;
         if (r == k && (f & 4)) return T(1);
@@ -125,13 +129,15 @@ struct TileLoader {
   template <int EXT>
   __device__ __forceinline__ void load(const Operand<T>& op, int64_t r0, int64_t k0, int64_t rows, int64_t K) {
     int form = op.form;
+    if constexpr (EXT == 3)
+      if (form >= kTri) form = (form - kTri) & 1;
     if (form == kSymLower) {
       if (r0 >= k0 + BK - 1) form = kRK;
       else if (r0 + R - 1 < k0) form = kKR;
     } else if (form == kSymUpper) {
       if (r0 + R - 1 <= k0) form = kRK;
       else if (r0 > k0 + BK - 1) form = kKR;
-    } else if (EXT == 1 && form >= kTri) {
+    } else if ((EXT == 1 || EXT == 4) && form >= kTri) {
       const int f = form - kTri;
       // strictly inside the triangle: the plain form; wholly outside: zeros
       const bool inside = (f & 2) ? k0 > r0 + R - 1 : k0 + BK - 1 < r0;
@@ -244,16 +250,27 @@ struct Cfg {
   static_assert(BK % 4 == 0, "BK is a multiple of the MFMA K (4)");
 };
 
-// The part of [kb, ke) where a triangular A operand can be nonzero for rows
-// [m0, m0 + BM): k <= r (lower) ends at m0 + BM, k >= r (upper) starts at m0.
-template <class T>
+// The part of [kb, ke) a triangular A operand contributes to rows
+// [m0, m0 + BM) of the product.  EXT 1: where it can be nonzero (k <= r,
+// lower: up to m0 + BM; k >= r, upper: from m0).  EXT 3: strictly inside the
+// triangle for every row (lower: k < m0; upper: k >= m0 + BM).  EXT 4: the
+// diagonal block [m0, m0 + BM).  3 and 4 together cover 1's range once.
+template <int EXT, class T>
 __device__ __forceinline__ void tri_k_window(const Operand<T>& A, int64_t m0, int BM, int64_t& kb, int64_t& ke) {
-  if (A.form < kTri) return;
-  if ((A.form - kTri) & 2) {
-    if (kb < m0) kb = m0;
-  } else if (ke > m0 + BM) {
-    ke = m0 + BM;
+  const bool upper = (A.form - kTri) & 2;
+  int64_t lo = 0, hi = INT64_MAX;
+  if (EXT == 1) {
+    if (upper) lo = m0;
+    else hi = m0 + BM;
+  } else if (EXT == 3) {
+    if (upper) lo = m0 + BM;
+    else hi = m0;
+  } else {
+    lo = m0;
+    hi = m0 + BM;
   }
+  if (kb < lo) kb = lo;
+  if (ke > hi) ke = hi;
 }
 
 // Accumulates one block tile of  sum_k X_A(m, k) X_B(n, k)  for k in [kb, ke)

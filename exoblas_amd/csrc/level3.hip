@@ -36,7 +36,7 @@ __global__ __launch_bounds__(C::NT) void gemm_kernel(int64_t M, int64_t N, int64
 #pragma unroll
       for (int q = 0; q < 4; ++q) acc[a][b][q] = T(0);
   int64_t kb = 0, ke = K;
-  if constexpr (EXT == 1) tri_k_window(A, m0, C::BM, kb, ke);
+  if constexpr (EXT == 1 || EXT == 3 || EXT == 4) tri_k_window<EXT>(A, m0, C::BM, kb, ke);
   tile_mma<C, EXT>(A, B, M, N, K, m0, n0, kb, ke, acc, reinterpret_cast<T*>(smem));
   tile_store<C, TRI>(Cp, ldc, cvec, M, N, m0, n0, alpha, acc);
 }
@@ -66,7 +66,8 @@ __global__ __launch_bounds__(C::NT) void gemm_splitk_kernel(int64_t M, int64_t N
 #pragma unroll
       for (int q = 0; q < 4; ++q) acc[a][b][q] = T(0);
   int64_t kb = s * ksplit, ke = kb + ksplit < K ? kb + ksplit : K;
-  if constexpr (EXT == 1) tri_k_window(A, m0, C::BM, kb, ke);  // an empty window leaves the sums zero
+  if constexpr (EXT == 1 || EXT == 3 || EXT == 4)
+    tri_k_window<EXT>(A, m0, C::BM, kb, ke);  // an empty window leaves the sums zero
   tile_mma<C, EXT>(A, B, M, N, K, m0, n0, kb, ke, acc, reinterpret_cast<T*>(smem));
   tile_store<C, kFull, true>(P + (s * ntiles + l) * C::BM * C::BN, C::BN, 1, C::BM, C::BN, 0, 0, T(1), acc, T(0));
 }
@@ -253,22 +254,27 @@ void run(Ctx& c, int64_t M, int64_t N, int64_t K, T alpha, const Operand<T>& A, 
   launch_rounds<Small, TRI, EXT>(c, M, N, K, alpha, A, B, Cp, ldc);
 }
 
-// Triangular-operand products (trmm): row tile t of a lower op(A) runs t + 1
-// K tiles and of an upper one the rest, so whole-tile launches wait for the
-// longest rows.  With at least half a round of large tiles, K is split four
-// ways instead: the slices outside a tile's window exit at once and the
-// dispatcher spreads the uneven remainder over the CUs.
+// Triangular-operand products (trmm).  With at least half a round of large
+// tiles, two launches on the same row tiling: the K range strictly inside the
+// triangle with plain operand loads (EXT 3), split four ways along K (row tile
+// t of a lower op(A) has t K blocks, of an upper one the rest: the slices
+// outside a tile's range exit at once and the dispatcher spreads the uneven
+// remainder over the CUs), then each row tile's diagonal block with the
+// per-element triangle (EXT 4).  Smaller products: one launch over the whole
+// triangle window (EXT 1).
 template <class T>
 void run_triangular(Ctx& c, int64_t M, int64_t N, int64_t K, T alpha, const Operand<T>& A, const Operand<T>& B,
                     T* Cp, int64_t ldc) {
   if (M <= 0 || N <= 0 || K <= 0) return;
   using Huge = typename Tiles<T>::Huge;
   const int64_t tiles = tile_count<Huge>(kFull, M, N);
-  const int64_t S = std::min<int64_t>(4, K / (4 * Huge::BK));
-  if (2 * tiles >= c.num_cus && S >= 2)
-    launch_plan<Huge, kFull, 1>(c, 0, S, M, N, K, alpha, A, B, Cp, ldc);
-  else
+  const int64_t S = std::min<int64_t>(4, K / (4 * Huge::BK));  // 8: 1065 -> 1131 us (strmm 4096^2)
+  if (2 * tiles >= c.num_cus && S >= 2) {
+    launch_plan<Huge, kFull, 3>(c, 0, S, M, N, K, alpha, A, B, Cp, ldc);
+    launch_cfg<Huge, kFull, 4>(c, 0, tiles, M, N, K, alpha, A, B, Cp, ldc);
+  } else {
     run<T, kFull, 1>(c, M, N, K, alpha, A, B, Cp, ldc);
+  }
 }
 
 template <class T>
