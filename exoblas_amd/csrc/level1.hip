@@ -77,6 +77,14 @@ template <class T, class Op>
 void launch_ew(Ctx& c, int64_t n, T* x, int64_t incx, T* y, int64_t incy, Op op) {
   if (n <= 0) return;
   constexpr int VN = Vec16<T>::n;
+  // Every vector the op touches walks down with unit stride: element i of
+  // each is at position n - 1 - i of the range ending at element 0, so the
+  // op pairs the same elements as a unit-stride call on those ranges.
+  constexpr bool ux = Op::rx || Op::wx, uy = Op::ry || Op::wy;
+  if ((!ux || incx == -1) && (!uy || incy == -1)) {
+    if (ux) x -= n - 1, incx = 1;
+    if (uy) y -= n - 1, incy = 1;
+  }
   const bool unit = (!(Op::rx || Op::wx) || (incx == 1 && aligned16(x))) &&
                     (!(Op::ry || Op::wy) || (incy == 1 && aligned16(y)));
   if (unit) {
@@ -293,6 +301,13 @@ void launch_reduce(Ctx& c, int64_t n, const T* x, int64_t incx, const T* y, int6
   // stream) streaming loads in flight per round: the best of a sweep over
   // block size, unroll and grid on MI355X (7.1 TB/s for a 2^26 dot).
   constexpr int U = Term::ry ? 2 : 4;
+  // both vectors walking down with unit stride: the same products over the
+  // unit-stride ranges ending at element 0 (summed in another order)
+  if (incx == -1 && (!Term::ry || incy == -1) && n > 0) {
+    x -= n - 1;
+    incx = 1;
+    if (Term::ry) y -= n - 1, incy = 1;
+  }
   const bool vec = incx == 1 && aligned16(x) && (!Term::ry || (incy == 1 && aligned16(y)));
   int64_t blocks = std::max<int64_t>(1, ceil_div(n, (int64_t)kNT * U * VN));
   blocks = std::min<int64_t>(blocks, (int64_t)c.num_cus * 2);

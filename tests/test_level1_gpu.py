@@ -24,7 +24,7 @@ pytestmark = pytest.mark.gpu
 
 DT = [np.float32, np.float64]
 SIZES = [(2, 1, 1), (321, 1, 1), (1000, 1, 1), (2, 2, 2), (321, 3, 3), (1000, 2, -3), (1000, -2, -4),
-         (0, 1, 1), (1 << 20, 1, 1), (4099, 1, 1)]
+         (0, 1, 1), (1 << 20, 1, 1), (4099, 1, 1), (4099, -1, -1), (1 << 20, -1, -1), (1001, -1, 1)]
 
 
 @pytest.fixture(scope="module")
