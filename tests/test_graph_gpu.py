@@ -201,3 +201,55 @@ def test_banded_syr2k_trmm_graph_replay(env):
         for name, a, b in zip(("sbmv", "tbmv", "tbsv", "tbsv wide", "syr2k", "trmm"), got, outs):
             assert torch.equal(a, b), f"replay {rep}: {name} differs from the eager call"
         assert bool(torch.isfinite(xs).all()) and bool(torch.isfinite(xw).all())
+
+
+@pytest.mark.parametrize("mnk", [(512, 512, 512), (64, 96, 257)])
+def test_small_gemm_one_launch_split_graph_replay(env, mnk):
+    """Small outputs with a long K split K over 64 x 64 tiles (slices in the
+    stream's scratch, summed in slice order by a second launch): a captured
+    call replayed on fresh inputs equals the eager call bit for bit, and the
+    product matches float64."""
+    torch, so, L = env
+    M, N, K = mnk
+    g = torch.Generator(device="cuda")
+    A, B, Cm = (torch.empty(sh, device="cuda") for sh in ((M, K), (K, N), (M, N)))
+    al = ctypes.byref(ctypes.c_float(1.5))
+    s = torch.cuda.Stream()
+
+    def fill(seed):
+        g.manual_seed(seed)
+        for t in (A, B, Cm):
+            t.uniform_(-1, 1, generator=g)
+
+    def launch(st):
+        assert so.exoblas_sgemm_stride_1_async(st, 111, 111, M, N, K, al, L.win2(A.data_ptr(), K),
+                                               L.win2(B.data_ptr(), N), L.win2(A.data_ptr(), K),
+                                               L.win2(B.data_ptr(), N), L.win2(Cm.data_ptr(), N)) == 0
+
+    fill(0)
+    torch.cuda.synchronize()
+    with torch.cuda.stream(s):
+        launch(s.cuda_stream)
+    s.synchronize()
+    graph = torch.cuda.CUDAGraph()
+    fill(1)
+    torch.cuda.synchronize()
+    with torch.cuda.graph(graph, stream=s):
+        launch(torch.cuda.current_stream().cuda_stream)
+    eps = float(np.finfo(np.float32).eps)
+    for rep in range(3):
+        fill(300 + rep)
+        C0 = Cm.double()
+        torch.cuda.synchronize()
+        graph.replay()
+        torch.cuda.synchronize()
+        got = Cm.clone()
+        want = C0 + 1.5 * (A.double() @ B.double())
+        mag = C0.abs() + 1.5 * (A.double().abs() @ B.double().abs())
+        assert bool(((got.double() - want).abs() <= 4 * (K + 2) * eps * mag).all()), f"replay {rep}: product"
+        fill(300 + rep)
+        torch.cuda.synchronize()
+        with torch.cuda.stream(s):
+            launch(s.cuda_stream)
+        s.synchronize()
+        assert torch.equal(got, Cm), f"replay {rep}: differs from the eager call"
