@@ -97,3 +97,17 @@ def test_no_cpu_fallback_in_product():
                 with open(os.path.join(dp, f)) as fh:
                     txt = fh.read()
                 assert "import oracle" not in txt and "from oracle" not in txt and "liboracle" not in txt, f
+
+
+def test_cmake_targets_match_the_generated_libraries():
+    """exoblas_amd/cmake/EXO_BLASConfig.cmake defines one EXO_BLAS::exo_<lib>
+    target per generated header (the reference's library targets)."""
+    import re
+
+    from exoblas_amd import generator as G
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    text = open(os.path.join(root, "exoblas_amd", "cmake", "EXO_BLASConfig.cmake")).read()
+    m = re.search(r"foreach\(_k ([^)]*)\)", text)
+    assert m, "no target list"
+    assert sorted(m.group(1).split()) == sorted(G.LIB_ORDER)
+    assert len(G.LIB_ORDER) == 26
