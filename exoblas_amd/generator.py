@@ -19,8 +19,10 @@ gfx950 and emits:
   needed, and calls the adapter ``exoblas::ops::<proc><T>`` with plain device
   pointers, element strides and scalar values.
 
-Tile shapes and launch parameters are not chosen here; the kernel adapters
-pick them from the autotuned table (``exoblas_amd/tuning.py``).
+Tile shapes and launch parameters are not chosen here: the kernel adapters
+pick them (``csrc/level3.hip`` plans, ``csrc/batch.hip`` tile classes), and the
+batched path reads the autotuned table ``exoblas_amd/tuning.json`` written by
+``exoblas_amd/autotune.py``.
 
 Run ``python -m exoblas_amd.generator`` to rewrite the files; the test suite
 checks that the committed files equal what this module produces.
