@@ -113,13 +113,13 @@ __device__ __forceinline__ void gemv_item(const Desc& d, uint32_t local) {
 
 template <bool FWD, bool TRANS, bool UNIT>
 __device__ __forceinline__ void trsv_item(const Desc& d, uint32_t local, unsigned* ctrl, unsigned marker,
-                                          trsv::Smem<float>& sm) {
-  trsv::step<float, FWD, TRANS, UNIT, trsv::kB, false>((int)local, d.n, d.a, d.lda, d.c, ctrl + kCtrlWords + d.flag_off, marker,
-                                      ctrl + 3, sm);
+                                          unsigned* err, trsv::Smem<float>& sm) {
+  trsv::step<float, FWD, TRANS, UNIT, trsv::kB, false>((int)local, d.n, d.a, d.lda, d.c,
+                                                       ctrl + kCtrlWords + d.flag_off, marker, err, sm);
 }
 
 __global__ __launch_bounds__(kNT) void batch_kernel(const Desc* __restrict__ descs, const uint64_t* __restrict__ order,
-                                                    uint32_t total, unsigned* ctrl) {
+                                                    uint32_t total, unsigned* ctrl, unsigned* err) {
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ uint32_t s_item;
   const unsigned marker = __hip_atomic_load(ctrl + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) + 1u;
@@ -143,10 +143,10 @@ __global__ __launch_bounds__(kNT) void batch_kernel(const Desc* __restrict__ des
       auto& sm = *reinterpret_cast<trsv::Smem<float>*>(smem);
       const bool trans = d.ta != 111, lower = d.tb == 122, unit = d.diag == 132;
       const bool fwd = lower != trans;
-      if (fwd && !trans) { if (unit) trsv_item<true, false, true>(d, local, ctrl, marker, sm); else trsv_item<true, false, false>(d, local, ctrl, marker, sm); }
-      if (fwd && trans) { if (unit) trsv_item<true, true, true>(d, local, ctrl, marker, sm); else trsv_item<true, true, false>(d, local, ctrl, marker, sm); }
-      if (!fwd && !trans) { if (unit) trsv_item<false, false, true>(d, local, ctrl, marker, sm); else trsv_item<false, false, false>(d, local, ctrl, marker, sm); }
-      if (!fwd && trans) { if (unit) trsv_item<false, true, true>(d, local, ctrl, marker, sm); else trsv_item<false, true, false>(d, local, ctrl, marker, sm); }
+      if (fwd && !trans) { if (unit) trsv_item<true, false, true>(d, local, ctrl, marker, err, sm); else trsv_item<true, false, false>(d, local, ctrl, marker, err, sm); }
+      if (fwd && trans) { if (unit) trsv_item<true, true, true>(d, local, ctrl, marker, err, sm); else trsv_item<true, true, false>(d, local, ctrl, marker, err, sm); }
+      if (!fwd && !trans) { if (unit) trsv_item<false, false, true>(d, local, ctrl, marker, err, sm); else trsv_item<false, false, false>(d, local, ctrl, marker, err, sm); }
+      if (!fwd && trans) { if (unit) trsv_item<false, true, true>(d, local, ctrl, marker, err, sm); else trsv_item<false, true, false>(d, local, ctrl, marker, err, sm); }
     }
   }
   if (threadIdx.x == 0) {
@@ -407,8 +407,7 @@ void DeviceBatch::upload(const Built& b, hipStream_t stream) {
       size_t c = 1024;
       while (c < nf) c <<= 1;
       EXO_HIP_CHECK(hipMalloc(&ctrl, c * sizeof(unsigned)));
-      EXO_HIP_CHECK(hipMemset(ctrl, 0, c * sizeof(unsigned)));
-      EXO_HIP_CHECK(hipDeviceSynchronize());
+      fill_zero(stream, ctrl, c * sizeof(unsigned));  // ordered before the launch on this stream
       cap_flags = c;
     }
   }
@@ -422,7 +421,9 @@ void DeviceBatch::upload(const Built& b, hipStream_t stream) {
 void launch(const DeviceBatch& d, const Built& b, hipStream_t stream) {
   if (b.total_items == 0) return;
   const int grid = (int)std::min<int64_t>(b.total_items, max_grid());
-  hipLaunchKernelGGL(batch_kernel, dim3(grid), dim3(kNT), kSmem, stream, d.desc, d.order, b.total_items, d.ctrl);
+  unsigned* err = stream_timeout_word(stream);
+  hipLaunchKernelGGL(batch_kernel, dim3(grid), dim3(kNT), kSmem, stream, d.desc, d.order, b.total_items, d.ctrl,
+                     err);
   EXO_LAUNCH_CHECK();
 }
 

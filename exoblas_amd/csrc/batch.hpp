@@ -31,7 +31,8 @@ struct Desc {
 };
 
 // ctrl[0] work queue, ctrl[1] finished workgroups, ctrl[2] launch count,
-// ctrl[3] timeout marker, ctrl[kCtrlWords + i] trsv block flags.
+// ctrl[3] unused, ctrl[kCtrlWords + i] trsv block flags.  Bounded waits
+// report to the timeout word of the stream the batch runs on.
 constexpr int kCtrlWords = 4;
 constexpr int kRows = 64;  // gemv / trsv rows per work item
 
@@ -66,7 +67,8 @@ struct DeviceBatch {
   void upload(const Built& b, hipStream_t stream);
 };
 
-// Enqueues the persistent kernel for an uploaded batch.
+// Enqueues the persistent kernel for an uploaded batch (timeouts go to the
+// stream's timeout word: take_timeout(stream) after completion).
 void launch(const DeviceBatch& d, const Built& b, hipStream_t stream);
 
 }  // namespace batch

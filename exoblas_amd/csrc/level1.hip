@@ -193,7 +193,7 @@ struct DsdotTerm {
   static constexpr bool ry = true;
 };
 
-constexpr int kReduceFlagBase = 2;  // [1] timeout; the 8-byte aligned partials from [2]
+constexpr int kReduceFlagBase = 2;  // the 8-byte aligned partials from [2]
 
 template <class Acc, class Out> struct Finish {
   Acc add;  // sdsdot's sb
@@ -206,12 +206,13 @@ template <class Acc, class Out> struct Finish {
 // t + kNT, ..., then the workgroup tree), so the result does not depend on
 // which workgroup finishes last.  *ticket: the stream's first counter; the
 // marker comes from the stream's launch generation (tagged.hpp), so the
-// launch can be replayed from a hipGraph; flags: [1] timeout, [2 ..] the
-// tagged partials.
+// launch can be replayed from a hipGraph; flags: [2 ..] the tagged
+// partials; err: the stream's timeout word.
 template <class T, class Term, class Out, int U>
 __global__ __launch_bounds__(kNT) void reduce_kernel(int64_t n, const T* __restrict__ x, int64_t incx,
                                                      const T* __restrict__ y, int64_t incy, int vec,
-                                                     unsigned* ticket, unsigned* gen, unsigned* flags, Out* result,
+                                                     unsigned* ticket, unsigned* gen, unsigned* flags, unsigned* err,
+                                                     Out* result,
                                                      Finish<typename Term::Acc, Out> fin, Term term) {
   using Acc = typename Term::Acc;
   using V = typename Vec16<T>::type;
@@ -280,7 +281,7 @@ __global__ __launch_bounds__(kNT) void reduce_kernel(int64_t n, const T* __restr
   Acc t = Acc(0);
   for (int i = threadIdx.x; i < (int)gridDim.x; i += 4 * kNT) {
     Acc v[4];
-    tagged::read_n<Acc, 4>(pub, i, kNT, (int)ceil_div((int)gridDim.x - i, kNT), v, marker, flags + 1);
+    tagged::read_n<Acc, 4>(pub, i, kNT, (int)ceil_div((int)gridDim.x - i, kNT), v, marker, err);
 #pragma unroll
     for (int k = 0; k < 4; ++k)
       if (i + k * kNT < (int)gridDim.x) t += v[k];
@@ -314,9 +315,14 @@ void launch_reduce(Ctx& c, int64_t n, const T* x, int64_t incx, const T* y, int6
   unsigned epoch = 0;  // (the host epoch is not used: the marker comes from the launch generation)
   unsigned* flags = c.epoch_flags((size_t)(kReduceFlagBase + tagged::words<Acc>(blocks)), &epoch);
   hipLaunchKernelGGL((reduce_kernel<T, Term, Out, U>), dim3((unsigned)blocks), dim3(kNT), 0, c.stream, n, x, incx,
-                     y, incy, vec ? 1 : 0, c.counters(), c.counters() + Ctx::kGeneration, flags, result,
+                     y, incy, vec ? 1 : 0, c.counters(), c.counters() + Ctx::kGeneration, flags, c.timeout_word(),
+                     result,
                      Finish<Acc, Out>{add}, Term{});
   EXO_LAUNCH_CHECK();
+}
+
+__global__ __launch_bounds__(kNT) void fill_zero_kernel(uint32_t* p, int64_t words) {
+  for (int64_t i = (int64_t)blockIdx.x * kNT + threadIdx.x; i < words; i += (int64_t)gridDim.x * kNT) p[i] = 0u;
 }
 
 __global__ void store_zero_kernel(char* p, int bytes) {
@@ -331,6 +337,16 @@ void write_zero(Ctx& c, Out* result) {
 }
 
 }  // namespace
+
+void fill_zero(hipStream_t stream, void* p, size_t bytes) {
+  if (bytes == 0) return;
+  if (bytes % 4 != 0 || (reinterpret_cast<uintptr_t>(p) & 3) != 0)
+    throw Error(kInvalidSize, "fill_zero: whole, aligned 32-bit words only");
+  const int64_t words = (int64_t)(bytes / 4);
+  const unsigned grid = (unsigned)std::min<int64_t>(ceil_div(words, kNT), 1024);
+  hipLaunchKernelGGL(fill_zero_kernel, dim3(grid), dim3(kNT), 0, stream, static_cast<uint32_t*>(p), words);
+  EXO_LAUNCH_CHECK();
+}
 
 namespace ops {
 

@@ -529,7 +529,7 @@ void launch_gemv_t(Ctx& c, int64_t Mc, int64_t Nr, T alpha, T beta, const T* A, 
     gen = c.counters() + Ctx::kGeneration;
     lt = c.counters() + Ctx::kLaunchTicket;
     unsigned* flags = c.epoch_flags((size_t)(2 + tagged::words<T>(splits * Mc)), &epoch);
-    err = flags + 1;
+    err = c.timeout_word();
     pub = flags + 2;  // 8-byte aligned
   }
   dim3 grid((unsigned)col_blocks, (unsigned)splits);

@@ -26,6 +26,14 @@ struct StreamState {
   unsigned* flags = nullptr;
   size_t flags_cap = 0;
   unsigned epoch = 0;
+  // pinned host word the kernels' bounded waits report to (device address
+  // timeout_dev)
+  unsigned* timeout_host = nullptr;
+  unsigned* timeout_dev = nullptr;
+  // grown-out scratch and flag areas: kept, not freed, so work queued or
+  // captured with the old addresses stays valid (bounded: each growth at
+  // least doubles)
+  std::vector<void*> retired;
 };
 
 namespace {
@@ -160,14 +168,32 @@ std::map<int, std::unique_ptr<DeviceState>> g_devices;
 thread_local int t_status = kOk;
 thread_local char t_message[512] = {0};
 
+bool capturing(hipStream_t s) {
+  hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+  if (hipStreamIsCapturing(s, &st) != hipSuccess) {
+    (void)hipGetLastError();
+    return false;
+  }
+  return st != hipStreamCaptureStatusNone;
+}
+
+// The state of stream s: its counters (zeroed by a fill kernel on s, so
+// ordered before any of the stream's later work without a device-wide
+// synchronisation) and its timeout word.
 StreamState* make_stream_state(int device, hipStream_t s) {
-  auto* ss = new StreamState();
+  if (capturing(s))
+    throw Error(kNotSupported,
+                "first call on a stream while it is being captured: make one call on the stream before capturing");
+  auto ss = std::make_unique<StreamState>();
   ss->stream = s;
   ss->device = device;
   EXO_HIP_CHECK(hipMalloc(&ss->counters, sizeof(unsigned) * Ctx::kCounters));
-  EXO_HIP_CHECK(hipMemset(ss->counters, 0, sizeof(unsigned) * Ctx::kCounters));
-  EXO_HIP_CHECK(hipDeviceSynchronize());
-  return ss;
+  fill_zero(s, ss->counters, sizeof(unsigned) * Ctx::kCounters);
+  EXO_HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&ss->timeout_host), 64,
+                              hipHostMallocMapped | hipHostMallocCoherent));
+  *ss->timeout_host = 0;
+  EXO_HIP_CHECK(hipHostGetDevicePointer(reinterpret_cast<void**>(&ss->timeout_dev), ss->timeout_host, 0));
+  return ss.release();
 }
 
 DeviceState& device_state(int device) {
@@ -222,15 +248,15 @@ void record_error(int code, const std::string& msg) {
 void* Ctx::scratch(size_t bytes, int slot) {
   if (bytes == 0) bytes = 256;
   if (ss->scratch_cap[slot] < bytes) {
-    if (ss->scratch[slot]) {
-      EXO_HIP_CHECK(hipStreamSynchronize(stream));
-      EXO_HIP_CHECK(hipFree(ss->scratch[slot]));
-      ss->scratch[slot] = nullptr;
-      ss->scratch_cap[slot] = 0;
-    }
+    if (capturing(stream))
+      throw Error(kNotSupported, "scratch must grow to " + std::to_string(bytes) +
+                                     " bytes while the stream is being captured: make the call once before capturing");
     size_t c = 1 << 20;
     while (c < bytes) c <<= 1;
-    EXO_HIP_CHECK(hipMalloc(&ss->scratch[slot], c));
+    void* p = nullptr;
+    EXO_HIP_CHECK(hipMalloc(&p, c));
+    if (ss->scratch[slot]) ss->retired.push_back(ss->scratch[slot]);
+    ss->scratch[slot] = p;
     ss->scratch_cap[slot] = c;
   }
   return ss->scratch[slot];
@@ -238,21 +264,68 @@ void* Ctx::scratch(size_t bytes, int slot) {
 
 unsigned* Ctx::counters() { return ss->counters; }
 
+unsigned* Ctx::timeout_word() { return ss->timeout_dev; }
+
 unsigned* Ctx::epoch_flags(size_t n, unsigned* epoch) {
   if (ss->flags_cap < n || ss->epoch >= 0x7fffffffu) {
-    EXO_HIP_CHECK(hipStreamSynchronize(stream));
-    if (ss->flags) EXO_HIP_CHECK(hipFree(ss->flags));
-    ss->flags = nullptr;
+    if (capturing(stream))
+      throw Error(kNotSupported, "the flag area must grow while the stream is being captured: make the call "
+                                 "once before capturing");
     size_t c = 1024;
     while (c < n) c <<= 1;
-    EXO_HIP_CHECK(hipMalloc(&ss->flags, c * sizeof(unsigned)));
-    EXO_HIP_CHECK(hipMemset(ss->flags, 0, c * sizeof(unsigned)));
-    EXO_HIP_CHECK(hipDeviceSynchronize());
+    unsigned* f = nullptr;
+    EXO_HIP_CHECK(hipMalloc(&f, c * sizeof(unsigned)));
+    fill_zero(stream, f, c * sizeof(unsigned));
+    if (ss->flags) ss->retired.push_back(ss->flags);
+    ss->flags = f;
     ss->flags_cap = c;
     ss->epoch = 0;
   }
   *epoch = ++ss->epoch;
   return ss->flags;
+}
+
+unsigned* stream_timeout_word(hipStream_t s) {
+  int dev = 0;
+  EXO_HIP_CHECK(hipGetDevice(&dev));
+  return stream_state(device_state(dev), s)->timeout_dev;
+}
+
+bool take_timeout(hipStream_t s) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return false;
+  DeviceState& ds = device_state(dev);
+  StreamState* ss = nullptr;
+  {
+    std::lock_guard<std::mutex> g(ds.map_mutex);
+    auto it = ds.streams.find(s);
+    if (it != ds.streams.end()) ss = it->second.get();
+  }
+  if (ss == nullptr) return false;
+  volatile unsigned* w = ss->timeout_host;
+  const unsigned v = *w;
+  if (v != 0) *w = 0;
+  return v != 0;
+}
+
+namespace {
+std::mutex g_spin_mutex;
+std::vector<int (*)(long long)>& spin_setters() {
+  static std::vector<int (*)(long long)> v;
+  return v;
+}
+}  // namespace
+
+void register_spin_setter(int (*setter)(long long)) {
+  std::lock_guard<std::mutex> g(g_spin_mutex);
+  spin_setters().push_back(setter);
+}
+
+int set_spin_limit(long long polls) {
+  std::lock_guard<std::mutex> g(g_spin_mutex);
+  for (auto f : spin_setters())
+    if (f(polls) != 0) return kHipError;
+  return kOk;
 }
 
 Call::Call(const char* name, Mode mode, void* src) : name_(name), mode_(mode) {
@@ -420,6 +493,11 @@ int Call::finish() {
         }
       }
       EXO_HIP_CHECK(hipStreamSynchronize(ctx_.stream));
+      volatile unsigned* w = ctx_.ss->timeout_host;
+      if (*w != 0) {
+        *w = 0;
+        fail(kTimeout, "a bounded wait between workgroups expired: the results are not valid");
+      }
     }
   } catch (const Error& e) {
     fail(e.code, e.what());
@@ -448,6 +526,29 @@ extern "C" int exoblas_init(void) {
     return e.code;
   }
   return 0;
+}
+
+extern "C" int exoblas_stream_check(void* stream_) {
+  hipStream_t stream = static_cast<hipStream_t>(stream_);
+  try {
+    EXO_HIP_CHECK(hipStreamSynchronize(stream));
+    if (exoblas::take_timeout(stream)) {
+      exoblas::record_error(exoblas::kTimeout,
+                            "exoblas_stream_check: a bounded wait between workgroups expired on this stream: "
+                            "results of its calls since the last check are not valid");
+      return exoblas::kTimeout;
+    }
+  } catch (const exoblas::Error& e) {
+    exoblas::record_error(e.code, e.what());
+    return e.code;
+  }
+  return 0;
+}
+
+extern "C" int exoblas_set_spin_limit(long long polls) {
+  const int st = exoblas::set_spin_limit(polls);
+  if (st != 0) exoblas::record_error(st, "exoblas_set_spin_limit: hipMemcpyToSymbol failed");
+  return st;
 }
 
 extern "C" int exoblas_synchronize(void) {

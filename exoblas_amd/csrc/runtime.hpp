@@ -72,9 +72,13 @@ struct Ctx {
   StreamState* ss = nullptr;
 
   // Stream-ordered device scratch of at least `bytes`, 256-byte aligned, in
-  // one of kSlots independent buffers.  A slot's memory is valid until the
-  // next scratch() call for that slot on this stream.  Growing a slot
-  // synchronises the stream first, so earlier kernels never lose theirs.
+  // one of kSlots independent buffers.  A slot's memory is valid for the
+  // work enqueued until the next scratch() call for that slot on this
+  // stream.  Growing a slot allocates a larger buffer and retires the old one
+  // without freeing it (kernels still queued, and hipGraphs captured
+  // earlier, keep valid pointers), so growth never synchronises.  Growth
+  // while the stream is being captured raises kNotSupported: make the
+  // largest call once before capturing.
   void* scratch(size_t bytes, int slot = 0);
   static constexpr int kSlots = 6;
   // kCounters zero-initialised 32-bit counters owned by this stream.  A kernel
@@ -82,6 +86,9 @@ struct Ctx {
   // kGeneration: the stream's launch generation, which single-launch kernels
   // read for their publication marker and advance once per launch (so their
   // markers change from launch to launch also when a hipGraph replays them).
+  // Work that uses them must be serialised on the stream: graphs captured on
+  // this stream and replayed on others share these counters, so such
+  // replays must not overlap each other or this stream's work.
   unsigned* counters();
   static constexpr int kCounters = 4096;
   static constexpr int kGeneration = kCounters - 1;
@@ -89,13 +96,36 @@ struct Ctx {
   // At least n 32-bit completion flags owned by this stream, plus a fresh
   // epoch value.  A kernel marks item i done by storing the epoch into
   // flags[i]; since every call gets a new epoch the flags never need
-  // clearing.  Zeroed when (re)allocated; the epoch restarts then.  Host
-  // epochs stay below 2^31; markers drawn from the launch generation have
-  // the top bit set, so the two never match each other's words.
+  // clearing.  Zeroed (stream-ordered) when (re)allocated; the epoch
+  // restarts then.  Host epochs stay below 2^31; markers drawn from the
+  // launch generation have the top bit set, so the two never match each
+  // other's words.  Grows like scratch(): the old area is retired, not freed.
   unsigned* epoch_flags(size_t n, unsigned* epoch);
+  // Device address of this stream's timeout word: a kernel whose bounded wait
+  // expires stores its (nonzero) marker there.  The word lives in pinned host
+  // memory, so the host reads it after the stream's work completes
+  // (take_timeout) without a copy.
+  unsigned* timeout_word();
 };
 
 void record_error(int code, const std::string& msg);
+
+// Zeroes `bytes` of device memory with a fill kernel enqueued on `stream`
+// (a kernel rather than hipMemsetAsync, so the clear is also correct where a
+// caller captures it in a hipGraph and replays it).
+void fill_zero(hipStream_t stream, void* p, size_t bytes);
+
+// The timeout word of `stream` on the current device (creating the stream's
+// state on first use), and whether a wait on it expired since the last
+// check (reads and clears the word; call once the stream's work is done).
+unsigned* stream_timeout_word(hipStream_t stream);
+bool take_timeout(hipStream_t stream);
+
+// Bounded waits: every kernel unit registers a setter of its copy of the
+// spin limit; set_spin_limit applies a value (polls; < 0 restores the
+// default) to all of them on the current device.
+void register_spin_setter(int (*setter)(long long));
+int set_spin_limit(long long polls);
 
 class Call {
  public:

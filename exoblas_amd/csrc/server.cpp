@@ -131,15 +131,22 @@ void exoblas_server_s::run() {
       if (!s.busy) continue;
       const hipError_t q = hipEventQuery(s.done);
       if (q == hipErrorNotReady) continue;
+      // a bounded wait that expired in the slot's work invalidates its results
+      bool timed_out = false;
+      try {
+        timed_out = q == hipSuccess && take_timeout(s.stream);
+      } catch (const Error&) {
+      }
+      const bool ok = q == hipSuccess && !timed_out;
       const double t = now_us();
       std::lock_guard<std::mutex> g(m);
       for (int64_t id : s.ids) {
-        reqs[id].state = q == hipSuccess ? 2 : 3;
+        reqs[id].state = ok ? 2 : 3;
         reqs[id].t_done = t;
       }
-      if (q != hipSuccess) {
+      if (!ok) {
         failed += (int64_t)s.ids.size();
-        error = hipGetErrorString(q);
+        error = timed_out ? "a bounded wait expired in a batch" : hipGetErrorString(q);
       }
       completed += (int64_t)s.ids.size();
       t_last = t;

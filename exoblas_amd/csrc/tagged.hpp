@@ -7,18 +7,33 @@
 // and high halves) per FP64 value.  Markers are the per-stream call epochs
 // of Ctx::epoch_flags, so words left by earlier calls never match.
 //
-// Waits are bounded: past kSpinLimit polls the reader records its marker in
-// *err and goes on, so no grid can hang the device.
+// Waits are bounded: past the spin limit (kSpinLimit polls unless the host
+// set another through set_spin_limit) the reader records its marker in *err,
+// the stream's timeout word in pinned host memory, and goes on, so no grid
+// can hang the device and the host learns that the results are not valid.
 #pragma once
 
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
 
+#include "runtime.hpp"
+
 namespace exoblas {
 namespace tagged {
 
 constexpr long long kSpinLimit = 1ll << 22;
+
+// This unit's copy of the spin limit, and its setter, registered with the
+// runtime at load time.
+static __constant__ long long g_spin_limit = kSpinLimit;
+namespace {
+int set_unit_spin_limit(long long polls) {
+  const long long v = polls < 0 ? kSpinLimit : polls;
+  return hipMemcpyToSymbol(HIP_SYMBOL(g_spin_limit), &v, sizeof(v)) == hipSuccess ? 0 : 1;
+}
+[[maybe_unused]] const bool g_spin_registered = (register_spin_setter(&set_unit_spin_limit), true);
+}  // namespace
 
 // 32-bit units of publication space for n values of T.
 template <class T>
@@ -31,9 +46,10 @@ __host__ __device__ inline int64_t words(int64_t n) {
 __device__ __forceinline__ unsigned poll_word(uint64_t* w64, unsigned marker, unsigned* err) {
   uint64_t w;
   long long it = 0;
+  const long long limit = g_spin_limit;
   while (((w = __hip_atomic_load(w64, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) >> 32) != marker) {
-    if (++it > kSpinLimit) {
-      __hip_atomic_store(err, marker, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (++it > limit) {
+      __hip_atomic_store(err, marker ? marker : 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
       break;
     }
     __builtin_amdgcn_s_sleep(1);

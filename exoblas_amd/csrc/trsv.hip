@@ -17,14 +17,14 @@ constexpr int64_t kUnitV = 132;
 using trsv::kB;
 using trsv::kNT;
 
-// flags[0]: ticket, flags[1]: timeout (marker on timeout), flags[2 ..]: the
-// publication words of trsv_core.hpp (8-byte aligned).  The marker is the
-// stream's launch generation (tagged.hpp), so a captured solve replays.
+// flags[0]: ticket, flags[2 ..]: the publication words of trsv_core.hpp
+// (8-byte aligned).  The marker is the stream's launch generation
+// (tagged.hpp), so a captured solve replays.  err: the stream's timeout word.
 constexpr int kFlagBase = 2;
 
 template <class T, bool FWD, bool TRANS, bool UNIT, int KB>
 __global__ __launch_bounds__(4 * KB) void trsv_kernel(int64_t n, const T* __restrict__ A, int64_t lda, T* x,
-                                                      unsigned* flags, unsigned* gen) {
+                                                      unsigned* flags, unsigned* gen, unsigned* err) {
   __shared__ trsv::Smem<T, T, KB> sm;
   __shared__ T mb[KB][KB + 1];
   __shared__ int s_p;
@@ -36,7 +36,10 @@ __global__ __launch_bounds__(4 * KB) void trsv_kernel(int64_t n, const T* __rest
     s_marker = marker;
   }
   __syncthreads();
-  trsv::step<T, FWD, TRANS, UNIT, KB>(s_p, n, A, lda, x, flags + kFlagBase, s_marker, flags + 1, sm, mb);
+  // tickets come from the stream's counter: a ticket past the last block
+  // (only if launches sharing the counter overlapped) runs no step
+  if (s_p >= nb) return;
+  trsv::step<T, FWD, TRANS, UNIT, KB>(s_p, n, A, lda, x, flags + kFlagBase, s_marker, err, sm, mb);
 }
 
 // 64-row blocks.  (128-row blocks halve the number of dependent steps but
@@ -51,10 +54,10 @@ void launch_trsv_kb(Ctx& c, bool unit, int64_t n, const T* A, int64_t lda, T* x)
   unsigned* gen = c.counters() + Ctx::kGeneration;
   if (unit)
     hipLaunchKernelGGL((trsv_kernel<T, FWD, TRANS, true, KB>), dim3((unsigned)nb), dim3(4 * KB), 0, c.stream, n, A,
-                       lda, x, flags, gen);
+                       lda, x, flags, gen, c.timeout_word());
   else
     hipLaunchKernelGGL((trsv_kernel<T, FWD, TRANS, false, KB>), dim3((unsigned)nb), dim3(4 * KB), 0, c.stream, n,
-                       A, lda, x, flags, gen);
+                       A, lda, x, flags, gen, c.timeout_word());
   EXO_LAUNCH_CHECK();
 }
 

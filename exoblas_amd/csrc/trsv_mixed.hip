@@ -33,7 +33,7 @@ template <class TO, bool FWD, bool TRANS, bool UNIT>
 __global__ __launch_bounds__(kNT) void trsv_mixed_kernel(int64_t n, const float* __restrict__ A, int64_t lda,
                                                          const TO* __restrict__ Aoff, int64_t ldo,
                                                          const double* __restrict__ rhs, double* x, float* xf,
-                                                         unsigned* flags, unsigned* gen) {
+                                                         unsigned* flags, unsigned* gen, unsigned* err) {
   __shared__ trsv::Smem<float, double> sm;
   __shared__ int s_p;
   const int nb = (int)((n + kB - 1) / kB);
@@ -45,6 +45,7 @@ __global__ __launch_bounds__(kNT) void trsv_mixed_kernel(int64_t n, const float*
     s_marker = marker;
   }
   __syncthreads();
+  if (s_p >= nb) return;  // only if launches sharing the stream's ticket overlapped
   const int p = s_p;
   const unsigned epoch = s_marker;
   unsigned* done = flags + kFlagBase;
@@ -76,7 +77,7 @@ __global__ __launch_bounds__(kNT) void trsv_mixed_kernel(int64_t n, const float*
       if (qq < p) {
         if (qq + 1 < p) load_m(qq + 1, m[h ^ 1]);
         float* xs = sm.xs[h];
-        trsv::fetch_block<float>(n, FWD ? qq : nb - 1 - qq, done, epoch, flags + 1, xs);
+        trsv::fetch_block<float>(n, FWD ? qq : nb - 1 - qq, done, epoch, err, xs);
         __syncthreads();
 #pragma unroll
         for (int e = 0; e < 16; ++e) acc += m[h][e] * xs[cq * 16 + e];
@@ -162,10 +163,10 @@ void solve(Ctx& c, bool unit, int64_t n, const float* A, int64_t lda, const TO* 
   unsigned* gen = c.counters() + Ctx::kGeneration;
   if (unit)
     hipLaunchKernelGGL((trsv_mixed_kernel<TO, FWD, TRANS, true>), dim3((unsigned)nb), dim3(kNT), 0, c.stream, n, A,
-                       lda, Aoff, ldo, rhs, x, xf, flags, gen);
+                       lda, Aoff, ldo, rhs, x, xf, flags, gen, c.timeout_word());
   else
     hipLaunchKernelGGL((trsv_mixed_kernel<TO, FWD, TRANS, false>), dim3((unsigned)nb), dim3(kNT), 0, c.stream, n, A,
-                       lda, Aoff, ldo, rhs, x, xf, flags, gen);
+                       lda, Aoff, ldo, rhs, x, xf, flags, gen, c.timeout_word());
   EXO_LAUNCH_CHECK();
 }
 

@@ -414,9 +414,15 @@ def emit_async_header() -> str:
            " * success, otherwise an EXOBLAS_STATUS_* code (exoblas_amd.h).\n"
            " * Values that workgroups exchange through memory (reductions, split\n"
            " * gemv, trsv) carry a marker drawn from a per-stream counter on the\n"
-           " * device, so a call captured in a hipGraph replays correctly.  The first\n"
-           " * call of a kind on a stream may allocate its buffers: make it once\n"
-           " * before capturing. */\n",
+           " * device, so a call captured in a hipGraph replays correctly.  A call\n"
+           " * that needs more scratch than its stream holds allocates a larger\n"
+           " * buffer (without synchronising; the old one stays valid for work\n"
+           " * already queued or captured); during capture such growth is refused\n"
+           " * with EXOBLAS_STATUS_NOT_SUPPORTED, so make the largest call once on\n"
+           " * the stream before capturing.  Calls on one stream, and replays of\n"
+           " * graphs captured on it, share its counters: serialise them.  Waits\n"
+           " * between workgroups are bounded; exoblas_stream_check reports one\n"
+           " * that expired (exoblas_amd.h). */\n",
            "#ifndef EXOBLAS_AMD_ASYNC_H\n#define EXOBLAS_AMD_ASYNC_H\n\n",
            "#include <hip/hip_runtime_api.h>\n#include \"exo_types.h\"\n\n",
            "#ifdef __cplusplus\nextern \"C\" {\n#endif\n\n"]

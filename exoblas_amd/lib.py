@@ -95,6 +95,10 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.exoblas_clear_error.restype = None
     lib.exoblas_init.restype = ctypes.c_int
     lib.exoblas_synchronize.restype = ctypes.c_int
+    lib.exoblas_stream_check.restype = ctypes.c_int
+    lib.exoblas_stream_check.argtypes = [ctypes.c_void_p]
+    lib.exoblas_set_spin_limit.restype = ctypes.c_int
+    lib.exoblas_set_spin_limit.argtypes = [ctypes.c_longlong]
     _extend(lib)
     _lib = lib
     return lib

@@ -32,6 +32,16 @@ const char *exoblas_last_error(void);
 void exoblas_clear_error(void);
 /* Waits for the library's own stream on the current device. */
 int exoblas_synchronize(void);
+/* Waits for `stream` (a hipStream_t) and reports whether a bounded wait
+ * between workgroups expired in the library's work on it since the last
+ * check: EXOBLAS_STATUS_TIMEOUT means the results of those calls are not
+ * valid.  The synchronous exo_* calls make this check themselves. */
+int exoblas_stream_check(void *stream);
+/* Bound of every wait between workgroups, in polls (< 0 restores the
+ * default of 2^22, about a second).  A test hook: a small bound makes waits
+ * give up early, so the timeout path can be exercised while every kernel
+ * still ends normally.  Applies to the current device. */
+int exoblas_set_spin_limit(long long polls);
 
 #ifdef __cplusplus
 }

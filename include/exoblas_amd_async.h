@@ -7,9 +7,15 @@
  * success, otherwise an EXOBLAS_STATUS_* code (exoblas_amd.h).
  * Values that workgroups exchange through memory (reductions, split
  * gemv, trsv) carry a marker drawn from a per-stream counter on the
- * device, so a call captured in a hipGraph replays correctly.  The first
- * call of a kind on a stream may allocate its buffers: make it once
- * before capturing. */
+ * device, so a call captured in a hipGraph replays correctly.  A call
+ * that needs more scratch than its stream holds allocates a larger
+ * buffer (without synchronising; the old one stays valid for work
+ * already queued or captured); during capture such growth is refused
+ * with EXOBLAS_STATUS_NOT_SUPPORTED, so make the largest call once on
+ * the stream before capturing.  Calls on one stream, and replays of
+ * graphs captured on it, share its counters: serialise them.  Waits
+ * between workgroups are bounded; exoblas_stream_check reports one
+ * that expired (exoblas_amd.h). */
 #ifndef EXOBLAS_AMD_ASYNC_H
 #define EXOBLAS_AMD_ASYNC_H
 
