@@ -293,14 +293,24 @@ __global__ __launch_bounds__(kNT) void reduce_kernel(int64_t n, const T* __restr
   }
 }
 
-template <class T, class Term, class Out>
-void launch_reduce(Ctx& c, int64_t n, const T* x, int64_t incx, const T* y, int64_t incy, Out* result,
-                   typename Term::Acc add) {
-  using Acc = typename Term::Acc;
+// Launch shape of a reduction over n elements on a device with `cus` CUs:
+// the (possibly re-based) operands, whether the 16-byte path applies and
+// the grid.  Two workgroups per CU, each lane with U = 2 (two streams) or 4
+// (one stream) streaming loads in flight per round: the best of a sweep over
+// block size, unroll and grid on MI355X (7.1 TB/s for a 2^26 dot).
+template <class T>
+struct ReducePlan {
+  const T* x;
+  int64_t incx;
+  const T* y;
+  int64_t incy;
+  bool vec;
+  int64_t blocks;
+};
+
+template <class T, class Term>
+ReducePlan<T> plan_reduce(int64_t n, const T* x, int64_t incx, const T* y, int64_t incy, int cus) {
   constexpr int VN = Vec16<T>::n;
-  // Two workgroups per CU, each lane with U = 2 (two streams) or 4 (one
-  // stream) streaming loads in flight per round: the best of a sweep over
-  // block size, unroll and grid on MI355X (7.1 TB/s for a 2^26 dot).
   constexpr int U = Term::ry ? 2 : 4;
   // both vectors walking down with unit stride: the same products over the
   // unit-stride ranges ending at element 0 (summed in another order)
@@ -309,15 +319,38 @@ void launch_reduce(Ctx& c, int64_t n, const T* x, int64_t incx, const T* y, int6
     incx = 1;
     if (Term::ry) y -= n - 1, incy = 1;
   }
-  const bool vec = incx == 1 && aligned16(x) && (!Term::ry || (incy == 1 && aligned16(y)));
-  int64_t blocks = std::max<int64_t>(1, ceil_div(n, (int64_t)kNT * U * VN));
-  blocks = std::min<int64_t>(blocks, (int64_t)c.num_cus * 2);
+  ReducePlan<T> p;
+  p.x = x;
+  p.incx = incx;
+  p.y = y;
+  p.incy = incy;
+  p.vec = incx == 1 && aligned16(x) && (!Term::ry || (incy == 1 && aligned16(y)));
+  p.blocks = std::min<int64_t>(std::max<int64_t>(1, ceil_div(n, (int64_t)kNT * U * VN)), (int64_t)cus * 2);
+  return p;
+}
+
+// Enqueues the one-launch reduction: ticket and gen are the launch's ticket
+// counter and launch generation, flags + kReduceFlagBase (8-byte aligned)
+// holds the tagged partials, err is the timeout word.
+template <class T, class Term, class Out>
+void enqueue_reduce(hipStream_t stream, const ReducePlan<T>& p, int64_t n, unsigned* ticket, unsigned* gen,
+                    unsigned* flags, unsigned* err, Out* result, typename Term::Acc add) {
+  using Acc = typename Term::Acc;
+  constexpr int U = Term::ry ? 2 : 4;
+  hipLaunchKernelGGL((reduce_kernel<T, Term, Out, U>), dim3((unsigned)p.blocks), dim3(kNT), 0, stream, n, p.x,
+                     p.incx, p.y, p.incy, p.vec ? 1 : 0, ticket, gen, flags, err, result, Finish<Acc, Out>{add},
+                     Term{});
+}
+
+template <class T, class Term, class Out>
+void launch_reduce(Ctx& c, int64_t n, const T* x, int64_t incx, const T* y, int64_t incy, Out* result,
+                   typename Term::Acc add) {
+  using Acc = typename Term::Acc;
+  const ReducePlan<T> p = plan_reduce<T, Term>(n, x, incx, y, incy, c.num_cus);
   unsigned epoch = 0;  // (the host epoch is not used: the marker comes from the launch generation)
-  unsigned* flags = c.epoch_flags((size_t)(kReduceFlagBase + tagged::words<Acc>(blocks)), &epoch);
-  hipLaunchKernelGGL((reduce_kernel<T, Term, Out, U>), dim3((unsigned)blocks), dim3(kNT), 0, c.stream, n, x, incx,
-                     y, incy, vec ? 1 : 0, c.counters(), c.counters() + Ctx::kGeneration, flags, c.timeout_word(),
-                     result,
-                     Finish<Acc, Out>{add}, Term{});
+  unsigned* flags = c.epoch_flags((size_t)(kReduceFlagBase + tagged::words<Acc>(p.blocks)), &epoch);
+  enqueue_reduce<T, Term, Out>(c.stream, p, n, c.counters(), c.counters() + Ctx::kGeneration, flags,
+                               c.timeout_word(), result, add);
   EXO_LAUNCH_CHECK();
 }
 
@@ -460,3 +493,99 @@ EXO_L1_INST(double)
 
 }  // namespace ops
 }  // namespace exoblas
+
+// ------------------------------------------------------------ allocation-free sdot
+//
+// exoblas_sdot_noalloc_async: the sdot reduction for callers that cannot
+// allocate device memory.  Its scratch (ticket, launch generation, timeout
+// word and the tagged per-workgroup partials) is the file-scope array
+// `scratch` below instead of the stream's buffers, so a call allocates
+// nothing.  The host never names the array in a launch: each device's copy
+// is resolved once with hipGetSymbolAddress and that address is what the
+// kernel receives.  Every call on a device shares its copy, so calls that
+// overlap on one device (on different streams, or replays of graphs that
+// overlap) race; serialise them.
+#include <array>
+#include <atomic>
+
+#include "exoblas_amd.h"
+#include "exoblas_amd_noalloc.h"
+
+namespace {
+
+constexpr int kNoAllocHeader = 4;            // [0] ticket, [1] generation, [2] timeout word, [3] unused
+constexpr int kNoAllocMaxBlocks = 2048;      // partial slots: 2 workgroups per CU on up to 1024 CUs
+// tagged partials: one 64-bit word (two 32-bit units) per FP32 value
+constexpr size_t kNoAllocWords = kNoAllocHeader + 2 * kNoAllocMaxBlocks;  // 16,400 bytes
+
+static __device__ __align__(256) unsigned scratch[kNoAllocWords];
+
+struct NoAllocDevice {
+  std::atomic<unsigned*> ptr{nullptr};  // this device's copy of `scratch`
+  std::atomic<int> cus{0};
+};
+constexpr int kMaxDevices = 64;
+std::array<NoAllocDevice, kMaxDevices> g_noalloc;
+std::mutex g_noalloc_mutex;
+
+int noalloc_fail(int code, const char* msg) {
+  exoblas::record_error(code, msg);
+  return code;
+}
+
+// Resolves the device's copy of `scratch` (with `dev` current) on the
+// device's first call; later calls read the cached address.
+int noalloc_device(int dev, unsigned** ptr, int* cus) {
+  NoAllocDevice& d = g_noalloc[dev];
+  unsigned* p = d.ptr.load(std::memory_order_acquire);
+  if (p == nullptr) {
+    std::lock_guard<std::mutex> g(g_noalloc_mutex);
+    p = d.ptr.load(std::memory_order_acquire);
+    if (p == nullptr) {
+      int n = 0;
+      if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0)
+        return noalloc_fail(exoblas::kHipError, "exoblas_sdot_noalloc_async: CU count query failed");
+      void* q = nullptr;
+      if (hipGetSymbolAddress(&q, HIP_SYMBOL(scratch)) != hipSuccess || q == nullptr)
+        return noalloc_fail(exoblas::kHipError, "exoblas_sdot_noalloc_async: hipGetSymbolAddress(scratch) failed");
+      d.cus.store(n, std::memory_order_relaxed);
+      p = static_cast<unsigned*>(q);
+      d.ptr.store(p, std::memory_order_release);
+    }
+  }
+  *ptr = p;
+  *cus = d.cus.load(std::memory_order_relaxed);
+  return exoblas::kOk;
+}
+
+}  // namespace
+
+extern "C" int exoblas_sdot_noalloc_async(hipStream_t stream, int_fast32_t n, struct exo_win_1f32c x,
+                                          struct exo_win_1f32c y, float* result) {
+  using namespace exoblas;
+  if (n < 0) return noalloc_fail(kInvalidSize, "exoblas_sdot_noalloc_async: n is negative");
+  if (result == nullptr || (n > 0 && (x.data == nullptr || y.data == nullptr)))
+    return noalloc_fail(kInvalidPointer, "exoblas_sdot_noalloc_async: null pointer");
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices)
+    return noalloc_fail(kHipError, "exoblas_sdot_noalloc_async: no usable current device");
+  unsigned* p = nullptr;
+  int cus = 0;
+  const int st = noalloc_device(dev, &p, &cus);
+  if (st != kOk) return st;
+  if (n == 0) {
+    hipLaunchKernelGGL(store_zero_kernel, dim3(1), dim3(64), 0, stream, reinterpret_cast<char*>(result),
+                       (int)sizeof(float));
+  } else {
+    const ReducePlan<float> plan = plan_reduce<float, DotTerm<float>>(n, x.data, x.strides[0], y.data,
+                                                                       y.strides[0], cus);
+    if (plan.blocks > kNoAllocMaxBlocks)
+      return noalloc_fail(kInvalidSize, "exoblas_sdot_noalloc_async: the grid needs more partial slots than the "
+                                        "fixed scratch holds (2048)");
+    // ticket p[0], generation p[1], timeout word p[2]; partials from p[4]
+    // (the kernel's flags + kReduceFlagBase with flags = p + 2)
+    enqueue_reduce<float, DotTerm<float>, float>(stream, plan, n, p, p + 1, p + 2, p + 2, result, 0.f);
+  }
+  if (hipGetLastError() != hipSuccess) return noalloc_fail(kHipError, "exoblas_sdot_noalloc_async: launch failed");
+  return kOk;
+}

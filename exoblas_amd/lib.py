@@ -99,6 +99,8 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.exoblas_stream_check.argtypes = [ctypes.c_void_p]
     lib.exoblas_set_spin_limit.restype = ctypes.c_int
     lib.exoblas_set_spin_limit.argtypes = [ctypes.c_longlong]
+    lib.exoblas_sdot_noalloc_async.restype = ctypes.c_int
+    lib.exoblas_sdot_noalloc_async.argtypes = [ctypes.c_void_p, c_int_fast32, Win1, Win1, ctypes.c_void_p]
     _extend(lib)
     _lib = lib
     return lib
