@@ -1,15 +1,19 @@
-"""Bandwidth-set benchmark of the exo_* BLAS on one MI355X.
+"""Benchmark of the exo_* BLAS on one MI355X.
 
-Workload (one "step" = one pass over the set, every operand resident in HBM):
+Default run: one JSON line whose headline is the bandwidth set, with every
+other configuration under "configs".
+
+Headline workload (one "step" = one pass over the set, every operand resident
+in HBM):
   saxpy, daxpy   n = 2^26, alpha = 0.75, x, y ~ U[-1,1]
   sdot, ddot     n = 2^26
   sgemv, dgemv   N and T, 8192 x 8192 row-major, lda = 8192, alpha = 0.5, beta = 1.5
   scopy, sswap   n = 2^26 (bit-exact data movement)
   srot           n = 2^26, c = 0.6, s = 0.8
 Each kernel is called through its device-pointer C-ABI entry point
-(exoblas_<k>_stride_1_async) on torch's current stream.  Operands rotate
-between two buffer sets (> 512 MB apart in total) so the 256 MB Infinity
-Cache cannot serve a kernel's inputs from the previous step.
+(exoblas_<k>_stride_1_async) on torch's current stream.  Two operand sets
+(> 512 MB each) alternate from kernel to kernel, so the 256 MB Infinity Cache
+never holds what the next kernel reads.
 
 value = sum of algorithmic bytes over the set / measured time per step (GB/s).
 Algorithmic bytes per call: axpy 3ns, dot 2ns, gemv (MN + N + 2M)s, copy 2ns,
@@ -17,7 +21,13 @@ swap 4ns, rot 4ns (s = element size).  roofline = the kernel with the largest
 share of the step, timed with HIP events around its launches on the same
 stream, against the 8 TB/s HBM3E peak.  cpu_baseline = the oracle's loop nests
 built with the reference's avx512 preset flags, one thread, timed on this
-host over a bounded sample of the same set.
+host over a bounded sample of the same set, with the same loop nests on every
+usable host thread (OpenMP) and the host's vendor CBLAS beside it.
+
+configs: config1 (sgemm 512^3 with beta), config3 (triangular and symmetric),
+config4 (split precision and the mixed solve), config5 (serving), families
+(the kernels the set leaves out); each with its own roofline object and CPU
+baseline, each runnable alone with --workload <name>.
 """
 from __future__ import annotations
 
@@ -105,10 +115,63 @@ def make_launchers(so, L, ops, n1, n2, stream):
     return f
 
 
-def cpu_baseline(n1, n2, budget_s=15.0):
-    """The oracle's loop nests (avx512 preset build), one thread, on a bounded
-    sample: every kernel of the set at its full size, repeated while the
-    budget lasts (at least once)."""
+def host_cpus():
+    """What this process may use of the host: the affinity mask and the
+    cgroup CPU quota (a container can see more CPUs than its quota lets it
+    run; threads beyond the quota only contend)."""
+    aff = len(os.sched_getaffinity(0))
+    quota = None
+    try:
+        with open("/sys/fs/cgroup/cpu.max") as f:  # cgroup v2: "<quota> <period>" or "max <period>"
+            q, per = f.read().split()
+            if q != "max":
+                quota = int(q) / int(per)
+    except (OSError, ValueError):
+        try:
+            with open("/sys/fs/cgroup/cpu/cpu.cfs_quota_us") as f:
+                q = int(f.read())
+            with open("/sys/fs/cgroup/cpu/cpu.cfs_period_us") as f:
+                per = int(f.read())
+            if q > 0:
+                quota = q / per
+        except (OSError, ValueError):
+            pass
+    use = aff if quota is None else max(1, min(aff, int(quota)))
+    return {"os_cpu_count": os.cpu_count(), "affinity": aff, "cgroup_quota_cpus": quota, "threads": use}
+
+
+def set_port_threads(n):
+    """Thread count of the all-cores oracle build (its OpenMP runtime)."""
+    from oracle import load
+    load("omp")
+    ctypes.CDLL("libgomp.so.1").omp_set_num_threads(ctypes.c_int(n))
+
+
+def cpu_rate(fn, budget_s, min_reps=1):
+    """(seconds per call, calls) of fn() repeated while the budget lasts,
+    after one untimed call."""
+    fn()
+    reps, t0 = 0, time.perf_counter()
+    while True:
+        fn()
+        reps += 1
+        el = time.perf_counter() - t0
+        if reps >= min_reps and el > budget_s:
+            return el / reps, reps
+
+
+def _mkl():
+    import ctypes.util
+    path = next((q for q in ("/opt/conda/lib/libmkl_rt.so", ctypes.util.find_library("mkl_rt") or "")
+                 if q and os.path.exists(q)), None)
+    return (ctypes.CDLL(path), os.path.basename(path)) if path else (None, None)
+
+
+def cpu_baseline(n1, n2, budget_s=15.0, kind="fast", threads=1):
+    """The oracle's loop nests (avx512 preset build), one thread ("fast") or
+    every usable host thread ("omp"), on a bounded sample: every kernel of
+    the set at its full size, repeated while the budget lasts (at least
+    once)."""
     import numpy as np
 
     from oracle import Oracle, _host_has_avx512
@@ -125,7 +188,7 @@ def cpu_baseline(n1, n2, budget_s=15.0):
     t_start = time.perf_counter()
     while True:
         for name, p, kind, nbytes in ks:
-            o = Oracle(np.float32 if p == "s" else np.float64, kind="fast")
+            o = Oracle(np.float32 if p == "s" else np.float64, kind=kind)
             d = data[p]
             t0 = time.perf_counter()
             if kind == "axpy":
@@ -147,9 +210,10 @@ def cpu_baseline(n1, n2, budget_s=15.0):
         reps_done += 1
         if time.perf_counter() - t_start > budget_s:
             break
-    return {"value": round(total_bytes / total_s / 1e9, 3), "unit": "GB/s", "cores": 1, "kind": "port",
+    return {"value": round(total_bytes / total_s / 1e9, 3), "unit": "GB/s", "cores": threads, "kind": "port",
             "sample": f"{reps_done} pass(es) over the full bandwidth set ({'avx512' if _host_has_avx512() else 'avx2'}"
-                      f" -O3 -ffast-math build of the oracle, 1 thread, host-resident operands)"}
+                      f" -O3 -ffast-math build of the oracle, {threads} thread(s)"
+                      f"{' (OpenMP over independent outer loops)' if kind == 'omp' else ''}, host-resident operands)"}
 
 
 def vendor_cblas(n1, n2, budget_s, threads):
@@ -229,8 +293,11 @@ def bench_bandwidth(a, torch, so, L):
     launch = [[make_launchers(so, L, s, a.n1, a.n2, sh)(name, p, kind) for name, p, kind, _ in ks] for s in sets]
 
     def step(i, evs=None):
-        fs = launch[i % 2]
-        for j, f in enumerate(fs):
+        # kernel j of step i uses operand set (i + j) % 2: consecutive kernels
+        # (and the same kernel in consecutive steps) never share operands, so
+        # the 256 MB Infinity Cache holds nothing the next kernel reads
+        for j in range(len(ks)):
+            f = launch[(i + j) % 2][j]
             if evs is not None:
                 evs[j][0].record(stream)
             rc = f()
@@ -293,11 +360,7 @@ def bench_bandwidth(a, torch, so, L):
         "kernels": kernels,
     }
     if not a.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(a.n1, a.n2, a.cpu_budget)
-        out["cpu_baseline"]["cores_on_host"] = os.cpu_count()
-        vend = [vendor_cblas(a.n1, a.n2, a.cpu_budget / 3, t) for t in (1, os.cpu_count() or 1)]
-        if vend[0] is not None:
-            out["cpu_baseline"]["vendor_cblas"] = vend
+        out["cpu_baseline"] = cpu_leg("bandwidth", a)
     return out
 
 
@@ -764,8 +827,657 @@ def bench_hostpath(a, torch, so, L):
                  "sgemm NN 512^3 (mscal + gemm), strsv L-N-N 4096", None, kernels)
 
 
-WORKLOADS = {"bandwidth": bench_bandwidth, "level3": bench_level3, "trsv": bench_trsv, "extended": bench_extended,
-             "precision": bench_precision, "serving": bench_serving, "hostpath": bench_hostpath}
+# ---------------------------------------------------------------- the other configurations
+# The default run also measures every other configuration and the kernel
+# families the bandwidth set leaves out, each with its own roofline and CPU
+# baseline, under "configs" in the same JSON line.  Each section also runs
+# alone with --workload <name> (for profiling).
+
+def _u(torch, g, shape, dt):
+    return torch.empty(shape, dtype=dt, device="cuda").uniform_(-1, 1, generator=g)
+
+
+def _roof(kernel, achieved, peak, unit, bound, traffic=None):
+    return {"bound": bound, "kernel": kernel, "achieved": round(achieved, 3), "peak": peak, "unit": unit,
+            "frac": round(achieved / peak, 4), "traffic": traffic}
+
+
+def _fl(v, p="s"):
+    return ctypes.byref((ctypes.c_float if p == "s" else ctypes.c_double)(v))
+
+
+def _timed_alt(torch, stream, fns, reps):
+    """Average ms per launch of the callables in fns, taken in turn (each
+    its own operand set), HIP events on `stream`."""
+    for f in fns:
+        f()
+    return _timed(torch, stream, lambda it=iter(range(1 << 30)): fns[next(it) % len(fns)](), reps)
+
+
+def _port(fn, work, div, unit, budget, sample, threads=1):
+    sec, reps = cpu_rate(fn, budget)
+    return {"value": round(work / sec / div, 4), "unit": unit, "cores": threads, "kind": "port",
+            "sample": f"{sample} ({reps} timed call(s), host-resident)"}
+
+
+def _mkl_call(budget, threads, fn):
+    mkl, name = _mkl()
+    if mkl is None:
+        return None, None
+    mkl.MKL_Set_Num_Threads(ctypes.c_int(threads))
+    sec, reps = cpu_rate(lambda: fn(mkl), budget)
+    return sec, (name, reps)
+
+
+def _vendor(budget, threads, fn, work, div, unit, sample):
+    sec, info = _mkl_call(budget, threads, fn)
+    if sec is None:
+        return None
+    return {"value": round(work / sec / div, 4), "unit": unit, "threads": threads,
+            "sample": f"{sample}, {info[0]} ({info[1]} timed call(s))"}
+
+
+def _ptr(x):
+    return x.ctypes.data_as(ctypes.c_void_p)
+
+
+def config1(a, torch, so, L):
+    """Config 1: sgemm NN 512^3, A, B, C ~ U[-1,1] from seed 0, alpha 1.5,
+    beta 0.5 applied as the wrapper does (mscal, then the product;
+    test/level3/gemm/exo_gemm_wrapper.h:18-20), device-resident."""
+    from oracle import Oracle
+    stream = torch.cuda.current_stream()
+    sh = stream.cuda_stream
+    n = 512
+    rng = np.random.default_rng(0)
+    A, B, C = (rng.uniform(-1, 1, (n, n)).astype(np.float32) for _ in range(3))
+    dA, dB, dC = (torch.from_numpy(t).cuda() for t in (A, B, C))
+    wa, wb, wc = L.win2(dA.data_ptr(), n), L.win2(dB.data_ptr(), n), L.win2(dC.data_ptr(), n)
+
+    def wrapper_sequence():
+        r1 = so.exoblas_smscal_rm_stride_1_async(sh, n, n, _fl(0.5), wc)
+        r2 = so.exoblas_sgemm_stride_1_async(sh, 111, 111, n, n, n, _fl(1.5), wa, wb, wa, wb, wc)
+        return r1 or r2
+    for _ in range(5):
+        wrapper_sequence()
+    ms = _timed(torch, stream, wrapper_sequence, 50)
+    flop = 2.0 * n ** 3
+    tf = flop / (ms * 1e-3) / 1e12
+    kernels = {"sgemm_NN_512_beta": {"us": round(ms * 1e3, 2), "TFLOP/s": round(tf, 2), "flop": flop,
+                                     "launches": "mscal + gemm"}}
+    out = {"workload": "sgemm NN 512^3, alpha 1.5, beta 0.5 (mscal + gemm), seed 0", "us": round(ms * 1e3, 2),
+           "TFLOP/s": round(tf, 2), "roofline": _roof("sgemm_NN_512_beta", tf, FP32_PEAK_TF, "TFLOP/s", "mfma"),
+           "kernels": kernels}
+    if not a.no_cpu:
+        out["cpu_baseline"] = cpu_leg("config1", a)
+    return out
+
+
+def _dd(torch, g, n, dt):
+    A = _u(torch, g, (n, n), dt) / n
+    A.diagonal().copy_(n + torch.empty(n, dtype=dt, device="cuda").uniform_(0, 1, generator=g))
+    return A
+
+
+def config3(a, torch, so, L):
+    """Config 3: s/d trsv lower / no-trans / non-unit at n = 4096 (the
+    diagonally dominant matrices, b ~ U[-1,1]; x reset before each timed
+    solve, outside the timed region), s/d syrk lower-N N = 4096 K = 1024 and
+    s/d symm left-lower M = N = 4096 (alpha 1.5)."""
+    from oracle import Oracle
+    stream = torch.cuda.current_stream()
+    sh = stream.cuda_stream
+    g = torch.Generator(device="cuda")
+    g.manual_seed(0)
+    kernels = {}
+    for p, dt, s_, peak in (("s", torch.float32, 4, FP32_PEAK_TF), ("d", torch.float64, 8, FP64_PEAK_TF)):
+        n = 4096
+        A = _dd(torch, g, n, dt)
+        b = torch.empty(n, dtype=dt, device="cuda").uniform_(-1, 1, generator=g)
+        x = b.clone()
+        f = getattr(so, f"exoblas_{p}trsv_rm_stride_1_async")
+        fn = (lambda f=f, A=A, x=x: f(sh, 122, 111, 131, n, L.win1(x.data_ptr(), 1), L.win2(A.data_ptr(), n)))
+        fn()
+        ms = _timed(torch, stream, fn, 20, before=lambda x=x, b=b: x.copy_(b))
+        nbytes = (n * (n + 1) // 2 + 2 * n) * s_
+        gbs = nbytes / (ms * 1e-3) / 1e9
+        kernels[f"{p}trsv_LNN_4096"] = {"us": round(ms * 1e3, 2), "GB/s": round(gbs, 1),
+                                        "frac": round(gbs / HBM_PEAK_GBS, 4), "bytes": nbytes, "bound": "hbm"}
+        del A, b, x
+        N, K = 4096, 1024
+        As, Cs = _u(torch, g, (N, K), dt), _u(torch, g, (N, N), dt)
+        f = getattr(so, f"exoblas_{p}syrk_rm_stride_1_async")
+        w = L.win2(As.data_ptr(), K)
+        fn = (lambda f=f, w=w, Cs=Cs, p=p: f(sh, 122, 111, N, K, _fl(1.5, p), w, w, w, w, L.win2(Cs.data_ptr(), N)))
+        fn()
+        ms = _timed(torch, stream, fn, 10)
+        flop = float(K) * N * (N + 1)
+        tf = flop / (ms * 1e-3) / 1e12
+        kernels[f"{p}syrk_LN_4096x1024"] = {"us": round(ms * 1e3, 2), "TFLOP/s": round(tf, 2),
+                                            "frac": round(tf / peak, 4), "flop": flop, "bound": "mfma"}
+        del As, Cs
+        M = 4096
+        Sa, Bm, Cm = (_u(torch, g, (M, M), dt) for _ in range(3))
+        f = getattr(so, f"exoblas_{p}symm_rm_stride_1_async")
+        fn = (lambda f=f, Sa=Sa, Bm=Bm, Cm=Cm, p=p: f(sh, 141, 122, M, M, _fl(1.5, p), L.win2(Sa.data_ptr(), M),
+                                                      L.win2(Sa.data_ptr(), M), L.win2(Bm.data_ptr(), M),
+                                                      L.win2(Cm.data_ptr(), M)))
+        fn()
+        ms = _timed(torch, stream, fn, 5)
+        flop = 2.0 * M ** 3
+        tf = flop / (ms * 1e-3) / 1e12
+        kernels[f"{p}symm_LL_4096"] = {"us": round(ms * 1e3, 2), "TFLOP/s": round(tf, 2),
+                                       "frac": round(tf / peak, 4), "flop": flop, "bound": "mfma"}
+        del Sa, Bm, Cm
+    torch.cuda.empty_cache()
+    dom = max(kernels, key=lambda k: kernels[k]["us"])
+    lowest = min(kernels, key=lambda k: kernels[k]["frac"])
+    kd = kernels[dom]
+    unit = "GB/s" if kd["bound"] == "hbm" else "TFLOP/s"
+    peak_d = HBM_PEAK_GBS if kd["bound"] == "hbm" else (FP32_PEAK_TF if dom[0] == "s" else FP64_PEAK_TF)
+    out = {"workload": "s/d trsv L-N-N 4096, s/d syrk L-N 4096x1024, s/d symm L-L 4096^2",
+           "roofline": _roof(dom, kd[unit], peak_d, unit, kd["bound"]), "lowest_frac_kernel": lowest,
+           "kernels": kernels}
+    if not a.no_cpu:
+        cpu = cpu_leg("config3", a)
+        cb = dict(cpu.get(dom, {}))
+        cb["kernels"] = {k: v for k, v in cpu.items() if k != "host"}
+        cb["host"] = cpu.get("host")
+        out["cpu_baseline"] = cb
+    return out
+
+
+def config4(a, torch, so, L):
+    """Config 4: sgemm NN n^3 (n = --n-gemm, 8192) plain FP32 MFMA against the
+    3-, 4- and 6-product split-BF16 forms, on U[-1,1] and on rows scaled by
+    2^U[-8,8], with both error metrics against float64 (max |C - C64| /
+    (|A||B|) and max |C - C64| / |C64|); dgemm NN n^3; ssyrk L-N n x n/4 plain
+    against the splits; strsv L-N-N n plain against the mixed solve (BF16 /
+    FP32 off-diagonal updates, FP64 diagonal blocks, one refinement step)."""
+    stream = torch.cuda.current_stream()
+    sh = stream.cuda_stream
+    g = torch.Generator(device="cuda")
+    g.manual_seed(0)
+    n = a.n_gemm
+    kernels = {}
+    one = _fl(1.0)
+    forms = ("plain", 3, 4, 6)
+
+    def peak_of(form):
+        return FP32_PEAK_TF if form == "plain" else BF16_PEAK_TF / form
+
+    def errs(C, truth, mag, mask=None):
+        err = (C.double() - truth).abs()
+        nw, rel = err / mag, err / truth.abs().clamp_min(1e-300)
+        if mask is not None:
+            nw, rel = nw[mask], rel[mask]
+        return nw.max().item(), rel.max().item()
+    for scaled in (False, True):
+        A = _u(torch, g, (n, n), torch.float32)
+        B = _u(torch, g, (n, n), torch.float32)
+        if scaled:
+            A *= torch.exp2(torch.empty((n, 1), device="cuda").uniform_(-8, 8, generator=g))
+            B *= torch.exp2(torch.empty((n, 1), device="cuda").uniform_(-8, 8, generator=g))
+        truth = A.double() @ B.double()
+        mag = A.double().abs() @ B.double().abs()
+        C = torch.zeros((n, n), device="cuda")
+        wa, wb, wc = L.win2(A.data_ptr(), n), L.win2(B.data_ptr(), n), L.win2(C.data_ptr(), n)
+        for form in forms:
+            if form == "plain":
+                fn = (lambda: so.exoblas_sgemm_stride_1_async(sh, 111, 111, n, n, n, one, wa, wb, wa, wb, wc))
+            else:
+                fn = (lambda form=form: so.exoblas_sgemm_split_async(sh, 111, 111, n, n, n, one, wa, wb, wa, wb,
+                                                                      wc, form))
+            C.zero_()
+            assert fn() == 0, L.last_error()
+            torch.cuda.synchronize()
+            nw, rel = errs(C, truth, mag)
+            key = f"sgemm_{form if form == 'plain' else 'split' + str(form)}"
+            e = kernels.setdefault(key, {"terms": None if form == "plain" else form})
+            e[f"max_err_normwise_{'scaled' if scaled else 'uniform'}"] = nw
+            e[f"max_rel_err_{'scaled' if scaled else 'uniform'}"] = rel
+            if not scaled:
+                ms = _timed(torch, stream, fn, 3)
+                tf = 2.0 * n ** 3 / (ms * 1e-3) / 1e12
+                e.update({"ms": round(ms, 3), "TFLOP/s": round(tf, 2), "frac": round(tf / peak_of(form), 4),
+                          "peak_equiv": round(peak_of(form), 1)})
+        del truth, mag, A, B, C
+        torch.cuda.empty_cache()
+    # dgemm NN n^3
+    A, B, C = (_u(torch, g, (n, n), torch.float64) for _ in range(3))
+    w = [L.win2(t.data_ptr(), n) for t in (A, B, C)]
+    fn = (lambda: so.exoblas_dgemm_stride_1_async(sh, 111, 111, n, n, n, _fl(1.5, "d"), w[0], w[1], w[0], w[1], w[2]))
+    fn()
+    ms = _timed(torch, stream, fn, 2)
+    tf = 2.0 * n ** 3 / (ms * 1e-3) / 1e12
+    kernels["dgemm_plain"] = {"ms": round(ms, 3), "TFLOP/s": round(tf, 2), "frac": round(tf / FP64_PEAK_TF, 4)}
+    del A, B, C
+    torch.cuda.empty_cache()
+    # ssyrk lower-N, N = n, K = n / 4
+    N, K = n, n // 4
+    tri = torch.ones((N, N), dtype=torch.bool, device="cuda").tril_()
+    for scaled in (False, True):
+        A = _u(torch, g, (N, K), torch.float32)
+        if scaled:
+            A *= torch.exp2(torch.empty((N, 1), device="cuda").uniform_(-8, 8, generator=g))
+        A64 = A.double()
+        truth = A64 @ A64.T
+        mag = A64.abs() @ A64.abs().T
+        del A64
+        C = torch.zeros((N, N), device="cuda")
+        wa, wc = L.win2(A.data_ptr(), K), L.win2(C.data_ptr(), N)
+        for form in forms:
+            if form == "plain":
+                fn = (lambda: so.exoblas_ssyrk_rm_stride_1_async(sh, 122, 111, N, K, one, wa, wa, wa, wa, wc))
+            else:
+                fn = (lambda form=form: so.exoblas_ssyrk_split_async(sh, 122, 111, N, K, one, wa, wa, wc, form))
+            C.zero_()
+            assert fn() == 0, L.last_error()
+            torch.cuda.synchronize()
+            nw, rel = errs(C, truth, mag, tri)
+            key = f"ssyrk_{form if form == 'plain' else 'split' + str(form)}"
+            e = kernels.setdefault(key, {"terms": None if form == "plain" else form})
+            e[f"max_err_normwise_{'scaled' if scaled else 'uniform'}"] = nw
+            e[f"max_rel_err_{'scaled' if scaled else 'uniform'}"] = rel
+            if not scaled:
+                ms = _timed(torch, stream, fn, 3)
+                flop = float(K) * N * (N + 1)
+                tf = flop / (ms * 1e-3) / 1e12
+                e.update({"ms": round(ms, 3), "TFLOP/s": round(tf, 2), "frac": round(tf / peak_of(form), 4)})
+        del A, C, truth, mag
+        torch.cuda.empty_cache()
+    del tri
+    # strsv n: plain vs mixed (+ one refinement step)
+    At = _dd(torch, g, n, torch.float32)
+    b = torch.empty(n, device="cuda").uniform_(-1, 1, generator=g)
+    x64 = torch.linalg.solve_triangular(At.double().tril(), b.double()[:, None], upper=False)[:, 0]
+    x = b.clone()
+    wA = L.win2(At.data_ptr(), n)
+    tforms = {"plain": lambda: so.exoblas_strsv_rm_stride_1_async(sh, 122, 111, 131, n, L.win1(x.data_ptr(), 1), wA),
+              "mixed_bf16": lambda: so.exoblas_strsv_mixed_async(sh, 122, 111, 131, n, L.win1(x.data_ptr(), 1), wA,
+                                                                 16, 1),
+              "mixed_fp32": lambda: so.exoblas_strsv_mixed_async(sh, 122, 111, 131, n, L.win1(x.data_ptr(), 1), wA,
+                                                                 32, 1)}
+    nbytes = (n * (n + 1) // 2 + 2 * n) * 4
+    for name, fn in tforms.items():
+        x.copy_(b)
+        assert fn() == 0, L.last_error()
+        torch.cuda.synchronize()
+        err = (x.double() - x64).abs()
+        ms = _timed(torch, stream, fn, 10, before=lambda: x.copy_(b))
+        kernels[f"strsv_{name}"] = {"us": round(ms * 1e3, 2), "GB/s": round(nbytes / (ms * 1e-3) / 1e9, 1),
+                                    "max_err": err.max().item(),
+                                    "max_rel_err": (err / x64.abs().clamp_min(1e-300)).max().item()}
+    del At, b, x, x64
+    torch.cuda.empty_cache()
+    plain = kernels["sgemm_plain"]
+
+    def le_plain(key, base):
+        return {f"{m}_{st}": kernels[key][f"{m}_{st}"] <= kernels[base][f"{m}_{st}"]
+                for m in ("max_err_normwise", "max_rel_err") for st in ("uniform", "scaled")}
+    summary = {f: {"speedup": round(plain["ms"] / kernels[f"sgemm_split{f}"]["ms"], 3),
+                   "error_le_plain": le_plain(f"sgemm_split{f}", "sgemm_plain"),
+                   "ssyrk_speedup": round(kernels["ssyrk_plain"]["ms"] / kernels[f"ssyrk_split{f}"]["ms"], 3),
+                   "ssyrk_error_le_plain": le_plain(f"ssyrk_split{f}", "ssyrk_plain")} for f in (3, 4, 6)}
+    out = {"workload": f"sgemm NN {n}^3 plain vs split-BF16 (3/4/6 products), uniform and row-scaled sets; dgemm NN "
+                       f"{n}^3; ssyrk L-N {N}x{K} plain vs split; strsv L-N-N {n} plain vs mixed + 1 refinement",
+           "roofline": _roof("sgemm_plain", plain["TFLOP/s"], FP32_PEAK_TF, "TFLOP/s", "mfma"),
+           "split_forms": summary, "kernels": kernels}
+    if not a.no_cpu:
+        out["cpu_baseline"] = cpu_leg("config4", a)
+    return out
+
+
+def config5(a, torch, so, L):
+    """Config 5: the seeded mixed request stream (60 % sgemm NN, 25 % strsv,
+    15 % sgemv N), batched persistent kernels against one launch per call,
+    same requests and device-resident operands, 64 GB admission budget;
+    Poisson arrivals at --rate and all at once (capacity)."""
+    from exoblas_amd import serving
+    s = serving.make_stream(a.requests, rate=a.rate, seed=0)
+    w = serving.Workload(s)
+    res = {}
+    for pattern, arrival in (("poisson", None), ("burst", np.zeros(a.requests))):
+        for mode, name in ((1, "per_call"), (0, "batched")):
+            w.reset()
+            torch.cuda.synchronize()
+            st, lat = serving.run(w, mode=mode, budget=64 << 30, streams=4, max_batch=a.max_batch,
+                                  arrival_us=arrival)
+            res[f"{pattern}_{name}"] = {k: (round(v, 3) if isinstance(v, float) else v) for k, v in st.items()}
+    cap = res["burst_batched"]["req_per_s"]
+    out = {"workload": f"{a.requests} requests, 60% sgemm / 25% strsv / 15% sgemv, Poisson {a.rate:.0f}/s and "
+                       f"all-at-once arrivals", "req_per_s": round(cap, 1),
+           "speedup_vs_per_call": round(cap / max(res["burst_per_call"]["req_per_s"], 1e-9), 3),
+           "p50_us": res["poisson_batched"]["p50_us"], "p99_us": res["poisson_batched"]["p99_us"],
+           "p50_us_per_call": res["poisson_per_call"]["p50_us"], "p99_us_per_call": res["poisson_per_call"]["p99_us"],
+           "roofline": {"bound": "mixed", "kernel": "batch_kernel", "achieved": None, "peak": None, "unit": None,
+                        "frac": None, "traffic": None},
+           "modes": res}
+    if not a.no_cpu:
+        out["cpu_baseline"] = cpu_leg("config5", a)
+    del w
+    torch.cuda.empty_cache()
+    return out
+
+
+def serving_cpu_baseline(s, budget):
+    """The same request stream, front to back, through the oracle's loop
+    nests on one host thread (gemm as mscal + gemm): requests per second over
+    as many requests as the budget allows."""
+    from exoblas_amd import native as N
+    from oracle import Oracle
+    o = Oracle(np.float32, kind="fast")
+    rng = np.random.default_rng(1)
+    pool = rng.uniform(-1, 1, 1 << 26).astype(np.float32)  # operand data, drawn before timing
+    done, busy = 0, 0.0
+    for i in range(len(s.kind)):
+        m, n, k = int(s.m[i]), int(s.n[i]), int(s.k[i])
+        if s.kind[i] == N.OP_GEMM:
+            A, B = pool[:m * k].reshape(m, k), pool[:k * n].reshape(k, n)
+            C = pool[-m * n:].reshape(m, n).copy()
+            t0 = time.perf_counter()
+            o.mscal(m, n, 0.5, C, n)
+            o.gemm(111, 111, m, n, k, 1.5, A, k, B, n, C, n)
+        elif s.kind[i] == N.OP_TRSV:
+            T = pool[:m * m].reshape(m, m) / m
+            T[np.arange(m), np.arange(m)] = m + 0.5
+            x = pool[:m].copy()
+            t0 = time.perf_counter()
+            o.trsv(122, 111, 131, m, x, 1, T, m)
+        else:
+            A = pool[:m * n].reshape(m, n)
+            y = pool[-m:].copy()
+            t0 = time.perf_counter()
+            o.gemv(111, m, n, 1.5, 0.5, A, n, pool[:n], 1, y, 1)
+        busy += time.perf_counter() - t0
+        done += 1
+        if busy > budget:
+            break
+    el = busy
+    return {"value": round(done / el, 2), "unit": "req/s", "cores": 1, "kind": "port",
+            "sample": f"the first {done} requests of the same stream in order, oracle loop nests, 1 thread"}
+
+
+def families(a, torch, so, L):
+    """The hot-path kernel families the bandwidth set leaves out, each at a
+    bandwidth-sized shape with two operand sets taken in turn (so no call
+    reads what the previous one touched): s/d asum, dsdot, sdsdot, s/d rotm
+    (flag -1), s/d scal at n = 2^26; s/d ger 8192^2; ssymv U and L, ssyr,
+    ssyr2, strmv L-N-N at n = 8192; s/d mscal and strmscal 8192^2.
+    GB/s over the compulsory bytes (s = element size): asum ns, dsdot and
+    sdsdot 8n, rotm 4ns, scal 2ns, ger 2MNs + (M+N)s, symv n(n+1)/2 s + 3ns,
+    syr n(n+1) s + ns, syr2 n(n+1) s + 2ns, trmv n(n+1)/2 s + 2ns, mscal
+    2MNs, trmscal n(n+1) s."""
+    from oracle import Oracle
+    stream = torch.cuda.current_stream()
+    sh = stream.cuda_stream
+    g = torch.Generator(device="cuda")
+    g.manual_seed(5)
+    n1, n2 = a.n1, a.n2
+    kernels = {}
+    w1 = lambda t: L.win1(t.data_ptr(), 1)
+    w2 = lambda t, ld: L.win2(t.data_ptr(), ld)
+    res = torch.zeros(2, dtype=torch.float64, device="cuda")
+    rp = ctypes.c_void_p(res.data_ptr())
+
+    def run(name, make, nbytes, reps=10):
+        fns = [make(k) for k in range(2)]
+        ms = _timed_alt(torch, stream, fns, reps)
+        gbs = nbytes / (ms * 1e-3) / 1e9
+        kernels[name] = {"us": round(ms * 1e3, 2), "GB/s": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4),
+                         "bytes": nbytes}
+    for p, dt, s_ in (("s", torch.float32, 4), ("d", torch.float64, 8)):
+        X = [_u(torch, g, n1, dt) for _ in range(2)]
+        Y = [_u(torch, g, n1, dt) for _ in range(2)]
+        run(f"{p}asum", lambda k: (lambda: getattr(so, f"exoblas_{p}asum_stride_1_async")(sh, n1, w1(X[k]), rp)),
+            n1 * s_)
+        H = (ctypes.c_float * 4)(0.6, 0.8, -0.8, 0.6) if p == "s" else (ctypes.c_double * 4)(0.6, 0.8, -0.8, 0.6)
+        run(f"{p}rotm_neg_one", lambda k: (lambda: getattr(so, f"exoblas_{p}rotm_flag_neg_one_stride_1_async")(
+            sh, n1, w1(X[k]), w1(Y[k]), H)), 4 * n1 * s_)
+        run(f"{p}scal", lambda k: (lambda: getattr(so, f"exoblas_{p}scal_stride_1_async")(
+            sh, n1, _fl(0.999, p), w1(X[k]))), 2 * n1 * s_)
+        if p == "s":
+            run("dsdot", lambda k: (lambda: so.exoblas_dsdot_stride_1_async(sh, n1, w1(X[k]), w1(Y[k]), rp)), 8 * n1)
+            run("sdsdot", lambda k: (lambda: so.exoblas_sdsdot_stride_1_async(sh, n1, _fl(0.25), w1(X[k]), w1(Y[k]),
+                                                                            rp)), 8 * n1)
+        del X, Y
+        Am = [_u(torch, g, (n2, n2), dt) for _ in range(2)]
+        xv = [_u(torch, g, n2, dt) for _ in range(2)]
+        yv = [_u(torch, g, n2, dt) for _ in range(2)]
+        run(f"{p}ger", lambda k: (lambda: getattr(so, f"exoblas_{p}ger_rm_stride_1_async")(
+            sh, n2, n2, _fl(1e-3, p), w1(xv[k]), w1(yv[k]), w2(Am[k], n2))), 2 * n2 * n2 * s_ + 2 * n2 * s_)
+        run(f"{p}mscal", lambda k: (lambda: getattr(so, f"exoblas_{p}mscal_rm_stride_1_async")(
+            sh, n2, n2, _fl(0.999, p), w2(Am[k], n2))), 2 * n2 * n2 * s_)
+        if p == "s":
+            tri = n2 * (n2 + 1) // 2
+            for u, uname in ((121, "U"), (122, "L")):
+                run(f"ssymv_{uname}", lambda k, u=u: (lambda: so.exoblas_ssymv_rm_stride_1_async(
+                    sh, u, n2, _fl(0.5), w2(Am[k], n2), w1(xv[k]), _fl(1.5), w1(yv[k]))), tri * 4 + 3 * n2 * 4)
+            run("ssyr_L", lambda k: (lambda: so.exoblas_ssyr_rm_stride_1_async(
+                sh, 122, n2, _fl(1e-3), w1(xv[k]), w1(xv[k]), w2(Am[k], n2))), 2 * tri * 4 + n2 * 4)
+            run("ssyr2_L", lambda k: (lambda: so.exoblas_ssyr2_rm_stride_1_async(
+                sh, 122, n2, _fl(1e-3), w1(xv[k]), w1(xv[k]), w1(yv[k]), w1(yv[k]), w2(Am[k], n2))),
+                2 * tri * 4 + 2 * n2 * 4)
+            run("strmv_LNN", lambda k: (lambda: so.exoblas_strmv_rm_stride_1_async(
+                sh, 122, 111, 131, n2, w1(xv[k]), w2(Am[k], n2))), tri * 4 + 2 * n2 * 4)
+            run("strmscal_L", lambda k: (lambda: so.exoblas_strmscal_rm_stride_1_async(
+                sh, 122, n2, _fl(0.999), w2(Am[k], n2))), 2 * tri * 4)
+        del Am, xv, yv
+        torch.cuda.empty_cache()
+    lowest = min(kernels, key=lambda k: kernels[k]["frac"])
+    dom = max(kernels, key=lambda k: kernels[k]["us"])
+    out = {"workload": "s/d asum, dsdot, sdsdot, s/d rotm, s/d scal n=2^26; s/d ger, s/d mscal 8192^2; ssymv U/L, "
+                       "ssyr, ssyr2, strmv, strmscal n=8192",
+           "roofline": _roof(dom, kernels[dom]["GB/s"], HBM_PEAK_GBS, "GB/s", "hbm"), "lowest_frac_kernel": lowest,
+           "kernels": kernels}
+    if not a.no_cpu:
+        cpu = cpu_leg("families", a)
+        cb = dict(cpu.get(dom, cpu.get("sger", {})))
+        cb["kernels"] = cpu
+        out["cpu_baseline"] = cb
+    return out
+
+
+
+# ---------------------------------------------------------------- CPU legs
+# Every CPU baseline runs in a child process that never imports torch: with
+# torch's OpenMP runtime in the process the vendor CBLAS's own threads
+# contend with it (MKL sgemm 2048^3 measured 963 GFLOP/s on 8 threads alone,
+# 108 GFLOP/s with torch imported, below its single-thread 133).
+
+def cpu_leg(name, a):
+    import subprocess
+    cmd = [sys.executable, os.path.abspath(__file__), "--cpu-leg", name, "--cpu-budget", str(a.cpu_budget),
+           "--n1", str(a.n1), "--n2", str(a.n2), "--n-gemm", str(a.n_gemm), "--requests", str(a.requests),
+           "--rate", str(a.rate)]
+    r = subprocess.run(cmd, capture_output=True, text=True, timeout=900)
+    if r.returncode != 0:
+        return {"error": f"cpu leg {name} failed: {r.stderr[-400:]}"}
+    return json.loads(r.stdout.strip().splitlines()[-1])
+
+
+def cpu_bandwidth(a):
+    hc = host_cpus()
+    cb = cpu_baseline(a.n1, a.n2, a.cpu_budget)
+    cb["host"] = hc
+    set_port_threads(hc["threads"])
+    cb["all_cores"] = cpu_baseline(a.n1, a.n2, a.cpu_budget / 3, kind="omp", threads=hc["threads"])
+    vend = [vendor_cblas(a.n1, a.n2, a.cpu_budget / 4, t) for t in sorted({1, hc["threads"]})]
+    if vend[0] is not None:
+        cb["vendor_cblas"] = vend
+    return cb
+
+
+def cpu_config1(a):
+    from oracle import Oracle
+    n = 512
+    flop = 2.0 * n ** 3
+    rng = np.random.default_rng(0)
+    A, B, C = (rng.uniform(-1, 1, (n, n)).astype(np.float32) for _ in range(3))
+    hc = host_cpus()
+    T = hc["threads"]
+    o = Oracle(np.float32, kind="fast")
+    Cc = C.copy()
+    seq = (lambda o=o: (o.mscal(n, n, 0.5, Cc, n), o.gemm(111, 111, n, n, n, 1.5, A, n, B, n, Cc, n)))
+    cb = _port(seq, flop, 1e9, "GFLOP/s", a.cpu_budget / 5, "the full workload: the oracle's mscal + gemm "
+               "loop nests, avx512 -O3 -ffast-math, 1 thread")
+    set_port_threads(T)
+    oo = Oracle(np.float32, kind="omp")
+    seq2 = (lambda o=oo: (o.mscal(n, n, 0.5, Cc, n), o.gemm(111, 111, n, n, n, 1.5, A, n, B, n, Cc, n)))
+    cb["all_cores"] = _port(seq2, flop, 1e9, "GFLOP/s", a.cpu_budget / 10, f"the same, OpenMP over rows",
+                            threads=T)
+
+    def mk(mkl):
+        mkl.cblas_sgemm(101, 111, 111, n, n, n, ctypes.c_float(1.5), _ptr(A), n, _ptr(B), n,
+                        ctypes.c_float(0.5), _ptr(Cc), n)
+    cb["vendor_cblas"] = [v for v in (_vendor(a.cpu_budget / 10, t, mk, flop, 1e9, "GFLOP/s",
+                                              "cblas_sgemm 512^3 with beta") for t in sorted({1, T})) if v]
+    cb["host"] = hc
+    return cb
+
+
+def cpu_config3(a):
+    from oracle import Oracle
+    hc = host_cpus()
+    T = hc["threads"]
+    rng = np.random.default_rng(0)
+    cpu = {}
+    for p, dt in (("s", np.float32), ("d", np.float64)):
+        o = Oracle(dt, kind="fast")
+        n = 4096
+        At = (rng.uniform(-1, 1, (n, n)) / n).astype(dt)
+        At[np.arange(n), np.arange(n)] = (n + rng.uniform(0, 1, n)).astype(dt)
+        bt = rng.uniform(-1, 1, n).astype(dt)
+        nb = (n * (n + 1) // 2 + 2 * n) * np.dtype(dt).itemsize
+        cpu[f"{p}trsv_LNN_4096"] = _port(lambda o=o, At=At, bt=bt: o.trsv(122, 111, 131, n, bt.copy(), 1, At, n),
+                                         nb, 1e9, "GB/s", a.cpu_budget / 20, "the full solve, 1 thread")
+        N, K = 1024, 256
+        As = rng.uniform(-1, 1, (N, K)).astype(dt)
+        Cs = rng.uniform(-1, 1, (N, N)).astype(dt)
+        cpu[f"{p}syrk_LN_4096x1024"] = _port(lambda o=o, As=As, Cs=Cs: o.syrk(122, 111, N, K, 1.5, As, K, Cs, N),
+                                             float(K) * N * (N + 1), 1e9, "GFLOP/s", a.cpu_budget / 20,
+                                             "rate on syrk L-N 1024x256, 1 thread")
+        M = 512
+        Sa, Bm, Cm = (rng.uniform(-1, 1, (M, M)).astype(dt) for _ in range(3))
+        cpu[f"{p}symm_LL_4096"] = _port(lambda o=o, Sa=Sa, Bm=Bm, Cm=Cm: o.symm(141, 122, M, M, 1.5, Sa, M, Bm, M,
+                                                                             Cm, M),
+                                        2.0 * M ** 3, 1e9, "GFLOP/s", a.cpu_budget / 20,
+                                        "rate on symm L-L 512^2, 1 thread")
+        # the vendor CBLAS at the full sizes
+        N, K, M = 4096, 1024, 4096
+        Af = rng.uniform(-1, 1, (N, K)).astype(dt)
+        Cf = rng.uniform(-1, 1, (N, N)).astype(dt)
+        Sf, Bf = (rng.uniform(-1, 1, (M, M)).astype(dt) for _ in range(2))
+        Tc = ctypes.c_float if p == "s" else ctypes.c_double
+        vend = {}
+        for t in sorted({1, T}):
+            vs = _vendor(a.cpu_budget / 30, t, lambda m, Af=Af, Cf=Cf: getattr(m, f"cblas_{p}syrk")(
+                101, 122, 111, N, K, Tc(1.5), _ptr(Af), K, Tc(1.0), _ptr(Cf), N), float(K) * N * (N + 1), 1e9,
+                "GFLOP/s", f"cblas_{p}syrk L-N 4096x1024")
+            vm = _vendor(a.cpu_budget / 30, t, lambda m, Sf=Sf, Bf=Bf, Cf=Cf: getattr(m, f"cblas_{p}symm")(
+                101, 141, 122, M, M, Tc(1.5), _ptr(Sf), M, _ptr(Bf), M, Tc(1.0), _ptr(Cf), M), 2.0 * M ** 3, 1e9,
+                "GFLOP/s", f"cblas_{p}symm L-L 4096^2")
+            vt = _vendor(a.cpu_budget / 60, t, lambda m, At=At, bt=bt: getattr(m, f"cblas_{p}trsv")(
+                101, 122, 111, 131, n, _ptr(At), n, _ptr(bt.copy()), 1), (n * (n + 1) // 2 + 2 * n) *
+                np.dtype(dt).itemsize, 1e9, "GB/s", f"cblas_{p}trsv L-N-N 4096")
+            vend[t] = {"syrk": vs, "symm": vm, "trsv": vt}
+        cpu[f"{p}_vendor_cblas"] = vend
+    cpu["host"] = hc
+    return cpu
+
+
+def cpu_config4(a):
+    hc = host_cpus()
+    T = hc["threads"]
+    rng = np.random.default_rng(0)
+    m = 2048
+    Am, Bm, Cm = (rng.uniform(-1, 1, (m, m)).astype(np.float32) for _ in range(3))
+    vend = [v for v in (_vendor(a.cpu_budget / 10, t, lambda mk: mk.cblas_sgemm(
+        101, 111, 111, m, m, m, ctypes.c_float(1.0), _ptr(Am), m, _ptr(Bm), m, ctypes.c_float(0.0), _ptr(Cm), m),
+        2.0 * m ** 3, 1e9, "GFLOP/s", "rate on cblas_sgemm NN 2048^3") for t in sorted({1, T})) if v]
+    from oracle import Oracle
+    o = Oracle(np.float32, kind="fast")
+    p5 = 512
+    A5, B5, C5 = (rng.uniform(-1, 1, (p5, p5)).astype(np.float32) for _ in range(3))
+    cb = _port(lambda: o.gemm(111, 111, p5, p5, p5, 1.0, A5, p5, B5, p5, C5, p5), 2.0 * p5 ** 3, 1e9, "GFLOP/s",
+               a.cpu_budget / 10, "rate of the oracle's gemm loop nest on 512^3, 1 thread")
+    cb["vendor_cblas"] = vend
+    cb["host"] = hc
+    return cb
+
+
+def cpu_config5(a):
+    from exoblas_amd import serving
+    return serving_cpu_baseline(serving.make_stream(a.requests, rate=a.rate, seed=0), a.cpu_budget / 5)
+
+
+def cpu_families(a):
+    from oracle import Oracle
+    rng = np.random.default_rng(0)
+    o = Oracle(np.float32, kind="fast")
+    m1, m2 = a.n1, a.n2
+    x, y = rng.uniform(-1, 1, m1).astype(np.float32), rng.uniform(-1, 1, m1).astype(np.float32)
+    A = rng.uniform(-1, 1, (m2, m2)).astype(np.float32)
+    xv, yv = rng.uniform(-1, 1, m2).astype(np.float32), rng.uniform(-1, 1, m2).astype(np.float32)
+    tri = m2 * (m2 + 1) // 2
+    b = a.cpu_budget / 40
+    cpu = {
+        "sasum": _port(lambda: o.asum(m1, x, 1), m1 * 4, 1e9, "GB/s", b, "sasum n=2^26, 1 thread"),
+        "dsdot": _port(lambda: o.dsdot(m1, x, 1, y, 1), 8 * m1, 1e9, "GB/s", b, "dsdot n=2^26, 1 thread"),
+        "srotm_neg_one": _port(lambda: o.rotm(-1, m1, x, 1, y, 1, np.array([0.6, 0.8, -0.8, 0.6], np.float32)),
+                               16 * m1, 1e9, "GB/s", b, "srotm n=2^26, 1 thread"),
+        "sscal": _port(lambda: o.scal(m1, 0.999, x, 1), 8 * m1, 1e9, "GB/s", b, "sscal n=2^26, 1 thread"),
+        "sger": _port(lambda: o.ger(m2, m2, 1e-3, xv, 1, yv, 1, A, m2), 8 * m2 * m2, 1e9, "GB/s", b,
+                      "sger 8192^2, 1 thread"),
+        "ssymv_L": _port(lambda: o.symv(122, m2, 0.5, A, m2, xv, 1, 1.5, yv.copy(), 1), 4 * tri, 1e9, "GB/s", b,
+                         "ssymv L 8192, 1 thread"),
+        "ssyr_L": _port(lambda: o.syr(122, m2, 1e-3, xv, 1, A, m2), 8 * tri, 1e9, "GB/s", b,
+                        "ssyr L 8192, 1 thread"),
+        "ssyr2_L": _port(lambda: o.syr2(122, m2, 1e-3, xv, 1, yv, 1, A, m2), 8 * tri, 1e9, "GB/s", b,
+                         "ssyr2 L 8192, 1 thread"),
+        "strmv_LNN": _port(lambda: o.trmv(122, 111, 131, m2, xv.copy(), 1, A, m2), 4 * tri, 1e9, "GB/s", b,
+                           "strmv L-N-N 8192, 1 thread"),
+        "smscal": _port(lambda: o.mscal(m2, m2, 0.999, A, m2), 8 * m2 * m2, 1e9, "GB/s", b,
+                        "smscal 8192^2, 1 thread"),
+    }
+    return cpu
+
+
+CPU_LEGS = {"bandwidth": cpu_bandwidth, "config1": cpu_config1, "config3": cpu_config3, "config4": cpu_config4,
+            "config5": cpu_config5, "families": cpu_families}
+
+CONFIGS = {"config1": config1, "config3": config3, "config4": config4, "config5": config5, "families": families}
+
+
+def run_config(name):
+    def w(a, torch, so, L):
+        r = CONFIGS[name](a, torch, so, L)
+        roof = r.get("roofline")
+        return _line(f"{name} of the default run", None, None, a, 0.0, "see kernels", r["workload"], roof,
+                     r.get("kernels", {}), section=r)
+    return w
+
+
+def bench_default(a, torch, so, L):
+    """The bandwidth-set headline line, with every other configuration and
+    the remaining kernel families under "configs" (skipped with
+    --no-configs)."""
+    out = bench_bandwidth(a, torch, so, L)
+    if not a.no_configs:
+        out["configs"] = {}
+        for name, fn in CONFIGS.items():
+            t0 = time.perf_counter()
+            print(f"bench: {name} ...", file=sys.stderr, flush=True)
+            r = fn(a, torch, so, L)
+            r["wall_s"] = round(time.perf_counter() - t0, 1)
+            out["configs"][name] = r
+    return out
+
+
+WORKLOADS = {"default": bench_default, "bandwidth": bench_bandwidth, "level3": bench_level3, "trsv": bench_trsv,
+             "extended": bench_extended, "precision": bench_precision, "serving": bench_serving,
+             "hostpath": bench_hostpath}
+WORKLOADS.update({name: run_config(name) for name in CONFIGS})
 
 
 def main(argv=None):
@@ -773,7 +1485,9 @@ def main(argv=None):
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--workload", choices=sorted(WORKLOADS), default="bandwidth")
+    ap.add_argument("--workload", choices=sorted(WORKLOADS), default="default")
+    ap.add_argument("--no-configs", action="store_true",
+                    help="default workload: the bandwidth set only, without the other configurations")
     ap.add_argument("--n1", type=int, default=1 << 26)
     ap.add_argument("--n2", type=int, default=8192)
     ap.add_argument("--n-gemm", type=int, default=8192)
@@ -781,10 +1495,14 @@ def main(argv=None):
     ap.add_argument("--rate", type=float, default=20000.0)
     ap.add_argument("--max-batch", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
-    ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--gbench-dir", default=None,
                     help="also write the measured runs as Google Benchmark JSON (<dir>/exoblas_amd/<kernel>.json)")
+    ap.add_argument("--cpu-leg", default=None, help=argparse.SUPPRESS)
     a = ap.parse_args(argv)
+    if a.cpu_leg:  # a child process: CPU baselines only, no torch, no GPU
+        print(json.dumps(CPU_LEGS[a.cpu_leg](a)), flush=True)
+        return 0
     if a.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {a.gpus} is not supported "
               "(the multi-GPU sharding is a design note, not a measured path)", file=sys.stderr)

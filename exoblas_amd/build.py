@@ -54,7 +54,7 @@ def _compile(src: str) -> str:
 
 def build_oracle() -> None:
     odir = os.path.join(ROOT, "oracle")
-    r = subprocess.run(["make", "-s", "-C", odir, "-j3", "-B"], capture_output=True, text=True)
+    r = subprocess.run(["make", "-s", "-C", odir, "-j5", "-B"], capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"oracle build failed\n{r.stdout}\n{r.stderr}")
 

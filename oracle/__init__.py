@@ -11,7 +11,9 @@ Three builds of the same source (oracle/Makefile):
                         truth oracle errors are stated against);
   liboracle_avx512.so   the same loop nests built with the reference's avx512
                         preset flags, timed as the CPU baseline;
-  liboracle_avx2.so     the same for hosts without AVX-512.
+  liboracle_avx2.so     the same for hosts without AVX-512;
+  liboracle_omp_*.so    the baseline loop nests on every host thread (OpenMP
+                        over the independent outer loops).
 
 Arrays are numpy arrays; vectors are passed as (array, offset-of-element-0,
 stride) so negative strides address downwards from element 0, as the
@@ -56,13 +58,17 @@ _libs = {}
 
 
 def load(kind: str = "strict") -> ctypes.CDLL:
-    """kind: "strict" (checker) or "fast" (CPU baseline: avx512 or avx2 build)."""
+    """kind: "strict" (checker), "fast" (CPU baseline, one thread: avx512 or
+    avx2 build) or "omp" (the same baseline on every host thread)."""
     if kind in _libs:
         return _libs[kind]
+    isa = "avx512" if _host_has_avx512() else "avx2"
     if kind == "strict":
         name = "liboracle.so"
+    elif kind == "omp":
+        name = f"liboracle_omp_{isa}.so"
     else:
-        name = "liboracle_avx512.so" if _host_has_avx512() else "liboracle_avx2.so"
+        name = f"liboracle_{isa}.so"
     path = os.path.join(HERE, name)
     if not os.path.exists(path):
         build()

@@ -24,6 +24,21 @@
 #include <type_traits>
 #include <vector>
 
+// The all-cores CPU baseline build (-fopenmp -DEXO_ORACLE_OMP) runs the outer
+// loops whose iterations are independent (elements, rows of the output) on
+// every host thread, and the level-1 reductions as OpenMP + reductions; loops
+// that carry a dependence or scatter (symv, trmv, trsv) stay sequential.
+// The checker build defines neither, so its loops are the spec's.
+#if defined(EXO_ORACLE_OMP)
+#include <omp.h>
+#define EXO_PRAGMA(x) _Pragma(#x)
+#define EXO_PAR_FOR EXO_PRAGMA(omp parallel for schedule(static))
+#define EXO_PAR_SUM(v) EXO_PRAGMA(omp parallel for schedule(static) reduction(+ : v))
+#else
+#define EXO_PAR_FOR
+#define EXO_PAR_SUM(v)
+#endif
+
 namespace {
 
 constexpr int64_t kNoTrans = 111;
@@ -102,6 +117,7 @@ template <class T, class W>
 W asum(int64_t n, const T* x_, int64_t incx) {
   V1<T, W> x(x_, n, incx, false);
   W result = 0;
+  EXO_PAR_SUM(result)
   for (int64_t i = 0; i < n; ++i) result += (x[i] > 0 ? x[i] : -x[i]);
   return result;
 }
@@ -111,6 +127,7 @@ template <class T, class W>
 void axpy(int64_t n, T alpha_, const T* x_, int64_t incx, T* y_, int64_t incy) {
   V1<T, W> x(x_, n, incx, false), y(y_, n, incy, true);
   W alpha = alpha_;
+  EXO_PAR_FOR
   for (int64_t i = 0; i < n; ++i) y[i] += alpha * x[i];
 }
 
@@ -118,6 +135,7 @@ void axpy(int64_t n, T alpha_, const T* x_, int64_t incx, T* y_, int64_t incy) {
 template <class T, class W>
 void axpy_alpha_1(int64_t n, const T* x_, int64_t incx, T* y_, int64_t incy) {
   V1<T, W> x(x_, n, incx, false), y(y_, n, incy, true);
+  EXO_PAR_FOR
   for (int64_t i = 0; i < n; ++i) y[i] += x[i];
 }
 
@@ -126,6 +144,7 @@ template <class T, class W>
 W dot(int64_t n, const T* x_, int64_t incx, const T* y_, int64_t incy) {
   V1<T, W> x(x_, n, incx, false), y(y_, n, incy, false);
   W result = 0;
+  EXO_PAR_SUM(result)
   for (int64_t i = 0; i < n; ++i) result += x[i] * y[i];
   return result;
 }
@@ -134,6 +153,7 @@ W dot(int64_t n, const T* x_, int64_t incx, const T* y_, int64_t incy) {
 template <class W>
 W dsdot(int64_t n, const float* x, int64_t incx, const float* y, int64_t incy) {
   W d_dot = 0;
+  EXO_PAR_SUM(d_dot)
   for (int64_t i = 0; i < n; ++i) {
     W d_x = (W)x[i * incx];
     W d_y = (W)y[i * incy];
@@ -153,12 +173,14 @@ float sdsdot(int64_t n, float sb, const float* x, int64_t incx, const float* y, 
 // src/level1/exo_copy.py:9-12
 template <class T>
 void copy(int64_t n, const T* x, int64_t incx, T* y, int64_t incy) {
+  EXO_PAR_FOR
   for (int64_t i = 0; i < n; ++i) y[i * incy] = x[i * incx];
 }
 
 // src/level1/swap.py:9-15
 template <class T>
 void swap(int64_t n, T* x, int64_t incx, T* y, int64_t incy) {
+  EXO_PAR_FOR
   for (int64_t i = 0; i < n; ++i) {
     T tmp = x[i * incx];
     x[i * incx] = y[i * incy];
@@ -171,6 +193,7 @@ template <class T, class W>
 void rot(int64_t n, T* x_, int64_t incx, T* y_, int64_t incy, T c_, T s_) {
   V1<T, W> x(x_, n, incx, true), y(y_, n, incy, true);
   W c = c_, s = s_;
+  EXO_PAR_FOR
   for (int64_t i = 0; i < n; ++i) {
     W xReg = x[i];
     x[i] = c * xReg + s * y[i];
@@ -185,6 +208,7 @@ template <class T, class W>
 void rotm(int flag, int64_t n, T* x_, int64_t incx, T* y_, int64_t incy, const T* H_) {
   V1<T, W> x(x_, n, incx, true), y(y_, n, incy, true);
   W h00 = H_[0], h01 = H_[1], h10 = H_[2], h11 = H_[3];
+  EXO_PAR_FOR
   for (int64_t i = 0; i < n; ++i) {
     W xReg = x[i];
     if (flag == -1) {
@@ -205,11 +229,13 @@ template <class T, class W>
 void scal(int64_t n, T alpha_, T* x_, int64_t incx) {
   V1<T, W> x(x_, n, incx, true);
   W alpha = alpha_;
+  EXO_PAR_FOR
   for (int64_t i = 0; i < n; ++i) x[i] = alpha * x[i];
 }
 
 template <class T>
 void scal_alpha_0(int64_t n, T* x, int64_t incx) {
+  EXO_PAR_FOR
   for (int64_t i = 0; i < n; ++i) x[i * incx] = 0;
 }
 
@@ -222,15 +248,28 @@ void gemv(int64_t trans, int64_t m, int64_t n, T alpha_, T beta_, const T* A_, i
           const T* x_, int64_t incx, T* y_, int64_t incy) {
   V1<T, W> x(x_, n, incx, false), y(y_, m, incy, true);
   W alpha = alpha_, beta = beta_;
+  EXO_PAR_FOR
   for (int64_t i = 0; i < m; ++i) y[i] = y[i] * beta;
   if (trans == kNoTrans) {
     M2<T, W> A(A_, m, n, lda, false);
+    EXO_PAR_FOR
     for (int64_t i = 0; i < m; ++i)
       for (int64_t j = 0; j < n; ++j) y[i] += alpha * (x[j] * A(i, j));
   } else {
     M2<T, W> AT(A_, n, m, lda, false);
+#if defined(EXO_ORACLE_OMP)
+    // each thread owns a contiguous range of y and walks the rows of AT over it
+#pragma omp parallel
+    {
+      const int64_t nt = omp_get_num_threads(), t = omp_get_thread_num();
+      const int64_t j0 = m * t / nt, j1 = m * (t + 1) / nt;
+      for (int64_t i = 0; i < n; ++i)
+        for (int64_t j = j0; j < j1; ++j) y[j] += alpha * x[i] * AT(i, j);
+    }
+#else
     for (int64_t i = 0; i < n; ++i)
       for (int64_t j = 0; j < m; ++j) y[j] += alpha * x[i] * AT(i, j);
+#endif
   }
 }
 
@@ -241,6 +280,7 @@ void ger(int64_t m, int64_t n, T alpha_, const T* x_, int64_t incx, const T* y_,
   V1<T, W> x(x_, m, incx, false), y(y_, n, incy, false);
   M2<T, W> A(A_, m, n, lda, true);
   W alpha = alpha_;
+  EXO_PAR_FOR
   for (int64_t i = 0; i < m; ++i)
     for (int64_t j = 0; j < n; ++j) A(i, j) += alpha * x[i] * y[j];
 }
@@ -280,6 +320,7 @@ void syr(int64_t uplo, int64_t n, T alpha_, const T* x_, int64_t incx, T* A_, in
   V1<T, W> x(x_, n, incx, false);
   M2<T, W> A(A_, n, n, lda, true);
   W alpha = alpha_;
+  EXO_PAR_FOR
   for (int64_t i = 0; i < n; ++i)
     for (int64_t j = 0; j < i + 1; ++j) {
       if (uplo == kUpper)
@@ -296,6 +337,7 @@ void syr2(int64_t uplo, int64_t n, T alpha_, const T* x_, int64_t incx, const T*
   V1<T, W> x(x_, n, incx, false), y(y_, n, incy, false);
   M2<T, W> A(A_, n, n, lda, true);
   W alpha = alpha_;
+  EXO_PAR_FOR
   for (int64_t i = 0; i < n; ++i)
     for (int64_t j = 0; j < i + 1; ++j) {
       if (uplo == kUpper)
@@ -380,6 +422,7 @@ void gemm(int64_t ta, int64_t tb, int64_t M, int64_t N, int64_t K, T alpha_, con
   M2<T, W> B(B_, tb == kNoTrans ? K : N, tb == kNoTrans ? N : K, ldb, false);
   M2<T, W> C(C_, M, N, ldc, true);
   W alpha = alpha_;
+  EXO_PAR_FOR
   for (int64_t i = 0; i < M; ++i)
     for (int64_t j = 0; j < N; ++j)
       for (int64_t k = 0; k < K; ++k) {
@@ -394,6 +437,7 @@ template <class T, class W>
 void mscal(int64_t M, int64_t N, T alpha_, T* A_, int64_t lda) {
   M2<T, W> A(A_, M, N, lda, true);
   W alpha = alpha_;
+  EXO_PAR_FOR
   for (int64_t i = 0; i < M; ++i)
     for (int64_t j = 0; j < N; ++j) A(i, j) = A(i, j) * alpha;
 }
@@ -403,6 +447,7 @@ template <class T, class W>
 void trmscal(int64_t uplo, int64_t N, T alpha_, T* A_, int64_t lda) {
   M2<T, W> A(A_, N, N, lda, true);
   W alpha = alpha_;
+  EXO_PAR_FOR
   for (int64_t i = 0; i < N; ++i)
     for (int64_t j = 0; j < N; ++j)
       if ((uplo == kUpper && j >= i) || (uplo != kUpper && j < i + 1)) A(i, j) = A(i, j) * alpha;
@@ -418,6 +463,7 @@ void symm(int64_t side, int64_t uplo, int64_t M, int64_t N, T alpha_, const T* A
   M2<T, W> B(B_, M, N, ldb, false);
   M2<T, W> C(C_, M, N, ldc, true);
   W alpha = alpha_;
+  EXO_PAR_FOR
   for (int64_t i = 0; i < M; ++i)
     for (int64_t j = 0; j < N; ++j) {
       if (side == kLeft) {
@@ -449,6 +495,7 @@ void syrk(int64_t uplo, int64_t trans, int64_t N, int64_t K, T alpha_, const T* 
   M2<T, W> A(A_, trans == kNoTrans ? N : K, trans == kNoTrans ? K : N, lda, false);
   M2<T, W> C(C_, N, N, ldc, true);
   W alpha = alpha_;
+  EXO_PAR_FOR
   for (int64_t i = 0; i < N; ++i)
     for (int64_t j = 0; j < N; ++j)
       if ((uplo == kUpper && j >= i) || (uplo != kUpper && j < i + 1))
