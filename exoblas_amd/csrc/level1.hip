@@ -7,6 +7,8 @@
 // launch: per-workgroup partial sums (wave shuffle + LDS) published as tagged
 // words (tagged.hpp), summed in a fixed order by the workgroup that finishes
 // last, so results are deterministic.
+#include <type_traits>
+
 #include "device.hpp"
 #include "ops.hpp"
 #include "tagged.hpp"
@@ -73,6 +75,9 @@ __global__ __launch_bounds__(kNT) void ew_strided_kernel(int64_t n, T* x, int64_
   }
 }
 
+template <class T> struct AxpyOp;
+template <class T> struct Axpy1Op;
+
 template <class T, class Op>
 void launch_ew(Ctx& c, int64_t n, T* x, int64_t incx, T* y, int64_t incy, Op op) {
   if (n <= 0) return;
@@ -89,11 +94,13 @@ void launch_ew(Ctx& c, int64_t n, T* x, int64_t incx, T* y, int64_t incy, Op op)
                     (!(Op::ry || Op::wy) || (incy == 1 && aligned16(y)));
   if (unit) {
     // One round per workgroup of 256 lanes with four 16-byte vectors per lane
-    // in flight: the best of a sweep over 64-1024 lanes, 1-8 vectors per lane
-    // and persistent grids on MI355X for every update in the set (axpy 80.6 %,
-    // copy 77.7 %, swap and rot 77 % of HBM peak, against 67-69 % for swap
-    // and rot with one vector per lane).
-    constexpr int NT = kNT, U = 4;
+    // in flight, one for FP32 axpy: measured on MI355X inside the bandwidth
+    // set, with the same builds on one box (2^26 elements, operands
+    // alternating between two sets): saxpy 85.5 % of HBM peak with one
+    // vector per lane against 80.5 % with four, while scopy (71.5 % against
+    // 77.2 %), sswap (71.5 / 76.9 %) and daxpy (76.3 / 79.9 %) lose with one.
+    constexpr bool kAxpy = std::is_same<Op, AxpyOp<T>>::value || std::is_same<Op, Axpy1Op<T>>::value;
+    constexpr int NT = kNT, U = (kAxpy && sizeof(T) == 4) ? 1 : 4;
     const int64_t nvec = n / VN;
     const int64_t blocks = std::max<int64_t>(1, ceil_div(nvec, (int64_t)NT * U));
     hipLaunchKernelGGL((ew_vec_kernel<T, Op, NT, U>), dim3((unsigned)blocks), dim3(NT), 0, c.stream, nvec, n, x, y,

@@ -8,15 +8,18 @@
 // gemv-T  a workgroup owns 64*VN consecutive columns and a chunk of rows;
 //         lanes hold column sums, waves split the rows and reduce through LDS;
 //         row splits are summed by the last workgroup of each column block.
-// ger, syr, syr2   row-wise rank-1/2 updates (read+write each element once),
-//         restricted to one triangle for syr/syr2.
-// symv    64x64 tiles of the stored triangle, each read once: a tile adds
-//         A_IJ x_J to y_I and A_IJ^T x_I to y_J; per-(tile-row, tile-column)
-//         partials are summed in a fixed order, so results are deterministic.
+// ger, syr, syr2   streaming rank-1/2 updates over the tiles that meet the
+//         stored part (tiles.hpp), several rows of loads in flight per lane.
+// symv    64-row tiles of the stored triangle, each element read once for
+//         both its row sum (A_ij x_j -> y_i) and its column sum (A_ij x_i ->
+//         y_j); per-tile partials are summed in a fixed order, so results are
+//         deterministic.
 // trmv    the triangular gemv-N / gemv-T kernels over a copy of x.
 #include "device.hpp"
 #include "ops.hpp"
 #include "tagged.hpp"
+#include "symv.hpp"
+#include "tiles.hpp"
 
 namespace exoblas {
 namespace {
@@ -158,8 +161,11 @@ __global__ __launch_bounds__(kNT) void gemv_n_row_kernel(int64_t M, int64_t N, T
   constexpr int VN = Vec16<T>::n;
   constexpr int UV = 8;
   const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
-  const int64_t row = (int64_t)blockIdx.x * (kNT / kWave) + wave;
-  if (row >= M) return;
+  // a lower triangle's longest rows are its last: take them first, so the
+  // short rows fill in behind them instead of trailing at the end
+  const int64_t rank = (int64_t)blockIdx.x * (kNT / kWave) + wave;
+  const int64_t row = TRI == kLowerTri ? M - 1 - rank : rank;
+  if (rank >= M) return;
   int64_t jl = 0, jh = N;  // columns [jl, jh) of this row
   if (TRI == kLowerTri) jh = UNIT ? row : row + 1;
   if (TRI == kUpperTri) jl = UNIT ? row + 1 : row;
@@ -199,13 +205,15 @@ __global__ __launch_bounds__(kNT) void gemv_n_row_kernel(int64_t M, int64_t N, T
   }
 }
 
-// Full gemv-N with a contiguous, aligned x of at most kNT * VN * UV
-// elements: the workgroup keeps its slice of x in registers (thread t holds
-// vectors t, t + kNT, ...) for all its rows, and reads R whole rows at a time
-// with every thread (R * UV streaming loads in flight per lane), so each row
-// is fetched as one contiguous burst and x is never re-read.  Rows
-// blockIdx.x * R + k * gridDim.x * R ...
-template <class T, int UV, int R>
+// gemv-N with a contiguous, aligned x of at most kNT * VN * UV elements: the
+// workgroup keeps its slice of x in registers (thread t holds vectors t,
+// t + kNT, ...) for all its rows, and reads R whole rows at a time with every
+// thread (R * UV streaming loads in flight per lane), so each row is fetched
+// as one contiguous burst and x is never re-read.  Rows blockIdx.x * R +
+// k * gridDim.x * R ...: dealt round-robin, so a triangle's rows of uneven
+// length balance over the workgroups.  TRI / UNIT as in gemv_n_kernel (a
+// lane loads only the vectors that meet its row's part of the triangle).
+template <class T, int UV, int R, int TRI = kFull, bool UNIT = false>
 __global__ __launch_bounds__(kNT) void gemv_n_wg_kernel(int64_t M, int64_t N, T alpha, T beta,
                                                         const T* __restrict__ A, int64_t lda,
                                                         const T* __restrict__ x, T* y, int64_t incy) {
@@ -228,6 +236,18 @@ __global__ __launch_bounds__(kNT) void gemv_n_wg_kernel(int64_t M, int64_t N, T 
     }
   }
   const T xt = t < tail ? x[nv * VN + t] : T(0);
+  // element (row, col) inside the row's part of the triangle
+  auto in_tri = [](int64_t row, int64_t col) {
+    if (TRI == kLowerTri) return UNIT ? col < row : col <= row;
+    if (TRI == kUpperTri) return UNIT ? col > row : col >= row;
+    return true;
+  };
+  // vector v meets the row's part
+  auto meets = [](int64_t row, int64_t v) {
+    if (TRI == kLowerTri) return v * VN <= (UNIT ? row - 1 : row);
+    if (TRI == kUpperTri) return v * VN + VN - 1 >= (UNIT ? row + 1 : row);
+    return true;
+  };
   for (int64_t r0 = (int64_t)blockIdx.x * R; r0 < M; r0 += (int64_t)gridDim.x * R) {
     V av[R][UV];
 #pragma unroll
@@ -237,24 +257,29 @@ __global__ __launch_bounds__(kNT) void gemv_n_wg_kernel(int64_t M, int64_t N, T 
 #pragma unroll
       for (int u = 0; u < UV; ++u) {
         const int64_t v = t + (int64_t)u * kNT;
-        if (v < nv) av[rr][u] = ld_stream<T>(Ar + v * VN);
+        if (v < nv && meets(row, v)) av[rr][u] = ld_stream<T>(Ar + v * VN);
       }
     }
     T acc[R];
 #pragma unroll
     for (int rr = 0; rr < R; ++rr) {
+      const int64_t row = r0 + rr < M ? r0 + rr : M - 1;
       acc[rr] = T(0);
 #pragma unroll
       for (int u = 0; u < UV; ++u) {
         const int64_t v = t + (int64_t)u * kNT;
-        if (v < nv)
+        if (v < nv && meets(row, v)) {
+          if (TRI == kFull) {
 #pragma unroll
-          for (int k = 0; k < VN; ++k) acc[rr] += vget<T>(av[rr][u], k) * vget<T>(xr[u], k);
+            for (int k = 0; k < VN; ++k) acc[rr] += vget<T>(av[rr][u], k) * vget<T>(xr[u], k);
+          } else {
+#pragma unroll
+            for (int k = 0; k < VN; ++k)
+              acc[rr] += in_tri(row, v * VN + k) ? vget<T>(av[rr][u], k) * vget<T>(xr[u], k) : T(0);
+          }
+        }
       }
-      if (t < tail) {
-        const int64_t row = r0 + rr < M ? r0 + rr : M - 1;
-        acc[rr] += A[row * lda + nv * VN + t] * xt;
-      }
+      if (t < tail && in_tri(row, nv * VN + t)) acc[rr] += A[row * lda + nv * VN + t] * xt;
       acc[rr] = wave_sum(acc[rr]);
       if (lane == 0) red[rr][wave] = acc[rr];
     }
@@ -263,7 +288,10 @@ __global__ __launch_bounds__(kNT) void gemv_n_wg_kernel(int64_t M, int64_t N, T 
       T s = red[t][0];
 #pragma unroll
       for (int w = 1; w < NW; ++w) s += red[t][w];
-      y[(r0 + t) * incy] = beta * y[(r0 + t) * incy] + alpha * s;
+      if (TRI != kFull)
+        y[(r0 + t) * incy] = UNIT ? s + x[r0 + t] : s;
+      else
+        y[(r0 + t) * incy] = beta * y[(r0 + t) * incy] + alpha * s;
     }
     __syncthreads();
   }
@@ -463,11 +491,11 @@ void launch_gemv_n(Ctx& c, int64_t M, int64_t N, T alpha, T beta, const T* A, in
     constexpr int R = 2;
     const unsigned grid = (unsigned)std::min<int64_t>(ceil_div(M, R), (int64_t)12 * c.num_cus);
     if (ceil_div(N, VN) <= (int64_t)kNT * 8)
-      hipLaunchKernelGGL((gemv_n_wg_kernel<T, 8, R>), dim3(grid), dim3(kNT), 0, c.stream, M, N, alpha, beta, A, lda,
-                         x, y, incy);
+      hipLaunchKernelGGL((gemv_n_wg_kernel<T, 8, R, TRI, UNIT>), dim3(grid), dim3(kNT), 0, c.stream, M, N, alpha,
+                         beta, A, lda, x, y, incy);
     else
-      hipLaunchKernelGGL((gemv_n_wg_kernel<T, 16, R>), dim3(grid), dim3(kNT), 0, c.stream, M, N, alpha, beta, A,
-                         lda, x, y, incy);
+      hipLaunchKernelGGL((gemv_n_wg_kernel<T, 16, R, TRI, UNIT>), dim3(grid), dim3(kNT), 0, c.stream, M, N, alpha,
+                         beta, A, lda, x, y, incy);
     EXO_LAUNCH_CHECK();
     return;
   }
@@ -550,161 +578,16 @@ void launch_gemv_t(Ctx& c, int64_t Mc, int64_t Nr, T alpha, T beta, const T* A, 
 
 // ------------------------------------------------------------ rank updates
 
-// A[i,j] += r1[i]*c1[j] + r2[i]*c2[j] over rows [0,m) and the column range
-// of TRI; r1[i] = alpha*u1[i], r2[i] = alpha*u2[i] (u2/c2 null for rank 1).
-// Workgroup (bx, by): rows by*RB .. +RB, column vectors bx*kNT .. +kNT.
-template <class T, bool VEC, int TRI, bool TWO>
-__global__ __launch_bounds__(kNT) void rank_update_kernel(int64_t m, int64_t n, T alpha, const T* u1,
-                                                          int64_t inc_u1, const T* c1, int64_t inc_c1,
-                                                          const T* u2, int64_t inc_u2, const T* c2,
-                                                          int64_t inc_c2, T* A, int64_t lda, int rows_per) {
-  using V = typename Vec16<T>::type;
-  constexpr int VN = Vec16<T>::n;
-  const int64_t j0 = ((int64_t)blockIdx.x * kNT + threadIdx.x) * VN;
-  if (j0 >= n) return;
-  const int64_t i_beg = (int64_t)blockIdx.y * rows_per;
-  const int64_t i_end = i_beg + rows_per < m ? i_beg + rows_per : m;
-  T cv1[VN], cv2[VN];
-#pragma unroll
-  for (int k = 0; k < VN; ++k) {
-    const int64_t j = j0 + k;
-    cv1[k] = j < n ? c1[j * inc_c1] : T(0);
-    cv2[k] = (TWO && j < n) ? c2[j * inc_c2] : T(0);
-  }
-  for (int64_t i = i_beg; i < i_end; ++i) {
-    int64_t jl = 0, jh = n;  // [jl, jh)
-    if (TRI == kLowerTri) jh = i + 1;
-    if (TRI == kUpperTri) jl = i;
-    if (j0 + VN <= jl || j0 >= jh) continue;
-    const T r1 = alpha * u1[i * inc_u1];
-    const T r2 = TWO ? alpha * u2[i * inc_u2] : T(0);
-    T* row = A + i * lda;
-    if (VEC && j0 >= jl && j0 + VN <= jh) {
-      V a = ld_stream<T>(row + j0);
-#pragma unroll
-      for (int k = 0; k < VN; ++k) {
-        if (TWO)
-          vget<T>(a, k) += r1 * cv1[k] + r2 * cv2[k];
-        else
-          vget<T>(a, k) += r1 * cv1[k];
-      }
-      st_stream<T>(row + j0, a);
-    } else {
-#pragma unroll
-      for (int k = 0; k < VN; ++k) {
-        const int64_t j = j0 + k;
-        if (j >= jl && j < jh) {
-          if (TWO)
-            row[j] += r1 * cv1[k] + r2 * cv2[k];
-          else
-            row[j] += r1 * cv1[k];
-        }
-      }
-    }
-  }
-}
-
+// A[i,j] += (alpha u1[i]) c1[j] (+ (alpha u2[i]) c2[j] for rank 2) over the
+// TRI part: the streaming tile update of tiles.hpp.
 template <class T, int TRI, bool TWO>
 void launch_rank_update(Ctx& c, int64_t m, int64_t n, T alpha, const T* u1, int64_t iu1, const T* c1,
                         int64_t ic1, const T* u2, int64_t iu2, const T* c2, int64_t ic2, T* A, int64_t lda) {
-  if (m <= 0 || n <= 0) return;
-  constexpr int VN = Vec16<T>::n;
-  const bool vec = (lda % VN) == 0 && aligned16(A);
-  const int64_t col_blocks = ceil_div(ceil_div(n, VN), kNT);
-  const int rows_per = 16;
-  dim3 grid((unsigned)col_blocks, (unsigned)ceil_div(m, rows_per));
-  if (vec)
-    hipLaunchKernelGGL((rank_update_kernel<T, true, TRI, TWO>), grid, dim3(kNT), 0, c.stream, m, n, alpha, u1, iu1,
-                       c1, ic1, u2, iu2, c2, ic2, A, lda, rows_per);
-  else
-    hipLaunchKernelGGL((rank_update_kernel<T, false, TRI, TWO>), grid, dim3(kNT), 0, c.stream, m, n, alpha, u1,
-                       iu1, c1, ic1, u2, iu2, c2, ic2, A, lda, rows_per);
-  EXO_LAUNCH_CHECK();
+  const tiles::UpdateArgs<T> g{alpha, u1, iu1, c1, ic1, u2, iu2, c2, ic2};
+  tiles::launch_update<T, TWO ? 2 : 1, TRI>(c.stream, m, n, A, lda, g);
 }
 
 // ------------------------------------------------------------ symv
-
-constexpr int kSymTile = 64;
-
-// One workgroup per stored 64x64 tile (I, J).  UPPER selects the stored
-// triangle (I <= J for upper, I >= J for lower).  Writes
-//   part[J * n + I*64 + r] = (A_IJ x_J)[r]            (row sums -> y_I)
-//   part[I * n + J*64 + c] = (A_IJ^T x_I)[c]  I != J  (column sums -> y_J)
-template <class T, bool UPPER>
-__global__ __launch_bounds__(kNT) void symv_tile_kernel(int64_t n, const T* __restrict__ A, int64_t lda,
-                                                        const T* __restrict__ x, int64_t incx, T* part,
-                                                        int64_t nt) {
-  __shared__ T tile[kSymTile][kSymTile + 1];
-  __shared__ T xi[kSymTile], xj[kSymTile];
-  // tile index -> (I, J) with I >= J (lower enumeration), then mirrored for upper
-  const int64_t t = blockIdx.x;
-  int64_t I = (int64_t)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
-  while (I * (I + 1) / 2 > t) --I;
-  while ((I + 1) * (I + 2) / 2 <= t) ++I;
-  int64_t J = t - I * (I + 1) / 2;
-  if (UPPER) {  // stored tile (J, I) with J <= I: rename so (I, J) is the stored tile
-    const int64_t tmp = I;
-    I = J;
-    J = tmp;
-  }
-  const int64_t r0 = I * kSymTile, c0 = J * kSymTile;
-  const int tc = threadIdx.x % kSymTile;
-  const int tr = threadIdx.x / kSymTile;
-  for (int k = 0; k < kSymTile / (kNT / kSymTile); ++k) {
-    const int r = tr + k * (kNT / kSymTile);
-    const int64_t gr = r0 + r, gc = c0 + tc;
-    T v = T(0);
-    bool stored = gr < n && gc < n;
-    if (I == J) stored = stored && (UPPER ? tc >= r : tc <= r);
-    if (stored) v = A[gr * lda + gc];
-    tile[r][tc] = v;
-  }
-  if (threadIdx.x < kSymTile) {
-    const int64_t g = r0 + threadIdx.x;
-    xi[threadIdx.x] = g < n ? x[g * incx] : T(0);
-  } else if (threadIdx.x < 2 * kSymTile) {
-    const int64_t g = c0 + threadIdx.x - kSymTile;
-    xj[threadIdx.x - kSymTile] = g < n ? x[g * incx] : T(0);
-  }
-  __syncthreads();
-  if (I == J) {  // complete the symmetric diagonal tile
-    for (int k = 0; k < kSymTile / (kNT / kSymTile); ++k) {
-      const int r = tr + k * (kNT / kSymTile);
-      if (UPPER ? tc < r : tc > r) tile[r][tc] = tile[tc][r];
-    }
-    __syncthreads();
-  }
-  // 4 threads per row / column, 16 elements each
-  const int q = threadIdx.x % 4;
-  const int idx = threadIdx.x / 4;  // 0..63
-  T rs = T(0), cs = T(0);
-#pragma unroll
-  for (int k = 0; k < 16; ++k) {
-    const int cc = q * 16 + k;
-    rs += tile[idx][cc] * xj[cc];
-    cs += tile[cc][idx] * xi[cc];
-  }
-  rs += __shfl_xor(rs, 1, kWave);
-  rs += __shfl_xor(rs, 2, kWave);
-  cs += __shfl_xor(cs, 1, kWave);
-  cs += __shfl_xor(cs, 2, kWave);
-  if (q == 0) {
-    const int64_t npad = nt * kSymTile;
-    part[J * npad + r0 + idx] = rs;
-    if (I != J) part[I * npad + c0 + idx] = cs;
-  }
-}
-
-template <class T>
-__global__ __launch_bounds__(kNT) void symv_finish_kernel(int64_t n, int64_t nt, T alpha, T beta, const T* part,
-                                                          T* y, int64_t incy) {
-  const int64_t i = (int64_t)blockIdx.x * kNT + threadIdx.x;
-  if (i >= n) return;
-  const int64_t npad = nt * kSymTile;
-  T s = T(0);
-  for (int64_t o = 0; o < nt; ++o) s += part[o * npad + i];
-  y[i * incy] = beta * y[i * incy] + alpha * s;
-}
 
 }  // namespace
 
@@ -757,18 +640,32 @@ template <class T>
 void symv_rm(Ctx& c, int64_t Uplo, int64_t n, T alpha, const T* A, int64_t lda, const T* x, int64_t incx,
              T beta, T* y, int64_t incy) {
   if (n <= 0) return;
-  const int64_t nt = ceil_div(n, kSymTile);
-  T* part = static_cast<T*>(c.scratch(sizeof(T) * nt * nt * kSymTile));
-  const int64_t tiles = nt * (nt + 1) / 2;
-  if (Uplo == kUpper)
-    hipLaunchKernelGGL((symv_tile_kernel<T, true>), dim3((unsigned)tiles), dim3(kNT), 0, c.stream, n, A, lda, x,
-                       incx, part, nt);
-  else
-    hipLaunchKernelGGL((symv_tile_kernel<T, false>), dim3((unsigned)tiles), dim3(kNT), 0, c.stream, n, A, lda, x,
-                       incx, part, nt);
+  constexpr int VN = Vec16<T>::n;
+  const bool upper = Uplo == kUpper;
+  const tiles::TileList tl = tiles::make_list<symv::kRB, kNT * VN>(upper ? tiles::kUpper : tiles::kLower, n, n);
+  const int64_t npad = tl.nb * symv::kRB;
+  T* part = static_cast<T*>(c.scratch(sizeof(T) * (size_t)((tl.nc + tl.nb) * npad)));
+  T* rowpart = part;
+  T* colpart = part + tl.nc * npad;
+  const int64_t ntiles = tl.total();
+  if (ntiles > 0x7fffffff) throw Error(kInvalidSize, "symv: n too large");
+  const bool vec = (lda % VN) == 0 && (n % VN) == 0 && aligned16(A);
+#define EXO_SYMV(U, V)                                                                                         \
+  hipLaunchKernelGGL((symv::tile_kernel<T, U, V, symv::kRB>), dim3((unsigned)ntiles), dim3(kNT), 0, c.stream, n, A, lda, x, \
+                     incx, rowpart, colpart, tl)
+  if (upper) {
+    if (vec) EXO_SYMV(true, true); else EXO_SYMV(true, false);
+  } else {
+    if (vec) EXO_SYMV(false, true); else EXO_SYMV(false, false);
+  }
+#undef EXO_SYMV
   EXO_LAUNCH_CHECK();
-  hipLaunchKernelGGL((symv_finish_kernel<T>), dim3((unsigned)ceil_div(n, kNT)), dim3(kNT), 0, c.stream, n, nt,
-                     alpha, beta, part, y, incy);
+  if (upper)
+    hipLaunchKernelGGL((symv::finish_kernel<T, true, symv::kRB>), dim3((unsigned)ceil_div(n, kWave)), dim3(kNT), 0, c.stream,
+                       n, alpha, beta, rowpart, colpart, tl, y, incy);
+  else
+    hipLaunchKernelGGL((symv::finish_kernel<T, false, symv::kRB>), dim3((unsigned)ceil_div(n, kWave)), dim3(kNT), 0, c.stream,
+                       n, alpha, beta, rowpart, colpart, tl, y, incy);
   EXO_LAUNCH_CHECK();
 }
 

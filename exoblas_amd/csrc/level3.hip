@@ -11,6 +11,7 @@
 #include "gemm_core.hpp"
 #include "level3.hpp"
 #include "ops.hpp"
+#include "tiles.hpp"
 
 namespace exoblas {
 namespace gemm {
@@ -335,55 +336,14 @@ __global__ __launch_bounds__(kNT) void copy_clear_kernel(int64_t i0, int64_t n, 
 }
 
 // A[i, j] *= alpha over the full M x N matrix (TRI 0) or the lower (1) /
-// upper (2) triangle of an N x N matrix.  blockIdx.y picks the row (i0 +
-// blockIdx.y); VEC uses 16-byte accesses for the aligned body of the row's
-// column range and scalar accesses for its ragged ends.
-template <class T, int TRI, bool VEC>
-__global__ __launch_bounds__(kNT) void mscal_kernel(int64_t i0, int64_t N, T alpha, T* A, int64_t lda) {
-  constexpr int VN = Vec16<T>::n;
-  using V = typename Vec16<T>::type;
-  const int64_t i = i0 + blockIdx.y;
-  int64_t jl = 0, jh = N;
-  if (TRI == 1) jh = i + 1 < N ? i + 1 : N;
-  if (TRI == 2) jl = i;
-  T* row = A + i * lda;
-  const int64_t tid = (int64_t)blockIdx.x * kNT + threadIdx.x, step = (int64_t)gridDim.x * kNT;
-  if (VEC) {
-    const int64_t a = std::min<int64_t>(ceil_div(jl, VN) * VN, jh);
-    const int64_t b = std::max<int64_t>(jh / VN * VN, a);
-    for (int64_t v = a / VN + tid; v < b / VN; v += step) {
-      V x = reinterpret_cast<V*>(row)[v];
-#pragma unroll
-      for (int e = 0; e < VN; ++e) vget<T>(x, e) = vget<T>(x, e) * alpha;
-      reinterpret_cast<V*>(row)[v] = x;
-    }
-    if (tid < VN) {
-      const int64_t h = jl + tid;
-      if (h < a) row[h] = row[h] * alpha;
-      const int64_t t = b + tid;
-      if (t < jh) row[t] = row[t] * alpha;
-    }
-  } else {
-    for (int64_t j = jl + tid; j < jh; j += step) row[j] = row[j] * alpha;
-  }
-}
-
+// upper (2) triangle of an N x N matrix: the streaming tile update of
+// tiles.hpp (each stored element read and written once, 8 rows of loads in
+// flight per lane).
 template <class T, int TRI>
 void launch_mscal(Ctx& c, int64_t M, int64_t N, T alpha, T* A, int64_t lda) {
-  if (M <= 0 || N <= 0) return;
-  constexpr int VN = Vec16<T>::n;
-  const bool vec = lda % VN == 0 && aligned16(A);
-  const int64_t per_row = vec ? ceil_div(N, VN) : N;
-  const unsigned gx = (unsigned)std::max<int64_t>(1, std::min<int64_t>(ceil_div(per_row, kNT), 64));
-  for (int64_t r0 = 0; r0 < M; r0 += 65535) {
-    const int64_t rows = std::min<int64_t>(65535, M - r0);
-    dim3 grid(gx, (unsigned)rows);
-    if (vec)
-      hipLaunchKernelGGL((mscal_kernel<T, TRI, true>), grid, dim3(kNT), 0, c.stream, r0, N, alpha, A, lda);
-    else
-      hipLaunchKernelGGL((mscal_kernel<T, TRI, false>), grid, dim3(kNT), 0, c.stream, r0, N, alpha, A, lda);
-    EXO_LAUNCH_CHECK();
-  }
+  tiles::UpdateArgs<T> g{};
+  g.alpha = alpha;
+  tiles::launch_update<T, 0, TRI>(c.stream, M, N, A, lda, g);
 }
 
 }  // namespace
