@@ -372,12 +372,36 @@ def bench_bandwidth(a, torch, so, L):
 FP32_PEAK_TF, FP64_PEAK_TF, BF16_PEAK_TF = 157.3, 78.6, 2500.0
 
 
+_SPIN = []
+
+
+def _spin(torch, stream, us=30.0):
+    """Enqueues about `us` microseconds of device spin on `stream` (cycles
+    per microsecond calibrated once), so that the next call is enqueued
+    before the device reaches it: the events around the call then bracket
+    device time only, not the host's enqueue time (which matters for calls of
+    a few microseconds)."""
+    if not _SPIN:
+        with torch.cuda.stream(stream):
+            torch.cuda._sleep(1000)
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            torch.cuda._sleep(200000)
+            e1.record(stream)
+        torch.cuda.synchronize()
+        _SPIN.append(200000 / max(e0.elapsed_time(e1) * 1e3, 1e-3))  # cycles per us
+    with torch.cuda.stream(stream):
+        torch.cuda._sleep(max(1000, int(_SPIN[0] * us)))
+
+
 def _timed(torch, stream, fn, reps, before=None):
-    """Average ms of fn() over reps launches, HIP events on `stream`."""
+    """Average ms of fn() over reps launches, HIP events on `stream`, each
+    call's events enqueued behind a device spin (_spin)."""
     ts = []
     for _ in range(reps):
         if before is not None:
             before()
+        _spin(torch, stream)
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         e0.record(stream)
         rc = fn()
@@ -388,6 +412,28 @@ def _timed(torch, stream, fn, reps, before=None):
         ts.append((e0, e1))
     torch.cuda.synchronize()
     return sum(a.elapsed_time(b) for a, b in ts) / reps
+
+
+def _timed_batch(torch, stream, fn, reps, per=20):
+    """Average device ms per call of fn() for calls of a few microseconds:
+    `per` calls back to back between one event pair, behind a device spin
+    long enough for the host to enqueue them all, so neither the host's
+    enqueue time nor the events' own cost is in the figure; inter-kernel
+    gaps of a stream are."""
+    tot = 0.0
+    for _ in range(reps):
+        _spin(torch, stream, 40.0 * per)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _k in range(per):
+            rc = fn()
+            if rc not in (0, None):
+                from exoblas_amd import lib as L
+                raise RuntimeError(L.last_error())
+        e1.record(stream)
+        torch.cuda.synchronize()
+        tot += e0.elapsed_time(e1) / per
+    return tot / reps
 
 
 def _line(metric, value, unit, a, ms, dtype, workload, roof, kernels, higher=True, **extra):
@@ -898,16 +944,24 @@ def config1(a, torch, so, L):
         r1 = so.exoblas_smscal_rm_stride_1_async(sh, n, n, _fl(0.5), wc)
         r2 = so.exoblas_sgemm_stride_1_async(sh, 111, 111, n, n, n, _fl(1.5), wa, wb, wa, wb, wc)
         return r1 or r2
+
+    def fused():  # CBLAS-shaped extension: beta in the product's epilogue, one launch
+        return so.exoblas_sgemm_ex_async(sh, 101, 111, 111, n, n, n, _fl(1.5), wa, wb, _fl(0.5), wc)
     for _ in range(5):
         wrapper_sequence()
-    ms = _timed(torch, stream, wrapper_sequence, 50)
+        fused()
+    ms_w = _timed_batch(torch, stream, wrapper_sequence, 10)
+    ms = _timed_batch(torch, stream, fused, 10)
     flop = 2.0 * n ** 3
-    tf = flop / (ms * 1e-3) / 1e12
+    tf, tf_w = flop / (ms * 1e-3) / 1e12, flop / (ms_w * 1e-3) / 1e12
     kernels = {"sgemm_NN_512_beta": {"us": round(ms * 1e3, 2), "TFLOP/s": round(tf, 2), "flop": flop,
-                                     "launches": "mscal + gemm"}}
-    out = {"workload": "sgemm NN 512^3, alpha 1.5, beta 0.5 (mscal + gemm), seed 0", "us": round(ms * 1e3, 2),
-           "TFLOP/s": round(tf, 2), "roofline": _roof("sgemm_NN_512_beta", tf, FP32_PEAK_TF, "TFLOP/s", "mfma"),
-           "kernels": kernels}
+                                     "launches": "one (exoblas_sgemm_ex_async, beta fused)",
+                                     "timing": "device time per call, 20 calls back to back behind a spin"},
+               "sgemm_NN_512_beta_wrapper": {"us": round(ms_w * 1e3, 2), "TFLOP/s": round(tf_w, 2), "flop": flop,
+                                             "launches": "mscal + gemm, as the reference's wrapper calls them"}}
+    out = {"workload": "sgemm NN 512^3, alpha 1.5, beta 0.5, seed 0 (beta fused, one launch)",
+           "us": round(ms * 1e3, 2), "TFLOP/s": round(tf, 2),
+           "roofline": _roof("sgemm_NN_512_beta", tf, FP32_PEAK_TF, "TFLOP/s", "mfma"), "kernels": kernels}
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_leg("config1", a)
     return out
@@ -1279,6 +1333,82 @@ def families(a, torch, so, L):
 
 
 
+def strided(a, torch, so, L):
+    """Strided and misaligned windows through the element-by-element kernels:
+    s/d axpy, sdot, scopy at incx = 2, incy = -3 and n = 2^26, saxpy on
+    unit-stride windows one element off 16-byte alignment, and sgemv N / T
+    8192^2 with incx = 2, incy = -3 (two operand sets taken in turn).  Two
+    byte counts per kernel: `bytes`, the elements moved (as for unit stride:
+    axpy 3ns, dot 2ns, copy 2ns, gemv (n^2 + 3n)s), and `touched`, the
+    memory lines the elements lie in (each window's span |inc| n s, counted
+    twice when written), which is what HBM has to move; `frac` is GB/s over
+    touched bytes against HBM peak."""
+    stream = torch.cuda.current_stream()
+    sh = stream.cuda_stream
+    g = torch.Generator(device="cuda")
+    g.manual_seed(11)
+    n1, n2 = a.n1, a.n2
+    kernels = {}
+    res = torch.zeros(2, dtype=torch.float64, device="cuda")
+    rp = ctypes.c_void_p(res.data_ptr())
+
+    def span(n, inc, s_):
+        return n * min(abs(inc) * s_, 128)
+
+    def win(t, n, inc, s_):  # element 0 of an n-vector at increment inc in buffer t
+        return L.win1(t.data_ptr() + ((n - 1) * abs(inc) * s_ if inc < 0 else 0), inc)
+
+    def run(name, make, nbytes, touched, reps=10):
+        fns = [make(k) for k in range(2)]
+        ms = _timed_alt(torch, stream, fns, reps)
+        gbs, gbt = nbytes / (ms * 1e-3) / 1e9, touched / (ms * 1e-3) / 1e9
+        kernels[name] = {"us": round(ms * 1e3, 2), "GB/s": round(gbs, 1), "GB/s_touched": round(gbt, 1),
+                         "frac": round(gbt / HBM_PEAK_GBS, 4), "bytes": nbytes, "touched": touched}
+    ix, iy = 2, -3
+    for p, dt, s_ in (("s", torch.float32, 4), ("d", torch.float64, 8)):
+        X = [_u(torch, g, 1 + (n1 - 1) * abs(ix), dt) for _ in range(2)]
+        Y = [_u(torch, g, 1 + (n1 - 1) * abs(iy), dt) for _ in range(2)]
+        t_xy = span(n1, ix, s_) + 2 * span(n1, iy, s_)
+        run(f"{p}axpy_inc2_-3", lambda k: (lambda: getattr(so, f"exoblas_{p}axpy_stride_any_async")(
+            sh, n1, _fl(0.5, p), win(X[k], n1, ix, s_), win(Y[k], n1, iy, s_))), 3 * n1 * s_, t_xy)
+        if p == "s":
+            run("sdot_inc2_-3", lambda k: (lambda: so.exoblas_sdot_stride_any_async(
+                sh, n1, win(X[k], n1, ix, 4), win(Y[k], n1, iy, 4), rp)), 2 * n1 * 4,
+                span(n1, ix, 4) + span(n1, iy, 4))
+            run("scopy_inc2_-3", lambda k: (lambda: so.exoblas_scopy_stride_any_async(
+                sh, n1, win(X[k], n1, ix, 4), win(Y[k], n1, iy, 4))), 2 * n1 * 4, span(n1, ix, 4) + 2 * span(n1, iy, 4))
+        del X, Y
+        torch.cuda.empty_cache()
+    Xm = [_u(torch, g, n1 + 1, torch.float32) for _ in range(2)]
+    Ym = [_u(torch, g, n1 + 1, torch.float32) for _ in range(2)]
+    run("saxpy_misaligned", lambda k: (lambda: so.exoblas_saxpy_stride_any_async(
+        sh, n1, _fl(0.5), L.win1(Xm[k].data_ptr() + 4, 1), L.win1(Ym[k].data_ptr() + 4, 1))), 12 * n1, 12 * n1)
+    del Xm, Ym
+    Am = [_u(torch, g, (n2, n2), torch.float32) for _ in range(2)]
+    xv = [_u(torch, g, 1 + (n2 - 1) * abs(ix), torch.float32) for _ in range(2)]
+    yv = [_u(torch, g, 1 + (n2 - 1) * abs(iy), torch.float32) for _ in range(2)]
+    gb = (n2 * n2 + 3 * n2) * 4
+    gt = n2 * n2 * 4 + span(n2, ix, 4) + 2 * span(n2, iy, 4)
+    for tr, tname in ((111, "N"), (112, "T")):
+        run(f"sgemv_{tname}_inc2_-3", lambda k, tr=tr: (lambda: so.exoblas_sgemv_rm_stride_any_async(
+            sh, tr, n2, n2, _fl(1.5), _fl(0.5), L.win2(Am[k].data_ptr(), n2), L.win2(Am[k].data_ptr(), n2),
+            win(xv[k], n2, ix, 4), win(yv[k], n2, iy, 4))), gb, gt)
+    del Am, xv, yv
+    torch.cuda.empty_cache()
+    lowest = min(kernels, key=lambda k: kernels[k]["frac"])
+    dom = max(kernels, key=lambda k: kernels[k]["us"])
+    out = {"workload": "s/d axpy, sdot, scopy incx=2 incy=-3 n=2^26; saxpy misaligned n=2^26; sgemv N/T 8192^2 "
+                       "incx=2 incy=-3",
+           "roofline": _roof(dom, kernels[dom]["GB/s_touched"], HBM_PEAK_GBS, "GB/s", "hbm"),
+           "lowest_frac_kernel": lowest, "kernels": kernels}
+    if not a.no_cpu:
+        cpu = cpu_leg("strided", a)
+        cb = dict(cpu.get("saxpy_inc2_-3", {}))
+        cb["kernels"] = cpu
+        out["cpu_baseline"] = cb
+    return out
+
+
 # ---------------------------------------------------------------- CPU legs
 # Every CPU baseline runs in a child process that never imports torch: with
 # torch's OpenMP runtime in the process the vendor CBLAS's own threads
@@ -1443,10 +1573,32 @@ def cpu_families(a):
     return cpu
 
 
-CPU_LEGS = {"bandwidth": cpu_bandwidth, "config1": cpu_config1, "config3": cpu_config3, "config4": cpu_config4,
-            "config5": cpu_config5, "families": cpu_families}
+def cpu_strided(a):
+    from oracle import Oracle
+    rng = np.random.default_rng(0)
+    o = Oracle(np.float32, kind="fast")
+    m1, m2 = a.n1, a.n2
+    x = rng.uniform(-1, 1, 1 + (m1 - 1) * 2).astype(np.float32)
+    y = rng.uniform(-1, 1, 1 + (m1 - 1) * 3).astype(np.float32)
+    A = rng.uniform(-1, 1, (m2, m2)).astype(np.float32)
+    xv = rng.uniform(-1, 1, 1 + (m2 - 1) * 2).astype(np.float32)
+    yv = rng.uniform(-1, 1, 1 + (m2 - 1) * 3).astype(np.float32)
+    b = a.cpu_budget / 12
+    return {
+        "saxpy_inc2_-3": _port(lambda: o.axpy(m1, 0.5, x, 2, y, -3), 12 * m1, 1e9, "GB/s", b,
+                               "saxpy n=2^26 incx 2 incy -3, 1 thread"),
+        "sdot_inc2_-3": _port(lambda: o.dot(m1, x, 2, y, -3), 8 * m1, 1e9, "GB/s", b,
+                              "sdot n=2^26 incx 2 incy -3, 1 thread"),
+        "sgemv_N_inc2_-3": _port(lambda: o.gemv(111, m2, m2, 1.5, 0.5, A, m2, xv, 2, yv, -3),
+                                 (m2 * m2 + 3 * m2) * 4, 1e9, "GB/s", b, "sgemv N 8192^2 incx 2 incy -3, 1 thread"),
+    }
 
-CONFIGS = {"config1": config1, "config3": config3, "config4": config4, "config5": config5, "families": families}
+
+CPU_LEGS = {"bandwidth": cpu_bandwidth, "config1": cpu_config1, "config3": cpu_config3, "config4": cpu_config4,
+            "config5": cpu_config5, "families": cpu_families, "strided": cpu_strided}
+
+CONFIGS = {"config1": config1, "config3": config3, "config4": config4, "config5": config5, "families": families,
+           "strided": strided}
 
 
 def run_config(name):

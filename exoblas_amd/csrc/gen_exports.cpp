@@ -5005,6 +5005,182 @@ extern "C" int exoblas_ssyrk_split_async(hipStream_t stream, int_fast32_t Uplo, 
   return call.finish();
 }
 
+// C = alpha op(A) op(B) + beta C with CBLAS dimensions in row-major (Order 101) or column-major (102) storage, C not read when beta == 0, beta applied in the product's epilogue; each window is the stored array (rows x leading dimension); the product as exo_<p>gemm_stride_1 (src/level3/gemm.py:17-47)
+extern "C" int exoblas_sgemm_ex(void *ctxt, int_fast32_t Order, int_fast32_t TransA, int_fast32_t TransB, int_fast32_t M, int_fast32_t N, int_fast32_t K, const float *alpha, struct exo_win_2f32c A, struct exo_win_2f32c B, const float *beta, struct exo_win_2f32 C) {
+  exoblas::Call call("exoblas_sgemm_ex", exoblas::Call::kSync, ctxt);
+  if (!(Order == 101 || Order == 102)) return call.fail_enum("Order", Order);
+  if (!(TransA == 111 || TransA == 112 || TransA == 113)) return call.fail_enum("TransA", TransA);
+  if (!(TransB == 111 || TransB == 112 || TransB == 113)) return call.fail_enum("TransB", TransB);
+  if (M < 0) return call.fail_size("M", M);
+  if (N < 0) return call.fail_size("N", N);
+  if (K < 0) return call.fail_size("K", K);
+  if ((true) && A.strides[1] != 1) return call.fail_stride("A", A.strides[1]);
+  if ((true) && ((Order == 101) == (TransA == 111) ? M : K) > 1 && A.strides[0] < ((Order == 101) == (TransA == 111) ? K : M)) return call.fail_ld("A", A.strides[0]);
+  if ((true) && B.strides[1] != 1) return call.fail_stride("B", B.strides[1]);
+  if ((true) && ((Order == 101) == (TransB == 111) ? K : N) > 1 && B.strides[0] < ((Order == 101) == (TransB == 111) ? N : K)) return call.fail_ld("B", B.strides[0]);
+  if ((true) && C.strides[1] != 1) return call.fail_stride("C", C.strides[1]);
+  if ((true) && (Order == 101 ? M : N) > 1 && C.strides[0] < (Order == 101 ? N : M)) return call.fail_ld("C", C.strides[0]);
+  call.window(0, A.data, 0, exoblas::span2_hi(((Order == 101) == (TransA == 111) ? M : K), ((Order == 101) == (TransA == 111) ? K : M), A.strides[0]), sizeof(float), false, true);
+  call.window(1, B.data, 0, exoblas::span2_hi(((Order == 101) == (TransB == 111) ? K : N), ((Order == 101) == (TransB == 111) ? N : K), B.strides[0]), sizeof(float), false, true);
+  call.window(2, C.data, 0, exoblas::span2_hi((Order == 101 ? M : N), (Order == 101 ? N : M), C.strides[0]), sizeof(float), true, true);
+  if (!call.stage()) return call.status();
+  const float alpha_v = call.scalar(alpha);
+  const float beta_v = call.scalar(beta);
+  if (!call.ok()) return call.status();
+  call.run([&] { exoblas::ops::gemm_ex(call.ctx(), (int64_t)Order, (int64_t)TransA, (int64_t)TransB, (int64_t)M, (int64_t)N, (int64_t)K, alpha_v, call.dev<const float>(0), (int64_t)A.strides[0], call.dev<const float>(1), (int64_t)B.strides[0], beta_v, call.dev<float>(2), (int64_t)C.strides[0]); });
+  return call.finish();
+}
+extern "C" int exoblas_sgemm_ex_async(hipStream_t stream, int_fast32_t Order, int_fast32_t TransA, int_fast32_t TransB, int_fast32_t M, int_fast32_t N, int_fast32_t K, const float *alpha, struct exo_win_2f32c A, struct exo_win_2f32c B, const float *beta, struct exo_win_2f32 C) {
+  exoblas::Call call("exoblas_sgemm_ex_async", exoblas::Call::kAsync, stream);
+  if (!(Order == 101 || Order == 102)) return call.fail_enum("Order", Order);
+  if (!(TransA == 111 || TransA == 112 || TransA == 113)) return call.fail_enum("TransA", TransA);
+  if (!(TransB == 111 || TransB == 112 || TransB == 113)) return call.fail_enum("TransB", TransB);
+  if (M < 0) return call.fail_size("M", M);
+  if (N < 0) return call.fail_size("N", N);
+  if (K < 0) return call.fail_size("K", K);
+  if ((true) && A.strides[1] != 1) return call.fail_stride("A", A.strides[1]);
+  if ((true) && ((Order == 101) == (TransA == 111) ? M : K) > 1 && A.strides[0] < ((Order == 101) == (TransA == 111) ? K : M)) return call.fail_ld("A", A.strides[0]);
+  if ((true) && B.strides[1] != 1) return call.fail_stride("B", B.strides[1]);
+  if ((true) && ((Order == 101) == (TransB == 111) ? K : N) > 1 && B.strides[0] < ((Order == 101) == (TransB == 111) ? N : K)) return call.fail_ld("B", B.strides[0]);
+  if ((true) && C.strides[1] != 1) return call.fail_stride("C", C.strides[1]);
+  if ((true) && (Order == 101 ? M : N) > 1 && C.strides[0] < (Order == 101 ? N : M)) return call.fail_ld("C", C.strides[0]);
+  call.window(0, A.data, 0, exoblas::span2_hi(((Order == 101) == (TransA == 111) ? M : K), ((Order == 101) == (TransA == 111) ? K : M), A.strides[0]), sizeof(float), false, true);
+  call.window(1, B.data, 0, exoblas::span2_hi(((Order == 101) == (TransB == 111) ? K : N), ((Order == 101) == (TransB == 111) ? N : K), B.strides[0]), sizeof(float), false, true);
+  call.window(2, C.data, 0, exoblas::span2_hi((Order == 101 ? M : N), (Order == 101 ? N : M), C.strides[0]), sizeof(float), true, true);
+  if (!call.stage()) return call.status();
+  const float alpha_v = call.scalar(alpha);
+  const float beta_v = call.scalar(beta);
+  if (!call.ok()) return call.status();
+  call.run([&] { exoblas::ops::gemm_ex(call.ctx(), (int64_t)Order, (int64_t)TransA, (int64_t)TransB, (int64_t)M, (int64_t)N, (int64_t)K, alpha_v, call.dev<const float>(0), (int64_t)A.strides[0], call.dev<const float>(1), (int64_t)B.strides[0], beta_v, call.dev<float>(2), (int64_t)C.strides[0]); });
+  return call.finish();
+}
+
+// C = alpha op(A) op(B) + beta C with CBLAS dimensions in row-major (Order 101) or column-major (102) storage, C not read when beta == 0, beta applied in the product's epilogue; each window is the stored array (rows x leading dimension); the product as exo_<p>gemm_stride_1 (src/level3/gemm.py:17-47)
+extern "C" int exoblas_dgemm_ex(void *ctxt, int_fast32_t Order, int_fast32_t TransA, int_fast32_t TransB, int_fast32_t M, int_fast32_t N, int_fast32_t K, const double *alpha, struct exo_win_2f64c A, struct exo_win_2f64c B, const double *beta, struct exo_win_2f64 C) {
+  exoblas::Call call("exoblas_dgemm_ex", exoblas::Call::kSync, ctxt);
+  if (!(Order == 101 || Order == 102)) return call.fail_enum("Order", Order);
+  if (!(TransA == 111 || TransA == 112 || TransA == 113)) return call.fail_enum("TransA", TransA);
+  if (!(TransB == 111 || TransB == 112 || TransB == 113)) return call.fail_enum("TransB", TransB);
+  if (M < 0) return call.fail_size("M", M);
+  if (N < 0) return call.fail_size("N", N);
+  if (K < 0) return call.fail_size("K", K);
+  if ((true) && A.strides[1] != 1) return call.fail_stride("A", A.strides[1]);
+  if ((true) && ((Order == 101) == (TransA == 111) ? M : K) > 1 && A.strides[0] < ((Order == 101) == (TransA == 111) ? K : M)) return call.fail_ld("A", A.strides[0]);
+  if ((true) && B.strides[1] != 1) return call.fail_stride("B", B.strides[1]);
+  if ((true) && ((Order == 101) == (TransB == 111) ? K : N) > 1 && B.strides[0] < ((Order == 101) == (TransB == 111) ? N : K)) return call.fail_ld("B", B.strides[0]);
+  if ((true) && C.strides[1] != 1) return call.fail_stride("C", C.strides[1]);
+  if ((true) && (Order == 101 ? M : N) > 1 && C.strides[0] < (Order == 101 ? N : M)) return call.fail_ld("C", C.strides[0]);
+  call.window(0, A.data, 0, exoblas::span2_hi(((Order == 101) == (TransA == 111) ? M : K), ((Order == 101) == (TransA == 111) ? K : M), A.strides[0]), sizeof(double), false, true);
+  call.window(1, B.data, 0, exoblas::span2_hi(((Order == 101) == (TransB == 111) ? K : N), ((Order == 101) == (TransB == 111) ? N : K), B.strides[0]), sizeof(double), false, true);
+  call.window(2, C.data, 0, exoblas::span2_hi((Order == 101 ? M : N), (Order == 101 ? N : M), C.strides[0]), sizeof(double), true, true);
+  if (!call.stage()) return call.status();
+  const double alpha_v = call.scalar(alpha);
+  const double beta_v = call.scalar(beta);
+  if (!call.ok()) return call.status();
+  call.run([&] { exoblas::ops::gemm_ex(call.ctx(), (int64_t)Order, (int64_t)TransA, (int64_t)TransB, (int64_t)M, (int64_t)N, (int64_t)K, alpha_v, call.dev<const double>(0), (int64_t)A.strides[0], call.dev<const double>(1), (int64_t)B.strides[0], beta_v, call.dev<double>(2), (int64_t)C.strides[0]); });
+  return call.finish();
+}
+extern "C" int exoblas_dgemm_ex_async(hipStream_t stream, int_fast32_t Order, int_fast32_t TransA, int_fast32_t TransB, int_fast32_t M, int_fast32_t N, int_fast32_t K, const double *alpha, struct exo_win_2f64c A, struct exo_win_2f64c B, const double *beta, struct exo_win_2f64 C) {
+  exoblas::Call call("exoblas_dgemm_ex_async", exoblas::Call::kAsync, stream);
+  if (!(Order == 101 || Order == 102)) return call.fail_enum("Order", Order);
+  if (!(TransA == 111 || TransA == 112 || TransA == 113)) return call.fail_enum("TransA", TransA);
+  if (!(TransB == 111 || TransB == 112 || TransB == 113)) return call.fail_enum("TransB", TransB);
+  if (M < 0) return call.fail_size("M", M);
+  if (N < 0) return call.fail_size("N", N);
+  if (K < 0) return call.fail_size("K", K);
+  if ((true) && A.strides[1] != 1) return call.fail_stride("A", A.strides[1]);
+  if ((true) && ((Order == 101) == (TransA == 111) ? M : K) > 1 && A.strides[0] < ((Order == 101) == (TransA == 111) ? K : M)) return call.fail_ld("A", A.strides[0]);
+  if ((true) && B.strides[1] != 1) return call.fail_stride("B", B.strides[1]);
+  if ((true) && ((Order == 101) == (TransB == 111) ? K : N) > 1 && B.strides[0] < ((Order == 101) == (TransB == 111) ? N : K)) return call.fail_ld("B", B.strides[0]);
+  if ((true) && C.strides[1] != 1) return call.fail_stride("C", C.strides[1]);
+  if ((true) && (Order == 101 ? M : N) > 1 && C.strides[0] < (Order == 101 ? N : M)) return call.fail_ld("C", C.strides[0]);
+  call.window(0, A.data, 0, exoblas::span2_hi(((Order == 101) == (TransA == 111) ? M : K), ((Order == 101) == (TransA == 111) ? K : M), A.strides[0]), sizeof(double), false, true);
+  call.window(1, B.data, 0, exoblas::span2_hi(((Order == 101) == (TransB == 111) ? K : N), ((Order == 101) == (TransB == 111) ? N : K), B.strides[0]), sizeof(double), false, true);
+  call.window(2, C.data, 0, exoblas::span2_hi((Order == 101 ? M : N), (Order == 101 ? N : M), C.strides[0]), sizeof(double), true, true);
+  if (!call.stage()) return call.status();
+  const double alpha_v = call.scalar(alpha);
+  const double beta_v = call.scalar(beta);
+  if (!call.ok()) return call.status();
+  call.run([&] { exoblas::ops::gemm_ex(call.ctx(), (int64_t)Order, (int64_t)TransA, (int64_t)TransB, (int64_t)M, (int64_t)N, (int64_t)K, alpha_v, call.dev<const double>(0), (int64_t)A.strides[0], call.dev<const double>(1), (int64_t)B.strides[0], beta_v, call.dev<double>(2), (int64_t)C.strides[0]); });
+  return call.finish();
+}
+
+// y = alpha op(A) x + beta y for an M x N matrix A in row-major (Order 101) or column-major (102) storage, y not read when beta == 0; element i of x at x.data + i * x.strides[0]; the product as exo_<p>gemv_rm_stride_any (src/level2/gemv.py:10-22)
+extern "C" int exoblas_sgemv_ex(void *ctxt, int_fast32_t Order, int_fast32_t TransA, int_fast32_t M, int_fast32_t N, const float *alpha, struct exo_win_2f32c A, struct exo_win_1f32c x, const float *beta, struct exo_win_1f32 y) {
+  exoblas::Call call("exoblas_sgemv_ex", exoblas::Call::kSync, ctxt);
+  if (!(Order == 101 || Order == 102)) return call.fail_enum("Order", Order);
+  if (!(TransA == 111 || TransA == 112 || TransA == 113)) return call.fail_enum("TransA", TransA);
+  if (M < 0) return call.fail_size("M", M);
+  if (N < 0) return call.fail_size("N", N);
+  if ((true) && A.strides[1] != 1) return call.fail_stride("A", A.strides[1]);
+  if ((true) && (Order == 101 ? M : N) > 1 && A.strides[0] < (Order == 101 ? N : M)) return call.fail_ld("A", A.strides[0]);
+  call.window(0, A.data, 0, exoblas::span2_hi((Order == 101 ? M : N), (Order == 101 ? N : M), A.strides[0]), sizeof(float), false, true);
+  call.window(1, x.data, exoblas::span_lo((TransA == 111 ? N : M), x.strides[0]), exoblas::span_hi((TransA == 111 ? N : M), x.strides[0]), sizeof(float), false, true);
+  call.window(2, y.data, exoblas::span_lo((TransA == 111 ? M : N), y.strides[0]), exoblas::span_hi((TransA == 111 ? M : N), y.strides[0]), sizeof(float), true, true);
+  if (!call.stage()) return call.status();
+  const float alpha_v = call.scalar(alpha);
+  const float beta_v = call.scalar(beta);
+  if (!call.ok()) return call.status();
+  call.run([&] { exoblas::ops::gemv_ex(call.ctx(), (int64_t)Order, (int64_t)TransA, (int64_t)M, (int64_t)N, alpha_v, call.dev<const float>(0), (int64_t)A.strides[0], call.dev<const float>(1), (int64_t)x.strides[0], beta_v, call.dev<float>(2), (int64_t)y.strides[0]); });
+  return call.finish();
+}
+extern "C" int exoblas_sgemv_ex_async(hipStream_t stream, int_fast32_t Order, int_fast32_t TransA, int_fast32_t M, int_fast32_t N, const float *alpha, struct exo_win_2f32c A, struct exo_win_1f32c x, const float *beta, struct exo_win_1f32 y) {
+  exoblas::Call call("exoblas_sgemv_ex_async", exoblas::Call::kAsync, stream);
+  if (!(Order == 101 || Order == 102)) return call.fail_enum("Order", Order);
+  if (!(TransA == 111 || TransA == 112 || TransA == 113)) return call.fail_enum("TransA", TransA);
+  if (M < 0) return call.fail_size("M", M);
+  if (N < 0) return call.fail_size("N", N);
+  if ((true) && A.strides[1] != 1) return call.fail_stride("A", A.strides[1]);
+  if ((true) && (Order == 101 ? M : N) > 1 && A.strides[0] < (Order == 101 ? N : M)) return call.fail_ld("A", A.strides[0]);
+  call.window(0, A.data, 0, exoblas::span2_hi((Order == 101 ? M : N), (Order == 101 ? N : M), A.strides[0]), sizeof(float), false, true);
+  call.window(1, x.data, exoblas::span_lo((TransA == 111 ? N : M), x.strides[0]), exoblas::span_hi((TransA == 111 ? N : M), x.strides[0]), sizeof(float), false, true);
+  call.window(2, y.data, exoblas::span_lo((TransA == 111 ? M : N), y.strides[0]), exoblas::span_hi((TransA == 111 ? M : N), y.strides[0]), sizeof(float), true, true);
+  if (!call.stage()) return call.status();
+  const float alpha_v = call.scalar(alpha);
+  const float beta_v = call.scalar(beta);
+  if (!call.ok()) return call.status();
+  call.run([&] { exoblas::ops::gemv_ex(call.ctx(), (int64_t)Order, (int64_t)TransA, (int64_t)M, (int64_t)N, alpha_v, call.dev<const float>(0), (int64_t)A.strides[0], call.dev<const float>(1), (int64_t)x.strides[0], beta_v, call.dev<float>(2), (int64_t)y.strides[0]); });
+  return call.finish();
+}
+
+// y = alpha op(A) x + beta y for an M x N matrix A in row-major (Order 101) or column-major (102) storage, y not read when beta == 0; element i of x at x.data + i * x.strides[0]; the product as exo_<p>gemv_rm_stride_any (src/level2/gemv.py:10-22)
+extern "C" int exoblas_dgemv_ex(void *ctxt, int_fast32_t Order, int_fast32_t TransA, int_fast32_t M, int_fast32_t N, const double *alpha, struct exo_win_2f64c A, struct exo_win_1f64c x, const double *beta, struct exo_win_1f64 y) {
+  exoblas::Call call("exoblas_dgemv_ex", exoblas::Call::kSync, ctxt);
+  if (!(Order == 101 || Order == 102)) return call.fail_enum("Order", Order);
+  if (!(TransA == 111 || TransA == 112 || TransA == 113)) return call.fail_enum("TransA", TransA);
+  if (M < 0) return call.fail_size("M", M);
+  if (N < 0) return call.fail_size("N", N);
+  if ((true) && A.strides[1] != 1) return call.fail_stride("A", A.strides[1]);
+  if ((true) && (Order == 101 ? M : N) > 1 && A.strides[0] < (Order == 101 ? N : M)) return call.fail_ld("A", A.strides[0]);
+  call.window(0, A.data, 0, exoblas::span2_hi((Order == 101 ? M : N), (Order == 101 ? N : M), A.strides[0]), sizeof(double), false, true);
+  call.window(1, x.data, exoblas::span_lo((TransA == 111 ? N : M), x.strides[0]), exoblas::span_hi((TransA == 111 ? N : M), x.strides[0]), sizeof(double), false, true);
+  call.window(2, y.data, exoblas::span_lo((TransA == 111 ? M : N), y.strides[0]), exoblas::span_hi((TransA == 111 ? M : N), y.strides[0]), sizeof(double), true, true);
+  if (!call.stage()) return call.status();
+  const double alpha_v = call.scalar(alpha);
+  const double beta_v = call.scalar(beta);
+  if (!call.ok()) return call.status();
+  call.run([&] { exoblas::ops::gemv_ex(call.ctx(), (int64_t)Order, (int64_t)TransA, (int64_t)M, (int64_t)N, alpha_v, call.dev<const double>(0), (int64_t)A.strides[0], call.dev<const double>(1), (int64_t)x.strides[0], beta_v, call.dev<double>(2), (int64_t)y.strides[0]); });
+  return call.finish();
+}
+extern "C" int exoblas_dgemv_ex_async(hipStream_t stream, int_fast32_t Order, int_fast32_t TransA, int_fast32_t M, int_fast32_t N, const double *alpha, struct exo_win_2f64c A, struct exo_win_1f64c x, const double *beta, struct exo_win_1f64 y) {
+  exoblas::Call call("exoblas_dgemv_ex_async", exoblas::Call::kAsync, stream);
+  if (!(Order == 101 || Order == 102)) return call.fail_enum("Order", Order);
+  if (!(TransA == 111 || TransA == 112 || TransA == 113)) return call.fail_enum("TransA", TransA);
+  if (M < 0) return call.fail_size("M", M);
+  if (N < 0) return call.fail_size("N", N);
+  if ((true) && A.strides[1] != 1) return call.fail_stride("A", A.strides[1]);
+  if ((true) && (Order == 101 ? M : N) > 1 && A.strides[0] < (Order == 101 ? N : M)) return call.fail_ld("A", A.strides[0]);
+  call.window(0, A.data, 0, exoblas::span2_hi((Order == 101 ? M : N), (Order == 101 ? N : M), A.strides[0]), sizeof(double), false, true);
+  call.window(1, x.data, exoblas::span_lo((TransA == 111 ? N : M), x.strides[0]), exoblas::span_hi((TransA == 111 ? N : M), x.strides[0]), sizeof(double), false, true);
+  call.window(2, y.data, exoblas::span_lo((TransA == 111 ? M : N), y.strides[0]), exoblas::span_hi((TransA == 111 ? M : N), y.strides[0]), sizeof(double), true, true);
+  if (!call.stage()) return call.status();
+  const double alpha_v = call.scalar(alpha);
+  const double beta_v = call.scalar(beta);
+  if (!call.ok()) return call.status();
+  call.run([&] { exoblas::ops::gemv_ex(call.ctx(), (int64_t)Order, (int64_t)TransA, (int64_t)M, (int64_t)N, alpha_v, call.dev<const double>(0), (int64_t)A.strides[0], call.dev<const double>(1), (int64_t)x.strides[0], beta_v, call.dev<double>(2), (int64_t)y.strides[0]); });
+  return call.finish();
+}
+
 // op(A) x = b with BF16 (update 16) or FP32 (update 32) off-diagonal updates, FP64 diagonal-block solves and `refine` steps of iterative refinement on an FP64 residual; arguments as exo_strsv_rm_stride_any (src/level2/trsv.py:10-52) plus update and refine
 extern "C" int exoblas_strsv_mixed(void *ctxt, int_fast32_t Uplo, int_fast32_t TransA, int_fast32_t Diag, int_fast32_t n, struct exo_win_1f32 x, struct exo_win_2f32c A, int_fast32_t update, int_fast32_t refine) {
   exoblas::Call call("exoblas_strsv_mixed", exoblas::Call::kSync, ctxt);

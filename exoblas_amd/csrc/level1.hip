@@ -3,7 +3,8 @@
 // All of these are HBM-bound.  The unit-stride path moves 16 bytes per lane
 // per access (global_load_dwordx4), unrolled so each lane keeps several loads
 // in flight; a wave64 touches 1 KiB contiguous per access.  Strided and
-// misaligned windows use a grid-stride scalar kernel.  Reductions are one
+// misaligned windows go element by element with 8 elements per lane in
+// flight.  Reductions are one
 // launch: per-workgroup partial sums (wave shuffle + LDS) published as tagged
 // words (tagged.hpp), summed in a fixed order by the workgroup that finishes
 // last, so results are deterministic.
@@ -62,16 +63,33 @@ __global__ __launch_bounds__(NT) void ew_vec_kernel(int64_t nvec, int64_t n, T* 
   }
 }
 
-template <class T, class Op>
-__global__ __launch_bounds__(kNT) void ew_strided_kernel(int64_t n, T* x, int64_t incx, T* y, int64_t incy,
-                                                         Op op) {
-  const int64_t step = (int64_t)gridDim.x * kNT;
-  for (int64_t i = (int64_t)blockIdx.x * kNT + threadIdx.x; i < n; i += step) {
-    T xs = Op::rx ? x[i * incx] : T(0);
-    T ys = Op::ry ? y[i * incy] : T(0);
-    op(xs, ys);
-    if (Op::wx) x[i * incx] = xs;
-    if (Op::wy) y[i * incy] = ys;
+// Strided or misaligned windows: element by element, U elements per lane in
+// flight (lane l of the workgroup takes elements l, l + NT, ... of its NT*U),
+// every load of a round issued before the first store, one round per
+// workgroup.  Consecutive lanes take consecutive elements, so a wave's access
+// spans 64 * |inc| elements: whole cache lines for |inc| = 1, and for larger
+// increments the lines the elements lie in (the bytes the hardware must move
+// anyway).
+template <class T, class Op, int NT, int U>
+__global__ __launch_bounds__(NT) void ew_strided_kernel(int64_t n, T* __restrict__ x, int64_t incx,
+                                                        T* __restrict__ y, int64_t incy, Op op) {
+  for (int64_t b = (int64_t)blockIdx.x * NT * U + threadIdx.x; b < n; b += (int64_t)gridDim.x * NT * U) {
+    T xs[U], ys[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int64_t i = b + u * NT;
+      xs[u] = (Op::rx && i < n) ? x[i * incx] : T(0);
+      ys[u] = (Op::ry && i < n) ? y[i * incy] : T(0);
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int64_t i = b + u * NT;
+      if (i < n) {
+        op(xs[u], ys[u]);
+        if (Op::wx) x[i * incx] = xs[u];
+        if (Op::wy) y[i * incy] = ys[u];
+      }
+    }
   }
 }
 
@@ -106,9 +124,10 @@ void launch_ew(Ctx& c, int64_t n, T* x, int64_t incx, T* y, int64_t incy, Op op)
     hipLaunchKernelGGL((ew_vec_kernel<T, Op, NT, U>), dim3((unsigned)blocks), dim3(NT), 0, c.stream, nvec, n, x, y,
                        op);
   } else {
-    const int64_t blocks = std::min<int64_t>(ceil_div(n, kNT), (int64_t)c.num_cus * 16);
-    hipLaunchKernelGGL((ew_strided_kernel<T, Op>), dim3((unsigned)blocks), dim3(kNT), 0, c.stream, n, x, incx,
-                       y, incy, op);
+    constexpr int U = 8;
+    const int64_t blocks = std::min<int64_t>(ceil_div(n, (int64_t)kNT * U), 0x7fffffff);
+    hipLaunchKernelGGL((ew_strided_kernel<T, Op, kNT, U>), dim3((unsigned)blocks), dim3(kNT), 0, c.stream, n, x,
+                       incx, y, incy, op);
   }
   EXO_LAUNCH_CHECK();
 }
@@ -267,9 +286,22 @@ __global__ __launch_bounds__(kNT) void reduce_kernel(int64_t n, const T* __restr
       if (t < n) acc[1 % U] += term(x[t], Term::ry ? y[t] : T(0));
     }
   } else {
-    const int64_t step = (int64_t)gridDim.x * kNT;
-    for (int64_t i = (int64_t)blockIdx.x * kNT + threadIdx.x; i < n; i += step)
-      acc[0] += term(x[i * incx], Term::ry ? y[i * incy] : T(0));
+    // strided or misaligned: element by element, 8 per lane in flight
+    // (consecutive lanes on consecutive elements), the grid sweeping rounds
+    constexpr int E = 8;
+    const int64_t span = (int64_t)kNT * E;
+    for (int64_t base = (int64_t)blockIdx.x * span + threadIdx.x; base < n; base += (int64_t)gridDim.x * span) {
+      T xs[E], ys[E];
+#pragma unroll
+      for (int e = 0; e < E; ++e) {
+        const int64_t i = base + e * kNT;
+        xs[e] = i < n ? x[i * incx] : T(0);
+        ys[e] = (Term::ry && i < n) ? y[i * incy] : T(0);
+      }
+#pragma unroll
+      for (int e = 0; e < E; ++e)
+        if (base + e * kNT < n) acc[e % U] += term(xs[e], ys[e]);
+    }
   }
   Acc s = acc[0];
 #pragma unroll
@@ -332,7 +364,10 @@ ReducePlan<T> plan_reduce(int64_t n, const T* x, int64_t incx, const T* y, int64
   p.y = y;
   p.incy = incy;
   p.vec = incx == 1 && aligned16(x) && (!Term::ry || (incy == 1 && aligned16(y)));
-  p.blocks = std::min<int64_t>(std::max<int64_t>(1, ceil_div(n, (int64_t)kNT * U * VN)), (int64_t)cus * 2);
+  if (p.vec)
+    p.blocks = std::min<int64_t>(std::max<int64_t>(1, ceil_div(n, (int64_t)kNT * U * VN)), (int64_t)cus * 2);
+  else  // element by element, 8 per lane: four workgroups per CU
+    p.blocks = std::min<int64_t>(std::max<int64_t>(1, ceil_div(n, (int64_t)kNT * 8)), (int64_t)cus * 4);
   return p;
 }
 
@@ -521,7 +556,8 @@ EXO_L1_INST(double)
 namespace {
 
 constexpr int kNoAllocHeader = 4;            // [0] ticket, [1] generation, [2] timeout word, [3] unused
-constexpr int kNoAllocMaxBlocks = 2048;      // partial slots: 2 workgroups per CU on up to 1024 CUs
+constexpr int kNoAllocMaxBlocks = 2048;      // partial slots: plan_reduce's largest grid (4 workgroups per
+                                             // CU, strided windows) on up to 512 CUs
 // tagged partials: one 64-bit word (two 32-bit units) per FP32 value
 constexpr size_t kNoAllocWords = kNoAllocHeader + 2 * kNoAllocMaxBlocks;  // 16,400 bytes
 

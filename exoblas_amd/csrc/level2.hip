@@ -33,7 +33,8 @@ enum Tri { kFull = 0, kLowerTri = 1, kUpperTri = 2 };
 
 // ------------------------------------------------------------ gemv-N
 
-// y[r] = beta*y[r] + alpha * sum_{c in row r's range} A[r,c] x[c]   (TRI == kFull)
+// y[r] = beta*y[r] + alpha * sum_{c in row r's range} A[r,c] x[c]   (TRI == kFull;
+// with ow set, y is not read: beta*0 in its place, for callers with beta == 0)
 // For TRI != kFull the row range is the triangle; UNIT skips the diagonal and
 // adds x[r]; the result overwrites y (trmv).
 // With gridDim.y > 1 each workgroup row covers `cols_per` columns and writes
@@ -42,7 +43,7 @@ template <class T, int R, int P, bool VEC, int TRI, bool UNIT>
 __global__ __launch_bounds__(kNT) void gemv_n_kernel(int64_t M, int64_t N, T alpha, T beta,
                                                      const T* __restrict__ A, int64_t lda,
                                                      const T* __restrict__ x, int64_t incx, T* y,
-                                                     int64_t incy, int64_t cols_per, T* part) {
+                                                     int64_t incy, int64_t cols_per, T* part, int ow) {
   using V = typename Vec16<T>::type;
   constexpr int VN = Vec16<T>::n;
   __shared__ __align__(16) T xs[P];
@@ -142,7 +143,7 @@ __global__ __launch_bounds__(kNT) void gemv_n_kernel(int64_t M, int64_t N, T alp
       } else if (TRI != kFull) {
         y[row * incy] = UNIT ? s + x[row * incx] : s;
       } else {
-        y[row * incy] = beta * y[row * incy] + alpha * s;
+        y[row * incy] = beta * (ow ? T(0) : y[row * incy]) + alpha * s;
       }
     }
   }
@@ -156,7 +157,7 @@ __global__ __launch_bounds__(kNT) void gemv_n_kernel(int64_t M, int64_t N, T alp
 template <class T, int TRI, bool UNIT>
 __global__ __launch_bounds__(kNT) void gemv_n_row_kernel(int64_t M, int64_t N, T alpha, T beta,
                                                          const T* __restrict__ A, int64_t lda,
-                                                         const T* __restrict__ x, T* y, int64_t incy) {
+                                                         const T* __restrict__ x, T* y, int64_t incy, int ow) {
   using V = typename Vec16<T>::type;
   constexpr int VN = Vec16<T>::n;
   constexpr int UV = 8;
@@ -201,7 +202,7 @@ __global__ __launch_bounds__(kNT) void gemv_n_row_kernel(int64_t M, int64_t N, T
     if (TRI != kFull)
       y[row * incy] = UNIT ? s + x[row] : s;
     else
-      y[row * incy] = beta * y[row * incy] + alpha * s;
+      y[row * incy] = beta * (ow ? T(0) : y[row * incy]) + alpha * s;
   }
 }
 
@@ -216,7 +217,7 @@ __global__ __launch_bounds__(kNT) void gemv_n_row_kernel(int64_t M, int64_t N, T
 template <class T, int UV, int R, int TRI = kFull, bool UNIT = false>
 __global__ __launch_bounds__(kNT) void gemv_n_wg_kernel(int64_t M, int64_t N, T alpha, T beta,
                                                         const T* __restrict__ A, int64_t lda,
-                                                        const T* __restrict__ x, T* y, int64_t incy) {
+                                                        const T* __restrict__ x, T* y, int64_t incy, int ow) {
   using V = typename Vec16<T>::type;
   constexpr int VN = Vec16<T>::n;
   constexpr int NW = kNT / kWave;
@@ -291,7 +292,7 @@ __global__ __launch_bounds__(kNT) void gemv_n_wg_kernel(int64_t M, int64_t N, T 
       if (TRI != kFull)
         y[(r0 + t) * incy] = UNIT ? s + x[r0 + t] : s;
       else
-        y[(r0 + t) * incy] = beta * y[(r0 + t) * incy] + alpha * s;
+        y[(r0 + t) * incy] = beta * (ow ? T(0) : y[(r0 + t) * incy]) + alpha * s;
     }
     __syncthreads();
   }
@@ -303,7 +304,7 @@ __global__ __launch_bounds__(kNT) void gemv_n_wg_kernel(int64_t M, int64_t N, T 
 template <class T, bool TRI, bool UNIT>
 __global__ __launch_bounds__(kNT) void gemv_finish_kernel(int64_t M, int splits, T alpha, T beta,
                                                           const T* part, const T* x, int64_t incx, T* y,
-                                                          int64_t incy) {
+                                                          int64_t incy, int ow) {
   constexpr int NW = kNT / kWave;
   __shared__ T red[NW][kWave];
   const int rl = threadIdx.x % kWave, g = threadIdx.x / kWave;
@@ -325,7 +326,7 @@ __global__ __launch_bounds__(kNT) void gemv_finish_kernel(int64_t M, int splits,
     if (TRI)
       y[r * incy] = UNIT ? t + x[r * incx] : t;
     else
-      y[r * incy] = beta * y[r * incy] + alpha * t;
+      y[r * incy] = beta * (ow ? T(0) : y[r * incy]) + alpha * t;
   }
 }
 
@@ -344,7 +345,8 @@ __global__ __launch_bounds__(NTT) void gemv_t_kernel(int64_t Mc, int64_t Nr, T a
                                                      const T* __restrict__ A, int64_t lda,
                                                      const T* __restrict__ x, int64_t incx, T* y,
                                                      int64_t incy, int64_t rows_per, unsigned* tickets,
-                                                     unsigned* gen, unsigned* lt, unsigned* err, unsigned* pub) {
+                                                     unsigned* gen, unsigned* lt, unsigned* err, unsigned* pub,
+                                                     int ow) {
   using V = typename Vec16<T>::type;
   constexpr int VN = Vec16<T>::n;
   constexpr int NW = NTT / kWave;
@@ -428,7 +430,7 @@ __global__ __launch_bounds__(NTT) void gemv_t_kernel(int64_t Mc, int64_t Nr, T a
     if (TRI != kFull)
       y[c * incy] = UNIT ? s + x[c * incx] : s;
     else
-      y[c * incy] = beta * y[c * incy] + alpha * s;
+      y[c * incy] = beta * (ow ? T(0) : y[c * incy]) + alpha * s;
   };
   for (int i = threadIdx.x; i < kWave * VN; i += NTT) {
     T s = T(0);
@@ -473,7 +475,8 @@ __global__ __launch_bounds__(NTT) void gemv_t_kernel(int64_t Mc, int64_t Nr, T a
 
 template <class T, int TRI, bool UNIT>
 void launch_gemv_n(Ctx& c, int64_t M, int64_t N, T alpha, T beta, const T* A, int64_t lda, const T* x,
-                   int64_t incx, T* y, int64_t incy) {
+                   int64_t incx, T* y, int64_t incy, bool overwrite = false) {
+  const int ow = overwrite ? 1 : 0;
   constexpr int VN = Vec16<T>::n;
   constexpr int P = 2048;               // x panel staged in LDS
   constexpr int64_t SG = 2048;          // column-split granularity
@@ -492,16 +495,16 @@ void launch_gemv_n(Ctx& c, int64_t M, int64_t N, T alpha, T beta, const T* A, in
     const unsigned grid = (unsigned)std::min<int64_t>(ceil_div(M, R), (int64_t)12 * c.num_cus);
     if (ceil_div(N, VN) <= (int64_t)kNT * 8)
       hipLaunchKernelGGL((gemv_n_wg_kernel<T, 8, R, TRI, UNIT>), dim3(grid), dim3(kNT), 0, c.stream, M, N, alpha,
-                         beta, A, lda, x, y, incy);
+                         beta, A, lda, x, y, incy, ow);
     else
       hipLaunchKernelGGL((gemv_n_wg_kernel<T, 16, R, TRI, UNIT>), dim3(grid), dim3(kNT), 0, c.stream, M, N, alpha,
-                         beta, A, lda, x, y, incy);
+                         beta, A, lda, x, y, incy, ow);
     EXO_LAUNCH_CHECK();
     return;
   }
   if (vec && incx == 1 && aligned16(x) && ceil_div(M, (int64_t)(kNT / kWave)) >= want) {
     hipLaunchKernelGGL((gemv_n_row_kernel<T, TRI, UNIT>), dim3((unsigned)ceil_div(M, kNT / kWave)), dim3(kNT), 0,
-                       c.stream, M, N, alpha, beta, A, lda, x, y, incy);
+                       c.stream, M, N, alpha, beta, A, lda, x, y, incy, ow);
     EXO_LAUNCH_CHECK();
     return;
   }
@@ -517,7 +520,7 @@ void launch_gemv_n(Ctx& c, int64_t M, int64_t N, T alpha, T beta, const T* A, in
   dim3 grid((unsigned)row_blocks, (unsigned)splits);
 #define EXO_GEMV_N(RR, V)                                                                                   \
   hipLaunchKernelGGL((gemv_n_kernel<T, RR, P, V, TRI, UNIT>), grid, dim3(kNT), 0, c.stream, M, N, alpha, beta, \
-                     A, lda, x, incx, y, incy, cols_per, part)
+                     A, lda, x, incx, y, incy, cols_per, part, ow)
   if (tall) {
     if (vec) EXO_GEMV_N(2, true); else EXO_GEMV_N(2, false);
   } else {
@@ -527,14 +530,15 @@ void launch_gemv_n(Ctx& c, int64_t M, int64_t N, T alpha, T beta, const T* A, in
   EXO_LAUNCH_CHECK();
   if (splits > 1) {
     hipLaunchKernelGGL((gemv_finish_kernel<T, TRI != kFull, UNIT>), dim3((unsigned)ceil_div(M, kWave)), dim3(kNT), 0,
-                       c.stream, M, (int)splits, alpha, beta, part, x, incx, y, incy);
+                       c.stream, M, (int)splits, alpha, beta, part, x, incx, y, incy, ow);
     EXO_LAUNCH_CHECK();
   }
 }
 
 template <class T, int TRI, bool UNIT>
 void launch_gemv_t(Ctx& c, int64_t Mc, int64_t Nr, T alpha, T beta, const T* A, int64_t lda, const T* x,
-                   int64_t incx, T* y, int64_t incy) {
+                   int64_t incx, T* y, int64_t incy, bool overwrite = false) {
+  const int ow = overwrite ? 1 : 0;
   constexpr int VN = Vec16<T>::n;
   constexpr int RU = 16;
   const bool vec = (lda % VN) == 0 && aligned16(A);
@@ -566,13 +570,13 @@ void launch_gemv_t(Ctx& c, int64_t Mc, int64_t Nr, T alpha, T beta, const T* A, 
   // FP64 keeps 4 waves of 16 rows (81 % against 75 %).
   if (sizeof(T) == 4 && vec && TRI == kFull)
     hipLaunchKernelGGL((gemv_t_kernel<T, 8, true, TRI, UNIT, 1024>), grid, dim3(1024), 0, c.stream, Mc, Nr, alpha,
-                       beta, A, lda, x, incx, y, incy, rows_per, tickets, gen, lt, err, pub);
+                       beta, A, lda, x, incx, y, incy, rows_per, tickets, gen, lt, err, pub, ow);
   else if (vec)
     hipLaunchKernelGGL((gemv_t_kernel<T, RU, true, TRI, UNIT>), grid, dim3(kNT), 0, c.stream, Mc, Nr, alpha, beta,
-                       A, lda, x, incx, y, incy, rows_per, tickets, gen, lt, err, pub);
+                       A, lda, x, incx, y, incy, rows_per, tickets, gen, lt, err, pub, ow);
   else
     hipLaunchKernelGGL((gemv_t_kernel<T, RU, false, TRI, UNIT>), grid, dim3(kNT), 0, c.stream, Mc, Nr, alpha,
-                       beta, A, lda, x, incx, y, incy, rows_per, tickets, gen, lt, err, pub);
+                       beta, A, lda, x, incx, y, incy, rows_per, tickets, gen, lt, err, pub, ow);
   EXO_LAUNCH_CHECK();
 }
 
@@ -602,6 +606,26 @@ void gemv_rm(Ctx& c, int64_t TransA, int64_t M, int64_t N, T alpha, T beta, cons
     launch_gemv_n<T, kFull, false>(c, M, N, alpha, beta, A, lda, x, incx, y, incy);
   else
     launch_gemv_t<T, kFull, false>(c, M, N, alpha, beta, AT, ldat, x, incx, y, incy);
+}
+
+// CBLAS-shaped gemv: y = alpha op(A) x + beta y for an M x N matrix A in
+// row-major (Order 101) or column-major (102) storage, y not read when beta
+// == 0.  A column-major A is the row-major N x M array S = A^T, so A x is the
+// transposed product of S and A^T x the plain one.
+template <class T>
+void gemv_ex_impl(Ctx& c, int64_t Order, int64_t TransA, int64_t M, int64_t N, T alpha, const T* A, int64_t lda,
+                  const T* x, int64_t incx, T beta, T* y, int64_t incy) {
+  const bool trans = TransA != kNoTrans, row = Order == 101;
+  const int64_t ny = trans ? N : M;
+  if (ny <= 0) return;
+  const bool ow = beta == T(0);
+  // y runs over the stored rows (row-major NoTrans, column-major Trans) or
+  // over the stored columns (the other two)
+  const int64_t srows = row ? M : N, scols = row ? N : M;
+  if (row == trans)
+    launch_gemv_t<T, kFull, false>(c, scols, srows, alpha, beta, A, lda, x, incx, y, incy, ow);
+  else
+    launch_gemv_n<T, kFull, false>(c, srows, scols, alpha, beta, A, lda, x, incx, y, incy, ow);
 }
 
 // src/level2/ger.py:9-15: A[i,j] += (alpha*x[i]) * y[j]
@@ -713,6 +737,15 @@ void trmv_rm(Ctx& c, int64_t Uplo, int64_t TransA, int64_t Diag, int64_t n, T* x
 
 EXO_L2_INST(float)
 EXO_L2_INST(double)
+
+void gemv_ex(Ctx& c, int64_t Order, int64_t TransA, int64_t M, int64_t N, float alpha, const float* A, int64_t lda,
+             const float* x, int64_t incx, float beta, float* y, int64_t incy) {
+  gemv_ex_impl<float>(c, Order, TransA, M, N, alpha, A, lda, x, incx, beta, y, incy);
+}
+void gemv_ex(Ctx& c, int64_t Order, int64_t TransA, int64_t M, int64_t N, double alpha, const double* A,
+             int64_t lda, const double* x, int64_t incx, double beta, double* y, int64_t incy) {
+  gemv_ex_impl<double>(c, Order, TransA, M, N, alpha, A, lda, x, incx, beta, y, incy);
+}
 
 }  // namespace ops
 }  // namespace exoblas

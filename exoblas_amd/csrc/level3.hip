@@ -3,10 +3,12 @@
 //
 // Every product is  C[i, j] += alpha * sum_k X_A(i, k) X_B(j, k)  with the
 // operand forms chosen on the host from TransA/TransB, Side/Uplo or
-// Uplo/Trans; syrk restricts the update to one triangle of C.  beta is not
-// applied here: the reference's wrappers scale C first with a separate
-// mscal / trmscal call (test/level3/gemm/exo_gemm_wrapper.h:18-20,
+// Uplo/Trans; syrk restricts the update to one triangle of C.  The exo_*
+// entry points apply no beta: the reference's wrappers scale C first with a
+// separate mscal / trmscal call (test/level3/gemm/exo_gemm_wrapper.h:18-20,
 // test/level3/syrk/exo_syrk_wrapper.h:18-20), and the same split is kept.
+// The CBLAS-shaped gemm_ex extension fuses beta into the epilogue instead
+// (BETA kernels: C = alpha * sum + beta * C, C not read when beta == 0).
 #include "device.hpp"
 #include "gemm_core.hpp"
 #include "level3.hpp"
@@ -18,11 +20,11 @@ namespace gemm {
 
 // Tiles [tile0, tile0 + gridDim.x) of the tile list (list_tile), one per
 // workgroup.
-template <class C, int TRI, int EXT = 0>
+template <class C, int TRI, int EXT = 0, bool BETA = false>
 __global__ __launch_bounds__(C::NT) void gemm_kernel(int64_t M, int64_t N, int64_t K, typename C::type alpha,
                                                      Operand<typename C::type> A, Operand<typename C::type> B,
                                                      typename C::type* Cp, int64_t ldc, int cvec, int64_t tiles_m,
-                                                     int64_t tiles_n, int64_t tile0) {
+                                                     int64_t tiles_n, int64_t tile0, typename C::type beta) {
   using T = typename C::type;
   extern __shared__ __align__(16) unsigned char smem[];
   int64_t tm, tn;
@@ -39,7 +41,7 @@ __global__ __launch_bounds__(C::NT) void gemm_kernel(int64_t M, int64_t N, int64
   int64_t kb = 0, ke = K;
   if constexpr (EXT == 1 || EXT == 3 || EXT == 4) tri_k_window<EXT>(A, m0, C::BM, kb, ke);
   tile_mma<C, EXT>(A, B, M, N, K, m0, n0, kb, ke, acc, reinterpret_cast<T*>(smem));
-  tile_store<C, TRI>(Cp, ldc, cvec, M, N, m0, n0, alpha, acc);
+  tile_store<C, TRI, BETA>(Cp, ldc, cvec, M, N, m0, n0, alpha, acc, beta);
 }
 
 // Split-K over tiles [tile0, tile0 + ntiles) of the list: workgroup
@@ -76,10 +78,10 @@ __global__ __launch_bounds__(C::NT) void gemm_splitk_kernel(int64_t M, int64_t N
 // Workgroup (l, r) adds rows [r * 256 / (BN / VN), ...) of tile tile0 + l:
 // each thread one 16-byte column group of one row, summed over the S slices
 // in order.
-template <class T, int TRI, int BM, int BN>
+template <class T, int TRI, int BM, int BN, bool BETA = false>
 __global__ __launch_bounds__(256) void splitk_reduce_kernel(int64_t M, int64_t N, int S, int64_t tiles_m,
                                                             int64_t tiles_n, int64_t tile0, int64_t ntiles, T alpha,
-                                                            const T* __restrict__ P, T* Cp, int64_t ldc) {
+                                                            const T* __restrict__ P, T* Cp, int64_t ldc, T beta) {
   using V = typename Vec16<T>::type;
   constexpr int VN = Vec16<T>::n, GPR = BN / VN, RPW = 256 / GPR;  // groups per row, rows per workgroup
   const int64_t l = blockIdx.x;
@@ -108,15 +110,18 @@ __global__ __launch_bounds__(256) void splitk_reduce_kernel(int64_t M, int64_t N
   for (int e = 0; e < VN; ++e) {
     const int64_t col = col0 + e;
     if (col >= N || (TRI == kLower && col > row) || (TRI == kUpper && col < row)) continue;
-    Cp[row * ldc + col] += alpha * sum[e];
+    if (BETA)
+      Cp[row * ldc + col] = alpha * sum[e] + (beta != T(0) ? beta * Cp[row * ldc + col] : T(0));
+    else
+      Cp[row * ldc + col] += alpha * sum[e];
   }
 }
 
 // Split-K over tiles [tile0, tile0 + ntiles) of the list, S slices.
-template <class C, int TRI, int EXT = 0>
+template <class C, int TRI, int EXT = 0, bool BETA = false>
 void launch_splitk(Ctx& c, int64_t S, int64_t tile0, int64_t ntiles, int64_t M, int64_t N, int64_t K,
                    typename C::type alpha, const Operand<typename C::type>& A, const Operand<typename C::type>& B,
-                   typename C::type* Cp, int64_t ldc) {
+                   typename C::type* Cp, int64_t ldc, typename C::type beta = 0) {
   using T = typename C::type;
   const int64_t tm = ceil_div(M, C::BM), tn = ceil_div(N, C::BN);
   const int64_t ksplit = ceil_div(ceil_div(K, S), C::BK) * C::BK;
@@ -128,25 +133,232 @@ void launch_splitk(Ctx& c, int64_t S, int64_t tile0, int64_t ntiles, int64_t M, 
                      c.stream, M, N, K, ksplit, A, B, P, tm, tn, tile0, ntiles);
   EXO_LAUNCH_CHECK();
   constexpr int RPW = 256 / (C::BN / Vec16<T>::n);
-  hipLaunchKernelGGL((splitk_reduce_kernel<T, TRI, C::BM, C::BN>), dim3((unsigned)ntiles, (unsigned)ceil_div(C::BM, RPW)),
-                     dim3(256), 0, c.stream, M, N, (int)S, tm, tn, tile0, ntiles, alpha, P, Cp, ldc);
+  hipLaunchKernelGGL((splitk_reduce_kernel<T, TRI, C::BM, C::BN, BETA>),
+                     dim3((unsigned)ntiles, (unsigned)ceil_div(C::BM, RPW)), dim3(256), 0, c.stream, M, N, (int)S, tm,
+                     tn, tile0, ntiles, alpha, P, Cp, ldc, beta);
   EXO_LAUNCH_CHECK();
 }
 
 // Tiles [tile0, tile0 + ntiles) of the list, one workgroup each.
-template <class C, int TRI, int EXT = 0>
+template <class C, int TRI, int EXT = 0, bool BETA = false>
 void launch_cfg(Ctx& c, int64_t tile0, int64_t ntiles, int64_t M, int64_t N, int64_t K, typename C::type alpha,
                 const Operand<typename C::type>& A, const Operand<typename C::type>& B, typename C::type* Cp,
-                int64_t ldc) {
+                int64_t ldc, typename C::type beta = 0) {
   using T = typename C::type;
   const int64_t tm = ceil_div(M, C::BM), tn = ceil_div(N, C::BN);
   const int cvec = (ldc % C::EB == 0) && (reinterpret_cast<uintptr_t>(Cp) % (C::EB * sizeof(T)) == 0);
   static std::atomic<uint64_t> lds_set{0};
-  ensure_lds_limit(reinterpret_cast<const void*>(&gemm_kernel<C, TRI, EXT>), C::LDS_BYTES, c.device, lds_set);
-  hipLaunchKernelGGL((gemm_kernel<C, TRI, EXT>), dim3((unsigned)ntiles), dim3(C::NT), C::LDS_BYTES, c.stream, M, N, K,
-                     alpha, A, B, Cp, ldc, cvec, tm, tn, tile0);
+  ensure_lds_limit(reinterpret_cast<const void*>(&gemm_kernel<C, TRI, EXT, BETA>), C::LDS_BYTES, c.device, lds_set);
+  hipLaunchKernelGGL((gemm_kernel<C, TRI, EXT, BETA>), dim3((unsigned)ntiles), dim3(C::NT), C::LDS_BYTES, c.stream, M,
+                     N, K, alpha, A, B, Cp, ldc, cvec, tm, tn, tile0, beta);
   EXO_LAUNCH_CHECK();
 }
+
+// Narrow products: few output tiles and a K long enough to share.  32 x 32
+// output tiles, one per workgroup of 8 waves; wave w accumulates the whole
+// tile over its K slice [w * ks, (w + 1) * ks) straight from global memory
+// (each operand element is read by one wave, so no LDS staging), and the 8
+// partial tiles are summed through LDS in wave order, so results do not
+// depend on timing.  An MFMA step's 4-deep reduction takes k = k0 + 4 * lg +
+// e for lane group lg at step e of a 16-deep k group: a row-major operand
+// row gives a lane its 4 values of a group in one 16-byte load.  One launch
+// replaces the split-K pair (partial products + reduction) for these shapes
+// and carries the beta epilogue.
+namespace narrow {
+constexpr int TM = 32, TN = 32, NW = 8, NT = 64 * NW, KG = 16, CG = 2;  // CG k groups per pipeline stage
+constexpr int RS = TN + 1;                                             // LDS row stride of a partial tile
+
+template <class T>
+constexpr size_t lds_bytes() {
+  return sizeof(T) * NW * TM * RS;
+}
+
+// X(r, k + e), e = 0..3, of a vector operand in form F (whole group in
+// range): one 16-byte load of a row (kRK) or 4 rows of one column (kKR).
+template <int F, class T>
+__device__ __forceinline__ void load4(const Operand<T>& op, int64_t r, int64_t k, T (&v)[4]) {
+  if constexpr (F == kRK) {
+    const T* p = op.p + r * op.ld + k;
+    if constexpr (sizeof(T) == 4) {
+      const VecN<T, 4> x = *reinterpret_cast<const VecN<T, 4>*>(p);
+#pragma unroll
+      for (int e = 0; e < 4; ++e) v[e] = x.v[e];
+    } else {
+      const VecN<T, 2> x0 = *reinterpret_cast<const VecN<T, 2>*>(p);
+      const VecN<T, 2> x1 = *reinterpret_cast<const VecN<T, 2>*>(p + 2);
+      v[0] = x0.v[0];
+      v[1] = x0.v[1];
+      v[2] = x1.v[0];
+      v[3] = x1.v[1];
+    }
+  } else {
+    const T* p = op.p + k * op.ld + r;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) v[e] = p[e * op.ld];
+  }
+}
+
+// The same with every element checked against rows x [.., ke).
+template <int F, class T>
+__device__ __forceinline__ void load4_masked(const Operand<T>& op, int64_t r, int64_t k, int64_t rows, int64_t ke,
+                                             T (&v)[4]) {
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    const int64_t kk = k + e;
+    v[e] = (r < rows && kk < ke) ? (F == kRK ? op.p[r * op.ld + kk] : op.p[kk * op.ld + r]) : T(0);
+  }
+}
+
+// FA / FB: the operands' forms (kRK or kKR), fixed at compile time so that
+// the loads carry no branches (a runtime form test around each load costs
+// the 512^3 FP32 product 11.2 -> 6.3 us on MI355X).
+template <class T, int FA, int FB, bool BETA>
+__global__ __launch_bounds__(NT) void narrow_kernel(int64_t M, int64_t N, int64_t K, int64_t ks, T alpha, T beta,
+                                                    Operand<T> A, Operand<T> B, T* Cp, int64_t ldc, int64_t tiles_n) {
+  using MF = Mfma<T>;
+  extern __shared__ __align__(16) unsigned char smem[];
+  T* red = reinterpret_cast<T*>(smem);
+  const int64_t t = xcd_regroup(blockIdx.x, gridDim.x);
+  const int64_t m0 = (t / tiles_n) * TM, n0 = (t % tiles_n) * TN;
+  const int wave = threadIdx.x / 64, lane = threadIdx.x % 64, lm = lane % 16, lg = lane / 16;
+  const int64_t kb = wave * ks, ke = kb + ks < K ? kb + ks : K;
+  typename MF::acc acc[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) acc[a][b][q] = T(0);
+  // one stage: CG k groups of both row groups of A and B
+  struct Stage {
+    T a[CG][2][4], b[CG][2][4];
+  };
+  auto mma = [&](const Stage& st) {
+#pragma unroll
+    for (int g = 0; g < CG; ++g)
+#pragma unroll
+      for (int e = 0; e < 4; ++e)
+#pragma unroll
+        for (int ta = 0; ta < 2; ++ta)
+#pragma unroll
+          for (int tb = 0; tb < 2; ++tb) acc[ta][tb] = MF::step(st.a[g][ta][e], st.b[g][tb][e], acc[ta][tb]);
+  };
+  // C is read before the product so its latency overlaps the K loop
+  constexpr int PT = TM * TN / NT;  // outputs per thread in the epilogue
+  T cin[PT];
+#pragma unroll
+  for (int j = 0; j < PT; ++j) {
+    const int idx = threadIdx.x + NT * j;
+    const int64_t gr = m0 + idx / TN, gc = n0 + idx % TN;
+    cin[j] = (gr < M && gc < N && (!BETA || beta != T(0))) ? Cp[gr * ldc + gc] : T(0);
+  }
+  const bool fast = A.vec && B.vec && m0 + TM <= M && n0 + TN <= N;
+  int64_t k = kb;
+  if (fast && ke > kb) {
+    // whole stages of CG groups, the next stage's loads issued before this
+    // stage's MFMAs
+    const int64_t nst = (ke - kb) / (CG * KG);
+    auto load = [&](Stage& st, int64_t k0) {
+#pragma unroll
+      for (int g = 0; g < CG; ++g)
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          load4<FA>(A, m0 + 16 * h + lm, k0 + g * KG + 4 * lg, st.a[g][h]);
+          load4<FB>(B, n0 + 16 * h + lm, k0 + g * KG + 4 * lg, st.b[g][h]);
+        }
+    };
+    Stage s0, s1;
+    int64_t i = 0;
+    if (nst > 0) load(s0, k);
+    for (; i + 2 <= nst; i += 2) {
+      load(s1, k + CG * KG);
+      mma(s0);
+      if (i + 2 < nst) load(s0, k + 2 * CG * KG);
+      mma(s1);
+      k += 2 * CG * KG;
+    }
+    if (i < nst) {
+      mma(s0);
+      k += CG * KG;
+    }
+  }
+  // the rest of the slice (and every slice of an edge tile) element-checked
+  for (; k < ke; k += KG) {
+    T a[2][4], b[2][4];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      load4_masked<FA>(A, m0 + 16 * h + lm, k + 4 * lg, M, ke, a[h]);
+      load4_masked<FB>(B, n0 + 16 * h + lm, k + 4 * lg, N, ke, b[h]);
+    }
+#pragma unroll
+    for (int e = 0; e < 4; ++e)
+#pragma unroll
+      for (int ta = 0; ta < 2; ++ta)
+#pragma unroll
+        for (int tb = 0; tb < 2; ++tb) acc[ta][tb] = MF::step(a[ta][e], b[tb][e], acc[ta][tb]);
+  }
+  // partial tiles to LDS, then each thread sums its outputs over the waves in order
+  T* mine = red + wave * TM * RS;
+#pragma unroll
+  for (int ta = 0; ta < 2; ++ta)
+#pragma unroll
+    for (int tb = 0; tb < 2; ++tb)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) mine[(16 * ta + MF::out_row(lg, q)) * RS + 16 * tb + lm] = acc[ta][tb][q];
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < PT; ++j) {
+    const int idx = threadIdx.x + NT * j, row = idx / TN, col = idx % TN;
+    T s = T(0);
+#pragma unroll
+    for (int w = 0; w < NW; ++w) s += red[w * TM * RS + row * RS + col];
+    const int64_t gr = m0 + row, gc = n0 + col;
+    if (gr < M && gc < N) {
+      T* cp = Cp + gr * ldc + gc;
+      if (BETA)
+        *cp = alpha * s + (beta != T(0) ? beta * cin[j] : T(0));
+      else
+        *cp = cin[j] + alpha * s;
+    }
+  }
+}
+
+inline int64_t tiles(int64_t M, int64_t N) { return ceil_div(M, TM) * ceil_div(N, TN); }
+
+template <class T, int FA, int FB, bool BETA>
+void launch_forms(Ctx& c, int64_t M, int64_t N, int64_t K, T alpha, const Operand<T>& A, const Operand<T>& B,
+                  T* Cp, int64_t ldc, T beta) {
+  const int64_t tn = ceil_div(N, TN), nt = tiles(M, N);
+  const int64_t ks = ceil_div(ceil_div(K, NW), KG) * KG;
+  static std::atomic<uint64_t> lds_set{0};
+  ensure_lds_limit(reinterpret_cast<const void*>(&narrow_kernel<T, FA, FB, BETA>), lds_bytes<T>(), c.device, lds_set);
+  hipLaunchKernelGGL((narrow_kernel<T, FA, FB, BETA>), dim3((unsigned)nt), dim3(NT), lds_bytes<T>(), c.stream, M, N,
+                     K, ks, alpha, beta, A, B, Cp, ldc, tn);
+  EXO_LAUNCH_CHECK();
+}
+
+template <class T, bool BETA>
+void launch(Ctx& c, int64_t M, int64_t N, int64_t K, T alpha, const Operand<T>& A, const Operand<T>& B, T* Cp,
+            int64_t ldc, T beta) {
+  if (A.form == kRK && B.form == kRK)
+    launch_forms<T, kRK, kRK, BETA>(c, M, N, K, alpha, A, B, Cp, ldc, beta);
+  else if (A.form == kRK)
+    launch_forms<T, kRK, kKR, BETA>(c, M, N, K, alpha, A, B, Cp, ldc, beta);
+  else if (B.form == kRK)
+    launch_forms<T, kKR, kRK, BETA>(c, M, N, K, alpha, A, B, Cp, ldc, beta);
+  else
+    launch_forms<T, kKR, kKR, BETA>(c, M, N, K, alpha, A, B, Cp, ldc, beta);
+}
+
+// Plain products (kRK / kKR operands: not symmetric, triangular or
+// concatenated; the caller checks the forms) whose 32 x 32 tiles give at
+// least half a round and at most two, with a K that gives every wave a stage
+// or more.
+inline bool fits(int64_t M, int64_t N, int64_t K, int64_t P) {
+  const int64_t nt = tiles(M, N);
+  return nt >= P / 2 && nt <= 2 * P && K >= NW * CG * KG;
+}
+}  // namespace narrow
 
 // Tile shapes, largest first (measured on 8192^3 NN: FP32 256 x 256 x 32 in
 // 8 waves of 128 x 64 127.6 TFLOP/s, 4 waves of 128 x 128 123.8, 128 x 128 x
@@ -180,13 +392,13 @@ int64_t tile_count(int tri, int64_t M, int64_t N) {
 
 // One configuration's launches: tiles [0, dp) one workgroup each, the rest
 // split S ways along K (S = 1: dp is every tile).
-template <class C, int TRI, int EXT = 0>
+template <class C, int TRI, int EXT = 0, bool BETA = false>
 void launch_plan(Ctx& c, int64_t dp, int64_t S, int64_t M, int64_t N, int64_t K, typename C::type alpha,
                  const Operand<typename C::type>& A, const Operand<typename C::type>& B, typename C::type* Cp,
-                 int64_t ldc) {
+                 int64_t ldc, typename C::type beta = 0) {
   const int64_t total = tile_count<C>(TRI, M, N);
-  if (dp > 0) launch_cfg<C, TRI, EXT>(c, 0, dp, M, N, K, alpha, A, B, Cp, ldc);
-  if (dp < total) launch_splitk<C, TRI, EXT>(c, S, dp, total - dp, M, N, K, alpha, A, B, Cp, ldc);
+  if (dp > 0) launch_cfg<C, TRI, EXT, BETA>(c, 0, dp, M, N, K, alpha, A, B, Cp, ldc, beta);
+  if (dp < total) launch_splitk<C, TRI, EXT, BETA>(c, S, dp, total - dp, M, N, K, alpha, A, B, Cp, ldc, beta);
 }
 
 // Split factor in [2, smax] that fills the last round of S * tiles units
@@ -209,17 +421,17 @@ inline int64_t best_split(int64_t tiles, int64_t P, int64_t smax, double need) {
 // Tiles [0, dp) of C's tile list run whole, the remaining ones split along K
 // into the stream's free CUs: the full rounds keep every CU busy and the
 // last, partial round is spread over all of them.
-template <class C, int TRI, int EXT = 0>
+template <class C, int TRI, int EXT = 0, bool BETA = false>
 void launch_rounds(Ctx& c, int64_t M, int64_t N, int64_t K, typename C::type alpha,
                    const Operand<typename C::type>& A, const Operand<typename C::type>& B, typename C::type* Cp,
-                   int64_t ldc) {
+                   int64_t ldc, typename C::type beta = 0) {
   const int64_t total = tile_count<C>(TRI, M, N), P = c.num_cus;
   const int64_t dp = total / P * P, rem = total - dp;
   const int64_t S = rem ? std::min<int64_t>({P / rem, K / (4 * C::BK), 16}) : 1;
   if (S < 2)
-    launch_plan<C, TRI, EXT>(c, total, 1, M, N, K, alpha, A, B, Cp, ldc);
+    launch_plan<C, TRI, EXT, BETA>(c, total, 1, M, N, K, alpha, A, B, Cp, ldc, beta);
   else
-    launch_plan<C, TRI, EXT>(c, dp, S, M, N, K, alpha, A, B, Cp, ldc);
+    launch_plan<C, TRI, EXT, BETA>(c, dp, S, M, N, K, alpha, A, B, Cp, ldc, beta);
 }
 
 // Plan choice, from measurements on MI355X (FP32 TFLOP/s, old rule -> this):
@@ -232,9 +444,9 @@ void launch_rounds(Ctx& c, int64_t M, int64_t N, int64_t K, typename C::type alp
 // that fills the rounds (sgemm 2048^3 81.2 -> 100.2), else 128 x 128 tiles,
 // else 64 x 64 tiles split along K for small outputs with a long K (sgemm
 // 512^3, 1024^3).
-template <class T, int TRI, int EXT = 0>
+template <class T, int TRI, int EXT = 0, bool BETA = false>
 void run(Ctx& c, int64_t M, int64_t N, int64_t K, T alpha, const Operand<T>& A, const Operand<T>& B, T* Cp,
-         int64_t ldc) {
+         int64_t ldc, T beta = 0) {
   if (M <= 0 || N <= 0 || K <= 0) return;
   using Tl = Tiles<T>;
   using Huge = typename Tl::Huge;
@@ -242,17 +454,20 @@ void run(Ctx& c, int64_t M, int64_t N, int64_t K, T alpha, const Operand<T>& A, 
   using Small = typename Tl::Small;
   const int64_t P = c.num_cus;
   const int64_t huge = tile_count<Huge>(TRI, M, N), big = tile_count<Big>(TRI, M, N);
-  if (huge >= P) return launch_rounds<Huge, TRI, EXT>(c, M, N, K, alpha, A, B, Cp, ldc);
-  if (big >= 2 * P) return launch_rounds<Big, TRI, EXT>(c, M, N, K, alpha, A, B, Cp, ldc);
+  if (huge >= P) return launch_rounds<Huge, TRI, EXT, BETA>(c, M, N, K, alpha, A, B, Cp, ldc, beta);
+  if (big >= 2 * P) return launch_rounds<Big, TRI, EXT, BETA>(c, M, N, K, alpha, A, B, Cp, ldc, beta);
   const int64_t sh = best_split(huge, P, std::min<int64_t>(16, K / (4 * Huge::BK)), 0.75);
-  if (sh >= 2) return launch_plan<Huge, TRI, EXT>(c, 0, sh, M, N, K, alpha, A, B, Cp, ldc);
-  if (big >= P) return launch_rounds<Big, TRI, EXT>(c, M, N, K, alpha, A, B, Cp, ldc);
+  if (sh >= 2) return launch_plan<Huge, TRI, EXT, BETA>(c, 0, sh, M, N, K, alpha, A, B, Cp, ldc, beta);
+  if (big >= P) return launch_rounds<Big, TRI, EXT, BETA>(c, M, N, K, alpha, A, B, Cp, ldc, beta);
+  const int64_t small = tile_count<Small>(TRI, M, N);
+  if constexpr (TRI == kFull && EXT == 0)
+    if (A.form <= kKR && B.form <= kKR && small < P && narrow::fits(M, N, K, P))
+      return narrow::launch<T, BETA>(c, M, N, K, alpha, A, B, Cp, ldc, beta);
   // Few output tiles and a long K: split K so every CU gets two workgroups,
   // each K slice at least 128 deep.
-  const int64_t small = tile_count<Small>(TRI, M, N);
   const int64_t S = std::min<int64_t>({ceil_div(2 * P, small), K / 128, 16});
-  if (S >= 2) return launch_plan<Small, TRI, EXT>(c, 0, S, M, N, K, alpha, A, B, Cp, ldc);
-  launch_rounds<Small, TRI, EXT>(c, M, N, K, alpha, A, B, Cp, ldc);
+  if (S >= 2) return launch_plan<Small, TRI, EXT, BETA>(c, 0, S, M, N, K, alpha, A, B, Cp, ldc, beta);
+  launch_rounds<Small, TRI, EXT, BETA>(c, M, N, K, alpha, A, B, Cp, ldc, beta);
 }
 
 // Triangular-operand products (trmm).  With at least half a round of large
@@ -320,6 +535,7 @@ constexpr int kNT = 256;
 constexpr int64_t kNoTrans = 111;
 constexpr int64_t kUpperV = 121;
 constexpr int64_t kLeftV = 141;
+constexpr int64_t kRowMajor = 101;
 
 // Bc[i, 0:n) = B[i, 0:n) and B[i, 0:n) = 0 for row i = blockIdx.y: trmm
 // reads its right-hand operand from the copy and accumulates into B.
@@ -359,6 +575,46 @@ void gemm(Ctx& c, int64_t TransA, int64_t TransB, int64_t M, int64_t N, int64_t 
   const Operand<T> a = TransA == kNoTrans ? operand(A, lda, kRK) : operand(AT, ldat, kKR);
   const Operand<T> b = TransB == kNoTrans ? operand(B, ldb, kKR) : operand(BT, ldbt, kRK);
   product<T>(c, kFull, M, N, K, alpha, a, b, Cp, ldc);
+}
+
+// CBLAS-shaped gemm: C = alpha op(A) op(B) + beta C, C not read when beta ==
+// 0, beta fused into the product's epilogue.  Order 102 (column-major) is the
+// row-major product C^T = op(B)^T op(A)^T on the same arrays: output rows are
+// C's columns, X_A reads op(B) and X_B reads op(A), each in the operand form
+// its storage gives.  alpha == 0 or K == 0 leaves only beta C (zeros when
+// beta == 0).
+template <class T>
+void gemm_ex_impl(Ctx& c, int64_t Order, int64_t TransA, int64_t TransB, int64_t M, int64_t N, int64_t K, T alpha,
+                  const T* A, int64_t lda, const T* B, int64_t ldb, T beta, T* Cp, int64_t ldc) {
+  using namespace gemm;
+  if (M <= 0 || N <= 0) return;
+  const bool row = Order == kRowMajor, ta = TransA != kNoTrans, tb = TransB != kNoTrans;
+  const int64_t rows = row ? M : N, cols = row ? N : M;
+  if (alpha == T(0) || K <= 0) {
+    if (beta == T(1)) return;
+    tiles::UpdateArgs<T> g{};
+    g.alpha = beta;
+    if (beta == T(0))
+      tiles::launch_update<T, 3, tiles::kFull>(c.stream, rows, cols, Cp, ldc, g);
+    else
+      tiles::launch_update<T, 0, tiles::kFull>(c.stream, rows, cols, Cp, ldc, g);
+    return;
+  }
+  if (row)
+    run<T, kFull, 0, true>(c, M, N, K, alpha, operand(A, lda, ta ? kKR : kRK), operand(B, ldb, tb ? kRK : kKR), Cp,
+                           ldc, beta);
+  else
+    run<T, kFull, 0, true>(c, N, M, K, alpha, operand(B, ldb, tb ? kKR : kRK), operand(A, lda, ta ? kRK : kKR), Cp,
+                           ldc, beta);
+}
+
+void gemm_ex(Ctx& c, int64_t Order, int64_t TransA, int64_t TransB, int64_t M, int64_t N, int64_t K, float alpha,
+             const float* A, int64_t lda, const float* B, int64_t ldb, float beta, float* Cp, int64_t ldc) {
+  gemm_ex_impl<float>(c, Order, TransA, TransB, M, N, K, alpha, A, lda, B, ldb, beta, Cp, ldc);
+}
+void gemm_ex(Ctx& c, int64_t Order, int64_t TransA, int64_t TransB, int64_t M, int64_t N, int64_t K, double alpha,
+             const double* A, int64_t lda, const double* B, int64_t ldb, double beta, double* Cp, int64_t ldc) {
+  gemm_ex_impl<double>(c, Order, TransA, TransB, M, N, K, alpha, A, lda, B, ldb, beta, Cp, ldc);
 }
 
 // src/level3/symm.py:16-47.  Left: C += alpha S B (S = Aleft, M x M);

@@ -107,6 +107,18 @@ void trmm_row_major_Left_Upper_Trans(Ctx&, int64_t m, int64_t n, T alpha, const 
 template <class T>
 void trmm_row_major_Left_Lower_Trans(Ctx&, int64_t m, int64_t n, T alpha, const T* A, int64_t lda, T* B,
                                      int64_t ldb, int64_t Diag);
+// ---------------------------------------------------------------- CBLAS-shaped forms
+// Order 101 (row-major) or 102 (column-major) operands, CBLAS dimensions and
+// beta: C = alpha op(A) op(B) + beta C and y = alpha op(A) x + beta y, with C
+// (y) not read when beta == 0.
+void gemm_ex(Ctx&, int64_t Order, int64_t TransA, int64_t TransB, int64_t M, int64_t N, int64_t K, float alpha,
+             const float* A, int64_t lda, const float* B, int64_t ldb, float beta, float* C, int64_t ldc);
+void gemm_ex(Ctx&, int64_t Order, int64_t TransA, int64_t TransB, int64_t M, int64_t N, int64_t K, double alpha,
+             const double* A, int64_t lda, const double* B, int64_t ldb, double beta, double* C, int64_t ldc);
+void gemv_ex(Ctx&, int64_t Order, int64_t TransA, int64_t M, int64_t N, float alpha, const float* A, int64_t lda,
+             const float* x, int64_t incx, float beta, float* y, int64_t incy);
+void gemv_ex(Ctx&, int64_t Order, int64_t TransA, int64_t M, int64_t N, double alpha, const double* A,
+             int64_t lda, const double* x, int64_t incx, double beta, double* y, int64_t incy);
 // ---------------------------------------------------------------- split precision
 // C += alpha op(A) op(B) (or one triangle of alpha A A^T / A^T A) with each
 // FP32 operand split into `terms`-product BF16 pieces on the BF16 MFMA.

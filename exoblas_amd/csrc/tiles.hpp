@@ -93,7 +93,8 @@ __host__ __device__ inline void row_range(int tri, int64_t i, int64_t n, int64_t
 }
 
 // Arguments of update_kernel: KIND 0 scales (A *= alpha), 1 adds
-// (alpha u1_i) c1_j, 2 also adds (alpha u2_i) c2_j.
+// (alpha u1_i) c1_j, 2 also adds (alpha u2_i) c2_j, 3 stores zeros (A not
+// read).
 template <class T>
 struct UpdateArgs {
   T alpha;
@@ -132,16 +133,17 @@ __global__ __launch_bounds__(256) void update_kernel(int64_t m, int64_t n, T* A,
 #pragma unroll
   for (int k = 0; k < VN; ++k) {
     const int64_t j = j0 + k;
-    cv1[k] = (KIND >= 1 && j < n) ? g.c1[j * g.ic1] : T(0);
+    cv1[k] = ((KIND == 1 || KIND == 2) && j < n) ? g.c1[j * g.ic1] : T(0);
     cv2[k] = (KIND == 2 && j < n) ? g.c2[j * g.ic2] : T(0);
   }
   auto upd = [&](T& a, T r1, T r2, int k) {
     if (KIND == 0) a = a * g.alpha;
+    if (KIND == 3) a = T(0);
     if (KIND == 1) a += r1 * cv1[k];
     if (KIND == 2) a += r1 * cv1[k] + r2 * cv2[k];
   };
   auto rows = [&](int64_t i, T& r1, T& r2) {
-    r1 = KIND >= 1 ? g.alpha * g.u1[i * g.iu1] : T(0);
+    r1 = (KIND == 1 || KIND == 2) ? g.alpha * g.u1[i * g.iu1] : T(0);
     r2 = KIND == 2 ? g.alpha * g.u2[i * g.iu2] : T(0);
   };
   // element by element over rows [e0, e1) (masked to the stored part)
@@ -172,7 +174,7 @@ __global__ __launch_bounds__(256) void update_kernel(int64_t m, int64_t n, T* A,
 #pragma unroll
   for (int u = 0; u < RB; ++u)
     if (i0 + u < i1) {
-      v[u] = ld_stream<T>(A + (i0 + u) * lda + j0);
+      if (KIND != 3) v[u] = ld_stream<T>(A + (i0 + u) * lda + j0);
       rows(i0 + u, r1[u], r2[u]);
     }
 #pragma unroll

@@ -252,8 +252,11 @@ PROCS: List[Proc] = [
                                      ("Upper", "Trans", "49-69", "24-28"), ("Lower", "Trans", "72-83", "38-42"))
 ]
 
+ORDER_VALUES = (101, 102)
+
 # Native entry points with no counterpart in the reference: the split-
-# precision and mixed-precision forms.  Exported as exoblas_<p><name> (host or
+# precision and mixed-precision forms, and CBLAS-shaped gemm / gemv (storage
+# order, beta fused, C / y not read when beta == 0).  Exported as exoblas_<p><name> (host or
 # device pointers, synchronous) and exoblas_<p><name>_async (device pointers,
 # stream-ordered), declared in include/exoblas_amd_ext.h.
 EXT_PROCS: List[Proc] = [
@@ -276,6 +279,26 @@ EXT_PROCS: List[Proc] = [
          "one triangle of C += alpha A A^T (or A^T A) in the split BF16 form; "
          "arguments as exo_ssyrk_rm_stride_1 (src/level3/syrk.py:17-32) without the duplicate windows",
          "exoblas_amd.precision", precisions=("s",)),
+    Proc("gemm_ex", "ext",
+         [enum("Order", ORDER_VALUES), enum("TransA", TRANS_VALUES), enum("TransB", TRANS_VALUES),
+          size("M"), size("N"), size("K"), scalar("alpha"),
+          win("A", "((Order == 101) == (TransA == 111) ? M : K)", "((Order == 101) == (TransA == 111) ? K : M)"),
+          win("B", "((Order == 101) == (TransB == 111) ? K : N)", "((Order == 101) == (TransB == 111) ? N : K)"),
+          scalar("beta"),
+          win("C", "(Order == 101 ? M : N)", "(Order == 101 ? N : M)", const=False)],
+         "C = alpha op(A) op(B) + beta C with CBLAS dimensions in row-major (Order 101) or column-major (102) "
+         "storage, C not read when beta == 0, beta applied in the product's epilogue; each window is the stored "
+         "array (rows x leading dimension); the product as exo_<p>gemm_stride_1 (src/level3/gemm.py:17-47)",
+         "exoblas_amd.cblas"),
+    Proc("gemv_ex", "ext",
+         [enum("Order", ORDER_VALUES), enum("TransA", TRANS_VALUES), size("M"), size("N"), scalar("alpha"),
+          win("A", "(Order == 101 ? M : N)", "(Order == 101 ? N : M)"),
+          win("x", "(TransA == 111 ? N : M)"), scalar("beta"),
+          win("y", "(TransA == 111 ? M : N)", const=False)],
+         "y = alpha op(A) x + beta y for an M x N matrix A in row-major (Order 101) or column-major (102) "
+         "storage, y not read when beta == 0; element i of x at x.data + i * x.strides[0]; the product as "
+         "exo_<p>gemv_rm_stride_any (src/level2/gemv.py:10-22)",
+         "exoblas_amd.cblas"),
     Proc("trsv_mixed", "ext",
          [enum("Uplo", UPLO_VALUES), enum("TransA", TRANS_VALUES), enum("Diag", DIAG_VALUES), size("n"),
           win("x", "n", const=False), win("A", "n", "n"), enum("update", (16, 32)), size("refine")],
