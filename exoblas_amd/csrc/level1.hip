@@ -25,9 +25,13 @@ constexpr int kNT = 256;
 // vectors l, l + NT, ..., l + (U-1) NT of the workgroup's NT*U), all loads
 // issued before the first store; the grid sweeps the vectors (one round per
 // workgroup when the grid covers them all).
+// x and y: the 16-byte aligned starts; `head` (< VN) elements before them
+// belong to the range too (windows whose start is off alignment by the same
+// amount in both vectors), done by the last workgroup with the < VN trailing
+// ones.
 template <class T, class Op, int NT, int U>
 __global__ __launch_bounds__(NT) void ew_vec_kernel(int64_t nvec, int64_t n, T* __restrict__ x, T* __restrict__ y,
-                                                    Op op) {
+                                                    Op op, int head) {
   using V = typename Vec16<T>::type;
   constexpr int VN = Vec16<T>::n;
   for (int64_t i0 = (int64_t)blockIdx.x * NT * U + threadIdx.x; i0 < nvec; i0 += (int64_t)gridDim.x * NT * U) {
@@ -51,9 +55,17 @@ __global__ __launch_bounds__(NT) void ew_vec_kernel(int64_t nvec, int64_t n, T* 
       }
     }
   }
-  if (blockIdx.x == gridDim.x - 1) {  // the < VN trailing elements
-    const int64_t t = nvec * VN + threadIdx.x;
-    if (t < n) {
+  if (blockIdx.x == gridDim.x - 1) {  // the < VN trailing elements, then the head
+    int64_t t = 0;  // a tail element at t >= 0 or a head element at t < 0
+    bool mine = false;
+    if (threadIdx.x < VN) {
+      t = nvec * VN + threadIdx.x;
+      mine = t < n;
+    } else if (threadIdx.x >= 64 && threadIdx.x < 64 + head) {
+      t = (int64_t)threadIdx.x - 64 - head;
+      mine = true;
+    }
+    if (mine) {
       T xs = Op::rx ? x[t] : T(0);
       T ys = Op::ry ? y[t] : T(0);
       op(xs, ys);
@@ -108,8 +120,31 @@ void launch_ew(Ctx& c, int64_t n, T* x, int64_t incx, T* y, int64_t incy, Op op)
     if (ux) x -= n - 1, incx = 1;
     if (uy) y -= n - 1, incy = 1;
   }
+  // unit-stride windows off 16-byte alignment by the same amount: the
+  // aligned part on the vector path, the head elements before it with the
+  // tail
+  int head = 0;
+  if ((!ux || incx == 1) && (!uy || incy == 1)) {
+    const uintptr_t ox = ux ? reinterpret_cast<uintptr_t>(x) % 16 : 0, oy = uy ? reinterpret_cast<uintptr_t>(y) % 16 : 0;
+    const uintptr_t off = ux ? ox : oy;
+    if ((!ux || ox == off) && (!uy || oy == off) && off % sizeof(T) == 0 && off != 0) {
+      const int64_t h = (int64_t)((16 - off) / sizeof(T));
+      if (h < n) {
+        head = (int)h;
+        if (ux) x += h;
+        if (uy) y += h;
+        n -= h;
+      }
+    }
+  }
   const bool unit = (!(Op::rx || Op::wx) || (incx == 1 && aligned16(x))) &&
                     (!(Op::ry || Op::wy) || (incy == 1 && aligned16(y)));
+  if (!unit && head) {  // not aligned after all: undo the peel
+    if (ux) x -= head;
+    if (uy) y -= head;
+    n += head;
+    head = 0;
+  }
   if (unit) {
     // One round per workgroup of 256 lanes with four 16-byte vectors per lane
     // in flight, one for FP32 axpy: measured on MI355X inside the bandwidth
@@ -122,7 +157,7 @@ void launch_ew(Ctx& c, int64_t n, T* x, int64_t incx, T* y, int64_t incy, Op op)
     const int64_t nvec = n / VN;
     const int64_t blocks = std::max<int64_t>(1, ceil_div(nvec, (int64_t)NT * U));
     hipLaunchKernelGGL((ew_vec_kernel<T, Op, NT, U>), dim3((unsigned)blocks), dim3(NT), 0, c.stream, nvec, n, x, y,
-                       op);
+                       op, head);
   } else {
     constexpr int U = 8;
     const int64_t blocks = std::min<int64_t>(ceil_div(n, (int64_t)kNT * U), 0x7fffffff);
@@ -236,7 +271,7 @@ template <class Acc, class Out> struct Finish {
 // partials; err: the stream's timeout word.
 template <class T, class Term, class Out, int U>
 __global__ __launch_bounds__(kNT) void reduce_kernel(int64_t n, const T* __restrict__ x, int64_t incx,
-                                                     const T* __restrict__ y, int64_t incy, int vec,
+                                                     const T* __restrict__ y, int64_t incy, int vec, int head,
                                                      unsigned* ticket, unsigned* gen, unsigned* flags, unsigned* err,
                                                      Out* result,
                                                      Finish<typename Term::Acc, Out> fin, Term term) {
@@ -281,9 +316,14 @@ __global__ __launch_bounds__(kNT) void reduce_kernel(int64_t n, const T* __restr
         for (int k = 0; k < VN; ++k) acc[0] += term(vget<T>(xv, k), Term::ry ? vget<T>(yv, k) : T(0));
       }
     }
-    if (blockIdx.x == 0) {
-      const int64_t t = nvec * VN + threadIdx.x;
-      if (t < n) acc[1 % U] += term(x[t], Term::ry ? y[t] : T(0));
+    if (blockIdx.x == 0) {  // the < VN trailing elements and the `head` ones before x
+      if (threadIdx.x < VN) {
+        const int64_t t = nvec * VN + threadIdx.x;
+        if (t < n) acc[1 % U] += term(x[t], Term::ry ? y[t] : T(0));
+      } else if (threadIdx.x >= 64 && threadIdx.x < 64 + head) {
+        const int64_t t = (int64_t)threadIdx.x - 64 - head;
+        acc[1 % U] += term(x[t], Term::ry ? y[t] : T(0));
+      }
     }
   } else {
     // strided or misaligned: element by element, 8 per lane in flight
@@ -345,6 +385,8 @@ struct ReducePlan {
   int64_t incy;
   bool vec;
   int64_t blocks;
+  int64_t n;  // elements from x on (n minus head)
+  int head;   // elements before x and y (unit-stride windows peeled to 16-byte alignment)
 };
 
 template <class T, class Term>
@@ -358,11 +400,29 @@ ReducePlan<T> plan_reduce(int64_t n, const T* x, int64_t incx, const T* y, int64
     incx = 1;
     if (Term::ry) y -= n - 1, incy = 1;
   }
+  // unit-stride windows off 16-byte alignment by the same amount: peel the
+  // elements before the aligned start (the kernel adds them)
+  int head = 0;
+  if (incx == 1 && (!Term::ry || incy == 1)) {
+    const uintptr_t ox = reinterpret_cast<uintptr_t>(x) % 16;
+    const uintptr_t oy = Term::ry ? reinterpret_cast<uintptr_t>(y) % 16 : ox;
+    if (ox == oy && ox != 0 && ox % sizeof(T) == 0) {
+      const int64_t h = (int64_t)((16 - ox) / sizeof(T));
+      if (h < n) {
+        head = (int)h;
+        x += h;
+        if (Term::ry) y += h;
+        n -= h;
+      }
+    }
+  }
   ReducePlan<T> p;
   p.x = x;
   p.incx = incx;
   p.y = y;
   p.incy = incy;
+  p.n = n;
+  p.head = head;
   p.vec = incx == 1 && aligned16(x) && (!Term::ry || (incy == 1 && aligned16(y)));
   if (p.vec)
     p.blocks = std::min<int64_t>(std::max<int64_t>(1, ceil_div(n, (int64_t)kNT * U * VN)), (int64_t)cus * 2);
@@ -379,9 +439,9 @@ void enqueue_reduce(hipStream_t stream, const ReducePlan<T>& p, int64_t n, unsig
                     unsigned* flags, unsigned* err, Out* result, typename Term::Acc add) {
   using Acc = typename Term::Acc;
   constexpr int U = Term::ry ? 2 : 4;
-  hipLaunchKernelGGL((reduce_kernel<T, Term, Out, U>), dim3((unsigned)p.blocks), dim3(kNT), 0, stream, n, p.x,
-                     p.incx, p.y, p.incy, p.vec ? 1 : 0, ticket, gen, flags, err, result, Finish<Acc, Out>{add},
-                     Term{});
+  hipLaunchKernelGGL((reduce_kernel<T, Term, Out, U>), dim3((unsigned)p.blocks), dim3(kNT), 0, stream, p.n, p.x,
+                     p.incx, p.y, p.incy, p.vec ? 1 : 0, p.head, ticket, gen, flags, err, result,
+                     Finish<Acc, Out>{add}, Term{});
 }
 
 template <class T, class Term, class Out>
