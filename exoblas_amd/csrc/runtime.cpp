@@ -545,6 +545,44 @@ extern "C" int exoblas_stream_check(void* stream_) {
   return 0;
 }
 
+extern "C" int exoblas_stream_release(void* stream_) {
+  using namespace exoblas;
+  hipStream_t s = static_cast<hipStream_t>(stream_);
+  try {
+    if (capturing(s)) throw Error(kNotSupported, "exoblas_stream_release: the stream is being captured");
+    std::unique_ptr<StreamState> ss;
+    {
+      std::lock_guard<std::mutex> gd(g_devices_mutex);
+      for (auto& d : g_devices) {
+        std::lock_guard<std::mutex> g(d.second->map_mutex);
+        auto it = d.second->streams.find(s);
+        if (it == d.second->streams.end()) continue;
+        ss = std::move(it->second);
+        d.second->streams.erase(it);
+        break;
+      }
+    }
+    if (!ss) return kOk;
+    std::lock_guard<std::mutex> g(ss->mutex);  // a call in progress on the stream finishes first
+    int prev = 0;
+    EXO_HIP_CHECK(hipGetDevice(&prev));
+    EXO_HIP_CHECK(hipSetDevice(ss->device));
+    hipError_t e = hipStreamSynchronize(s);
+    for (int i = 0; i < Ctx::kSlots && e == hipSuccess; ++i)
+      if (ss->scratch[i]) e = hipFree(ss->scratch[i]);
+    for (size_t i = 0; i < ss->retired.size() && e == hipSuccess; ++i) e = hipFree(ss->retired[i]);
+    if (e == hipSuccess && ss->flags) e = hipFree(ss->flags);
+    if (e == hipSuccess && ss->counters) e = hipFree(ss->counters);
+    if (e == hipSuccess && ss->timeout_host) e = hipHostFree(ss->timeout_host);
+    (void)hipSetDevice(prev);
+    EXO_HIP_CHECK(e);
+  } catch (const Error& e) {
+    record_error(e.code, e.what());
+    return e.code;
+  }
+  return kOk;
+}
+
 extern "C" int exoblas_set_spin_limit(long long polls) {
   const int st = exoblas::set_spin_limit(polls);
   if (st != 0) exoblas::record_error(st, "exoblas_set_spin_limit: hipMemcpyToSymbol failed");

@@ -34,6 +34,7 @@ enum Status : int {
   kNotSupported = 7,
   kInvalidPointer = 8,
   kTimeout = 9,
+  kCancelled = 10,
 };
 
 // Element offset range [lo, hi) covered by a 1-D window of n elements with

@@ -13,6 +13,14 @@
 // every admitted request on its own through the plain exo_* device entry
 // points instead (gemm as mscal + gemm, as the reference wrapper does), the
 // one-launch-per-call comparison.
+//
+// Failure handling: a request is validated on its own at submission (a
+// malformed one is rejected with its status and never joins a batch); a
+// queued request can be cancelled (it is skipped at admission; work already
+// launched always runs to completion, so kernels never stop early); a batch
+// whose launch fails, or whose bounded waits expire (the stream's timeout
+// word), marks its own requests failed and the loop carries on with the next
+// ones.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -43,7 +51,7 @@ struct Req {
   exoblas_batch_op op;
   int64_t bytes;
   double t_submit = 0, t_done = 0;
-  int state = 0;  // 0 queued, 1 in flight, 2 done, 3 failed
+  int state = 0;  // 0 queued, 1 in flight, 2 done, 3 failed, 4 cancelled
 };
 
 struct Slot {
@@ -68,7 +76,7 @@ struct exoblas_server_s {
   std::deque<Req> reqs;             // by id
   std::deque<int64_t> fifo;         // queued ids
   int64_t budget = 0, in_flight = 0, peak = 0;
-  int64_t completed = 0, rejected = 0, batches = 0, failed = 0;
+  int64_t completed = 0, rejected = 0, batches = 0, failed = 0, cancelled = 0;
   double t_first = 0, t_last = 0;
   int mode = 0, max_batch = 0;
   std::vector<Slot> slots;
@@ -165,6 +173,10 @@ void exoblas_server_s::run() {
         s.ids.clear();
         s.bytes = 0;
         while (!fifo.empty() && (int64_t)s.ids.size() < max_batch) {
+          if (reqs[fifo.front()].state == 4) {  // cancelled while queued
+            fifo.pop_front();
+            continue;
+          }
           Req& r = reqs[fifo.front()];
           if (in_flight + s.bytes + r.bytes > budget) break;  // FIFO: wait for memory
           s.bytes += r.bytes;
@@ -244,14 +256,22 @@ extern "C" int64_t exoblas_server_submit(exoblas_server s, const exoblas_batch_o
   r.op = *op;
   r.bytes = batch::operand_bytes(*op);
   r.t_submit = now_us();
+  // the request on its own: a malformed one never joins a batch
+  std::string why;
+  batch::Built probe;
+  const int valid = batch::build(op, 1, probe, why);
   std::lock_guard<std::mutex> g(s->m);
   if (s->reqs.empty()) s->t_first = r.t_submit;
   const int64_t id = (int64_t)s->reqs.size();
-  if (r.bytes > s->budget) {
+  if (valid != kOk || r.bytes > s->budget) {
     r.state = 3;
     r.t_done = r.t_submit;
     s->reqs.push_back(r);
     ++s->rejected;
+    if (valid != kOk) {
+      record_error(valid, "exoblas_server_submit: " + why);
+      return -valid;
+    }
     record_error(kAllocError, "exoblas_server_submit: request larger than the HBM budget");
     return -kAllocError;
   }
@@ -277,8 +297,21 @@ extern "C" int exoblas_server_replay(exoblas_server s, const exoblas_batch_op* o
 extern "C" int exoblas_server_drain(exoblas_server s) {
   if (!s) return kInvalidPointer;
   std::unique_lock<std::mutex> g(s->m);
-  s->cv.wait(g, [s] { return s->completed + s->rejected >= (int64_t)s->reqs.size(); });
+  s->cv.wait(g, [s] { return s->completed + s->rejected + s->cancelled >= (int64_t)s->reqs.size(); });
   return s->failed ? kHipError : kOk;
+}
+
+extern "C" int exoblas_server_cancel(exoblas_server s, int64_t id) {
+  if (!s) return -kInvalidPointer;
+  std::lock_guard<std::mutex> g(s->m);
+  if (id < 0 || id >= (int64_t)s->reqs.size()) return -kInvalidSize;
+  Req& r = s->reqs[id];
+  if (r.state != 0) return 0;  // launched, finished, rejected or already cancelled
+  r.state = 4;
+  r.t_done = now_us();
+  ++s->cancelled;
+  s->cv.notify_all();
+  return 1;
 }
 
 extern "C" int exoblas_server_done(exoblas_server s, int64_t id) {
@@ -286,7 +319,7 @@ extern "C" int exoblas_server_done(exoblas_server s, int64_t id) {
   std::lock_guard<std::mutex> g(s->m);
   if (id < 0 || id >= (int64_t)s->reqs.size()) return -kInvalidSize;
   const int st = s->reqs[id].state;
-  return st == 2 ? 1 : st == 3 ? -kHipError : 0;
+  return st == 2 ? 1 : st == 3 ? -kHipError : st == 4 ? -kCancelled : 0;
 }
 
 extern "C" int exoblas_server_stats_get(exoblas_server s, exoblas_server_stats* out) {
@@ -302,6 +335,8 @@ extern "C" int exoblas_server_stats_get(exoblas_server s, exoblas_server_stats* 
   out->rejected = s->rejected;
   out->batches = s->batches;
   out->peak_bytes_in_flight = s->peak;
+  out->cancelled = s->cancelled;
+  out->failed = s->failed;
   if (!lat.empty()) {
     auto pct = [&](double p) {
       const size_t i = (size_t)std::min<double>(lat.size() - 1, std::max(0.0, p * lat.size() - 1e-9));
@@ -340,6 +375,7 @@ extern "C" int exoblas_server_destroy(exoblas_server s) {
   for (auto& sl : s->slots) {
     (void)hipStreamSynchronize(sl.stream);
     sl.dev.release();
+    (void)exoblas_stream_release(sl.stream);  // the library's state for the slot's stream
     (void)hipStreamDestroy(sl.stream);
     (void)hipEventDestroy(sl.done);
   }

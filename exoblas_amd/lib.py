@@ -97,6 +97,8 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.exoblas_synchronize.restype = ctypes.c_int
     lib.exoblas_stream_check.restype = ctypes.c_int
     lib.exoblas_stream_check.argtypes = [ctypes.c_void_p]
+    lib.exoblas_stream_release.restype = ctypes.c_int
+    lib.exoblas_stream_release.argtypes = [ctypes.c_void_p]
     lib.exoblas_set_spin_limit.restype = ctypes.c_int
     lib.exoblas_set_spin_limit.argtypes = [ctypes.c_longlong]
     lib.exoblas_sdot_noalloc_async.restype = ctypes.c_int

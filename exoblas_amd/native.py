@@ -39,7 +39,8 @@ class ServerStats(ctypes.Structure):
     """struct exoblas_server_stats"""
     _fields_ = [("submitted", _I64), ("completed", _I64), ("rejected", _I64), ("batches", _I64),
                 ("p50_us", ctypes.c_double), ("p99_us", ctypes.c_double), ("mean_us", ctypes.c_double),
-                ("max_us", ctypes.c_double), ("wall_s", ctypes.c_double), ("peak_bytes_in_flight", _I64)]
+                ("max_us", ctypes.c_double), ("wall_s", ctypes.c_double), ("peak_bytes_in_flight", _I64),
+                ("cancelled", _I64), ("failed", _I64)]
 
 
 SYMBOLS = [
@@ -56,6 +57,7 @@ SYMBOLS = [
     ("exoblas_server_replay", _i, (_P, _P, _P, _I64)),
     ("exoblas_server_drain", _i, (_P,)),
     ("exoblas_server_done", _i, (_P, _I64)),
+    ("exoblas_server_cancel", _i, (_P, _I64)),
     ("exoblas_server_stats_get", _i, (_P, ctypes.POINTER(ServerStats))),
     ("exoblas_server_latencies", _I64, (_P, _P, _I64)),
     ("exoblas_server_destroy", _i, (_P,)),

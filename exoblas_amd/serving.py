@@ -6,10 +6,10 @@ log-uniform in [32, 2048] on a diagonally dominant matrix (off-diagonal
 U[-1,1]/n, diagonal n + U[0,1]), 15 % sgemv N with m, n log-uniform in
 [64, 8192]; Poisson arrivals at `rate` requests per second; everything from
 one seeded generator.  ``Workload`` puts every operand in device memory
-(torch tensors): read-only gemm / gemv inputs are windows of shared pools, each
-trsv order gets its own matrix, every request its own output, and builds the
-exoblas_batch_op array.  ``run`` replays the stream through a server and
-returns its statistics.
+(torch tensors), either shared (every request reads the same few matrices) or
+pooled (each request its own window of large pools), every request its own
+output, and builds the exoblas_batch_op array.  ``run`` replays the stream
+through a server and returns its statistics.
 """
 from __future__ import annotations
 
@@ -60,26 +60,68 @@ def make_stream(count: int = 20000, rate: float = 20000.0, seed: int = 0) -> Str
 
 
 class Workload:
-    """Device-resident operands for a stream, and its op array."""
+    """Device-resident operands for a stream, and its op array.
 
-    def __init__(self, s: Stream, seed: int = 1):
+    Two layouts.  Shared (``pool_bytes`` None): the read-only gemm / gemv
+    inputs are windows at the start of two 1024^2 matrices, an 8192^2 matrix
+    and one x, each trsv order gets its own matrix.  Pooled: every request
+    reads its own window at a random 256-byte aligned offset of pools of
+    ``pool_bytes`` each (gemm A and B with ld 1024, gemv A with ld 8192 and
+    its x), and each trsv takes the leading n x n block (ld 2048) of one of
+    ``tri_pool`` 2048^2 diagonally dominant matrices, so consecutive requests
+    do not find each other's operands in the caches.  Either way every
+    request writes its own output window."""
+
+    def __init__(self, s: Stream, seed: int = 1, pool_bytes: Optional[int] = None, tri_pool: int = 24):
         import torch
         self.torch = torch
         self.s = s
+        self.pooled = pool_bytes is not None
         g = torch.Generator(device="cuda")
         g.manual_seed(seed)
         dev = "cuda"
+        rng = np.random.default_rng(seed)
 
         def u(*shape):
             return torch.empty(shape, dtype=torch.float32, device=dev).uniform_(-1, 1, generator=g)
-        self.gA, self.gB = u(1024, 1024), u(1024, 1024)
-        self.vA, self.vx = u(8192, 8192), u(8192)
-        self.tri: Dict[int, "torch.Tensor"] = {}
-        for nn in np.unique(s.m[s.kind == N.OP_TRSV]):
-            nn = int(nn)
-            T = u(nn, nn) / nn
+
+        def dd(nn, ld):
+            T = u(nn, ld) / nn
             T.diagonal().copy_(nn + torch.empty(nn, device=dev).uniform_(0, 1, generator=g))
-            self.tri[nn] = T
+            return T
+        count = len(s.kind)
+        self.src = [None] * count  # per request: (tensor, offset, ld) of A and of B / x
+        if not self.pooled:
+            self.gA, self.gB = u(1024, 1024), u(1024, 1024)
+            self.vA, self.vx = u(8192, 8192), u(8192)
+            self.tri: Dict[int, "torch.Tensor"] = {}
+            for nn in np.unique(s.m[s.kind == N.OP_TRSV]):
+                self.tri[int(nn)] = dd(int(nn), int(nn))
+            for i in range(count):
+                m, n, k = int(s.m[i]), int(s.n[i]), int(s.k[i])
+                if s.kind[i] == N.OP_GEMM:
+                    self.src[i] = ((self.gA, 0, 1024), (self.gB, 0, 1024))
+                elif s.kind[i] == N.OP_GEMV:
+                    self.src[i] = ((self.vA, 0, 8192), (self.vx, 0, 1))
+                else:
+                    self.src[i] = ((self.tri[m], 0, m), None)
+        else:
+            pe = max(int(pool_bytes) // 4, 8192 * 8192)
+            self.pA, self.pB = u(pe), u(pe)
+            self.pV = u(((pe + 8191) // 8192) * 8192)
+            self.pX = u(pe // 4)
+            self.tri_pool = [dd(2048, 2048) for _ in range(tri_pool)]
+
+            def off(total, need):
+                return int(rng.integers(0, (total - need) // 64 + 1)) * 64
+            for i in range(count):
+                m, n, k = int(s.m[i]), int(s.n[i]), int(s.k[i])
+                if s.kind[i] == N.OP_GEMM:
+                    self.src[i] = ((self.pA, off(pe, m * 1024), 1024), (self.pB, off(pe, k * 1024), 1024))
+                elif s.kind[i] == N.OP_GEMV:
+                    self.src[i] = ((self.pV, off(self.pV.numel(), m * 8192), 8192), (self.pX, off(self.pX.numel(), n), 1))
+                else:
+                    self.src[i] = ((self.tri_pool[int(rng.integers(0, tri_pool))], 0, 2048), None)
         # outputs: one arena, each request its own window
         out = np.where(s.kind == N.OP_GEMM, s.m * s.n, s.m)
         self.out_off = np.concatenate([[0], np.cumsum(out + 64)])  # 256-byte separation
@@ -98,6 +140,28 @@ class Workload:
             return self.out[o:o + int(s.m[i] * s.n[i])].view(int(s.m[i]), int(s.n[i]))
         return self.out[o:o + int(s.m[i])]
 
+    def operands(self, i):
+        """Request i's read-only operands as host arrays: (A, B) for gemm, (A,
+        x) for gemv, (T,) for trsv, each contiguous."""
+        s = self.s
+        m, n, k = int(s.m[i]), int(s.n[i]), int(s.k[i])
+        (ta, oa, la), bsrc = self.src[i]
+
+        def mat(t, o, ld, r, c):
+            flat = t.reshape(-1)
+            return flat[o:o + (r - 1) * ld + c].cpu().numpy() if r > 0 else np.zeros(0, np.float32), ld
+
+        def dense(t, o, ld, r, c):
+            v, ld = mat(t, o, ld, r, c)
+            return np.ascontiguousarray(np.lib.stride_tricks.as_strided(v, (r, c), (4 * ld, 4)))
+        if s.kind[i] == N.OP_GEMM:
+            tb, ob, lb = bsrc
+            return dense(ta, oa, la, m, k), dense(tb, ob, lb, k, n)
+        if s.kind[i] == N.OP_GEMV:
+            tb, ob, _ = bsrc
+            return dense(ta, oa, la, m, n), tb.reshape(-1)[ob:ob + n].cpu().numpy()
+        return (dense(ta, oa, la, m, m),)
+
     def _ops(self) -> np.ndarray:
         s = self.s
         ops = np.zeros(len(s.kind), N.OP_DTYPE)
@@ -109,27 +173,23 @@ class Workload:
         ops["alpha"] = 1.0
         g = s.kind == N.OP_GEMM
         ops["trans_b"][g] = NO_TRANS
-        ops["a"][g] = self.gA.data_ptr()
-        ops["lda"][g] = 1024
-        ops["b"][g] = self.gB.data_ptr()
-        ops["ldb"][g] = 1024
         ops["ldc"][g] = s.n[g]
         ops["beta"][g] = 0.5
         v = s.kind == N.OP_GEMV
-        ops["a"][v] = self.vA.data_ptr()
-        ops["lda"][v] = 8192
-        ops["b"][v] = self.vx.data_ptr()
-        ops["ldb"][v] = 1
         ops["ldc"][v] = 1
         ops["beta"][v] = 0.5
-        t = np.flatnonzero(s.kind == N.OP_TRSV)
+        t = s.kind == N.OP_TRSV
         ops["trans_b"][t] = LOWER
         ops["diag"][t] = NON_UNIT
         ops["ldc"][t] = 1
-        for i in t:
-            T = self.tri[int(s.m[i])]
-            ops["a"][i] = T.data_ptr()
-            ops["lda"][i] = int(s.m[i])
+        for i in range(len(s.kind)):
+            (ta, oa, la), bsrc = self.src[i]
+            ops["a"][i] = ta.data_ptr() + 4 * oa
+            ops["lda"][i] = la
+            if bsrc is not None:
+                tb, ob, lb = bsrc
+                ops["b"][i] = tb.data_ptr() + 4 * ob
+                ops["ldb"][i] = lb
         return ops
 
     def operand_bytes(self) -> np.ndarray:

@@ -23,6 +23,7 @@ extern "C" {
 #define EXOBLAS_STATUS_NOT_SUPPORTED 7
 #define EXOBLAS_STATUS_INVALID_POINTER 8
 #define EXOBLAS_STATUS_TIMEOUT 9
+#define EXOBLAS_STATUS_CANCELLED 10
 
 /* Creates the per-device state (stream, counters) for the current HIP device. */
 int exoblas_init(void);
@@ -37,6 +38,14 @@ int exoblas_synchronize(void);
  * check: EXOBLAS_STATUS_TIMEOUT means the results of those calls are not
  * valid.  The synchronous exo_* calls make this check themselves. */
 int exoblas_stream_check(void *stream);
+/* Releases what the library holds for `stream` (a hipStream_t): waits for
+ * its queued work, then frees the stream's scratch buffers, counters and
+ * timeout word.  Call it before destroying a stream the library has been
+ * used on (HIP may hand the same handle to a later stream, which then starts
+ * fresh); graphs captured on the stream must not be replayed afterwards.
+ * Returns 0 (also for a stream the library has not seen) or a status;
+ * EXOBLAS_STATUS_NOT_SUPPORTED while the stream is being captured. */
+int exoblas_stream_release(void *stream);
 /* Bound of every wait between workgroups, in polls (< 0 restores the
  * default of 2^22, about a second).  A test hook: a small bound makes waits
  * give up early, so the timeout path can be exercised while every kernel

@@ -77,20 +77,32 @@ typedef struct exoblas_server_stats {
   double p50_us, p99_us, mean_us, max_us; /* submit -> completion observed */
   double wall_s;                          /* first submit -> last completion */
   int64_t peak_bytes_in_flight;
+  int64_t cancelled; /* cancelled while queued (exoblas_server_cancel) */
+  int64_t failed;    /* launched in a batch that failed (launch error or an expired wait) */
 } exoblas_server_stats;
 
 int exoblas_server_create(exoblas_server *srv, int64_t hbm_budget, int32_t streams, int32_t mode,
                           int32_t max_batch);
-/* Queues one problem; returns its request id (>= 0) or -status.  A problem
- * larger than the whole budget is rejected (EXOBLAS_STATUS_ALLOC_ERROR). */
+/* Queues one problem; returns its request id (>= 0) or -status.  The problem
+ * is validated on its own first (a malformed one is rejected with its
+ * status: -EXOBLAS_STATUS_INVALID_*); a problem larger than the whole budget
+ * is rejected (EXOBLAS_STATUS_ALLOC_ERROR).  Rejected requests still take an
+ * id. */
 int64_t exoblas_server_submit(exoblas_server srv, const exoblas_batch_op *op);
 /* Submits ops[i] at arrival_us[i] microseconds after the call starts (times
  * ascending), pacing from the calling thread; returns once all are submitted.
  * Request ids are assigned in order.  Returns 0 or a status. */
 int exoblas_server_replay(exoblas_server srv, const exoblas_batch_op *ops, const double *arrival_us, int64_t count);
-/* Blocks until every submitted request has completed. */
+/* Blocks until every submitted request has completed, been rejected or been
+ * cancelled. */
 int exoblas_server_drain(exoblas_server srv);
-/* 1 if request `id` has completed, 0 if not, -status on error. */
+/* Cancels request `id` if it is still queued: returns 1 (it will never run
+ * and its output is not touched), 0 if it was already launched, finished,
+ * rejected or cancelled (launched work always runs to completion), -status
+ * for an unknown id. */
+int exoblas_server_cancel(exoblas_server srv, int64_t id);
+/* 1 if request `id` has completed, 0 if not, -EXOBLAS_STATUS_CANCELLED if it
+ * was cancelled, -EXOBLAS_STATUS_HIP_ERROR if it failed or was rejected. */
 int exoblas_server_done(exoblas_server srv, int64_t id);
 int exoblas_server_stats_get(exoblas_server srv, exoblas_server_stats *out);
 /* Latency of each completed request in submission order (us); returns the

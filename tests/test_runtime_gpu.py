@@ -156,6 +156,9 @@ def test_growth_during_capture_refused(env):
     hip = _hip()
     raw = ctypes.c_void_p()
     assert hip.hipStreamCreate(ctypes.byref(raw)) == 0
+    # HIP may recycle the handle of a stream destroyed earlier in this
+    # process; releasing it drops any state the library kept for that one
+    assert so.exoblas_stream_release(raw.value) == 0
     try:
         s = torch.cuda.ExternalStream(raw.value)
         big = 1 << 26
@@ -185,4 +188,38 @@ def test_growth_during_capture_refused(env):
         del graph, graph2
     finally:
         torch.cuda.synchronize()
+        assert so.exoblas_stream_release(raw.value) == 0
         hip.hipStreamDestroy(raw)
+
+
+def test_stream_release(env):
+    """exoblas_stream_release frees a stream's state: the next call on the
+    same handle builds it afresh (and gives the same result), releasing an
+    unknown or already released stream is a no-op, and a released stream can
+    be destroyed while later calls on other streams still work."""
+    torch, so, L = env
+    hip = _hip()
+    raw = ctypes.c_void_p()
+    assert hip.hipStreamCreate(ctypes.byref(raw)) == 0
+    try:
+        n = 1 << 22
+        x = torch.ones(n, device="cuda")
+        res = torch.zeros(1, device="cuda")
+        torch.cuda.synchronize()
+        for _ in range(2):
+            assert so.exoblas_sdot_stride_1_async(raw.value, n, L.win1(x.data_ptr(), 1), L.win1(x.data_ptr(), 1),
+                                                  ctypes.c_void_p(res.data_ptr())) == 0
+            assert so.exoblas_stream_check(raw.value) == 0
+            assert res.item() == float(n)
+            res.zero_()
+            torch.cuda.synchronize()
+            assert so.exoblas_stream_release(raw.value) == 0
+            assert so.exoblas_stream_release(raw.value) == 0  # already released
+    finally:
+        torch.cuda.synchronize()
+        hip.hipStreamDestroy(raw)
+    s = torch.cuda.Stream()
+    assert so.exoblas_sdot_stride_1_async(s.cuda_stream, 1000, L.win1(x.data_ptr(), 1), L.win1(x.data_ptr(), 1),
+                                          ctypes.c_void_p(res.data_ptr())) == 0
+    s.synchronize()
+    assert res.item() == 1000.0
