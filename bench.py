@@ -358,10 +358,36 @@ def bench_bandwidth(a, torch, so, L):
         "roofline": {"bound": "hbm", "kernel": dname, "achieved": round(dach, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(dach / HBM_PEAK_GBS, 4), "traffic": traffic},
         "kernels": kernels,
+        "reference_copy": _runtime_copy(torch, stream, sets, a),
     }
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_leg("bandwidth", a)
     return out
+
+
+def _runtime_copy(torch, stream, sets, a):
+    """The HIP runtime's own device-to-device copy (hipMemcpyAsync, which a
+    contiguous torch copy_ issues) of the same n-element FP32 vectors as
+    scopy, two sets taken in turn: the read-write stream ceiling this box
+    gives a vendor kernel, beside scopy / sswap / srot."""
+    xs = [st.v["s"]["x"] for st in sets]
+    ys = [st.v["s"]["y"] for st in sets]
+    for k in range(4):
+        ys[k % 2].copy_(xs[k % 2])
+    torch.cuda.synchronize()
+    reps, tot = 10, 0.0
+    for r in range(reps):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        ys[r % 2].copy_(xs[r % 2])
+        e1.record(stream)
+        torch.cuda.synchronize()
+        tot += e0.elapsed_time(e1)
+    ms = tot / reps
+    b = 2 * a.n1 * 4
+    gbs = b / (ms * 1e-3) / 1e9
+    return {"kernel": "hipMemcpyAsync device-to-device (runtime blit kernel)", "us": round(ms * 1e3, 2),
+            "GB/s": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4), "bytes": b}
 
 
 # ---------------------------------------------------------------- other workloads

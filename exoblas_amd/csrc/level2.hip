@@ -486,13 +486,16 @@ void launch_gemv_n(Ctx& c, int64_t M, int64_t N, T alpha, T beta, const T* A, in
   // ones: 8 rows per wave and the columns split across workgroups, summed by
   // a second launch.
   // Rows of up to 16 vectors per thread and enough of them for 8 workgroups
-  // per CU: whole-workgroup rows with x in registers, two rows at a time, 12
-  // workgroups per CU (best of a sweep over 1-4 rows and 1-16 workgroups per
-  // CU on MI355X: 8192 x 8192 FP64 77 -> 83 % of HBM peak, FP32 76 -> 77 %).
+  // per CU: whole-workgroup rows with x in registers, two rows at a time, 16
+  // workgroups per CU for FP32 and 12 for FP64 (best of sweeps over 1-4 rows
+  // and 4-24 workgroups per CU on MI355X at 8192 x 8192: FP32 80.5 % of HBM
+  // peak at 16 against 77.7 % at 12 and 73.0 % at 4; FP64 82.8 % at 12,
+  // within 0.3 % of the best).
   if (TRI == kFull && vec && incx == 1 && aligned16(x) && ceil_div(N, VN) <= (int64_t)kNT * 16 &&
       M >= 2 * 8 * (int64_t)c.num_cus) {
     constexpr int R = 2;
-    const unsigned grid = (unsigned)std::min<int64_t>(ceil_div(M, R), (int64_t)12 * c.num_cus);
+    constexpr int64_t kPerCu = sizeof(T) == 4 ? 16 : 12;
+    const unsigned grid = (unsigned)std::min<int64_t>(ceil_div(M, R), kPerCu * c.num_cus);
     if (ceil_div(N, VN) <= (int64_t)kNT * 8)
       hipLaunchKernelGGL((gemv_n_wg_kernel<T, 8, R, TRI, UNIT>), dim3(grid), dim3(kNT), 0, c.stream, M, N, alpha,
                          beta, A, lda, x, y, incy, ow);
