@@ -1199,10 +1199,22 @@ def config4(a, torch, so, L):
                    "error_le_plain": le_plain(f"sgemm_split{f}", "sgemm_plain"),
                    "ssyrk_speedup": round(kernels["ssyrk_plain"]["ms"] / kernels[f"ssyrk_split{f}"]["ms"], 3),
                    "ssyrk_error_le_plain": le_plain(f"ssyrk_split{f}", "ssyrk_plain")} for f in (3, 4, 6)}
+    # per error metric, the fastest form whose sgemm and ssyrk errors are at
+    # most plain FP32's on both sets (max relative error is the stated
+    # precision metric; the normwise error |C - C64| / (|A||B|) is the other)
+    claims = {}
+    for m in ("max_rel_err", "max_err_normwise"):
+        ok = [f for f in (3, 4, 6) if all(summary[f]["error_le_plain"][f"{m}_{st}"] and
+                                          summary[f]["ssyrk_error_le_plain"][f"{m}_{st}"]
+                                          for st in ("uniform", "scaled"))]
+        best = max(ok, key=lambda f: summary[f]["speedup"]) if ok else None
+        claims[m] = {"form": f"{best} products" if best else None,
+                     "sgemm_speedup": summary[best]["speedup"] if best else None,
+                     "ssyrk_speedup": summary[best]["ssyrk_speedup"] if best else None}
     out = {"workload": f"sgemm NN {n}^3 plain vs split-BF16 (3/4/6 products), uniform and row-scaled sets; dgemm NN "
                        f"{n}^3; ssyrk L-N {N}x{K} plain vs split; strsv L-N-N {n} plain vs mixed + 1 refinement",
            "roofline": _roof("sgemm_plain", plain["TFLOP/s"], FP32_PEAK_TF, "TFLOP/s", "mfma"),
-           "split_forms": summary, "kernels": kernels}
+           "split_forms": summary, "precision_claims": claims, "kernels": kernels}
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_leg("config4", a)
     return out
