@@ -25,8 +25,8 @@ constexpr int kFlagBase = 2;
 template <class T, bool FWD, bool TRANS, bool UNIT, int KB>
 __global__ __launch_bounds__(4 * KB) void trsv_kernel(int64_t n, const T* __restrict__ A, int64_t lda, T* x,
                                                       unsigned* flags, unsigned* gen, unsigned* err) {
-  __shared__ trsv::Smem<T, T, KB> sm;
-  __shared__ T mb[KB][KB + 1];
+  __shared__ __align__(16) trsv::Smem<T, T, KB> sm;
+  __shared__ __align__(16) T mb[KB][trsv::mb_stride<T, KB>()];
   __shared__ int s_p;
   __shared__ unsigned s_marker;
   const int nb = (int)((n + KB - 1) / KB);

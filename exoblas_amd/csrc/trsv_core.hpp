@@ -36,6 +36,19 @@ constexpr int kNT = 256;  // threads per workgroup at kB
 
 // TI: precision of the diagonal-block inverse and of the block mat-vec.
 // KB: rows per block; a block's workgroup has 4 * KB threads (4 per row).
+template <class T, int N>
+struct alignas(N * sizeof(T)) VecN {
+  T v[N];
+};
+
+// Row stride of the previous-block staging buffer `mb` (and of G, stored
+// over it): a multiple of 16 bytes, so a thread's 16 consecutive entries of
+// a row are 16-byte LDS reads.
+template <class T, int KB>
+constexpr int mb_stride() {
+  return KB + 16 / (int)sizeof(T);
+}
+
 template <class T, class TI = T, int KB = kB>
 struct Smem {
   TI inv[KB][KB + 1];
@@ -182,13 +195,14 @@ __device__ __forceinline__ void invert_diag(int64_t n, const T* A, int64_t lda, 
 }
 
 // One processing step p of the solve; called by all 4 * KB threads of a
-// workgroup.
+// workgroup.  With PF the last part of the step, the only part that waits on
+// the previous block, runs on one wave (KB = 64 rows, one per lane).
 // PF: load each panel block one iteration ahead (a kernel that keeps other
 // work items in its registers passes false).
 template <class T, bool FWD, bool TRANS, bool UNIT, int KB = kB, bool PF = true>
 __device__ __forceinline__ void step(int p, int64_t n, const T* __restrict__ A, int64_t lda, T* x, unsigned* pub,
                                      unsigned marker, unsigned* err, Smem<T, T, KB>& sm,
-                                     T (*mb)[KB + 1] = nullptr) {
+                                     T (*mb)[mb_stride<T, KB>()] = nullptr) {
   constexpr int CW = KB / 4;  // columns per thread in the block mat-vecs
   const int nb = (int)((n + KB - 1) / KB);
   const int t = threadIdx.x;
@@ -196,12 +210,24 @@ __device__ __forceinline__ void step(int p, int64_t n, const T* __restrict__ A, 
   const int64_t r0 = bi * KB;
   // With PF, this block's right-hand side is read before anything waits
   // (only this workgroup writes these rows of x, at the end of the step).
-  T b = T(0);
+  // With PF, this block's right-hand side goes to vs (local order) before
+  // anything waits (only this workgroup writes these rows of x, at the end
+  // of the step); invert_diag's barriers order it before its use.
   if (PF && t < KB) {
     const int64_t g = row_of<FWD, KB>(r0, t);
-    if (g < n) b = x[g];
+    sm.vs[t] = g < n ? x[g] : T(0);
   }
   invert_diag<T, T, FWD, TRANS, UNIT, KB>(n, A, lda, r0, sm.inv);
+  // wave 0's rows of inv(D) and G for the end of the step (PF)
+  constexpr bool kInvInRegs = sizeof(T) == 4;
+  T ir[kInvInRegs ? KB : 1];
+  T gr[PF ? KB : 1];
+  if constexpr (PF && kInvInRegs) {
+    if (t < KB) {
+#pragma unroll
+      for (int c = 0; c < KB; ++c) ir[c] = sm.inv[t][c];
+    }
+  }
   // NoTrans: 4 threads per row, CW consecutive columns each (contiguous
   // loads); Trans: one column of KB rows per KB threads, CW columns per
   // group (consecutive lanes read consecutive addresses).
@@ -228,11 +254,26 @@ __device__ __forceinline__ void step(int p, int64_t n, const T* __restrict__ A, 
       __syncthreads();
 #pragma unroll
       for (int e = 0; e < CW; ++e) gq[e] = T(0);
+      constexpr int VN = 16 / (int)sizeof(T);
 #pragma unroll 8
       for (int k = 0; k < KB; ++k) {  // inv(D) is lower triangular in local order: zeros above
         const T iv = sm.inv[i4][k];
+        const VecN<T, VN>* row = reinterpret_cast<const VecN<T, VN>*>(&mb[k][q4 * CW]);
 #pragma unroll
-        for (int e = 0; e < CW; ++e) gq[e] += iv * mb[k][q4 * CW + e];
+        for (int v = 0; v < CW / VN; ++v) {
+          const VecN<T, VN> m4 = row[v];
+#pragma unroll
+          for (int e = 0; e < VN; ++e) gq[v * VN + e] += iv * m4.v[e];
+        }
+      }
+      // G over mb (every thread has read mb), then wave 0 takes its rows
+      __syncthreads();
+#pragma unroll
+      for (int e = 0; e < CW; ++e) mb[i4][q4 * CW + e] = gq[e];
+      __syncthreads();
+      if (t < KB) {
+#pragma unroll
+        for (int c = 0; c < KB; ++c) gr[c] = mb[t][c];
       }
     }
   }
@@ -282,40 +323,94 @@ __device__ __forceinline__ void step(int p, int64_t n, const T* __restrict__ A, 
       }
     }
   }
-  sm.red[cq][r] = acc;
-  __syncthreads();
-  if (t < KB) {  // right-hand side in local order
-    const int64_t g = row_of<FWD, KB>(r0, t);
-    const int nr = (int)(g - r0);
-    T v = T(0);
-    if (g < n) {
-      T s = T(0);
+  if constexpr (!PF) {
+    // v = rhs - panel sums in local order, then x_local = inv(D) v with 4
+    // threads per row
+    sm.red[cq][r] = acc;
+    __syncthreads();
+    if (t < KB) {
+      const int64_t g = row_of<FWD, KB>(r0, t);
+      const int nr = (int)(g - r0);
+      T v = T(0);
+      if (g < n) {
+        T s = T(0);
 #pragma unroll
-      for (int w = 0; w < 4; ++w) s += sm.red[w][nr];
-      v = (PF ? b : x[g]) - s;
+        for (int w = 0; w < 4; ++w) s += sm.red[w][nr];
+        v = x[g] - s;
+      }
+      sm.vs[t] = v;
     }
-    sm.vs[t] = v;
-  }
-  __syncthreads();
-  {  // x_local = inv(D) v: 4 threads per row, CW columns each
+    __syncthreads();
     T s = T(0);
 #pragma unroll
     for (int e = 0; e < CW; ++e) s += sm.inv[i4][q4 * CW + e] * sm.vs[q4 * CW + e];
     s += __shfl_xor(s, 1, 64);
     s += __shfl_xor(s, 2, 64);
-    if (PF && p > 0) {  // x = y0 - G x_{p-1}
-      T* xs = sm.xs[0];
-      fetch_block<T, KB>(n, FWD ? (int64_t)(p - 1) : (int64_t)(nb - p), pub, marker, err, xs);
-      __syncthreads();
-      T u = T(0);
-#pragma unroll
-      for (int e = 0; e < CW; ++e) u += gq[e] * xs[q4 * CW + e];
-      u += __shfl_xor(u, 1, 64);
-      u += __shfl_xor(u, 2, 64);
-      s -= u;
-    }
     const int64_t g = row_of<FWD, KB>(r0, i4);
     if (q4 == 0 && g < n) publish_value<T>(g, s, x, pub, marker);
+  } else {
+    static_assert(KB <= 64, "the chain part of a step runs on one wave, one row per lane");
+    // v = rhs - panel sums (vs holds the rhs in local order): NoTrans keeps
+    // a row's four partial sums in adjacent lanes, so two shuffles reduce
+    // them; Trans spreads them over the waves, so they meet in LDS.
+    if constexpr (!TRANS) {
+      T ps = acc + __shfl_xor(acc, 1, 64);
+      ps += __shfl_xor(ps, 2, 64);
+      if (cq == 0 && gi < n) {
+        const int k = FWD ? r : KB - 1 - r;
+        sm.vs[k] -= ps;
+      }
+    } else {
+      sm.red[cq][r] = acc;
+      __syncthreads();
+      if (t < KB) {
+        const int64_t g = row_of<FWD, KB>(r0, t);
+        if (g < n) {
+          const int nr = (int)(g - r0);
+          sm.vs[t] -= (sm.red[0][nr] + sm.red[1][nr]) + (sm.red[2][nr] + sm.red[3][nr]);
+        }
+      }
+    }
+    __syncthreads();
+    // The rest on wave 0, lane i = local row i, with its rows of inv(D) (FP32:
+    // in registers since the inversion) and G in registers: y0 = inv(D) v,
+    // then, once x_{p-1} is published, x = y0 - G x_{p-1}; the wave passes
+    // x_{p-1} through LDS with no workgroup barrier on the chain (LDS
+    // operations of one wave complete in order).
+    if (t < KB) {
+      constexpr int VN = 16 / (int)sizeof(T);
+      const VecN<T, VN>* vv = reinterpret_cast<const VecN<T, VN>*>(sm.vs);
+      T y[4] = {T(0), T(0), T(0), T(0)};
+#pragma unroll
+      for (int c = 0; c < KB / VN; ++c) {
+        const VecN<T, VN> v4 = vv[c];
+#pragma unroll
+        for (int e = 0; e < VN; ++e) {
+          const int k = c * VN + e;
+          const T iv = kInvInRegs ? ir[k] : sm.inv[t][k];
+          y[k % 4] += iv * v4.v[e];
+        }
+      }
+      T xv = (y[0] + y[1]) + (y[2] + y[3]);
+      if (p > 0) {
+        const int64_t bj = FWD ? (int64_t)(p - 1) : (int64_t)(nb - p);
+        const int64_t gx = bj * KB + t;
+        T* xs = sm.xs[0];
+        xs[t] = gx < n ? tagged::read<T>(pub, gx, marker, err) : T(0);
+        __builtin_amdgcn_wave_barrier();
+        const VecN<T, VN>* x4 = reinterpret_cast<const VecN<T, VN>*>(xs);
+        T u[4] = {T(0), T(0), T(0), T(0)};
+#pragma unroll
+        for (int c = 0; c < KB / VN; ++c) {
+          const VecN<T, VN> xx = x4[c];
+#pragma unroll
+          for (int e = 0; e < VN; ++e) u[(c * VN + e) % 4] += gr[c * VN + e] * xx.v[e];
+        }
+        xv -= (u[0] + u[1]) + (u[2] + u[3]);
+      }
+      const int64_t g = row_of<FWD, KB>(r0, t);
+      if (g < n) publish_value<T>(g, xv, x, pub, marker);
+    }
   }
   __syncthreads();  // the shared buffers are reused by the caller's next step
 }
