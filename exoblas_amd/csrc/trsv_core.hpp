@@ -96,12 +96,11 @@ __device__ __forceinline__ int64_t row_of(int64_t r0, int i) {
 // One level of the pairwise merges of invert_diag (block size SZ -> 2 SZ),
 // then the next level.
 template <class TI, int KB, int SZ>
-__device__ __forceinline__ void merge_level(TI (&L)[KB][KB + 1]) {
+__device__ __forceinline__ void merge_level(TI (&L)[KB][KB + 1], int t) {
   if constexpr (SZ < KB) {
     constexpr int NT = 4 * KB;
     constexpr int PER = (KB / (2 * SZ)) * SZ * SZ / NT;  // outputs per thread
     static_assert(PER >= 1 && (KB / (2 * SZ)) * SZ * SZ % NT == 0, "merge layout");
-    const int t = threadIdx.x;
     int base[PER], r[PER], c[PER];
 #pragma unroll
     for (int e = 0; e < PER; ++e) {
@@ -134,18 +133,21 @@ __device__ __forceinline__ void merge_level(TI (&L)[KB][KB + 1]) {
 #pragma unroll
     for (int e = 0; e < PER; ++e) L[base[e] + SZ + r[e]][base[e] + c[e]] = pv[e];
     __syncthreads();
-    merge_level<TI, KB, 2 * SZ>(L);
+    merge_level<TI, KB, 2 * SZ>(L, t);
   }
 }
 
 // Loads the diagonal block of block bi (rows r0 ..) in local order, converted
 // to TI, and inverts it in place (16 x 16 diagonal blocks, then two pairwise
 // merges: 40 barriers instead of the 128 of a column-by-column inverse at
-// KB = 64); all 4 * KB threads call this.
+// KB = 64).  4 * KB threads with indices t = 0 .. 4 KB - 1 work on one block;
+// every thread of the workgroup calls this (the same number of barriers), so
+// a workgroup of 8 KB threads inverts two blocks at once, each half passing
+// its own t and L.
 template <class T, class TI, bool FWD, bool TRANS, bool UNIT, int KB = kB>
-__device__ __forceinline__ void invert_diag(int64_t n, const T* A, int64_t lda, int64_t r0, TI (&L)[KB][KB + 1]) {
+__device__ __forceinline__ void invert_diag(int64_t n, const T* A, int64_t lda, int64_t r0, TI (&L)[KB][KB + 1],
+                                            int t) {
   constexpr int NT = 4 * KB;
-  const int t = threadIdx.x;
   for (int idx = t; idx < KB * KB; idx += NT) {
     // NoTrans reads rows of A, Trans reads columns: keep lanes on consecutive addresses
     const int a = idx / KB, b = idx % KB;
@@ -191,7 +193,7 @@ __device__ __forceinline__ void invert_diag(int64_t n, const T* A, int64_t lda, 
   //    block rows / columns with compile-time bounds (the strictly upper
   //    parts of inv(A) and inv(C) hold zeros), so they unroll and the LDS
   //    reads overlap.
-  merge_level<TI, KB, 16>(L);
+  merge_level<TI, KB, 16>(L, t);
 }
 
 // One processing step p of the solve; called by all 4 * KB threads of a
@@ -208,8 +210,6 @@ __device__ __forceinline__ void step(int p, int64_t n, const T* __restrict__ A, 
   const int t = threadIdx.x;
   const int64_t bi = FWD ? p : nb - 1 - p;
   const int64_t r0 = bi * KB;
-  // With PF, this block's right-hand side is read before anything waits
-  // (only this workgroup writes these rows of x, at the end of the step).
   // With PF, this block's right-hand side goes to vs (local order) before
   // anything waits (only this workgroup writes these rows of x, at the end
   // of the step); invert_diag's barriers order it before its use.
@@ -217,7 +217,21 @@ __device__ __forceinline__ void step(int p, int64_t n, const T* __restrict__ A, 
     const int64_t g = row_of<FWD, KB>(r0, t);
     sm.vs[t] = g < n ? x[g] : T(0);
   }
-  invert_diag<T, T, FWD, TRANS, UNIT, KB>(n, A, lda, r0, sm.inv);
+  // PF: M(this block, block p-1) is staged in mb before the inversion, so
+  // its loads overlap the inversion's (whose first barrier orders them);
+  // mb[k][c] = M(row_of(k), column c of block p-1).
+  if constexpr (PF) {
+    if (p > 0) {
+      const int64_t cb0 = (FWD ? (int64_t)(p - 1) : (int64_t)(nb - p)) * KB;
+      for (int idx = t; idx < KB * KB; idx += 4 * KB) {
+        const int a = idx / KB, c2 = idx % KB;
+        const int k = TRANS ? c2 : a, c = TRANS ? a : c2;  // lanes on consecutive addresses
+        const int64_t gk = row_of<FWD, KB>(r0, k), gc = cb0 + c;
+        mb[k][c] = (gk < n && gc < n) ? mat<T, TRANS>(A, lda, gk, gc) : T(0);
+      }
+    }
+  }
+  invert_diag<T, T, FWD, TRANS, UNIT, KB>(n, A, lda, r0, sm.inv, t);
   // wave 0's rows of inv(D) and G for the end of the step (PF)
   constexpr bool kInvInRegs = sizeof(T) == 4;
   T ir[kInvInRegs ? KB : 1];
@@ -238,20 +252,12 @@ __device__ __forceinline__ void step(int p, int64_t n, const T* __restrict__ A, 
   // PF: the previous block's update is folded into inv(D) ahead of time,
   // G = inv(D) M(this block, block p-1), so once x_{p-1} appears the step is
   // one block mat-vec x = y0 - G x_{p-1} (y0 = inv(D)(b - the other blocks)).
-  // mb holds M(row_of(k), block p-1 column c); this thread keeps
-  // G[i4][q4*CW .. +CW] in registers.
+  // mb holds M(row_of(k), block p-1 column c); this thread forms
+  // G[i4][q4*CW .. +CW].
   T gq[CW];
   const int pp = PF && p > 0 ? p - 1 : p;  // blocks the panel loop covers
   if constexpr (PF) {
     if (p > 0) {
-      const int64_t cb0 = (FWD ? (int64_t)(p - 1) : (int64_t)(nb - p)) * KB;
-      for (int idx = t; idx < KB * KB; idx += 4 * KB) {
-        const int a = idx / KB, c2 = idx % KB;
-        const int k = TRANS ? c2 : a, c = TRANS ? a : c2;  // lanes on consecutive addresses
-        const int64_t gk = row_of<FWD, KB>(r0, k), gc = cb0 + c;
-        mb[k][c] = (gk < n && gc < n) ? mat<T, TRANS>(A, lda, gk, gc) : T(0);
-      }
-      __syncthreads();
 #pragma unroll
       for (int e = 0; e < CW; ++e) gq[e] = T(0);
       constexpr int VN = 16 / (int)sizeof(T);

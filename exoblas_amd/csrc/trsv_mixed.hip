@@ -65,7 +65,7 @@ __global__ __launch_bounds__(kNT) void trsv_mixed_kernel(int64_t n, const float*
   unsigned* done = flags + kFlagBase;
   const int64_t bi = FWD ? p : nb - 1 - p;
   const int64_t r0 = bi * kB;
-  trsv::invert_diag<float, double, FWD, TRANS, UNIT>(n, A, lda, r0, sm.inv);
+  trsv::invert_diag<float, double, FWD, TRANS, UNIT>(n, A, lda, r0, sm.inv, (int)threadIdx.x);
   const int r = TRANS ? t % kB : t / 4;
   const int cq = TRANS ? t / kB : t % 4;
   const int64_t gi = r0 + r;
