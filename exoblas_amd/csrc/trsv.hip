@@ -27,6 +27,7 @@ __global__ __launch_bounds__(4 * KB) void trsv_kernel(int64_t n, const T* __rest
                                                       unsigned* flags, unsigned* gen, unsigned* err) {
   __shared__ __align__(16) trsv::Smem<T, T, KB> sm;
   __shared__ __align__(16) T mb[KB][trsv::mb_stride<T, KB>()];
+  __shared__ __align__(16) T mb2[KB][trsv::mb_stride<T, KB>()];
   __shared__ int s_p;
   __shared__ unsigned s_marker;
   const int nb = (int)((n + KB - 1) / KB);
@@ -39,7 +40,7 @@ __global__ __launch_bounds__(4 * KB) void trsv_kernel(int64_t n, const T* __rest
   // tickets come from the stream's counter: a ticket past the last block
   // (only if launches sharing the counter overlapped) runs no step
   if (s_p >= nb) return;
-  trsv::step<T, FWD, TRANS, UNIT, KB>(s_p, n, A, lda, x, flags + kFlagBase, s_marker, err, sm, mb);
+  trsv::step<T, FWD, TRANS, UNIT, KB>(s_p, n, A, lda, x, flags + kFlagBase, s_marker, err, sm, mb, mb2);
 }
 
 // 64-row blocks.  (128-row blocks halve the number of dependent steps but

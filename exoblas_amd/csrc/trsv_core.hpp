@@ -13,10 +13,15 @@
 //      diagonal, each after the step that owns it has published its part of x;
 //   3. forms v = b - (panel . x) and x_block = inv(D) v with all its threads,
 //      so the serial part of the chain is one small mat-vec per block.
-// The standalone solve (PF) also folds the previous block into the inverse
-// ahead of time, G = inv(D) M(p, p-1): the panel covers blocks 0 .. p-2,
-// y0 = inv(D)(b - panel . x), and once x_{p-1} appears the step is
-// x_block = y0 - G x_{p-1}.
+// The standalone solve (PF) also folds the two previous blocks into the
+// inverse ahead of time, G1 = inv(D) M(p, p-1) and G2 = inv(D) M(p, p-2)
+// (matrix-core products, fold_g): the panel covers blocks 0 .. p-3,
+// y0 = inv(D)(b - panel . x), and the step ends x_block = y0 - G2 x_{p-2}
+// - G1 x_{p-1}, so once x_{p-1} appears the chain's critical path is one
+// block mat-vec.  The diagonal-block inverse takes 16 x 16 blocks by
+// right-looking substitution, then two pairwise merges on the matrix cores
+// (merge_mfma); every load of the step's own blocks is issued before any is
+// used.
 // Publication of x: the tagged words of tagged.hpp, one value per row.
 // Waits are bounded: past the spin limit the step records `marker` in *err
 // and finishes.
@@ -26,6 +31,7 @@
 
 #include <cstdint>
 
+#include "gemm_core.hpp"
 #include "tagged.hpp"
 
 namespace exoblas {
@@ -137,6 +143,111 @@ __device__ __forceinline__ void merge_level(TI (&L)[KB][KB + 1], int t) {
   }
 }
 
+// The merges of invert_diag at KB = 64 on the matrix cores (v_mfma_*_16x16x4;
+// 256 threads).  Step j of a 16-deep product lets lane group lg take
+// k = out_row(lg, j): that is the row of the accumulator layout this lane
+// holds in register j, so a product left in registers is the B operand of
+// the next product as it stands.
+//   16 -> 32: waves 0 and 1 each merge one pair (rows 32 w ..): P = B inv(A),
+//             then -inv(C) P from registers, written over B (no other wave
+//             reads that B).
+//   32 -> 64: wave w forms 16 x 16 tile (I, J) = (w / 2, w % 2) of
+//             P = B inv(A) (K tiles J .. 1: inv(A) is lower triangular), P
+//             goes over B, then -(inv(C) P) tile (I, J) from K tiles 0 .. I.
+template <class TI>
+__device__ __forceinline__ void merge_mfma(TI (&L)[64][65], int t) {
+  using MF = gemm::Mfma<TI>;
+  using Acc = typename MF::acc;
+  const int w = t / 64, lane = t % 64, lm = lane % 16, lg = lane / 16;
+  // 16 -> 32
+  if (w < 2) {
+    const int o = 32 * w;  // A = L[o ..][o ..], B = L[o+16 ..][o ..], C = L[o+16 ..][o+16 ..]
+    Acc P, Q;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) P[q] = Q[q] = TI(0);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int k = MF::out_row(lg, j);
+      P = MF::step(L[o + 16 + lm][o + k], L[o + k][o + lm], P);
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int k = MF::out_row(lg, j);
+      Q = MF::step(L[o + 16 + lm][o + 16 + k], P[j], Q);
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) L[o + 16 + MF::out_row(lg, q)][o + lm] = -Q[q];
+  }
+  __syncthreads();
+  // 32 -> 64: A = L[0 ..][0 ..], B = L[32 ..][0 ..], C = L[32 ..][32 ..]
+  const int I = w / 2, J = w % 2;
+  Acc P, Q;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) P[q] = Q[q] = TI(0);
+#pragma unroll
+  for (int K = 0; K < 2; ++K) {
+    if (K < J) continue;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int k = 16 * K + MF::out_row(lg, j);
+      P = MF::step(L[32 + 16 * I + lm][k], L[k][16 * J + lm], P);
+    }
+  }
+  __syncthreads();  // every wave has read B
+#pragma unroll
+  for (int q = 0; q < 4; ++q) L[32 + 16 * I + MF::out_row(lg, q)][16 * J + lm] = P[q];
+  __syncthreads();
+#pragma unroll
+  for (int K = 0; K < 2; ++K) {
+    if (K > I) continue;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int k = 16 * K + MF::out_row(lg, j);
+      const TI bv = K == I ? P[j] : L[32 + k][16 * J + lm];
+      Q = MF::step(L[32 + 16 * I + lm][32 + k], bv, Q);
+    }
+  }
+  __syncthreads();  // every wave has read P
+#pragma unroll
+  for (int q = 0; q < 4; ++q) L[32 + 16 * I + MF::out_row(lg, q)][16 * J + lm] = -Q[q];
+  __syncthreads();
+}
+
+// Entries of a KB x KB block each of the 4 * KB threads of a block step
+// loads (KB / 4).
+template <int KB>
+constexpr int per_thread() {
+  return KB * KB / (4 * KB);
+}
+
+// This thread's entries of the diagonal block of block bi (rows r0 ..) in
+// local order, converted to TI: entry e is local (i, j) of idx = t + 4 KB e
+// (NoTrans: i = idx / KB, j = idx % KB; Trans the other way round, so lanes
+// read consecutive addresses either way).  Every load is issued before any
+// is used (one memory round trip for the whole block, not one per entry).
+template <class T, class TI, bool FWD, bool TRANS, bool UNIT, int KB = kB>
+__device__ __forceinline__ void load_diag(int64_t n, const T* A, int64_t lda, int64_t r0, int t,
+                                          TI (&v)[per_thread<KB>()]) {
+  constexpr int NT = 4 * KB;
+#pragma unroll
+  for (int e = 0; e < per_thread<KB>(); ++e) {
+    const int idx = t + NT * e;
+    const int a = idx / KB, b = idx % KB;
+    const int i = TRANS ? b : a;  // local row
+    const int j = TRANS ? a : b;  // local column
+    const int64_t gi = row_of<FWD, KB>(r0, i), gj = row_of<FWD, KB>(r0, j);
+    TI w = TI(0);
+    if (j == i)
+      w = (UNIT || gi >= n) ? TI(1) : (TI)mat<T, TRANS>(A, lda, gi, gj);
+    else if (j < i && gi < n && gj < n)
+      w = (TI)mat<T, TRANS>(A, lda, gi, gj);
+    v[e] = w;
+  }
+}
+
+template <class TI, bool TRANS, int KB>
+__device__ __forceinline__ void invert_loaded(TI (&L)[KB][KB + 1], int t, const TI (&v)[per_thread<KB>()]);
+
 // Loads the diagonal block of block bi (rows r0 ..) in local order, converted
 // to TI, and inverts it in place (16 x 16 diagonal blocks, then two pairwise
 // merges: 40 barriers instead of the 128 of a column-by-column inverse at
@@ -147,36 +258,47 @@ __device__ __forceinline__ void merge_level(TI (&L)[KB][KB + 1], int t) {
 template <class T, class TI, bool FWD, bool TRANS, bool UNIT, int KB = kB>
 __device__ __forceinline__ void invert_diag(int64_t n, const T* A, int64_t lda, int64_t r0, TI (&L)[KB][KB + 1],
                                             int t) {
+  TI v[per_thread<KB>()];
+  load_diag<T, TI, FWD, TRANS, UNIT, KB>(n, A, lda, r0, t, v);
+  invert_loaded<TI, TRANS, KB>(L, t, v);
+}
+
+// Writes the entries load_diag gave this thread into L and inverts L in place.
+template <class TI, bool TRANS, int KB>
+__device__ __forceinline__ void invert_loaded(TI (&L)[KB][KB + 1], int t, const TI (&v)[per_thread<KB>()]) {
   constexpr int NT = 4 * KB;
-  for (int idx = t; idx < KB * KB; idx += NT) {
-    // NoTrans reads rows of A, Trans reads columns: keep lanes on consecutive addresses
+#pragma unroll
+  for (int e = 0; e < per_thread<KB>(); ++e) {
+    const int idx = t + NT * e;
     const int a = idx / KB, b = idx % KB;
-    const int i = TRANS ? b : a;  // local row
-    const int j = TRANS ? a : b;  // local column
-    const int64_t gi = row_of<FWD, KB>(r0, i), gj = row_of<FWD, KB>(r0, j);
-    TI v = TI(0);
-    if (j == i)
-      v = (UNIT || gi >= n) ? TI(1) : (TI)mat<T, TRANS>(A, lda, gi, gj);
-    else if (j < i && gi < n && gj < n)
-      v = (TI)mat<T, TRANS>(A, lda, gi, gj);
-    L[i][j] = v;
+    L[TRANS ? b : a][TRANS ? a : b] = v[e];
   }
   __syncthreads();
   // 1. The KB/16 diagonal 16 x 16 blocks by forward substitution, no
   //    barriers: thread 16 * blk + j computes column j of inv(D_blk) in
-  //    registers, X[i][j] = (delta_ij - sum_{k<i} D[i][k] X[k][j]) / D[i][i]
-  //    (so X[k][j] = 0 for k < j), reading D by broadcast from LDS; the
-  //    columns are written over D once every thread has read it.
+  //    registers, right-looking: once X[i][j] = s_i / D[i][i] is known it
+  //    updates the later rows' sums s_m -= D[m][i] X[i][j], so the
+  //    dependent path is one multiply-add and one multiply per row (the
+  //    reciprocals of the diagonal are taken first, one per thread and
+  //    passed round the 16 threads of the block; X[k][j] = 0 for k < j); D
+  //    is read by broadcast from LDS, and the columns are written over D
+  //    once every thread has read it.
   {
     TI xc[16];
     const int b = (t / 16) * 16, j = t % 16;
     if (t < KB) {
+      const TI rj = TI(1) / L[b + j][b + j];
+      TI rd[16];
+#pragma unroll
+      for (int i = 0; i < 16; ++i) rd[i] = __shfl(rj, i, 16);
+      TI sr[16];
+#pragma unroll
+      for (int i = 0; i < 16; ++i) sr[i] = i == j ? TI(1) : TI(0);
 #pragma unroll
       for (int i = 0; i < 16; ++i) {
-        TI s = i == j ? TI(1) : TI(0);
+        xc[i] = sr[i] * rd[i];
 #pragma unroll
-        for (int k = 0; k < i; ++k) s -= L[b + i][b + k] * xc[k];
-        xc[i] = s / L[b + i][b + i];
+        for (int m = i + 1; m < 16; ++m) sr[m] -= L[b + m][b + i] * xc[i];
       }
     }
     __syncthreads();
@@ -188,12 +310,52 @@ __device__ __forceinline__ void invert_diag(int64_t n, const T* A, int64_t lda, 
     __syncthreads();
   }
   // 2. Merge inverted diagonal blocks pairwise, 16 -> 32 -> ... -> KB:
-  //    inv([A 0; B C]) = [inv(A) 0; -inv(C) B inv(A), inv(C)], the product
-  //    formed in registers and written over B.  The sums run over whole
-  //    block rows / columns with compile-time bounds (the strictly upper
-  //    parts of inv(A) and inv(C) hold zeros), so they unroll and the LDS
-  //    reads overlap.
-  merge_level<TI, KB, 16>(L, t);
+  //    inv([A 0; B C]) = [inv(A) 0; -inv(C) B inv(A), inv(C)], written over
+  //    B.  At KB = 64 the products run on the matrix cores (merge_mfma);
+  //    otherwise they are formed in registers from LDS with compile-time
+  //    bounds (the strictly upper parts of inv(A) and inv(C) hold zeros).
+  if constexpr (KB == 64)
+    merge_mfma<TI>(L, t);
+  else
+    merge_level<TI, KB, 16>(L, t);
+}
+
+// G = inv(D) M on the matrix cores (v_mfma_*_16x16x4, FP32 or FP64), over
+// M in place: wave w forms the 16-column tile w of G for all four 16-row
+// tiles; inv(D) is lower triangular, so row tile I sums K tiles 0 .. I only
+// (40 MFMA steps per wave instead of 64).  Within a K tile, lane group lg
+// takes k = 16 K + 4 lg + j in step j (A and B agree on it, which is all the
+// product needs).  Called by all 256 threads; ends with a barrier.
+template <class T, int KB>
+__device__ __forceinline__ void fold_g(const T (&inv)[KB][KB + 1], T (*mb)[mb_stride<T, KB>()], int t) {
+  static_assert(KB == 64, "four waves, one 16-column tile each");
+  using MF = gemm::Mfma<T>;
+  const int J = t / 64, lane = t % 64, lm = lane % 16, lg = lane / 16;
+  typename MF::acc acc[4];
+#pragma unroll
+  for (int I = 0; I < 4; ++I)
+#pragma unroll
+    for (int q = 0; q < 4; ++q) acc[I][q] = T(0);
+#pragma unroll
+  for (int K = 0; K < 4; ++K) {
+    T b[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) b[j] = mb[16 * K + 4 * lg + j][16 * J + lm];
+#pragma unroll
+    for (int I = K; I < 4; ++I) {
+      T a[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) a[j] = inv[16 * I + lm][16 * K + 4 * lg + j];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) acc[I] = MF::step(a[j], b[j], acc[I]);
+    }
+  }
+  __syncthreads();  // every wave has read mb
+#pragma unroll
+  for (int I = 0; I < 4; ++I)
+#pragma unroll
+    for (int q = 0; q < 4; ++q) mb[16 * I + MF::out_row(lg, q)][16 * J + lm] = acc[I][q];
+  __syncthreads();
 }
 
 // One processing step p of the solve; called by all 4 * KB threads of a
@@ -204,34 +366,58 @@ __device__ __forceinline__ void invert_diag(int64_t n, const T* A, int64_t lda, 
 template <class T, bool FWD, bool TRANS, bool UNIT, int KB = kB, bool PF = true>
 __device__ __forceinline__ void step(int p, int64_t n, const T* __restrict__ A, int64_t lda, T* x, unsigned* pub,
                                      unsigned marker, unsigned* err, Smem<T, T, KB>& sm,
-                                     T (*mb)[mb_stride<T, KB>()] = nullptr) {
+                                     T (*mb)[mb_stride<T, KB>()] = nullptr, T (*mb2)[mb_stride<T, KB>()] = nullptr) {
   constexpr int CW = KB / 4;  // columns per thread in the block mat-vecs
   const int nb = (int)((n + KB - 1) / KB);
   const int t = threadIdx.x;
   const int64_t bi = FWD ? p : nb - 1 - p;
   const int64_t r0 = bi * KB;
-  // With PF, this block's right-hand side goes to vs (local order) before
-  // anything waits (only this workgroup writes these rows of x, at the end
-  // of the step); invert_diag's barriers order it before its use.
+  // With PF, this block's right-hand side (to vs, local order; only this
+  // workgroup writes these rows of x, at the end of the step) and
+  // M(this block, block p-1) (to mb: mb[k][c] = M(row_of(k), column c of
+  // block p-1)) are read before anything waits, together with the diagonal
+  // block: every load is issued before the first is used, so the three
+  // cost one memory round trip; the inversion's first barrier orders the
+  // LDS writes before their use.
+  constexpr int PT = per_thread<KB>();
+  T rhs = T(0);
   if (PF && t < KB) {
     const int64_t g = row_of<FWD, KB>(r0, t);
-    sm.vs[t] = g < n ? x[g] : T(0);
+    if (g < n) rhs = x[g];
   }
-  // PF: M(this block, block p-1) is staged in mb before the inversion, so
-  // its loads overlap the inversion's (whose first barrier orders them);
-  // mb[k][c] = M(row_of(k), column c of block p-1).
-  if constexpr (PF) {
-    if (p > 0) {
-      const int64_t cb0 = (FWD ? (int64_t)(p - 1) : (int64_t)(nb - p)) * KB;
-      for (int idx = t; idx < KB * KB; idx += 4 * KB) {
-        const int a = idx / KB, c2 = idx % KB;
-        const int k = TRANS ? c2 : a, c = TRANS ? a : c2;  // lanes on consecutive addresses
-        const int64_t gk = row_of<FWD, KB>(r0, k), gc = cb0 + c;
-        mb[k][c] = (gk < n && gc < n) ? mat<T, TRANS>(A, lda, gk, gc) : T(0);
-      }
+  // M(this block, block p - d) for d = 1, 2 (the blocks folded into G1, G2)
+  auto load_prev = [&](int d, T (&m)[PT]) {
+    const int64_t cb0 = (FWD ? (int64_t)(p - d) : (int64_t)(nb - 1 - p + d)) * KB;
+#pragma unroll
+    for (int e = 0; e < PT; ++e) {
+      const int idx = t + 4 * KB * e;
+      const int a = idx / KB, c2 = idx % KB;
+      const int k = TRANS ? c2 : a, c = TRANS ? a : c2;  // lanes on consecutive addresses
+      const int64_t gk = row_of<FWD, KB>(r0, k), gc = cb0 + c;
+      m[e] = (gk < n && gc < n) ? mat<T, TRANS>(A, lda, gk, gc) : T(0);
     }
+  };
+  auto stage_prev = [&](const T (&m)[PT], T (*dst)[mb_stride<T, KB>()]) {
+#pragma unroll
+    for (int e = 0; e < PT; ++e) {
+      const int idx = t + 4 * KB * e;
+      const int a = idx / KB, c2 = idx % KB;
+      dst[TRANS ? c2 : a][TRANS ? a : c2] = m[e];
+    }
+  };
+  T mv[PT], mv2[PT];
+  if constexpr (PF) {
+    if (p > 0) load_prev(1, mv);
+    if (p > 1) load_prev(2, mv2);
   }
-  invert_diag<T, T, FWD, TRANS, UNIT, KB>(n, A, lda, r0, sm.inv, t);
+  T dv[PT];
+  load_diag<T, T, FWD, TRANS, UNIT, KB>(n, A, lda, r0, t, dv);
+  if (PF && t < KB) sm.vs[t] = rhs;
+  if constexpr (PF) {
+    if (p > 0) stage_prev(mv, mb);
+    if (p > 1) stage_prev(mv2, mb2);
+  }
+  invert_loaded<T, TRANS, KB>(sm.inv, t, dv);
   // wave 0's rows of inv(D) and G for the end of the step (PF)
   constexpr bool kInvInRegs = sizeof(T) == 4;
   T ir[kInvInRegs ? KB : 1];
@@ -249,39 +435,25 @@ __device__ __forceinline__ void step(int p, int64_t n, const T* __restrict__ A, 
   const int cq = TRANS ? t / KB : t % 4;
   const int64_t gi = r0 + r;
   const int i4 = t / 4, q4 = t % 4;  // row (local order) and column quarter of the inv(D) products
-  // PF: the previous block's update is folded into inv(D) ahead of time,
-  // G = inv(D) M(this block, block p-1), so once x_{p-1} appears the step is
-  // one block mat-vec x = y0 - G x_{p-1} (y0 = inv(D)(b - the other blocks)).
-  // mb holds M(row_of(k), block p-1 column c); this thread forms
-  // G[i4][q4*CW .. +CW].
-  T gq[CW];
-  const int pp = PF && p > 0 ? p - 1 : p;  // blocks the panel loop covers
+  // PF: the two previous blocks' updates are folded into inv(D) ahead of
+  // time, G1 = inv(D) M(this block, block p-1) and G2 = inv(D) M(this block,
+  // block p-2), so the panel loop covers blocks 0 .. p-3 and
+  // x_block = y0 - G2 x_{p-2} - G1 x_{p-1} (y0 = inv(D)(b - the other
+  // blocks)): the panel's last block and y0 are a whole step ahead of the
+  // chain, G2 x_{p-2} half a step, and once x_{p-1} appears the step is one
+  // block mat-vec.  mb / mb2 hold M(row_of(k), column c of block p-1 / p-2);
+  // the products run on the matrix cores (fold_g) and go over mb / mb2;
+  // wave 0 keeps its rows of G1 in registers and reads G2 from LDS.
+  const int pp = PF ? (p > 1 ? p - 2 : 0) : p;  // blocks the panel loop covers
   if constexpr (PF) {
     if (p > 0) {
-#pragma unroll
-      for (int e = 0; e < CW; ++e) gq[e] = T(0);
-      constexpr int VN = 16 / (int)sizeof(T);
-#pragma unroll 8
-      for (int k = 0; k < KB; ++k) {  // inv(D) is lower triangular in local order: zeros above
-        const T iv = sm.inv[i4][k];
-        const VecN<T, VN>* row = reinterpret_cast<const VecN<T, VN>*>(&mb[k][q4 * CW]);
-#pragma unroll
-        for (int v = 0; v < CW / VN; ++v) {
-          const VecN<T, VN> m4 = row[v];
-#pragma unroll
-          for (int e = 0; e < VN; ++e) gq[v * VN + e] += iv * m4.v[e];
-        }
-      }
-      // G over mb (every thread has read mb), then wave 0 takes its rows
-      __syncthreads();
-#pragma unroll
-      for (int e = 0; e < CW; ++e) mb[i4][q4 * CW + e] = gq[e];
-      __syncthreads();
+      fold_g<T, KB>(sm.inv, mb, t);
       if (t < KB) {
 #pragma unroll
         for (int c = 0; c < KB; ++c) gr[c] = mb[t][c];
       }
     }
+    if (p > 1) fold_g<T, KB>(sm.inv, mb2, t);
   }
   // This thread's CW matrix entries of block q, one bounds test for the
   // whole run when it is inside the matrix (so NoTrans loads merge into
@@ -355,7 +527,7 @@ __device__ __forceinline__ void step(int p, int64_t n, const T* __restrict__ A, 
     const int64_t g = row_of<FWD, KB>(r0, i4);
     if (q4 == 0 && g < n) publish_value<T>(g, s, x, pub, marker);
   } else {
-    static_assert(KB <= 64, "the chain part of a step runs on one wave, one row per lane");
+    static_assert(KB == 64, "the chain part of a step runs on one wave, one row per lane");
     // v = rhs - panel sums (vs holds the rhs in local order): NoTrans keeps
     // a row's four partial sums in adjacent lanes, so two shuffles reduce
     // them; Trans spreads them over the waves, so they meet in LDS.
@@ -378,18 +550,27 @@ __device__ __forceinline__ void step(int p, int64_t n, const T* __restrict__ A, 
       }
     }
     __syncthreads();
-    // The rest on wave 0, lane i = local row i, with its rows of inv(D) (FP32:
-    // in registers since the inversion) and G in registers: y0 = inv(D) v,
-    // then, once x_{p-1} is published, x = y0 - G x_{p-1}; the wave passes
-    // x_{p-1} through LDS with no workgroup barrier on the chain (LDS
-    // operations of one wave complete in order).
+    // The rest on wave 0, lane i = local row i: y0 = inv(D) v (FP32: its
+    // rows of inv(D) in registers since the inversion), then
+    // y0 - G2 x_{p-2} (G2 and x_{p-2} from LDS), then, once x_{p-1} is
+    // published, x = y0 - G2 x_{p-2} - G1 x_{p-1} with the rows of G1 in
+    // registers.  x passes through LDS with no workgroup barrier on the
+    // chain (LDS operations of one wave complete in order).  (Measured: a
+    // second wave forming y0, or y0 - G2 x_{p-2}, and handing it over
+    // through LDS and a flag gave no shorter step: the step is the
+    // publication's round trip plus the G1 product.)
     if (t < KB) {
       constexpr int VN = 16 / (int)sizeof(T);
       const VecN<T, VN>* vv = reinterpret_cast<const VecN<T, VN>*>(sm.vs);
+      // every LDS read issued before the first product (one LDS latency,
+      // not one per pair of reads)
+      VecN<T, VN> vr[KB / VN];
+#pragma unroll
+      for (int c = 0; c < KB / VN; ++c) vr[c] = vv[c];
       T y[4] = {T(0), T(0), T(0), T(0)};
 #pragma unroll
       for (int c = 0; c < KB / VN; ++c) {
-        const VecN<T, VN> v4 = vv[c];
+        const VecN<T, VN> v4 = vr[c];
 #pragma unroll
         for (int e = 0; e < VN; ++e) {
           const int k = c * VN + e;
@@ -398,19 +579,44 @@ __device__ __forceinline__ void step(int p, int64_t n, const T* __restrict__ A, 
         }
       }
       T xv = (y[0] + y[1]) + (y[2] + y[3]);
-      if (p > 0) {
-        const int64_t bj = FWD ? (int64_t)(p - 1) : (int64_t)(nb - p);
-        const int64_t gx = bj * KB + t;
-        T* xs = sm.xs[0];
-        xs[t] = gx < n ? tagged::read<T>(pub, gx, marker, err) : T(0);
+      if (p > 1) {
+        const int64_t g2 = (FWD ? (int64_t)(p - 2) : (int64_t)(nb - 1 - p + 2)) * KB + t;
+        T* xs = sm.xs[1];
+        xs[t] = g2 < n ? tagged::read<T>(pub, g2, marker, err) : T(0);
         __builtin_amdgcn_wave_barrier();
         const VecN<T, VN>* x4 = reinterpret_cast<const VecN<T, VN>*>(xs);
+        const VecN<T, VN>* g4 = reinterpret_cast<const VecN<T, VN>*>(mb2[t]);
         T u[4] = {T(0), T(0), T(0), T(0)};
 #pragma unroll
         for (int c = 0; c < KB / VN; ++c) {
-          const VecN<T, VN> xx = x4[c];
+          const VecN<T, VN> xx = x4[c], gg = g4[c];
 #pragma unroll
-          for (int e = 0; e < VN; ++e) u[(c * VN + e) % 4] += gr[c * VN + e] * xx.v[e];
+          for (int e = 0; e < VN; ++e) u[(c * VN + e) % 4] += gg.v[e] * xx.v[e];
+        }
+        xv -= (u[0] + u[1]) + (u[2] + u[3]);
+      }
+      if (p > 0) {
+        const int64_t bj = FWD ? (int64_t)(p - 1) : (int64_t)(nb - p);
+        const int64_t g1 = bj * KB + t;
+        T* xs = sm.xs[0];
+        xs[t] = g1 < n ? tagged::read<T>(pub, g1, marker, err) : T(0);
+        __builtin_amdgcn_wave_barrier();
+        const VecN<T, VN>* x4 = reinterpret_cast<const VecN<T, VN>*>(xs);
+        // x read in chunks whose loads are all issued before the products
+        // (FP64: four, so the rows of G1 and the chunk fit the VGPRs)
+        constexpr int CH = sizeof(T) == 4 ? KB / VN : KB / VN / 4;
+        T u[4] = {T(0), T(0), T(0), T(0)};
+#pragma unroll
+        for (int c0 = 0; c0 < KB / VN; c0 += CH) {
+          VecN<T, VN> xr[CH];
+#pragma unroll
+          for (int c = 0; c < CH; ++c) xr[c] = x4[c0 + c];
+#pragma unroll
+          for (int c = 0; c < CH; ++c) {
+            const VecN<T, VN> xx = xr[c];
+#pragma unroll
+            for (int e = 0; e < VN; ++e) u[((c0 + c) * VN + e) % 4] += gr[(c0 + c) * VN + e] * xx.v[e];
+          }
         }
         xv -= (u[0] + u[1]) + (u[2] + u[3]);
       }
