@@ -143,74 +143,88 @@ __device__ __forceinline__ void merge_level(TI (&L)[KB][KB + 1], int t) {
   }
 }
 
-// The merges of invert_diag at KB = 64 on the matrix cores (v_mfma_*_16x16x4;
-// 256 threads).  Step j of a 16-deep product lets lane group lg take
-// k = out_row(lg, j): that is the row of the accumulator layout this lane
-// holds in register j, so a product left in registers is the B operand of
-// the next product as it stands.
-//   16 -> 32: waves 0 and 1 each merge one pair (rows 32 w ..): P = B inv(A),
-//             then -inv(C) P from registers, written over B (no other wave
-//             reads that B).
-//   32 -> 64: wave w forms 16 x 16 tile (I, J) = (w / 2, w % 2) of
-//             P = B inv(A) (K tiles J .. 1: inv(A) is lower triangular), P
-//             goes over B, then -(inv(C) P) tile (I, J) from K tiles 0 .. I.
-template <class TI>
-__device__ __forceinline__ void merge_mfma(TI (&L)[64][65], int t) {
-  using MF = gemm::Mfma<TI>;
-  using Acc = typename MF::acc;
-  const int w = t / 64, lane = t % 64, lm = lane % 16, lg = lane / 16;
-  // 16 -> 32
-  if (w < 2) {
-    const int o = 32 * w;  // A = L[o ..][o ..], B = L[o+16 ..][o ..], C = L[o+16 ..][o+16 ..]
-    Acc P, Q;
+// The merges of invert_diag on the matrix cores (v_mfma_*_16x16x4; KB = 64
+// or 128 with 4 KB threads, i.e. KB / 16 waves).  Level SZ (16 -> 32 -> ..
+// -> KB) merges KB / (2 SZ) pairs [A 0; B C] of inverted SZ x SZ blocks:
+// P = B inv(A) (inv(A) lower triangular: K tiles J .. SZ/16 - 1), P goes
+// over B, then -(inv(C) P) (K tiles 0 .. I) goes over P.  The 16 x 16
+// output tiles of a level are dealt to the waves round-robin.  Step j of a
+// 16-deep product lets lane group lg take k = out_row(lg, j): the row of
+// the accumulator layout this lane holds in register j, so the tile of P a
+// wave formed itself is the B operand of its diagonal K tile as it stands.
+template <class TI, int KB, int SZ>
+__device__ __forceinline__ void merge_mfma(TI (&L)[KB][KB + 1], int t) {
+  if constexpr (SZ < KB) {
+    using MF = gemm::Mfma<TI>;
+    using Acc = typename MF::acc;
+    constexpr int NW = KB / 16;                  // waves
+    constexpr int TS = SZ / 16;                  // tiles per side of a block
+    constexpr int NTILE = (KB / (2 * SZ)) * TS * TS;
+    constexpr int PER = (NTILE + NW - 1) / NW;   // tiles per wave
+    const int w = t / 64, lane = t % 64, lm = lane % 16, lg = lane / 16;
+    Acc P[PER];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) P[q] = Q[q] = TI(0);
+    for (int e = 0; e < PER; ++e) {
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int k = MF::out_row(lg, j);
-      P = MF::step(L[o + 16 + lm][o + k], L[o + k][o + lm], P);
+      for (int q = 0; q < 4; ++q) P[e][q] = TI(0);
+      const int tau = w + NW * e;
+      if (tau < NTILE) {
+        const int o = 2 * SZ * (tau / (TS * TS)), I = (tau % (TS * TS)) / TS, J = tau % TS;
+#pragma unroll
+        for (int K = 0; K < TS; ++K) {
+          if (K < J) continue;
+#pragma unroll
+          for (int j = 0; j < 4; ++j) {
+            const int k = 16 * K + MF::out_row(lg, j);
+            P[e] = MF::step(L[o + SZ + 16 * I + lm][o + k], L[o + k][o + 16 * J + lm], P[e]);
+          }
+        }
+      }
     }
+    __syncthreads();  // every wave has read B
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int k = MF::out_row(lg, j);
-      Q = MF::step(L[o + 16 + lm][o + 16 + k], P[j], Q);
+    for (int e = 0; e < PER; ++e) {
+      const int tau = w + NW * e;
+      if (tau < NTILE) {
+        const int o = 2 * SZ * (tau / (TS * TS)), I = (tau % (TS * TS)) / TS, J = tau % TS;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) L[o + SZ + 16 * I + MF::out_row(lg, q)][o + 16 * J + lm] = P[e][q];
+      }
     }
+    __syncthreads();
+    Acc Q[PER];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) L[o + 16 + MF::out_row(lg, q)][o + lm] = -Q[q];
+    for (int e = 0; e < PER; ++e) {
+#pragma unroll
+      for (int q = 0; q < 4; ++q) Q[e][q] = TI(0);
+      const int tau = w + NW * e;
+      if (tau < NTILE) {
+        const int o = 2 * SZ * (tau / (TS * TS)), I = (tau % (TS * TS)) / TS, J = tau % TS;
+#pragma unroll
+        for (int K = 0; K < TS; ++K) {
+          if (K > I) continue;
+#pragma unroll
+          for (int j = 0; j < 4; ++j) {
+            const int k = 16 * K + MF::out_row(lg, j);
+            const TI bv = K == I ? P[e][j] : L[o + SZ + k][o + 16 * J + lm];
+            Q[e] = MF::step(L[o + SZ + 16 * I + lm][o + SZ + k], bv, Q[e]);
+          }
+        }
+      }
+    }
+    __syncthreads();  // every wave has read P
+#pragma unroll
+    for (int e = 0; e < PER; ++e) {
+      const int tau = w + NW * e;
+      if (tau < NTILE) {
+        const int o = 2 * SZ * (tau / (TS * TS)), I = (tau % (TS * TS)) / TS, J = tau % TS;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) L[o + SZ + 16 * I + MF::out_row(lg, q)][o + 16 * J + lm] = -Q[e][q];
+      }
+    }
+    __syncthreads();
+    merge_mfma<TI, KB, 2 * SZ>(L, t);
   }
-  __syncthreads();
-  // 32 -> 64: A = L[0 ..][0 ..], B = L[32 ..][0 ..], C = L[32 ..][32 ..]
-  const int I = w / 2, J = w % 2;
-  Acc P, Q;
-#pragma unroll
-  for (int q = 0; q < 4; ++q) P[q] = Q[q] = TI(0);
-#pragma unroll
-  for (int K = 0; K < 2; ++K) {
-    if (K < J) continue;
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int k = 16 * K + MF::out_row(lg, j);
-      P = MF::step(L[32 + 16 * I + lm][k], L[k][16 * J + lm], P);
-    }
-  }
-  __syncthreads();  // every wave has read B
-#pragma unroll
-  for (int q = 0; q < 4; ++q) L[32 + 16 * I + MF::out_row(lg, q)][16 * J + lm] = P[q];
-  __syncthreads();
-#pragma unroll
-  for (int K = 0; K < 2; ++K) {
-    if (K > I) continue;
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int k = 16 * K + MF::out_row(lg, j);
-      const TI bv = K == I ? P[j] : L[32 + k][16 * J + lm];
-      Q = MF::step(L[32 + 16 * I + lm][32 + k], bv, Q);
-    }
-  }
-  __syncthreads();  // every wave has read P
-#pragma unroll
-  for (int q = 0; q < 4; ++q) L[32 + 16 * I + MF::out_row(lg, q)][16 * J + lm] = -Q[q];
-  __syncthreads();
 }
 
 // Entries of a KB x KB block each of the 4 * KB threads of a block step
@@ -311,38 +325,39 @@ __device__ __forceinline__ void invert_loaded(TI (&L)[KB][KB + 1], int t, const 
   }
   // 2. Merge inverted diagonal blocks pairwise, 16 -> 32 -> ... -> KB:
   //    inv([A 0; B C]) = [inv(A) 0; -inv(C) B inv(A), inv(C)], written over
-  //    B.  At KB = 64 the products run on the matrix cores (merge_mfma);
+  //    B.  At KB = 64 and 128 the products run on the matrix cores (merge_mfma);
   //    otherwise they are formed in registers from LDS with compile-time
   //    bounds (the strictly upper parts of inv(A) and inv(C) hold zeros).
-  if constexpr (KB == 64)
-    merge_mfma<TI>(L, t);
+  if constexpr (KB == 64 || KB == 128)
+    merge_mfma<TI, KB, 16>(L, t);
   else
     merge_level<TI, KB, 16>(L, t);
 }
 
 // G = inv(D) M on the matrix cores (v_mfma_*_16x16x4, FP32 or FP64), over
-// M in place: wave w forms the 16-column tile w of G for all four 16-row
-// tiles; inv(D) is lower triangular, so row tile I sums K tiles 0 .. I only
-// (40 MFMA steps per wave instead of 64).  Within a K tile, lane group lg
-// takes k = 16 K + 4 lg + j in step j (A and B agree on it, which is all the
-// product needs).  Called by all 256 threads; ends with a barrier.
+// M in place: with KB / 16 waves, wave w forms the 16-column tile w of G for
+// all KB / 16 row tiles; inv(D) is lower triangular, so row tile I sums K
+// tiles 0 .. I only.  Within a K tile, lane group lg takes k = 16 K + 4 lg + j
+// in step j (A and B agree on it, which is all the product needs).  Called
+// by all 4 KB threads; ends with a barrier.
 template <class T, int KB>
 __device__ __forceinline__ void fold_g(const T (&inv)[KB][KB + 1], T (*mb)[mb_stride<T, KB>()], int t) {
-  static_assert(KB == 64, "four waves, one 16-column tile each");
+  static_assert(KB == 64 || KB == 128, "KB / 16 waves, one 16-column tile each");
+  constexpr int RT = KB / 16;
   using MF = gemm::Mfma<T>;
   const int J = t / 64, lane = t % 64, lm = lane % 16, lg = lane / 16;
-  typename MF::acc acc[4];
+  typename MF::acc acc[RT];
 #pragma unroll
-  for (int I = 0; I < 4; ++I)
+  for (int I = 0; I < RT; ++I)
 #pragma unroll
     for (int q = 0; q < 4; ++q) acc[I][q] = T(0);
 #pragma unroll
-  for (int K = 0; K < 4; ++K) {
+  for (int K = 0; K < RT; ++K) {
     T b[4];
 #pragma unroll
     for (int j = 0; j < 4; ++j) b[j] = mb[16 * K + 4 * lg + j][16 * J + lm];
 #pragma unroll
-    for (int I = K; I < 4; ++I) {
+    for (int I = K; I < RT; ++I) {
       T a[4];
 #pragma unroll
       for (int j = 0; j < 4; ++j) a[j] = inv[16 * I + lm][16 * K + 4 * lg + j];
@@ -352,7 +367,7 @@ __device__ __forceinline__ void fold_g(const T (&inv)[KB][KB + 1], T (*mb)[mb_st
   }
   __syncthreads();  // every wave has read mb
 #pragma unroll
-  for (int I = 0; I < 4; ++I)
+  for (int I = 0; I < RT; ++I)
 #pragma unroll
     for (int q = 0; q < 4; ++q) mb[16 * I + MF::out_row(lg, q)][16 * J + lm] = acc[I][q];
   __syncthreads();
