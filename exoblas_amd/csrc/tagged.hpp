@@ -73,6 +73,36 @@ __device__ __forceinline__ T read(unsigned* pub, int64_t g, unsigned marker, uns
   }
 }
 
+// Value g0 of pub0 and g1 of pub1 (either may be skipped: v0 / v1 false
+// reads as 0): every word is loaded before any is checked, so when both are
+// already published the pair costs one memory round trip, not two.
+template <class T>
+__device__ __forceinline__ void read2(unsigned* pub0, int64_t g0, bool v0, unsigned* pub1, int64_t g1, bool v1,
+                                      unsigned marker, unsigned* err, T& x0, T& x1) {
+  constexpr int W = sizeof(T) / 4;
+  uint64_t* a = reinterpret_cast<uint64_t*>(pub0) + W * g0;
+  uint64_t* b = reinterpret_cast<uint64_t*>(pub1) + W * g1;
+  uint64_t wa[W], wb[W];
+#pragma unroll
+  for (int h = 0; h < W; ++h) {
+    if (v0) wa[h] = __hip_atomic_load(a + h, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (v1) wb[h] = __hip_atomic_load(b + h, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  unsigned la[W], lb[W];
+#pragma unroll
+  for (int h = 0; h < W; ++h) {
+    la[h] = v0 ? ((wa[h] >> 32) == marker ? (unsigned)wa[h] : poll_word(a + h, marker, err)) : 0u;
+    lb[h] = v1 ? ((wb[h] >> 32) == marker ? (unsigned)wb[h] : poll_word(b + h, marker, err)) : 0u;
+  }
+  if constexpr (W == 1) {
+    x0 = __uint_as_float(la[0]);
+    x1 = __uint_as_float(lb[0]);
+  } else {
+    x0 = __longlong_as_double((long long)(((uint64_t)la[1] << 32) | la[0]));
+    x1 = __longlong_as_double((long long)(((uint64_t)lb[1] << 32) | lb[0]));
+  }
+}
+
 // Values g0, g0 + stride, ..., g0 + (K-1) stride of `pub` (those with index
 // k < count) into v[k]: all words are loaded before any is checked, so the
 // K reads overlap instead of forming a chain of memory round trips.

@@ -1,18 +1,22 @@
 // Mixed-precision strsv with iterative refinement.
 //
-// update = 32 (FP32-stored updates): classical refinement around the plain
-// FP32 solve.
-//   x_hi = solve(b)                 the FP32 look-back solve (trsv.hip)
-//   repeat `refine` times:
-//     r = b - op(A) (x_hi + x_lo)   FP64 accumulation over the FP32 matrix,
-//                                   b read from the caller's vector, r stored
-//                                   in FP32 as the next right-hand side
-//     d = solve(r);  x_lo += d      (x_lo kept in FP32: it is ~eps |x|)
-//   x = x_hi + x_lo                 added in FP64, rounded once to FP32
-// The solution is carried as the FP32 pair (x_hi, x_lo), so no FP64 vector
-// is stored; each solve is the plain kernel's chain (the previous block
-// folded into the inverse ahead of time), so a refined solve costs two plain
-// solves plus a residual pass.
+// update = 32 (FP32-stored updates):
+//   refine = 1: one launch (trsv_refined_kernel): the FP32 look-back solve
+//   and, trailing it by a few microseconds, the correction solve
+//   d = solve(b - op(A) x) with an FP64 residual, sharing each block's
+//   inverse and folds; x + d added in FP64 and rounded once (8192: 165 us
+//   against 115 us for the plain solve and 275 us for the three-launch
+//   form below).
+//   refine > 1: classical refinement around the plain FP32 solve,
+//     x_hi = solve(b)                 the FP32 look-back solve (trsv.hip)
+//     repeat `refine` times:
+//       r = b - op(A) (x_hi + x_lo)   FP64 accumulation over the FP32 matrix,
+//                                     b read from the caller's vector, r
+//                                     stored in FP32 as the next right-hand side
+//       d = solve(r);  x_lo += d      (x_lo kept in FP32: it is ~eps |x|)
+//     x = x_hi + x_lo                 added in FP64, rounded once to FP32
+//   The solution is carried as the FP32 pair (x_hi, x_lo), so no FP64
+//   vector is stored.
 //
 // update = 16 (BF16-stored updates):
 //   solve   op(A) x0 = b   off-diagonal block updates in FP32 arithmetic on a
@@ -256,6 +260,369 @@ __global__ void to_bf16_kernel(int64_t rows, int64_t cols, const float* A, int64
     out[i * cols + j] = (__bf16)A[i * lda + j];
 }
 
+// One refinement fused into the solve (FP32, update = 32, refine = 1): the
+// look-back solve of trsv_core.hpp (64-row blocks, G1 = inv(D) M(p, p-1) and
+// G2 = inv(D) M(p, p-2) folded ahead of time) runs two chains in one
+// launch.  Block p first publishes x_p as the plain solve does; then forms
+// its residual rows r_p = b_p - sum_{j <= p} M(p, j) x_j in FP64 (the panel
+// part accumulated in the same loop that consumes x_j, from the same loaded
+// entries; the last three blocks from raw entries loaded after the panel),
+// and solves the correction chain d_p = inv(D)(r_p - sum_{j <= p-3} M(p, j)
+// d_j) - G2 d_{p-2} - G1 d_{p-1} with the same inverse and folds; the panel
+// loop consumes d_{q-2} beside x_q (each panel block loaded once: a ring of
+// the last four blocks' entries), so the correction chain trails the
+// solve by about a step and the refined solve costs about one plain solve.
+// The result x = x_p + d_p is added in FP64 and rounded once.  x_j and d_j
+// are published as tagged words in two areas of the stream's flag space.
+template <bool FWD, bool TRANS, bool UNIT>
+__global__ __launch_bounds__(kNT) void trsv_refined_kernel(int64_t n, const float* __restrict__ A, int64_t lda,
+                                                           float* x, int64_t incx, unsigned* flags, unsigned* gen,
+                                                           unsigned* err) {
+  using trsv::VecN;
+  constexpr int KB = kB, CW = KB / 4, PT = trsv::per_thread<KB>(), VN = 4;
+  constexpr int MS = trsv::mb_stride<float, KB>();
+  __shared__ __align__(16) trsv::Smem<float> sm;
+  __shared__ __align__(16) float mb[KB][MS];   // M(p, p-1), then G1
+  __shared__ __align__(16) float mb2[KB][MS];  // M(p, p-2), then G2
+  __shared__ __align__(16) float xd[2][KB];    // d blocks of the panel
+  __shared__ __align__(16) float xt[4][KB];    // the panel's last d blocks
+  __shared__ __align__(16) float xp[KB];       // x_p, natural order within the block
+  __shared__ __align__(16) float bs[KB];       // b, natural order
+  __shared__ __align__(16) float vd[KB];       // r_p - d panel, local order
+  __shared__ double rred[4][KB];
+  // raw M(p, p-1), M(p, p-2) (rows in local order, natural columns) and D
+  // (local order), kept for the residual
+  __shared__ __align__(16) float rb1[KB][MS];
+  __shared__ __align__(16) float rb2[KB][MS];
+  __shared__ float rd[KB][KB + 1];
+  __shared__ __align__(16) float tail0[4][KB][CW];  // wave 0's entries of the panel's last blocks
+  __shared__ int s_p;
+  __shared__ unsigned s_marker;
+  const int nb = (int)((n + KB - 1) / KB);
+  const int t = threadIdx.x;
+  if (t == 0) {
+    unsigned marker;
+    s_p = (int)tagged::generation_ticket(flags, (unsigned)nb, gen, &marker);
+    s_marker = marker;
+  }
+  __syncthreads();
+  if (s_p >= nb) return;  // only if launches sharing the stream's ticket overlapped
+  const int p = s_p;
+  const unsigned marker = s_marker;
+  unsigned* pubx = flags + kFlagBase;
+  unsigned* pubd = pubx + trsv::pub_words<float>(n);
+  const int64_t bi = FWD ? p : nb - 1 - p;
+  const int64_t r0 = bi * KB;
+  auto blk = [&](int q) -> int64_t { return FWD ? q : nb - 1 - q; };  // natural block of step q
+  // right-hand side, M(p, p-1), M(p, p-2) and the diagonal block: every load
+  // issued before the first is used
+  float rhs = 0.f;
+  if (t < KB) {
+    const int64_t g = r0 + t;  // natural order
+    if (g < n) rhs = x[g * incx];
+  }
+  auto load_prev = [&](int d, float (&m)[PT]) {
+    const int64_t cb0 = blk(p - d) * KB;
+#pragma unroll
+    for (int e = 0; e < PT; ++e) {
+      const int idx = t + 4 * KB * e;
+      const int a = idx / KB, c2 = idx % KB;
+      const int k = TRANS ? c2 : a, c = TRANS ? a : c2;
+      const int64_t gk = trsv::row_of<FWD, KB>(r0, k), gc = cb0 + c;
+      m[e] = (gk < n && gc < n) ? trsv::mat<float, TRANS>(A, lda, gk, gc) : 0.f;
+    }
+  };
+  auto stage_prev = [&](const float (&m)[PT], float (*dst)[MS]) {
+#pragma unroll
+    for (int e = 0; e < PT; ++e) {
+      const int idx = t + 4 * KB * e;
+      const int a = idx / KB, c2 = idx % KB;
+      dst[TRANS ? c2 : a][TRANS ? a : c2] = m[e];
+    }
+  };
+  float mv[PT], mv2[PT], dv[PT];
+  if (p > 0) load_prev(1, mv);
+  if (p > 1) load_prev(2, mv2);
+  trsv::load_diag<float, float, FWD, TRANS, UNIT, KB>(n, A, lda, r0, t, dv);
+  if (t < KB) {
+    bs[t] = rhs;
+    sm.vs[FWD ? t : KB - 1 - t] = rhs;  // local order
+  }
+  if (p > 0) {
+    stage_prev(mv, mb);
+    stage_prev(mv, rb1);
+  }
+  if (p > 1) {
+    stage_prev(mv2, mb2);
+    stage_prev(mv2, rb2);
+  }
+#pragma unroll
+  for (int e = 0; e < PT; ++e) {  // load_diag's (i, j) for this thread's entry e
+    const int idx = t + 4 * KB * e;
+    const int a = idx / KB, c2 = idx % KB;
+    rd[TRANS ? c2 : a][TRANS ? a : c2] = dv[e];
+  }
+  trsv::invert_loaded<float, TRANS, KB>(sm.inv, t, dv);
+  if (p > 0) trsv::fold_g<float, KB>(sm.inv, mb, t);
+  if (p > 1) trsv::fold_g<float, KB>(sm.inv, mb2, t);
+  // the panel: NoTrans 4 threads per row, CW consecutive columns each; Trans
+  // one column of KB rows per KB threads, CW columns per group
+  const int r = TRANS ? t % KB : t / 4;
+  const int cq = TRANS ? t / KB : t % 4;
+  const int64_t gi = r0 + r;
+  auto load_m = [&](int q, float (&m)[CW]) {
+    const int64_t c0 = blk(q) * KB + cq * CW;
+    if (gi < n && c0 + CW <= n) {
+#pragma unroll
+      for (int e = 0; e < CW; ++e) m[e] = trsv::mat<float, TRANS>(A, lda, gi, c0 + e);
+    } else {
+#pragma unroll
+      for (int e = 0; e < CW; ++e) m[e] = (gi < n && c0 + e < n) ? trsv::mat<float, TRANS>(A, lda, gi, c0 + e) : 0.f;
+    }
+  };
+  const int pp = p > 1 ? p - 2 : 0;  // panel blocks 0 .. p-3
+  // The panel loop consumes d_{q-LAG} beside x_q (a second stream of the
+  // same entries, LAG blocks behind): the correction chain trails the solve
+  // by a few microseconds (a step of it starts after its block's x and
+  // residual), and a lag of four blocks keeps the solve's panel from
+  // waiting on it.
+  // Wave 0 polls before it issues its own loads: a poll waits for every
+  // load its wave has in flight.
+  constexpr int LAG = 4;
+  const int j0 = pp > LAG ? pp - LAG : 0;  // the panel's last LAG blocks
+  float acc = 0.f, accd = 0.f;
+  double accr = 0.0;
+  float m[2][CW], md[2][CW];
+  auto prefetch = [&](int qq, int h) {
+    if (qq + 1 < pp) load_m(qq + 1, m[(h + 1) & 1]);
+    if (qq + 1 >= LAG && qq + 1 < pp) load_m(qq + 1 - LAG, md[(h + 1) & 1]);
+  };
+  if (pp > 0) load_m(0, m[0]);
+  for (int q = 0; q < pp; q += 2) {
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int qq = q + h;
+      if (qq < pp) {
+        if (t < KB) {  // x_qq and d_{qq-LAG}, both loads in flight together
+          const int64_t gx = blk(qq) * KB + t;
+          const int64_t gd = blk(qq >= LAG ? qq - LAG : 0) * KB + t;
+          float xv, dv;
+          tagged::read2<float>(pubx, gx, gx < n, pubd, gd, qq >= LAG && gd < n, marker, err, xv, dv);
+          sm.xs[h][t] = xv;
+          if (qq >= LAG) xd[h][t] = dv;
+          prefetch(qq, h);
+        }
+        __syncthreads();
+        // (waves 1-3 issue theirs after the barrier, so the barrier does not
+        // wait for their issue)
+        if (t >= KB) prefetch(qq, h);
+#pragma unroll
+        for (int e = 0; e < CW; ++e) {
+          const float xv = sm.xs[h][cq * CW + e];
+          acc += m[h][e] * xv;
+          accr += (double)m[h][e] * (double)xv;
+        }
+        if (qq >= LAG) {
+#pragma unroll
+          for (int e = 0; e < CW; ++e) accd += md[h][e] * xd[h][cq * CW + e];
+        }
+        if (t < KB && qq >= j0) {
+#pragma unroll
+          for (int e = 0; e < CW; ++e) tail0[qq - j0][t][e] = m[h][e];
+        }
+      }
+    }
+  }
+  // the entries of the panel's last LAG blocks, whose d the loop above did
+  // not reach: waves 1-3 read them again during the chain, wave 0 kept
+  // them in LDS (so its polls never wait behind loads of its own)
+  float mtl[LAG][CW];
+  auto load_tail = [&]() {
+#pragma unroll
+    for (int k = 0; k < LAG; ++k) {
+      if (j0 + k < pp) {
+        load_m(j0 + k, mtl[k]);
+      } else {
+#pragma unroll
+        for (int e = 0; e < CW; ++e) mtl[k][e] = 0.f;
+      }
+    }
+  };
+  // v = b - panel sums (vs holds b, local order)
+  if constexpr (!TRANS) {
+    float ps = acc + __shfl_xor(acc, 1, 64);
+    ps += __shfl_xor(ps, 2, 64);
+    if (cq == 0 && gi < n) sm.vs[FWD ? r : KB - 1 - r] -= ps;
+  } else {
+    sm.red[cq][r] = acc;
+    __syncthreads();
+    if (t < KB) {
+      const int64_t g = trsv::row_of<FWD, KB>(r0, t);
+      if (g < n) {
+        const int nr = (int)(g - r0);
+        sm.vs[t] -= (sm.red[0][nr] + sm.red[1][nr]) + (sm.red[2][nr] + sm.red[3][nr]);
+      }
+    }
+  }
+  __syncthreads();
+  if (t >= KB) load_tail();  // during the chain, which waves 1-3 do not take part in
+  // wave 0, lane = local row t: z = inv(D) v - G2 u_{p-2} - G1 u_{p-1} for
+  // u = x (first chain) or d (correction chain); v in vv (local order), the
+  // two previous blocks through sm.xs[1] / sm.xs[0]
+  auto chain = [&](const float* vv, unsigned* pub, const float* pre2) -> float {
+    float y[4] = {0.f, 0.f, 0.f, 0.f};
+    {
+      const VecN<float, VN>* v4p = reinterpret_cast<const VecN<float, VN>*>(vv);
+      VecN<float, VN> vr[KB / VN];
+#pragma unroll
+      for (int c = 0; c < KB / VN; ++c) vr[c] = v4p[c];
+#pragma unroll
+      for (int c = 0; c < KB / VN; ++c)
+#pragma unroll
+        for (int e = 0; e < VN; ++e) y[e] += sm.inv[t][c * VN + e] * vr[c].v[e];
+    }
+    float z = (y[0] + y[1]) + (y[2] + y[3]);
+    if (p > 1) {
+      const float* xs = pre2;  // the p-2 block, already in LDS, or polled now
+      if (!pre2) {
+        const int64_t g2 = blk(p - 2) * KB + t;
+        sm.xs[1][t] = g2 < n ? tagged::read<float>(pub, g2, marker, err) : 0.f;
+        __builtin_amdgcn_wave_barrier();
+        xs = sm.xs[1];
+      }
+      const VecN<float, VN>* x4 = reinterpret_cast<const VecN<float, VN>*>(xs);
+      const VecN<float, VN>* g4 = reinterpret_cast<const VecN<float, VN>*>(mb2[t]);
+      float u[4] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int c = 0; c < KB / VN; ++c) {
+        const VecN<float, VN> xx = x4[c], gg = g4[c];
+#pragma unroll
+        for (int e = 0; e < VN; ++e) u[e] += gg.v[e] * xx.v[e];
+      }
+      z -= (u[0] + u[1]) + (u[2] + u[3]);
+    }
+    if (p > 0) {
+      float gr[KB];  // this row of G1, read before the wait
+      {
+        const VecN<float, VN>* g4 = reinterpret_cast<const VecN<float, VN>*>(mb[t]);
+#pragma unroll
+        for (int c = 0; c < KB / VN; ++c) {
+          const VecN<float, VN> gg = g4[c];
+#pragma unroll
+          for (int e = 0; e < VN; ++e) gr[c * VN + e] = gg.v[e];
+        }
+      }
+      const int64_t g1 = blk(p - 1) * KB + t;
+      float* xs = sm.xs[0];
+      xs[t] = g1 < n ? tagged::read<float>(pub, g1, marker, err) : 0.f;
+      __builtin_amdgcn_wave_barrier();
+      const VecN<float, VN>* x4 = reinterpret_cast<const VecN<float, VN>*>(xs);
+      VecN<float, VN> xr[KB / VN];
+#pragma unroll
+      for (int c = 0; c < KB / VN; ++c) xr[c] = x4[c];
+      float u[4] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int c = 0; c < KB / VN; ++c)
+#pragma unroll
+        for (int e = 0; e < VN; ++e) u[e] += gr[c * VN + e] * xr[c].v[e];
+      z -= (u[0] + u[1]) + (u[2] + u[3]);
+    }
+    return z;
+  };
+  float xmine = 0.f;  // wave 0: x_p of local row t
+  const int64_t gmine = trsv::row_of<FWD, KB>(r0, t);
+  if (t < KB) {
+    xmine = chain(sm.vs, pubx, nullptr);
+    if (gmine < n) tagged::publish<float>(pubx, gmine, xmine, marker);
+    xp[gmine - r0] = xmine;
+  }
+  // wave 0 issues the first polls of the tail's d blocks now, so their
+  // round trip overlaps the residual (checked after it).  (d_{p-2} is
+  // polled later, in the chain: waiting on the newest block before the
+  // older work is done lengthens every step of the correction chain.)
+  uint64_t* wd64 = reinterpret_cast<uint64_t*>(pubd);
+  uint64_t wt[LAG];
+  auto tail_g = [&](int k) -> int64_t { return blk(j0 + k) * KB + t; };
+  auto tail_on = [&](int k) -> bool { return j0 + k < pp && tail_g(k) < n; };
+  if (t < KB) {
+#pragma unroll
+    for (int k = 0; k < LAG; ++k)
+      if (tail_on(k)) wt[k] = __hip_atomic_load(wd64 + tail_g(k), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  __syncthreads();
+  // residual rows in FP64: the panel part, then the last three blocks
+  {
+    double s = accr;
+    const int cb = cq * CW;
+    // (the x buffers of blocks that do not exist are never read: they hold
+    // whatever the LDS held)
+    const int kl = FWD ? r : KB - 1 - r;  // local order of this thread's row
+    if (p > 1) {
+#pragma unroll
+      for (int e = 0; e < CW; ++e) s += (double)rb2[kl][cb + e] * (double)sm.xs[1][cb + e];
+    }
+    if (p > 0) {
+#pragma unroll
+      for (int e = 0; e < CW; ++e) s += (double)rb1[kl][cb + e] * (double)sm.xs[0][cb + e];
+    }
+#pragma unroll
+    for (int e = 0; e < CW; ++e) s += (double)rd[kl][FWD ? cb + e : KB - 1 - (cb + e)] * (double)xp[cb + e];
+    if constexpr (!TRANS) {
+      s += __shfl_xor(s, 1, 64);
+      s += __shfl_xor(s, 2, 64);
+      if (cq == 0) rred[0][r] = s;
+    } else {
+      rred[cq][r] = s;
+    }
+  }
+  // the tail's d blocks into LDS
+  if (t < KB) {
+#pragma unroll
+    for (int k = 0; k < LAG; ++k) {
+      float v = 0.f;
+      if (tail_on(k))
+        v = __uint_as_float((wt[k] >> 32) == marker ? (unsigned)wt[k]
+                                                     : tagged::poll_word(wd64 + tail_g(k), marker, err));
+      xt[k][t] = v;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < LAG; ++k) {
+    if (j0 + k < pp) {
+#pragma unroll
+      for (int e = 0; e < CW; ++e) accd += (t < KB ? tail0[k][t][e] : mtl[k][e]) * xt[k][cq * CW + e];
+    }
+  }
+  // vd = (r_p - d panel), local order
+  if constexpr (!TRANS) {
+    float pd = accd + __shfl_xor(accd, 1, 64);
+    pd += __shfl_xor(pd, 2, 64);
+    if (cq == 0) {
+      const double rv = (double)bs[r] - rred[0][r];
+      vd[FWD ? r : KB - 1 - r] = gi < n ? (float)(rv - (double)pd) : 0.f;
+    }
+  } else {
+    sm.red[cq][r] = accd;
+    __syncthreads();
+    if (t < KB) {
+      const int nr = (int)(gmine - r0);
+      const double rv = (double)bs[nr] - ((rred[0][nr] + rred[1][nr]) + (rred[2][nr] + rred[3][nr]));
+      const float pd = (sm.red[0][nr] + sm.red[1][nr]) + (sm.red[2][nr] + sm.red[3][nr]);
+      vd[t] = gmine < n ? (float)(rv - (double)pd) : 0.f;
+    }
+  }
+  __syncthreads();
+  if (t < KB) {
+    const float d = chain(vd, pubd, nullptr);
+    if (gmine < n) {
+      tagged::publish<float>(pubd, gmine, d, marker);
+      x[gmine * incx] = (float)((double)xmine + (double)d);
+    }
+  }
+  __syncthreads();
+}
+
 template <class TO, bool FWD, bool TRANS>
 void solve(Ctx& c, bool unit, int64_t n, const float* A, int64_t lda, const TO* Aoff, int64_t ldo,
            const double* rhs, double* x, float* xf) {
@@ -279,6 +646,30 @@ void solve_any(Ctx& c, bool fwd, bool trans, bool unit, int64_t n, const float* 
   if (fwd && trans) solve<TO, true, true>(c, unit, n, A, lda, Aoff, ldo, rhs, x, xf);
   if (!fwd && !trans) solve<TO, false, false>(c, unit, n, A, lda, Aoff, ldo, rhs, x, xf);
   if (!fwd && trans) solve<TO, false, true>(c, unit, n, A, lda, Aoff, ldo, rhs, x, xf);
+}
+
+template <bool FWD, bool TRANS>
+void refined_launch(Ctx& c, bool unit, int64_t n, const float* A, int64_t lda, float* x, int64_t incx) {
+  const int64_t nb = ceil_div(n, kB);
+  if (nb > 0x7fffffff) throw Error(kInvalidSize, "trsv: n too large");
+  unsigned epoch = 0;  // (not used: the marker comes from the launch generation)
+  unsigned* flags = c.epoch_flags(2 * (size_t)trsv::pub_words<float>(n) + kFlagBase, &epoch);
+  unsigned* gen = c.counters() + Ctx::kGeneration;
+  if (unit)
+    hipLaunchKernelGGL((trsv_refined_kernel<FWD, TRANS, true>), dim3((unsigned)nb), dim3(kNT), 0, c.stream, n, A,
+                       lda, x, incx, flags, gen, c.timeout_word());
+  else
+    hipLaunchKernelGGL((trsv_refined_kernel<FWD, TRANS, false>), dim3((unsigned)nb), dim3(kNT), 0, c.stream, n, A,
+                       lda, x, incx, flags, gen, c.timeout_word());
+  EXO_LAUNCH_CHECK();
+}
+
+void refined(Ctx& c, bool fwd, bool trans, bool unit, int64_t n, const float* A, int64_t lda, float* x,
+             int64_t incx) {
+  if (fwd && !trans) refined_launch<true, false>(c, unit, n, A, lda, x, incx);
+  if (fwd && trans) refined_launch<true, true>(c, unit, n, A, lda, x, incx);
+  if (!fwd && !trans) refined_launch<false, false>(c, unit, n, A, lda, x, incx);
+  if (!fwd && trans) refined_launch<false, true>(c, unit, n, A, lda, x, incx);
 }
 
 void residual(Ctx& c, bool lower, bool trans, bool unit, int64_t n, const float* A, int64_t lda, const double* b,
@@ -338,6 +729,10 @@ void trsv_mixed(Ctx& c, int64_t Uplo, int64_t TransA, int64_t Diag, int64_t n, f
   if (update == 32) {
     // scratch slot 2: x_hi, x_lo, r / d (FP32); b stays in the caller's x
     // until the final combine overwrites it
+    if (refine == 1) {  // the refinement fused into the solve, one launch
+      refined(c, fwd, trans, unit, n, A, lda, x, incx);
+      return;
+    }
     float* xh = static_cast<float*>(c.scratch((size_t)n * 3 * sizeof(float) + 1024, 2));
     float* xl = xh + ceil_div(n, 64) * 64;
     float* rd = xl + ceil_div(n, 64) * 64;
