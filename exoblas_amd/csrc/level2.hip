@@ -162,11 +162,23 @@ __global__ __launch_bounds__(kNT) void gemv_n_row_kernel(int64_t M, int64_t N, T
   constexpr int VN = Vec16<T>::n;
   constexpr int UV = 8;
   const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
-  // a lower triangle's longest rows are its last: take them first, so the
-  // short rows fill in behind them instead of trailing at the end
-  const int64_t rank = (int64_t)blockIdx.x * (kNT / kWave) + wave;
-  const int64_t row = TRI == kLowerTri ? M - 1 - rank : rank;
-  if (rank >= M) return;
+  int64_t row = (int64_t)blockIdx.x * (kNT / kWave) + wave;
+  if (TRI != kFull) {
+    // a triangle's rows pair up to equal length: waves 2j and 2j + 1 of
+    // workgroup w take rows i = 2w + j and M - 1 - i, so every workgroup
+    // reads about M + 1 elements (dealt in order, 8 workgroups per CU at
+    // M = 8192 gave the CUs with the long rows 12 % more than the mean)
+    const int64_t h = (M + 1) / 2;  // rows i < h from the top, M - h from the bottom
+    const int64_t i = 2 * (int64_t)blockIdx.x + wave / 2;
+    if (wave % 2 == 0) {
+      if (i >= h) return;
+      row = i;
+    } else {
+      if (i >= M - h) return;
+      row = M - 1 - i;
+    }
+  }
+  if (row >= M) return;
   int64_t jl = 0, jh = N;  // columns [jl, jh) of this row
   if (TRI == kLowerTri) jh = UNIT ? row : row + 1;
   if (TRI == kUpperTri) jl = UNIT ? row + 1 : row;
@@ -697,7 +709,8 @@ void symv_rm(Ctx& c, int64_t Uplo, int64_t n, T alpha, const T* A, int64_t lda, 
   if (n <= 0) return;
   constexpr int VN = Vec16<T>::n;
   const bool upper = Uplo == kUpper;
-  const tiles::TileList tl = tiles::make_list<symv::kRB, kNT * VN>(upper ? tiles::kUpper : tiles::kLower, n, n);
+  const tiles::TileList tl =
+      tiles::make_list<symv::kRB, kWave * VN * symv::kQ>(upper ? tiles::kUpper : tiles::kLower, n, n);
   const int64_t npad = tl.nb * symv::kRB;
   T* part = static_cast<T*>(c.scratch(sizeof(T) * (size_t)((tl.nc + tl.nb) * npad)));
   T* rowpart = part;
@@ -715,12 +728,13 @@ void symv_rm(Ctx& c, int64_t Uplo, int64_t n, T alpha, const T* A, int64_t lda, 
   }
 #undef EXO_SYMV
   EXO_LAUNCH_CHECK();
+  const dim3 fgrid((unsigned)ceil_div(n, kWave)), fblock(symv::kFW * kWave);
   if (upper)
-    hipLaunchKernelGGL((symv::finish_kernel<T, true, symv::kRB>), dim3((unsigned)ceil_div(n, kWave)), dim3(kNT), 0, c.stream,
-                       n, alpha, beta, rowpart, colpart, tl, y, incy);
+    hipLaunchKernelGGL((symv::finish_kernel<T, true, symv::kRB>), fgrid, fblock, 0, c.stream, n, alpha, beta, rowpart,
+                       colpart, tl, y, incy);
   else
-    hipLaunchKernelGGL((symv::finish_kernel<T, false, symv::kRB>), dim3((unsigned)ceil_div(n, kWave)), dim3(kNT), 0, c.stream,
-                       n, alpha, beta, rowpart, colpart, tl, y, incy);
+    hipLaunchKernelGGL((symv::finish_kernel<T, false, symv::kRB>), fgrid, fblock, 0, c.stream, n, alpha, beta, rowpart,
+                       colpart, tl, y, incy);
   EXO_LAUNCH_CHECK();
 }
 

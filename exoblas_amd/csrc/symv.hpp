@@ -19,24 +19,30 @@ namespace symv {
 constexpr int kNT = 256;
 constexpr int kRB = 64;  // rows per tile
 
+constexpr int kQ = 2;   // 16-byte vectors per lane and tile row: W = 64 x VN x kQ columns
+constexpr int kFW = 16;  // waves of finish_kernel
+
 // One workgroup per RB x W tile of the stored triangle (tiles.hpp), RB rows
-// and W = 256 x VN columns, each element read once.  Wave w takes rows
-// r0 + w RB / 4 .. (RW = RB / 4 rows) across the whole tile: lane l loads the 16-byte column
-// vectors l, l + 64, l + 128, l + 192 of each row (4 rows, 16 loads in flight
-// per lane), keeping RW row sums (A_ij x_j) and 4 x VN column sums (A_ij x_i,
-// off the diagonal) in registers.  Row sums are reduced over the wave's lanes
+// and W = 64 x VN x Q columns, each element read once.  Wave w takes rows
+// r0 + w RB / 4 .. (RW = RB / 4 rows) across the whole tile: lane l loads the
+// 16-byte column vectors l, l + 64, .., l + 64 (Q - 1) of each row (4 rows
+// in flight), keeping RW row sums (A_ij x_j) and Q x VN column sums (A_ij
+// x_i, off the diagonal) in registers.  (Measured on MI355X, n = 8192 FP32:
+// Q = 2 takes 9 % less time than Q = 4, whose 576 tiles of 256 KiB leave
+// the CUs that get a third tile running alone at the end; Q = 1 and 32-row
+// tiles gain nothing more.)  Row sums are reduced over the wave's lanes
 // (a butterfly: lane l ends with row l % RW) into rowpart[c][row]; column sums
 // are added over the 4 waves through LDS into colpart[b][column].
 // finish_kernel adds both in a fixed order, so results are
 // deterministic.
 // VEC: 16-byte loads (aligned rows and n a multiple of VN).
-template <class T, bool UPPER, bool VEC, int RB>
+template <class T, bool UPPER, bool VEC, int RB, int Q = kQ>
 __global__ __launch_bounds__(kNT) void tile_kernel(int64_t n, const T* __restrict__ A, int64_t lda,
                                                         const T* __restrict__ x, int64_t incx, T* rowpart,
                                                         T* colpart, tiles::TileList tl) {
   using V = typename Vec16<T>::type;
-  constexpr int VN = Vec16<T>::n, W = kNT * VN;
-  constexpr int NW = kNT / kWave, RW = RB / NW, Q = W / (kWave * VN), U = 2;  // RW rows per wave, Q = 4
+  constexpr int VN = Vec16<T>::n, W = kWave * VN * Q;
+  constexpr int NW = kNT / kWave, RW = RB / NW, U = 2;  // RW rows per wave
   static_assert(RW >= 2 * U && RW % (2 * U) == 0 && RW <= 16, "rows per wave");
   __shared__ T red[NW][W];
   int64_t b, c;
@@ -178,13 +184,13 @@ __global__ __launch_bounds__(kNT) void tile_kernel(int64_t n, const T* __restric
 
 // y_i = beta y_i + alpha (sum of row i's tile row sums, in column-block
 // order, + sum of the column sums for column i, in row-block order).  A
-// workgroup finishes 64 consecutive i; its 4 waves take every 4th partial
+// workgroup finishes 64 consecutive i; its NW waves take every NW-th partial
 // (8 loads in flight per lane) and are combined in wave order through LDS,
 // so the sum's association is fixed (deterministic).
-template <class T, bool UPPER, int RB>
-__global__ __launch_bounds__(kNT) void finish_kernel(int64_t n, T alpha, T beta, const T* rowpart,
+template <class T, bool UPPER, int RB, int Q = kQ, int NW = kFW>
+__global__ __launch_bounds__(NW * kWave) void finish_kernel(int64_t n, T alpha, T beta, const T* rowpart,
                                                           const T* colpart, tiles::TileList tl, T* y, int64_t incy) {
-  constexpr int VN = Vec16<T>::n, W = kNT * VN, NW = kNT / kWave;
+  constexpr int VN = Vec16<T>::n, W = kWave * VN * Q;
   __shared__ T red[2][NW][kWave];
   const int lane = threadIdx.x % kWave, wave = threadIdx.x / kWave;
   const int64_t i = (int64_t)blockIdx.x * kWave + lane;

@@ -57,6 +57,56 @@ __device__ __forceinline__ unsigned poll_word(uint64_t* w64, unsigned marker, un
   return (unsigned)w;
 }
 
+// Polls the two words of one FP64 value (published together) until both
+// carry `marker` (bounded), reloading both on every poll: polling one and
+// then checking the other's earlier load would cost a second round trip
+// whenever the value was not yet published at the first load.
+__device__ __forceinline__ void poll_pair(uint64_t* w64, unsigned marker, unsigned* err, unsigned& lo,
+                                          unsigned& hi) {
+  uint64_t w0, w1;
+  long long it = 0;
+  const long long limit = g_spin_limit;
+  while (true) {
+    w0 = __hip_atomic_load(w64, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    w1 = __hip_atomic_load(w64 + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if ((w0 >> 32) == marker && (w1 >> 32) == marker) break;
+    if (++it > limit) {
+      __hip_atomic_store(err, marker ? marker : 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+      break;
+    }
+    __builtin_amdgcn_s_sleep(1);
+  }
+  lo = (unsigned)w0;
+  hi = (unsigned)w1;
+}
+
+// The low 32 bits of the W words at w64 (one value: W = 1 FP32, 2 FP64),
+// given their first loads w[]: those that already carry `marker` are used
+// as loaded, otherwise the value is polled until it is whole.
+template <int W>
+__device__ __forceinline__ void settle(uint64_t* w64, const uint64_t (&w)[W], unsigned marker, unsigned* err,
+                                       unsigned (&lo)[W]) {
+  if constexpr (W == 1) {
+    lo[0] = (w[0] >> 32) == marker ? (unsigned)w[0] : poll_word(w64, marker, err);
+  } else {
+    if ((w[0] >> 32) == marker && (w[1] >> 32) == marker) {
+      lo[0] = (unsigned)w[0];
+      lo[1] = (unsigned)w[1];
+    } else {
+      poll_pair(w64, marker, err, lo[0], lo[1]);
+    }
+  }
+}
+
+// The value whose 32-bit pieces (low first) are lo[].
+template <class T, int W>
+__device__ __forceinline__ T compose(const unsigned (&lo)[W]) {
+  if constexpr (W == 1)
+    return __uint_as_float(lo[0]);
+  else
+    return __longlong_as_double((long long)(((uint64_t)lo[1] << 32) | lo[0]));
+}
+
 // Value g of the publication area `pub` (8-byte aligned), once published.
 template <class T>
 __device__ __forceinline__ T read(unsigned* pub, int64_t g, unsigned marker, unsigned* err) {
@@ -65,11 +115,11 @@ __device__ __forceinline__ T read(unsigned* pub, int64_t g, unsigned marker, uns
     return __uint_as_float(poll_word(w64 + g, marker, err));
   } else {
     // both halves loaded before either is checked: one round trip, not two
-    const uint64_t w0 = __hip_atomic_load(w64 + 2 * g, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    const uint64_t w1 = __hip_atomic_load(w64 + 2 * g + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    const unsigned lo = (w0 >> 32) == marker ? (unsigned)w0 : poll_word(w64 + 2 * g, marker, err);
-    const unsigned hi = (w1 >> 32) == marker ? (unsigned)w1 : poll_word(w64 + 2 * g + 1, marker, err);
-    return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+    const uint64_t w[2] = {__hip_atomic_load(w64 + 2 * g, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT),
+                           __hip_atomic_load(w64 + 2 * g + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)};
+    unsigned lo[2];
+    settle<2>(w64 + 2 * g, w, marker, err, lo);
+    return compose<T, 2>(lo);
   }
 }
 
@@ -88,19 +138,11 @@ __device__ __forceinline__ void read2(unsigned* pub0, int64_t g0, bool v0, unsig
     if (v0) wa[h] = __hip_atomic_load(a + h, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     if (v1) wb[h] = __hip_atomic_load(b + h, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
-  unsigned la[W], lb[W];
-#pragma unroll
-  for (int h = 0; h < W; ++h) {
-    la[h] = v0 ? ((wa[h] >> 32) == marker ? (unsigned)wa[h] : poll_word(a + h, marker, err)) : 0u;
-    lb[h] = v1 ? ((wb[h] >> 32) == marker ? (unsigned)wb[h] : poll_word(b + h, marker, err)) : 0u;
-  }
-  if constexpr (W == 1) {
-    x0 = __uint_as_float(la[0]);
-    x1 = __uint_as_float(lb[0]);
-  } else {
-    x0 = __longlong_as_double((long long)(((uint64_t)la[1] << 32) | la[0]));
-    x1 = __longlong_as_double((long long)(((uint64_t)lb[1] << 32) | lb[0]));
-  }
+  unsigned la[W] = {}, lb[W] = {};
+  if (v0) settle<W>(a, wa, marker, err, la);
+  if (v1) settle<W>(b, wb, marker, err, lb);
+  x0 = compose<T, W>(la);
+  x1 = compose<T, W>(lb);
 }
 
 // Values g0, g0 + stride, ..., g0 + (K-1) stride of `pub` (those with index
@@ -122,14 +164,8 @@ __device__ __forceinline__ void read_n(unsigned* pub, int64_t g0, int64_t stride
   for (int k = 0; k < K; ++k) {
     if (k >= count) continue;
     unsigned lo[W];
-#pragma unroll
-    for (int h = 0; h < W; ++h)
-      lo[h] = (w[k][h] >> 32) == marker ? (unsigned)w[k][h]
-                                        : poll_word(w64 + W * (g0 + k * stride) + h, marker, err);
-    if constexpr (W == 1)
-      v[k] = __uint_as_float(lo[0]);
-    else
-      v[k] = __longlong_as_double((long long)(((uint64_t)lo[1] << 32) | lo[0]));
+    settle<W>(w64 + W * (g0 + k * stride), w[k], marker, err, lo);
+    v[k] = compose<T, W>(lo);
   }
 }
 
