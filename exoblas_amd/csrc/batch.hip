@@ -58,7 +58,7 @@ __device__ __forceinline__ void gemm_item(const Desc& d, uint32_t local, float* 
     for (int b = 0; b < C::SN; ++b) acc[a][b] = gemm::v4f{0.f, 0.f, 0.f, 0.f};
   gemm::tile_mma<C>(A, B, d.m, d.n, d.k, m0, n0, 0, d.k, acc, lds);
   const int cvec = (d.ldc % C::EB == 0) && ((reinterpret_cast<uintptr_t>(d.c) % (C::EB * 4)) == 0);
-  gemm::tile_store<C, gemm::kFull, true>(d.c, d.ldc, cvec, d.m, d.n, m0, n0, d.alpha, acc, d.beta);
+  gemm::tile_store<C, gemm::kFull, true, true>(d.c, d.ldc, cvec, d.m, d.n, m0, n0, d.alpha, acc, d.beta);
   __syncthreads();  // LDS reused by the next item
 }
 
@@ -106,7 +106,7 @@ __device__ __forceinline__ void gemv_item(const Desc& d, uint32_t local) {
     const int64_t row = rb + r;
     if (lane == 0 && row < d.m) {
       float* yp = d.c + row * d.ldc;
-      *yp = d.alpha * s + (d.beta != 0.f ? d.beta * *yp : 0.f);
+      *yp = d.alpha * s + d.beta * *yp;  // y read for every beta, as the per-call gemv does
     }
   }
 }

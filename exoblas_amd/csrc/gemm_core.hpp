@@ -338,8 +338,10 @@ __device__ __forceinline__ void tile_mma(const Operand<typename C::type>& A, con
 
 // C[m, n] += alpha * acc for the block tile at (m0, n0), restricted to the
 // TRI triangle of C (n <= m for kLower, n >= m for kUpper).  With BETA the
-// update is C = alpha * acc + beta * C instead, C not read when beta == 0.
-template <class C, int TRI, bool BETA = false>
+// update is C = alpha * acc + beta * C instead, C not read when beta == 0
+// unless READ0 (then beta * C is formed for every beta, as the reference's
+// mscal + gemm does: NaN or Inf in C stay NaN with beta == 0).
+template <class C, int TRI, bool BETA = false, bool READ0 = false>
 __device__ __forceinline__ void tile_store(typename C::type* Cp, int64_t ldc, int cvec, int64_t M, int64_t N,
                                            int64_t m0, int64_t n0, typename C::type alpha,
                                            const typename Mfma<typename C::type>::acc (&acc)[C::SM][C::SN],
@@ -365,13 +367,13 @@ __device__ __forceinline__ void tile_store(typename C::type* Cp, int64_t ldc, in
         if (TRI == kUpper) all_in = all_in && col0 >= row;
         if (all_in) {
           VecN<T, E> v;
-          if (!BETA || beta != T(0)) v = *reinterpret_cast<VecN<T, E>*>(cp);
+          if (!BETA || READ0 || beta != T(0)) v = *reinterpret_cast<VecN<T, E>*>(cp);
 #pragma unroll
           for (int e = 0; e < E; ++e) {
             if (!BETA)
               v.v[e] += alpha * acc[ta][gb * E + e][q];
             else
-              v.v[e] = alpha * acc[ta][gb * E + e][q] + (beta != T(0) ? beta * v.v[e] : T(0));
+              v.v[e] = alpha * acc[ta][gb * E + e][q] + (READ0 || beta != T(0) ? beta * v.v[e] : T(0));
           }
           *reinterpret_cast<VecN<T, E>*>(cp) = v;
         } else {
@@ -385,7 +387,7 @@ __device__ __forceinline__ void tile_store(typename C::type* Cp, int64_t ldc, in
               if (!BETA)
                 cp[e] += alpha * acc[ta][gb * E + e][q];
               else
-                cp[e] = alpha * acc[ta][gb * E + e][q] + (beta != T(0) ? beta * cp[e] : T(0));
+                cp[e] = alpha * acc[ta][gb * E + e][q] + (READ0 || beta != T(0) ? beta * cp[e] : T(0));
             }
           }
         }

@@ -15,8 +15,8 @@
 extern "C" {
 #endif
 
-#define EXOBLAS_OP_GEMM 1 /* C = alpha op(A) op(B) + beta C   (row-major; beta == 0: C not read) */
-#define EXOBLAS_OP_GEMV 2 /* y = alpha A x + beta y           (A m x n row-major, NoTrans)       */
+#define EXOBLAS_OP_GEMM 1 /* C = alpha op(A) op(B) + beta C   (row-major; C read for every beta)  */
+#define EXOBLAS_OP_GEMV 2 /* y = alpha A x + beta y           (A m x n row-major, NoTrans; y too) */
 #define EXOBLAS_OP_TRSV 3 /* x = op(A)^-1 x                   (A n x n row-major, one triangle)  */
 
 typedef struct exoblas_batch_op {

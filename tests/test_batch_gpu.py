@@ -123,6 +123,48 @@ def test_batch_run_mixed(env, rng):
     _check(probs)
 
 
+def test_batch_beta_zero_reads_output(env, rng):
+    """beta = 0 with NaN / Inf already in C and y: the batch forms beta * C as
+    the per-call path does (the reference's mscal, then the product), so those
+    entries stay NaN in both; finite entries get alpha op(A) op(B) (gemm) and
+    alpha A x (gemv)."""
+    torch, so, native = env
+    probs = []
+    for M, N, K in [(70, 90, 33), (256, 256, 64)]:
+        A = rng.uniform(-1, 1, (M, K)).astype(np.float32)
+        B = rng.uniform(-1, 1, (K, N)).astype(np.float32)
+        C = rng.uniform(-1, 1, (M, N)).astype(np.float32)
+        C[::7, ::5] = np.nan
+        C[3::11, 2::9] = np.inf
+        probs.append(("gemm", dict(M=M, N=N, K=K, ta=NO_TRANS, tb=NO_TRANS, A=A, B=B, C=C, alpha=1.5, beta=0.0)))
+    m, n = 300, 500
+    y = rng.uniform(-1, 1, m).astype(np.float32)
+    y[::13] = np.nan
+    probs.append(("gemv", dict(m=m, n=n, A=rng.uniform(-1, 1, (m, n)).astype(np.float32),
+                               x=rng.uniform(-1, 1, n).astype(np.float32), y=y, alpha=0.75, beta=0.0)))
+    ops, keep = _to_device(torch, native, probs)
+    assert so.exoblas_batch_run(torch.cuda.current_stream().cuda_stream, ops.ctypes.data, len(ops)) == 0
+    torch.cuda.synchronize()
+    o = Oracle(np.float32)
+    for kind, p in probs:
+        got = p["out"].cpu().numpy()
+        bad = ~np.isfinite(p["C"] if kind == "gemm" else p["y"])
+        assert np.isnan(got[bad]).all(), f"batched {kind}: beta = 0 must keep NaN from 0 * NaN / Inf"
+        if kind == "gemm":
+            want = np.zeros_like(p["C"])
+            o.gemm(NO_TRANS, NO_TRANS, p["M"], p["N"], p["K"], p["alpha"], p["A"], p["K"], p["B"], p["N"], want,
+                   p["N"])
+            mag = p["alpha"] * (np.abs(p["A"]).astype(np.float64) @ np.abs(p["B"]))
+            K = p["K"]
+        else:
+            want = np.zeros(p["m"], np.float32)
+            o.gemv(NO_TRANS, p["m"], p["n"], p["alpha"], 0.0, p["A"], p["n"], p["x"], 1, want, 1)
+            mag = p["alpha"] * (np.abs(p["A"]).astype(np.float64) @ np.abs(p["x"]))
+            K = p["n"]
+        assert_bound(np.where(bad, 0, got), np.where(bad, 0, want), np.where(bad, 1, mag), K, np.float32,
+                     what=f"batched {kind} with beta = 0")
+
+
 @pytest.mark.parametrize("tile", [0, 1, 2])
 def test_batch_every_tile(env, rng, tile):
     torch, so, native = env
