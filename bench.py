@@ -765,9 +765,13 @@ def bench_precision(a, torch, so, L):
     x64 = torch.linalg.solve_triangular(A.double().tril(), b.double()[:, None], upper=False)[:, 0]
     x = b.clone()
     wA = L.win2(A.data_ptr(), n)
+    packed = torch.empty(n * n, dtype=torch.bfloat16, device="cuda")  # the BF16 copy, made once
+    assert so.exoblas_strsv_mixed_pack_async(sh, n, wA, packed.data_ptr()) == 0, L.last_error()
     forms = {"plain": lambda: so.exoblas_strsv_rm_stride_1_async(sh, 122, 111, 131, n, L.win1(x.data_ptr(), 1), wA),
              "mixed_bf16": lambda: so.exoblas_strsv_mixed_async(sh, 122, 111, 131, n, L.win1(x.data_ptr(), 1), wA,
                                                                 16, 1),
+             "mixed_bf16_packed": lambda: so.exoblas_strsv_mixed_packed_async(
+                 sh, 122, 111, 131, n, L.win1(x.data_ptr(), 1), wA, packed.data_ptr(), 1),
              "mixed_fp32": lambda: so.exoblas_strsv_mixed_async(sh, 122, 111, 131, n, L.win1(x.data_ptr(), 1), wA,
                                                                 32, 1)}
     for name, fn in forms.items():
@@ -780,7 +784,7 @@ def bench_precision(a, torch, so, L):
         kernels[f"strsv_{name}_{n}"] = {"max_err_normwise": (err.max() / x64.abs().max()).item(),
                                          "max_rel_err": (err / x64.abs().clamp_min(1e-300)).max().item(),
                                          "us": round(ms * 1e3, 2), "GB/s": round(nbytes / (ms * 1e-3) / 1e9, 1)}
-    del A, b, x, x64
+    del A, b, x, x64, packed
     torch.cuda.empty_cache()
     # the headline form: 4 products (the fastest whose error is at most plain
     # FP32's on both sets)
@@ -1173,9 +1177,13 @@ def config4(a, torch, so, L):
     x64 = torch.linalg.solve_triangular(At.double().tril(), b.double()[:, None], upper=False)[:, 0]
     x = b.clone()
     wA = L.win2(At.data_ptr(), n)
+    packed = torch.empty(n * n, dtype=torch.bfloat16, device="cuda")  # the BF16 copy, made once
+    assert so.exoblas_strsv_mixed_pack_async(sh, n, wA, packed.data_ptr()) == 0, L.last_error()
     tforms = {"plain": lambda: so.exoblas_strsv_rm_stride_1_async(sh, 122, 111, 131, n, L.win1(x.data_ptr(), 1), wA),
               "mixed_bf16": lambda: so.exoblas_strsv_mixed_async(sh, 122, 111, 131, n, L.win1(x.data_ptr(), 1), wA,
                                                                  16, 1),
+              "mixed_bf16_packed": lambda: so.exoblas_strsv_mixed_packed_async(
+                  sh, 122, 111, 131, n, L.win1(x.data_ptr(), 1), wA, packed.data_ptr(), 1),
               "mixed_fp32": lambda: so.exoblas_strsv_mixed_async(sh, 122, 111, 131, n, L.win1(x.data_ptr(), 1), wA,
                                                                  32, 1)}
     nbytes = (n * (n + 1) // 2 + 2 * n) * 4
@@ -1188,7 +1196,7 @@ def config4(a, torch, so, L):
         kernels[f"strsv_{name}"] = {"us": round(ms * 1e3, 2), "GB/s": round(nbytes / (ms * 1e-3) / 1e9, 1),
                                     "max_err": err.max().item(),
                                     "max_rel_err": (err / x64.abs().clamp_min(1e-300)).max().item()}
-    del At, b, x, x64
+    del At, b, x, x64, packed
     torch.cuda.empty_cache()
     plain = kernels["sgemm_plain"]
 

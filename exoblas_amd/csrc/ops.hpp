@@ -130,8 +130,12 @@ void syrk_split(Ctx&, int64_t Uplo, int64_t Trans, int64_t N, int64_t K, float a
 
 // op(A) x = b in FP32 with BF16 (update 16) / FP32 (update 32) off-diagonal
 // updates, FP64 diagonal blocks and `refine` FP64-residual refinement steps.
+// With update 16 and `packed` (trsv_mixed_pack of the same A), the BF16
+// copy is read from there instead of being made by the call.
 void trsv_mixed(Ctx&, int64_t Uplo, int64_t TransA, int64_t Diag, int64_t n, float* x, int64_t incx, const float* A,
-                int64_t lda, int64_t update, int64_t refine);
+                int64_t lda, int64_t update, int64_t refine, const void* packed = nullptr);
+// The BF16 copy the update-16 solve reads: n x n, row stride n, at `packed`.
+void trsv_mixed_pack(Ctx&, int64_t n, const float* A, int64_t lda, void* packed);
 
 }  // namespace ops
 }  // namespace exoblas

@@ -720,8 +720,18 @@ void residual_f(Ctx& c, bool lower, bool trans, bool unit, int64_t n, const floa
 
 namespace ops {
 
+void trsv_mixed_pack(Ctx& c, int64_t n, const float* A, int64_t lda, void* packed) {
+  __bf16* p = static_cast<__bf16*>(packed);
+  for (int64_t r0 = 0; r0 < n; r0 += 65535) {
+    const int64_t rows = std::min<int64_t>(65535, n - r0);
+    hipLaunchKernelGGL(to_bf16_kernel, dim3((unsigned)std::min<int64_t>(ceil_div(n, 256), 64), (unsigned)rows),
+                       dim3(256), 0, c.stream, rows, n, A + r0 * lda, lda, p + r0 * n);
+    EXO_LAUNCH_CHECK();
+  }
+}
+
 void trsv_mixed(Ctx& c, int64_t Uplo, int64_t TransA, int64_t Diag, int64_t n, float* x, int64_t incx,
-                const float* A, int64_t lda, int64_t update, int64_t refine) {
+                const float* A, int64_t lda, int64_t update, int64_t refine, const void* packed) {
   if (n <= 0) return;
   const bool trans = TransA != 111, lower = Uplo == 122, unit = Diag == 132;
   const bool fwd = lower != trans;
@@ -765,15 +775,12 @@ void trsv_mixed(Ctx& c, int64_t Uplo, int64_t TransA, int64_t Diag, int64_t n, f
   hipLaunchKernelGGL(widen_kernel, g1, dim3(256), 0, c.stream, n, x, incx, b);
   EXO_LAUNCH_CHECK();
   const __bf16* Abf = nullptr;
-  if (update == 16) {  // BF16 copy of the matrix for the off-diagonal updates (scratch slot 3)
-    __bf16* p = static_cast<__bf16*>(c.scratch((size_t)n * n * sizeof(__bf16), 3));
-    for (int64_t r0 = 0; r0 < n; r0 += 65535) {
-      const int64_t rows = std::min<int64_t>(65535, n - r0);
-      hipLaunchKernelGGL(to_bf16_kernel, dim3((unsigned)std::min<int64_t>(ceil_div(n, 256), 64), (unsigned)rows),
-                         dim3(256), 0, c.stream, rows, n, A + r0 * lda, lda, p + r0 * n);
-      EXO_LAUNCH_CHECK();
-    }
-    Abf = p;
+  if (update == 16 && packed) {  // the caller's copy (trsv_mixed_pack of this A)
+    Abf = static_cast<const __bf16*>(packed);
+  } else if (update == 16) {  // BF16 copy of the matrix for the off-diagonal updates (scratch slot 3)
+    void* p = c.scratch((size_t)n * n * sizeof(__bf16), 3);
+    trsv_mixed_pack(c, n, A, lda, p);
+    Abf = static_cast<const __bf16*>(p);
   }
   auto run_solve = [&](const double* rhs, double* out) {
     if (Abf)
@@ -794,3 +801,53 @@ void trsv_mixed(Ctx& c, int64_t Uplo, int64_t TransA, int64_t Diag, int64_t n, f
 
 }  // namespace ops
 }  // namespace exoblas
+
+// ------------------------------------------------------------ packed entry points
+//
+// No counterpart in the reference: the BF16 copy of the update-16 mixed
+// solve made once by the caller (exoblas_strsv_mixed_pack_async) and reused
+// by any number of solves with the same matrix (exoblas_strsv_mixed_packed_
+// async), instead of being remade by every exoblas_strsv_mixed call.
+// Arguments and checks as the generated exoblas_strsv_mixed_async.
+#include "exoblas_amd_mixed.h"
+#include "runtime.hpp"
+
+extern "C" int exoblas_strsv_mixed_pack_async(hipStream_t stream, int_fast32_t n, struct exo_win_2f32c A,
+                                              void* packed) {
+  exoblas::Call call("exoblas_strsv_mixed_pack_async", exoblas::Call::kAsync, stream);
+  if (n < 0) return call.fail_size("n", n);
+  if (A.strides[1] != 1) return call.fail_stride("A", A.strides[1]);
+  if (n > 1 && A.strides[0] < n) return call.fail_ld("A", A.strides[0]);
+  call.window(0, A.data, 0, exoblas::span2_hi(n, n, A.strides[0]), sizeof(float), false, true);
+  call.window(1, packed, 0, (int64_t)n * n, sizeof(__bf16), true, true);
+  if (!call.stage()) return call.status();
+  call.run([&] {
+    exoblas::ops::trsv_mixed_pack(call.ctx(), (int64_t)n, call.dev<const float>(0), (int64_t)A.strides[0],
+                                  call.dev<void>(1));
+  });
+  return call.finish();
+}
+
+extern "C" int exoblas_strsv_mixed_packed_async(hipStream_t stream, int_fast32_t Uplo, int_fast32_t TransA,
+                                                int_fast32_t Diag, int_fast32_t n, struct exo_win_1f32 x,
+                                                struct exo_win_2f32c A, const void* packed, int_fast32_t refine) {
+  exoblas::Call call("exoblas_strsv_mixed_packed_async", exoblas::Call::kAsync, stream);
+  if (!(Uplo == 121 || Uplo == 122)) return call.fail_enum("Uplo", Uplo);
+  if (!(TransA == 111 || TransA == 112 || TransA == 113)) return call.fail_enum("TransA", TransA);
+  if (!(Diag == 131 || Diag == 132)) return call.fail_enum("Diag", Diag);
+  if (n < 0) return call.fail_size("n", n);
+  if (refine < 0) return call.fail_size("refine", refine);
+  if (A.strides[1] != 1) return call.fail_stride("A", A.strides[1]);
+  if (n > 1 && A.strides[0] < n) return call.fail_ld("A", A.strides[0]);
+  call.window(0, x.data, exoblas::span_lo(n, x.strides[0]), exoblas::span_hi(n, x.strides[0]), sizeof(float), true,
+              true);
+  call.window(1, A.data, 0, exoblas::span2_hi(n, n, A.strides[0]), sizeof(float), false, true);
+  call.window(2, packed, 0, (int64_t)n * n, sizeof(__bf16), false, true);
+  if (!call.stage()) return call.status();
+  call.run([&] {
+    exoblas::ops::trsv_mixed(call.ctx(), (int64_t)Uplo, (int64_t)TransA, (int64_t)Diag, (int64_t)n,
+                             call.dev<float>(0), (int64_t)x.strides[0], call.dev<const float>(1),
+                             (int64_t)A.strides[0], 16, (int64_t)refine, call.dev<const void>(2));
+  });
+  return call.finish();
+}

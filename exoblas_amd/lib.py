@@ -103,6 +103,11 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.exoblas_set_spin_limit.argtypes = [ctypes.c_longlong]
     lib.exoblas_sdot_noalloc_async.restype = ctypes.c_int
     lib.exoblas_sdot_noalloc_async.argtypes = [ctypes.c_void_p, c_int_fast32, Win1, Win1, ctypes.c_void_p]
+    lib.exoblas_strsv_mixed_pack_async.restype = ctypes.c_int
+    lib.exoblas_strsv_mixed_pack_async.argtypes = [ctypes.c_void_p, c_int_fast32, Win2, ctypes.c_void_p]
+    lib.exoblas_strsv_mixed_packed_async.restype = ctypes.c_int
+    lib.exoblas_strsv_mixed_packed_async.argtypes = [ctypes.c_void_p] + [c_int_fast32] * 4 + [Win1, Win2, ctypes.c_void_p,
+                                                                                             c_int_fast32]
     _extend(lib)
     _lib = lib
     return lib

@@ -192,3 +192,34 @@ def test_trsv_mixed_strided(so, uplo, trans, incx, rng):
     mask = np.ones(buf.size, bool)
     mask[::step] = False
     assert np.array_equal(buf[mask], before[mask])
+
+
+@pytest.mark.parametrize("uplo,trans", [(LOWER, NO_TRANS), (UPPER, TRANS), (UPPER, NO_TRANS)])
+def test_trsv_mixed_packed(so, uplo, trans, rng):
+    """The BF16 copy packed once and reused: two solves with different
+    right-hand sides through exoblas_strsv_mixed_packed_async give the same
+    bits as exoblas_strsv_mixed_async with update 16 (same kernels, the same
+    copy), and a bad argument is rejected with its status."""
+    import torch
+    from exoblas_amd import lib as L
+    n = 700
+    A = (rng.uniform(-1, 1, (n, n)) / n).astype(np.float32)
+    A[np.arange(n), np.arange(n)] = (n + rng.uniform(0, 1, n)).astype(np.float32)
+    dA = torch.from_numpy(A).cuda()
+    packed = torch.empty(n * n, dtype=torch.bfloat16, device="cuda")
+    sh = torch.cuda.current_stream().cuda_stream
+    wA = L.win2(dA.data_ptr(), n)
+    assert so.exoblas_strsv_mixed_pack_async(sh, n, wA, packed.data_ptr()) == 0, L.last_error()
+    for refine in (1, 0):
+        b = torch.from_numpy(rng.uniform(-1, 1, n).astype(np.float32)).cuda()
+        x1, x2 = b.clone(), b.clone()
+        assert so.exoblas_strsv_mixed_packed_async(sh, uplo, trans, 131, n, L.win1(x1.data_ptr(), 1), wA,
+                                                   packed.data_ptr(), refine) == 0, L.last_error()
+        assert so.exoblas_strsv_mixed_async(sh, uplo, trans, 131, n, L.win1(x2.data_ptr(), 1), wA, 16, refine) == 0
+        torch.cuda.synchronize()
+        assert torch.equal(x1, x2), f"packed solve differs (refine {refine})"
+    x = torch.zeros(n, device="cuda")
+    assert so.exoblas_strsv_mixed_packed_async(sh, uplo, trans, 131, n, L.win1(x.data_ptr(), 1), wA,
+                                               packed.data_ptr(), -1) != 0
+    assert so.exoblas_strsv_mixed_packed_async(sh, 120, trans, 131, n, L.win1(x.data_ptr(), 1), wA,
+                                               packed.data_ptr(), 1) != 0
