@@ -292,16 +292,17 @@ def bench_bandwidth(a, torch, so, L):
     ks = kernel_set(a.n1, a.n2)
     launch = [[make_launchers(so, L, s, a.n1, a.n2, sh)(name, p, kind) for name, p, kind, _ in ks] for s in sets]
 
-    def step(i, evs=None):
+    def step(i, evs=None, only=None):
         # kernel j of step i uses operand set (i + j) % 2: consecutive kernels
         # (and the same kernel in consecutive steps) never share operands, so
         # the 256 MB Infinity Cache holds nothing the next kernel reads
         for j in range(len(ks)):
             f = launch[(i + j) % 2][j]
-            if evs is not None:
+            timed = evs is not None and (only is None or j == only)
+            if timed:
                 evs[j][0].record(stream)
             rc = f()
-            if evs is not None:
+            if timed:
                 evs[j][1].record(stream)
             if rc != 0:
                 raise RuntimeError(f"{ks[j][0]} failed: {L.last_error()}")
@@ -309,14 +310,19 @@ def bench_bandwidth(a, torch, so, L):
     for i in range(a.warmup):
         step(i)
     torch.cuda.synchronize()
-    # the timed steps carry no events (an event record between kernels is
-    # instrumentation, not work); the per-kernel times come from as many
-    # steps again, each kernel between its own pair of events
+    # the timed steps carry one event pair, around the kernel with the most
+    # bytes (the roofline's kernel); events between the other kernels would be
+    # instrumentation, not work: the per-kernel times come from as many steps
+    # again, each kernel between its own pair of events
+    dom = max(range(len(ks)), key=lambda j: ks[j][3])
+    dev = [[(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in ks]
+           for _ in range(a.steps)]
     t0 = time.perf_counter()
     for i in range(a.steps):
-        step(i)
+        step(i, dev[i], only=dom)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
+    dom_ms = sum(dev[i][dom][0].elapsed_time(dev[i][dom][1]) for i in range(a.steps)) / a.steps
     evs = [[(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in ks]
            for _ in range(a.steps)]
     for i in range(a.steps):
@@ -344,9 +350,8 @@ def bench_bandwidth(a, torch, so, L):
                 f, rec = gbench.level1_records(p, kind, a.n1, ms * 1e3, a.steps)
                 groups.setdefault(f, []).append(rec)
         _gbench_emit(a, groups)
-    dom = max(range(len(ks)), key=lambda j: per_k_ms[j])
     dname, _, _, dbytes = ks[dom]
-    dach = dbytes / (per_k_ms[dom] * 1e-3) / 1e9
+    dach = dbytes / (dom_ms * 1e-3) / 1e9
     # HBM bytes per call of the dominant kernel from the FETCH_SIZE / WRITE_SIZE
     # counter runs (scaled by a plain copy of known size), at the default sizes
     traffic = None
