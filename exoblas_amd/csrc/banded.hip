@@ -1,5 +1,6 @@
 // Banded level-2 kernels for gfx950: gbmv, sbmv, tbmv (one band-gather
-// kernel) and tbsv (a one-wave sequential solve).
+// kernel), tbsv (a blocked solve for narrow bands, a one-wave sequential solve
+// for wide ones).
 //
 // Band storage (row-major, the reference's layout): a band matrix row i keeps
 // its in-band entries in A[i, c0 + d] for offsets d = j - i of the band,
@@ -17,11 +18,14 @@
 // band_mv_kernel (wider bands): a group of G lanes per output row (G = 1, 4,
 // 16 or 64 by band width) sums its row's products lane-strided and reduces
 // with xor shuffles in a fixed order.  Both are deterministic.
-// tbsv_window_kernel (k <= 32): one wave solves blocks of 64 rows from LDS
-// while it prefetches the next block, with the last k solutions in registers.
+// tbsv, k <= 32: a blocked solve (per-block responses to the incoming
+// state, a k x k chain over the blocks, a per-block finish), or one wave over
+// all rows for short systems.
 // tbsv_kernel (wider bands): one wave walks the rows in dependency order;
 // lane l handles offsets d = 1 + l, 1 + l + 64, ...; the last k solutions live
 // in an LDS ring, so each step is one wave reduction and one division.
+#include <type_traits>
+
 #include "device.hpp"
 #include "ops.hpp"
 
@@ -275,31 +279,81 @@ __global__ __launch_bounds__(kWave) void tbsv_kernel(int64_t n, int64_t k, int f
   }
 }
 
-// Narrow bands (k <= KMAX <= 32): the rows are solved in blocks of 64 in
-// dependency order.  While block b is solved, every lane loads row l of
-// block b + 1 (its pivot, its KMAX offset entries, 0 past the band or the
-// matrix, and its right-hand side) into registers; after block b they go to
-// LDS.  The solve itself reads the block from LDS (uniform addresses) and
-// keeps the last KMAX solutions in a register window, so a step is KMAX
-// multiply-adds in four chains and one multiply by the pivot's reciprocal
-// (taken when the row is fetched), with no memory round trip.
+}  // namespace
+
+// The narrow-band solve's kernels have external linkage: inside the unnamed
+// namespace the compiler allocates tbsv_rows_kernel<..., kResponses> twice
+// the registers (256 against 118 for FP32, k <= 16) and spills.
+namespace tbsv_blocked {
+
+// Narrow bands (k <= KMAX <= 32).  Rows are taken in dependency order s
+// (row i = s forward, n - 1 - s backward): x_s = (b_s - sum_{d=1..k} a_s(d)
+// x_{s-d}) / pivot_s.  Blocked solve: the rows are cut into blocks of blen
+// (a multiple of 64).  A block's solutions are affine in the last KMAX
+// solutions of the block before it (its incoming "state", entry e = the
+// solution e + 1 rows back), so
+//   tbsv_rows_kernel<kResponses>: every block at once, lane c < KMAX solves
+//     the block from state e_c with a zero right-hand side and lane KMAX from
+//     a zero state with b; the final windows give the k x k map H and the
+//     offset z of w_out = z + H w_in, stored as M[block][e][c] (c = KMAX: z);
+//   tbsv_chain_kernel: one wave runs w_b = z_b + H_b w_{b-1} over the blocks
+//     (a k x k product per block instead of blen rows);
+//   tbsv_rows_kernel<kFinish>: every block at once, re-solved from its
+//     incoming state W[b - 1].
+// Short systems use tbsv_rows_kernel<kWhole>: one wave over all rows.
+//
+// Row step (all modes): a workgroup stages 64 rows at a time in LDS (row r:
+// a(d) at [d - 1], the pivot's reciprocal at [KMAX], b at [KMAX + 1]), rows
+// past the block padded with zeros so every 64-row pass runs whole; the
+// next 64 rows are loaded into registers meanwhile.  The pass is unrolled in
+// groups of KMAX rows with a circular register window (slot u holds the
+// group's row u), so a row is KMAX - 1 multiply-adds in four chains that do
+// not involve the previous row, one multiply-add with it and one multiply.
+enum TbsvMode : int { kWhole = 0, kResponses = 1, kFinish = 2 };
+
+// Value of v in lane l (uniform l), through scalar registers.
+template <class T>
+__device__ __forceinline__ T lane_value(T v, int l) {
+  if constexpr (sizeof(T) == 4) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+  } else {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffff), l);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+  }
+}
+
 template <class T, int KMAX>
-__global__ __launch_bounds__(kWave) void tbsv_window_kernel(int64_t n, int64_t k, int fwd, int col, int64_t c0,
-                                                            int unit, const T* __restrict__ A, int64_t lda, T* x,
-                                                            int64_t incx) {
-  __shared__ T arow[kWave][KMAX + 1];  // [r][0] 1 / pivot, [r][d] entry of offset d
-  __shared__ T bv[kWave];
+constexpr int tbsv_pitch() {  // M row pitch: H row, z, padding to 16 bytes
+  return KMAX + 16 / (int)sizeof(T);
+}
+
+template <class T, int KMAX, int MODE>
+__global__ __launch_bounds__(kWave) void tbsv_rows_kernel(int64_t n, int64_t k, int fwd, int col, int64_t c0,
+                                                          int unit, const T* __restrict__ A, int64_t lda, T* x,
+                                                          int64_t incx, int64_t blen, T* M, const T* W) {
+  constexpr int P = tbsv_pitch<T, KMAX>();
+  constexpr int VW = 16 / (int)sizeof(T);  // values per 16-byte LDS read
+  constexpr int RP = KMAX + VW;            // LDS row pitch
+  constexpr int NV = (KMAX + 2 + VW - 1) / VW;
+  static_assert(NV * VW <= RP && KMAX < kWave && kWave % KMAX == 0, "tbsv row layout");
+  using V = typename std::conditional<sizeof(T) == 4, float4, double2>::type;
+  __shared__ __align__(16) T arow[kWave * RP];
+  __shared__ T xs[kWave];
   const int lane = threadIdx.x;
-  const int64_t nb = ceil_div(n, kWave);
+  const int64_t blk = blockIdx.x;
+  const int64_t sb = blk * blen;
+  const int64_t se = sb + blen < n ? sb + blen : n;
+  const int64_t passes = ceil_div(se - sb, kWave);
   T ra[KMAX + 1], rb = T(0);
   auto row_of = [&](int64_t s) { return fwd ? s : n - 1 - s; };
-  // fetch only issues loads (every one unconditional, from the row's diagonal
-  // where the entry lies outside the band or the matrix), so none waits for
-  // another; stash, after the block's solve, masks them, takes the pivot's
-  // reciprocal and writes LDS.
-  auto fetch = [&](int64_t blk) {
-    const int64_t s = blk * kWave + lane;
-    const int64_t i = row_of(s < n ? s : n - 1);
+  // fetch only issues loads (all unconditional: from the row's diagonal
+  // where the entry lies outside the band or the matrix); stash masks them
+  // and writes LDS after the pass that used the previous rows.
+  auto fetch = [&](int64_t pass) {
+    const int64_t s = sb + pass * kWave + lane;
+    const int64_t i = row_of(s < se ? s : se - 1);
     const int64_t dg = i * lda + c0;
     ra[0] = A[dg];
 #pragma unroll
@@ -310,70 +364,200 @@ __global__ __launch_bounds__(kWave) void tbsv_window_kernel(int64_t n, int64_t k
     }
     rb = x[i * incx];
   };
-  auto stash = [&](int64_t blk) {
-    const int64_t s = blk * kWave + lane;
-    const int64_t i = row_of(s < n ? s : n - 1);
-    arow[lane][0] = unit ? T(1) : T(1) / ra[0];
+  auto stash = [&](int64_t pass) {
+    const int64_t s = sb + pass * kWave + lane;
+    const bool real = s < se;
+    const int64_t i = row_of(real ? s : se - 1);
+    T* ar = arow + lane * RP;
 #pragma unroll
     for (int d = 1; d <= KMAX; ++d) {
       const int64_t j = fwd ? i - d : i + d;
-      arow[lane][d] = (d <= k && j >= 0 && j < n) ? ra[d] : T(0);
+      ar[d - 1] = (real && d <= k && j >= 0 && j < n) ? ra[d] : T(0);
     }
-    bv[lane] = rb;
+    ar[KMAX] = real ? (unit ? T(1) : T(1) / ra[0]) : T(0);
+    ar[KMAX + 1] = real ? rb : T(0);
   };
-  T xw[KMAX];  // xw[d - 1] = solution of the row d steps back
+  T xw[KMAX];
 #pragma unroll
-  for (int d = 0; d < KMAX; ++d) xw[d] = T(0);
+  for (int u = 0; u < KMAX; ++u) {
+    const int e = KMAX - 1 - u;  // before row 0, slot u holds state entry e
+    if constexpr (MODE == kWhole) xw[u] = T(0);
+    if constexpr (MODE == kResponses) xw[u] = lane == e ? T(1) : T(0);
+    if constexpr (MODE == kFinish) xw[u] = blk > 0 ? W[(blk - 1) * kWave + e] : T(0);
+  }
   fetch(0);
   stash(0);
   __syncthreads();
-  for (int64_t blk = 0; blk < nb; ++blk) {
-    if (blk + 1 < nb) fetch(blk + 1);
-    const int cnt = (int)(n - blk * kWave < kWave ? n - blk * kWave : kWave);
-    // FP32: branch-free and unrolled, so the LDS reads of the next rows issue
-    // ahead of this row's chain; lane r keeps row r's solution and the block's
-    // solutions are stored together afterwards (11.1 -> 10.2 ms at n = 2^16,
-    // k = 16).  FP64 stores each row as it is solved (13.8 ms against 17.0
-    // with the block store, whose registers it runs short of).
-    constexpr bool kBlockStore = sizeof(T) == 4;
-    T mine = T(0);
-#pragma unroll 4
-    for (int r = 0; r < cnt; ++r) {
-      T acc[4] = {T(0), T(0), T(0), T(0)};
+  // LDS rows are read LDD rows ahead into a ring of register rows; the
+  // compiler barriers keep each read where it is written.
+  constexpr int LDD = (KMAX + 2) * (int)sizeof(T) <= 160 ? 2 : 1;
+  T rv[LDD + 1][NV * VW];
+  auto lds_row = [&](int slot, int r) {
+    const V* p = reinterpret_cast<const V*>(arow + r * RP);
 #pragma unroll
-      for (int d = 1; d <= KMAX; ++d) acc[(d - 1) & 3] += arow[r][d] * xw[d - 1];
-      const T xi = (bv[r] - ((acc[0] + acc[1]) + (acc[2] + acc[3]))) * arow[r][0];
-#pragma unroll
-      for (int d = KMAX - 1; d > 0; --d) xw[d] = xw[d - 1];
-      xw[0] = xi;
-      if constexpr (kBlockStore)
-        mine = lane == r ? xi : mine;
-      else if (lane == 0)
-        x[row_of(blk * kWave + r) * incx] = xi;
+    for (int q = 0; q < NV; ++q) {
+      const V v = p[q];
+      if constexpr (sizeof(T) == 4) {
+        rv[slot][4 * q] = v.x;
+        rv[slot][4 * q + 1] = v.y;
+        rv[slot][4 * q + 2] = v.z;
+        rv[slot][4 * q + 3] = v.w;
+      } else {
+        rv[slot][2 * q] = v.x;
+        rv[slot][2 * q + 1] = v.y;
+      }
     }
-    if (kBlockStore && lane < cnt) x[row_of(blk * kWave + lane) * incx] = mine;
+  };
+  for (int64_t pass = 0; pass < passes; ++pass) {
+    if (pass + 1 < passes) fetch(pass + 1);
+#pragma unroll
+    for (int r = 0; r < LDD; ++r) lds_row(r, r);
+#pragma unroll
+    for (int g = 0; g < kWave / KMAX; ++g) {
+#pragma unroll
+      for (int u = 0; u < KMAX; ++u) {
+        const int r = g * KMAX + u;
+        if (r + LDD < kWave) lds_row((r + LDD) % (LDD + 1), r + LDD);
+        asm volatile("" ::: "memory");
+        __builtin_amdgcn_sched_barrier(0);
+        const T* ar = rv[r % (LDD + 1)];
+        T t = ar[KMAX + 1];
+        if constexpr (MODE == kResponses) t = lane == KMAX ? t : T(0);
+        T acc[4] = {T(0), T(0), T(0), T(0)};
+#pragma unroll
+        for (int d = KMAX; d >= 2; --d) acc[d & 3] += ar[d - 1] * xw[(u - d + KMAX) % KMAX];
+        t -= (acc[0] + acc[1]) + (acc[2] + acc[3]);
+        t -= ar[0] * xw[(u - 1 + KMAX) % KMAX];
+        xw[u] = t * ar[KMAX];
+      }
+      // Every mode stores the group's solutions (kResponses does not use
+      // them): without this store between groups the compiler keeps enough
+      // values live to spill.
+      if (lane == 0) {
+#pragma unroll
+        for (int u = 0; u < KMAX; ++u) xs[g * KMAX + u] = xw[u];
+      }
+    }
     __syncthreads();
-    if (blk + 1 < nb) stash(blk + 1);
+    if constexpr (MODE != kResponses) {
+      const int64_t s = sb + pass * kWave + lane;
+      if (s < se) x[row_of(s) * incx] = xs[lane];
+    }
+    if (pass + 1 < passes) stash(pass + 1);
     __syncthreads();
   }
+  if constexpr (MODE == kResponses) {
+    // full blocks only, so the window ends on slot KMAX - 1 (the last row)
+    if (lane < P) {
+#pragma unroll
+      for (int e = 0; e < KMAX; ++e) M[(blk * KMAX + e) * P + lane] = xw[KMAX - 1 - e];
+    }
+  }
 }
+
+// w_b = z_b + H_b w_{b-1} for b < steps (w_{-1} = 0), one wave: lane e
+// keeps state entry e and takes row e of block b's map, prefetched D blocks
+// ahead (D sized so the loads in flight stay within the vector memory
+// counter).  The loop body has no branch: M holds D more blocks than are
+// used and W room for the padded steps (their values are never read), and
+// the sched barriers keep the loads in issue order so the waits count them.
+// W holds kWave values per block.
+template <class T, int KMAX, int D>
+__global__ __launch_bounds__(kWave) void tbsv_chain_kernel(int64_t steps, const T* __restrict__ M, T* W) {
+  constexpr int P = tbsv_pitch<T, KMAX>();
+  const int lane = threadIdx.x;
+  const int e = lane < KMAX ? lane : KMAX - 1;
+  T buf[D][KMAX + 1];
+  const T* row = M + e * P;
+  auto load = [&](int q) {
+#pragma unroll
+    for (int c = 0; c <= KMAX; ++c) buf[q][c] = row[c];
+    row += KMAX * P;
+  };
+#pragma unroll
+  for (int q = 0; q < D; ++q) {
+    load(q);
+    __builtin_amdgcn_sched_barrier(0);
+  }
+  T w = T(0);
+  T* wp = W + lane;
+  for (int64_t b0 = 0; b0 < steps; b0 += D) {
+#pragma unroll
+    for (int q = 0; q < D; ++q) {
+      T wv[KMAX];
+#pragma unroll
+      for (int c = 0; c < KMAX; ++c) wv[c] = lane_value(w, c);
+      T acc[4] = {buf[q][KMAX], T(0), T(0), T(0)};
+#pragma unroll
+      for (int c = 0; c < KMAX; ++c) acc[c & 3] += buf[q][c] * wv[c];
+      w = (acc[0] + acc[1]) + (acc[2] + acc[3]);
+      *wp = w;
+      wp += kWave;
+      load(q);
+      __builtin_amdgcn_sched_barrier(0);
+    }
+  }
+}
+
+template <class T, int KMAX>
+constexpr int tbsv_chain_depth() {
+  constexpr int loads = ((KMAX + 1) * (int)sizeof(T) + 15) / 16;  // per block row
+  constexpr int d = 60 / (loads + 1);
+  return d < 2 ? 2 : (d > 8 ? 8 : d);
+}
+
+// Systems of at least this many rows take the blocked solve.
+constexpr int64_t kTbsvBlockedMin = 512;
+
+// Block length: the blocked solve costs about 2 blen row steps plus n / blen
+// chain steps of similar length, so blen near sqrt(n / 2), as a power of two
+// (64 .. 8192).
+inline int64_t tbsv_block_rows(int64_t n) {
+  int64_t blen = kWave;
+  while (blen < 8192 && 4 * blen * blen <= (n * 6) / 5) blen *= 2;
+  return blen;
+}
+
+template <class T, int KMAX>
+void launch_tbsv_narrow(Ctx& c, int64_t n, int64_t k, bool fwd, bool col, int64_t c0, bool unit, const T* A,
+                        int64_t lda, T* x, int64_t incx) {
+  if (n < kTbsvBlockedMin) {
+    hipLaunchKernelGGL((tbsv_rows_kernel<T, KMAX, kWhole>), dim3(1), dim3(kWave), 0, c.stream, n, k, (int)fwd,
+                       (int)col, c0, (int)unit, A, lda, x, incx, n, (T*)nullptr, (const T*)nullptr);
+    EXO_LAUNCH_CHECK();
+    return;
+  }
+  constexpr int D = tbsv_chain_depth<T, KMAX>();
+  constexpr int P = tbsv_pitch<T, KMAX>();
+  const int64_t blen = tbsv_block_rows(n);
+  const int64_t nb = ceil_div(n, blen);
+  const int64_t rows = nb + 2 * D;  // blocks of M and W, with the chain's padding
+  const size_t m_elems = (size_t)rows * KMAX * P;
+  T* M = static_cast<T*>(c.scratch(sizeof(T) * (m_elems + (size_t)rows * kWave), 3));
+  T* W = M + m_elems;
+  hipLaunchKernelGGL((tbsv_rows_kernel<T, KMAX, kResponses>), dim3((unsigned)(nb - 1)), dim3(kWave), 0, c.stream, n,
+                     k, (int)fwd, (int)col, c0, (int)unit, A, lda, x, incx, blen, M, (const T*)nullptr);
+  EXO_LAUNCH_CHECK();
+  hipLaunchKernelGGL((tbsv_chain_kernel<T, KMAX, D>), dim3(1), dim3(kWave), 0, c.stream, nb - 1, (const T*)M, W);
+  EXO_LAUNCH_CHECK();
+  hipLaunchKernelGGL((tbsv_rows_kernel<T, KMAX, kFinish>), dim3((unsigned)nb), dim3(kWave), 0, c.stream, n, k,
+                     (int)fwd, (int)col, c0, (int)unit, A, lda, x, incx, blen, (T*)nullptr, (const T*)W);
+  EXO_LAUNCH_CHECK();
+}
+
+}  // namespace tbsv_blocked
+
+namespace {
 
 template <class T>
 void launch_tbsv(Ctx& c, int64_t n, int64_t k, bool fwd, bool col, int64_t c0, bool unit, const T* A, int64_t lda,
                  T* x, int64_t incx) {
   if (n <= 0) return;
-#define EXO_TBSV_W(KM)                                                                                          \
-  if (k <= KM) {                                                                                                \
-    hipLaunchKernelGGL((tbsv_window_kernel<T, KM>), dim3(1), dim3(kWave), 0, c.stream, n, k, (int)fwd, (int)col, \
-                       c0, (int)unit, A, lda, x, incx);                                                         \
-    EXO_LAUNCH_CHECK();                                                                                         \
-    return;                                                                                                     \
-  }
-  EXO_TBSV_W(4)
-  EXO_TBSV_W(8)
-  EXO_TBSV_W(16)
-  EXO_TBSV_W(32)
-#undef EXO_TBSV_W
+  using tbsv_blocked::launch_tbsv_narrow;
+  if (k <= 4) return launch_tbsv_narrow<T, 4>(c, n, k, fwd, col, c0, unit, A, lda, x, incx);
+  if (k <= 8) return launch_tbsv_narrow<T, 8>(c, n, k, fwd, col, c0, unit, A, lda, x, incx);
+  if (k <= 16) return launch_tbsv_narrow<T, 16>(c, n, k, fwd, col, c0, unit, A, lda, x, incx);
+  if (k <= 32) return launch_tbsv_narrow<T, 32>(c, n, k, fwd, col, c0, unit, A, lda, x, incx);
   int64_t R = 1;
   while (R < k + 1) R <<= 1;
   const size_t bytes = sizeof(T) * (size_t)R;

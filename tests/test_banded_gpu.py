@@ -162,6 +162,29 @@ def test_tbsv(C, dt, uplo, trans, diag, n, k, ld_diff, incx, rng):
 
 
 @pytest.mark.parametrize("dt", DT)
+@pytest.mark.parametrize("uplo,trans,diag", list(itertools.product([UPPER, LOWER], [NO_TRANS, TRANS],
+                                                                      [NON_UNIT, UNIT])))
+@pytest.mark.parametrize("n", [512, 4097, 65539])
+@pytest.mark.parametrize("k", [0, 3, 8, 16, 31, 32])
+@pytest.mark.parametrize("incx", [1, -2])
+def test_tbsv_blocked(C, dt, uplo, trans, diag, n, k, incx, rng):
+    """Systems of 512 rows and more with k <= 32 take the blocked solve
+    (blocks of 64 / 64 / 256 rows here, the last one partial for n = 4097 and
+    65539): every Uplo / Trans / Diag form, band widths at and between the
+    window sizes (4, 8, 16, 32), a negative increment."""
+    A = _tri_band(rng, n, k, k + 1, uplo, dt)
+    if diag == UNIT:
+        A[:, :k + 1] /= (k + 1)
+    x = vec(rng, n, incx, dt)
+    xo = x.copy()
+    Oracle(dt).tbsv(uplo, trans, diag, n, k, xo, incx, A, A.shape[1])
+    fn(C, dt, "tbsv")(RM, uplo, trans, diag, n, k, A, A.shape[1], x, incx)
+    want = elems(xo, n, incx).astype(np.float64)
+    got = elems(x, n, incx).astype(np.float64)
+    assert np.max(np.abs(got - want)) <= 8 * (k + 2) * eps(dt) * (np.max(np.abs(want)) + 1), "tbsv blocked"
+
+
+@pytest.mark.parametrize("dt", DT)
 @pytest.mark.parametrize("uplo,trans", list(itertools.product([UPPER, LOWER], [NO_TRANS, TRANS])))
 def test_banded_large_device(C, dt, uplo, trans):
     """n = 20000 bands of 64 (tbsv) and 200 (the products) on device buffers."""
