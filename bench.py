@@ -605,7 +605,8 @@ def bench_extended(a, torch, so, L):
     flop; B restored before each timed call, outside the timed region), and
     the banded level-2 kernels on n = 2^22 rows (s/d gbmv kl = ku = 16, sbmv
     upper k = 16, tbmv upper no-trans k = 16; band + vector bytes) and tbsv
-    (upper no-trans, k = 16, n = 2^16, a latency-bound chain)."""
+    (upper no-trans, k = 16, n = 2^16, and k = 1024, n = 4096; latency-bound
+    chains)."""
     stream = torch.cuda.current_stream()
     sh = stream.cuda_stream
     g = torch.Generator(device="cuda")
@@ -665,6 +666,13 @@ def bench_extended(a, torch, so, L):
         run(f"{p}tbsv_UN_{ns}_k{k}",
             lambda f=f, Ad=Ad, xs=xs: f(sh, ns, k, L.win1(xs.data_ptr(), 1), L.win2(Ad.data_ptr(), k + 1), 0),
             (ns * (k + 1) + 2 * ns) * s, "GB/s", HBM_PEAK_GBS, 5, before=lambda xs=xs, b=b: xs.copy_(b))
+        nw, kw = 4096, 1024  # the reference benchmark's largest tbsv shape (N = 4096, K = N / 4)
+        Aw = u((nw, kw + 1), dt)
+        Aw[:, 0] = kw + 1 + torch.empty(nw, dtype=dt, device="cuda").uniform_(0, 1, generator=g)
+        bw, xw = u((nw,), dt), torch.empty(nw, dtype=dt, device="cuda")
+        run(f"{p}tbsv_UN_{nw}_k{kw}",
+            lambda f=f, Aw=Aw, xw=xw: f(sh, nw, kw, L.win1(xw.data_ptr(), 1), L.win2(Aw.data_ptr(), kw + 1), 0),
+            (nw * (kw + 1) + 2 * nw) * s, "GB/s", HBM_PEAK_GBS, 5, before=lambda xw=xw, bw=bw: xw.copy_(bw))
     M = 4096
     At, B0 = u((M, M), torch.float32), u((M, M), torch.float32)
     Bw = B0.clone()
@@ -685,7 +693,7 @@ def bench_extended(a, torch, so, L):
     tot_ms = sum(v["us"] for v in kernels.values()) / 1e3
     return _line("TFLOP/s (syr2k, trmm) and GB/s (banded) per kernel", k["TFLOP/s"], "TFLOP/s", a, tot_ms,
                  "f32+f64", "extended: s/d syr2k L-N 4096x1024, strmm L-L-N-N 4096^2, s/d gbmv/sbmv/tbmv n=2^22 "
-                 "k=16, s/d tbsv n=2^16 k=16", roof, kernels)
+                 "k=16, s/d tbsv n=2^16 k=16 and n=4096 k=1024", roof, kernels)
 
 
 def bench_precision(a, torch, so, L):

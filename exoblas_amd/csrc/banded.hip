@@ -1,6 +1,6 @@
 // Banded level-2 kernels for gfx950: gbmv, sbmv, tbmv (one band-gather
-// kernel), tbsv (a blocked solve for narrow bands, a one-wave sequential solve
-// for wide ones).
+// kernel), tbsv (a blocked solve for narrow bands, trsv's block solver for
+// wide ones).
 //
 // Band storage (row-major, the reference's layout): a band matrix row i keeps
 // its in-band entries in A[i, c0 + d] for offsets d = j - i of the band,
@@ -21,9 +21,8 @@
 // tbsv, k <= 32: a blocked solve (per-block responses to the incoming
 // state, a k x k chain over the blocks, a per-block finish), or one wave over
 // all rows for short systems.
-// tbsv_kernel (wider bands): one wave walks the rows in dependency order;
-// lane l handles offsets d = 1 + l, 1 + l + 64, ...; the last k solutions live
-// in an LDS ring, so each step is one wave reduction and one division.
+// tbsv, k > 32: the block solver of trsv (64-row blocks chained between
+// workgroups) over the band seen as a dense triangle, zero outside the band.
 #include <type_traits>
 
 #include "device.hpp"
@@ -242,41 +241,6 @@ template <class T>
 __global__ __launch_bounds__(kNT) void gather_kernel(int64_t n, const T* x, int64_t incx, T* out) {
   for (int64_t i = (int64_t)blockIdx.x * kNT + threadIdx.x; i < n; i += (int64_t)gridDim.x * kNT)
     out[i] = x[i * incx];
-}
-
-// One wave solves op(A) x = b in place for a band of k off-diagonals.  fwd:
-// rows 0, 1, ... (x_i needs x_{i-d}); else rows n-1, n-2, ... (x_{i+d}).
-// Entry (i, j = i -/+ d) is A[i, c0 + (j - i)] (row-wise) or A[j, c0 - (j - i)]
-// (column-wise); the diagonal is A[i, c0] (1 when unit).  ring holds
-// R >= k + 1 solved values, indexed by row mod R.
-template <class T>
-__global__ __launch_bounds__(kWave) void tbsv_kernel(int64_t n, int64_t k, int fwd, int col, int64_t c0, int unit,
-                                                     const T* __restrict__ A, int64_t lda, T* x, int64_t incx,
-                                                     int64_t R) {
-  extern __shared__ __align__(16) unsigned char smem[];
-  T* ring = reinterpret_cast<T*>(smem);
-  const int lane = threadIdx.x;
-  const int64_t sgn = fwd ? -1 : 1;  // j = i + sgn * d
-  for (int64_t s = 0; s < n; ++s) {
-    const int64_t i = fwd ? s : n - 1 - s;
-    const T b = x[i * incx];
-    const T piv = unit ? T(1) : A[i * lda + c0];
-    T part = T(0);
-    for (int64_t d = 1 + lane; d <= k; d += kWave) {
-      const int64_t j = i + sgn * d;
-      if (j < 0 || j >= n) break;  // d only grows: the rest are out of range too
-      const int64_t off = j - i;
-      const T a = col ? A[j * lda + (c0 - off)] : A[i * lda + (c0 + off)];
-      part += a * ring[j & (R - 1)];
-    }
-    const T dot = wave_sum(part);
-    const T xi = (b - dot) / piv;
-    if (lane == 0) {
-      ring[i & (R - 1)] = xi;
-      x[i * incx] = xi;
-    }
-    __syncthreads();  // the ring entry is visible to the next row's reads
-  }
 }
 
 }  // namespace
@@ -558,19 +522,8 @@ void launch_tbsv(Ctx& c, int64_t n, int64_t k, bool fwd, bool col, int64_t c0, b
   if (k <= 8) return launch_tbsv_narrow<T, 8>(c, n, k, fwd, col, c0, unit, A, lda, x, incx);
   if (k <= 16) return launch_tbsv_narrow<T, 16>(c, n, k, fwd, col, c0, unit, A, lda, x, incx);
   if (k <= 32) return launch_tbsv_narrow<T, 32>(c, n, k, fwd, col, c0, unit, A, lda, x, incx);
-  int64_t R = 1;
-  while (R < k + 1) R <<= 1;
-  const size_t bytes = sizeof(T) * (size_t)R;
-  constexpr size_t kMaxLds = 128 << 10;
-  if (bytes > kMaxLds)
-    throw Error(kInvalidSize, "tbsv: band of " + std::to_string(k) + " off-diagonals exceeds the solver's " +
-                                  std::to_string(kMaxLds / sizeof(T) - 1) + " (use trsv)");
-  static std::atomic<uint64_t> lds_set{0};
-  if (bytes > (64 << 10))
-    ensure_lds_limit(reinterpret_cast<const void*>(&tbsv_kernel<T>), kMaxLds, c.device, lds_set);
-  hipLaunchKernelGGL((tbsv_kernel<T>), dim3(1), dim3(kWave), bytes, c.stream, n, k, (int)fwd, (int)col, c0,
-                     (int)unit, A, lda, x, incx, R);
-  EXO_LAUNCH_CHECK();
+  // wider bands: the dense block solver over the band (zero outside it)
+  ops::trsv_band_rm<T>(c, fwd, col, unit, n, k, A, lda, c0, x, incx);
 }
 
 }  // namespace

@@ -50,6 +50,11 @@ void trmv_rm(Ctx&, int64_t Uplo, int64_t TransA, int64_t Diag, int64_t n, T* x, 
 template <class T>
 void trsv_rm(Ctx&, int64_t Uplo, int64_t TransA, int64_t Diag, int64_t n, T* x, int64_t incx, const T* A,
              int64_t lda);
+// Band solve through the block solver (trsv.hip): kb off-diagonals, row-major
+// band storage with the diagonal in column c0.
+template <class T>
+void trsv_band_rm(Ctx&, bool fwd, bool trans, bool unit, int64_t n, int64_t kb, const T* A, int64_t lda, int64_t c0,
+                  T* x, int64_t incx);
 
 // ---------------------------------------------------------------- banded level 2
 template <class T>

@@ -87,8 +87,15 @@ __device__ __forceinline__ void publish_value(int64_t g, T v, T* x, unsigned* pu
   tagged::publish<T>(pub, g, v, marker);
 }
 
-template <class T, bool TRANS>
-__device__ __forceinline__ T mat(const T* A, int64_t lda, int64_t i, int64_t j) {
+// M(i, j) of a dense triangle; with BAND, of a band of kb off-diagonals
+// whose storage the dense addressing reaches (row-major band storage seen
+// with leading dimension lda - 1 from the diagonal's column): entries
+// farther than kb from the diagonal are zero and not read.
+template <class T, bool TRANS, bool BAND = false>
+__device__ __forceinline__ T mat(const T* A, int64_t lda, int64_t i, int64_t j, int64_t kb = 0) {
+  if constexpr (BAND) {
+    if (i - j > kb || j - i > kb) return T(0);
+  }
   return TRANS ? A[j * lda + i] : A[i * lda + j];
 }
 
@@ -239,9 +246,9 @@ constexpr int per_thread() {
 // (NoTrans: i = idx / KB, j = idx % KB; Trans the other way round, so lanes
 // read consecutive addresses either way).  Every load is issued before any
 // is used (one memory round trip for the whole block, not one per entry).
-template <class T, class TI, bool FWD, bool TRANS, bool UNIT, int KB = kB>
+template <class T, class TI, bool FWD, bool TRANS, bool UNIT, int KB = kB, bool BAND = false>
 __device__ __forceinline__ void load_diag(int64_t n, const T* A, int64_t lda, int64_t r0, int t,
-                                          TI (&v)[per_thread<KB>()]) {
+                                          TI (&v)[per_thread<KB>()], int64_t kb = 0) {
   constexpr int NT = 4 * KB;
 #pragma unroll
   for (int e = 0; e < per_thread<KB>(); ++e) {
@@ -252,9 +259,9 @@ __device__ __forceinline__ void load_diag(int64_t n, const T* A, int64_t lda, in
     const int64_t gi = row_of<FWD, KB>(r0, i), gj = row_of<FWD, KB>(r0, j);
     TI w = TI(0);
     if (j == i)
-      w = (UNIT || gi >= n) ? TI(1) : (TI)mat<T, TRANS>(A, lda, gi, gj);
+      w = (UNIT || gi >= n) ? TI(1) : (TI)mat<T, TRANS, BAND>(A, lda, gi, gj, kb);
     else if (j < i && gi < n && gj < n)
-      w = (TI)mat<T, TRANS>(A, lda, gi, gj);
+      w = (TI)mat<T, TRANS, BAND>(A, lda, gi, gj, kb);
     v[e] = w;
   }
 }
@@ -378,10 +385,13 @@ __device__ __forceinline__ void fold_g(const T (&inv)[KB][KB + 1], T (*mb)[mb_st
 // the previous block, runs on one wave (KB = 64 rows, one per lane).
 // PF: load each panel block one iteration ahead (a kernel that keeps other
 // work items in its registers passes false).
-template <class T, bool FWD, bool TRANS, bool UNIT, int KB = kB, bool PF = true>
+// BAND: M is a band of kb off-diagonals (see mat); the panel starts at the
+// first block that reaches into the band.
+template <class T, bool FWD, bool TRANS, bool UNIT, int KB = kB, bool PF = true, bool BAND = false>
 __device__ __forceinline__ void step(int p, int64_t n, const T* __restrict__ A, int64_t lda, T* x, unsigned* pub,
                                      unsigned marker, unsigned* err, Smem<T, T, KB>& sm,
-                                     T (*mb)[mb_stride<T, KB>()] = nullptr, T (*mb2)[mb_stride<T, KB>()] = nullptr) {
+                                     T (*mb)[mb_stride<T, KB>()] = nullptr, T (*mb2)[mb_stride<T, KB>()] = nullptr,
+                                     int64_t kb = 0) {
   constexpr int CW = KB / 4;  // columns per thread in the block mat-vecs
   const int nb = (int)((n + KB - 1) / KB);
   const int t = threadIdx.x;
@@ -409,7 +419,7 @@ __device__ __forceinline__ void step(int p, int64_t n, const T* __restrict__ A, 
       const int a = idx / KB, c2 = idx % KB;
       const int k = TRANS ? c2 : a, c = TRANS ? a : c2;  // lanes on consecutive addresses
       const int64_t gk = row_of<FWD, KB>(r0, k), gc = cb0 + c;
-      m[e] = (gk < n && gc < n) ? mat<T, TRANS>(A, lda, gk, gc) : T(0);
+      m[e] = (gk < n && gc < n) ? mat<T, TRANS, BAND>(A, lda, gk, gc, kb) : T(0);
     }
   };
   auto stage_prev = [&](const T (&m)[PT], T (*dst)[mb_stride<T, KB>()]) {
@@ -426,7 +436,7 @@ __device__ __forceinline__ void step(int p, int64_t n, const T* __restrict__ A, 
     if (p > 1) load_prev(2, mv2);
   }
   T dv[PT];
-  load_diag<T, T, FWD, TRANS, UNIT, KB>(n, A, lda, r0, t, dv);
+  load_diag<T, T, FWD, TRANS, UNIT, KB, BAND>(n, A, lda, r0, t, dv, kb);
   if (PF && t < KB) sm.vs[t] = rhs;
   if constexpr (PF) {
     if (p > 0) stage_prev(mv, mb);
@@ -460,6 +470,8 @@ __device__ __forceinline__ void step(int p, int64_t n, const T* __restrict__ A, 
   // the products run on the matrix cores (fold_g) and go over mb / mb2;
   // wave 0 keeps its rows of G1 in registers and reads G2 from LDS.
   const int pp = PF ? (p > 1 ? p - 2 : 0) : p;  // blocks the panel loop covers
+  // a band's blocks p - q with (p - q - 1) KB >= kb hold no entry of it
+  const int q0 = BAND ? (int)(p - ceil_div(kb, KB) > 0 ? p - ceil_div(kb, KB) : 0) : 0;
   if constexpr (PF) {
     if (p > 0) {
       fold_g<T, KB>(sm.inv, mb, t);
@@ -476,12 +488,15 @@ __device__ __forceinline__ void step(int p, int64_t n, const T* __restrict__ A, 
   auto load_m = [&](int q, T (&m)[CW]) {
     const int64_t bj = FWD ? q : nb - 1 - q;
     const int64_t c0 = bj * KB + cq * CW;
-    if (gi < n && c0 + CW <= n) {
+    // (a band's run is whole when all CW entries lie inside the band)
+    const bool whole = !BAND || (gi - c0 <= kb && c0 + CW - 1 - gi <= kb);
+    if (whole && gi < n && c0 + CW <= n) {
 #pragma unroll
       for (int e = 0; e < CW; ++e) m[e] = mat<T, TRANS>(A, lda, gi, c0 + e);
     } else {
 #pragma unroll
-      for (int e = 0; e < CW; ++e) m[e] = (gi < n && c0 + e < n) ? mat<T, TRANS>(A, lda, gi, c0 + e) : T(0);
+      for (int e = 0; e < CW; ++e)
+        m[e] = (gi < n && c0 + e < n) ? mat<T, TRANS, BAND>(A, lda, gi, c0 + e, kb) : T(0);
     }
   };
   // Block q's entries are loaded one iteration ahead, so an iteration costs
@@ -489,7 +504,7 @@ __device__ __forceinline__ void step(int p, int64_t n, const T* __restrict__ A, 
   // then keeps ahead of the chain of published x blocks.
   T acc = T(0);
   if constexpr (!PF) {
-    for (int q = 0; q < p; ++q) {
+    for (int q = q0; q < p; ++q) {
       T m[CW];
       load_m(q, m);
       T* xs = sm.xs[q & 1];
@@ -500,8 +515,8 @@ __device__ __forceinline__ void step(int p, int64_t n, const T* __restrict__ A, 
     }
   }
   T m[2][CW];
-  if (PF && pp > 0) load_m(0, m[0]);
-  for (int q = 0; PF && q < pp; q += 2) {
+  if (PF && pp > q0) load_m(q0, m[0]);
+  for (int q = q0; PF && q < pp; q += 2) {
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
       const int qq = q + h;

@@ -185,6 +185,27 @@ def test_tbsv_blocked(C, dt, uplo, trans, diag, n, k, incx, rng):
 
 
 @pytest.mark.parametrize("dt", DT)
+@pytest.mark.parametrize("uplo,trans,diag", list(itertools.product([UPPER, LOWER], [NO_TRANS, TRANS],
+                                                                      [NON_UNIT, UNIT])))
+@pytest.mark.parametrize("n,k", [(700, 33), (700, 64), (700, 200), (4096, 64), (4096, 1024), (4096, 4095)])
+@pytest.mark.parametrize("incx", [1, -2])
+def test_tbsv_wide(C, dt, uplo, trans, diag, n, k, incx, rng):
+    """Bands of more than 32 off-diagonals take trsv's block solver over the
+    band; the shapes include the reference benchmark's N = 4096, K = N / 4
+    and N / 64 (stbsv/bench.cpp:68-71) and a full triangle (K = N - 1)."""
+    A = _tri_band(rng, n, k, k + 1, uplo, dt)
+    if diag == UNIT:
+        A[:, :k + 1] /= (k + 1)
+    x = vec(rng, n, incx, dt)
+    xo = x.copy()
+    Oracle(dt).tbsv(uplo, trans, diag, n, k, xo, incx, A, A.shape[1])
+    fn(C, dt, "tbsv")(RM, uplo, trans, diag, n, k, A, A.shape[1], x, incx)
+    want = elems(xo, n, incx).astype(np.float64)
+    got = elems(x, n, incx).astype(np.float64)
+    assert np.max(np.abs(got - want)) <= 8 * (k + 2) * eps(dt) * (np.max(np.abs(want)) + 1), "tbsv wide"
+
+
+@pytest.mark.parametrize("dt", DT)
 @pytest.mark.parametrize("uplo,trans", list(itertools.product([UPPER, LOWER], [NO_TRANS, TRANS])))
 def test_banded_large_device(C, dt, uplo, trans):
     """n = 20000 bands of 64 (tbsv) and 200 (the products) on device buffers."""
