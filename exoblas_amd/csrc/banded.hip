@@ -21,8 +21,9 @@
 // tbsv, k <= 32: a blocked solve (per-block responses to the incoming
 // state, a k x k chain over the blocks, a per-block finish), or one wave over
 // all rows for short systems.
-// tbsv, k > 32: the block solver of trsv (64-row blocks chained between
-// workgroups) over the band seen as a dense triangle, zero outside the band.
+// tbsv, k > 32 (and short systems with k <= 32, where it is faster): the
+// block solver of trsv (64-row blocks chained between workgroups) over the
+// band seen as a dense triangle, zero outside the band.
 #include <type_traits>
 
 #include "device.hpp"
@@ -517,13 +518,24 @@ template <class T>
 void launch_tbsv(Ctx& c, int64_t n, int64_t k, bool fwd, bool col, int64_t c0, bool unit, const T* A, int64_t lda,
                  T* x, int64_t incx) {
   if (n <= 0) return;
+  // Measured on MI355X (s/d, n = 64 .. 16384): trsv's block solver takes
+  // about 15 us + 0.65 us per 64 rows; the one-wave solve (n < 512) about
+  // 6 us + n row steps of 0.005 + 0.0055 KMAX us; the blocked solve (n >= 512)
+  // 13-40 us up to n = 2048, where the block solver overtakes it for
+  // KMAX = 32 up to n ~ 1536.  Each system goes to the cheaper of the two.
+  const int km = k <= 4 ? 4 : k <= 8 ? 8 : k <= 16 ? 16 : 32;
+  const bool short_band = n < tbsv_blocked::kTbsvBlockedMin &&
+                          6.0 + (double)n * (0.005 + 0.0055 * km) > 15.0 + (double)n * (0.65 / 64);
+  if (k > 32 || short_band || (km == 32 && n <= 1536)) {
+    // the block solver over the band seen as a dense triangle (zero outside it)
+    ops::trsv_band_rm<T>(c, fwd, col, unit, n, k, A, lda, c0, x, incx);
+    return;
+  }
   using tbsv_blocked::launch_tbsv_narrow;
   if (k <= 4) return launch_tbsv_narrow<T, 4>(c, n, k, fwd, col, c0, unit, A, lda, x, incx);
   if (k <= 8) return launch_tbsv_narrow<T, 8>(c, n, k, fwd, col, c0, unit, A, lda, x, incx);
   if (k <= 16) return launch_tbsv_narrow<T, 16>(c, n, k, fwd, col, c0, unit, A, lda, x, incx);
-  if (k <= 32) return launch_tbsv_narrow<T, 32>(c, n, k, fwd, col, c0, unit, A, lda, x, incx);
-  // wider bands: the dense block solver over the band (zero outside it)
-  ops::trsv_band_rm<T>(c, fwd, col, unit, n, k, A, lda, c0, x, incx);
+  launch_tbsv_narrow<T, 32>(c, n, k, fwd, col, c0, unit, A, lda, x, incx);
 }
 
 }  // namespace

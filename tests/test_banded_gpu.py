@@ -164,13 +164,14 @@ def test_tbsv(C, dt, uplo, trans, diag, n, k, ld_diff, incx, rng):
 @pytest.mark.parametrize("dt", DT)
 @pytest.mark.parametrize("uplo,trans,diag", list(itertools.product([UPPER, LOWER], [NO_TRANS, TRANS],
                                                                       [NON_UNIT, UNIT])))
-@pytest.mark.parametrize("n", [512, 4097, 65539])
+@pytest.mark.parametrize("n", [40, 512, 4097, 65539])
 @pytest.mark.parametrize("k", [0, 3, 8, 16, 31, 32])
 @pytest.mark.parametrize("incx", [1, -2])
 def test_tbsv_blocked(C, dt, uplo, trans, diag, n, k, incx, rng):
-    """Systems of 512 rows and more with k <= 32 take the blocked solve
-    (blocks of 64 / 64 / 256 rows here, the last one partial for n = 4097 and
-    65539): every Uplo / Trans / Diag form, band widths at and between the
+    """Narrow bands (k <= 32): n = 40 takes the one-wave solve, 4097 and 65539
+    the blocked solve (blocks of 64 / 256 rows, the last one partial), 512 the
+    blocked solve for k <= 16 and trsv's block solver for k > 16 (faster
+    there): every Uplo / Trans / Diag form, band widths at and between the
     window sizes (4, 8, 16, 32), a negative increment."""
     A = _tri_band(rng, n, k, k + 1, uplo, dt)
     if diag == UNIT:
