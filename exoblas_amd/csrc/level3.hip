@@ -44,6 +44,39 @@ __global__ __launch_bounds__(C::NT) void gemm_kernel(int64_t M, int64_t N, int64
   tile_store<C, TRI, BETA>(Cp, ldc, cvec, M, N, m0, n0, alpha, acc, beta);
 }
 
+// Triangular-operand products (trmm) on row tiles in pairs (q, tm - 1 - q):
+// row tile r's K window holds r + 1 K tiles for a lower op(A) and tm - r for
+// an upper one, so every pair holds tm + 1 and ceil(tm / 2) x tn workgroups
+// of equal work need no split of K and no reduction.
+template <class C>
+__global__ __launch_bounds__(C::NT) void gemm_pair_kernel(int64_t M, int64_t N, int64_t K, typename C::type alpha,
+                                                          Operand<typename C::type> A, Operand<typename C::type> B,
+                                                          typename C::type* Cp, int64_t ldc, int cvec,
+                                                          int64_t tiles_m, int64_t tiles_n) {
+  using T = typename C::type;
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int64_t id = xcd_regroup(blockIdx.x, gridDim.x);
+  const int64_t q = id / tiles_n, n0 = (id % tiles_n) * C::BN;
+  if (q >= (tiles_m + 1) / 2) return;
+  for (int h = 0; h < 2; ++h) {
+    const int64_t tm = h == 0 ? q : tiles_m - 1 - q;
+    if (h == 1 && tm == q) break;  // odd tile count: the middle row tile once
+    const int64_t m0 = tm * C::BM;
+    typename Mfma<T>::acc acc[C::SM][C::SN];
+#pragma unroll
+    for (int a = 0; a < C::SM; ++a)
+#pragma unroll
+      for (int b = 0; b < C::SN; ++b)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) acc[a][b][e] = T(0);
+    int64_t kb = 0, ke = K;
+    tri_k_window<1>(A, m0, C::BM, kb, ke);
+    tile_mma<C, 1>(A, B, M, N, K, m0, n0, kb, ke, acc, reinterpret_cast<T*>(smem));
+    tile_store<C, kFull, false>(Cp, ldc, cvec, M, N, m0, n0, alpha, acc, T(0));
+    __syncthreads();  // the second tile's first LDS stores follow every read of the first's
+  }
+}
+
 // Split-K over tiles [tile0, tile0 + ntiles) of the list: workgroup
 // s * ntiles + l runs the K slice [s * ksplit, (s + 1) * ksplit) of tile
 // tile0 + l and writes its raw sums as a dense BM x BN block
@@ -482,6 +515,18 @@ template <class T>
 void run_triangular(Ctx& c, int64_t M, int64_t N, int64_t K, T alpha, const Operand<T>& A, const Operand<T>& B,
                     T* Cp, int64_t ldc) {
   if (M <= 0 || N <= 0 || K <= 0) return;
+  using Big = typename Tiles<T>::Big;
+  const int64_t btm = ceil_div(M, Big::BM), btn = ceil_div(N, Big::BN);
+  if (K == M && ((btm + 1) / 2) * btn >= c.num_cus) {
+    // square triangle and a full round of row-tile pairs: one balanced launch
+    const int cvec = (ldc % Big::EB == 0) && (reinterpret_cast<uintptr_t>(Cp) % (Big::EB * sizeof(T)) == 0);
+    static std::atomic<uint64_t> lds_set{0};
+    ensure_lds_limit(reinterpret_cast<const void*>(&gemm_pair_kernel<Big>), Big::LDS_BYTES, c.device, lds_set);
+    hipLaunchKernelGGL((gemm_pair_kernel<Big>), dim3((unsigned)(((btm + 1) / 2) * btn)), dim3(Big::NT),
+                       Big::LDS_BYTES, c.stream, M, N, K, alpha, A, B, Cp, ldc, cvec, btm, btn);
+    EXO_LAUNCH_CHECK();
+    return;
+  }
   using Huge = typename Tiles<T>::Huge;
   const int64_t tiles = tile_count<Huge>(kFull, M, N);
   const int64_t S = std::min<int64_t>(4, K / (4 * Huge::BK));  // 8: 1065 -> 1131 us (strmm 4096^2)
