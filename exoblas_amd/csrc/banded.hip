@@ -272,8 +272,8 @@ namespace tbsv_blocked {
 // past the block padded with zeros so every 64-row pass runs whole; the
 // next 64 rows are loaded into registers meanwhile.  The pass is unrolled in
 // groups of KMAX rows with a circular register window (slot u holds the
-// group's row u), so a row is KMAX - 1 multiply-adds in four chains that do
-// not involve the previous row, one multiply-add with it and one multiply.
+// group's row u), so a row is KMAX multiply-adds in one chain (the previous
+// row's solution last) and one multiply.
 enum TbsvMode : int { kWhole = 0, kResponses = 1, kFinish = 2 };
 
 // Value of v in lane l (uniform l), through scalar registers.
@@ -386,13 +386,14 @@ __global__ __launch_bounds__(kWave) void tbsv_rows_kernel(int64_t n, int64_t k, 
         asm volatile("" ::: "memory");
         __builtin_amdgcn_sched_barrier(0);
         const T* ar = rv[r % (LDD + 1)];
+        // one multiply-add chain, farthest term first (as the reference's
+        // loop runs): a wave issues a VALU instruction about every 7 cycles
+        // dependent or not, so split accumulators would only add the adds
+        // that join them
         T t = ar[KMAX + 1];
         if constexpr (MODE == kResponses) t = lane == KMAX ? t : T(0);
-        T acc[4] = {T(0), T(0), T(0), T(0)};
 #pragma unroll
-        for (int d = KMAX; d >= 2; --d) acc[d & 3] += ar[d - 1] * xw[(u - d + KMAX) % KMAX];
-        t -= (acc[0] + acc[1]) + (acc[2] + acc[3]);
-        t -= ar[0] * xw[(u - 1 + KMAX) % KMAX];
+        for (int d = KMAX; d >= 1; --d) t -= ar[d - 1] * xw[(u - d + KMAX) % KMAX];
         xw[u] = t * ar[KMAX];
       }
       // Every mode stores the group's solutions (kResponses does not use
@@ -452,10 +453,10 @@ __global__ __launch_bounds__(kWave) void tbsv_chain_kernel(int64_t steps, const 
       T wv[KMAX];
 #pragma unroll
       for (int c = 0; c < KMAX; ++c) wv[c] = lane_value(w, c);
-      T acc[4] = {buf[q][KMAX], T(0), T(0), T(0)};
+      T acc = buf[q][KMAX];  // one chain (see tbsv_rows_kernel)
 #pragma unroll
-      for (int c = 0; c < KMAX; ++c) acc[c & 3] += buf[q][c] * wv[c];
-      w = (acc[0] + acc[1]) + (acc[2] + acc[3]);
+      for (int c = 0; c < KMAX; ++c) acc += buf[q][c] * wv[c];
+      w = acc;
       *wp = w;
       wp += kWave;
       load(q);
