@@ -453,10 +453,13 @@ __global__ __launch_bounds__(kWave) void tbsv_chain_kernel(int64_t steps, const 
       T wv[KMAX];
 #pragma unroll
       for (int c = 0; c < KMAX; ++c) wv[c] = lane_value(w, c);
-      T acc = buf[q][KMAX];  // one chain (see tbsv_rows_kernel)
+      // four accumulators: the products start as the first lane values
+      // arrive while later readlanes are still issuing (one chain measured
+      // 26.8 against 25.7 us for 255 steps, FP32 k = 16)
+      T acc[4] = {buf[q][KMAX], T(0), T(0), T(0)};
 #pragma unroll
-      for (int c = 0; c < KMAX; ++c) acc += buf[q][c] * wv[c];
-      w = acc;
+      for (int c = 0; c < KMAX; ++c) acc[c & 3] += buf[q][c] * wv[c];
+      w = (acc[0] + acc[1]) + (acc[2] + acc[3]);
       *wp = w;
       wp += kWave;
       load(q);
