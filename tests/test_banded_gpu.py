@@ -188,6 +188,27 @@ def test_tbsv_blocked(C, dt, uplo, trans, diag, n, k, incx, rng):
 @pytest.mark.parametrize("dt", DT)
 @pytest.mark.parametrize("uplo,trans,diag", list(itertools.product([UPPER, LOWER], [NO_TRANS, TRANS],
                                                                       [NON_UNIT, UNIT])))
+@pytest.mark.parametrize("n,k", [(52, 32), (53, 32), (108, 16), (109, 16), (231, 8), (232, 8), (511, 8),
+                                 (512, 8), (1536, 32), (1537, 32)])
+def test_tbsv_dispatch_edges(C, dt, uplo, trans, diag, n, k, rng):
+    """Both sides of every point where tbsv changes solver: the one-wave solve
+    / trsv's block solver at n = 53 (window 32), 109 (16) and 232 (8), the
+    one-wave / blocked solve at n = 512, the block solver / blocked solve at
+    n = 1537 (window 32)."""
+    A = _tri_band(rng, n, k, k + 1, uplo, dt)
+    if diag == UNIT:
+        A[:, :k + 1] /= (k + 1)
+    x = vec(rng, n, 1, dt)
+    xo = x.copy()
+    Oracle(dt).tbsv(uplo, trans, diag, n, k, xo, 1, A, A.shape[1])
+    fn(C, dt, "tbsv")(RM, uplo, trans, diag, n, k, A, A.shape[1], x, 1)
+    want = xo.astype(np.float64)
+    assert np.max(np.abs(x.astype(np.float64) - want)) <= 8 * (k + 2) * eps(dt) * (np.max(np.abs(want)) + 1)
+
+
+@pytest.mark.parametrize("dt", DT)
+@pytest.mark.parametrize("uplo,trans,diag", list(itertools.product([UPPER, LOWER], [NO_TRANS, TRANS],
+                                                                      [NON_UNIT, UNIT])))
 @pytest.mark.parametrize("n,k", [(700, 33), (700, 64), (700, 200), (4096, 64), (4096, 1024), (4096, 4095)])
 @pytest.mark.parametrize("incx", [1, -2])
 def test_tbsv_wide(C, dt, uplo, trans, diag, n, k, incx, rng):
